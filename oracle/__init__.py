@@ -1,0 +1,471 @@
+"""The oracle: test infrastructure, not part of the product.
+
+A ctypes face over ``liboracle.so``, a single-threaded FP64 C++ restatement of maille-core's
+3D linear-elastic solve path (see ``maille_oracle.cpp`` for the per-function citations and
+for what pins each part). Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+CPU-baseline leg may import this module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+OK, SINGULAR, NOT_CONVERGED, INVALID, OUT_OF_PATTERN = range(5)
+TOLERANCE = 1e-12  # iterative.rs:9
+MAX_ITERATIONS = 10_000  # iterative.rs:11
+SHIFTS = (0.0, 1e-3, 1e-2, 1e-1, 1.0, 10.0)  # iterative.rs:14
+
+
+def build() -> None:
+    """Compile liboracle.so from maille_oracle.cpp (g++, no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int64)
+_U8 = C.POINTER(C.c_uint8)
+_INT = C.POINTER(C.c_int)
+
+
+def _declare(L):
+    sig = {
+        "or_solid_d": (None, [C.c_double, C.c_double, _D]),
+        "or_tet_shape": (None, [C.c_int, C.c_double, C.c_double, C.c_double, _D, _D]),
+        "or_mapping3d": (C.c_double, [C.c_int, _D, _D, _D]),
+        "or_solid_stiffness": (None, [C.c_int, _D, C.c_double, C.c_double, _D]),
+        "or_solid_stresses": (None, [C.c_int, _D, C.c_double, C.c_double, _D, _D]),
+        "or_face_pressure": (C.c_int, [C.c_int, _D, C.c_int, C.c_double, _INT, _D]),
+        "or_dot": (C.c_double, [_D, _D, C.c_int64]),
+        "or_graph_new": (_P, [C.c_int64, C.c_int64, _I, _I]),
+        "or_graph_from_csr": (_P, [C.c_int64, _I, _I]),
+        "or_graph_ntargets": (C.c_int64, [_P]),
+        "or_graph_get": (None, [_P, _I, _I]),
+        "or_graph_rcm": (None, [_P, _I]),
+        "or_graph_free": (None, [_P]),
+        "or_csr_from_graph": (_P, [_P, _I]),
+        "or_csr_nnz": (C.c_int64, [_P]),
+        "or_csr_add": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_double]),
+        "or_csr_get": (None, [_P, _I, _I, _D]),
+        "or_csr_from_arrays": (_P, [C.c_int64, _I, _I, _D]),
+        "or_csr_multiply": (None, [_P, _D, _D]),
+        "or_csr_ic0": (C.c_int, [_P, C.c_double, _D]),
+        "or_csr_precondition": (None, [_P, _D, _D, _D]),
+        "or_csr_pcg": (C.c_int, [_P, _D, _D, C.c_double, C.c_int64, _I, _D, _INT]),
+        "or_csr_free": (None, [_P]),
+        "or_sky_new": (_P, [C.c_int64, _I]),
+        "or_sky_add": (None, [_P, C.c_int64, C.c_int64, C.c_double]),
+        "or_sky_stored": (C.c_int64, [_P]),
+        "or_sky_get": (C.c_double, [_P, C.c_int64, C.c_int64]),
+        "or_sky_factorize": (C.c_int, [_P]),
+        "or_sky_solve": (None, [_P, _D, _D]),
+        "or_sky_free": (None, [_P]),
+        "or_system_new": (_P, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _U8, _D,
+                               C.c_int, _INT]),
+        "or_system_nfree": (C.c_int64, [_P]),
+        "or_system_nnz": (C.c_int64, [_P]),
+        "or_system_order": (None, [_P, _I]),
+        "or_system_csr": (None, [_P, _I, _I, _D]),
+        "or_system_coupling": (None, [_P, _D]),
+        "or_system_factorize": (C.c_int, [_P, _INT]),
+        "or_system_lvalues": (None, [_P, _D]),
+        "or_system_solve": (C.c_int, [_P, _D, _D, C.c_double, C.c_int64, _I, _D]),
+        "or_system_times": (None, [_P, _D]),
+        "or_system_free": (None, [_P]),
+        "or_reactions": (None, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _U8, _D, _D,
+                                _D]),
+        "or_stresses": (None, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _D, _D]),
+        "or_add_pressures": (None, [C.c_int64, _D, C.c_int, _I, C.c_int64, _I, _I, _D, _D]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _d(a):
+    return a.ctypes.data_as(_D)
+
+
+def _i(a):
+    return a.ctypes.data_as(_I)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+# ------------------------------------------------------------------ element level
+def solid_d(young: float, nu: float) -> np.ndarray:
+    """element.rs:193 — 6x6 isotropic D, Voigt (xx, yy, zz, yz, xz, xy)."""
+    d = np.zeros((6, 6))
+    lib().or_solid_d(young, nu, _d(d))
+    return d
+
+
+def tet_shape(nen: int, xi: float, eta: float, zeta: float):
+    """shape.rs:108/138 — (N, dN/dxi) of Tet4/Tet10."""
+    n = np.zeros(nen)
+    dn = np.zeros((nen, 3))
+    lib().or_tet_shape(nen, xi, eta, zeta, _d(n), _d(dn))
+    return n, dn
+
+
+def mapping3d(x: np.ndarray, dn: np.ndarray):
+    """shape.rs:167 — (det J, dN/dx)."""
+    x = _f64(x)
+    dn = _f64(dn)
+    out = np.zeros_like(dn)
+    det = lib().or_mapping3d(len(x), _d(x), _d(dn), _d(out))
+    return det, out
+
+
+def solid_stiffness(x: np.ndarray, young: float, nu: float) -> np.ndarray:
+    """element.rs:223 with the rule of assembly.rs:204-230 — Ke (3n x 3n)."""
+    x = _f64(x)
+    m = 3 * len(x)
+    ke = np.zeros((m, m))
+    lib().or_solid_stiffness(len(x), _d(x), young, nu, _d(ke))
+    return ke
+
+
+def solid_stresses(x: np.ndarray, young: float, nu: float, ue: np.ndarray) -> np.ndarray:
+    """element.rs:270 — per Gauss point: x, y, z, sxx, syy, szz, tyz, txz, txy."""
+    x = _f64(x)
+    ue = _f64(ue)
+    ngp = 1 if len(x) == 4 else 4
+    out = np.zeros((ngp, 9))
+    lib().or_solid_stresses(len(x), _d(x), young, nu, _d(ue), _d(out))
+    return out
+
+
+def face_pressure(x: np.ndarray, face: int, value: float):
+    """assembly.rs:650 — (positions, forces[n_face_nodes, 3])."""
+    x = _f64(x)
+    pos = np.zeros(6, dtype=np.int32)
+    forces = np.zeros(18)
+    nn = lib().or_face_pressure(len(x), _d(x), face, value,
+                                pos.ctypes.data_as(_INT), _d(forces))
+    return pos[:nn].astype(np.int64), forces[: 3 * nn].reshape(nn, 3)
+
+
+def dot(a, b) -> float:
+    """solve.rs:100 — the 8-lane dot."""
+    a = _f64(a)
+    b = _f64(b)
+    return lib().or_dot(_d(a), _d(b), len(a))
+
+
+# ------------------------------------------------------------------ graph, RCM, CSR
+def _elements_csr(elements):
+    lens = [len(e) for e in elements]
+    off = np.zeros(len(elements) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(lens)
+    items = np.array([v for e in elements for v in e], dtype=np.int64)
+    return off, items
+
+
+class Graph:
+    """iterative.rs:19-77 — vertex adjacency from element vertex lists."""
+
+    def __init__(self, handle, n):
+        self._h = handle
+        self.n = n
+
+    @classmethod
+    def from_elements(cls, n, elements):
+        off, items = _elements_csr(elements)
+        if len(items) == 0:
+            items = np.zeros(1, dtype=np.int64)
+        return cls(lib().or_graph_new(n, len(off) - 1, _i(off), _i(items)), n)
+
+    @classmethod
+    def from_element_array(cls, n, conn):
+        conn = _i64(conn)
+        ne, k = conn.shape
+        off = np.arange(ne + 1, dtype=np.int64) * k
+        return cls(lib().or_graph_new(n, ne, _i(off), _i(conn)), n)
+
+    @classmethod
+    def from_lists(cls, lists):
+        off = np.zeros(len(lists) + 1, dtype=np.int64)
+        off[1:] = np.cumsum([len(x) for x in lists])
+        tgt = np.array([v for x in lists for v in x] or [0], dtype=np.int64)
+        return cls(lib().or_graph_from_csr(len(lists), _i(off), _i(tgt)), len(lists))
+
+    def arrays(self):
+        nt = lib().or_graph_ntargets(self._h)
+        off = np.zeros(self.n + 1, dtype=np.int64)
+        tgt = np.zeros(max(nt, 1), dtype=np.int64)
+        lib().or_graph_get(self._h, _i(off), _i(tgt))
+        return off, tgt[:nt]
+
+    def neighbours(self, v):
+        off, tgt = self.arrays()
+        return tgt[off[v]:off[v + 1]]
+
+    def rcm(self) -> np.ndarray:
+        """solve.rs:200 — order[new] = old."""
+        order = np.zeros(max(self.n, 1), dtype=np.int64)
+        lib().or_graph_rcm(self._h, _i(order))
+        return order[: self.n]
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.or_graph_free(self._h)
+            self._h = None
+
+
+class Csr:
+    """iterative.rs:82-189 — lower triangle, ascending columns, diagonal last."""
+
+    def __init__(self, handle, n):
+        self._h = handle
+        self.n = n
+
+    @classmethod
+    def zeros(cls, graph: Graph, order):
+        order = _i64(order)
+        return cls(lib().or_csr_from_graph(graph._h, _i(order)), graph.n)
+
+    @classmethod
+    def from_arrays(cls, offsets, cols, values):
+        offsets, cols, values = _i64(offsets), _i64(cols), _f64(values)
+        n = len(offsets) - 1
+        return cls(lib().or_csr_from_arrays(n, _i(offsets), _i(cols), _d(values)), n)
+
+    def nnz(self):
+        return lib().or_csr_nnz(self._h)
+
+    def add(self, i, j, v):
+        if lib().or_csr_add(self._h, i, j, v) != OK:
+            raise KeyError("entry outside the pattern")
+
+    def arrays(self):
+        nnz = self.nnz()
+        off = np.zeros(self.n + 1, dtype=np.int64)
+        cols = np.zeros(nnz, dtype=np.int64)
+        vals = np.zeros(nnz)
+        lib().or_csr_get(self._h, _i(off), _i(cols), _d(vals))
+        return off, cols, vals
+
+    def multiply(self, x):
+        x = _f64(x)
+        y = np.zeros(self.n)
+        lib().or_csr_multiply(self._h, _d(x), _d(y))
+        return y
+
+    def incomplete_cholesky(self, shift=0.0):
+        l = np.zeros(self.nnz())
+        return l if lib().or_csr_ic0(self._h, shift, _d(l)) else None
+
+    def precondition(self, l, r):
+        l, r = _f64(l), _f64(r)
+        z = np.zeros(self.n)
+        lib().or_csr_precondition(self._h, _d(l), _d(r), _d(z))
+        return z
+
+    def pcg(self, b, tol=TOLERANCE, maxit=MAX_ITERATIONS):
+        """Pcg::new + Pcg::solve: (status, x, iterations, residual, shift_index)."""
+        b = _f64(b)
+        x = np.zeros(self.n)
+        it = np.zeros(1, dtype=np.int64)
+        res = np.zeros(1)
+        sh = C.c_int(-1)
+        st = lib().or_csr_pcg(self._h, _d(b), _d(x), tol, maxit, _i(it), _d(res), C.byref(sh))
+        return st, x, int(it[0]), float(res[0]), sh.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.or_csr_free(self._h)
+            self._h = None
+
+
+class Skyline:
+    """solve.rs:10-139 — skyline LL^T, the FP64 direct solver."""
+
+    def __init__(self, first):
+        first = _i64(first)
+        self.n = len(first)
+        self._h = lib().or_sky_new(self.n, _i(first))
+
+    def add(self, i, j, v):
+        lib().or_sky_add(self._h, i, j, v)
+
+    def get(self, i, j):
+        return lib().or_sky_get(self._h, i, j)
+
+    def stored(self):
+        return lib().or_sky_stored(self._h)
+
+    def factorize(self):
+        return lib().or_sky_factorize(self._h)
+
+    def solve(self, b):
+        b = _f64(b)
+        x = np.zeros(self.n)
+        lib().or_sky_solve(self._h, _d(b), _d(x))
+        return x
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.or_sky_free(self._h)
+            self._h = None
+
+
+# ------------------------------------------------------------------ the static path
+class System:
+    """model.rs:1389-1517 — FreeSystem over a Tet4/Tet10 mesh: free numbering, Graph, RCM,
+    Csr (iterative) or skyline (direct), scatter, factorisation and solve."""
+
+    def __init__(self, nodes, conn, young, nu, fixed, imposed=None, iterative=True):
+        self.nodes = _f64(nodes)
+        self.conn = _i64(conn)
+        ne, self.nen = self.conn.shape
+        self.young = _f64(np.broadcast_to(young, (ne,)))
+        self.nu = _f64(np.broadcast_to(nu, (ne,)))
+        self.fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+        ndof = 3 * len(self.nodes)
+        self.imposed = _f64(np.zeros(ndof) if imposed is None else imposed)
+        self.iterative = bool(iterative)
+        st = C.c_int(0)
+        self._h = lib().or_system_new(len(self.nodes), _d(self.nodes), ne, self.nen,
+                                      _i(self.conn), _d(self.young), _d(self.nu),
+                                      self.fixed.ctypes.data_as(_U8), _d(self.imposed),
+                                      int(self.iterative), C.byref(st))
+        if st.value != OK:
+            raise RuntimeError(f"oracle FreeSystem::new failed: status {st.value}")
+        self.n = lib().or_system_nfree(self._h)
+        self.shift_index = -1
+
+    def order(self):
+        o = np.zeros(self.n, dtype=np.int64)
+        lib().or_system_order(self._h, _i(o))
+        return o
+
+    def csr(self):
+        nnz = lib().or_system_nnz(self._h)
+        off = np.zeros(self.n + 1, dtype=np.int64)
+        cols = np.zeros(nnz, dtype=np.int64)
+        vals = np.zeros(nnz)
+        lib().or_system_csr(self._h, _i(off), _i(cols), _d(vals))
+        return off, cols, vals
+
+    def coupling(self):
+        c = np.zeros(self.n)
+        lib().or_system_coupling(self._h, _d(c))
+        return c
+
+    def factorize(self):
+        sh = C.c_int(-1)
+        st = lib().or_system_factorize(self._h, C.byref(sh))
+        self.shift_index = sh.value
+        return st
+
+    def l_values(self):
+        l = np.zeros(lib().or_system_nnz(self._h))
+        lib().or_system_lvalues(self._h, _d(l))
+        return l
+
+    def solve(self, f, tol=TOLERANCE, maxit=MAX_ITERATIONS):
+        """(status, u over all DOFs, iterations, relative residual)."""
+        f = _f64(f)
+        u = np.zeros(3 * len(self.nodes))
+        it = np.zeros(1, dtype=np.int64)
+        res = np.zeros(1)
+        st = lib().or_system_solve(self._h, _d(f), _d(u), tol, maxit, _i(it), _d(res))
+        return st, u, int(it[0]), float(res[0])
+
+    def times(self):
+        t = np.zeros(4)
+        lib().or_system_times(self._h, _d(t))
+        return dict(zip(("pattern", "assembly", "factor", "solve"), t))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.or_system_free(self._h)
+            self._h = None
+
+
+def add_pressures(nodes, conn, pressures, f):
+    """model.rs:1113-1121 — dead face pressures (element, face, value) added to f in order."""
+    if len(pressures) == 0:
+        return f
+    nodes, conn = _f64(nodes), _i64(conn)
+    p = list(pressures)
+    pe = _i64([q[0] for q in p])
+    pf = _i64([q[1] for q in p])
+    pv = _f64([q[2] for q in p])
+    lib().or_add_pressures(len(nodes), _d(nodes), conn.shape[1], _i(conn), len(p), _i(pe),
+                           _i(pf), _d(pv), _d(f))
+    return f
+
+
+def reactions(nodes, conn, young, nu, fixed, f, u):
+    """model.rs:586-593 — R = K u - F on the fixed DOFs."""
+    nodes, conn = _f64(nodes), _i64(conn)
+    ne, nen = conn.shape
+    young = _f64(np.broadcast_to(young, (ne,)))
+    nu = _f64(np.broadcast_to(nu, (ne,)))
+    fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+    f, u = _f64(f), _f64(u)
+    r = np.zeros(3 * len(nodes))
+    lib().or_reactions(len(nodes), _d(nodes), ne, nen, _i(conn), _d(young), _d(nu),
+                       fixed.ctypes.data_as(_U8), _d(f), _d(u), _d(r))
+    return r
+
+
+def stresses(nodes, conn, young, nu, u):
+    """model.rs:596-598 — Gauss-point stresses of every element, shape (ne, ngp, 9)."""
+    nodes, conn = _f64(nodes), _i64(conn)
+    ne, nen = conn.shape
+    young = _f64(np.broadcast_to(young, (ne,)))
+    nu = _f64(np.broadcast_to(nu, (ne,)))
+    u = _f64(u)
+    ngp = 1 if nen == 4 else 4
+    out = np.zeros((ne, ngp, 9))
+    lib().or_stresses(len(nodes), _d(nodes), ne, nen, _i(conn), _d(young), _d(nu), _d(u),
+                      _d(out))
+    return out
+
+
+def model_solve(model, iterative=True, tol=TOLERANCE, maxit=MAX_ITERATIONS):
+    """model.rs:548-610 — the whole static solve on a ``maille_amd.meshes.Problem``-like object
+    (nodes, conn, young, nu, fixed, imposed, f with nodal loads, pressures). Returns a dict
+    with displacements, reactions, stresses, iterations, residual, status."""
+    f = model.f.copy()
+    add_pressures(model.nodes, model.conn, model.pressures, f)
+    sys_ = System(model.nodes, model.conn, model.young, model.nu, model.fixed,
+                  model.imposed, iterative=iterative)
+    st = sys_.factorize()
+    if st != OK:
+        return {"status": st}
+    st, u, it, res = sys_.solve(f, tol, maxit)
+    out = {"status": st, "displacements": u, "iterations": it, "residual": res,
+           "shift_index": sys_.shift_index, "times": sys_.times(), "f": f}
+    if st == OK:
+        out["reactions"] = reactions(model.nodes, model.conn, model.young, model.nu,
+                                     model.fixed, f, u)
+        out["stresses"] = stresses(model.nodes, model.conn, model.young, model.nu, u)
+    return out

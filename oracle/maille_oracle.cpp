@@ -1,0 +1,990 @@
+// maille oracle — TEST INFRASTRUCTURE, NOT PART OF THE PRODUCT.
+//
+// A single-threaded FP64 CPU restatement of maille-core's 3D linear-elastic solve path
+// (timothee-faget/maille v0.14.0). It is the checker the HIP library is compared against and
+// the CPU baseline timed beside it. Only tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline may load it; the product library never links or calls it.
+//
+// Every function follows one reference function literally, including loop and summation order,
+// so that it reproduces the reference's floating-point results operation for operation (the
+// reference is Rust, which never contracts a*b+c into an FMA: build with -ffp-contract=off).
+// Citations are file:line into crates/maille-core of the reference.
+//
+// Pinning (what checks each part, since the reference cannot run here):
+//   shape/quadrature/Mapping3d ... closed forms: partition of unity, Kronecker, det = 6V,
+//                                  grad x = I, quadrature exactness (tests/test_oracle_*.py)
+//   solid_d ....................... Lame constants closed form
+//   solid_stiffness ............... symmetry, 6 rigid modes in the null space, u'Ku = V e'De
+//   face_pressure ................. A/3 per Tet4 vertex, A/3 per Tet10 mid-node, resultant
+//   Graph / RCM / Csr pattern ..... brute-force pure-Python restatement on small graphs,
+//                                   permutation + profile shrink, reference unit tests
+//   IC(0) ......................... tridiagonal => exact Cholesky; dense-pattern => Cholesky
+//   PCG ........................... grid Laplacian recovers the chosen x; skyline agreement
+//   skyline Cholesky .............. numpy dense solve; reference 4x4 known answer
+//   Model::solve .................. patch test, uniaxial traction, Timoshenko cantilever,
+//                                   hydrostatic cube (reference integration tests)
+// Parity unpinned by any reference fixture (the reference ships none and cannot run here):
+// the exact RCM permutation, exact PCG iteration counts and bitwise CSR values.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+typedef int64_t i64;
+
+namespace {
+
+// ---------------------------------------------------------------- quadrature.rs
+struct TetGP { double xi, eta, zeta, w; };
+struct TriGP { double xi, eta, w; };
+
+// quadrature.rs:100-105 (TETRA_1)
+const TetGP TETRA_1[1] = {{0.25, 0.25, 0.25, 1.0 / 6.0}};
+// quadrature.rs:108-138 (TET_A, TET_B, TETRA_4)
+const double TET_A = 0.585410196624968500;
+const double TET_B = 0.138196601125010500;
+const TetGP TETRA_4[4] = {{TET_B, TET_B, TET_B, 1.0 / 24.0},
+                          {TET_A, TET_B, TET_B, 1.0 / 24.0},
+                          {TET_B, TET_A, TET_B, 1.0 / 24.0},
+                          {TET_B, TET_B, TET_A, 1.0 / 24.0}};
+// quadrature.rs:19-35 (TRIANGLE_3)
+const TriGP TRIANGLE_3[3] = {{1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0},
+                             {2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0},
+                             {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0}};
+// quadrature.rs:38-54 (TRIANGLE_6, Dunavant)
+const double T6A = 0.445948490915965;
+const double T6WA = 0.223381589678011 / 2.0;
+const double T6B = 0.091576213509771;
+const double T6WB = 0.109951743655322 / 2.0;
+const TriGP TRIANGLE_6[6] = {{T6A, T6A, T6WA},
+                             {1.0 - 2.0 * T6A, T6A, T6WA},
+                             {T6A, 1.0 - 2.0 * T6A, T6WA},
+                             {T6B, T6B, T6WB},
+                             {1.0 - 2.0 * T6B, T6B, T6WB},
+                             {T6B, 1.0 - 2.0 * T6B, T6WB}};
+
+// ---------------------------------------------------------------- shape.rs
+// shape.rs:129 and :135
+const int TET_FACES[4][3] = {{0, 1, 2}, {0, 3, 1}, {1, 3, 2}, {2, 3, 0}};
+const int TET10_EDGES[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+
+// shape.rs:95-111 (tet_barycentric, tet4)
+void tet4_shape(double xi, double eta, double zeta, double* n, double (*dn)[3]) {
+  n[0] = 1.0 - xi - eta - zeta; n[1] = xi; n[2] = eta; n[3] = zeta;
+  const double dl[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int i = 0; i < 4; ++i)
+    for (int c = 0; c < 3; ++c) dn[i][c] = dl[i][c];
+}
+
+// shape.rs:138-153 (tet10)
+void tet10_shape(double xi, double eta, double zeta, double* n, double (*dn)[3]) {
+  const double l[4] = {1.0 - xi - eta - zeta, xi, eta, zeta};
+  const double dl[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int i = 0; i < 4; ++i) {
+    n[i] = l[i] * (2.0 * l[i] - 1.0);
+    for (int c = 0; c < 3; ++c) dn[i][c] = (4.0 * l[i] - 1.0) * dl[i][c];
+  }
+  for (int k = 0; k < 6; ++k) {
+    int a = TET10_EDGES[k][0], b = TET10_EDGES[k][1];
+    n[4 + k] = 4.0 * l[a] * l[b];
+    for (int c = 0; c < 3; ++c) dn[4 + k][c] = 4.0 * (l[a] * dl[b][c] + l[b] * dl[a][c]);
+  }
+}
+
+void tet_shape(int nen, double xi, double eta, double zeta, double* n, double (*dn)[3]) {
+  if (nen == 4) tet4_shape(xi, eta, zeta, n, dn);
+  else tet10_shape(xi, eta, zeta, n, dn);
+}
+
+// shape.rs:10-15 (tri3) and :27-47 (tri6), values only
+void tri_values(int nn, double xi, double eta, double* n) {
+  if (nn == 3) {
+    n[0] = 1.0 - xi - eta; n[1] = xi; n[2] = eta;
+    return;
+  }
+  double l1 = 1.0 - xi - eta, l2 = xi, l3 = eta;
+  n[0] = l1 * (2.0 * l1 - 1.0);
+  n[1] = l2 * (2.0 * l2 - 1.0);
+  n[2] = l3 * (2.0 * l3 - 1.0);
+  n[3] = 4.0 * l1 * l2;
+  n[4] = 4.0 * l2 * l3;
+  n[5] = 4.0 * l3 * l1;
+}
+
+// shape.rs:157-190 (Mapping3d::new)
+double mapping3d(int nen, const double (*x)[3], const double (*dn)[3], double (*dn_dx)[3]) {
+  double j[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int k = 0; k < nen; ++k)
+    for (int i = 0; i < 3; ++i)
+      for (int c = 0; c < 3; ++c) j[i][c] += dn[k][i] * x[k][c];
+  auto cof = [&](int r, int c) {
+    int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+    return j[r1][c1] * j[r2][c2] - j[r1][c2] * j[r2][c1];
+  };
+  double det = 0.0;
+  for (int c = 0; c < 3; ++c) det += j[0][c] * cof(0, c);
+  double inv[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) inv[r][c] = cof(c, r) / det;
+  for (int k = 0; k < nen; ++k)
+    for (int r = 0; r < 3; ++r) {
+      double s = 0.0;
+      for (int c = 0; c < 3; ++c) s += inv[r][c] * dn[k][c];
+      dn_dx[k][r] = s;
+    }
+  return det;
+}
+
+// ---------------------------------------------------------------- element.rs
+// element.rs:193-206 (solid_d)
+void solid_d(double young, double nu, double d[6][6]) {
+  double c = young / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  double g = young / (2.0 * (1.0 + nu));
+  for (int i = 0; i < 6; ++i)
+    for (int k = 0; k < 6; ++k) d[i][k] = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) d[i][k] = (i == k) ? c * (1.0 - nu) : c * nu;
+  for (int i = 3; i < 6; ++i) d[i][i] = g;
+}
+
+// element.rs:209-220 (solid_strain_displacement): column i of B, Voigt (xx,yy,zz,yz,xz,xy)
+void b_column(const double (*g)[3], int i, double col[6]) {
+  const double gx = g[i / 3][0], gy = g[i / 3][1], gz = g[i / 3][2];
+  switch (i % 3) {
+    case 0: col[0] = gx; col[1] = 0; col[2] = 0; col[3] = 0; col[4] = gz; col[5] = gy; break;
+    case 1: col[0] = 0; col[1] = gy; col[2] = 0; col[3] = gz; col[4] = 0; col[5] = gx; break;
+    default: col[0] = 0; col[1] = 0; col[2] = gz; col[3] = gy; col[4] = gx; col[5] = 0; break;
+  }
+}
+
+const TetGP* rule_of(int nen, int* ngp) {
+  // assembly.rs:204-230 (SolidKernel::tet4 -> TETRA_1, ::tet10 -> TETRA_4)
+  if (nen == 4) { *ngp = 1; return TETRA_1; }
+  *ngp = 4;
+  return TETRA_4;
+}
+
+// element.rs:223-243 (solid_stiffness); ke is m x m row-major, ke[j*m+i] = ke[j][i]
+void solid_stiffness(int nen, const double (*x)[3], const double d[6][6], double* ke) {
+  const int m = 3 * nen;
+  int ngp;
+  const TetGP* rule = rule_of(nen, &ngp);
+  std::fill(ke, ke + m * m, 0.0);
+  double n[10], dn[10][3], g[10][3];
+  for (int q = 0; q < ngp; ++q) {
+    tet_shape(nen, rule[q].xi, rule[q].eta, rule[q].zeta, n, dn);
+    double det = mapping3d(nen, x, dn, g);
+    double factor = rule[q].w * std::fabs(det);
+    for (int i = 0; i < m; ++i) {
+      double bi[6], db[6];
+      b_column(g, i, bi);
+      for (int r = 0; r < 6; ++r) {
+        double s = 0.0;
+        for (int c = 0; c < 6; ++c) s += d[r][c] * bi[c];
+        db[r] = s;
+      }
+      for (int jj = 0; jj < m; ++jj) {
+        double bj[6];
+        b_column(g, jj, bj);
+        double s = 0.0;
+        for (int r = 0; r < 6; ++r) s += bj[r] * db[r];
+        ke[jj * m + i] += factor * s;
+      }
+    }
+  }
+}
+
+// element.rs:270-302 (solid_stresses) without thermal strain; out per GP:
+// x, y, z, sxx, syy, szz, tyz, txz, txy (Voigt order after the position)
+void solid_stresses(int nen, const double (*x)[3], const double d[6][6], const double* ue,
+                    double* out) {
+  int ngp;
+  const TetGP* rule = rule_of(nen, &ngp);
+  const int m = 3 * nen;
+  double n[10], dn[10][3], g[10][3];
+  for (int q = 0; q < ngp; ++q) {
+    tet_shape(nen, rule[q].xi, rule[q].eta, rule[q].zeta, n, dn);
+    mapping3d(nen, x, dn, g);
+    double eps[6];
+    for (int r = 0; r < 6; ++r) {
+      double s = 0.0;
+      for (int k = 0; k < m; ++k) {
+        double col[6];
+        b_column(g, k, col);
+        s += col[r] * ue[k];
+      }
+      eps[r] = s;  // minus 0.0 (no thermal strain): identity
+    }
+    double* o = out + 9 * q;
+    for (int k = 0; k < 3; ++k) {
+      double s = 0.0;
+      for (int a = 0; a < nen; ++a) s += n[a] * x[a][k];
+      o[k] = s;
+    }
+    for (int r = 0; r < 6; ++r) {
+      double s = 0.0;
+      for (int c = 0; c < 6; ++c) s += d[r][c] * eps[c];
+      o[3 + r] = s;
+    }
+  }
+}
+
+// mesh.rs:184-198 (face_positions, tet part)
+int face_positions(int nen, int face, int* pos) {
+  const int* f = TET_FACES[face];
+  pos[0] = f[0]; pos[1] = f[1]; pos[2] = f[2];
+  if (nen == 4) return 3;
+  auto mid = [](int a, int b) {
+    for (int k = 0; k < 6; ++k)
+      if ((TET10_EDGES[k][0] == a && TET10_EDGES[k][1] == b) ||
+          (TET10_EDGES[k][0] == b && TET10_EDGES[k][1] == a))
+        return 4 + k;
+    return 4;
+  };
+  pos[3] = mid(f[0], f[1]); pos[4] = mid(f[1], f[2]); pos[5] = mid(f[2], f[0]);
+  return 6;
+}
+
+// assembly.rs:527-593 (face_rule, tet branch) + :650-674 (face_pressure)
+// forces: 3 per face node, in face_positions order. Returns the number of face nodes.
+int face_pressure(int nen, const double (*xe)[3], int face, double value, int* pos,
+                  double* forces) {
+  int nn = face_positions(nen, face, pos);
+  double c[6][3];
+  for (int a = 0; a < nn; ++a)
+    for (int k = 0; k < 3; ++k) c[a][k] = xe[pos[a]][k];
+  double u[3], v[3];
+  for (int k = 0; k < 3; ++k) { u[k] = c[1][k] - c[0][k]; v[k] = c[2][k] - c[0][k]; }
+  double cross[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2],
+                     u[0] * v[1] - u[1] * v[0]};
+  double sq = 0.0;
+  for (int k = 0; k < 3; ++k) sq += cross[k] * cross[k];
+  double twice_area = std::sqrt(sq);
+  int ngp = (nn == 3) ? 3 : 6;
+  const TriGP* rule = (nn == 3) ? TRIANGLE_3 : TRIANGLE_6;
+  double outward[3];
+  for (int k = 0; k < 3; ++k)
+    outward[k] = (c[0][k] + c[1][k] + c[2][k]) / 3.0 -
+                 (xe[0][k] + xe[1][k] + xe[2][k] + xe[3][k]) / 4.0;
+  double dot = 0.0;
+  for (int k = 0; k < 3; ++k) dot += cross[k] * outward[k];
+  double sign = dot < 0.0 ? -1.0 : 1.0;
+  double normal[3];
+  for (int k = 0; k < 3; ++k) normal[k] = sign * cross[k] / twice_area;
+  for (int a = 0; a < 3 * nn; ++a) forces[a] = 0.0;
+  for (int q = 0; q < ngp; ++q) {
+    double w = rule[q].w * twice_area;
+    double nv[6];
+    tri_values(nn, rule[q].xi, rule[q].eta, nv);
+    for (int a = 0; a < nn; ++a)
+      for (int k = 0; k < 3; ++k) forces[3 * a + k] -= value * w * nv[a] * normal[k];
+  }
+  return nn;
+}
+
+// ---------------------------------------------------------------- solve.rs: dot
+// solve.rs:100-110: 8 lanes over chunks of 8, tail summed separately
+double dot8(const double* a, const double* b, i64 n) {
+  i64 chunks = n / 8;
+  double tail = 0.0;
+  for (i64 k = chunks * 8; k < n; ++k) tail += a[k] * b[k];
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (i64 c = 0; c < chunks; ++c)
+    for (int k = 0; k < 8; ++k) s[k] += a[8 * c + k] * b[8 * c + k];
+  return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7])) + tail;
+}
+
+// ---------------------------------------------------------------- iterative.rs: Graph
+struct Graph {
+  std::vector<i64> offsets, targets;
+  i64 n() const { return (i64)offsets.size() - 1; }
+  i64 degree(i64 v) const { return offsets[v + 1] - offsets[v]; }
+};
+
+// iterative.rs:38-76 (Graph::from_elements); elements given as CSR (eoff, eitems)
+Graph graph_from_elements(i64 n, i64 ne, const i64* eoff, const i64* eit) {
+  std::vector<i64> incident(n + 1, 0);
+  for (i64 e = 0; e < ne; ++e)
+    for (i64 p = eoff[e]; p < eoff[e + 1]; ++p) incident[eit[p] + 1] += 1;
+  for (i64 v = 0; v < n; ++v) incident[v + 1] += incident[v];
+  std::vector<i64> cursor(incident);
+  std::vector<i64> of_vertex(incident[n]);
+  for (i64 e = 0; e < ne; ++e)
+    for (i64 p = eoff[e]; p < eoff[e + 1]; ++p) of_vertex[cursor[eit[p]]++] = e;
+  Graph g;
+  g.offsets.reserve(n + 1);
+  g.offsets.push_back(0);
+  std::vector<i64> mark(n, -1);
+  for (i64 v = 0; v < n; ++v) {
+    mark[v] = v;
+    for (i64 q = incident[v]; q < incident[v + 1]; ++q) {
+      i64 e = of_vertex[q];
+      for (i64 p = eoff[e]; p < eoff[e + 1]; ++p) {
+        i64 w = eit[p];
+        if (mark[w] != v) { mark[w] = v; g.targets.push_back(w); }
+      }
+    }
+    g.offsets.push_back((i64)g.targets.size());
+  }
+  return g;
+}
+
+// solve.rs:248-270 (bfs_levels)
+std::vector<std::vector<i64>> bfs_levels(const Graph& g, i64 start, const std::vector<char>& excl) {
+  std::vector<char> seen(excl);
+  seen[start] = 1;
+  std::vector<std::vector<i64>> levels;
+  levels.push_back({start});
+  for (;;) {
+    std::vector<i64> next;
+    for (i64 v : levels.back())
+      for (i64 p = g.offsets[v]; p < g.offsets[v + 1]; ++p) {
+        i64 w = g.targets[p];
+        if (!seen[w]) { seen[w] = 1; next.push_back(w); }
+      }
+    if (next.empty()) return levels;
+    levels.push_back(std::move(next));
+  }
+}
+
+// solve.rs:200-245 (reverse_cuthill_mckee): order[new] = old
+std::vector<i64> rcm(const Graph& g) {
+  const i64 n = g.n();
+  std::vector<char> visited(n, 0);
+  std::vector<i64> order;
+  order.reserve(n);
+  for (;;) {
+    // first unvisited vertex of minimum degree (min_by_key keeps the first minimum)
+    i64 seed = -1;
+    for (i64 v = 0; v < n; ++v)
+      if (!visited[v] && (seed < 0 || g.degree(v) < g.degree(seed))) seed = v;
+    if (seed < 0) break;
+    i64 start = seed;
+    auto levels = bfs_levels(g, start, visited);
+    for (;;) {
+      const auto& last = levels.back();
+      i64 far = -1;
+      for (i64 v : last)
+        if (far < 0 || g.degree(v) < g.degree(far)) far = v;
+      if (far < 0) break;
+      auto far_levels = bfs_levels(g, far, visited);
+      if (far_levels.size() <= levels.size()) break;
+      start = far;
+      levels = std::move(far_levels);
+    }
+    i64 head = (i64)order.size();
+    visited[start] = 1;
+    order.push_back(start);
+    std::vector<i64> next;
+    while (head < (i64)order.size()) {
+      i64 v = order[head++];
+      next.clear();
+      for (i64 p = g.offsets[v]; p < g.offsets[v + 1]; ++p)
+        if (!visited[g.targets[p]]) next.push_back(g.targets[p]);
+      std::stable_sort(next.begin(), next.end(),
+                       [&](i64 a, i64 b) { return g.degree(a) < g.degree(b); });
+      for (i64 w : next)
+        if (!visited[w]) { visited[w] = 1; order.push_back(w); }
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+// ---------------------------------------------------------------- iterative.rs: Csr
+struct Csr {
+  std::vector<i64> offsets, cols;
+  std::vector<double> values;
+  i64 n() const { return (i64)offsets.size() - 1; }
+
+  // iterative.rs:91-114 (Csr::zeros)
+  static Csr zeros(const Graph& g, const std::vector<i64>& order, const std::vector<i64>& rank) {
+    Csr a;
+    a.offsets.reserve(order.size() + 1);
+    a.offsets.push_back(0);
+    for (i64 r = 0; r < (i64)order.size(); ++r) {
+      i64 v = order[r];
+      size_t start = a.cols.size();
+      for (i64 p = g.offsets[v]; p < g.offsets[v + 1]; ++p) {
+        i64 c = rank[g.targets[p]];
+        if (c < r) a.cols.push_back(c);
+      }
+      std::sort(a.cols.begin() + start, a.cols.end());
+      a.cols.push_back(r);
+      a.offsets.push_back((i64)a.cols.size());
+    }
+    a.values.assign(a.cols.size(), 0.0);
+    return a;
+  }
+
+  // iterative.rs:121-128 (Csr::add); returns false where the reference panics
+  bool add(i64 i, i64 j, double v) {
+    if (j > i) std::swap(i, j);
+    auto b = cols.begin() + offsets[i], e = cols.begin() + offsets[i + 1];
+    auto it = std::lower_bound(b, e, j);
+    if (it == e || *it != j) return false;
+    values[it - cols.begin()] += v;
+    return true;
+  }
+
+  // iterative.rs:131-142 (multiply)
+  void multiply(const double* x, double* y) const {
+    const i64 nn = n();
+    std::fill(y, y + nn, 0.0);
+    for (i64 i = 0; i < nn; ++i) {
+      i64 start = offsets[i], end = offsets[i + 1] - 1;
+      double s = values[end] * x[i];
+      for (i64 p = start; p < end; ++p) {
+        s += values[p] * x[cols[p]];
+        y[cols[p]] += values[p] * x[i];
+      }
+      y[i] += s;
+    }
+  }
+
+  // iterative.rs:146-170 (incomplete_cholesky); empty on failure
+  bool incomplete_cholesky(double shift, std::vector<double>& l) const {
+    const i64 nn = n();
+    l.assign(values.size(), 0.0);
+    std::vector<double> w(nn, 0.0);
+    for (i64 i = 0; i < nn; ++i) {
+      i64 start = offsets[i], diag = offsets[i + 1] - 1;
+      for (i64 p = start; p < diag; ++p) {
+        i64 j = cols[p];
+        i64 js = offsets[j], jd = offsets[j + 1] - 1;
+        double s = 0.0;
+        for (i64 q = js; q < jd; ++q) s += l[q] * w[cols[q]];
+        l[p] = (values[p] - s) / l[jd];
+        w[j] = l[p];
+      }
+      double squares = 0.0;
+      for (i64 p = start; p < diag; ++p) squares += l[p] * l[p];
+      double pivot = values[diag] * (1.0 + shift) - squares;
+      if (std::isnan(pivot) || pivot <= 1e-12 * values[diag]) return false;
+      l[diag] = std::sqrt(pivot);
+      for (i64 p = start; p < diag; ++p) w[cols[p]] = 0.0;
+    }
+    return true;
+  }
+
+  // iterative.rs:173-188 (precondition)
+  void precondition(const std::vector<double>& l, const double* r, double* z) const {
+    const i64 nn = n();
+    std::copy(r, r + nn, z);
+    for (i64 i = 0; i < nn; ++i) {
+      i64 start = offsets[i], diag = offsets[i + 1] - 1;
+      double s = 0.0;
+      for (i64 p = start; p < diag; ++p) s += l[p] * z[cols[p]];
+      z[i] = (z[i] - s) / l[diag];
+    }
+    for (i64 i = nn - 1; i >= 0; --i) {
+      i64 start = offsets[i], diag = offsets[i + 1] - 1;
+      z[i] /= l[diag];
+      for (i64 p = start; p < diag; ++p) z[cols[p]] -= l[p] * z[i];
+    }
+  }
+};
+
+// iterative.rs:9-14
+const double TOLERANCE = 1e-12;
+const i64 MAX_ITERATIONS = 10000;
+const double SHIFTS[6] = {0.0, 1e-3, 1e-2, 1e-1, 1.0, 10.0};
+
+enum Status { OK = 0, SINGULAR = 1, NOT_CONVERGED = 2, INVALID = 3, OUT_OF_PATTERN = 4 };
+
+// iterative.rs:201-207 (Pcg::new): first shift of the ladder that factors
+int pcg_new(const Csr& a, std::vector<double>& l, int* shift_index) {
+  for (int s = 0; s < 6; ++s)
+    if (a.incomplete_cholesky(SHIFTS[s], l)) {
+      if (shift_index) *shift_index = s;
+      return OK;
+    }
+  return SINGULAR;
+}
+
+// iterative.rs:210-248 (Pcg::solve), with the tolerance and iteration cap as arguments
+int pcg_solve(const Csr& a, const std::vector<double>& l, const double* b, double* x,
+              double tol, i64 maxit, i64* iterations, double* residual) {
+  const i64 n = a.n();
+  double b_norm = std::sqrt(dot8(b, b, n));
+  std::fill(x, x + n, 0.0);
+  *iterations = 0;
+  *residual = 0.0;
+  if (b_norm == 0.0) return OK;
+  std::vector<double> r(b, b + n), z(n), p(n), ap(n);
+  a.precondition(l, r.data(), z.data());
+  p = z;
+  double rz = dot8(r.data(), z.data(), n);
+  for (i64 it = 0; it < maxit; ++it) {
+    a.multiply(p.data(), ap.data());
+    double curvature = dot8(p.data(), ap.data(), n);
+    if (std::isnan(curvature) || curvature <= 0.0) { *iterations = it + 1; return SINGULAR; }
+    double alpha = rz / curvature;
+    for (i64 k = 0; k < n; ++k) {
+      x[k] += alpha * p[k];
+      r[k] -= alpha * ap[k];
+    }
+    double rn = std::sqrt(dot8(r.data(), r.data(), n));
+    *iterations = it + 1;
+    *residual = rn / b_norm;
+    if (rn <= tol * b_norm) return OK;
+    a.precondition(l, r.data(), z.data());
+    double rz_next = dot8(r.data(), z.data(), n);
+    double beta = rz_next / rz;
+    rz = rz_next;
+    for (i64 k = 0; k < n; ++k) p[k] = z[k] + beta * p[k];
+  }
+  return NOT_CONVERGED;
+}
+
+// ---------------------------------------------------------------- solve.rs: skyline
+struct Skyline {
+  std::vector<i64> first, start;
+  std::vector<double> data;
+  i64 n() const { return (i64)first.size(); }
+  // solve.rs:19-31 (SkylineMatrix::zeros)
+  void zeros(std::vector<i64> f) {
+    first = std::move(f);
+    start.assign(first.size() + 1, 0);
+    for (size_t i = 0; i < first.size(); ++i) start[i + 1] = start[i] + (i64)i + 1 - first[i];
+    data.assign(start[first.size()], 0.0);
+  }
+  // solve.rs:47-54
+  double get(i64 i, i64 j) const {
+    if (j > i) std::swap(i, j);
+    return j < first[i] ? 0.0 : data[start[i] + j - first[i]];
+  }
+  // solve.rs:57-60
+  void add(i64 i, i64 j, double v) {
+    if (j > i) std::swap(i, j);
+    data[start[i] + j - first[i]] += v;
+  }
+  // solve.rs:70-94 (factorize)
+  int factorize() {
+    const i64 nn = n();
+    double max_diag = 0.0;
+    for (i64 i = 0; i < nn; ++i) max_diag = std::fmax(max_diag, get(i, i));
+    double tol = 1e-12 * max_diag;
+    for (i64 i = 0; i < nn; ++i) {
+      i64 fi = first[i];
+      for (i64 j = fi; j <= i; ++j) {
+        i64 k0 = std::max(fi, first[j]);
+        i64 ri = start[i] - fi, rj = start[j] - first[j];
+        double s = dot8(&data[ri + k0], &data[rj + k0], j - k0);
+        double a = data[ri + j] - s;
+        if (i == j) {
+          if (std::isnan(a) || a <= tol) return SINGULAR;
+          data[ri + j] = std::sqrt(a);
+        } else {
+          data[ri + j] = a / data[rj + j];
+        }
+      }
+    }
+    return OK;
+  }
+  // solve.rs:118-138 (Cholesky::solve)
+  void solve(const double* b, double* x) const {
+    const i64 nn = n();
+    std::copy(b, b + nn, x);
+    for (i64 i = 0; i < nn; ++i) {
+      const double* row = &data[start[i]];
+      i64 fi = first[i];
+      double diag = row[i - fi];
+      double s = 0.0;
+      for (i64 k = fi; k < i; ++k) s += row[k - fi] * x[k];
+      x[i] = (x[i] - s) / diag;
+    }
+    for (i64 i = nn - 1; i >= 0; --i) {
+      const double* row = &data[start[i]];
+      i64 fi = first[i];
+      x[i] /= row[i - fi];
+      for (i64 k = fi; k < i; ++k) x[k] -= row[k - fi] * x[i];
+    }
+  }
+};
+
+// ---------------------------------------------------------------- model.rs: FreeSystem
+struct Mesh {
+  i64 n_nodes, n_el;
+  int nen;
+  const double* xyz;   // n_nodes x 3
+  const i64* conn;     // n_el x nen
+  const double* young; // n_el
+  const double* nu;    // n_el
+  void coords(i64 e, double (*x)[3]) const {
+    for (int a = 0; a < nen; ++a)
+      for (int k = 0; k < 3; ++k) x[a][k] = xyz[3 * conn[e * nen + a] + k];
+  }
+  // assembly.rs:64-79 -> SolidKernel::stiffness (:241-248)
+  void stiffness(i64 e, double* ke) const {
+    double x[10][3], d[6][6];
+    coords(e, x);
+    solid_d(young[e], nu[e], d);
+    solid_stiffness(nen, x, d, ke);
+  }
+  // assembly.rs:20-32 (element_dofs, Tet branch)
+  i64 dof(i64 e, int local) const { return 3 * conn[e * nen + local / 3] + local % 3; }
+};
+
+struct System {
+  Mesh mesh;
+  bool iterative = true;
+  i64 ndof = 0, count = 0;
+  std::vector<i64> free_of;  // global dof -> free index or -1
+  std::vector<i64> order, rank;
+  Csr csr;
+  Skyline sky;
+  std::vector<double> l;
+  std::vector<double> coupling;
+  std::vector<double> imposed;
+  std::vector<char> fixed;
+  int shift_index = -1;
+  bool factored = false;
+  double t_pattern = 0, t_assembly = 0, t_factor = 0, t_solve = 0;
+};
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// model.rs:1442-1499 (FreeSystem::new) up to the factorisation
+int system_build(System& s) {
+  const Mesh& m = s.mesh;
+  const int ml = 3 * m.nen;
+  double t0 = now();
+  s.free_of.assign(s.ndof, -1);
+  s.count = 0;
+  for (i64 g = 0; g < s.ndof; ++g)
+    if (!s.fixed[g]) s.free_of[g] = s.count++;
+  std::vector<i64> eoff(m.n_el + 1, 0), eit;
+  eit.reserve(m.n_el * ml);
+  for (i64 e = 0; e < m.n_el; ++e) {
+    for (int a = 0; a < ml; ++a) {
+      i64 f = s.free_of[m.dof(e, a)];
+      if (f >= 0) eit.push_back(f);
+    }
+    eoff[e + 1] = (i64)eit.size();
+  }
+  Graph graph = graph_from_elements(s.count, m.n_el, eoff.data(), eit.data());
+  s.order = rcm(graph);
+  s.rank.assign(s.count, 0);
+  for (i64 r = 0; r < s.count; ++r) s.rank[s.order[r]] = r;
+  if (s.iterative) {
+    s.csr = Csr::zeros(graph, s.order, s.rank);
+  } else {
+    std::vector<i64> first(s.count);
+    for (i64 i = 0; i < s.count; ++i) first[i] = i;
+    for (i64 a = 0; a < s.count; ++a)
+      for (i64 p = graph.offsets[a]; p < graph.offsets[a + 1]; ++p)
+        first[s.rank[a]] = std::min(first[s.rank[a]], s.rank[graph.targets[p]]);
+    s.sky.zeros(std::move(first));
+  }
+  double t1 = now();
+  s.t_pattern = t1 - t0;
+  s.coupling.assign(s.count, 0.0);
+  std::vector<double> ke(ml * ml);
+  for (i64 e = 0; e < m.n_el; ++e) {
+    m.stiffness(e, ke.data());
+    for (int li = 0; li < ml; ++li) {
+      i64 a = s.free_of[m.dof(e, li)];
+      if (a < 0) continue;
+      for (int lj = 0; lj < ml; ++lj) {
+        i64 gj = m.dof(e, lj);
+        i64 b = s.free_of[gj];
+        double k = ke[li * ml + lj];
+        if (b >= 0) {
+          if (s.rank[b] <= s.rank[a]) {
+            if (s.iterative) {
+              if (!s.csr.add(s.rank[a], s.rank[b], k)) return OUT_OF_PATTERN;
+            } else {
+              s.sky.add(s.rank[a], s.rank[b], k);
+            }
+          }
+        } else {
+          s.coupling[s.rank[a]] += k * s.imposed[gj];
+        }
+      }
+    }
+  }
+  s.t_assembly = now() - t1;
+  return OK;
+}
+
+// model.rs:1496 (kff.factorize) -> Pcg::new or SkylineMatrix::factorize
+int system_factorize(System& s) {
+  double t0 = now();
+  int st = s.iterative ? pcg_new(s.csr, s.l, &s.shift_index) : s.sky.factorize();
+  s.t_factor = now() - t0;
+  s.factored = (st == OK);
+  return st;
+}
+
+// model.rs:1502-1516 (FreeSystem::solve)
+int system_solve(System& s, const double* f, double* u, double tol, i64 maxit, i64* iters,
+                 double* resid) {
+  double t0 = now();
+  std::vector<double> rhs(s.count), x(s.count);
+  for (i64 k = 0; k < s.count; ++k) rhs[k] = -s.coupling[k];
+  for (i64 g = 0; g < s.ndof; ++g)
+    if (s.free_of[g] >= 0) rhs[s.rank[s.free_of[g]]] += f[g];
+  int st = OK;
+  *iters = 0;
+  *resid = 0.0;
+  if (s.iterative) st = pcg_solve(s.csr, s.l, rhs.data(), x.data(), tol, maxit, iters, resid);
+  else s.sky.solve(rhs.data(), x.data());
+  for (i64 g = 0; g < s.ndof; ++g) u[g] = s.imposed[g];
+  for (i64 g = 0; g < s.ndof; ++g)
+    if (s.free_of[g] >= 0) u[g] = x[s.rank[s.free_of[g]]];
+  s.t_solve = now() - t0;
+  return st;
+}
+
+}  // namespace
+
+// ================================================================ C ABI (for ctypes)
+extern "C" {
+
+int or_version() { return 1; }
+
+// element.rs:193 (solid_d), d row-major 6x6
+void or_solid_d(double young, double nu, double* d) {
+  double dd[6][6];
+  solid_d(young, nu, dd);
+  std::memcpy(d, dd, sizeof(dd));
+}
+
+// shape.rs:108/138: n[nen], dn[nen*3]
+void or_tet_shape(int nen, double xi, double eta, double zeta, double* n, double* dn) {
+  tet_shape(nen, xi, eta, zeta, n, (double(*)[3])dn);
+}
+
+// shape.rs:167 (Mapping3d::new): returns det, writes dn_dx[nen*3]
+double or_mapping3d(int nen, const double* x, const double* dn, double* dn_dx) {
+  return mapping3d(nen, (const double(*)[3])x, (const double(*)[3])dn, (double(*)[3])dn_dx);
+}
+
+// element.rs:223 via assembly.rs:241: coords[nen*3] -> ke[m*m] row-major
+void or_solid_stiffness(int nen, const double* x, double young, double nu, double* ke) {
+  double d[6][6];
+  solid_d(young, nu, d);
+  solid_stiffness(nen, (const double(*)[3])x, d, ke);
+}
+
+// element.rs:270: stresses at the Gauss points (9 values each)
+void or_solid_stresses(int nen, const double* x, double young, double nu, const double* ue,
+                       double* out) {
+  double d[6][6];
+  solid_d(young, nu, d);
+  solid_stresses(nen, (const double(*)[3])x, d, ue, out);
+}
+
+// assembly.rs:650: returns the face node count; pos[6], forces[18]
+int or_face_pressure(int nen, const double* xe, int face, double value, int* pos,
+                     double* forces) {
+  if (face < 0 || face > 3) return 0;
+  return face_pressure(nen, (const double(*)[3])xe, face, value, pos, forces);
+}
+
+double or_dot(const double* a, const double* b, i64 n) { return dot8(a, b, n); }
+
+// ---- Graph / RCM on explicit arrays (iterative.rs:38, solve.rs:200)
+void* or_graph_new(i64 n, i64 ne, const i64* eoff, const i64* eit) {
+  return new Graph(graph_from_elements(n, ne, eoff, eit));
+}
+void* or_graph_from_csr(i64 n, const i64* off, const i64* tgt) {
+  Graph* g = new Graph;
+  g->offsets.assign(off, off + n + 1);
+  g->targets.assign(tgt, tgt + off[n]);
+  return g;
+}
+i64 or_graph_ntargets(void* h) { return (i64)((Graph*)h)->targets.size(); }
+void or_graph_get(void* h, i64* off, i64* tgt) {
+  Graph* g = (Graph*)h;
+  std::copy(g->offsets.begin(), g->offsets.end(), off);
+  std::copy(g->targets.begin(), g->targets.end(), tgt);
+}
+void or_graph_rcm(void* h, i64* order) {
+  auto o = rcm(*(Graph*)h);
+  std::copy(o.begin(), o.end(), order);
+}
+void or_graph_free(void* h) { delete (Graph*)h; }
+
+// ---- Csr on explicit arrays (iterative.rs:82-188) for the unit tests
+void* or_csr_from_graph(void* gh, const i64* order) {
+  Graph* g = (Graph*)gh;
+  std::vector<i64> o(order, order + g->n()), r(g->n());
+  for (i64 k = 0; k < g->n(); ++k) r[o[k]] = k;
+  return new Csr(Csr::zeros(*g, o, r));
+}
+i64 or_csr_nnz(void* h) { return (i64)((Csr*)h)->cols.size(); }
+int or_csr_add(void* h, i64 i, i64 j, double v) { return ((Csr*)h)->add(i, j, v) ? OK : OUT_OF_PATTERN; }
+void or_csr_get(void* h, i64* off, i64* cols, double* vals) {
+  Csr* a = (Csr*)h;
+  if (off) std::copy(a->offsets.begin(), a->offsets.end(), off);
+  if (cols) std::copy(a->cols.begin(), a->cols.end(), cols);
+  if (vals) std::copy(a->values.begin(), a->values.end(), vals);
+}
+void* or_csr_from_arrays(i64 n, const i64* off, const i64* cols, const double* vals) {
+  Csr* a = new Csr;
+  a->offsets.assign(off, off + n + 1);
+  a->cols.assign(cols, cols + off[n]);
+  a->values.assign(vals, vals + off[n]);
+  return a;
+}
+void or_csr_multiply(void* h, const double* x, double* y) { ((Csr*)h)->multiply(x, y); }
+// IC(0) with one shift: returns 1 on success and writes l[nnz]
+int or_csr_ic0(void* h, double shift, double* l) {
+  std::vector<double> lv;
+  if (!((Csr*)h)->incomplete_cholesky(shift, lv)) return 0;
+  std::copy(lv.begin(), lv.end(), l);
+  return 1;
+}
+void or_csr_precondition(void* h, const double* l, const double* r, double* z) {
+  Csr* a = (Csr*)h;
+  std::vector<double> lv(l, l + a->values.size());
+  a->precondition(lv, r, z);
+}
+// Pcg::new + Pcg::solve on the Csr: status, iterations, relative residual, shift index
+int or_csr_pcg(void* h, const double* b, double* x, double tol, i64 maxit, i64* iters,
+               double* resid, int* shift_index) {
+  Csr* a = (Csr*)h;
+  std::vector<double> l;
+  int st = pcg_new(*a, l, shift_index);
+  if (st != OK) return st;
+  return pcg_solve(*a, l, b, x, tol, maxit, iters, resid);
+}
+void or_csr_free(void* h) { delete (Csr*)h; }
+
+// ---- dense skyline (solve.rs:10-139) on an explicit profile
+void* or_sky_new(i64 n, const i64* first) {
+  Skyline* s = new Skyline;
+  s->zeros(std::vector<i64>(first, first + n));
+  return s;
+}
+void or_sky_add(void* h, i64 i, i64 j, double v) { ((Skyline*)h)->add(i, j, v); }
+i64 or_sky_stored(void* h) { return (i64)((Skyline*)h)->data.size(); }
+double or_sky_get(void* h, i64 i, i64 j) { return ((Skyline*)h)->get(i, j); }
+int or_sky_factorize(void* h) { return ((Skyline*)h)->factorize(); }
+void or_sky_solve(void* h, const double* b, double* x) { ((Skyline*)h)->solve(b, x); }
+void or_sky_free(void* h) { delete (Skyline*)h; }
+
+// ---- the static linear path (model.rs:548-610) on a Tet4/Tet10 mesh
+// fixed/imposed: per global DOF (3 per node). iterative: 1 = Csr + Pcg, 0 = skyline.
+void* or_system_new(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64* conn,
+                    const double* young, const double* nu, const uint8_t* fixed,
+                    const double* imposed, int iterative, int* status) {
+  System* s = new System;
+  s->mesh = Mesh{n_nodes, n_el, nen, xyz, conn, young, nu};
+  s->iterative = iterative != 0;
+  s->ndof = 3 * n_nodes;
+  s->fixed.assign(fixed, fixed + s->ndof);
+  s->imposed.assign(imposed, imposed + s->ndof);
+  *status = system_build(*s);
+  return s;
+}
+i64 or_system_nfree(void* h) { return ((System*)h)->count; }
+i64 or_system_nnz(void* h) {
+  System* s = (System*)h;
+  return s->iterative ? (i64)s->csr.cols.size() : (i64)s->sky.data.size();
+}
+void or_system_order(void* h, i64* order) {
+  System* s = (System*)h;
+  std::copy(s->order.begin(), s->order.end(), order);
+}
+void or_system_csr(void* h, i64* off, i64* cols, double* vals) {
+  System* s = (System*)h;
+  if (off) std::copy(s->csr.offsets.begin(), s->csr.offsets.end(), off);
+  if (cols) std::copy(s->csr.cols.begin(), s->csr.cols.end(), cols);
+  if (vals) std::copy(s->csr.values.begin(), s->csr.values.end(), vals);
+}
+void or_system_coupling(void* h, double* c) {
+  System* s = (System*)h;
+  std::copy(s->coupling.begin(), s->coupling.end(), c);
+}
+int or_system_factorize(void* h, int* shift_index) {
+  System* s = (System*)h;
+  int st = system_factorize(*s);
+  *shift_index = s->shift_index;
+  return st;
+}
+void or_system_lvalues(void* h, double* l) {
+  System* s = (System*)h;
+  std::copy(s->l.begin(), s->l.end(), l);
+}
+int or_system_solve(void* h, const double* f, double* u, double tol, i64 maxit, i64* iters,
+                    double* resid) {
+  System* s = (System*)h;
+  if (!s->factored) return INVALID;
+  return system_solve(*s, f, u, tol, maxit, iters, resid);
+}
+// phase wall times: pattern (graph + RCM + Csr::zeros), assembly, factor, solve
+void or_system_times(void* h, double* t) {
+  System* s = (System*)h;
+  t[0] = s->t_pattern; t[1] = s->t_assembly; t[2] = s->t_factor; t[3] = s->t_solve;
+}
+void or_system_free(void* h) { delete (System*)h; }
+
+// model.rs:586-593 + :1353-1369 (product): reactions R = K u - F on the fixed DOFs, 0 elsewhere
+void or_reactions(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64* conn,
+                  const double* young, const double* nu, const uint8_t* fixed, const double* f,
+                  const double* u, double* reactions) {
+  Mesh m{n_nodes, n_el, nen, xyz, conn, young, nu};
+  const int ml = 3 * nen;
+  const i64 ndof = 3 * n_nodes;
+  std::fill(reactions, reactions + ndof, 0.0);
+  std::vector<double> ke(ml * ml);
+  for (i64 e = 0; e < n_el; ++e) {
+    bool any = false;
+    for (int a = 0; a < ml; ++a) any = any || fixed[m.dof(e, a)];
+    if (!any) continue;
+    m.stiffness(e, ke.data());
+    for (int li = 0; li < ml; ++li) {
+      i64 g = m.dof(e, li);
+      if (!fixed[g]) continue;
+      double s = 0.0;
+      for (int lj = 0; lj < ml; ++lj) s += ke[li * ml + lj] * u[m.dof(e, lj)];
+      reactions[g] += s;
+    }
+  }
+  for (i64 g = 0; g < ndof; ++g)
+    if (fixed[g]) reactions[g] -= f[g];
+}
+
+// model.rs:596-598 -> element_stresses for every element: out[n_el * ngp * 9]
+void or_stresses(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64* conn,
+                 const double* young, const double* nu, const double* u, double* out) {
+  Mesh m{n_nodes, n_el, nen, xyz, conn, young, nu};
+  int ngp;
+  rule_of(nen, &ngp);
+  const int ml = 3 * nen;
+  for (i64 e = 0; e < n_el; ++e) {
+    double x[10][3], d[6][6], ue[30];
+    m.coords(e, x);
+    solid_d(young[e], nu[e], d);
+    for (int a = 0; a < ml; ++a) ue[a] = u[m.dof(e, a)];
+    solid_stresses(nen, x, d, ue, out + e * ngp * 9);
+  }
+}
+
+// model.rs:1070-1147 (assemble), the loads part: nodal loads already in f; adds the dead
+// pressure loads (element, face, value) in order through FacePressure::add_load.
+void or_add_pressures(i64 n_nodes, const double* xyz, int nen, const i64* conn, i64 np,
+                      const i64* p_el, const i64* p_face, const double* p_val, double* f) {
+  (void)n_nodes;
+  for (i64 k = 0; k < np; ++k) {
+    i64 e = p_el[k];
+    double xe[10][3];
+    for (int a = 0; a < nen; ++a)
+      for (int c = 0; c < 3; ++c) xe[a][c] = xyz[3 * conn[e * nen + a] + c];
+    int pos[6];
+    double forces[18];
+    int nn = face_pressure(nen, xe, (int)p_face[k], p_val[k], pos, forces);
+    for (int a = 0; a < nn; ++a)
+      for (int c = 0; c < 3; ++c) f[3 * conn[e * nen + pos[a]] + c] += forces[3 * a + c];
+  }
+}
+
+}  // extern "C"
