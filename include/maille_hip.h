@@ -1,0 +1,188 @@
+/* maille_hip — MI355X (gfx950) implementation of maille-core's 3D linear-elastic solve path:
+ * Tet4/Tet10 element stiffness, CSR assembly in the reverse-Cuthill-McKee lower-triangle
+ * layout, IC(0) factorisation with the diagonal-shift ladder, and preconditioned conjugate
+ * gradients.
+ *
+ * The C ABI is what a Rust FFI crate binds (extern "C", plain pointers and sizes). Each entry
+ * point cites the reference item it replaces as crates/maille-core/<file>:<line> of
+ * timothee-faget/maille v0.14.0.
+ *
+ * Conventions
+ *  - Host buffers are caller-allocated; entry points with the _device suffix take device
+ *    pointers (hipMalloc'd, on the handle's device) instead.
+ *  - Handles own their device memory (as Pcg owns its Csr by move, iterative.rs:194-197).
+ *    A handle may be used from any one thread at a time; calls on one handle are serialised by
+ *    the caller, as in the reference (single-threaded, Send types).
+ *  - No exception or abort crosses the ABI; every entry point returns an ml_status.
+ *  - Indices are int64_t at the boundary (Rust usize); the device stores int32 and the entry
+ *    points reject sizes that do not fit.
+ */
+#ifndef MAILLE_HIP_H
+#define MAILLE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The first three mirror MailleError (error.rs:224-229). */
+typedef enum ml_status {
+  ML_OK = 0,
+  ML_SINGULAR = 1,            /* MailleError::Singular (error.rs:224) */
+  ML_NOT_CONVERGED = 2,       /* MailleError::IterativeNotConverged (error.rs:226-229) */
+  ML_INVALID_ARGUMENT = 3,    /* null pointer, size out of range, bad element family */
+  ML_OUT_OF_PATTERN = 4,      /* Csr::add outside the profile (iterative.rs:126 panics) */
+  ML_HIP_ERROR = 5,           /* a HIP runtime call failed; see ml_last_error() */
+  ML_INVALID_MESH = 6,        /* Mesh::new refusals (mesh.rs:257-308), see ml_solve_info */
+  ML_DEVICE_TIMEOUT = 7       /* a dataflow kernel's bounded wait expired (never expected) */
+} ml_status;
+
+/* Arithmetic of the solve. ML_FP64 is the drop-in (the reference computes in f64).
+ * ML_MIXED: FP32 SpMV and FP32 IC(0) inside FP64 iterative refinement; the true residual
+ * b - A x, the PCG dot products and the refinement update stay FP64. */
+typedef enum ml_precision { ML_FP64 = 0, ML_MIXED = 1 } ml_precision;
+
+/* SolverKind (solve.rs:171-195). ML_AUTO is iterative from ML_ITERATIVE_FROM free DOFs. */
+typedef enum ml_solver_kind { ML_AUTO = 0, ML_DIRECT = 1, ML_ITERATIVE = 2 } ml_solver_kind;
+#define ML_ITERATIVE_FROM 100000 /* solve.rs:185 */
+#define ML_TOLERANCE 1e-12       /* iterative.rs:9 */
+#define ML_MAX_ITERATIONS 10000  /* iterative.rs:11 */
+
+typedef struct ml_solve_info {
+  int64_t iterations;        /* PCG iterations (summed over refinement sweeps in ML_MIXED) */
+  double residual;           /* recursive relative residual |r|/|b| at exit (iterative.rs:246) */
+  double true_residual;      /* |b - A x| / |b| computed in FP64 after the solve */
+  int32_t shift_index;       /* index into the shift ladder {0,1e-3,1e-2,0.1,1,10} used */
+  int32_t refinements;       /* FP64 refinement sweeps (ML_MIXED), 0 for ML_FP64 */
+  int64_t bad_element;       /* ML_INVALID_MESH: first offending element, else -1 */
+  int32_t mesh_error;        /* ML_INVALID_MESH: 1 degenerate, 2 inverted, 3 curved Tet10 edge,
+                                4 non-finite coordinate, 5 unknown node, 6 bad material */
+  int32_t reserved;
+  double seconds[6];         /* device wall time: assemble, factor, solve, reactions, stresses,
+                                host pattern build */
+} ml_solve_info;
+
+/* Last error text of the calling thread (HIP error string or argument diagnosis). */
+const char* ml_last_error(void);
+/* Number of visible HIP devices (0 without a GPU); never fails. */
+int ml_device_count(void);
+
+/* ------------------------------------------------------------------ element kernels */
+
+/* solid_d (element.rs:193-206): row-major 6x6 isotropic D, Voigt (xx,yy,zz,yz,xz,xy). Host. */
+ml_status ml_solid_d(double young, double nu, double* d);
+
+/* solid_stiffness (element.rs:223-243) for a batch, as element_stiffness dispatches it
+ * (assembly.rs:64-79, 204-248): Tet4 with TETRA_1, Tet10 with TETRA_4.
+ * coords: n_elements x nen x 3, young/nu: n_elements, ke: n_elements x (3nen)^2 row-major
+ * (ke[e][j][i] as the reference's Vec<Vec<f64>>). Host buffers; computed on the device. */
+ml_status ml_stiffness_batch(int nen, int64_t n_elements, const double* coords,
+                             const double* young, const double* nu, double* ke);
+
+/* ------------------------------------------------------------------ pattern */
+
+typedef struct ml_pattern ml_pattern;
+
+/* Graph::from_elements (iterative.rs:38-76) + reverse_cuthill_mckee (solve.rs:200-245) +
+ * Csr::zeros (iterative.rs:91-114) on the host: n vertices, element vertex lists given as
+ * CSR (elem_offsets[n_elements+1], elem_items). */
+ml_status ml_pattern_build(int64_t n, int64_t n_elements, const int64_t* elem_offsets,
+                           const int64_t* elem_items, ml_pattern** out);
+int64_t ml_pattern_n(const ml_pattern* p);
+int64_t ml_pattern_nnz(const ml_pattern* p);
+/* order[n] (order[new] = old, solve.rs:200), offsets[n+1], cols[nnz] (lower triangle,
+ * ascending columns, diagonal last — iterative.rs:79-86). Any pointer may be null. */
+ml_status ml_pattern_get(const ml_pattern* p, int64_t* order, int64_t* offsets, int64_t* cols);
+/* Graph neighbour lists (iterative.rs:16-22): offsets[n+1], targets[ml_pattern_ntargets]. */
+int64_t ml_pattern_ntargets(const ml_pattern* p);
+ml_status ml_pattern_graph(const ml_pattern* p, int64_t* offsets, int64_t* targets);
+void ml_pattern_free(ml_pattern* p);
+
+/* ------------------------------------------------------------------ Pcg on a given Csr */
+
+typedef struct ml_pcg ml_pcg;
+
+/* Pcg::new (iterative.rs:201-207): uploads a lower-triangle CSR in the reference layout
+ * (n rows, offsets[n+1], cols[nnz], values[nnz]) and factors IC(0) on the device, trying the
+ * shifts {0, 1e-3, 1e-2, 0.1, 1, 10} in turn. ML_SINGULAR if every shift fails. */
+ml_status ml_pcg_create(int64_t n, const int64_t* offsets, const int64_t* cols,
+                        const double* values, ml_precision precision, ml_pcg** out);
+/* Pcg::solve (iterative.rs:210-248) for nrhs right-hand sides stored one after the other
+ * (b[k*n .. k*n+n)), x likewise. tol <= 0 and maxit <= 0 select the reference's 1e-12 and
+ * 10000. A zero right-hand side gives exactly x = 0. info may be null; with nrhs > 1 it
+ * holds the largest iteration count and residual. */
+ml_status ml_pcg_solve(ml_pcg* h, const double* b, double* x, int64_t nrhs, double tol,
+                       int64_t maxit, ml_solve_info* info);
+ml_status ml_pcg_solve_device(ml_pcg* h, const double* b_device, double* x_device,
+                              double tol, int64_t maxit, ml_solve_info* info);
+/* y = A x (Csr::multiply, iterative.rs:131-142), host buffers. */
+ml_status ml_pcg_multiply(ml_pcg* h, const double* x, double* y);
+/* z = (L L^T)^-1 r (Csr::precondition, iterative.rs:173-188), host buffers. */
+ml_status ml_pcg_precondition(ml_pcg* h, const double* r, double* z);
+/* The IC(0) factor values aligned with the CSR (Pcg::l, iterative.rs:196) and the shift. */
+ml_status ml_pcg_factor(ml_pcg* h, double* l, int32_t* shift_index);
+void ml_pcg_destroy(ml_pcg* h);
+
+/* ------------------------------------------------------------------ the static model */
+
+typedef struct ml_model ml_model;
+
+/* Model::new + FreeSystem::new up to the factorisation (model.rs:251, 1442-1499) for a mesh
+ * of one element family: nodes[n_nodes*3], conn[n_elements*nen] (nen 4 = Tet4, 10 = Tet10),
+ * young/nu per element (material.rs:16-34), fixed[3*n_nodes] (0/1) with imposed values
+ * (model.rs:1133-1139). Validates the mesh (mesh.rs:257-348), builds free numbering, Graph,
+ * RCM and the CSR pattern on the host, uploads and assembles K_ff on the device.
+ * solver: ML_AUTO/ML_ITERATIVE run the device IC(0)-PCG; ML_DIRECT is refused
+ * (ML_INVALID_ARGUMENT: the skyline Cholesky is not part of this library). */
+ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elements, int nen,
+                          const int64_t* conn, const double* young, const double* nu,
+                          const uint8_t* fixed, const double* imposed, ml_solver_kind solver,
+                          ml_precision precision, ml_model** out, ml_solve_info* info);
+/* Recompute the element stiffnesses and K_ff values on the device (after ml_model_set_young). */
+ml_status ml_model_assemble(ml_model* m);
+/* Replace the per-element Young's moduli (n_elements values, host) without touching the
+ * pattern; call ml_model_assemble and ml_model_factor afterwards. */
+ml_status ml_model_set_young(ml_model* m, const double* young);
+/* kff.factorize (model.rs:1496) -> Pcg::new: device IC(0) with the shift ladder. */
+ml_status ml_model_factor(ml_model* m, ml_solve_info* info);
+/* FreeSystem::solve (model.rs:1502-1516) + reactions (model.rs:586-593): f[3*n_nodes] nodal
+ * loads (model.rs:1090-1121, pressures included by the caller or ml_model_add_pressures);
+ * u[3*n_nodes] out; reactions[3*n_nodes] out or null. Host buffers. */
+ml_status ml_model_solve(ml_model* m, const double* f, double* u, double* reactions,
+                         ml_solve_info* info);
+/* Same with device buffers (f, u, reactions of 3*n_nodes doubles; reactions may be null). */
+ml_status ml_model_solve_device(ml_model* m, const double* f_device, double* u_device,
+                                double* reactions_device, ml_solve_info* info);
+/* element_stresses for every element (model.rs:596-598, element.rs:270-302): out[n_elements
+ * x ngp x 9] = x, y, z, sxx, syy, szz, tyz, txz, txy per Gauss point (1 for Tet4, 4 Tet10). */
+ml_status ml_model_stresses(ml_model* m, const double* u, double* out);
+/* face_pressure + FacePressure::add_load (assembly.rs:638-674) for dead pressures
+ * (element, face, value), added to f[3*n_nodes] in the given order. Host. */
+ml_status ml_model_add_pressures(ml_model* m, int64_t count, const int64_t* elements,
+                                 const int64_t* faces, const double* values, double* f);
+/* Sizes: free DOFs, stored lower-triangle entries, DAG depth of the IC(0) factor. */
+int64_t ml_model_nfree(const ml_model* m);
+int64_t ml_model_nnz(const ml_model* m);
+int64_t ml_model_levels(const ml_model* m);
+/* K_ff in the reference layout (after assembly): order[nfree] (RCM, over free indices),
+ * offsets[nfree+1], cols[nnz], values[nnz]; any pointer may be null. */
+ml_status ml_model_csr(ml_model* m, int64_t* order, int64_t* offsets, int64_t* cols,
+                       double* values);
+/* Compliance sensitivity of SIMP-type moduli: for C = f.u with u the last solution,
+ * dC/dE_e = -u_e^T (K_e/E_e) u_e (the adjoint of compliance is u itself). out[n_elements]. */
+ml_status ml_model_compliance_gradient(ml_model* m, double* out);
+void ml_model_destroy(ml_model* m);
+
+/* Kernel-duration accounting with HIP events on the model's stream (no counterpart in the
+ * reference; for benchmarks). Returns the accumulated milliseconds and launch counts per kind
+ * (0 SpMV FP64, 1 SpMV FP32, 2 preconditioner FP64 (forward + backward solve), 3
+ * preconditioner FP32, 4 IC(0), 5 element stiffness, 6 gather assembly, 7 PCG vector updates)
+ * into ms[8] and counts[8] (either may be null), then resets and enables (1) or disables (0)
+ * accounting; enable < 0 only reads. */
+ml_status ml_model_profile(ml_model* m, int enable, double* ms, int64_t* counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAILLE_HIP_H */

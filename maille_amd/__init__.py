@@ -1,0 +1,450 @@
+"""maille_amd — MI355X implementation of maille-core's 3D linear-elastic solve path.
+
+A ctypes face over ``libmaille_hip.so`` (include/maille_hip.h) that mirrors the reference's
+public items for this path (crates/maille-core of timothee-faget/maille v0.14.0):
+
+    Graph.from_elements / reverse_cuthill_mckee   iterative.rs:38, solve.rs:200
+    Csr.zeros / Csr.add                            iterative.rs:91, :121
+    Pcg(csr) / Pcg.solve                           iterative.rs:201, :210
+    solid_d / solid_stiffness                      element.rs:193, :223
+    Model(...).solve()                             model.rs:548
+
+Errors mirror MailleError (error.rs): ``Singular`` and ``IterativeNotConverged`` are raised as
+the exceptions of the same names. There is no CPU fallback: without the built library or a GPU
+the calls that compute raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libmaille_hip.so")
+_lib = None
+
+ML_OK, ML_SINGULAR, ML_NOT_CONVERGED, ML_INVALID_ARGUMENT, ML_OUT_OF_PATTERN, ML_HIP_ERROR, \
+    ML_INVALID_MESH, ML_DEVICE_TIMEOUT = range(8)
+FP64, MIXED = 0, 1
+AUTO, DIRECT, ITERATIVE = 0, 1, 2
+TOLERANCE = 1e-12          # iterative.rs:9
+MAX_ITERATIONS = 10_000    # iterative.rs:11
+ITERATIVE_FROM = 100_000   # solve.rs:185
+
+EXPORTS = (
+    "ml_last_error", "ml_device_count", "ml_solid_d", "ml_stiffness_batch",
+    "ml_pattern_build", "ml_pattern_n", "ml_pattern_nnz", "ml_pattern_get",
+    "ml_pattern_ntargets", "ml_pattern_graph", "ml_pattern_free",
+    "ml_pcg_create", "ml_pcg_solve", "ml_pcg_solve_device", "ml_pcg_multiply",
+    "ml_pcg_precondition", "ml_pcg_factor", "ml_pcg_destroy",
+    "ml_model_create", "ml_model_assemble", "ml_model_set_young", "ml_model_factor",
+    "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_add_pressures",
+    "ml_model_nfree", "ml_model_nnz", "ml_model_levels", "ml_model_csr",
+    "ml_model_compliance_gradient", "ml_model_destroy", "ml_model_profile",
+)
+
+
+class MailleError(Exception):
+    """Base of the errors the library reports (error.rs:8)."""
+
+
+class Singular(MailleError):
+    """MailleError::Singular (error.rs:224)."""
+
+
+class IterativeNotConverged(MailleError):
+    """MailleError::IterativeNotConverged { iterations, residual } (error.rs:226-229)."""
+
+    def __init__(self, iterations, residual):
+        super().__init__(f"PCG did not converge: {iterations} iterations, residual {residual:e}")
+        self.iterations = iterations
+        self.residual = residual
+
+
+class InvalidMesh(MailleError):
+    """Mesh::new refusals (mesh.rs:257-348): ``kind`` and the offending ``element``."""
+
+    KINDS = {1: "degenerate tetrahedron", 2: "inverted tetrahedron", 3: "curved Tet10 edge",
+             4: "non-finite coordinate", 5: "unknown node", 6: "bad material", 7: "empty mesh"}
+
+    def __init__(self, kind, element):
+        super().__init__(f"{self.KINDS.get(kind, kind)} (element/node {element})")
+        self.kind = kind
+        self.element = element
+
+
+class DeviceError(MailleError):
+    """A HIP failure or an expired device wait."""
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("residual", C.c_double),
+                ("true_residual", C.c_double), ("shift_index", C.c_int32),
+                ("refinements", C.c_int32), ("bad_element", C.c_int64),
+                ("mesh_error", C.c_int32), ("reserved", C.c_int32),
+                ("seconds", C.c_double * 6)]
+
+    def as_dict(self):
+        return {"iterations": self.iterations, "residual": self.residual,
+                "true_residual": self.true_residual, "shift_index": self.shift_index,
+                "refinements": self.refinements, "seconds": list(self.seconds)}
+
+
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int64)
+_I32 = C.POINTER(C.c_int32)
+_U8 = C.POINTER(C.c_uint8)
+_S = C.POINTER(SolveInfo)
+_i64 = C.c_int64
+_int = C.c_int
+
+
+def lib():
+    """The loaded library; raises if it has not been built (no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise ImportError(f"{_LIB_PATH} is missing: run `python -m maille_amd.build` first")
+        L = C.CDLL(_LIB_PATH)
+        sig = {
+            "ml_last_error": (C.c_char_p, []),
+            "ml_device_count": (_int, []),
+            "ml_solid_d": (_int, [C.c_double, C.c_double, _D]),
+            "ml_stiffness_batch": (_int, [_int, _i64, _D, _D, _D, _D]),
+            "ml_pattern_build": (_int, [_i64, _i64, _I, _I, _PP]),
+            "ml_pattern_n": (_i64, [_P]),
+            "ml_pattern_nnz": (_i64, [_P]),
+            "ml_pattern_get": (_int, [_P, _I, _I, _I]),
+            "ml_pattern_ntargets": (_i64, [_P]),
+            "ml_pattern_graph": (_int, [_P, _I, _I]),
+            "ml_pattern_free": (None, [_P]),
+            "ml_pcg_create": (_int, [_i64, _I, _I, _D, _int, _PP]),
+            "ml_pcg_solve": (_int, [_P, _D, _D, _i64, C.c_double, _i64, _S]),
+            "ml_pcg_solve_device": (_int, [_P, _P, _P, C.c_double, _i64, _S]),
+            "ml_pcg_multiply": (_int, [_P, _D, _D]),
+            "ml_pcg_precondition": (_int, [_P, _D, _D]),
+            "ml_pcg_factor": (_int, [_P, _D, _I32]),
+            "ml_pcg_destroy": (None, [_P]),
+            "ml_model_create": (_int, [_i64, _D, _i64, _int, _I, _D, _D, _U8, _D, _int, _int,
+                                       _PP, _S]),
+            "ml_model_assemble": (_int, [_P]),
+            "ml_model_set_young": (_int, [_P, _D]),
+            "ml_model_factor": (_int, [_P, _S]),
+            "ml_model_solve": (_int, [_P, _D, _D, _D, _S]),
+            "ml_model_solve_device": (_int, [_P, _P, _P, _P, _S]),
+            "ml_model_stresses": (_int, [_P, _D, _D]),
+            "ml_model_add_pressures": (_int, [_P, _i64, _I, _I, _D, _D]),
+            "ml_model_nfree": (_i64, [_P]),
+            "ml_model_nnz": (_i64, [_P]),
+            "ml_model_levels": (_i64, [_P]),
+            "ml_model_csr": (_int, [_P, _I, _I, _I, _D]),
+            "ml_model_compliance_gradient": (_int, [_P, _D]),
+            "ml_model_destroy": (None, [_P]),
+            "ml_model_profile": (_int, [_P, _int, _D, _I]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status, info=None):
+    if status == ML_OK:
+        return
+    if status == ML_SINGULAR:
+        raise Singular("singular matrix")
+    if status == ML_NOT_CONVERGED:
+        it = info.iterations if info is not None else -1
+        res = info.residual if info is not None else float("nan")
+        raise IterativeNotConverged(it, res)
+    if status == ML_INVALID_MESH and info is not None:
+        raise InvalidMesh(info.mesh_error, info.bad_element)
+    msg = lib().ml_last_error().decode(errors="replace")
+    if status in (ML_INVALID_ARGUMENT, ML_OUT_OF_PATTERN):
+        raise ValueError(msg or f"status {status}")
+    raise DeviceError(f"status {status}: {msg}")
+
+
+def _d(a):
+    return a.ctypes.data_as(_D)
+
+
+def _i(a):
+    return a.ctypes.data_as(_I)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ai64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def device_count() -> int:
+    return lib().ml_device_count()
+
+
+# ------------------------------------------------------------------ element level
+def solid_d(young: float, nu: float) -> np.ndarray:
+    """element.rs:193 — 6x6 isotropic D in Voigt order (xx, yy, zz, yz, xz, xy)."""
+    d = np.zeros((6, 6))
+    _check(lib().ml_solid_d(young, nu, _d(d)))
+    return d
+
+
+def stiffness_batch(coords: np.ndarray, young, nu) -> np.ndarray:
+    """solid_stiffness (element.rs:223) on the device for a batch: coords (ne, nen, 3) ->
+    Ke (ne, 3nen, 3nen) with Ke[e][j][i] as the reference's ``ke[j][i]``."""
+    coords = _f64(coords)
+    ne, nen, _ = coords.shape
+    young = _f64(np.broadcast_to(young, (ne,)))
+    nu = _f64(np.broadcast_to(nu, (ne,)))
+    m = 3 * nen
+    ke = np.zeros((ne, m, m))
+    _check(lib().ml_stiffness_batch(nen, ne, _d(coords), _d(young), _d(nu), _d(ke)))
+    return ke
+
+
+def solid_stiffness(coords: np.ndarray, young: float, nu: float) -> np.ndarray:
+    """solid_stiffness of one tetrahedron (Tet4: 4x3 coords, Tet10: 10x3)."""
+    return stiffness_batch(np.asarray(coords)[None], young, nu)[0]
+
+
+# ------------------------------------------------------------------ graph, RCM, CSR pattern
+class Graph:
+    """iterative.rs:19-77 — DOF adjacency, built with its RCM order and lower pattern."""
+
+    def __init__(self, n, elements):
+        lens = [len(e) for e in elements]
+        off = np.zeros(len(elements) + 1, dtype=np.int64)
+        off[1:] = np.cumsum(lens)
+        items = np.array([v for e in elements for v in e] or [0], dtype=np.int64)
+        h = C.c_void_p()
+        _check(lib().ml_pattern_build(n, len(elements), _i(off), _i(items), C.byref(h)))
+        self._h = h
+        self.n = n
+
+    @classmethod
+    def from_elements(cls, n, elements):
+        return cls(n, elements)
+
+    def neighbours(self, v):
+        off, tgt = self.arrays()
+        return tgt[off[v]:off[v + 1]]
+
+    def arrays(self):
+        nt = lib().ml_pattern_ntargets(self._h)
+        off = np.zeros(self.n + 1, dtype=np.int64)
+        tgt = np.zeros(max(nt, 1), dtype=np.int64)
+        _check(lib().ml_pattern_graph(self._h, _i(off), _i(tgt)))
+        return off, tgt[:nt]
+
+    def reverse_cuthill_mckee(self):
+        """solve.rs:200 — order[new] = old."""
+        order = np.zeros(self.n, dtype=np.int64)
+        _check(lib().ml_pattern_get(self._h, _i(order), None, None))
+        return order
+
+    def lower_pattern(self):
+        """Csr::zeros in the RCM numbering (iterative.rs:91): offsets, cols."""
+        nnz = lib().ml_pattern_nnz(self._h)
+        off = np.zeros(self.n + 1, dtype=np.int64)
+        cols = np.zeros(nnz, dtype=np.int64)
+        _check(lib().ml_pattern_get(self._h, None, _i(off), _i(cols)))
+        return off, cols
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.ml_pattern_free(self._h)
+            self._h = None
+
+
+def reverse_cuthill_mckee(graph: Graph) -> np.ndarray:
+    return graph.reverse_cuthill_mckee()
+
+
+class Csr:
+    """iterative.rs:82-128 — host-side lower-triangle CSR (ascending columns, diagonal last)
+    whose values are handed to the device by ``Pcg``."""
+
+    def __init__(self, offsets, cols, values=None):
+        self.offsets = _ai64(offsets)
+        self.cols = _ai64(cols)
+        self.values = np.zeros(len(self.cols)) if values is None else _f64(values).copy()
+
+    @classmethod
+    def zeros(cls, graph: Graph, order=None):
+        """Csr::zeros (iterative.rs:91) in the graph's RCM numbering."""
+        off, cols = graph.lower_pattern()
+        return cls(off, cols)
+
+    def n(self):
+        return len(self.offsets) - 1
+
+    def add(self, i, j, v):
+        """Csr::add (iterative.rs:121): v into (i, j) and its mirror, inside the pattern."""
+        if j > i:
+            i, j = j, i
+        row = slice(self.offsets[i], self.offsets[i + 1])
+        k = np.searchsorted(self.cols[row], j)
+        if k >= row.stop - row.start or self.cols[row][k] != j:
+            raise ValueError("terme hors du profil")
+        self.values[row.start + k] += v
+
+
+class Pcg:
+    """IC(0)-preconditioned CG on the device (iterative.rs:194-248). ``precision`` FP64 is the
+    drop-in; MIXED runs an FP32 operator inside FP64 refinement."""
+
+    def __init__(self, a: Csr, precision: int = FP64):
+        h = C.c_void_p()
+        _check(lib().ml_pcg_create(a.n(), _i(a.offsets), _i(a.cols), _d(a.values), precision,
+                                   C.byref(h)))
+        self._h = h
+        self.n = a.n()
+        self.nnz = len(a.cols)
+
+    def solve(self, b, tol=0.0, maxit=0, info=False):
+        """Pcg::solve: x with |r| <= 1e-12 |b| (or raises like the reference). b may hold
+        several right-hand sides as rows of a 2-D array."""
+        b = _f64(b)
+        nrhs = 1 if b.ndim == 1 else b.shape[0]
+        x = np.zeros_like(b)
+        si = SolveInfo()
+        _check(lib().ml_pcg_solve(self._h, _d(b), _d(x), nrhs, tol, maxit, C.byref(si)), si)
+        return (x, si.as_dict()) if info else x
+
+    def multiply(self, x):
+        x = _f64(x)
+        y = np.zeros(self.n)
+        _check(lib().ml_pcg_multiply(self._h, _d(x), _d(y)))
+        return y
+
+    def precondition(self, r):
+        r = _f64(r)
+        z = np.zeros(self.n)
+        _check(lib().ml_pcg_precondition(self._h, _d(r), _d(z)))
+        return z
+
+    def factor(self):
+        """(L values aligned with the CSR, shift index)."""
+        l = np.zeros(self.nnz)
+        sh = C.c_int32(-1)
+        _check(lib().ml_pcg_factor(self._h, _d(l), C.byref(sh)))
+        return l, sh.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.ml_pcg_destroy(self._h)
+            self._h = None
+
+
+# ------------------------------------------------------------------ the static model
+class Model:
+    """Model::new + Model::solve (model.rs:251-548) for a Tet4/Tet10 mesh on the device.
+    ``problem`` is a ``maille_amd.meshes.Problem`` (nodes, conn, young, nu, fixed, imposed,
+    nodal loads f, dead pressures)."""
+
+    def __init__(self, problem, precision: int = FP64, solver: int = AUTO):
+        self.problem = problem
+        nodes = _f64(problem.nodes)
+        conn = _ai64(problem.conn)
+        ne, nen = conn.shape
+        young = _f64(np.broadcast_to(problem.young, (ne,)))
+        nu = _f64(np.broadcast_to(problem.nu, (ne,)))
+        fixed = np.ascontiguousarray(problem.fixed, dtype=np.uint8)
+        imposed = _f64(problem.imposed)
+        self.ndof = 3 * len(nodes)
+        self.ne, self.nen = ne, nen
+        h = C.c_void_p()
+        si = SolveInfo()
+        _check(lib().ml_model_create(len(nodes), _d(nodes), ne, nen, _i(conn), _d(young),
+                                     _d(nu), fixed.ctypes.data_as(_U8), _d(imposed), solver,
+                                     precision, C.byref(h), C.byref(si)), si)
+        self._h = h
+        self.create_info = si.as_dict()
+
+    def loads(self):
+        """Model::assemble's F (model.rs:1089-1121): nodal loads plus dead pressures."""
+        f = _f64(self.problem.f).copy()
+        p = self.problem.pressures
+        if p:
+            arr = np.array(p, dtype=np.float64)
+            e, fc, v = _ai64(arr[:, 0]), _ai64(arr[:, 1]), _f64(arr[:, 2])
+            _check(lib().ml_model_add_pressures(self._h, len(e), _i(e), _i(fc), _d(v), _d(f)))
+        return f
+
+    def solve(self, f=None, reactions=True):
+        """(displacements over all DOFs, reactions or None, info dict)."""
+        f = self.loads() if f is None else _f64(f)
+        u = np.zeros(self.ndof)
+        r = np.zeros(self.ndof) if reactions else None
+        si = SolveInfo()
+        _check(lib().ml_model_solve(self._h, _d(f), _d(u), _d(r) if reactions else None,
+                                    C.byref(si)), si)
+        return u, r, si.as_dict()
+
+    def solve_device(self, f_ptr, u_ptr, r_ptr=None):
+        """Solve with device buffers (integer device addresses of 3N doubles)."""
+        si = SolveInfo()
+        _check(lib().ml_model_solve_device(self._h, f_ptr, u_ptr, r_ptr, C.byref(si)), si)
+        return si.as_dict()
+
+    def factor(self):
+        si = SolveInfo()
+        _check(lib().ml_model_factor(self._h, C.byref(si)), si)
+        return si.as_dict()
+
+    def assemble(self):
+        _check(lib().ml_model_assemble(self._h))
+
+    def set_young(self, young):
+        young = _f64(young)
+        _check(lib().ml_model_set_young(self._h, _d(young)))
+
+    def stresses(self, u):
+        u = _f64(u)
+        ngp = 1 if self.nen == 4 else 4
+        out = np.zeros((self.ne, ngp, 9))
+        _check(lib().ml_model_stresses(self._h, _d(u), _d(out)))
+        return out
+
+    def nfree(self):
+        return lib().ml_model_nfree(self._h)
+
+    def nnz(self):
+        return lib().ml_model_nnz(self._h)
+
+    def levels(self):
+        return lib().ml_model_levels(self._h)
+
+    def csr(self):
+        """(order, offsets, cols, values) of K_ff in the reference layout."""
+        n, nnz = self.nfree(), self.nnz()
+        order = np.zeros(n, dtype=np.int64)
+        off = np.zeros(n + 1, dtype=np.int64)
+        cols = np.zeros(nnz, dtype=np.int64)
+        vals = np.zeros(nnz)
+        _check(lib().ml_model_csr(self._h, _i(order), _i(off), _i(cols), _d(vals)))
+        return order, off, cols, vals
+
+    def profile(self, enable=-1):
+        """Kernel durations (ms) and launch counts accumulated since the last reset; enable
+        1/0 resets and switches the accounting on/off."""
+        ms = np.zeros(8)
+        cnt = np.zeros(8, dtype=np.int64)
+        _check(lib().ml_model_profile(self._h, enable, _d(ms), _i(cnt)))
+        names = ("spmv64", "spmv32", "precond64", "precond32", "ic0", "stiffness",
+                 "assembly", "vector")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(names)}
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.ml_model_destroy(self._h)
+            self._h = None

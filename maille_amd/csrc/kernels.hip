@@ -1,0 +1,743 @@
+// Device kernels of the 3D linear-elastic solve path, written for gfx950 (CDNA4, 64-wide
+// wavefronts). Included once, by solver.hip.
+//
+// Floating-point contraction is switched off in the kernels that restate the reference's
+// arithmetic (element stiffness, assembly, SpMV, IC(0), triangular solves, PCG updates): the
+// reference is Rust, which never fuses a*b+c, and these kernels reproduce its rounding
+// operation for operation. Only the reductions (dot products) use a different summation order.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace mlk {
+
+// ------------------------------------------------------------------ small device helpers
+__device__ inline uint64_t bits_of(double v) { return __double_as_longlong(v); }
+__device__ inline uint32_t bits_of(float v) { return __float_as_uint(v); }
+__device__ inline bool is_sentinel(double v) { return bits_of(v) == SENTINEL64; }
+__device__ inline bool is_sentinel(float v) { return bits_of(v) == SENTINEL32; }
+
+template <typename T> struct Bits;
+template <> struct Bits<double> { typedef unsigned long long U; static constexpr U S = SENTINEL64; };
+template <> struct Bits<float> { typedef unsigned int U; static constexpr U S = SENTINEL32; };
+
+template <typename T>
+__device__ inline T from_bits(typename Bits<T>::U u);
+template <>
+__device__ inline double from_bits<double>(unsigned long long u) { return __longlong_as_double(u); }
+template <>
+__device__ inline float from_bits<float>(unsigned int u) { return __uint_as_float(u); }
+
+// A store other wavefronts of this kernel may wait for: device-scope, so it is written through
+// to the point of coherence shared by every XCD.
+template <typename T>
+__device__ inline void publish(T* p, T v) {
+  typedef typename Bits<T>::U U;
+  __hip_atomic_store(reinterpret_cast<U*>(p), (U)bits_of(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Value of a slot written once per launch by publish(): a plain load answers when the slot is
+// already written (a slot only ever holds the sentinel or its final value); otherwise poll
+// with device-scope loads. The wait is bounded: past `limit` ticks of the 100 MHz real-time
+// counter (or once another wave has given up) it raises *abort and returns 0, so every
+// wavefront always reaches the end of the kernel.
+template <typename T>
+__device__ inline T await_value(T* p, int* abort_flag, long long limit) {
+  typedef typename Bits<T>::U U;
+  T v = *p;
+  if (!is_sentinel(v)) return v;
+  long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
+  for (;;) {
+    U u = __hip_atomic_load(reinterpret_cast<U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (u != Bits<T>::S) return from_bits<T>(u);
+    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return T(0);
+    if ((long long)__builtin_amdgcn_s_memrealtime() - t0 > limit) {
+      __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return T(0);
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+__device__ inline double lane_broadcast(double v, int lane) {
+  int lo = __builtin_amdgcn_readlane((int)(__double_as_longlong(v) & 0xffffffffll), lane);
+  int hi = __builtin_amdgcn_readlane((int)(__double_as_longlong(v) >> 32), lane);
+  return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ inline float lane_broadcast(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Deterministic block sum (fixed tree), result valid in thread 0.
+template <int BS>
+__device__ inline double block_sum(double v) {
+  __shared__ double red[BS];
+  red[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------------ fills and copies
+template <typename T>
+__global__ void k_fill_bits(T* p, int64_t n, typename Bits<T>::U value) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) reinterpret_cast<typename Bits<T>::U*>(p)[i] = value;
+}
+
+__global__ void k_fill_f64(double* p, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_i32(int* p, int64_t n, int v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <typename S, typename D>
+__global__ void k_gather(int64_t n, const int* __restrict__ src, const S* __restrict__ from,
+                         D* __restrict__ to) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    int s = src[i];
+    to[i] = s >= 0 ? (D)from[s] : D(0);
+  }
+}
+
+template <typename S, typename D>
+__global__ void k_convert(int64_t n, const S* __restrict__ from, D* __restrict__ to) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) to[i] = (D)from[i];
+}
+
+// ------------------------------------------------------------------ element stiffness
+// Tet shape functions (shape.rs:95-153) and Mapping3d (shape.rs:157-190) at one Gauss point;
+// g receives dN/dx. Returns det J.
+template <int NEN>
+__device__ inline double tet_mapping(const double (&x)[NEN][3], double xi, double eta,
+                                     double zeta, double (&g)[NEN][3]) {
+#pragma clang fp contract(off)
+  double dn[NEN][3];
+  const double dl[4][3] = {{-1.0, -1.0, -1.0}, {1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+  if constexpr (NEN == 4) {
+    for (int a = 0; a < 4; ++a)
+      for (int c = 0; c < 3; ++c) dn[a][c] = dl[a][c];
+  } else {
+    const double l[4] = {1.0 - xi - eta - zeta, xi, eta, zeta};
+    const int ed[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+    for (int a = 0; a < 4; ++a)
+      for (int c = 0; c < 3; ++c) dn[a][c] = (4.0 * l[a] - 1.0) * dl[a][c];
+    for (int k = 0; k < 6; ++k)
+      for (int c = 0; c < 3; ++c)
+        dn[4 + k][c] = 4.0 * (l[ed[k][0]] * dl[ed[k][1]][c] + l[ed[k][1]] * dl[ed[k][0]][c]);
+  }
+  double j[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int k = 0; k < NEN; ++k)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) j[r][c] += dn[k][r] * x[k][c];
+  double cof[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      cof[r][c] = j[r1][c1] * j[r2][c2] - j[r1][c2] * j[r2][c1];
+    }
+  double det = 0.0;
+  for (int c = 0; c < 3; ++c) det += j[0][c] * cof[0][c];
+  double inv[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) inv[r][c] = cof[c][r] / det;
+  for (int k = 0; k < NEN; ++k)
+    for (int r = 0; r < 3; ++r) {
+      double s = 0.0;
+      for (int c = 0; c < 3; ++c) s += inv[r][c] * dn[k][c];
+      g[k][r] = s;
+    }
+  return det;
+}
+
+// Column i of B (element.rs:209-220), Voigt (xx, yy, zz, yz, xz, xy).
+template <int NEN>
+__device__ inline void b_column(const double (&g)[NEN][3], int i, double (&col)[6]) {
+  const double gx = g[i / 3][0], gy = g[i / 3][1], gz = g[i / 3][2];
+  const int c = i % 3;
+  col[0] = c == 0 ? gx : 0.0;
+  col[1] = c == 1 ? gy : 0.0;
+  col[2] = c == 2 ? gz : 0.0;
+  col[3] = c == 0 ? 0.0 : (c == 1 ? gz : gy);
+  col[4] = c == 0 ? gz : (c == 1 ? 0.0 : gx);
+  col[5] = c == 0 ? gy : (c == 1 ? gx : 0.0);
+}
+
+// solid_d (element.rs:193-206)
+__device__ inline void solid_d(double young, double nu, double (&d)[6][6]) {
+#pragma clang fp contract(off)
+  double c = young / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  double gm = young / (2.0 * (1.0 + nu));
+  for (int r = 0; r < 6; ++r)
+    for (int k = 0; k < 6; ++k) d[r][k] = 0.0;
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k) d[r][k] = (r == k) ? c * (1.0 - nu) : c * nu;
+  for (int r = 3; r < 6; ++r) d[r][r] = gm;
+}
+
+template <int NEN> struct Rule;
+template <> struct Rule<4> { static constexpr int N = 1; };
+template <> struct Rule<10> { static constexpr int N = 4; };
+
+// Gauss points: TETRA_1 (quadrature.rs:100-105) for Tet4, TETRA_4 (:108-138) for Tet10.
+__device__ inline void gauss_point(int nen, int q, double& xi, double& eta, double& zeta,
+                                   double& w) {
+  const double A = 0.585410196624968500, B = 0.138196601125010500;
+  if (nen == 4) { xi = eta = zeta = 0.25; w = 1.0 / 6.0; return; }
+  xi = q == 1 ? A : B;
+  eta = q == 2 ? A : B;
+  zeta = q == 3 ? A : B;
+  w = 1.0 / 24.0;
+}
+
+// solid_stiffness (element.rs:223-243): one thread computes column i of Ke for element e,
+// i.e. ke[j][i] for every j, accumulated over the Gauss points in the reference's order.
+// Output slot of ke[j][i] is (j*M + i) * ne + e (entry-major, elements contiguous).
+template <int NEN>
+__global__ __launch_bounds__(256) void k_stiffness(int64_t ne, const double* __restrict__ xyz,
+                                                   const int* __restrict__ conn,
+                                                   const double* __restrict__ young,
+                                                   const double* __restrict__ nu,
+                                                   double* __restrict__ ke) {
+#pragma clang fp contract(off)
+  constexpr int M = 3 * NEN;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)M * ne) return;
+  const int i = (int)(t / ne);
+  const int64_t e = t - (int64_t)i * ne;
+  double x[NEN][3];
+  for (int a = 0; a < NEN; ++a) {
+    int node = conn[e * NEN + a];
+    for (int c = 0; c < 3; ++c) x[a][c] = xyz[3 * (int64_t)node + c];
+  }
+  double d[6][6];
+  solid_d(young[e], nu[e], d);
+  double acc[M];
+  for (int j = 0; j < M; ++j) acc[j] = 0.0;
+  for (int q = 0; q < Rule<NEN>::N; ++q) {
+    double xi, eta, zeta, w;
+    gauss_point(NEN, q, xi, eta, zeta, w);
+    double g[NEN][3];
+    double det = tet_mapping<NEN>(x, xi, eta, zeta, g);
+    double factor = w * fabs(det);
+    double bi[6], db[6];
+    b_column<NEN>(g, i, bi);
+    for (int r = 0; r < 6; ++r) {
+      double s = 0.0;
+      for (int c = 0; c < 6; ++c) s += d[r][c] * bi[c];
+      db[r] = s;
+    }
+    for (int j = 0; j < M; ++j) {
+      double bj[6];
+      b_column<NEN>(g, j, bj);
+      double s = 0.0;
+      for (int r = 0; r < 6; ++r) s += bj[r] * db[r];
+      acc[j] += factor * s;
+    }
+  }
+  for (int j = 0; j < M; ++j) ke[((int64_t)j * M + i) * ne + e] = acc[j];
+}
+
+// solid_stresses (element.rs:270-302) at Gauss point q of element e, thread per (e, q):
+// x, y, z, sxx, syy, szz, tyz, txz, txy.
+template <int NEN>
+__global__ __launch_bounds__(256) void k_stresses(int64_t ne, const double* __restrict__ xyz,
+                                                  const int* __restrict__ conn,
+                                                  const double* __restrict__ young,
+                                                  const double* __restrict__ nu,
+                                                  const double* __restrict__ u,
+                                                  double* __restrict__ out) {
+#pragma clang fp contract(off)
+  constexpr int M = 3 * NEN, NG = Rule<NEN>::N;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ne * NG) return;
+  const int64_t e = t / NG;
+  const int q = (int)(t - e * NG);
+  double x[NEN][3], ue[M];
+  for (int a = 0; a < NEN; ++a) {
+    int64_t node = conn[e * NEN + a];
+    for (int c = 0; c < 3; ++c) {
+      x[a][c] = xyz[3 * node + c];
+      ue[3 * a + c] = u[3 * node + c];
+    }
+  }
+  double d[6][6];
+  solid_d(young[e], nu[e], d);
+  double xi, eta, zeta, w;
+  gauss_point(NEN, q, xi, eta, zeta, w);
+  double g[NEN][3];
+  tet_mapping<NEN>(x, xi, eta, zeta, g);
+  double eps[6];
+  for (int r = 0; r < 6; ++r) {
+    double s = 0.0;
+    for (int k = 0; k < M; ++k) {
+      double col[6];
+      b_column<NEN>(g, k, col);
+      s += col[r] * ue[k];
+    }
+    eps[r] = s;
+  }
+  // shape values for the Gauss-point position
+  double n[NEN];
+  {
+    const double l[4] = {1.0 - xi - eta - zeta, xi, eta, zeta};
+    if constexpr (NEN == 4) {
+      for (int a = 0; a < 4; ++a) n[a] = l[a];
+    } else {
+      const int ed[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+      for (int a = 0; a < 4; ++a) n[a] = l[a] * (2.0 * l[a] - 1.0);
+      for (int k = 0; k < 6; ++k) n[4 + k] = 4.0 * l[ed[k][0]] * l[ed[k][1]];
+    }
+  }
+  double* o = out + t * 9;
+  for (int k = 0; k < 3; ++k) {
+    double s = 0.0;
+    for (int a = 0; a < NEN; ++a) s += n[a] * x[a][k];
+    o[k] = s;
+  }
+  for (int r = 0; r < 6; ++r) {
+    double s = 0.0;
+    for (int c = 0; c < 6; ++c) s += d[r][c] * eps[c];
+    o[3 + r] = s;
+  }
+}
+
+// ------------------------------------------------------------------ assembly
+// Csr::add in the reference's scatter order (model.rs:1480-1492): entry q sums its element
+// contributions in element order, starting from 0.0.
+__global__ __launch_bounds__(256) void k_assemble(int64_t nnz, const int64_t* __restrict__ off,
+                                                  const uint32_t* __restrict__ slot,
+                                                  const double* __restrict__ ke,
+                                                  double* __restrict__ vals) {
+#pragma clang fp contract(off)
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnz) return;
+  double v = 0.0;
+  for (int64_t c = off[q]; c < off[q + 1]; ++c) v += ke[slot[c]];
+  vals[q] = v;
+}
+
+// coupling[a] = sum of ke * u_c over fixed columns, element order (model.rs:1489)
+__global__ __launch_bounds__(256) void k_coupling(int64_t nfree, const int64_t* __restrict__ off,
+                                                  const uint32_t* __restrict__ slot,
+                                                  const int* __restrict__ dof,
+                                                  const double* __restrict__ ke,
+                                                  const double* __restrict__ imposed,
+                                                  double* __restrict__ coupling) {
+#pragma clang fp contract(off)
+  int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nfree) return;
+  double c = 0.0;
+  for (int64_t k = off[a]; k < off[a + 1]; ++k) c += ke[slot[k]] * imposed[dof[k]];
+  coupling[a] = c;
+}
+
+// FreeSystem::solve's right-hand side (model.rs:1503-1508): rhs[rank] = -coupling + f
+__global__ __launch_bounds__(256) void k_rhs(int64_t ndof, const int* __restrict__ rank_of_dof,
+                                             const double* __restrict__ f,
+                                             const double* __restrict__ coupling,
+                                             double* __restrict__ rhs) {
+#pragma clang fp contract(off)
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ndof) return;
+  int r = rank_of_dof[g];
+  if (r < 0) return;
+  double c = coupling ? coupling[r] : 0.0;
+  rhs[r] = -c + f[g];
+}
+
+// model.rs:1509-1514: u[g] = x[rank] on free DOFs, the imposed value on fixed ones
+__global__ __launch_bounds__(256) void k_scatter_u(int64_t ndof, const int* __restrict__ rank_of_dof,
+                                                   const double* __restrict__ x,
+                                                   const double* __restrict__ imposed,
+                                                   double* __restrict__ u) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ndof) return;
+  int r = rank_of_dof[g];
+  u[g] = r >= 0 ? x[r] : imposed[g];
+}
+
+// product (model.rs:1353-1369) on the fixed DOFs, then R -= F (model.rs:586-593)
+template <int NEN>
+__global__ __launch_bounds__(256) void k_reactions(int64_t nfixed, const int* __restrict__ rdof,
+                                                   const int64_t* __restrict__ roff,
+                                                   const int64_t* __restrict__ item,
+                                                   const double* __restrict__ ke,
+                                                   const int* __restrict__ conn, int64_t ne,
+                                                   const double* __restrict__ u,
+                                                   const double* __restrict__ f,
+                                                   double* __restrict__ reactions) {
+#pragma clang fp contract(off)
+  constexpr int M = 3 * NEN;
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nfixed) return;
+  double y = 0.0;
+  for (int64_t c = roff[k]; c < roff[k + 1]; ++c) {
+    int64_t e = item[c] / M;
+    int li = (int)(item[c] - e * M);
+    double s = 0.0;
+    for (int lj = 0; lj < M; ++lj) {
+      int64_t h = 3 * (int64_t)conn[e * NEN + lj / 3] + lj % 3;
+      s += ke[((int64_t)li * M + lj) * ne + e] * u[h];
+    }
+    y += s;
+  }
+  int g = rdof[k];
+  reactions[g] = y - f[g];
+}
+
+// ------------------------------------------------------------------ SpMV (sliced ELL)
+// y = A x (Csr::multiply, iterative.rs:131-142). Lane j of a slice sums row j in the order
+// diagonal, lower entries, upper entries: the reference's accumulation order, so the FP64
+// result is bit-identical. Optionally accumulates x.y per block (the PCG curvature p.Ap).
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ soff,
+                                              const int* __restrict__ scol,
+                                              const int* __restrict__ slen,
+                                              const T* __restrict__ sval,
+                                              const T* __restrict__ x, T* __restrict__ y,
+                                              double* __restrict__ partial,
+                                              const PcgScalars* __restrict__ S) {
+#pragma clang fp contract(off)
+  if (S && S->done) return;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  double dotv = 0.0;
+  if (j < n) {
+    const int base = soff[j >> 6] + (j & 63);
+    const int len = slen[j];
+    T s = sval[base] * x[scol[base]];
+    for (int k = 1; k < len; ++k) {
+      const int at = base + (k << 6);
+      s += sval[at] * x[scol[at]];
+    }
+    y[j] = s;
+    dotv = (double)x[j] * (double)s;
+  }
+  if (partial) {
+    double b = block_sum<256>(dotv);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+  }
+}
+
+// ------------------------------------------------------------------ reductions
+template <typename A, typename B>
+__global__ __launch_bounds__(256) void k_dot(int64_t n, const A* __restrict__ a,
+                                             const B* __restrict__ b, double* __restrict__ partial) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = i < n ? (double)a[i] * (double)b[i] : 0.0;
+  double s = block_sum<256>(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Sum of `count` partials in a fixed order, by one block of 256 threads.
+__device__ inline double sum_partials(const double* __restrict__ partial, int count) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < count; k += 256) v += partial[k];
+  return block_sum<256>(v);
+}
+
+// r.r of b for the initial norm; b == 0 => done with x = 0 (iterative.rs:211-216)
+__global__ __launch_bounds__(256) void k_init_norm(const double* __restrict__ partial, int count,
+                                                   PcgScalars* S, double tol) {
+  double s = sum_partials(partial, count);
+  if (threadIdx.x == 0) {
+    S->bnorm = sqrt(s);
+    S->rscale = tol * S->bnorm;
+    S->it = 0;
+    S->status = 0;
+    S->done = S->bnorm == 0.0 ? 1 : 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_init_rz(const double* __restrict__ partial, int count,
+                                                 PcgScalars* S) {
+  double s = sum_partials(partial, count);
+  if (threadIdx.x == 0 && !S->done) S->rz = s;
+}
+
+// curvature p.Ap -> alpha, or Singular (iterative.rs:224-228)
+__global__ __launch_bounds__(256) void k_alpha(const double* __restrict__ partial, int count,
+                                               PcgScalars* S) {
+  if (S->done) return;
+  double curvature = sum_partials(partial, count);
+  if (threadIdx.x == 0) {
+    if (isnan(curvature) || curvature <= 0.0) {
+      S->status = 1;
+      S->done = 1;
+      S->it += 1;
+    } else {
+      S->alpha = S->rz / curvature;
+    }
+  }
+}
+
+// |r| after the update; converged (iterative.rs:233-235) or out of iterations (:244-247)
+__global__ __launch_bounds__(256) void k_converged(const double* __restrict__ partial, int count,
+                                                   PcgScalars* S) {
+  if (S->done) return;
+  double rr = sum_partials(partial, count);
+  if (threadIdx.x == 0) {
+    S->rnorm = sqrt(rr);
+    S->it += 1;
+    if (S->rnorm <= S->rscale) {
+      S->done = 1;
+      S->status = 0;
+    } else if (S->it >= S->maxit) {
+      S->done = 1;
+      S->status = 2;
+    }
+  }
+}
+
+// beta = rz' / rz (iterative.rs:237-239)
+__global__ __launch_bounds__(256) void k_beta(const double* __restrict__ partial, int count,
+                                              PcgScalars* S) {
+  if (S->done) return;
+  double rz_next = sum_partials(partial, count);
+  if (threadIdx.x == 0) {
+    S->beta = rz_next / S->rz;
+    S->rz = rz_next;
+  }
+}
+
+// x += alpha p; r -= alpha Ap (iterative.rs:229-232); partial r.r; fills the two triangular
+// solve outputs with the sentinel for the coming preconditioner application.
+template <typename TV>
+__global__ __launch_bounds__(256) void k_update_xr(int n, const PcgScalars* __restrict__ S,
+                                                   double* __restrict__ x, double* __restrict__ r,
+                                                   const TV* __restrict__ p,
+                                                   const TV* __restrict__ ap,
+                                                   TV* __restrict__ r_low, TV* __restrict__ fill_a,
+                                                   TV* __restrict__ fill_b,
+                                                   double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  if (S->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    const double alpha = S->alpha;
+    x[i] += alpha * (double)p[i];
+    double ri = r[i] - alpha * (double)ap[i];
+    r[i] = ri;
+    if (r_low) r_low[i] = (TV)ri;
+    v = ri * ri;
+    typedef typename Bits<TV>::U U;
+    reinterpret_cast<U*>(fill_a)[i] = Bits<TV>::S;
+    reinterpret_cast<U*>(fill_b)[i] = Bits<TV>::S;
+  }
+  double s = block_sum<256>(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// p = z + beta p (iterative.rs:240-242)
+template <typename TV>
+__global__ __launch_bounds__(256) void k_update_p(int n, const PcgScalars* __restrict__ S,
+                                                  TV* __restrict__ p, const TV* __restrict__ z) {
+#pragma clang fp contract(off)
+  if (S->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (TV)((double)z[i] + S->beta * (double)p[i]);
+}
+
+// r = b - A x in FP64 (the refinement's true residual), partial r.r
+__global__ __launch_bounds__(256) void k_residual(int n, const double* __restrict__ b,
+                                                  const double* __restrict__ ax,
+                                                  double* __restrict__ r,
+                                                  double* __restrict__ partial) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    double ri = b[i] - ax[i];
+    r[i] = ri;
+    v = ri * ri;
+  }
+  double s = block_sum<256>(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_axpy_add(int n, double* __restrict__ x,
+                                                  const double* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += d[i];
+}
+
+// ------------------------------------------------------------------ triangular solves
+// Dataflow solve of a lower-triangular system in "logical" row order: logical row l has its
+// strictly-lower entries [off[l], off[l+1]) with logical columns col[] ascending and the
+// diagonal diag[l]. Vectors are indexed physically: phys(l) = REV ? n-1-l : l, so the backward
+// solve L^T z = y runs through the same kernel on the transposed layout.
+//
+//   SUBSEQ = false: out_l = (rhs_l - sum_p val_p out_col) / diag_l  (forward loop,
+//                   iterative.rs:175-179: sum in ascending column order, then subtract)
+//   SUBSEQ = true:  out_l = ((rhs_l - val_1 out_1) - val_2 out_2 ...) / diag_l (backward loop,
+//                   iterative.rs:180-186: subtractions in decreasing row order)
+//
+// Each wavefront owns blocks of 64 consecutive logical rows, lane t owning row s+t (blocks
+// w, w+G, w+2G... for wave w, so every dependency of the lowest unfinished block is finished).
+// Phase 1 lets every lane consume, in order, the entries whose value is already available;
+// phase 2 resolves the block's rows one after the other: lane k finishes its row (waiting for
+// any late value), publishes it and broadcasts it, and the lanes whose next entry is that row
+// consume it at once. Each row is thus summed sequentially in the reference's order.
+template <typename T, bool REV, bool SUBSEQ>
+__global__ __launch_bounds__(64) void k_trisolve(int n, int nblocks, const int* __restrict__ off,
+                                                 const int* __restrict__ col,
+                                                 const T* __restrict__ val,
+                                                 const T* __restrict__ diag,
+                                                 const T* __restrict__ rhs, T* out,
+                                                 const double* __restrict__ dotvec,
+                                                 double* __restrict__ partial,
+                                                 const PcgScalars* __restrict__ S,
+                                                 int* abort_flag, long long limit) {
+#pragma clang fp contract(off)
+  if (S && S->done) return;
+  __shared__ T own[64];
+  const int lane = threadIdx.x;
+  double dotacc = 0.0;
+  for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const int s = b * 64;
+    const int rows = min(64, n - s);
+    const bool active = lane < rows;
+    const int l = s + lane;
+    const int pl = REV ? n - 1 - l : l;
+    int p = 0, end = 0;
+    T acc = T(0), rl = T(0), dl = T(1);
+    if (active) {
+      p = off[l];
+      end = off[l + 1];
+      rl = rhs[pl];
+      dl = diag[l];
+      acc = SUBSEQ ? rl : T(0);
+    }
+    // phase 1: entries already available, in order
+    while (p < end) {
+      const int j = col[p];
+      if (j >= s) break;
+      const T v = out[REV ? n - 1 - j : j];
+      if (is_sentinel(v)) break;
+      if (SUBSEQ) acc = acc - val[p] * v;
+      else acc = acc + val[p] * v;
+      ++p;
+    }
+    int ncol = (p < end) ? col[p] : n;
+    T nval = (p < end) ? val[p] : T(0);
+    // phase 2: rows of the block in order
+    for (int k = 0; k < rows; ++k) {
+      T y = T(0);
+      if (lane == k) {
+        while (p < end) {
+          const int j = col[p];
+          const T v = j >= s ? own[j - s] : await_value(out + (REV ? n - 1 - j : j), abort_flag, limit);
+          if (SUBSEQ) acc = acc - val[p] * v;
+          else acc = acc + val[p] * v;
+          ++p;
+        }
+        y = SUBSEQ ? acc / dl : (rl - acc) / dl;
+        own[k] = y;
+        publish(out + pl, y);
+        if (dotvec) dotacc += dotvec[pl] * (double)y;
+      }
+      y = lane_broadcast(y, k);
+      if (lane > k && ncol == s + k) {
+        if (SUBSEQ) acc = acc - nval * y;
+        else acc = acc + nval * y;
+        ++p;
+        ncol = (p < end) ? col[p] : n;
+        nval = (p < end) ? val[p] : T(0);
+      }
+    }
+    __syncthreads();
+  }
+  if (partial) {
+    double v = dotacc;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) partial[blockIdx.x] = v;
+  }
+}
+
+// ------------------------------------------------------------------ IC(0)
+// Csr::incomplete_cholesky (iterative.rs:146-170), one wavefront per row, rows i = w, w+G, ...
+// Row i: for each entry p (column j, ascending) the sum over row j's entries whose column k is
+// also a computed column of row i is taken in ascending k order, exactly the reference's sum
+// over its dense work vector; l_ij = (a_ij - s) / l_jj; then the pivot a_ii(1+shift) - sum of
+// squares must exceed 1e-12 a_ii. Rows of L are published entry by entry; a failed pivot
+// raises *fail and every later row only writes zeros so the launch drains quickly.
+template <typename T, int MAXROW>
+__global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
+                                            const int* __restrict__ col,
+                                            const T* __restrict__ a, T* l, T shift,
+                                            int* fail, int* abort_flag, long long limit) {
+#pragma clang fp contract(off)
+  __shared__ int scol[MAXROW];
+  __shared__ T sw[MAXROW];
+  const int lane = threadIdx.x;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int start = off[i], dg = off[i + 1] - 1, len = dg - start;
+    const bool failed = __builtin_amdgcn_readfirstlane(
+                            __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0;
+    if (failed || len > MAXROW) {
+      if (len > MAXROW && lane == 0)
+        __hip_atomic_store(fail, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int t = lane; t <= len; t += 64) publish(l + start + t, T(0));
+      continue;
+    }
+    for (int t = lane; t < len; t += 64) scol[t] = col[start + t];
+    __syncthreads();
+    for (int q = 0; q < len; ++q) {
+      const int j = scol[q];
+      const int js = off[j], jd = off[j + 1] - 1;
+      T sacc = T(0);
+      for (int base = js; base < jd; base += 64) {
+        const int idx = base + lane;
+        T term = T(0);
+        bool hit = false;
+        if (idx < jd) {
+          const int k = col[idx];
+          int lo = 0, hi = q;  // search k among the computed columns scol[0..q)
+          while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (scol[mid] < k) lo = mid + 1;
+            else hi = mid;
+          }
+          if (lo < q && scol[lo] == k) {
+            const T ljk = await_value(l + idx, abort_flag, limit);
+            term = ljk * sw[lo];
+            hit = true;
+          }
+        }
+        unsigned long long mask = __ballot(hit);
+        while (mask) {
+          const int bl = __builtin_ctzll(mask);
+          sacc = sacc + lane_broadcast(term, bl);
+          mask &= mask - 1;
+        }
+      }
+      const T ljj = await_value(l + jd, abort_flag, limit);
+      const T lij = (a[start + q] - sacc) / ljj;
+      if (lane == 0) sw[q] = lij;
+      __syncthreads();
+    }
+    T squares = T(0);
+    for (int q = 0; q < len; ++q) squares = squares + sw[q] * sw[q];
+    const T pivot = a[dg] * (T(1) + shift) - squares;
+    const bool bad = isnan(pivot) || pivot <= T(1e-12) * a[dg];
+    if (bad && lane == 0) __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int t = lane; t < len; t += 64) publish(l + start + t, sw[t]);
+    if (lane == 0) publish(l + dg, bad ? T(0) : (T)sqrt(pivot));
+    __syncthreads();
+  }
+}
+
+}  // namespace mlk

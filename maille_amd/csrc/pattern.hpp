@@ -1,0 +1,67 @@
+// Host-side sparsity work of the solve path: the DOF graph, reverse Cuthill-McKee numbering and
+// the lower-triangle CSR pattern (bit-exact to maille-core), plus the device layouts derived
+// from them (gather lists for assembly, the sliced-ELL symmetric SpMV layout, the transposed
+// factor layout and the row blocks of the dataflow triangular solves).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ml {
+
+typedef int64_t i64;
+
+// iterative.rs:16-22: neighbours of v are targets[offsets[v]..offsets[v+1]], in first-seen order
+struct Graph {
+  std::vector<i64> offsets;
+  std::vector<int32_t> targets;
+  i64 n() const { return (i64)offsets.size() - 1; }
+  i64 degree(i64 v) const { return offsets[v + 1] - offsets[v]; }
+};
+
+// iterative.rs:38-76 (Graph::from_elements); elements as CSR lists of vertices < n
+Graph graph_from_elements(i64 n, i64 n_elements, const i64* eoff, const i64* eitems);
+
+// solve.rs:200-245 (reverse_cuthill_mckee): order[new] = old
+std::vector<i64> reverse_cuthill_mckee(const Graph& g);
+
+// iterative.rs:79-114: lower triangle in the numbering rank, ascending columns, diagonal last
+struct LowerPattern {
+  std::vector<i64> offsets;
+  std::vector<int32_t> cols;
+  i64 n() const { return (i64)offsets.size() - 1; }
+  i64 nnz() const { return (i64)cols.size(); }
+  // Csr::add's binary search (iterative.rs:121-128); -1 when (i, j) is outside the profile
+  i64 find(i64 i, i64 j) const;
+};
+LowerPattern lower_pattern(const Graph& g, const std::vector<i64>& order,
+                           const std::vector<i64>& rank);
+
+// Sliced ELL, 64 rows per slice, of the full symmetric matrix; row j holds, in this order,
+// the diagonal, the lower entries (ascending column) and the upper entries (ascending row):
+// exactly the order in which Csr::multiply (iterative.rs:131-142) accumulates y[j], so a lane
+// that sums its row sequentially reproduces the reference bit for bit.
+struct SellLayout {
+  static constexpr int C = 64;
+  i64 n = 0, slices = 0;
+  std::vector<i64> slice_offsets;  // slices + 1, in entries
+  std::vector<int32_t> cols;       // padding: the row itself
+  std::vector<int64_t> src;        // lower-CSR index of each entry, -1 for padding
+  i64 entries() const { return (i64)cols.size(); }
+};
+SellLayout sell_layout(const LowerPattern& lp);
+
+// Rows of L^T (the columns of L): for row j, the rows i > j with l_ij != 0, ascending i, and
+// the lower-CSR index of l_ij.
+struct TransposeLayout {
+  std::vector<i64> offsets;
+  std::vector<int32_t> rows;
+  std::vector<int64_t> src;
+};
+TransposeLayout transpose_layout(const LowerPattern& lp);
+
+// DAG depth of the lower-triangular solve (longest dependency chain, in rows).
+i64 dag_depth(const LowerPattern& lp);
+
+}  // namespace ml

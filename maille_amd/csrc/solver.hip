@@ -1,0 +1,1239 @@
+// Host orchestration of the device solve path and the C ABI declared in include/maille_hip.h.
+//
+//   Operator : K_ff on the device (reference lower CSR, sliced-ELL copy for the SpMV, forward
+//              and transposed layouts of the IC(0) factor) + the PCG driver (FP64 drop-in, or
+//              FP32 operator inside FP64 iterative refinement).
+//   Model    : mesh -> host plan (pattern.cpp, plan.cpp) -> element kernels and gather
+//              assembly on the device -> Operator -> reactions and stresses.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/maille_hip.h"
+#include "kernels.hip"
+#include "pattern.hpp"
+#include "plan.hpp"
+
+namespace ml {
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIP_CHECK(expr)                                                                 \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_));                \
+  } while (0)
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Device allocation owned by a handle.
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) count = 1;
+    HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const T* host, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count) HIP_CHECK(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  template <class V>
+  void upload_vec(const V& v, hipStream_t s) {
+    upload(v.data(), v.size(), s);
+  }
+};
+
+inline unsigned grid_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// Kernel-duration accounting with HIP events on the handle's stream (enabled on demand).
+enum ProfKind { P_SPMV64, P_SPMV32, P_TRI64, P_TRI32, P_IC0, P_STIFF, P_ASM, P_VEC, P_COUNT };
+
+struct Profiler {
+  bool on = false;
+  struct Rec { int kind; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  double ms[P_COUNT] = {0};
+  int64_t count[P_COUNT] = {0};
+  hipEvent_t begin(hipStream_t s) {
+    if (!on) return nullptr;
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipEventRecord(e, s));
+    return e;
+  }
+  void end(int kind, hipEvent_t a, hipStream_t s) {
+    if (!on || !a) return;
+    hipEvent_t b;
+    HIP_CHECK(hipEventCreate(&b));
+    HIP_CHECK(hipEventRecord(b, s));
+    pending.push_back({kind, a, b});
+  }
+  void resolve() {
+    for (auto& r : pending) {
+      HIP_CHECK(hipEventSynchronize(r.b));
+      float t = 0;
+      HIP_CHECK(hipEventElapsedTime(&t, r.a, r.b));
+      ms[r.kind] += t;
+      count[r.kind] += 1;
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    pending.clear();
+  }
+  void reset() {
+    resolve();
+    for (int k = 0; k < P_COUNT; ++k) { ms[k] = 0; count[k] = 0; }
+  }
+};
+
+int device_cus() {
+  int dev = 0, cus = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return cus;
+}
+
+// Waves of a persistent 64-thread kernel that are resident at once on the whole device.
+int resident_waves(const void* kernel) {
+  int per_cu = 0;
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0));
+  if (per_cu < 1) per_cu = 1;
+  return per_cu * device_cus();
+}
+
+// 2 s of the 100 MHz real-time counter: the bound of every dataflow wait.
+constexpr long long WAIT_LIMIT = 200000000ll;
+
+}  // namespace
+
+// ====================================================================== Operator
+struct Operator {
+  int n = 0;
+  int64_t nnz = 0;
+  int precision = ML_FP64;
+  hipStream_t stream = nullptr;
+  Profiler prof;
+  int64_t depth = 0;
+  int max_row = 0;
+  int shift_index = -1;
+  bool factored = false;
+
+  // reference lower CSR
+  std::vector<int64_t> h_off;
+  std::vector<int32_t> h_col;
+  DBuf<int> d_off, d_col;
+  DBuf<double> d_a;      // values
+  DBuf<float> d_a32;
+  // sliced ELL of the full symmetric matrix
+  int slices = 0;
+  DBuf<int> d_soff, d_scol, d_slen, d_ssrc;
+  DBuf<double> d_sval;
+  DBuf<float> d_sval32;
+  // triangular layouts: forward (strictly lower rows) and backward (rows of L^T, reversed)
+  DBuf<int> d_foff, d_fcol, d_fsrc, d_fdsrc, d_boff, d_bcol, d_bsrc, d_bdsrc;
+  DBuf<double> d_l, d_fval, d_bval, d_fdiag, d_bdiag;
+  DBuf<float> d_l32, d_fval32, d_bval32, d_fdiag32, d_bdiag32;
+  int64_t nf = 0;  // strictly lower entries
+  // workspace
+  DBuf<double> w_x, w_r, w_p, w_ap, w_y, w_z, w_b, w_d, w_ax, w_partial;
+  DBuf<float> w_r32, w_p32, w_ap32, w_y32, w_z32;
+  DBuf<mlk::PcgScalars> d_S, d_Sin;
+  DBuf<int> d_flags;  // [0] abort, [1] IC(0) failure
+  int tri_grid64 = 0, tri_grid32 = 0, ic_grid = 0;
+
+  ~Operator() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void init_stream() {
+    if (!stream) HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  }
+
+  // Upload the pattern and every layout derived from it.
+  void set_pattern(const LowerPattern& lp) {
+    init_stream();
+    n = (int)lp.n();
+    nnz = lp.nnz();
+    h_off = lp.offsets;
+    h_col = lp.cols;
+    depth = dag_depth(lp);
+    max_row = 0;
+    for (int i = 0; i < n; ++i) max_row = std::max<int>(max_row, (int)(lp.offsets[i + 1] - lp.offsets[i]));
+    std::vector<int> off32(n + 1);
+    for (int i = 0; i <= n; ++i) off32[i] = (int)lp.offsets[i];
+    d_off.upload_vec(off32, stream);
+    d_col.upload_vec(lp.cols, stream);
+    // sliced ELL
+    SellLayout s = sell_layout(lp);
+    slices = (int)s.slices;
+    std::vector<int> soff(s.slice_offsets.begin(), s.slice_offsets.end());
+    std::vector<int> slen(n), ssrc(s.src.begin(), s.src.end());
+    {
+      // row lengths: lower row + upper entries
+      std::vector<int> up(n, 0);
+      for (int i = 0; i < n; ++i)
+        for (int64_t p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) up[lp.cols[p]] += 1;
+      for (int j = 0; j < n; ++j) slen[j] = (int)(lp.offsets[j + 1] - lp.offsets[j]) + up[j];
+    }
+    d_soff.upload_vec(soff, stream);
+    d_scol.upload_vec(s.cols, stream);
+    d_slen.upload_vec(slen, stream);
+    d_ssrc.upload_vec(ssrc, stream);
+    // forward layout: strictly lower part of each row, diagonal apart
+    nf = nnz - n;
+    std::vector<int> foff(n + 1), fcol(nf), fsrc(nf), fdsrc(n);
+    for (int i = 0; i <= n; ++i) foff[i] = (int)(lp.offsets[i] - i);
+    for (int i = 0; i < n; ++i) {
+      int64_t o = foff[i];
+      for (int64_t p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p, ++o) {
+        fcol[o] = lp.cols[p];
+        fsrc[o] = (int)p;
+      }
+      fdsrc[i] = (int)(lp.offsets[i + 1] - 1);
+    }
+    d_foff.upload_vec(foff, stream);
+    d_fcol.upload_vec(fcol, stream);
+    d_fsrc.upload_vec(fsrc, stream);
+    d_fdsrc.upload_vec(fdsrc, stream);
+    // backward layout: logical row l = n-1-j holds the rows i > j of column j of L, in
+    // decreasing i (ascending logical column n-1-i)
+    TransposeLayout t = transpose_layout(lp);
+    std::vector<int> boff(n + 1, 0), bcol(nf), bsrc(nf), bdsrc(n);
+    for (int l = 0; l < n; ++l) {
+      int j = n - 1 - l;
+      boff[l + 1] = boff[l] + (int)(t.offsets[j + 1] - t.offsets[j]);
+    }
+    for (int l = 0; l < n; ++l) {
+      int j = n - 1 - l;
+      int64_t o = boff[l];
+      for (int64_t q = t.offsets[j + 1] - 1; q >= t.offsets[j]; --q, ++o) {
+        bcol[o] = n - 1 - t.rows[q];
+        bsrc[o] = (int)t.src[q];
+      }
+      bdsrc[l] = (int)(lp.offsets[j + 1] - 1);
+    }
+    d_boff.upload_vec(boff, stream);
+    d_bcol.upload_vec(bcol, stream);
+    d_bsrc.upload_vec(bsrc, stream);
+    d_bdsrc.upload_vec(bdsrc, stream);
+    // values and workspace
+    d_a.alloc(nnz);
+    d_l.alloc(nnz);
+    d_sval.alloc(d_scol.n);
+    d_fval.alloc(nf);
+    d_bval.alloc(nf);
+    d_fdiag.alloc(n);
+    d_bdiag.alloc(n);
+    for (auto* b : {&w_x, &w_r, &w_p, &w_ap, &w_y, &w_z, &w_b, &w_d, &w_ax}) b->alloc(n);
+    tri_grid64 = std::min<int>((n + 63) / 64,
+                               resident_waves((const void*)mlk::k_trisolve<double, false, false>));
+    tri_grid32 = std::min<int>((n + 63) / 64,
+                               resident_waves((const void*)mlk::k_trisolve<float, false, false>));
+    ic_grid = std::min<int>(n, resident_waves((const void*)mlk::k_ic0<double, 512>));
+    int64_t parts = std::max<int64_t>({grid_for(n), (int64_t)tri_grid64, (int64_t)tri_grid32, 1});
+    w_partial.alloc(parts);
+    d_S.alloc(1);
+    d_Sin.alloc(1);
+    d_flags.alloc(2);
+    mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 2, 0);
+    HIP_CHECK(hipGetLastError());
+    if (precision == ML_MIXED) alloc_mixed();
+    factored = false;
+    // the host layouts above are temporaries: wait for their uploads before they go away
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+
+  void alloc_mixed() {
+    d_a32.alloc(nnz);
+    d_l32.alloc(nnz);
+    d_sval32.alloc(d_scol.n);
+    d_fval32.alloc(nf);
+    d_bval32.alloc(nf);
+    d_fdiag32.alloc(n);
+    d_bdiag32.alloc(n);
+    for (auto* b : {&w_r32, &w_p32, &w_ap32, &w_y32, &w_z32}) b->alloc(n);
+  }
+
+  // After d_a holds the lower values: refresh the SpMV copies.
+  void values_changed() {
+    mlk::k_gather<double, double><<<grid_for(d_scol.n), 256, 0, stream>>>(
+        (int64_t)d_scol.n, d_ssrc.p, d_a.p, d_sval.p);
+    HIP_CHECK(hipGetLastError());
+    if (precision == ML_MIXED) {
+      mlk::k_convert<double, float><<<grid_for(nnz), 256, 0, stream>>>(nnz, d_a.p, d_a32.p);
+      mlk::k_gather<double, float><<<grid_for(d_scol.n), 256, 0, stream>>>(
+          (int64_t)d_scol.n, d_ssrc.p, d_a.p, d_sval32.p);
+      HIP_CHECK(hipGetLastError());
+    }
+    factored = false;
+  }
+
+  int64_t read_flag(int k) {
+    int v = 0;
+    HIP_CHECK(hipMemcpyAsync(&v, d_flags.p + k, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return v;
+  }
+
+  template <typename T>
+  int run_ic0(const T* a, T* l, T shift) {
+    mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 2, 0);
+    mlk::k_fill_bits<T><<<grid_for(nnz), 256, 0, stream>>>(l, nnz, mlk::Bits<T>::S);
+    hipEvent_t ev = prof.begin(stream);
+    if (max_row <= 64)
+      mlk::k_ic0<T, 64><<<ic_grid, 64, 0, stream>>>(n, d_off.p, d_col.p, a, l, shift,
+                                                    d_flags.p + 1, d_flags.p, WAIT_LIMIT);
+    else
+      mlk::k_ic0<T, 512><<<ic_grid, 64, 0, stream>>>(n, d_off.p, d_col.p, a, l, shift,
+                                                     d_flags.p + 1, d_flags.p, WAIT_LIMIT);
+    HIP_CHECK(hipGetLastError());
+    prof.end(P_IC0, ev, stream);
+    int flags[2];
+    HIP_CHECK(hipMemcpyAsync(flags, d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if (flags[0]) return ML_DEVICE_TIMEOUT;
+    if (flags[1] == 2) throw ArgError("matrix row longer than 512 entries");
+    return flags[1] ? ML_SINGULAR : ML_OK;
+  }
+
+  // Pcg::new (iterative.rs:201-207): the first shift of the ladder whose IC(0) succeeds.
+  int factor() {
+    static const double SHIFTS[6] = {0.0, 1e-3, 1e-2, 1e-1, 1.0, 10.0};
+    factored = false;
+    shift_index = -1;
+    int st = ML_SINGULAR;
+    for (int s = 0; s < 6; ++s) {
+      st = run_ic0<double>(d_a.p, d_l.p, SHIFTS[s]);
+      if (st == ML_DEVICE_TIMEOUT) return st;
+      if (st == ML_OK) { shift_index = s; break; }
+    }
+    if (st != ML_OK) return ML_SINGULAR;
+    gather_factor<double>(d_l.p, d_fval.p, d_bval.p, d_fdiag.p, d_bdiag.p);
+    if (precision == ML_MIXED) {
+      int st32 = ML_SINGULAR;
+      for (int s = 0; s < 6; ++s) {
+        st32 = run_ic0<float>(d_a32.p, d_l32.p, (float)SHIFTS[s]);
+        if (st32 == ML_DEVICE_TIMEOUT) return st32;
+        if (st32 == ML_OK) break;
+      }
+      if (st32 != ML_OK) return ML_SINGULAR;
+      gather_factor<float>(d_l32.p, d_fval32.p, d_bval32.p, d_fdiag32.p, d_bdiag32.p);
+    }
+    factored = true;
+    return ML_OK;
+  }
+
+  template <typename T>
+  void gather_factor(const T* l, T* fval, T* bval, T* fdiag, T* bdiag) {
+    mlk::k_gather<T, T><<<grid_for(nf), 256, 0, stream>>>(nf, d_fsrc.p, l, fval);
+    mlk::k_gather<T, T><<<grid_for(nf), 256, 0, stream>>>(nf, d_bsrc.p, l, bval);
+    mlk::k_gather<T, T><<<grid_for(n), 256, 0, stream>>>(n, d_fdsrc.p, l, fdiag);
+    mlk::k_gather<T, T><<<grid_for(n), 256, 0, stream>>>(n, d_bdsrc.p, l, bdiag);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // ---------------------------------------------------------------- kernels
+  void spmv64(const double* x, double* y, double* partial, const mlk::PcgScalars* S) {
+    hipEvent_t ev = prof.begin(stream);
+    mlk::k_spmv<double><<<grid_for(n), 256, 0, stream>>>(n, d_soff.p, d_scol.p, d_slen.p,
+                                                         d_sval.p, x, y, partial, S);
+    HIP_CHECK(hipGetLastError());
+    prof.end(P_SPMV64, ev, stream);
+  }
+  void spmv32(const float* x, float* y, double* partial, const mlk::PcgScalars* S) {
+    hipEvent_t ev = prof.begin(stream);
+    mlk::k_spmv<float><<<grid_for(n), 256, 0, stream>>>(n, d_soff.p, d_scol.p, d_slen.p,
+                                                        d_sval32.p, x, y, partial, S);
+    HIP_CHECK(hipGetLastError());
+    prof.end(P_SPMV32, ev, stream);
+  }
+  // z = (L L^T)^-1 r through y; partial gets r.z per wave when dotvec is given
+  template <typename T>
+  void precondition(const T* r, T* y, T* z, const double* dotvec, double* partial,
+                    const mlk::PcgScalars* S) {
+    const bool f32 = std::is_same<T, float>::value;
+    const T* fval = f32 ? (const T*)d_fval32.p : (const T*)d_fval.p;
+    const T* bval = f32 ? (const T*)d_bval32.p : (const T*)d_bval.p;
+    const T* fdiag = f32 ? (const T*)d_fdiag32.p : (const T*)d_fdiag.p;
+    const T* bdiag = f32 ? (const T*)d_bdiag32.p : (const T*)d_bdiag.p;
+    int grid = f32 ? tri_grid32 : tri_grid64;
+    int nblocks = (n + 63) / 64;
+    hipEvent_t ev = prof.begin(stream);
+    mlk::k_trisolve<T, false, false><<<grid, 64, 0, stream>>>(
+        n, nblocks, d_foff.p, d_fcol.p, fval, fdiag, r, y, nullptr, nullptr, S, d_flags.p,
+        WAIT_LIMIT);
+    HIP_CHECK(hipGetLastError());
+    mlk::k_trisolve<T, true, true><<<grid, 64, 0, stream>>>(
+        n, nblocks, d_boff.p, d_bcol.p, bval, bdiag, y, z, dotvec, partial, S, d_flags.p,
+        WAIT_LIMIT);
+    HIP_CHECK(hipGetLastError());
+    prof.end(f32 ? P_TRI32 : P_TRI64, ev, stream);
+  }
+
+  template <typename T>
+  void fill_sentinel(T* p) {
+    mlk::k_fill_bits<T><<<grid_for(n), 256, 0, stream>>>(p, n, mlk::Bits<T>::S);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  void init_scalars(mlk::PcgScalars* S, int64_t maxit) {
+    mlk::PcgScalars h;
+    std::memset(&h, 0, sizeof(h));
+    h.maxit = maxit;
+    // blocking copy from the stack: the stream must not hold kernels still reading S
+    HIP_CHECK(hipStreamSynchronize(stream));
+    HIP_CHECK(hipMemcpy(S, &h, sizeof(h), hipMemcpyHostToDevice));
+  }
+
+  mlk::PcgScalars read_scalars(const mlk::PcgScalars* S) {
+    mlk::PcgScalars h;
+    HIP_CHECK(hipMemcpyAsync(&h, S, sizeof(h), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return h;
+  }
+
+  // Preconditioned CG (iterative.rs:210-248) on the device. T: arithmetic of the operator and
+  // of p, Ap, z (double for the drop-in, float inside refinement). x and r are FP64. b, x and
+  // r are device vectors in solver numbering; returns status and fills S.
+  template <typename T>
+  int pcg(const double* b, double* x, double* r, double tol, int64_t maxit,
+          mlk::PcgScalars* S, mlk::PcgScalars* hS) {
+    const bool f32 = std::is_same<T, float>::value;
+    T* p = f32 ? (T*)w_p32.p : (T*)w_p.p;
+    T* ap = f32 ? (T*)w_ap32.p : (T*)w_ap.p;
+    T* y = f32 ? (T*)w_y32.p : (T*)w_y.p;
+    T* z = f32 ? (T*)w_z32.p : (T*)w_z.p;
+    T* rlow = f32 ? (T*)w_r32.p : nullptr;
+    const T* rin = f32 ? (const T*)w_r32.p : (const T*)r;
+    const int vb = grid_for(n);
+    double* part = w_partial.p;
+    init_scalars(S, maxit);
+    mlk::k_fill_f64<<<vb, 256, 0, stream>>>(x, n, 0.0);
+    HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    mlk::k_dot<double, double><<<vb, 256, 0, stream>>>(n, b, b, part);
+    mlk::k_init_norm<<<1, 256, 0, stream>>>(part, vb, S, tol);
+    HIP_CHECK(hipGetLastError());
+    if (f32) mlk::k_convert<double, float><<<vb, 256, 0, stream>>>(n, r, (float*)rlow);
+    fill_sentinel<T>(y);
+    fill_sentinel<T>(z);
+    precondition<T>(rin, y, z, r, part, S);
+    mlk::k_init_rz<<<1, 256, 0, stream>>>(part, f32 ? tri_grid32 : tri_grid64, S);
+    HIP_CHECK(hipMemcpyAsync(p, z, sizeof(T) * n, hipMemcpyDeviceToDevice, stream));
+    HIP_CHECK(hipGetLastError());
+    *hS = read_scalars(S);
+    if (read_flag(0)) return ML_DEVICE_TIMEOUT;
+    if (hS->done) return ML_OK;  // b == 0: x = 0 exactly
+    int64_t launched = 0, batch = 4;
+    const int tri_parts = f32 ? tri_grid32 : tri_grid64;
+    for (;;) {
+      for (int64_t k = 0; k < batch; ++k) {
+        if (f32) spmv32((const float*)p, (float*)ap, part, S);
+        else spmv64((const double*)p, (double*)ap, part, S);
+        mlk::k_alpha<<<1, 256, 0, stream>>>(part, vb, S);
+        hipEvent_t ev = prof.begin(stream);
+        mlk::k_update_xr<T><<<vb, 256, 0, stream>>>(n, S, x, r, p, ap, rlow, y, z, part);
+        prof.end(P_VEC, ev, stream);
+        mlk::k_converged<<<1, 256, 0, stream>>>(part, vb, S);
+        precondition<T>(rin, y, z, r, part, S);
+        mlk::k_beta<<<1, 256, 0, stream>>>(part, tri_parts, S);
+        ev = prof.begin(stream);
+        mlk::k_update_p<T><<<vb, 256, 0, stream>>>(n, S, p, z);
+        prof.end(P_VEC, ev, stream);
+        HIP_CHECK(hipGetLastError());
+      }
+      launched += batch;
+      *hS = read_scalars(S);
+      if (read_flag(0)) return ML_DEVICE_TIMEOUT;
+      if (hS->done) break;
+      if (launched > maxit + 8) break;  // cannot happen: k_converged stops at maxit
+      batch = std::min<int64_t>(32, batch * 2);
+    }
+    if (hS->status == 1) return ML_SINGULAR;
+    if (hS->status == 2) return ML_NOT_CONVERGED;
+    return ML_OK;
+  }
+
+  // |b - A x| / |b| in FP64 (uses w_ax, w_d as scratch)
+  double true_residual(const double* b, const double* x, double bnorm) {
+    const int vb = grid_for(n);
+    spmv64(x, w_ax.p, nullptr, nullptr);
+    mlk::k_residual<<<vb, 256, 0, stream>>>(n, b, w_ax.p, w_d.p, w_partial.p);
+    mlk::k_init_norm<<<1, 256, 0, stream>>>(w_partial.p, vb, d_Sin.p, 1.0);
+    HIP_CHECK(hipGetLastError());
+    mlk::PcgScalars h = read_scalars(d_Sin.p);
+    return bnorm > 0 ? h.bnorm / bnorm : 0.0;
+  }
+
+  // Solve A x = b (device vectors, solver numbering).
+  int solve(const double* b, double* x, double tol, int64_t maxit, ml_solve_info* info) {
+    if (!factored) {
+      int st = factor();
+      if (st != ML_OK) return st;
+    }
+    if (tol <= 0) tol = ML_TOLERANCE;
+    if (maxit <= 0) maxit = ML_MAX_ITERATIONS;
+    mlk::PcgScalars hS;
+    int st;
+    int64_t iterations = 0;
+    int refinements = 0;
+    double residual = 0, bnorm = 0;
+    if (precision == ML_FP64) {
+      st = pcg<double>(b, x, w_r.p, tol, maxit, d_S.p, &hS);
+      iterations = hS.it;
+      bnorm = hS.bnorm;
+      residual = bnorm > 0 ? hS.rnorm / bnorm : 0.0;
+    } else {
+      st = refine(b, x, tol, maxit, &iterations, &refinements, &bnorm, &residual);
+    }
+    if (info) {
+      info->iterations = iterations;
+      info->residual = residual;
+      info->shift_index = shift_index;
+      info->refinements = refinements;
+      info->true_residual = (st == ML_OK && bnorm > 0) ? true_residual(b, x, bnorm) : residual;
+    }
+    return st;
+  }
+
+  // FP64 iterative refinement around the FP32 PCG: x += PCG32(b - A x) until |b - A x| <=
+  // tol |b|. Inner solves stop at a relative residual of 1e-6 (or what is left to gain).
+  int refine(const double* b, double* x, double tol, int64_t maxit, int64_t* iterations,
+             int* sweeps, double* bnorm_out, double* residual) {
+    const int vb = grid_for(n);
+    mlk::k_fill_f64<<<vb, 256, 0, stream>>>(x, n, 0.0);
+    HIP_CHECK(hipMemcpyAsync(w_b.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    mlk::k_dot<double, double><<<vb, 256, 0, stream>>>(n, b, b, w_partial.p);
+    mlk::k_init_norm<<<1, 256, 0, stream>>>(w_partial.p, vb, d_S.p, tol);
+    HIP_CHECK(hipGetLastError());
+    mlk::PcgScalars out = read_scalars(d_S.p);
+    const double bnorm = out.bnorm;
+    *bnorm_out = bnorm;
+    *iterations = 0;
+    *sweeps = 0;
+    *residual = 0;
+    if (bnorm == 0.0) return ML_OK;
+    double rnorm = bnorm;
+    // residual r lives in w_b (starts as b); the correction in w_d's partner w_x? use w_ax
+    const double inner_tol = 1e-6;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+      if (rnorm <= tol * bnorm) break;
+      double itol = std::max(inner_tol, 0.5 * tol * bnorm / rnorm);
+      mlk::PcgScalars hin;
+      int st = pcg<float>(w_b.p, w_d.p, w_r.p, itol, maxit - *iterations, d_Sin.p, &hin);
+      *iterations += hin.it;
+      *sweeps += 1;
+      if (st == ML_DEVICE_TIMEOUT || st == ML_SINGULAR) return st;
+      mlk::k_axpy_add<<<vb, 256, 0, stream>>>(n, x, w_d.p);
+      spmv64(x, w_ax.p, nullptr, nullptr);
+      mlk::k_residual<<<vb, 256, 0, stream>>>(n, b, w_ax.p, w_b.p, w_partial.p);
+      mlk::k_init_norm<<<1, 256, 0, stream>>>(w_partial.p, vb, d_S.p, tol);
+      HIP_CHECK(hipGetLastError());
+      out = read_scalars(d_S.p);
+      rnorm = out.bnorm;
+      *residual = rnorm / bnorm;
+      if (*iterations >= maxit && rnorm > tol * bnorm) return ML_NOT_CONVERGED;
+    }
+    return rnorm <= tol * bnorm ? ML_OK : ML_NOT_CONVERGED;
+  }
+};
+
+// ====================================================================== Model
+struct Model {
+  MeshView mesh;
+  std::vector<double> nodes, young, nu, imposed;
+  std::vector<int64_t> conn;
+  std::vector<uint8_t> fixed;
+  Plan plan;
+  Operator op;
+  int ngp = 1;
+  DBuf<double> d_xyz, d_young, d_nu, d_ke, d_imposed, d_coupling, d_f, d_u, d_R, d_x, d_rhs;
+  DBuf<int> d_conn, d_rank, d_cpl_dof, d_rea_dof;
+  DBuf<int64_t> d_asm_off, d_cpl_off, d_rea_off, d_rea_item;
+  DBuf<uint32_t> d_asm_slot, d_cpl_slot;
+  bool assembled = false;
+  bool have_u = false;
+
+  hipStream_t s() { return op.stream; }
+
+  int create(int64_t n_nodes, const double* xyz, int64_t ne, int nen_, const int64_t* cn,
+             const double* E, const double* v, const uint8_t* fx, const double* imp,
+             int precision, ml_solve_info* info) {
+    if (nen_ != 4 && nen_ != 10) throw ArgError("nen must be 4 (Tet4) or 10 (Tet10)");
+    nodes.assign(xyz, xyz + 3 * n_nodes);
+    conn.assign(cn, cn + ne * nen_);
+    young.assign(E, E + ne);
+    nu.assign(v, v + ne);
+    fixed.assign(fx, fx + 3 * n_nodes);
+    imposed.assign(3 * n_nodes, 0.0);
+    if (imp) imposed.assign(imp, imp + 3 * n_nodes);
+    mesh = MeshView{n_nodes, ne, nen_, nodes.data(), conn.data(), young.data(), nu.data()};
+    ngp = nen_ == 4 ? 1 : 4;
+    i64 where = -1;
+    int merr = check_mesh(mesh, &where);
+    if (info) { info->mesh_error = merr; info->bad_element = where; }
+    if (merr) return ML_INVALID_MESH;
+    std::string err;
+    int st = build_plan(mesh, fixed.data(), imposed.data(), plan, &err);
+    if (st) throw ArgError(err);
+    if (info) info->seconds[5] = plan.seconds;
+    op.precision = precision;
+    op.set_pattern(plan.lower);
+    upload();
+    assemble();
+    return ML_OK;
+  }
+
+  void upload() {
+    hipStream_t st = s();
+    d_xyz.upload_vec(nodes, st);
+    std::vector<int> c32(conn.begin(), conn.end());
+    d_conn.upload_vec(c32, st);
+    d_young.upload_vec(young, st);
+    d_nu.upload_vec(nu, st);
+    d_imposed.upload_vec(imposed, st);
+    d_rank.upload_vec(plan.rank_of_dof, st);
+    d_asm_off.upload_vec(plan.asm_off, st);
+    d_asm_slot.upload_vec(plan.asm_slot, st);
+    d_cpl_off.upload_vec(plan.cpl_off, st);
+    d_cpl_slot.upload_vec(plan.cpl_slot, st);
+    d_cpl_dof.upload_vec(plan.cpl_dof, st);
+    d_rea_dof.upload_vec(plan.rea_dof, st);
+    d_rea_off.upload_vec(plan.rea_off, st);
+    d_rea_item.upload_vec(plan.rea_item, st);
+    d_ke.alloc((size_t)plan.m * plan.m * plan.n_elements);
+    d_coupling.alloc(plan.nfree);
+    d_x.alloc(plan.nfree);
+    d_rhs.alloc(plan.nfree);
+    d_f.alloc(plan.ndof);
+    d_u.alloc(plan.ndof);
+    d_R.alloc(plan.ndof);
+    HIP_CHECK(hipStreamSynchronize(st));
+    // the host copies of the big lists are no longer needed
+    std::vector<uint32_t>().swap(plan.asm_slot);
+  }
+
+  void stiffness() {
+    const int64_t ne = plan.n_elements;
+    hipEvent_t ev = op.prof.begin(s());
+    if (plan.nen == 4)
+      mlk::k_stiffness<4><<<grid_for(12 * ne), 256, 0, s()>>>(ne, d_xyz.p, d_conn.p, d_young.p,
+                                                             d_nu.p, d_ke.p);
+    else
+      mlk::k_stiffness<10><<<grid_for(30 * ne), 256, 0, s()>>>(ne, d_xyz.p, d_conn.p, d_young.p,
+                                                              d_nu.p, d_ke.p);
+    HIP_CHECK(hipGetLastError());
+    op.prof.end(P_STIFF, ev, s());
+  }
+
+  // model.rs:1480-1492 on the device: Ke of every element, gathered into K_ff and K_fc u_c.
+  void assemble() {
+    stiffness();
+    hipEvent_t ev = op.prof.begin(s());
+    mlk::k_assemble<<<grid_for(op.nnz), 256, 0, s()>>>(op.nnz, d_asm_off.p, d_asm_slot.p, d_ke.p,
+                                                       op.d_a.p);
+    HIP_CHECK(hipGetLastError());
+    op.prof.end(P_ASM, ev, s());
+    if (!plan.cpl_slot.empty()) {
+      mlk::k_coupling<<<grid_for(plan.nfree), 256, 0, s()>>>(plan.nfree, d_cpl_off.p,
+                                                             d_cpl_slot.p, d_cpl_dof.p, d_ke.p,
+                                                             d_imposed.p, d_coupling.p);
+      HIP_CHECK(hipGetLastError());
+    }
+    op.values_changed();
+    assembled = true;
+  }
+
+  int solve_device(const double* f, double* u, double* R, ml_solve_info* info) {
+    double t0 = now_s();
+    if (!op.factored) {
+      int st = op.factor();
+      if (info) info->shift_index = op.shift_index;
+      if (st != ML_OK) return st;
+    }
+    double t1 = now_s();
+    mlk::k_rhs<<<grid_for(plan.ndof), 256, 0, s()>>>(
+        plan.ndof, d_rank.p, f, plan.cpl_slot.empty() ? nullptr : d_coupling.p, d_rhs.p);
+    HIP_CHECK(hipGetLastError());
+    int st = op.solve(d_rhs.p, d_x.p, 0, 0, info);
+    double t2 = now_s();
+    if (st != ML_OK && st != ML_NOT_CONVERGED) return st;
+    mlk::k_scatter_u<<<grid_for(plan.ndof), 256, 0, s()>>>(plan.ndof, d_rank.p, d_x.p,
+                                                           d_imposed.p, u);
+    HIP_CHECK(hipGetLastError());
+    if (R) reactions(u, f, R);
+    HIP_CHECK(hipStreamSynchronize(s()));
+    have_u = true;
+    if (info) {
+      info->seconds[1] = t1 - t0;
+      info->seconds[2] = t2 - t1;
+      info->seconds[3] = now_s() - t2;
+    }
+    return st;
+  }
+
+  void reactions(const double* u, const double* f, double* R) {
+    mlk::k_fill_f64<<<grid_for(plan.ndof), 256, 0, s()>>>(R, plan.ndof, 0.0);
+    int64_t nfixed = (int64_t)plan.rea_dof.size();
+    if (nfixed) {
+      if (plan.nen == 4)
+        mlk::k_reactions<4><<<grid_for(nfixed), 256, 0, s()>>>(
+            nfixed, d_rea_dof.p, d_rea_off.p, d_rea_item.p, d_ke.p, d_conn.p, plan.n_elements, u,
+            f, R);
+      else
+        mlk::k_reactions<10><<<grid_for(nfixed), 256, 0, s()>>>(
+            nfixed, d_rea_dof.p, d_rea_off.p, d_rea_item.p, d_ke.p, d_conn.p, plan.n_elements, u,
+            f, R);
+    }
+    HIP_CHECK(hipGetLastError());
+  }
+
+  void stresses(const double* u_dev, double* out_dev) {
+    const int64_t ne = plan.n_elements;
+    if (plan.nen == 4)
+      mlk::k_stresses<4><<<grid_for(ne), 256, 0, s()>>>(ne, d_xyz.p, d_conn.p, d_young.p, d_nu.p,
+                                                       u_dev, out_dev);
+    else
+      mlk::k_stresses<10><<<grid_for(4 * ne), 256, 0, s()>>>(ne, d_xyz.p, d_conn.p, d_young.p,
+                                                            d_nu.p, u_dev, out_dev);
+    HIP_CHECK(hipGetLastError());
+  }
+};
+
+}  // namespace ml
+
+// face_rule + face_pressure + FacePressure::add_load (assembly.rs:527-593, 638-674) for one
+// tetrahedron face on the host: f_a = -p sum_g w_g 2A N_a n, n outward (centroid test),
+// TRIANGLE_3 on a Tet4 face, TRIANGLE_6 on a Tet10 face, added to f in face-node order.
+void face_pressure_add(int nen, const int64_t* enodes, const double* xyz, int face, double value,
+                       double* f) {
+  static const int FACES[4][3] = {{0, 1, 2}, {0, 3, 1}, {1, 3, 2}, {2, 3, 0}};
+  static const int EDGES[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+  auto mid = [](int a, int b) {
+    for (int k = 0; k < 6; ++k)
+      if ((EDGES[k][0] == a && EDGES[k][1] == b) || (EDGES[k][0] == b && EDGES[k][1] == a))
+        return 4 + k;
+    return 4;
+  };
+  int pos[6];
+  const int* fc = FACES[face];
+  pos[0] = fc[0]; pos[1] = fc[1]; pos[2] = fc[2];
+  int nn = 3;
+  if (nen == 10) {
+    pos[3] = mid(fc[0], fc[1]); pos[4] = mid(fc[1], fc[2]); pos[5] = mid(fc[2], fc[0]);
+    nn = 6;
+  }
+  auto X = [&](int local, int k) { return xyz[3 * enodes[local] + k]; };
+  double u[3], v[3];
+  for (int k = 0; k < 3; ++k) {
+    u[k] = X(pos[1], k) - X(pos[0], k);
+    v[k] = X(pos[2], k) - X(pos[0], k);
+  }
+  double cross[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+  double sq = 0.0;
+  for (int k = 0; k < 3; ++k) sq += cross[k] * cross[k];
+  const double twice_area = std::sqrt(sq);
+  double outward[3];
+  for (int k = 0; k < 3; ++k)
+    outward[k] = (X(pos[0], k) + X(pos[1], k) + X(pos[2], k)) / 3.0 -
+                 (X(0, k) + X(1, k) + X(2, k) + X(3, k)) / 4.0;
+  double dot = 0.0;
+  for (int k = 0; k < 3; ++k) dot += cross[k] * outward[k];
+  const double sign = dot < 0.0 ? -1.0 : 1.0;
+  double normal[3];
+  for (int k = 0; k < 3; ++k) normal[k] = sign * cross[k] / twice_area;
+  // quadrature.rs:19-54
+  const double A6 = 0.445948490915965, WA = 0.223381589678011 / 2.0;
+  const double B6 = 0.091576213509771, WB = 0.109951743655322 / 2.0;
+  const double tri3[3][3] = {{1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0},
+                             {2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0},
+                             {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0}};
+  const double tri6[6][3] = {{A6, A6, WA}, {1.0 - 2.0 * A6, A6, WA}, {A6, 1.0 - 2.0 * A6, WA},
+                             {B6, B6, WB}, {1.0 - 2.0 * B6, B6, WB}, {B6, 1.0 - 2.0 * B6, WB}};
+  double forces[18] = {0};
+  const int ng = nn == 3 ? 3 : 6;
+  for (int g = 0; g < ng; ++g) {
+    const double* gp = nn == 3 ? tri3[g] : tri6[g];
+    const double w = gp[2] * twice_area;
+    const double xi = gp[0], eta = gp[1];
+    double nv[6];
+    if (nn == 3) {
+      nv[0] = 1.0 - xi - eta; nv[1] = xi; nv[2] = eta;
+    } else {
+      const double l1 = 1.0 - xi - eta, l2 = xi, l3 = eta;
+      nv[0] = l1 * (2.0 * l1 - 1.0);
+      nv[1] = l2 * (2.0 * l2 - 1.0);
+      nv[2] = l3 * (2.0 * l3 - 1.0);
+      nv[3] = 4.0 * l1 * l2;
+      nv[4] = 4.0 * l2 * l3;
+      nv[5] = 4.0 * l3 * l1;
+    }
+    for (int a = 0; a < nn; ++a)
+      for (int c = 0; c < 3; ++c) forces[3 * a + c] -= value * w * nv[a] * normal[c];
+  }
+  for (int a = 0; a < nn; ++a)
+    for (int c = 0; c < 3; ++c) f[3 * enodes[pos[a]] + c] += forces[3 * a + c];
+}
+
+// ====================================================================== C ABI
+using namespace ml;
+
+struct ml_pcg {
+  Operator op;
+};
+struct ml_model {
+  Model m;
+};
+struct ml_pattern {
+  Graph graph;
+  std::vector<int64_t> order;
+  LowerPattern lower;
+};
+
+namespace {
+
+template <class F>
+ml_status guarded(F body) {
+  try {
+    return (ml_status)body();
+  } catch (const ArgError& e) {
+    g_last_error = e.what();
+    return ML_INVALID_ARGUMENT;
+  } catch (const HipError& e) {
+    g_last_error = e.what();
+    return ML_HIP_ERROR;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return ML_HIP_ERROR;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return ML_HIP_ERROR;
+  }
+}
+
+// Validates a lower-triangle CSR in the reference layout (ascending columns, diagonal last).
+LowerPattern pattern_from_csr(int64_t n, const int64_t* off, const int64_t* cols) {
+  if (n <= 0 || n >= INT32_MAX || !off || !cols) throw ArgError("bad CSR arguments");
+  LowerPattern lp;
+  lp.offsets.assign(off, off + n + 1);
+  if (lp.offsets[0] != 0 || lp.offsets[n] >= INT32_MAX) throw ArgError("bad CSR offsets");
+  lp.cols.resize(lp.offsets[n]);
+  for (int64_t i = 0; i < n; ++i) {
+    if (lp.offsets[i + 1] <= lp.offsets[i]) throw ArgError("empty CSR row");
+    for (int64_t p = lp.offsets[i]; p < lp.offsets[i + 1]; ++p) {
+      int64_t c = cols[p];
+      bool last = p == lp.offsets[i + 1] - 1;
+      if (last ? c != i : (c < 0 || c >= i || (p > lp.offsets[i] && c <= cols[p - 1])))
+        throw ArgError("CSR is not lower-triangular with ascending columns and diagonal last");
+      lp.cols[p] = (int32_t)c;
+    }
+  }
+  return lp;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ml_last_error(void) { return g_last_error.c_str(); }
+
+int ml_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+ml_status ml_solid_d(double young, double nu, double* d) {
+  if (!d) return ML_INVALID_ARGUMENT;
+  // element.rs:193-206, host
+  double c = young / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  double g = young / (2.0 * (1.0 + nu));
+  for (int i = 0; i < 36; ++i) d[i] = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) d[6 * i + k] = (i == k) ? c * (1.0 - nu) : c * nu;
+  for (int i = 3; i < 6; ++i) d[6 * i + i] = g;
+  return ML_OK;
+}
+
+ml_status ml_stiffness_batch(int nen, int64_t ne, const double* coords, const double* young,
+                             const double* nu, double* ke) {
+  return guarded([&]() -> int {
+    if ((nen != 4 && nen != 10) || ne <= 0 || !coords || !young || !nu || !ke)
+      throw ArgError("bad stiffness batch arguments");
+    const int m = 3 * nen;
+    if ((int64_t)m * m * ne >= (int64_t)INT32_MAX * 4) throw ArgError("batch too large");
+    hipStream_t st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    std::unique_ptr<void, void (*)(void*)> guard(st, [](void* p) { (void)hipStreamDestroy((hipStream_t)p); });
+    DBuf<double> dx, dE, dv, dk;
+    DBuf<int> dc;
+    dx.upload(coords, (size_t)ne * nen * 3, st);
+    std::vector<int> c(ne * nen);
+    for (int64_t k = 0; k < ne * nen; ++k) c[k] = (int)k;
+    dc.upload_vec(c, st);
+    dE.upload(young, ne, st);
+    dv.upload(nu, ne, st);
+    dk.alloc((size_t)m * m * ne);
+    if (nen == 4)
+      mlk::k_stiffness<4><<<grid_for(12 * ne), 256, 0, st>>>(ne, dx.p, dc.p, dE.p, dv.p, dk.p);
+    else
+      mlk::k_stiffness<10><<<grid_for(30 * ne), 256, 0, st>>>(ne, dx.p, dc.p, dE.p, dv.p, dk.p);
+    HIP_CHECK(hipGetLastError());
+    std::vector<double> h((size_t)m * m * ne);
+    HIP_CHECK(hipMemcpyAsync(h.data(), dk.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    for (int64_t e = 0; e < ne; ++e)
+      for (int j = 0; j < m * m; ++j) ke[e * m * m + j] = h[(size_t)j * ne + e];
+    return ML_OK;
+  });
+}
+
+ml_status ml_pattern_build(int64_t n, int64_t ne, const int64_t* eoff, const int64_t* eit,
+                           ml_pattern** out) {
+  return guarded([&]() -> int {
+    if (!out || n <= 0 || n >= INT32_MAX || ne < 0 || !eoff || (ne > 0 && !eit))
+      throw ArgError("bad pattern arguments");
+    for (int64_t e = 0; e < ne; ++e)
+      for (int64_t p = eoff[e]; p < eoff[e + 1]; ++p)
+        if (eit[p] < 0 || eit[p] >= n) throw ArgError("element vertex out of range");
+    auto* h = new ml_pattern;
+    h->graph = graph_from_elements(n, ne, eoff, eit);
+    h->order = reverse_cuthill_mckee(h->graph);
+    std::vector<int64_t> rank(n);
+    for (int64_t r = 0; r < n; ++r) rank[h->order[r]] = r;
+    h->lower = lower_pattern(h->graph, h->order, rank);
+    *out = h;
+    return ML_OK;
+  });
+}
+
+int64_t ml_pattern_n(const ml_pattern* p) { return p ? p->lower.n() : 0; }
+int64_t ml_pattern_nnz(const ml_pattern* p) { return p ? p->lower.nnz() : 0; }
+int64_t ml_pattern_ntargets(const ml_pattern* p) { return p ? (int64_t)p->graph.targets.size() : 0; }
+
+ml_status ml_pattern_get(const ml_pattern* p, int64_t* order, int64_t* offsets, int64_t* cols) {
+  if (!p) return ML_INVALID_ARGUMENT;
+  if (order) std::copy(p->order.begin(), p->order.end(), order);
+  if (offsets) std::copy(p->lower.offsets.begin(), p->lower.offsets.end(), offsets);
+  if (cols) std::copy(p->lower.cols.begin(), p->lower.cols.end(), cols);
+  return ML_OK;
+}
+
+ml_status ml_pattern_graph(const ml_pattern* p, int64_t* offsets, int64_t* targets) {
+  if (!p) return ML_INVALID_ARGUMENT;
+  if (offsets) std::copy(p->graph.offsets.begin(), p->graph.offsets.end(), offsets);
+  if (targets) std::copy(p->graph.targets.begin(), p->graph.targets.end(), targets);
+  return ML_OK;
+}
+
+void ml_pattern_free(ml_pattern* p) { delete p; }
+
+ml_status ml_pcg_create(int64_t n, const int64_t* offsets, const int64_t* cols,
+                        const double* values, ml_precision precision, ml_pcg** out) {
+  return guarded([&]() -> int {
+    if (!out || !values) throw ArgError("null argument");
+    if (precision != ML_FP64 && precision != ML_MIXED) throw ArgError("bad precision");
+    LowerPattern lp = pattern_from_csr(n, offsets, cols);
+    std::unique_ptr<ml_pcg> h(new ml_pcg);
+    h->op.precision = precision;
+    h->op.set_pattern(lp);
+    h->op.d_a.upload(values, lp.nnz(), h->op.stream);
+    h->op.values_changed();
+    int st = h->op.factor();
+    if (st != ML_OK) return st;
+    *out = h.release();
+    return ML_OK;
+  });
+}
+
+ml_status ml_pcg_solve_device(ml_pcg* h, const double* b, double* x, double tol, int64_t maxit,
+                              ml_solve_info* info) {
+  return guarded([&]() -> int {
+    if (!h || !b || !x) throw ArgError("null argument");
+    double t0 = now_s();
+    int st = h->op.solve(b, x, tol, maxit, info);
+    if (info) info->seconds[2] = now_s() - t0;
+    return st;
+  });
+}
+
+ml_status ml_pcg_solve(ml_pcg* h, const double* b, double* x, int64_t nrhs, double tol,
+                       int64_t maxit, ml_solve_info* info) {
+  return guarded([&]() -> int {
+    if (!h || !b || !x || nrhs <= 0) throw ArgError("bad arguments");
+    Operator& op = h->op;
+    const int n = op.n;
+    ml_solve_info worst;
+    std::memset(&worst, 0, sizeof(worst));
+    int result = ML_OK;
+    for (int64_t k = 0; k < nrhs; ++k) {
+      HIP_CHECK(hipMemcpyAsync(op.w_b.p, b + k * n, sizeof(double) * n, hipMemcpyHostToDevice,
+                               op.stream));
+      ml_solve_info one;
+      std::memset(&one, 0, sizeof(one));
+      // w_b is clobbered by refinement, so solve from a copy in w_ax's partner
+      DBuf<double> bcopy;
+      bcopy.alloc(n);
+      HIP_CHECK(hipMemcpyAsync(bcopy.p, op.w_b.p, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                               op.stream));
+      DBuf<double> xd;
+      xd.alloc(n);
+      int st = op.solve(bcopy.p, xd.p, tol, maxit, &one);
+      HIP_CHECK(hipMemcpyAsync(x + k * n, xd.p, sizeof(double) * n, hipMemcpyDeviceToHost,
+                               op.stream));
+      HIP_CHECK(hipStreamSynchronize(op.stream));
+      if (st != ML_OK && result == ML_OK) result = st;
+      if (one.iterations >= worst.iterations) worst = one;
+    }
+    if (info) *info = worst;
+    return result;
+  });
+}
+
+ml_status ml_pcg_multiply(ml_pcg* h, const double* x, double* y) {
+  return guarded([&]() -> int {
+    if (!h || !x || !y) throw ArgError("null argument");
+    Operator& op = h->op;
+    HIP_CHECK(hipMemcpyAsync(op.w_p.p, x, sizeof(double) * op.n, hipMemcpyHostToDevice, op.stream));
+    op.spmv64(op.w_p.p, op.w_ap.p, nullptr, nullptr);
+    HIP_CHECK(hipMemcpyAsync(y, op.w_ap.p, sizeof(double) * op.n, hipMemcpyDeviceToHost, op.stream));
+    HIP_CHECK(hipStreamSynchronize(op.stream));
+    return ML_OK;
+  });
+}
+
+ml_status ml_pcg_precondition(ml_pcg* h, const double* r, double* z) {
+  return guarded([&]() -> int {
+    if (!h || !r || !z) throw ArgError("null argument");
+    Operator& op = h->op;
+    if (!op.factored) {
+      int st = op.factor();
+      if (st != ML_OK) return st;
+    }
+    HIP_CHECK(hipMemcpyAsync(op.w_r.p, r, sizeof(double) * op.n, hipMemcpyHostToDevice, op.stream));
+    op.fill_sentinel<double>(op.w_y.p);
+    op.fill_sentinel<double>(op.w_z.p);
+    mlk::k_fill_i32<<<1, 64, 0, op.stream>>>(op.d_flags.p, 2, 0);
+    op.precondition<double>(op.w_r.p, op.w_y.p, op.w_z.p, nullptr, nullptr, nullptr);
+    HIP_CHECK(hipMemcpyAsync(z, op.w_z.p, sizeof(double) * op.n, hipMemcpyDeviceToHost, op.stream));
+    HIP_CHECK(hipStreamSynchronize(op.stream));
+    if (op.read_flag(0)) return ML_DEVICE_TIMEOUT;
+    return ML_OK;
+  });
+}
+
+ml_status ml_pcg_factor(ml_pcg* h, double* l, int32_t* shift_index) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    Operator& op = h->op;
+    if (!op.factored) {
+      int st = op.factor();
+      if (st != ML_OK) return st;
+    }
+    if (l) {
+      HIP_CHECK(hipMemcpyAsync(l, op.d_l.p, sizeof(double) * op.nnz, hipMemcpyDeviceToHost, op.stream));
+      HIP_CHECK(hipStreamSynchronize(op.stream));
+    }
+    if (shift_index) *shift_index = op.shift_index;
+    return ML_OK;
+  });
+}
+
+void ml_pcg_destroy(ml_pcg* h) { delete h; }
+
+ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elements, int nen,
+                          const int64_t* conn, const double* young, const double* nu,
+                          const uint8_t* fixed, const double* imposed, ml_solver_kind solver,
+                          ml_precision precision, ml_model** out, ml_solve_info* info) {
+  return guarded([&]() -> int {
+    if (!out || !nodes || !conn || !young || !nu || !fixed) throw ArgError("null argument");
+    if (solver == ML_DIRECT)
+      throw ArgError("the direct (skyline) solver is not part of the device library");
+    if (precision != ML_FP64 && precision != ML_MIXED) throw ArgError("bad precision");
+    if (info) {
+      std::memset(info, 0, sizeof(*info));
+      info->bad_element = -1;
+    }
+    std::unique_ptr<ml_model> h(new ml_model);
+    int st = h->m.create(n_nodes, nodes, n_elements, nen, conn, young, nu, fixed, imposed,
+                         precision, info);
+    if (st != ML_OK) return st;
+    *out = h.release();
+    return ML_OK;
+  });
+}
+
+ml_status ml_model_assemble(ml_model* h) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    h->m.assemble();
+    HIP_CHECK(hipStreamSynchronize(h->m.s()));
+    return ML_OK;
+  });
+}
+
+ml_status ml_model_set_young(ml_model* h, const double* young) {
+  return guarded([&]() -> int {
+    if (!h || !young) throw ArgError("null argument");
+    Model& m = h->m;
+    for (int64_t e = 0; e < m.plan.n_elements; ++e)
+      if (!(std::isfinite(young[e]) && young[e] > 0.0)) throw ArgError("Young's modulus must be > 0");
+    m.young.assign(young, young + m.plan.n_elements);
+    m.d_young.upload_vec(m.young, m.s());
+    m.op.factored = false;
+    m.assembled = false;
+    return ML_OK;
+  });
+}
+
+ml_status ml_model_factor(ml_model* h, ml_solve_info* info) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    if (!h->m.assembled) h->m.assemble();
+    double t0 = now_s();
+    int st = h->m.op.factor();
+    if (info) {
+      info->shift_index = h->m.op.shift_index;
+      info->seconds[1] = now_s() - t0;
+    }
+    return st;
+  });
+}
+
+ml_status ml_model_solve_device(ml_model* h, const double* f, double* u, double* R,
+                                ml_solve_info* info) {
+  return guarded([&]() -> int {
+    if (!h || !f || !u) throw ArgError("null argument");
+    if (!h->m.assembled) h->m.assemble();
+    return h->m.solve_device(f, u, R, info);
+  });
+}
+
+ml_status ml_model_solve(ml_model* h, const double* f, double* u, double* R, ml_solve_info* info) {
+  return guarded([&]() -> int {
+    if (!h || !f || !u) throw ArgError("null argument");
+    Model& m = h->m;
+    if (!m.assembled) m.assemble();
+    HIP_CHECK(hipMemcpyAsync(m.d_f.p, f, sizeof(double) * m.plan.ndof, hipMemcpyHostToDevice, m.s()));
+    int st = m.solve_device(m.d_f.p, m.d_u.p, R ? m.d_R.p : nullptr, info);
+    if (st != ML_OK && st != ML_NOT_CONVERGED) return st;
+    HIP_CHECK(hipMemcpyAsync(u, m.d_u.p, sizeof(double) * m.plan.ndof, hipMemcpyDeviceToHost, m.s()));
+    if (R)
+      HIP_CHECK(hipMemcpyAsync(R, m.d_R.p, sizeof(double) * m.plan.ndof, hipMemcpyDeviceToHost, m.s()));
+    HIP_CHECK(hipStreamSynchronize(m.s()));
+    return st;
+  });
+}
+
+ml_status ml_model_stresses(ml_model* h, const double* u, double* out) {
+  return guarded([&]() -> int {
+    if (!h || !u || !out) throw ArgError("null argument");
+    Model& m = h->m;
+    HIP_CHECK(hipMemcpyAsync(m.d_u.p, u, sizeof(double) * m.plan.ndof, hipMemcpyHostToDevice, m.s()));
+    DBuf<double> d_out;
+    size_t count = (size_t)m.plan.n_elements * m.ngp * 9;
+    d_out.alloc(count);
+    m.stresses(m.d_u.p, d_out.p);
+    HIP_CHECK(hipMemcpyAsync(out, d_out.p, count * sizeof(double), hipMemcpyDeviceToHost, m.s()));
+    HIP_CHECK(hipStreamSynchronize(m.s()));
+    return ML_OK;
+  });
+}
+
+int64_t ml_model_nfree(const ml_model* h) { return h ? h->m.plan.nfree : 0; }
+int64_t ml_model_nnz(const ml_model* h) { return h ? h->m.plan.lower.nnz() : 0; }
+int64_t ml_model_levels(const ml_model* h) { return h ? h->m.plan.depth : 0; }
+
+ml_status ml_model_csr(ml_model* h, int64_t* order, int64_t* offsets, int64_t* cols,
+                       double* values) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    Model& m = h->m;
+    if (order) std::copy(m.plan.order.begin(), m.plan.order.end(), order);
+    if (offsets) std::copy(m.plan.lower.offsets.begin(), m.plan.lower.offsets.end(), offsets);
+    if (cols) std::copy(m.plan.lower.cols.begin(), m.plan.lower.cols.end(), cols);
+    if (values) {
+      HIP_CHECK(hipMemcpyAsync(values, m.op.d_a.p, sizeof(double) * m.op.nnz, hipMemcpyDeviceToHost, m.s()));
+      HIP_CHECK(hipStreamSynchronize(m.s()));
+    }
+    return ML_OK;
+  });
+}
+
+ml_status ml_model_compliance_gradient(ml_model* h, double* out) {
+  (void)h;
+  (void)out;
+  g_last_error = "not implemented";
+  return ML_INVALID_ARGUMENT;
+}
+
+ml_status ml_model_add_pressures(ml_model* h, int64_t count, const int64_t* elements,
+                                 const int64_t* faces, const double* values, double* f) {
+  return guarded([&]() -> int {
+    if (!h || (count > 0 && (!elements || !faces || !values)) || !f) throw ArgError("null argument");
+    Model& m = h->m;
+    for (int64_t k = 0; k < count; ++k) {
+      if (elements[k] < 0 || elements[k] >= m.plan.n_elements || faces[k] < 0 || faces[k] > 3)
+        throw ArgError("pressure on a face that does not exist");
+      face_pressure_add(m.plan.nen, m.conn.data() + elements[k] * m.plan.nen, m.nodes.data(),
+                        (int)faces[k], values[k], f);
+    }
+    return ML_OK;
+  });
+}
+
+void ml_model_destroy(ml_model* h) { delete h; }
+
+// Profiling hooks for the benchmark: accumulate kernel durations measured with HIP events on
+// the handle's stream. kinds: 0 spmv64, 1 spmv32, 2 precondition64 (fwd+bwd), 3
+// precondition32, 4 ic0, 5 stiffness, 6 assembly, 7 vector updates.
+ml_status ml_model_profile(ml_model* h, int enable, double* ms, int64_t* counts) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    Profiler& p = h->m.op.prof;
+    p.resolve();
+    if (ms)
+      for (int k = 0; k < P_COUNT; ++k) ms[k] = p.ms[k];
+    if (counts)
+      for (int k = 0; k < P_COUNT; ++k) counts[k] = p.count[k];
+    if (enable >= 0) {
+      p.reset();
+      p.on = enable != 0;
+    }
+    return ML_OK;
+  });
+}
+
+}  // extern "C"

@@ -397,6 +397,22 @@ class System:
         st = lib().or_system_solve(self._h, _d(f), _d(u), tol, maxit, _i(it), _d(res))
         return st, u, int(it[0]), float(res[0])
 
+    def true_residual(self, f, u):
+        """|b - K_ff x| / |b| in the solver numbering, b = f_free - coupling, x = u_free."""
+        order = self.order()
+        off, cols, vals = self.csr()
+        free = np.flatnonzero(np.asarray(self.fixed) == 0)
+        rank = np.empty(len(order), dtype=np.int64)
+        rank[order] = np.arange(len(order))
+        b = np.zeros(self.n)
+        b[rank] = np.asarray(f)[free]
+        b -= self.coupling()
+        x = np.zeros(self.n)
+        x[rank] = np.asarray(u)[free]
+        a = Csr.from_arrays(off, cols, vals)
+        nb = np.linalg.norm(b)
+        return float(np.linalg.norm(b - a.multiply(x)) / (nb if nb > 0 else 1.0))
+
     def times(self):
         t = np.zeros(4)
         lib().or_system_times(self._h, _d(t))
@@ -464,6 +480,8 @@ def model_solve(model, iterative=True, tol=TOLERANCE, maxit=MAX_ITERATIONS):
     st, u, it, res = sys_.solve(f, tol, maxit)
     out = {"status": st, "displacements": u, "iterations": it, "residual": res,
            "shift_index": sys_.shift_index, "times": sys_.times(), "f": f}
+    if iterative and st == OK:
+        out["true_residual"] = sys_.true_residual(f, u)
     if st == OK:
         out["reactions"] = reactions(model.nodes, model.conn, model.young, model.nu,
                                      model.fixed, f, u)

@@ -1,0 +1,152 @@
+"""Parity of the HIP path with the oracle (the FP64 restatement of the reference), through the
+C ABI. Bit-exact where the device restates the reference's arithmetic operation for operation
+(element stiffness, assembled K_ff, SpMV, IC(0) factor, triangular solves, reactions,
+stresses); displacements of the PCG solve within 1e-10 relative L2 of the oracle's PCG (FP64)
+and 1e-8 (mixed), the north-star tolerances."""
+import numpy as np
+import pytest
+
+import oracle
+from maille_amd import meshes as M
+from conftest import rel_l2, relative_gap
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems():
+    return {
+        "reference_cantilever": M.reference_cantilever(),
+        "cantilever_tet4": M.cantilever_tet4(),
+        "jittered_cube_8": M.jittered_cube(8),
+        "cantilever_tet10_small": M.cantilever_tet10((12, 3, 3), (60.0, 15.0, 15.0)),
+    }
+
+
+PROBLEMS = _problems()
+
+
+def _oracle_system(p):
+    s = oracle.System(p.nodes, p.conn, p.young, p.nu, p.fixed, p.imposed)
+    return s
+
+
+@pytest.mark.parametrize("nen", [4, 10])
+def test_stiffness_batch_is_bit_exact(gpu, nen):
+    rng = np.random.default_rng(7)
+    nodes, tets = M.box_grid((3, 2, 2), (3.0, 2.0, 2.0))
+    if nen == 10:
+        nodes, tets = M.to_quadratic_tet(nodes, tets)
+    coords = nodes[tets]
+    young = rng.uniform(5e4, 2.5e5, len(tets))
+    nu = rng.uniform(0.1, 0.45, len(tets))
+    print(f"stiffness batch nen={nen} ne={len(tets)}", flush=True)
+    ke = gpu.stiffness_batch(coords, young, nu)
+    for e in range(len(tets)):
+        ref = oracle.solid_stiffness(coords[e], young[e], nu[e])
+        assert np.array_equal(ke[e], ref), f"element {e}"
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_assembled_matrix_is_bit_exact(gpu, name):
+    p = PROBLEMS[name]
+    print(f"assemble {name} ndof={p.ndof}", flush=True)
+    m = gpu.Model(p)
+    s = _oracle_system(p)
+    order, off, cols, vals = m.csr()
+    ooff, ocols, ovals = s.csr()
+    assert np.array_equal(order, s.order())
+    assert np.array_equal(off, ooff) and np.array_equal(cols, ocols)
+    assert np.array_equal(vals, ovals)
+
+
+@pytest.mark.parametrize("name", ["reference_cantilever", "jittered_cube_8"])
+def test_spmv_factor_and_preconditioner(gpu, name):
+    p = PROBLEMS[name]
+    s = _oracle_system(p)
+    off, cols, vals = s.csr()
+    print(f"pcg operator {name} n={s.n}", flush=True)
+    pcg = gpu.Pcg(gpu.Csr(off, cols, vals))
+    csr = oracle.Csr.from_arrays(off, cols, vals)
+    x = np.sin(0.37 * np.arange(s.n))
+    assert np.array_equal(pcg.multiply(x), csr.multiply(x))  # bit-exact SpMV
+    assert s.factorize() == oracle.OK
+    l, shift = pcg.factor()
+    assert shift == s.shift_index
+    assert np.array_equal(l, s.l_values())  # bit-exact IC(0)
+    r = np.cos(0.11 * np.arange(s.n))
+    assert np.array_equal(pcg.precondition(r), csr.precondition(s.l_values(), r))
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_fp64_solve_matches_oracle_pcg(gpu, name):
+    p = PROBLEMS[name]
+    print(f"fp64 solve {name}", flush=True)
+    ref = oracle.model_solve(p, iterative=True)
+    assert ref["status"] == oracle.OK
+    m = gpu.Model(p)
+    u, r, info = m.solve()
+    assert rel_l2(u, ref["displacements"]) <= 1e-10
+    assert abs(info["iterations"] - ref["iterations"]) <= 3
+    # the recursive residual of PCG drifts from b - Ax; the device drifts like the oracle
+    assert info["true_residual"] <= 10 * ref["true_residual"] + 1e-12
+    assert relative_gap(r, ref["reactions"]) < 1e-8
+    st = m.stresses(u)
+    assert relative_gap(st[..., 3:], ref["stresses"][..., 3:]) < 1e-7
+    assert np.array_equal(st[..., :3], ref["stresses"][..., :3])
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_mixed_solve_matches_oracle(gpu, name):
+    p = PROBLEMS[name]
+    print(f"mixed solve {name}", flush=True)
+    ref = oracle.model_solve(p, iterative=True)
+    m = gpu.Model(p, precision=gpu.MIXED)
+    u, r, info = m.solve()
+    assert rel_l2(u, ref["displacements"]) <= 1e-8
+    assert info["refinements"] >= 1
+
+
+def test_iterative_matches_direct_on_reference_cantilever(gpu):
+    """tests/iterative.rs:69-83 with the device PCG against the oracle's skyline Cholesky."""
+    p = PROBLEMS["reference_cantilever"]
+    direct = oracle.model_solve(p, iterative=False)
+    u, r, _ = gpu.Model(p).solve()
+    assert relative_gap(u, direct["displacements"]) < 1e-8
+    assert relative_gap(r, direct["reactions"]) < 1e-8
+
+
+def test_shift_ladder_on_the_device(gpu):
+    """iterative.rs:201-205: Kershaw's matrix fails unshifted IC(0); the device climbs the
+    ladder to the same shift as the oracle and solves."""
+    from test_oracle import KERSHAW, kershaw_csr
+    a = kershaw_csr(oracle)
+    st, x_ref, _, _, shift = a.pcg(np.array([1.0, 2.0, 3.0, 4.0]))
+    off, cols, vals = a.arrays()
+    pcg = gpu.Pcg(gpu.Csr(off, cols, vals))
+    l, dshift = pcg.factor()
+    assert dshift == shift > 0
+    x = pcg.solve(np.array([1.0, 2.0, 3.0, 4.0]))
+    assert np.abs(KERSHAW @ x - [1, 2, 3, 4]).max() < 1e-9
+
+
+def test_singular_and_zero_right_hand_side(gpu):
+    """iterative.rs:211-216 (zero b gives exactly x = 0) and tests/iterative.rs:200-224 (an
+    unconstrained structure is reported as Singular or IterativeNotConverged)."""
+    p = PROBLEMS["cantilever_tet4"]
+    m = gpu.Model(p)
+    u, _, info = m.solve(np.zeros(p.ndof))
+    assert np.all(u == 0) and info["iterations"] == 0
+    q = M.reference_cantilever((4, 1, 1))
+    q.fixed[:] = False
+    q.f[:] = 0.0
+    q.f[2] = 1.0
+    with pytest.raises((gpu.Singular, gpu.IterativeNotConverged)):
+        gpu.Model(q).solve()
+
+
+def test_pressure_loads_match_oracle(gpu):
+    p = M.jittered_cube(6)
+    m = gpu.Model(p)
+    f = m.loads()
+    g = oracle.add_pressures(p.nodes, p.conn, p.pressures, p.f.copy())
+    assert np.array_equal(f, g)

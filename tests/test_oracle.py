@@ -1,0 +1,594 @@
+"""Pins the oracle (the FP64 CPU restatement of the reference path) without running the
+reference: closed forms, invariants, brute-force reductions and the reference's own
+analytic tests, restated. Each test names the reference test or property it follows."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from maille_amd import meshes as M
+from conftest import relative_gap
+
+E, NU = 210e3, 0.3
+
+
+# ------------------------------------------------------------------ shape and quadrature
+@pytest.mark.parametrize("nen", [4, 10])
+def test_shape_functions_kronecker_and_partition_of_unity(nen):
+    """shape.rs:336-374: N_i(node_j) = delta_ij, sum N = 1, sum dN = 0."""
+    ref = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    if nen == 10:
+        ref = np.vstack([ref, [(ref[a] + ref[b]) / 2 for a, b in M.TET10_EDGES]])
+    for j, p in enumerate(ref):
+        n, dn = oracle.tet_shape(nen, *p)
+        assert np.allclose(n, np.eye(nen)[j], atol=1e-15)
+    n, dn = oracle.tet_shape(nen, 0.2, 0.3, 0.1)
+    assert abs(n.sum() - 1) < 1e-15 and np.abs(dn.sum(axis=0)).max() < 1e-14
+
+
+def test_tet10_derivatives_match_finite_differences_and_reproduce_quadratics():
+    """shape.rs:348-374."""
+    p = np.array([0.21, 0.17, 0.33])
+    n0, dn = oracle.tet_shape(10, *p)
+    h = 1e-6
+    for c in range(3):
+        e = np.zeros(3)
+        e[c] = h
+        fd = (oracle.tet_shape(10, *(p + e))[0] - oracle.tet_shape(10, *(p - e))[0]) / (2 * h)
+        assert np.abs(fd - dn[:, c]).max() < 1e-8
+    ref = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    nodes = np.vstack([ref, [(ref[a] + ref[b]) / 2 for a, b in M.TET10_EDGES]])
+    f = lambda x: 1 + 2 * x[0] - x[1] + 3 * x[0] * x[2] + x[1] ** 2  # noqa: E731
+    assert abs(np.dot(n0, [f(x) for x in nodes]) - f(p)) < 1e-14
+
+
+@pytest.mark.parametrize("nen", [4, 10])
+def test_mapping_determinant_is_six_volumes_and_gradient_of_x_is_identity(nen):
+    """shape.rs:377-405."""
+    rng = np.random.default_rng(1)
+    x4 = np.array([[0, 0, 0], [2, 0.1, 0], [0.3, 1.5, 0.2], [0.1, 0.2, 1.2]]) + rng.normal(0, 0.05, (4, 3))
+    x = x4 if nen == 4 else np.vstack([x4, [(x4[a] + x4[b]) / 2 for a, b in M.TET10_EDGES]])
+    six_v = M.signed_volume(x4, np.array([[0, 1, 2, 3]]))[0]
+    for q in [(0.25, 0.25, 0.25), (0.1, 0.2, 0.3)]:
+        _, dn = oracle.tet_shape(nen, *q)
+        det, g = oracle.mapping3d(x, dn)
+        assert abs(det - six_v) < 1e-13 * abs(six_v)
+        assert np.abs(x.T @ g - np.eye(3)).max() < 1e-13
+
+
+# ------------------------------------------------------------------ element
+def test_solid_d_is_the_lame_matrix():
+    """element.rs:770-779: lambda + 2mu on the diagonal, lambda off it, mu in shear."""
+    lam = E * NU / ((1 + NU) * (1 - 2 * NU))
+    mu = E / (2 * (1 + NU))
+    d = oracle.solid_d(E, NU)
+    expect = np.zeros((6, 6))
+    expect[:3, :3] = lam
+    expect[np.arange(3), np.arange(3)] = lam + 2 * mu
+    expect[np.arange(3, 6), np.arange(3, 6)] = mu
+    assert np.allclose(d, expect, rtol=1e-14, atol=0)
+
+
+def _random_tet(nen, seed=3):
+    rng = np.random.default_rng(seed)
+    x4 = np.array([[0, 0, 0], [1.2, 0.1, 0], [0.2, 0.9, 0.1], [0.1, 0.3, 1.1]]) + rng.normal(0, 0.05, (4, 3))
+    if nen == 4:
+        return x4
+    return np.vstack([x4, [(x4[a] + x4[b]) / 2 for a, b in M.TET10_EDGES]])
+
+
+@pytest.mark.parametrize("nen", [4, 10])
+def test_stiffness_symmetry_rigid_modes_and_energy(nen):
+    """element.rs:782-839: K symmetric, the 6 rigid modes in its null space, and for a linear
+    field u = eps x the energy u'Ku equals V eps'D eps."""
+    x = _random_tet(nen)
+    k = oracle.solid_stiffness(x, E, NU)
+    assert np.abs(k - k.T).max() < 1e-10 * np.abs(k).max()
+    modes = []
+    for c in range(3):
+        t = np.zeros((len(x), 3))
+        t[:, c] = 1
+        modes.append(t.ravel())
+    for axis in range(3):
+        w = np.zeros(3)
+        w[axis] = 1
+        modes.append(np.cross(w, x).ravel())
+    for mvec in modes:
+        assert np.abs(k @ mvec).max() < 1e-9 * np.abs(k).max() * np.abs(mvec).max()
+    eps = np.array([1e-3, -2e-3, 0.5e-3, 3e-3, -1e-3, 2e-3])
+    grad = np.array([[eps[0], eps[5] / 2, eps[4] / 2],
+                     [eps[5] / 2, eps[1], eps[3] / 2],
+                     [eps[4] / 2, eps[3] / 2, eps[2]]])
+    u = (x @ grad.T).ravel()
+    vol = abs(M.signed_volume(x[:4], np.array([[0, 1, 2, 3]]))[0]) / 6
+    energy = vol * eps @ oracle.solid_d(E, NU) @ eps
+    assert abs(u @ k @ u - energy) < 1e-9 * abs(energy)
+
+
+@pytest.mark.parametrize("nen", [4, 10])
+def test_stresses_of_a_linear_field_are_d_eps(nen):
+    """element.rs:840-860."""
+    x = _random_tet(nen)
+    eps = np.array([1e-3, -2e-3, 0.5e-3, 3e-3, -1e-3, 2e-3])
+    grad = np.array([[eps[0], eps[5] / 2, eps[4] / 2],
+                     [eps[5] / 2, eps[1], eps[3] / 2],
+                     [eps[4] / 2, eps[3] / 2, eps[2]]])
+    s = oracle.solid_stresses(x, E, NU, (x @ grad.T).ravel())
+    sig = oracle.solid_d(E, NU) @ eps
+    assert np.abs(s[:, 3:] - sig).max() < 1e-9 * np.abs(sig).max()
+
+
+@pytest.mark.parametrize("nen", [4, 10])
+def test_face_pressure_splits_the_resultant(nen):
+    """assembly.rs:1299-1326: Tet4 face -> -p A n / 3 per vertex; Tet10 face -> 0 at the
+    vertices and -p A n / 3 per mid-node; the normal points out of the element."""
+    x = _random_tet(nen)
+    for face in range(4):
+        pos, forces = oracle.face_pressure(x, face, 2.0)
+        v = x[pos[:3]]
+        cr = np.cross(v[1] - v[0], v[2] - v[0])
+        area = np.linalg.norm(cr) / 2
+        n = cr / np.linalg.norm(cr)
+        if np.dot(n, v.mean(axis=0) - x[:4].mean(axis=0)) < 0:
+            n = -n
+        per = -2.0 * area * n / 3
+        if nen == 4:
+            assert np.abs(forces - per).max() < 1e-12
+        else:
+            assert np.abs(forces[:3]).max() < 1e-12
+            assert np.abs(forces[3:] - per).max() < 1e-12
+
+
+# ------------------------------------------------------------------ dot, graph, RCM
+def test_dot_matches_the_plain_sum_for_every_length_around_the_chunk_size():
+    """solve.rs:349-359."""
+    for n in range(40):
+        a = 0.5 + np.arange(n, dtype=float)
+        b = 2.0 - 0.25 * np.arange(n, dtype=float)
+        plain = float(np.sum(a * b))
+        assert abs(oracle.dot(a, b) - plain) <= 1e-12 * (1 + abs(plain))
+
+
+def test_graph_lists_each_neighbour_once():
+    """iterative.rs:285-293."""
+    g = oracle.Graph.from_elements(4, [[0, 1, 2], [1, 2, 3]])
+    assert sorted(g.neighbours(1).tolist()) == [0, 2, 3]
+    assert len(g.neighbours(0)) == 2
+
+
+def _rcm_python(adj):
+    """solve.rs:200-270 restated independently in Python with Rust's iterator semantics
+    (min_by_key keeps the first minimum, sort_by_key is stable)."""
+    n = len(adj)
+    deg = [len(a) for a in adj]
+    visited = [False] * n
+    order = []
+
+    def levels_from(start):
+        seen = visited[:]
+        seen[start] = True
+        levels = [[start]]
+        while True:
+            nxt = []
+            for v in levels[-1]:
+                for w in adj[v]:
+                    if not seen[w]:
+                        seen[w] = True
+                        nxt.append(w)
+            if not nxt:
+                return levels
+            levels.append(nxt)
+
+    while True:
+        cand = [v for v in range(n) if not visited[v]]
+        if not cand:
+            break
+        seed = min(cand, key=lambda v: deg[v])  # first minimum
+        start, levels = seed, levels_from(seed)
+        while True:
+            far = min(levels[-1], key=lambda v: deg[v])
+            fl = levels_from(far)
+            if len(fl) <= len(levels):
+                break
+            start, levels = far, fl
+        head = len(order)
+        visited[start] = True
+        order.append(start)
+        while head < len(order):
+            v = order[head]
+            head += 1
+            nxt = sorted([w for w in adj[v] if not visited[w]], key=lambda w: deg[w])
+            for w in nxt:
+                if not visited[w]:
+                    visited[w] = True
+                    order.append(w)
+    return order[::-1]
+
+
+def test_rcm_matches_an_independent_restatement_on_random_graphs():
+    rng = np.random.default_rng(11)
+    for trial in range(40):
+        n = int(rng.integers(1, 40))
+        ne = int(rng.integers(0, 30))
+        elements = [list(rng.choice(n, size=min(n, int(rng.integers(1, 5))), replace=False))
+                    for _ in range(ne)]
+        g = oracle.Graph.from_elements(n, elements)
+        off, tgt = g.arrays()
+        adj = [tgt[off[v]:off[v + 1]].tolist() for v in range(n)]
+        # Graph::from_elements order: incident elements in order, first-seen neighbours
+        expect_adj = []
+        for v in range(n):
+            seen, row = {v}, []
+            for e in elements:
+                if v in e:
+                    for w in e:
+                        if w not in seen:
+                            seen.add(w)
+                            row.append(int(w))
+            expect_adj.append(row)
+        assert adj == expect_adj
+        assert g.rcm().tolist() == _rcm_python(adj), f"trial {trial}"
+
+
+def test_rcm_is_a_permutation_that_shrinks_the_profile_of_a_shuffled_grid():
+    """solve.rs:387-416."""
+    side = 6
+    shuffle = [3, 0, 5, 1, 4, 2]
+    idx = lambda i, j: shuffle[i] * side + shuffle[j]  # noqa: E731
+    adj = [[] for _ in range(side * side)]
+    for i in range(side):
+        for j in range(side):
+            if i + 1 < side:
+                adj[idx(i, j)].append(idx(i + 1, j))
+                adj[idx(i + 1, j)].append(idx(i, j))
+            if j + 1 < side:
+                adj[idx(i, j)].append(idx(i, j + 1))
+                adj[idx(i, j + 1)].append(idx(i, j))
+    order = oracle.Graph.from_lists(adj).rcm()
+    assert sorted(order.tolist()) == list(range(side * side))
+
+    def profile(order):
+        rank = np.empty(len(order), dtype=int)
+        rank[order] = np.arange(len(order))
+        return sum(rank[v] - min([rank[w] for w in adj[v]] + [rank[v]]) + 1
+                   for v in range(len(adj)))
+
+    before, after = profile(np.arange(side * side)), profile(order)
+    assert after <= side * side * (side + 1) and after < before
+
+
+def test_rcm_handles_disconnected_graphs_and_isolated_vertices():
+    """solve.rs:418-425."""
+    order = oracle.Graph.from_lists([[2], [3], [0], [1], []]).rcm()
+    assert sorted(order.tolist()) == [0, 1, 2, 3, 4]
+
+
+# ------------------------------------------------------------------ IC(0), PCG, skyline
+def _grid_system(n):
+    """iterative.rs:257-283: (4 + degree, -1) Laplacian of an n x n grid, RCM-numbered."""
+    idx = lambda i, j: i + n * j  # noqa: E731
+    elements = [[idx(i, j), idx(i + 1, j), idx(i + 1, j + 1), idx(i, j + 1)]
+                for j in range(n - 1) for i in range(n - 1)]
+    g = oracle.Graph.from_elements(n * n, elements)
+    order = g.rcm()
+    rank = np.empty(n * n, dtype=int)
+    rank[order] = np.arange(n * n)
+    a = oracle.Csr.zeros(g, order)
+    for v in range(n * n):
+        nb = g.neighbours(v)
+        a.add(rank[v], rank[v], 1.0 + len(nb))
+        for w in nb[nb < v]:
+            a.add(rank[v], rank[w], -1.0)
+    x = np.sin(0.37 * np.arange(n * n))
+    return a, x, a.multiply(x)
+
+
+def test_pcg_recovers_the_chosen_solution():
+    """iterative.rs:302-309."""
+    a, x, b = _grid_system(30)
+    st, got, it, res, _ = a.pcg(b)
+    assert st == oracle.OK and np.abs(got - x).max() < 1e-8
+
+
+def test_pcg_solves_a_zero_right_hand_side_exactly():
+    """iterative.rs:311-324."""
+    a, _, b = _grid_system(10)
+    st, x, it, _, _ = a.pcg(np.zeros(100))
+    assert st == oracle.OK and it == 0 and np.all(x == 0)
+    st, x, _, _, _ = a.pcg(b)
+    assert np.abs(a.multiply(x) - b).max() < 1e-8
+
+
+def test_incomplete_cholesky_of_a_tridiagonal_matrix_is_exact():
+    """iterative.rs:326-346."""
+    g = oracle.Graph.from_elements(4, [[0, 1], [1, 2], [2, 3]])
+    a = oracle.Csr.zeros(g, np.arange(4))
+    for i in range(4):
+        a.add(i, i, 2.0)
+        if i > 0:
+            a.add(i, i - 1, -1.0)
+    l = a.incomplete_cholesky(0.0)
+    z = a.precondition(l, [1.0, 0.0, 0.0, 1.0])
+    assert np.abs(a.multiply(z) - [1, 0, 0, 1]).max() < 1e-13
+
+
+def test_incomplete_cholesky_without_dropped_fill_is_the_cholesky_factor():
+    """With a full lower pattern IC(0) has nothing to drop: it is numpy's Cholesky."""
+    rng = np.random.default_rng(5)
+    n = 9
+    b = rng.normal(size=(n, n))
+    dense = b @ b.T + n * np.eye(n)
+    elements = [list(range(n))]
+    g = oracle.Graph.from_elements(n, elements)
+    a = oracle.Csr.zeros(g, np.arange(n))
+    for i in range(n):
+        for j in range(i + 1):
+            a.add(i, j, dense[i, j])
+    l = a.incomplete_cholesky(0.0)
+    off, cols, _ = a.arrays()
+    lmat = np.zeros((n, n))
+    for i in range(n):
+        lmat[i, cols[off[i]:off[i + 1]]] = l[off[i]:off[i + 1]]
+    assert np.abs(lmat - np.linalg.cholesky(dense)).max() < 1e-12
+
+
+def test_singular_chain_is_reported():
+    """iterative.rs:348-362."""
+    g = oracle.Graph.from_elements(3, [[0, 1], [1, 2]])
+    a = oracle.Csr.zeros(g, np.arange(3))
+    for i, j, v in [(0, 0, 1.0), (1, 1, 2.0), (2, 2, 1.0), (1, 0, -1.0), (2, 1, -1.0)]:
+        a.add(i, j, v)
+    st = a.pcg(np.ones(3))[0]
+    assert st != oracle.OK
+
+
+KERSHAW = np.array([[3.0, -2, 0, 2], [-2, 3, -2, 0], [0, -2, 3, -2], [2, 0, -2, 3]])
+
+
+def kershaw_csr(module):
+    """Kershaw's SPD matrix on a 4-cycle: unshifted IC(0) breaks down on it."""
+    g = module.Graph.from_elements(4, [[0, 1], [1, 2], [2, 3], [3, 0]])
+    a = module.Csr.zeros(g, np.arange(4))
+    for i in range(4):
+        for j in range(i + 1):
+            if KERSHAW[i, j] != 0:
+                a.add(i, j, KERSHAW[i, j])
+    return a
+
+
+def test_shift_ladder_recovers_a_failed_ic0():
+    """iterative.rs:12-14, 201-205: the first shift of the ladder that factors is used."""
+    assert np.linalg.eigvalsh(KERSHAW).min() > 0
+    a = kershaw_csr(oracle)
+    st, x, it, res, shift = a.pcg(np.array([1.0, 2.0, 3.0, 4.0]))
+    assert shift > 0
+    for k in range(shift):
+        assert a.incomplete_cholesky(oracle.SHIFTS[k]) is None
+    assert a.incomplete_cholesky(oracle.SHIFTS[shift]) is not None
+    assert st == oracle.OK and np.abs(KERSHAW @ x - [1, 2, 3, 4]).max() < 1e-9
+
+
+def test_skyline_solves_the_known_spd_system():
+    """solve.rs:287-305."""
+    a = np.array([[4, 1, 0, 0.5], [1, 5, 2, 0], [0, 2, 6, 1], [0.5, 0, 1, 3.0]])
+    x = np.array([1, -2, 3, 0.5])
+    s = oracle.Skyline([0, 0, 0, 0])
+    for i in range(4):
+        for j in range(i + 1):
+            s.add(i, j, a[i, j])
+    assert s.factorize() == oracle.OK
+    assert np.abs(s.solve(a @ x) - x).max() < 1e-13
+
+
+def test_skyline_variable_profile():
+    """solve.rs:307-332."""
+    n = 5
+    full, sky = oracle.Skyline([0] * n), oracle.Skyline([0, 0, 1, 2, 1])
+    for m in (full, sky):
+        for i in range(n):
+            m.add(i, i, 2.0)
+            if i > 0:
+                m.add(i, i - 1, -1.0)
+        m.add(1, 4, 0.5)
+    assert sky.stored() == 11
+    b = np.array([1.0, 0.0, 2.0, -1.0, 3.0])
+    full.factorize()
+    sky.factorize()
+    assert np.abs(full.solve(b) - sky.solve(b)).max() < 1e-13
+
+
+def test_skyline_singular():
+    """solve.rs:361-368."""
+    s = oracle.Skyline([0, 0])
+    for i, j, v in [(0, 0, 1.0), (1, 0, 1.0), (1, 1, 1.0)]:
+        s.add(i, j, v)
+    assert s.factorize() == oracle.SINGULAR
+
+
+# ------------------------------------------------------------------ the static model
+def _box_problem(n, size, quadratic, fixed_fn, imposed_fn=None):
+    nodes, tets = M.box_grid(n, size)
+    if quadratic:
+        nodes, tets = M.to_quadratic_tet(nodes, tets)
+    ne = len(tets)
+    fixed = np.zeros(3 * len(nodes), dtype=bool)
+    imposed = np.zeros(3 * len(nodes))
+    for v, p in enumerate(nodes):
+        for c in range(3):
+            val = fixed_fn(p, c)
+            if val is not None:
+                fixed[3 * v + c] = True
+                imposed[3 * v + c] = val
+    return M.Problem("box", nodes, tets, np.full(ne, E), np.full(ne, NU), fixed, imposed,
+                     np.zeros(3 * len(nodes)))
+
+
+EPS = np.array([1e-3, -2e-3, 0.5e-3, 3e-3, -1e-3, 2e-3])
+
+
+def _patch_exact(p):
+    e = EPS
+    return np.array([e[0] * p[0] + e[5] / 2 * p[1] + e[4] / 2 * p[2],
+                     e[5] / 2 * p[0] + e[1] * p[1] + e[3] / 2 * p[2],
+                     e[4] / 2 * p[0] + e[3] / 2 * p[1] + e[2] * p[2]])
+
+
+def patch_problem(quadratic):
+    """tests/solid.rs:52-116: linear field imposed on the boundary of a 2 x 1 x 1.5 box."""
+    size = np.array([2.0, 1.0, 1.5])
+
+    def fixed(p, c):
+        on = np.any(np.abs(p) < 1e-12) or np.any(np.abs(p - size) < 1e-12)
+        return _patch_exact(p)[c] if on else None
+
+    return _box_problem((3, 2, 2), size, quadratic, fixed)
+
+
+@pytest.mark.parametrize("quadratic", [False, True])
+@pytest.mark.parametrize("iterative", [False, True])
+def test_uniform_strain_patch_is_exact(quadratic, iterative):
+    """tests/solid.rs:52-116."""
+    p = patch_problem(quadratic)
+    r = oracle.model_solve(p, iterative=iterative)
+    assert r["status"] == oracle.OK
+    exact = np.array([_patch_exact(x) for x in p.nodes]).ravel()
+    assert np.abs(r["displacements"] - exact).max() < 1e-12
+    sig = oracle.solid_d(E, NU) @ EPS
+    assert np.abs(r["stresses"][..., 3:] - sig).max() <= 1e-9 * np.abs(sig).max()
+
+
+def uniaxial_problem(quadratic, length=4.0, delta=0.01):
+    def fixed(p, c):
+        if abs(p[0] - length) < 1e-12 and c == 0:
+            return delta
+        return 0.0 if abs(p[c]) < 1e-12 else None
+
+    return _box_problem((4, 1, 1), (length, 1.0, 1.0), quadratic, fixed)
+
+
+@pytest.mark.parametrize("quadratic", [False, True])
+def test_uniaxial_traction_gives_e_delta_over_l_and_poisson_contraction(quadratic):
+    """tests/solid.rs:119-175."""
+    length, delta = 4.0, 0.01
+    p = uniaxial_problem(quadratic, length, delta)
+    r = oracle.model_solve(p, iterative=False)
+    strain = delta / length
+    s = r["stresses"]
+    assert np.abs(s[..., 3] - E * strain).max() <= 1e-9 * E * strain
+    assert np.abs(s[..., 4]).max() < 1e-9 * E * strain
+    u = r["displacements"].reshape(-1, 3)
+    assert np.abs(u[:, 1] + NU * strain * p.nodes[:, 1]).max() < 1e-12
+    assert np.abs(u[:, 2] + NU * strain * p.nodes[:, 2]).max() < 1e-12
+    tip = np.flatnonzero(np.abs(p.nodes[:, 0] - length) < 1e-12)
+    pull = r["reactions"][3 * tip].sum()
+    assert abs(pull - E * strain) <= 1e-9 * E * strain
+
+
+def test_tet10_cantilever_matches_timoshenko_beam_theory():
+    """tests/solid.rs:181-248: Tet10 cantilever 10 x 1 x 1 under a unit end load, within 2%."""
+    length, b, h, load = 10.0, 1.0, 1.0, 1.0
+    p = M.cantilever_tet10((20, 2, 2), (length, b, h), E, NU, (0.0, 0.0, -load / (b * h)))
+    r = oracle.model_solve(p, iterative=True)
+    nodes = p.nodes
+    centre = np.flatnonzero((np.abs(nodes[:, 0] - length) < 1e-12)
+                            & (np.abs(nodes[:, 1] - b / 2) < 1e-12)
+                            & (np.abs(nodes[:, 2] - h / 2) < 1e-12))[0]
+    inertia, area, g = b * h ** 3 / 12, b * h, E / (2 * (1 + NU))
+    theory = load * length ** 3 / (3 * E * inertia) + load * length / (5 / 6 * g * area)
+    tip = -r["displacements"][3 * centre + 2]
+    assert abs(tip / theory - 1) < 0.02
+
+
+def test_cube_under_hydrostatic_pressure_shrinks_uniformly():
+    """tests/pressure.rs:195-226: pressure p on the three faces x=1, y=1, z=1 of a unit cube on
+    rollers at x=0, y=0, z=0 gives u = -p (1 - 2 nu) / E x."""
+    for quadratic in (False, True):
+        def fixed(p, c):
+            return 0.0 if abs(p[c]) < 1e-12 else None
+
+        prob = _box_problem((2, 2, 2), (1.0, 1.0, 1.0), quadratic, fixed)
+        pres = []
+        for axis in range(3):
+            e, f = M.faces_on(prob.nodes, prob.conn,
+                              lambda x, a=axis: np.abs(x[:, a] - 1.0) < 1e-12)
+            pres += list(zip(e.tolist(), f.tolist(), [5.0] * len(e)))
+        pres.sort()
+        prob.pressures = pres
+        r = oracle.model_solve(prob, iterative=False)
+        strain = -5.0 * (1 - 2 * NU) / E
+        assert np.abs(r["displacements"] - strain * prob.nodes.ravel()).max() < 1e-12
+
+
+def test_iterative_static_solution_matches_the_direct_one():
+    """tests/iterative.rs:69-83 on the oracle: PCG vs skyline on the Tet10 cantilever."""
+    p = M.reference_cantilever()
+    it = oracle.model_solve(p, iterative=True)
+    di = oracle.model_solve(p, iterative=False)
+    assert relative_gap(it["displacements"], di["displacements"]) < 1e-8
+    assert relative_gap(it["reactions"], di["reactions"]) < 1e-8
+    assert relative_gap(it["stresses"][..., 3], di["stresses"][..., 3]) < 1e-7
+
+
+def test_unconstrained_structure_is_reported():
+    """tests/iterative.rs:200-224."""
+    p = M.reference_cantilever((4, 1, 1))
+    p.fixed[:] = False
+    p.f[:] = 0.0
+    p.f[2] = 1.0
+    r = oracle.model_solve(p, iterative=True)
+    assert r["status"] in (oracle.SINGULAR, oracle.NOT_CONVERGED)
+
+
+def test_workload_sizes_match_the_kuhn_enumeration():
+    """Element, node, DOF and stored-entry counts of the benchmark meshes."""
+    p1 = M.cantilever_tet4()
+    assert (len(p1.conn), len(p1.nodes), p1.ndof, p1.nfree) == (1920, 525, 1575, 1500)
+    s = oracle.System(p1.nodes, p1.conn, p1.young, p1.nu, p1.fixed)
+    assert len(s.csr()[1]) == 26931
+    pr = M.reference_cantilever()
+    s = oracle.System(pr.nodes, pr.conn, pr.young, pr.nu, pr.fixed)
+    assert len(s.csr()[1]) == 48756
+    nodes, tets = M.box_grid((40, 8, 8), (40.0, 8.0, 8.0))
+    nodes, tets = M.to_quadratic_tet(nodes, tets)
+    assert (len(tets), len(nodes)) == (15360, 23409)
+
+
+def test_jittered_mesh_is_valid_and_seeded():
+    a = M.jittered_cube(6)
+    b = M.jittered_cube(6)
+    assert np.array_equal(a.nodes, b.nodes) and np.array_equal(a.young, b.young)
+    assert M.check_tetrahedra(a.nodes, a.conn) == -1
+    assert abs(np.median(np.log(M.jittered_cube(20).young / 200000.0))) < 0.02
+
+
+def test_box_grid_matches_a_literal_loop_restatement():
+    """tests/common/mod.rs:74-114 written as its loops, on a small box."""
+    n, size = (3, 2, 2), (3.0, 2.0, 1.0)
+    nodes, tets = M.box_grid(n, size)
+    idx = lambda p: p[0] + (n[0] + 1) * (p[1] + (n[1] + 1) * p[2])  # noqa: E731
+    expect = []
+    for k in range(n[2]):
+        for j in range(n[1]):
+            for i in range(n[0]):
+                for axes in itertools.permutations(range(3)):
+                    p = [i, j, k]
+                    tet = [idx(p)]
+                    for a in axes:
+                        p[a] += 1
+                        tet.append(idx(p))
+                    c = nodes[tet]
+                    if np.linalg.det(c[1:] - c[0]) < 0:
+                        tet[1], tet[2] = tet[2], tet[1]
+                    expect.append(tet)
+    assert np.array_equal(tets, np.array(expect))
+    q_nodes, q = M.to_quadratic_tet(nodes, tets)
+    mids, order = {}, len(nodes)
+    for t in tets:
+        for a, b in M.TET10_EDGES:
+            key = (min(t[a], t[b]), max(t[a], t[b]))
+            if key not in mids:
+                mids[key] = order
+                order += 1
+    for e, t in enumerate(tets):
+        assert [mids[(min(t[a], t[b]), max(t[a], t[b]))] for a, b in M.TET10_EDGES] == q[e, 4:].tolist()
