@@ -74,7 +74,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=100)
-    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--precision", choices=["mixed", "mixed_ic0", "fp64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=8)
     args = ap.parse_args()
@@ -93,9 +93,10 @@ def main():
     log(f"mesh: {len(problem.conn)} elements, {problem.nfree} free DOF ({time.time() - t0:.1f} s)")
 
     models = {}
-    for prec in (["mixed", "fp64"] if args.precision == "mixed" else ["fp64"]):
+    codes = {"fp64": ma.FP64, "mixed": ma.MIXED, "mixed_ic0": ma.MIXED_IC0}
+    for prec in ([args.precision, "fp64"] if args.precision != "fp64" else ["fp64"]):
         t1 = time.time()
-        m = ma.Model(problem, precision=ma.MIXED if prec == "mixed" else ma.FP64)
+        m = ma.Model(problem, precision=codes[prec])
         log(f"{prec}: model created in {time.time() - t1:.1f} s (host pattern "
             f"{m.create_info['seconds'][5]:.1f} s), nnz_L {m.nnz()}, DAG depth {m.levels()}")
         models[prec] = m
@@ -135,7 +136,7 @@ def main():
 
     # FP64 drop-in timed the same way (beside the headline)
     extra = {}
-    if head == "mixed":
+    if head != "fp64":
         mf = models["fp64"]
         mf.profile(enable=1)
         torch.cuda.synchronize()
@@ -145,7 +146,7 @@ def main():
         torch.cuda.synchronize()
         fp64_step = (time.perf_counter() - t64) / args.steps
         prof64 = mf.profile(enable=0)
-        u_m = u_dev["mixed"].cpu().numpy()
+        u_m = u_dev[head].cpu().numpy()
         u_64 = u_dev["fp64"].cpu().numpy()
         rel = float(np.linalg.norm(u_m - u_64) / np.linalg.norm(u_64))
         extra = {"fp64_s_per_step": round(fp64_step, 4),
@@ -160,11 +161,12 @@ def main():
 
     n, nnz_l = m.nfree(), m.nnz()
     nnz_full = 2 * nnz_l - n
-    val = 4 if head == "mixed" else 8
+    aval = 4 if head == "mixed" else 8
+    mval = 8 if head == "fp64" else 4
     kinds = {"spmv": "spmv32" if head == "mixed" else "spmv64",
-             "precondition": "precond32" if head == "mixed" else "precond64"}
-    bytes_per = {"spmv": spmv_bytes(n, nnz_full, val),
-                 "precondition": 2 * trisolve_bytes(n, nnz_l, val)}
+             "precondition": "precond64" if head == "fp64" else "precond32"}
+    bytes_per = {"spmv": spmv_bytes(n, nnz_full, aval),
+                 "precondition": 2 * trisolve_bytes(n, nnz_l, mval)}
     kernels = {}
     for name, kind in kinds.items():
         ms, cnt = prof[kind]
@@ -195,7 +197,8 @@ def main():
         "metric": METRIC, "value": round(per_step, 4), "unit": "s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per_step * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32+fp64" if head == "mixed" else "fp64", "data": "synthetic",
+        "dtype": {"fp64": "fp64", "mixed": "fp32+fp64", "mixed_ic0": "fp64+fp32-ic0"}[head],
+        "data": "synthetic",
         "config": {"workload": f"jittered_cube_{args.cells}", "free_dof": n, "nnz_L": nnz_l,
                    "elements": len(problem.conn), "precision": head,
                    "step": "assemble + IC(0) + PCG to 1e-12 + reactions, inputs in HBM",

@@ -39,9 +39,12 @@ typedef enum ml_status {
 } ml_status;
 
 /* Arithmetic of the solve. ML_FP64 is the drop-in (the reference computes in f64).
- * ML_MIXED: FP32 SpMV and FP32 IC(0) inside FP64 iterative refinement; the true residual
- * b - A x, the PCG dot products and the refinement update stay FP64. */
-typedef enum ml_precision { ML_FP64 = 0, ML_MIXED = 1 } ml_precision;
+ * ML_MIXED: FP32 SpMV and FP32 IC(0)/triangular solves inside one PCG whose iterate x,
+ * residual r, alpha/beta dot products and refinement stay FP64: the recursive residual is
+ * replaced by the FP64 true residual b - A x each time it has dropped tenfold (reliable
+ * updates), until a replacement stops reducing it.
+ * ML_MIXED_IC0: FP64 SpMV and PCG with an FP32 IC(0) factor and FP32 triangular solves. */
+typedef enum ml_precision { ML_FP64 = 0, ML_MIXED = 1, ML_MIXED_IC0 = 2 } ml_precision;
 
 /* SolverKind (solve.rs:171-195). ML_AUTO is iterative from ML_ITERATIVE_FROM free DOFs. */
 typedef enum ml_solver_kind { ML_AUTO = 0, ML_DIRECT = 1, ML_ITERATIVE = 2 } ml_solver_kind;
@@ -54,7 +57,7 @@ typedef struct ml_solve_info {
   double residual;           /* recursive relative residual |r|/|b| at exit (iterative.rs:246) */
   double true_residual;      /* |b - A x| / |b| computed in FP64 after the solve */
   int32_t shift_index;       /* index into the shift ladder {0,1e-3,1e-2,0.1,1,10} used */
-  int32_t refinements;       /* FP64 refinement sweeps (ML_MIXED), 0 for ML_FP64 */
+  int32_t refinements;       /* FP64 true-residual replacements (ML_MIXED), else 0 */
   int64_t bad_element;       /* ML_INVALID_MESH: first offending element, else -1 */
   int32_t mesh_error;        /* ML_INVALID_MESH: 1 degenerate, 2 inverted, 3 curved Tet10 edge,
                                 4 non-finite coordinate, 5 unknown node, 6 bad material */

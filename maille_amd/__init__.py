@@ -26,7 +26,7 @@ _lib = None
 
 ML_OK, ML_SINGULAR, ML_NOT_CONVERGED, ML_INVALID_ARGUMENT, ML_OUT_OF_PATTERN, ML_HIP_ERROR, \
     ML_INVALID_MESH, ML_DEVICE_TIMEOUT = range(8)
-FP64, MIXED = 0, 1
+FP64, MIXED, MIXED_IC0 = 0, 1, 2
 AUTO, DIRECT, ITERATIVE = 0, 1, 2
 TOLERANCE = 1e-12          # iterative.rs:9
 MAX_ITERATIONS = 10_000    # iterative.rs:11

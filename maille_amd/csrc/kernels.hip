@@ -410,9 +410,10 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
                                               const T* __restrict__ sval,
                                               const T* __restrict__ x, T* __restrict__ y,
                                               double* __restrict__ partial,
-                                              const PcgScalars* __restrict__ S) {
+                                              const PcgScalars* __restrict__ S,
+                                              int only_on_update = 0) {
 #pragma clang fp contract(off)
-  if (S && S->done) return;
+  if (S && (S->done || (only_on_update && !S->update))) return;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   double dotv = 0.0;
   if (j < n) {
@@ -459,6 +460,10 @@ __global__ __launch_bounds__(256) void k_init_norm(const double* __restrict__ pa
     S->it = 0;
     S->status = 0;
     S->done = S->bnorm == 0.0 ? 1 : 0;
+    S->rmax = S->bnorm;
+    S->prev_true = S->bnorm;
+    S->update = 0;
+    S->updates = 0;
   }
 }
 
@@ -492,7 +497,9 @@ __global__ __launch_bounds__(256) void k_converged(const double* __restrict__ pa
   if (threadIdx.x == 0) {
     S->rnorm = sqrt(rr);
     S->it += 1;
-    if (S->rnorm <= S->rscale) {
+    if (S->allow && S->rnorm < S->delta * S->rmax) {
+      S->update = 1;  // decide after the true residual (k_replaced)
+    } else if (S->rnorm <= S->rscale) {
       S->done = 1;
       S->status = 0;
     } else if (S->it >= S->maxit) {
@@ -500,6 +507,52 @@ __global__ __launch_bounds__(256) void k_converged(const double* __restrict__ pa
       S->status = 2;
     }
   }
+}
+
+// After a residual replacement: the true |r| becomes the recursive one; replacements stop
+// once one fails to halve the previous true residual (the FP64 floor is reached).
+__global__ __launch_bounds__(256) void k_replaced(const double* __restrict__ partial, int count,
+                                                  PcgScalars* S) {
+  if (S->done || !S->update) return;
+  double rr = sum_partials(partial, count);
+  if (threadIdx.x == 0) {
+    const double t = sqrt(rr);
+    if (t > 0.5 * S->prev_true) S->allow = 0;
+    S->prev_true = t;
+    S->rmax = t;
+    S->rnorm = t;
+    S->updates += 1;
+    S->update = 0;
+    if (t <= S->rscale) {
+      S->done = 1;
+      S->status = 0;
+    } else if (S->it >= S->maxit) {
+      S->done = 1;
+      S->status = 2;
+    }
+  }
+}
+
+// r = b - A x (the FP64 true residual), its low-precision copy and partial r.r, only when a
+// replacement is due
+template <typename TM>
+__global__ __launch_bounds__(256) void k_replace_residual(int n, const PcgScalars* __restrict__ S,
+                                                          const double* __restrict__ b,
+                                                          const double* __restrict__ ax,
+                                                          double* __restrict__ r,
+                                                          TM* __restrict__ r_low,
+                                                          double* __restrict__ partial) {
+  if (S->done || !S->update) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    const double ri = b[i] - ax[i];
+    r[i] = ri;
+    if (r_low) r_low[i] = (TM)ri;
+    v = ri * ri;
+  }
+  double s = block_sum<256>(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // beta = rz' / rz (iterative.rs:237-239)
@@ -515,13 +568,13 @@ __global__ __launch_bounds__(256) void k_beta(const double* __restrict__ partial
 
 // x += alpha p; r -= alpha Ap (iterative.rs:229-232); partial r.r; fills the two triangular
 // solve outputs with the sentinel for the coming preconditioner application.
-template <typename TV>
+template <typename TV, typename TM>
 __global__ __launch_bounds__(256) void k_update_xr(int n, const PcgScalars* __restrict__ S,
                                                    double* __restrict__ x, double* __restrict__ r,
                                                    const TV* __restrict__ p,
                                                    const TV* __restrict__ ap,
-                                                   TV* __restrict__ r_low, TV* __restrict__ fill_a,
-                                                   TV* __restrict__ fill_b,
+                                                   TM* __restrict__ r_low, TM* __restrict__ fill_a,
+                                                   TM* __restrict__ fill_b,
                                                    double* __restrict__ partial) {
 #pragma clang fp contract(off)
   if (S->done) return;
@@ -532,20 +585,20 @@ __global__ __launch_bounds__(256) void k_update_xr(int n, const PcgScalars* __re
     x[i] += alpha * (double)p[i];
     double ri = r[i] - alpha * (double)ap[i];
     r[i] = ri;
-    if (r_low) r_low[i] = (TV)ri;
+    if (r_low) r_low[i] = (TM)ri;
     v = ri * ri;
-    typedef typename Bits<TV>::U U;
-    reinterpret_cast<U*>(fill_a)[i] = Bits<TV>::S;
-    reinterpret_cast<U*>(fill_b)[i] = Bits<TV>::S;
+    typedef typename Bits<TM>::U U;
+    reinterpret_cast<U*>(fill_a)[i] = Bits<TM>::S;
+    reinterpret_cast<U*>(fill_b)[i] = Bits<TM>::S;
   }
   double s = block_sum<256>(v);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // p = z + beta p (iterative.rs:240-242)
-template <typename TV>
+template <typename TV, typename TM>
 __global__ __launch_bounds__(256) void k_update_p(int n, const PcgScalars* __restrict__ S,
-                                                  TV* __restrict__ p, const TV* __restrict__ z) {
+                                                  TV* __restrict__ p, const TM* __restrict__ z) {
 #pragma clang fp contract(off)
   if (S->done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -587,10 +640,22 @@ __global__ __launch_bounds__(256) void k_axpy_add(int n, double* __restrict__ x,
 //
 // Each wavefront owns blocks of 64 consecutive logical rows, lane t owning row s+t (blocks
 // w, w+G, w+2G... for wave w, so every dependency of the lowest unfinished block is finished).
-// Phase 1 lets every lane consume, in order, the entries whose value is already available;
-// phase 2 resolves the block's rows one after the other: lane k finishes its row (waiting for
-// any late value), publishes it and broadcasts it, and the lanes whose next entry is that row
-// consume it at once. Each row is thus summed sequentially in the reference's order.
+// Phase 1: every lane consumes, in order, its entries outside the block whose values are
+// already published (loads issued eight at a time). The entries inside the block go to a
+// triangular coefficient table in LDS (column k holds rows k+1..63) with a dependency bitmask
+// per lane. Phase 2 resolves the block's rows in order: lane k finishes its row (waiting for
+// any late value), publishes it and broadcasts it; every lane that depends on row s+k and has
+// no earlier entry pending consumes it with the coefficient it prefetched from LDS one step
+// ahead. Each row is thus summed in the reference's order and no in-block step waits on
+// global memory.
+__device__ inline int tri_index(int k, int t) { return k * 63 - (k * (k - 1)) / 2 + (t - k - 1); }
+
+template <typename T, bool SUBSEQ>
+__device__ inline T accumulate(T acc, T a, T v) {
+#pragma clang fp contract(off)
+  return SUBSEQ ? acc - a * v : acc + a * v;
+}
+
 template <typename T, bool REV, bool SUBSEQ>
 __global__ __launch_bounds__(64) void k_trisolve(int n, int nblocks, const int* __restrict__ off,
                                                  const int* __restrict__ col,
@@ -603,6 +668,7 @@ __global__ __launch_bounds__(64) void k_trisolve(int n, int nblocks, const int* 
                                                  int* abort_flag, long long limit) {
 #pragma clang fp contract(off)
   if (S && S->done) return;
+  __shared__ T coef[2016];
   __shared__ T own[64];
   const int lane = threadIdx.x;
   double dotacc = 0.0;
@@ -621,28 +687,59 @@ __global__ __launch_bounds__(64) void k_trisolve(int n, int nblocks, const int* 
       dl = diag[l];
       acc = SUBSEQ ? rl : T(0);
     }
-    // phase 1: entries already available, in order
-    while (p < end) {
-      const int j = col[p];
-      if (j >= s) break;
-      const T v = out[REV ? n - 1 - j : j];
-      if (is_sentinel(v)) break;
-      if (SUBSEQ) acc = acc - val[p] * v;
-      else acc = acc + val[p] * v;
-      ++p;
+    // phase 1: published entries outside the block, in order, eight loads in flight
+    bool stalled = false;
+    while (!stalled && p < end) {
+      int jj[8];
+      T aa[8], vv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        jj[u] = (p + u < end) ? col[p + u] : n;
+        aa[u] = (p + u < end) ? val[p + u] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        vv[u] = jj[u] < s ? out[REV ? n - 1 - jj[u] : jj[u]] : T(0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (!stalled) {
+          if (jj[u] >= s || is_sentinel(vv[u])) {
+            stalled = true;
+          } else {
+            acc = accumulate<T, SUBSEQ>(acc, aa[u], vv[u]);
+            ++p;
+          }
+        }
+      }
     }
-    int ncol = (p < end) ? col[p] : n;
-    T nval = (p < end) ? val[p] : T(0);
+    // entries inside the block: coefficient table and dependency mask
+    int first_int = p;
+    while (first_int < end && col[first_int] < s) ++first_int;
+    const bool ext_done = (p == first_int);
+    unsigned long long mask = 0ull;
+    for (int e = first_int; e < end; ++e) {
+      const int k = col[e] - s;
+      mask |= 1ull << k;
+      coef[tri_index(k, lane)] = val[e];
+    }
+    __syncthreads();
+    T cnext = (mask & 1ull) ? coef[tri_index(0, lane)] : T(0);
+    unsigned long long consumed = 0ull;
     // phase 2: rows of the block in order
     for (int k = 0; k < rows; ++k) {
       T y = T(0);
       if (lane == k) {
-        while (p < end) {
+        while (p < first_int) {
           const int j = col[p];
-          const T v = j >= s ? own[j - s] : await_value(out + (REV ? n - 1 - j : j), abort_flag, limit);
-          if (SUBSEQ) acc = acc - val[p] * v;
-          else acc = acc + val[p] * v;
+          const T v = await_value(out + (REV ? n - 1 - j : j), abort_flag, limit);
+          acc = accumulate<T, SUBSEQ>(acc, val[p], v);
           ++p;
+        }
+        unsigned long long rest = mask & ~consumed;
+        while (rest) {
+          const int j = __builtin_ctzll(rest);
+          rest &= rest - 1;
+          acc = accumulate<T, SUBSEQ>(acc, coef[tri_index(j, lane)], own[j]);
         }
         y = SUBSEQ ? acc / dl : (rl - acc) / dl;
         own[k] = y;
@@ -650,13 +747,11 @@ __global__ __launch_bounds__(64) void k_trisolve(int n, int nblocks, const int* 
         if (dotvec) dotacc += dotvec[pl] * (double)y;
       }
       y = lane_broadcast(y, k);
-      if (lane > k && ncol == s + k) {
-        if (SUBSEQ) acc = acc - nval * y;
-        else acc = acc + nval * y;
-        ++p;
-        ncol = (p < end) ? col[p] : n;
-        nval = (p < end) ? val[p] : T(0);
+      if (ext_done && lane > k && ((mask >> k) & 1ull)) {
+        acc = accumulate<T, SUBSEQ>(acc, cnext, y);
+        consumed |= 1ull << k;
       }
+      if (k + 1 < 64 && ((mask >> (k + 1)) & 1ull)) cnext = coef[tri_index(k + 1, lane)];
     }
     __syncthreads();
   }

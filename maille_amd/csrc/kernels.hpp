@@ -23,7 +23,14 @@ struct PcgScalars {
   int64_t maxit;
   int32_t done;      // 1 once converged or failed: every later kernel returns at once
   int32_t status;    // 0 ok, 1 singular, 2 not converged
-  int32_t pad[2];
+  // reliable updates (FP32 operator): the recursive residual is replaced by the FP64 true
+  // residual b - A x whenever it has dropped by `delta` since the last replacement
+  int32_t update;    // 1 while a replacement is due in the current iteration
+  int32_t allow;     // replacements still allowed (cleared once they stop reducing |r|)
+  double delta;
+  double rmax;       // |r| at the last replacement
+  double prev_true;  // true |r| at the last replacement
+  int64_t updates;   // replacements done
 };
 
 }  // namespace mlk

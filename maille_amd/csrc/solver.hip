@@ -2,7 +2,7 @@
 //
 //   Operator : K_ff on the device (reference lower CSR, sliced-ELL copy for the SpMV, forward
 //              and transposed layouts of the IC(0) factor) + the PCG driver (FP64 drop-in, or
-//              FP32 operator inside FP64 iterative refinement).
+//              FP32 operator and/or FP32 preconditioner with FP64 residual replacement).
 //   Model    : mesh -> host plan (pattern.cpp, plan.cpp) -> element kernels and gather
 //              assembly on the device -> Operator -> reactions and stresses.
 #include <hip/hip_runtime.h>
@@ -270,7 +270,7 @@ struct Operator {
     d_flags.alloc(2);
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 2, 0);
     HIP_CHECK(hipGetLastError());
-    if (precision == ML_MIXED) alloc_mixed();
+    if (precision != ML_FP64) alloc_mixed();
     factored = false;
     // the host layouts above are temporaries: wait for their uploads before they go away
     HIP_CHECK(hipStreamSynchronize(stream));
@@ -292,7 +292,7 @@ struct Operator {
     mlk::k_gather<double, double><<<grid_for(d_scol.n), 256, 0, stream>>>(
         (int64_t)d_scol.n, d_ssrc.p, d_a.p, d_sval.p);
     HIP_CHECK(hipGetLastError());
-    if (precision == ML_MIXED) {
+    if (precision != ML_FP64) {
       mlk::k_convert<double, float><<<grid_for(nnz), 256, 0, stream>>>(nnz, d_a.p, d_a32.p);
       mlk::k_gather<double, float><<<grid_for(d_scol.n), 256, 0, stream>>>(
           (int64_t)d_scol.n, d_ssrc.p, d_a.p, d_sval32.p);
@@ -342,7 +342,7 @@ struct Operator {
     }
     if (st != ML_OK) return ML_SINGULAR;
     gather_factor<double>(d_l.p, d_fval.p, d_bval.p, d_fdiag.p, d_bdiag.p);
-    if (precision == ML_MIXED) {
+    if (precision != ML_FP64) {
       int st32 = ML_SINGULAR;
       for (int s = 0; s < 6; ++s) {
         st32 = run_ic0<float>(d_a32.p, d_l32.p, (float)SHIFTS[s]);
@@ -366,20 +366,6 @@ struct Operator {
   }
 
   // ---------------------------------------------------------------- kernels
-  void spmv64(const double* x, double* y, double* partial, const mlk::PcgScalars* S) {
-    hipEvent_t ev = prof.begin(stream);
-    mlk::k_spmv<double><<<grid_for(n), 256, 0, stream>>>(n, d_soff.p, d_scol.p, d_slen.p,
-                                                         d_sval.p, x, y, partial, S);
-    HIP_CHECK(hipGetLastError());
-    prof.end(P_SPMV64, ev, stream);
-  }
-  void spmv32(const float* x, float* y, double* partial, const mlk::PcgScalars* S) {
-    hipEvent_t ev = prof.begin(stream);
-    mlk::k_spmv<float><<<grid_for(n), 256, 0, stream>>>(n, d_soff.p, d_scol.p, d_slen.p,
-                                                        d_sval32.p, x, y, partial, S);
-    HIP_CHECK(hipGetLastError());
-    prof.end(P_SPMV32, ev, stream);
-  }
   // z = (L L^T)^-1 r through y; partial gets r.z per wave when dotvec is given
   template <typename T>
   void precondition(const T* r, T* y, T* z, const double* dotvec, double* partial,
@@ -409,10 +395,12 @@ struct Operator {
     HIP_CHECK(hipGetLastError());
   }
 
-  void init_scalars(mlk::PcgScalars* S, int64_t maxit) {
+  void init_scalars(mlk::PcgScalars* S, int64_t maxit, bool reliable) {
     mlk::PcgScalars h;
     std::memset(&h, 0, sizeof(h));
     h.maxit = maxit;
+    h.allow = reliable ? 1 : 0;
+    h.delta = 0.1;
     // blocking copy from the stack: the stream must not hold kernels still reading S
     HIP_CHECK(hipStreamSynchronize(stream));
     HIP_CHECK(hipMemcpy(S, &h, sizeof(h), hipMemcpyHostToDevice));
@@ -425,52 +413,71 @@ struct Operator {
     return h;
   }
 
-  // Preconditioned CG (iterative.rs:210-248) on the device. T: arithmetic of the operator and
-  // of p, Ap, z (double for the drop-in, float inside refinement). x and r are FP64. b, x and
-  // r are device vectors in solver numbering; returns status and fills S.
-  template <typename T>
+  template <typename TA>
+  void spmv(const TA* x, TA* y, double* partial, const mlk::PcgScalars* S, int gated = 0) {
+    const bool f32 = std::is_same<TA, float>::value;
+    hipEvent_t ev = prof.begin(stream);
+    mlk::k_spmv<TA><<<grid_for(n), 256, 0, stream>>>(
+        n, d_soff.p, d_scol.p, d_slen.p, f32 ? (const TA*)d_sval32.p : (const TA*)d_sval.p, x, y,
+        partial, S, gated);
+    HIP_CHECK(hipGetLastError());
+    prof.end(f32 ? P_SPMV32 : P_SPMV64, ev, stream);
+  }
+
+  // Preconditioned CG (iterative.rs:210-248) on the device. TA: arithmetic of the operator and
+  // of p and Ap; TM: arithmetic of the IC(0) factor and z. x, r and every dot product are
+  // FP64. With an FP32 operator the recursive residual is replaced by the FP64 true residual
+  // b - A x each time it has dropped tenfold (reliable updates), until replacements stop
+  // helping; convergence is the reference's |r| <= tol |b| on the residual in hand. b, x, r
+  // are device vectors in solver numbering; returns a status and fills *hS.
+  template <typename TA, typename TM>
   int pcg(const double* b, double* x, double* r, double tol, int64_t maxit,
           mlk::PcgScalars* S, mlk::PcgScalars* hS) {
-    const bool f32 = std::is_same<T, float>::value;
-    T* p = f32 ? (T*)w_p32.p : (T*)w_p.p;
-    T* ap = f32 ? (T*)w_ap32.p : (T*)w_ap.p;
-    T* y = f32 ? (T*)w_y32.p : (T*)w_y.p;
-    T* z = f32 ? (T*)w_z32.p : (T*)w_z.p;
-    T* rlow = f32 ? (T*)w_r32.p : nullptr;
-    const T* rin = f32 ? (const T*)w_r32.p : (const T*)r;
+    const bool a32 = std::is_same<TA, float>::value;
+    const bool m32 = std::is_same<TM, float>::value;
+    TA* p = a32 ? (TA*)w_p32.p : (TA*)w_p.p;
+    TA* ap = a32 ? (TA*)w_ap32.p : (TA*)w_ap.p;
+    TM* y = m32 ? (TM*)w_y32.p : (TM*)w_y.p;
+    TM* z = m32 ? (TM*)w_z32.p : (TM*)w_z.p;
+    TM* rlow = m32 ? (TM*)w_r32.p : nullptr;
+    const TM* rin = m32 ? (const TM*)w_r32.p : (const TM*)r;
     const int vb = grid_for(n);
+    const int tri_parts = m32 ? tri_grid32 : tri_grid64;
     double* part = w_partial.p;
-    init_scalars(S, maxit);
+    init_scalars(S, maxit, a32);
     mlk::k_fill_f64<<<vb, 256, 0, stream>>>(x, n, 0.0);
     HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
     mlk::k_dot<double, double><<<vb, 256, 0, stream>>>(n, b, b, part);
     mlk::k_init_norm<<<1, 256, 0, stream>>>(part, vb, S, tol);
     HIP_CHECK(hipGetLastError());
-    if (f32) mlk::k_convert<double, float><<<vb, 256, 0, stream>>>(n, r, (float*)rlow);
-    fill_sentinel<T>(y);
-    fill_sentinel<T>(z);
-    precondition<T>(rin, y, z, r, part, S);
-    mlk::k_init_rz<<<1, 256, 0, stream>>>(part, f32 ? tri_grid32 : tri_grid64, S);
-    HIP_CHECK(hipMemcpyAsync(p, z, sizeof(T) * n, hipMemcpyDeviceToDevice, stream));
+    if (m32) mlk::k_convert<double, float><<<vb, 256, 0, stream>>>(n, r, (float*)rlow);
+    fill_sentinel<TM>(y);
+    fill_sentinel<TM>(z);
+    precondition<TM>(rin, y, z, r, part, S);
+    mlk::k_init_rz<<<1, 256, 0, stream>>>(part, tri_parts, S);
+    mlk::k_convert<TM, TA><<<vb, 256, 0, stream>>>(n, z, p);
     HIP_CHECK(hipGetLastError());
     *hS = read_scalars(S);
     if (read_flag(0)) return ML_DEVICE_TIMEOUT;
     if (hS->done) return ML_OK;  // b == 0: x = 0 exactly
     int64_t launched = 0, batch = 4;
-    const int tri_parts = f32 ? tri_grid32 : tri_grid64;
     for (;;) {
       for (int64_t k = 0; k < batch; ++k) {
-        if (f32) spmv32((const float*)p, (float*)ap, part, S);
-        else spmv64((const double*)p, (double*)ap, part, S);
+        spmv<TA>(p, ap, part, S);
         mlk::k_alpha<<<1, 256, 0, stream>>>(part, vb, S);
         hipEvent_t ev = prof.begin(stream);
-        mlk::k_update_xr<T><<<vb, 256, 0, stream>>>(n, S, x, r, p, ap, rlow, y, z, part);
+        mlk::k_update_xr<TA, TM><<<vb, 256, 0, stream>>>(n, S, x, r, p, ap, rlow, y, z, part);
         prof.end(P_VEC, ev, stream);
         mlk::k_converged<<<1, 256, 0, stream>>>(part, vb, S);
-        precondition<T>(rin, y, z, r, part, S);
+        if (a32) {
+          spmv<double>(x, w_ax.p, nullptr, S, 1);
+          mlk::k_replace_residual<TM><<<vb, 256, 0, stream>>>(n, S, b, w_ax.p, r, rlow, part);
+          mlk::k_replaced<<<1, 256, 0, stream>>>(part, vb, S);
+        }
+        precondition<TM>(rin, y, z, r, part, S);
         mlk::k_beta<<<1, 256, 0, stream>>>(part, tri_parts, S);
         ev = prof.begin(stream);
-        mlk::k_update_p<T><<<vb, 256, 0, stream>>>(n, S, p, z);
+        mlk::k_update_p<TA, TM><<<vb, 256, 0, stream>>>(n, S, p, z);
         prof.end(P_VEC, ev, stream);
         HIP_CHECK(hipGetLastError());
       }
@@ -489,7 +496,7 @@ struct Operator {
   // |b - A x| / |b| in FP64 (uses w_ax, w_d as scratch)
   double true_residual(const double* b, const double* x, double bnorm) {
     const int vb = grid_for(n);
-    spmv64(x, w_ax.p, nullptr, nullptr);
+    spmv<double>(x, w_ax.p, nullptr, nullptr);
     mlk::k_residual<<<vb, 256, 0, stream>>>(n, b, w_ax.p, w_d.p, w_partial.p);
     mlk::k_init_norm<<<1, 256, 0, stream>>>(w_partial.p, vb, d_Sin.p, 1.0);
     HIP_CHECK(hipGetLastError());
@@ -507,66 +514,18 @@ struct Operator {
     if (maxit <= 0) maxit = ML_MAX_ITERATIONS;
     mlk::PcgScalars hS;
     int st;
-    int64_t iterations = 0;
-    int refinements = 0;
-    double residual = 0, bnorm = 0;
-    if (precision == ML_FP64) {
-      st = pcg<double>(b, x, w_r.p, tol, maxit, d_S.p, &hS);
-      iterations = hS.it;
-      bnorm = hS.bnorm;
-      residual = bnorm > 0 ? hS.rnorm / bnorm : 0.0;
-    } else {
-      st = refine(b, x, tol, maxit, &iterations, &refinements, &bnorm, &residual);
-    }
+    if (precision == ML_FP64) st = pcg<double, double>(b, x, w_r.p, tol, maxit, d_S.p, &hS);
+    else if (precision == ML_MIXED) st = pcg<float, float>(b, x, w_r.p, tol, maxit, d_S.p, &hS);
+    else st = pcg<double, float>(b, x, w_r.p, tol, maxit, d_S.p, &hS);
     if (info) {
-      info->iterations = iterations;
-      info->residual = residual;
+      info->iterations = hS.it;
+      info->residual = hS.bnorm > 0 ? hS.rnorm / hS.bnorm : 0.0;
       info->shift_index = shift_index;
-      info->refinements = refinements;
-      info->true_residual = (st == ML_OK && bnorm > 0) ? true_residual(b, x, bnorm) : residual;
+      info->refinements = (int32_t)hS.updates;
+      info->true_residual = (st == ML_OK && hS.bnorm > 0) ? true_residual(b, x, hS.bnorm)
+                                                          : info->residual;
     }
     return st;
-  }
-
-  // FP64 iterative refinement around the FP32 PCG: x += PCG32(b - A x) until |b - A x| <=
-  // tol |b|. Inner solves stop at a relative residual of 1e-6 (or what is left to gain).
-  int refine(const double* b, double* x, double tol, int64_t maxit, int64_t* iterations,
-             int* sweeps, double* bnorm_out, double* residual) {
-    const int vb = grid_for(n);
-    mlk::k_fill_f64<<<vb, 256, 0, stream>>>(x, n, 0.0);
-    HIP_CHECK(hipMemcpyAsync(w_b.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
-    mlk::k_dot<double, double><<<vb, 256, 0, stream>>>(n, b, b, w_partial.p);
-    mlk::k_init_norm<<<1, 256, 0, stream>>>(w_partial.p, vb, d_S.p, tol);
-    HIP_CHECK(hipGetLastError());
-    mlk::PcgScalars out = read_scalars(d_S.p);
-    const double bnorm = out.bnorm;
-    *bnorm_out = bnorm;
-    *iterations = 0;
-    *sweeps = 0;
-    *residual = 0;
-    if (bnorm == 0.0) return ML_OK;
-    double rnorm = bnorm;
-    // residual r lives in w_b (starts as b); the correction in w_d's partner w_x? use w_ax
-    const double inner_tol = 1e-6;
-    for (int sweep = 0; sweep < 60; ++sweep) {
-      if (rnorm <= tol * bnorm) break;
-      double itol = std::max(inner_tol, 0.5 * tol * bnorm / rnorm);
-      mlk::PcgScalars hin;
-      int st = pcg<float>(w_b.p, w_d.p, w_r.p, itol, maxit - *iterations, d_Sin.p, &hin);
-      *iterations += hin.it;
-      *sweeps += 1;
-      if (st == ML_DEVICE_TIMEOUT || st == ML_SINGULAR) return st;
-      mlk::k_axpy_add<<<vb, 256, 0, stream>>>(n, x, w_d.p);
-      spmv64(x, w_ax.p, nullptr, nullptr);
-      mlk::k_residual<<<vb, 256, 0, stream>>>(n, b, w_ax.p, w_b.p, w_partial.p);
-      mlk::k_init_norm<<<1, 256, 0, stream>>>(w_partial.p, vb, d_S.p, tol);
-      HIP_CHECK(hipGetLastError());
-      out = read_scalars(d_S.p);
-      rnorm = out.bnorm;
-      *residual = rnorm / bnorm;
-      if (*iterations >= maxit && rnorm > tol * bnorm) return ML_NOT_CONVERGED;
-    }
-    return rnorm <= tol * bnorm ? ML_OK : ML_NOT_CONVERGED;
   }
 };
 
@@ -964,7 +923,7 @@ ml_status ml_pcg_create(int64_t n, const int64_t* offsets, const int64_t* cols,
                         const double* values, ml_precision precision, ml_pcg** out) {
   return guarded([&]() -> int {
     if (!out || !values) throw ArgError("null argument");
-    if (precision != ML_FP64 && precision != ML_MIXED) throw ArgError("bad precision");
+    if (precision < ML_FP64 || precision > ML_MIXED_IC0) throw ArgError("bad precision");
     LowerPattern lp = pattern_from_csr(n, offsets, cols);
     std::unique_ptr<ml_pcg> h(new ml_pcg);
     h->op.precision = precision;
@@ -1003,7 +962,6 @@ ml_status ml_pcg_solve(ml_pcg* h, const double* b, double* x, int64_t nrhs, doub
                                op.stream));
       ml_solve_info one;
       std::memset(&one, 0, sizeof(one));
-      // w_b is clobbered by refinement, so solve from a copy in w_ax's partner
       DBuf<double> bcopy;
       bcopy.alloc(n);
       HIP_CHECK(hipMemcpyAsync(bcopy.p, op.w_b.p, sizeof(double) * n, hipMemcpyDeviceToDevice,
@@ -1027,7 +985,7 @@ ml_status ml_pcg_multiply(ml_pcg* h, const double* x, double* y) {
     if (!h || !x || !y) throw ArgError("null argument");
     Operator& op = h->op;
     HIP_CHECK(hipMemcpyAsync(op.w_p.p, x, sizeof(double) * op.n, hipMemcpyHostToDevice, op.stream));
-    op.spmv64(op.w_p.p, op.w_ap.p, nullptr, nullptr);
+    op.spmv<double>(op.w_p.p, op.w_ap.p, nullptr, nullptr);
     HIP_CHECK(hipMemcpyAsync(y, op.w_ap.p, sizeof(double) * op.n, hipMemcpyDeviceToHost, op.stream));
     HIP_CHECK(hipStreamSynchronize(op.stream));
     return ML_OK;
@@ -1081,7 +1039,7 @@ ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elemen
     if (!out || !nodes || !conn || !young || !nu || !fixed) throw ArgError("null argument");
     if (solver == ML_DIRECT)
       throw ArgError("the direct (skyline) solver is not part of the device library");
-    if (precision != ML_FP64 && precision != ML_MIXED) throw ArgError("bad precision");
+    if (precision < ML_FP64 || precision > ML_MIXED_IC0) throw ArgError("bad precision");
     if (info) {
       std::memset(info, 0, sizeof(*info));
       info->bad_element = -1;

@@ -95,15 +95,19 @@ def test_fp64_solve_matches_oracle_pcg(gpu, name):
     assert np.array_equal(st[..., :3], ref["stresses"][..., :3])
 
 
+@pytest.mark.parametrize("precision", ["MIXED", "MIXED_IC0"])
 @pytest.mark.parametrize("name", list(PROBLEMS))
-def test_mixed_solve_matches_oracle(gpu, name):
+def test_mixed_solve_matches_oracle(gpu, name, precision):
+    """FP32 operator and/or preconditioner with FP64 iterate, residual and dots: within
+    1e-8 relative L2 of the oracle's FP64 PCG (the north-star bound for mixed precision)."""
     p = PROBLEMS[name]
-    print(f"mixed solve {name}", flush=True)
+    print(f"{precision} solve {name}", flush=True)
     ref = oracle.model_solve(p, iterative=True)
-    m = gpu.Model(p, precision=gpu.MIXED)
+    m = gpu.Model(p, precision=getattr(gpu, precision))
     u, r, info = m.solve()
+    print(f"  iterations {info['iterations']} (oracle {ref['iterations']}), replacements "
+          f"{info['refinements']}, true residual {info['true_residual']:.1e}", flush=True)
     assert rel_l2(u, ref["displacements"]) <= 1e-8
-    assert info["refinements"] >= 1
 
 
 def test_iterative_matches_direct_on_reference_cantilever(gpu):
