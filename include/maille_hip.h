@@ -126,6 +126,13 @@ ml_status ml_pcg_precondition(ml_pcg* h, const double* r, double* z);
 /* The IC(0) factor values aligned with the CSR (Pcg::l, iterative.rs:196) and the shift. */
 ml_status ml_pcg_factor(ml_pcg* h, double* l, int32_t* shift_index);
 void ml_pcg_destroy(ml_pcg* h);
+/* Diagnostics (no counterpart in the reference): one preconditioner application with a
+ * per-block timeline of both triangular solves. nblocks[2] receives the block counts; with
+ * non-null traces (capacity int64 each, >= 4 per block) each block records its start, the end
+ * of its staging, its end (device wall-clock ticks) and (wave << 8 | lanes still waiting on an
+ * external value after staging). */
+ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* trace_fw,
+                       int64_t* trace_bw, int64_t capacity);
 
 /* ------------------------------------------------------------------ the static model */
 

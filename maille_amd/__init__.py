@@ -165,6 +165,8 @@ def _check(status, info=None):
     if status == ML_INVALID_MESH and info is not None:
         raise InvalidMesh(info.mesh_error, info.bad_element)
     msg = lib().ml_last_error().decode(errors="replace")
+    if status == ML_DEVICE_TIMEOUT:
+        raise DeviceError(f"device wait expired: {msg}")
     if status in (ML_INVALID_ARGUMENT, ML_OUT_OF_PATTERN):
         raise ValueError(msg or f"status {status}")
     raise DeviceError(f"status {status}: {msg}")

@@ -37,22 +37,19 @@ __device__ inline void publish(T* p, T v) {
                      __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Value of a slot written once per launch by publish(): a plain load answers when the slot is
-// already written (a slot only ever holds the sentinel or its final value); otherwise poll
-// with device-scope loads. The wait is bounded: past `limit` ticks of the 100 MHz real-time
-// counter (or once another wave has given up) it raises *abort and returns 0, so every
-// wavefront always reaches the end of the kernel.
+// Value of a slot written once per launch by publish(), polled with device-scope loads (a
+// plain load may hit a stale line of this XCD's L2 that still holds the sentinel). The wait
+// is bounded: past `limit` ticks of the device's steady wall clock (or once another wave has
+// given up) it raises *abort and returns 0, so every wavefront reaches the end of the kernel.
 template <typename T>
 __device__ inline T await_value(T* p, int* abort_flag, long long limit) {
   typedef typename Bits<T>::U U;
-  T v = *p;
-  if (!is_sentinel(v)) return v;
-  long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
+  long long t0 = wall_clock64();
   for (;;) {
     U u = __hip_atomic_load(reinterpret_cast<U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (u != Bits<T>::S) return from_bits<T>(u);
     if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return T(0);
-    if ((long long)__builtin_amdgcn_s_memrealtime() - t0 > limit) {
+    if (wall_clock64() - t0 > limit) {
       __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return T(0);
     }
@@ -628,130 +625,188 @@ __global__ __launch_bounds__(256) void k_axpy_add(int n, double* __restrict__ x,
 }
 
 // ------------------------------------------------------------------ triangular solves
-// Dataflow solve of a lower-triangular system in "logical" row order: logical row l has its
-// strictly-lower entries [off[l], off[l+1]) with logical columns col[] ascending and the
-// diagonal diag[l]. Vectors are indexed physically: phys(l) = REV ? n-1-l : l, so the backward
-// solve L^T z = y runs through the same kernel on the transposed layout.
+// Dataflow solve of a lower-triangular system in "logical" row order. Vectors are indexed
+// physically, phys(l) = REV ? n-1-l : l, so the backward solve L^T z = y runs through the same
+// kernel on the transposed layout.
 //
 //   SUBSEQ = false: out_l = (rhs_l - sum_p val_p out_col) / diag_l  (forward loop,
 //                   iterative.rs:175-179: sum in ascending column order, then subtract)
 //   SUBSEQ = true:  out_l = ((rhs_l - val_1 out_1) - val_2 out_2 ...) / diag_l (backward loop,
 //                   iterative.rs:180-186: subtractions in decreasing row order)
 //
-// Each wavefront owns blocks of 64 consecutive logical rows, lane t owning row s+t (blocks
-// w, w+G, w+2G... for wave w, so every dependency of the lowest unfinished block is finished).
-// Phase 1: every lane consumes, in order, its entries outside the block whose values are
-// already published (loads issued eight at a time). The entries inside the block go to a
-// triangular coefficient table in LDS (column k holds rows k+1..63) with a dependency bitmask
-// per lane. Phase 2 resolves the block's rows in order: lane k finishes its row (waiting for
-// any late value), publishes it and broadcasts it; every lane that depends on row s+k and has
-// no earlier entry pending consumes it with the coefficient it prefetched from LDS one step
-// ahead. Each row is thus summed in the reference's order and no in-block step waits on
-// global memory.
-__device__ inline int tri_index(int k, int t) { return k * 63 - (k * (k - 1)) / 2 + (t - k - 1); }
-
+// Rows are cut on the host into blocks of at most 64 consecutive rows that never straddle a
+// break of the dependency chain (bstart[]); each row's entries are split into those whose
+// column lies before its block (xcol/xval, "external") and those inside it (ik = column minus
+// block start, ival), both stored block after block. Wave w owns blocks w, w+G, ... (the
+// launch is cooperative, so every block of a lower index is finished or in progress).
+// Per block: the wave stages the block's entries and the external values already published
+// into LDS with coalesced loads; every lane consumes, in order, the external values that were
+// ready. Then the rows are resolved one after the other: lane k finishes its row (polling any
+// late external value, eight device-scope loads per round), publishes and broadcasts it, and
+// each lane whose next entry is that row consumes it at once. Every row is summed in the
+// reference's order, and no in-block step waits on global memory.
 template <typename T, bool SUBSEQ>
 __device__ inline T accumulate(T acc, T a, T v) {
 #pragma clang fp contract(off)
   return SUBSEQ ? acc - a * v : acc + a * v;
 }
 
+constexpr int TRI_XCAP = 768;  // external entries a block may hold (host-enforced)
+constexpr int TRI_ICAP = 512;  // internal entries a block may hold (host-enforced)
+
 template <typename T, bool REV, bool SUBSEQ>
-__global__ __launch_bounds__(64) void k_trisolve(int n, int nblocks, const int* __restrict__ off,
-                                                 const int* __restrict__ col,
-                                                 const T* __restrict__ val,
-                                                 const T* __restrict__ diag,
-                                                 const T* __restrict__ rhs, T* out,
-                                                 const double* __restrict__ dotvec,
-                                                 double* __restrict__ partial,
-                                                 const PcgScalars* __restrict__ S,
-                                                 int* abort_flag, long long limit) {
+__global__ __launch_bounds__(64) void k_trisolve(
+    int n, int nblocks, const int* __restrict__ border, const int* __restrict__ bstart,
+    const int* __restrict__ xoff, const int* __restrict__ xcol, const T* __restrict__ xval,
+    const int* __restrict__ ioff, const unsigned char* __restrict__ ik,
+    const T* __restrict__ ival, const T* __restrict__ diag, const T* __restrict__ rhs, T* out,
+    const double* __restrict__ dotvec, double* __restrict__ partial,
+    const PcgScalars* __restrict__ S, int* abort_flag, long long limit,
+    long long* __restrict__ trace) {
 #pragma clang fp contract(off)
   if (S && S->done) return;
-  __shared__ T coef[2016];
+  typedef typename Bits<T>::U U;
+  __shared__ T sxv[TRI_XCAP];
+  __shared__ T sxa[TRI_XCAP];
+  __shared__ int sxc[TRI_XCAP];
+  __shared__ T sia[TRI_ICAP];
+  __shared__ unsigned char sik[TRI_ICAP];
   __shared__ T own[64];
   const int lane = threadIdx.x;
   double dotacc = 0.0;
-  for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
-    const int s = b * 64;
-    const int rows = min(64, n - s);
+  for (int kk = blockIdx.x; kk < nblocks; kk += gridDim.x) {
+    const int b = border[kk];
+    const int s = bstart[b];
+    const int e = bstart[b + 1];
+    const int rows = e - s;
+    const int xb = xoff[s], nx = xoff[e] - xb;
+    const int ib = ioff[s], ni = ioff[e] - ib;
+    long long tb = trace ? wall_clock64() : 0;
+    // stage the block: external columns, coefficients and the values published so far, read
+    // at device scope (a plain load could hit a stale line of this XCD's L2 that still holds
+    // the sentinel)
+#pragma unroll 4
+    for (int q = lane; q < nx; q += 64) {
+      const int c = xcol[xb + q];
+      sxc[q] = c;
+      sxa[q] = xval[xb + q];
+      sxv[q] = from_bits<T>(__hip_atomic_load(reinterpret_cast<U*>(out + (REV ? n - 1 - c : c)),
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+#pragma unroll 4
+    for (int q = lane; q < ni; q += 64) {
+      sik[q] = ik[ib + q];
+      sia[q] = ival[ib + q];
+    }
     const bool active = lane < rows;
     const int l = s + lane;
     const int pl = REV ? n - 1 - l : l;
-    int p = 0, end = 0;
+    int p = 0, x1 = 0, ip = 0, i1 = 0;
     T acc = T(0), rl = T(0), dl = T(1);
+    double dv = 0.0;
     if (active) {
-      p = off[l];
-      end = off[l + 1];
+      p = xoff[l] - xb;
+      x1 = xoff[l + 1] - xb;
+      ip = ioff[l] - ib;
+      i1 = ioff[l + 1] - ib;
       rl = rhs[pl];
       dl = diag[l];
+      if (dotvec) dv = dotvec[pl];
       acc = SUBSEQ ? rl : T(0);
     }
-    // phase 1: published entries outside the block, in order, eight loads in flight
-    bool stalled = false;
-    while (!stalled && p < end) {
-      int jj[8];
-      T aa[8], vv[8];
+    __syncthreads();
+    // external values that were ready, in order
+    while (p < x1 && !is_sentinel(sxv[p])) {
+      acc = accumulate<T, SUBSEQ>(acc, sxa[p], sxv[p]);
+      ++p;
+    }
+    bool ext_done = p == x1;
+    long long tp = 0;
+    int late = 0;
+    if (trace) {
+      tp = wall_clock64();
+      late = __popcll(__ballot(active && !ext_done));
+    }
+    int nk = ip < i1 ? (int)sik[ip] : 64;
+    T nv = ip < i1 ? sia[ip] : T(0);
+    long long t0 = -1;
+    for (int k = 0; k < rows; ++k) {
+      // Row k may not start before its external values are in: the whole wave polls, each
+      // lane with pending entries loading its next eight, so one round trip serves every row
+      // of the block. A lane whose external part completes catches up, in order, on the
+      // internal entries broadcast while it was waiting.
+      while (__builtin_amdgcn_readlane((int)(p < x1), k)) {
+        if (p < x1) {
+          int jj[8];
+          U vv[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        jj[u] = (p + u < end) ? col[p + u] : n;
-        aa[u] = (p + u < end) ? val[p + u] : T(0);
-      }
+          for (int u = 0; u < 8; ++u) jj[u] = (p + u < x1) ? sxc[p + u] : -1;
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        vv[u] = jj[u] < s ? out[REV ? n - 1 - jj[u] : jj[u]] : T(0);
+          for (int u = 0; u < 8; ++u)
+            vv[u] = jj[u] >= 0 ? __hip_atomic_load(reinterpret_cast<U*>(out + (REV ? n - 1 - jj[u] : jj[u])),
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : (U)0;
+          bool waiting = false;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (!stalled) {
-          if (jj[u] >= s || is_sentinel(vv[u])) {
-            stalled = true;
+          for (int u = 0; u < 8; ++u) {
+            if (!waiting && jj[u] >= 0) {
+              if (vv[u] == Bits<T>::S) {
+                waiting = true;
+              } else {
+                acc = accumulate<T, SUBSEQ>(acc, sxa[p], from_bits<T>(vv[u]));
+                ++p;
+              }
+            }
+          }
+          if (!ext_done && p == x1) {
+            ext_done = true;
+            while (ip < i1 && (int)sik[ip] < k) {
+              acc = accumulate<T, SUBSEQ>(acc, sia[ip], own[sik[ip]]);
+              ++ip;
+            }
+            nk = ip < i1 ? (int)sik[ip] : 64;
+            nv = ip < i1 ? sia[ip] : T(0);
+          }
+        }
+        if (__builtin_amdgcn_readlane((int)(p < x1), k)) {
+          const long long now = wall_clock64();
+          if (t0 < 0) t0 = now;
+          if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+              now - t0 > limit) {
+            // give up: the host reports ML_DEVICE_TIMEOUT; every lane stops waiting
+            __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (!ext_done) {
+              p = x1;
+              ext_done = true;
+              while (ip < i1 && (int)sik[ip] < k) ++ip;
+              nk = ip < i1 ? (int)sik[ip] : 64;
+              nv = ip < i1 ? sia[ip] : T(0);
+            }
           } else {
-            acc = accumulate<T, SUBSEQ>(acc, aa[u], vv[u]);
-            ++p;
+            __builtin_amdgcn_s_sleep(1);
           }
         }
       }
-    }
-    // entries inside the block: coefficient table and dependency mask
-    int first_int = p;
-    while (first_int < end && col[first_int] < s) ++first_int;
-    const bool ext_done = (p == first_int);
-    unsigned long long mask = 0ull;
-    for (int e = first_int; e < end; ++e) {
-      const int k = col[e] - s;
-      mask |= 1ull << k;
-      coef[tri_index(k, lane)] = val[e];
-    }
-    __syncthreads();
-    T cnext = (mask & 1ull) ? coef[tri_index(0, lane)] : T(0);
-    unsigned long long consumed = 0ull;
-    // phase 2: rows of the block in order
-    for (int k = 0; k < rows; ++k) {
       T y = T(0);
       if (lane == k) {
-        while (p < first_int) {
-          const int j = col[p];
-          const T v = await_value(out + (REV ? n - 1 - j : j), abort_flag, limit);
-          acc = accumulate<T, SUBSEQ>(acc, val[p], v);
-          ++p;
-        }
-        unsigned long long rest = mask & ~consumed;
-        while (rest) {
-          const int j = __builtin_ctzll(rest);
-          rest &= rest - 1;
-          acc = accumulate<T, SUBSEQ>(acc, coef[tri_index(j, lane)], own[j]);
-        }
         y = SUBSEQ ? acc / dl : (rl - acc) / dl;
         own[k] = y;
         publish(out + pl, y);
-        if (dotvec) dotacc += dotvec[pl] * (double)y;
+        dotacc += dv * (double)y;
       }
       y = lane_broadcast(y, k);
-      if (ext_done && lane > k && ((mask >> k) & 1ull)) {
-        acc = accumulate<T, SUBSEQ>(acc, cnext, y);
-        consumed |= 1ull << k;
+      if (ext_done && lane > k && nk == k) {
+        acc = accumulate<T, SUBSEQ>(acc, nv, y);
+        ++ip;
+        nk = ip < i1 ? (int)sik[ip] : 64;
+        nv = ip < i1 ? sia[ip] : T(0);
       }
-      if (k + 1 < 64 && ((mask >> (k + 1)) & 1ull)) cnext = coef[tri_index(k + 1, lane)];
+    }
+    if (trace && lane == 0) {
+      trace[4 * (long long)b + 0] = tb;
+      trace[4 * (long long)b + 1] = tp;
+      trace[4 * (long long)b + 2] = wall_clock64();
+      trace[4 * (long long)b + 3] = ((long long)blockIdx.x << 8) | late;
     }
     __syncthreads();
   }
@@ -773,7 +828,8 @@ template <typename T, int MAXROW>
 __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
                                             const int* __restrict__ col,
                                             const T* __restrict__ a, T* l, T shift,
-                                            int* fail, int* abort_flag, long long limit) {
+                                            int* fail, int* abort_flag, long long limit,
+                                            int* where) {
 #pragma clang fp contract(off)
   __shared__ int scol[MAXROW];
   __shared__ T sw[MAXROW];
@@ -820,6 +876,13 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
         }
       }
       const T ljj = await_value(l + jd, abort_flag, limit);
+      if (lane == 0 && __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
+          __hip_atomic_load(where, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) {
+        __hip_atomic_store(where, i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(where + 1, j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(where + 2, (int)blockIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(where + 3, (int)gridDim.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
       const T lij = (a[start + q] - sacc) / ljj;
       if (lane == 0) sw[q] = lij;
       __syncthreads();

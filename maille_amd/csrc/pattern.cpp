@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include <thread>
 
 namespace ml {
@@ -279,6 +280,96 @@ i64 dag_depth(const LowerPattern& lp) {
     depth = std::max<i64>(depth, m + 1);
   }
   return depth;
+}
+
+std::vector<int32_t> dataflow_blocks(int64_t n, const int32_t* off, const int32_t* col,
+                                     int max_rows) {
+  std::vector<int32_t> level(n, 0);
+  std::vector<int32_t> starts;
+  starts.reserve(n / 16 + 2);
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t m = 0;
+    for (int64_t p = off[i]; p < off[i + 1]; ++p) m = std::max(m, level[col[p]] + 1);
+    level[i] = m;
+    if (i == 0 || i - starts.back() >= max_rows || level[i] <= level[i - 1])
+      starts.push_back((int32_t)i);
+  }
+  starts.push_back((int32_t)n);
+  return starts;
+}
+
+TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const int32_t* src,
+                     const int32_t* dsrc, int max_rows, int xcap, int icap, std::string* err) {
+  TriLayout t;
+  std::vector<int32_t> level(n, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t m = 0;
+    for (int64_t p = off[i]; p < off[i + 1]; ++p) m = std::max(m, level[col[p]] + 1);
+    level[i] = m;
+  }
+  t.bstart.push_back(0);
+  int64_t bx = 0, bi = 0;  // external / internal entries of the open block
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t len = off[i + 1] - off[i];
+    if (len > xcap) {
+      *err = "matrix row longer than the triangular-solve block capacity";
+      return t;
+    }
+    if (i > 0) {
+      const int64_t s = t.bstart.back();
+      int64_t nin = 0;
+      for (int64_t p = off[i]; p < off[i + 1]; ++p) nin += col[p] >= s;
+      const bool cut = i - s >= max_rows || level[i] <= level[i - 1] ||
+                       bx + (len - nin) > xcap || bi + nin > icap;
+      if (cut) {
+        t.bstart.push_back((int32_t)i);
+        bx = 0;
+        bi = 0;
+      }
+    }
+    const int64_t s = t.bstart.back();
+    for (int64_t p = off[i]; p < off[i + 1]; ++p) (col[p] >= s ? bi : bx) += 1;
+    t.max_block_ext = std::max<int>(t.max_block_ext, (int)bx);
+    t.max_block_int = std::max<int>(t.max_block_int, (int)bi);
+  }
+  t.bstart.push_back((int32_t)n);
+  t.xoff.assign(n + 1, 0);
+  t.ioff.assign(n + 1, 0);
+  t.dsrc.assign(dsrc, dsrc + n);
+  for (size_t b = 0; b + 1 < t.bstart.size(); ++b) {
+    const int32_t s = t.bstart[b];
+    for (int32_t i = s; i < t.bstart[b + 1]; ++i) {
+      for (int64_t p = off[i]; p < off[i + 1]; ++p) {
+        if (col[p] < s) {
+          t.xcol.push_back(col[p]);
+          t.xsrc.push_back(src[p]);
+        } else {
+          t.ik.push_back(col[p] - s);
+          t.isrc.push_back(src[p]);
+        }
+      }
+      t.xoff[i + 1] = (int32_t)t.xcol.size();
+      t.ioff[i + 1] = (int32_t)t.ik.size();
+    }
+  }
+  // Processing order: blocks by their level in the block dependency graph (a block depends
+  // on the blocks holding its external columns), ties in index order. It is a topological
+  // order, so a wave never waits for a block that comes after its own in the sequence.
+  const int64_t nb = (int64_t)t.bstart.size() - 1;
+  std::vector<int32_t> block_of(n), blev(nb, 0);
+  for (int64_t b = 0; b < nb; ++b)
+    for (int32_t i = t.bstart[b]; i < t.bstart[b + 1]; ++i) block_of[i] = (int32_t)b;
+  for (int64_t b = 0; b < nb; ++b) {
+    int32_t m = 0;
+    for (int32_t q = t.xoff[t.bstart[b]]; q < t.xoff[t.bstart[b + 1]]; ++q)
+      m = std::max(m, blev[block_of[t.xcol[q]]] + 1);
+    blev[b] = m;
+  }
+  t.border.resize(nb);
+  for (int64_t b = 0; b < nb; ++b) t.border[b] = (int32_t)b;
+  std::stable_sort(t.border.begin(), t.border.end(),
+                   [&](int32_t a, int32_t c) { return blev[a] < blev[c]; });
+  return t;
 }
 
 }  // namespace ml

@@ -64,4 +64,29 @@ TransposeLayout transpose_layout(const LowerPattern& lp);
 // DAG depth of the lower-triangular solve (longest dependency chain, in rows).
 i64 dag_depth(const LowerPattern& lp);
 
+// Row blocks of the dataflow triangular solve over a strictly-lower layout (off[n+1], col[]):
+// consecutive rows, at most `max_rows` per block, and a new block wherever a row's level (its
+// longest dependency chain) does not exceed the previous row's. Inside a block the rows then
+// form one dependency chain, so solving them one after the other costs no false ordering;
+// across blocks the solve only waits for true dependencies. Returns the block starts, n last.
+std::vector<int32_t> dataflow_blocks(int64_t n, const int32_t* off, const int32_t* col,
+                                     int max_rows);
+
+// One dataflow triangular solve in logical row order (strictly-lower layout off/col, entry
+// sources src into the factor's value array, diagonal sources dsrc). Rows are cut into blocks
+// as dataflow_blocks does, also capped so a block holds at most xcap entries outside it and
+// icap inside it; each row's entries are split into the external part (columns before the
+// block) and the internal part (columns inside, stored as the 0..63 offset from the block
+// start), both block-contiguous so a wave loads a whole block with coalesced reads.
+struct TriLayout {
+  std::vector<int32_t> bstart;           // blocks + 1
+  std::vector<int32_t> border;           // processing order: blocks sorted by block-DAG level
+  std::vector<int32_t> xoff, xcol, xsrc;  // external entries: rows + 1, entries, sources
+  std::vector<int32_t> ioff, ik, isrc;    // internal entries
+  std::vector<int32_t> dsrc;              // diagonal source per row
+  int max_block_ext = 0, max_block_int = 0;
+};
+TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const int32_t* src,
+                     const int32_t* dsrc, int max_rows, int xcap, int icap, std::string* err);
+
 }  // namespace ml

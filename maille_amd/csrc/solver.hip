@@ -126,16 +126,45 @@ int device_cus() {
   return cus;
 }
 
-// Waves of a persistent 64-thread kernel that are resident at once on the whole device.
+// Waves of a persistent 64-thread kernel that are resident at once on the whole device. The
+// occupancy the runtime reports for LDS-bound kernels can be one workgroup per CU more than
+// the hardware actually holds (measured on gfx950: 26/CU reported for 6 KB of LDS, 25 hold;
+// 13/CU for 12 KB, 12 hold), and a dataflow kernel whose waves are not all resident waits
+// forever, so one workgroup per CU is kept in reserve.
 int resident_waves(const void* kernel) {
   int per_cu = 0;
   HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0));
-  if (per_cu < 1) per_cu = 1;
+  per_cu = std::max(1, per_cu - 1);
   return per_cu * device_cus();
 }
 
-// 2 s of the 100 MHz real-time counter: the bound of every dataflow wait.
-constexpr long long WAIT_LIMIT = 200000000ll;
+// The bound of every dataflow wait: 2 s of the device's steady wall clock (wall_clock64 ticks
+// at hipDeviceAttributeWallClockRate kHz).
+long long wait_limit() {
+  int dev = 0, khz = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  if (khz <= 0) khz = 100000;
+  return 2000ll * khz;
+}
+
+// Launches a persistent dataflow kernel (64-thread blocks that wait on each other's results)
+// cooperatively, so the runtime guarantees that every block of the grid is resident at once;
+// `grid` is lowered (and remembered) while the device refuses it as too large.
+template <typename... Args>
+void launch_dataflow(void (*kernel)(Args...), int& grid, hipStream_t stream,
+                     Args... args) {
+  void* params[] = {(void*)&args...};
+  for (;;) {
+    hipError_t e = hipLaunchCooperativeKernel((const void*)kernel, dim3(grid), dim3(64), params,
+                                              0, stream);
+    if (e == hipSuccess) return;
+    (void)hipGetLastError();
+    if (e != hipErrorCooperativeLaunchTooLarge || grid <= 1)
+      throw HipError(std::string("cooperative launch: ") + hipGetErrorString(e));
+    grid = std::max(1, grid / 2);
+  }
+}
 
 }  // namespace
 
@@ -162,17 +191,46 @@ struct Operator {
   DBuf<int> d_soff, d_scol, d_slen, d_ssrc;
   DBuf<double> d_sval;
   DBuf<float> d_sval32;
-  // triangular layouts: forward (strictly lower rows) and backward (rows of L^T, reversed)
-  DBuf<int> d_foff, d_fcol, d_fsrc, d_fdsrc, d_boff, d_bcol, d_bsrc, d_bdsrc;
-  DBuf<double> d_l, d_fval, d_bval, d_fdiag, d_bdiag;
-  DBuf<float> d_l32, d_fval32, d_bval32, d_fdiag32, d_bdiag32;
+  // the IC(0) factor and the layouts of its two triangular solves
+  DBuf<double> d_l;
+  DBuf<float> d_l32;
+  struct TriDev {
+    int nblk = 0;
+    DBuf<int> border, bstart, xoff, xcol, xsrc, ioff, isrc, dsrc;
+    DBuf<unsigned char> ik;
+    DBuf<double> xval, ival, diag;
+    DBuf<float> xval32, ival32, diag32;
+    void set(const TriLayout& t, hipStream_t st) {
+      nblk = (int)t.bstart.size() - 1;
+      border.upload_vec(t.border, st);
+      bstart.upload_vec(t.bstart, st);
+      xoff.upload_vec(t.xoff, st);
+      xcol.upload_vec(t.xcol, st);
+      xsrc.upload_vec(t.xsrc, st);
+      ioff.upload_vec(t.ioff, st);
+      isrc.upload_vec(t.isrc, st);
+      dsrc.upload_vec(t.dsrc, st);
+      std::vector<unsigned char> k8(t.ik.begin(), t.ik.end());
+      ik.upload_vec(k8, st);
+      HIP_CHECK(hipStreamSynchronize(st));
+      xval.alloc(t.xcol.size());
+      ival.alloc(t.ik.size());
+      diag.alloc(t.dsrc.size());
+    }
+    void alloc32() {
+      xval32.alloc(xval.n);
+      ival32.alloc(ival.n);
+      diag32.alloc(diag.n);
+    }
+  } fw, bw;
   int64_t nf = 0;  // strictly lower entries
   // workspace
   DBuf<double> w_x, w_r, w_p, w_ap, w_y, w_z, w_b, w_d, w_ax, w_partial;
   DBuf<float> w_r32, w_p32, w_ap32, w_y32, w_z32;
   DBuf<mlk::PcgScalars> d_S, d_Sin;
-  DBuf<int> d_flags;  // [0] abort, [1] IC(0) failure
+  DBuf<int> d_flags;  // [0] abort, [1] IC(0) failure, [2..5] where an IC(0) wait gave up
   int tri_grid64 = 0, tri_grid32 = 0, ic_grid = 0;
+  long long wait_ticks = 0;
 
   ~Operator() {
     if (stream) (void)hipStreamDestroy(stream);
@@ -180,6 +238,7 @@ struct Operator {
 
   void init_stream() {
     if (!stream) HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    if (!wait_ticks) wait_ticks = wait_limit();
   }
 
   // Upload the pattern and every layout derived from it.
@@ -212,7 +271,8 @@ struct Operator {
     d_scol.upload_vec(s.cols, stream);
     d_slen.upload_vec(slen, stream);
     d_ssrc.upload_vec(ssrc, stream);
-    // forward layout: strictly lower part of each row, diagonal apart
+    // forward solve: strictly lower rows of L; backward solve: logical row l = n-1-j holds the
+    // rows i > j of column j of L in decreasing i (ascending logical column n-1-i)
     nf = nnz - n;
     std::vector<int> foff(n + 1), fcol(nf), fsrc(nf), fdsrc(n);
     for (int i = 0; i <= n; ++i) foff[i] = (int)(lp.offsets[i] - i);
@@ -224,12 +284,6 @@ struct Operator {
       }
       fdsrc[i] = (int)(lp.offsets[i + 1] - 1);
     }
-    d_foff.upload_vec(foff, stream);
-    d_fcol.upload_vec(fcol, stream);
-    d_fsrc.upload_vec(fsrc, stream);
-    d_fdsrc.upload_vec(fdsrc, stream);
-    // backward layout: logical row l = n-1-j holds the rows i > j of column j of L, in
-    // decreasing i (ascending logical column n-1-i)
     TransposeLayout t = transpose_layout(lp);
     std::vector<int> boff(n + 1, 0), bcol(nf), bsrc(nf), bdsrc(n);
     for (int l = 0; l < n; ++l) {
@@ -245,30 +299,29 @@ struct Operator {
       }
       bdsrc[l] = (int)(lp.offsets[j + 1] - 1);
     }
-    d_boff.upload_vec(boff, stream);
-    d_bcol.upload_vec(bcol, stream);
-    d_bsrc.upload_vec(bsrc, stream);
-    d_bdsrc.upload_vec(bdsrc, stream);
+    std::string err;
+    fw.set(tri_layout(n, foff.data(), fcol.data(), fsrc.data(), fdsrc.data(), 64, mlk::TRI_XCAP,
+                      mlk::TRI_ICAP, &err), stream);
+    bw.set(tri_layout(n, boff.data(), bcol.data(), bsrc.data(), bdsrc.data(), 64, mlk::TRI_XCAP,
+                      mlk::TRI_ICAP, &err), stream);
+    if (!err.empty()) throw ArgError(err);
     // values and workspace
     d_a.alloc(nnz);
     d_l.alloc(nnz);
     d_sval.alloc(d_scol.n);
-    d_fval.alloc(nf);
-    d_bval.alloc(nf);
-    d_fdiag.alloc(n);
-    d_bdiag.alloc(n);
     for (auto* b : {&w_x, &w_r, &w_p, &w_ap, &w_y, &w_z, &w_b, &w_d, &w_ax}) b->alloc(n);
-    tri_grid64 = std::min<int>((n + 63) / 64,
-                               resident_waves((const void*)mlk::k_trisolve<double, false, false>));
-    tri_grid32 = std::min<int>((n + 63) / 64,
-                               resident_waves((const void*)mlk::k_trisolve<float, false, false>));
-    ic_grid = std::min<int>(n, resident_waves((const void*)mlk::k_ic0<double, 512>));
+    const int nblk = std::max(fw.nblk, bw.nblk);
+    tri_grid64 = std::min<int>(nblk, resident_waves((const void*)mlk::k_trisolve<double, false, false>));
+    tri_grid32 = std::min<int>(nblk, resident_waves((const void*)mlk::k_trisolve<float, false, false>));
+    ic_grid = std::min<int>(n, std::min(resident_waves((const void*)mlk::k_ic0<double, 512>),
+                                        resident_waves((const void*)mlk::k_ic0<float, 512>)));
     int64_t parts = std::max<int64_t>({grid_for(n), (int64_t)tri_grid64, (int64_t)tri_grid32, 1});
     w_partial.alloc(parts);
     d_S.alloc(1);
     d_Sin.alloc(1);
-    d_flags.alloc(2);
+    d_flags.alloc(6);
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 2, 0);
+    mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p + 2, 4, -1);
     HIP_CHECK(hipGetLastError());
     if (precision != ML_FP64) alloc_mixed();
     factored = false;
@@ -280,10 +333,8 @@ struct Operator {
     d_a32.alloc(nnz);
     d_l32.alloc(nnz);
     d_sval32.alloc(d_scol.n);
-    d_fval32.alloc(nf);
-    d_bval32.alloc(nf);
-    d_fdiag32.alloc(n);
-    d_bdiag32.alloc(n);
+    fw.alloc32();
+    bw.alloc32();
     for (auto* b : {&w_r32, &w_p32, &w_ap32, &w_y32, &w_z32}) b->alloc(n);
   }
 
@@ -311,20 +362,32 @@ struct Operator {
   template <typename T>
   int run_ic0(const T* a, T* l, T shift) {
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 2, 0);
+    mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p + 2, 4, -1);
     mlk::k_fill_bits<T><<<grid_for(nnz), 256, 0, stream>>>(l, nnz, mlk::Bits<T>::S);
     hipEvent_t ev = prof.begin(stream);
+    const int* off = d_off.p;
+    const int* col = d_col.p;
+    int* fail = d_flags.p + 1;
+    int* abort_flag = d_flags.p;
+    int* where = d_flags.p + 2;
+    long long limit = wait_ticks;
+    int nn = n;
     if (max_row <= 64)
-      mlk::k_ic0<T, 64><<<ic_grid, 64, 0, stream>>>(n, d_off.p, d_col.p, a, l, shift,
-                                                    d_flags.p + 1, d_flags.p, WAIT_LIMIT);
+      launch_dataflow(mlk::k_ic0<T, 64>, ic_grid, stream, nn, off, col, a, l, shift, fail,
+                      abort_flag, limit, where);
     else
-      mlk::k_ic0<T, 512><<<ic_grid, 64, 0, stream>>>(n, d_off.p, d_col.p, a, l, shift,
-                                                     d_flags.p + 1, d_flags.p, WAIT_LIMIT);
-    HIP_CHECK(hipGetLastError());
+      launch_dataflow(mlk::k_ic0<T, 512>, ic_grid, stream, nn, off, col, a, l, shift, fail,
+                      abort_flag, limit, where);
     prof.end(P_IC0, ev, stream);
-    int flags[2];
+    int flags[6];
     HIP_CHECK(hipMemcpyAsync(flags, d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
-    if (flags[0]) return ML_DEVICE_TIMEOUT;
+    if (flags[0]) {
+      g_last_error = "IC(0) wait expired: row " + std::to_string(flags[2]) + " waiting on row " +
+                     std::to_string(flags[3]) + ", wave " + std::to_string(flags[4]) + " of " +
+                     std::to_string(flags[5]) + ", limit " + std::to_string(wait_ticks) + " ticks";
+      return ML_DEVICE_TIMEOUT;
+    }
     if (flags[1] == 2) throw ArgError("matrix row longer than 512 entries");
     return flags[1] ? ML_SINGULAR : ML_OK;
   }
@@ -341,7 +404,7 @@ struct Operator {
       if (st == ML_OK) { shift_index = s; break; }
     }
     if (st != ML_OK) return ML_SINGULAR;
-    gather_factor<double>(d_l.p, d_fval.p, d_bval.p, d_fdiag.p, d_bdiag.p);
+    gather_factor<double>(d_l.p);
     if (precision != ML_FP64) {
       int st32 = ML_SINGULAR;
       for (int s = 0; s < 6; ++s) {
@@ -350,18 +413,23 @@ struct Operator {
         if (st32 == ML_OK) break;
       }
       if (st32 != ML_OK) return ML_SINGULAR;
-      gather_factor<float>(d_l32.p, d_fval32.p, d_bval32.p, d_fdiag32.p, d_bdiag32.p);
+      gather_factor<float>(d_l32.p);
     }
     factored = true;
     return ML_OK;
   }
 
   template <typename T>
-  void gather_factor(const T* l, T* fval, T* bval, T* fdiag, T* bdiag) {
-    mlk::k_gather<T, T><<<grid_for(nf), 256, 0, stream>>>(nf, d_fsrc.p, l, fval);
-    mlk::k_gather<T, T><<<grid_for(nf), 256, 0, stream>>>(nf, d_bsrc.p, l, bval);
-    mlk::k_gather<T, T><<<grid_for(n), 256, 0, stream>>>(n, d_fdsrc.p, l, fdiag);
-    mlk::k_gather<T, T><<<grid_for(n), 256, 0, stream>>>(n, d_bdsrc.p, l, bdiag);
+  void gather_factor(const T* l) {
+    const bool f32 = std::is_same<T, float>::value;
+    for (TriDev* t : {&fw, &bw}) {
+      T* xv = f32 ? (T*)t->xval32.p : (T*)t->xval.p;
+      T* iv = f32 ? (T*)t->ival32.p : (T*)t->ival.p;
+      T* dv = f32 ? (T*)t->diag32.p : (T*)t->diag.p;
+      mlk::k_gather<T, T><<<grid_for(t->xval.n), 256, 0, stream>>>((int64_t)t->xval.n, t->xsrc.p, l, xv);
+      mlk::k_gather<T, T><<<grid_for(t->ival.n), 256, 0, stream>>>((int64_t)t->ival.n, t->isrc.p, l, iv);
+      mlk::k_gather<T, T><<<grid_for(n), 256, 0, stream>>>((int64_t)n, t->dsrc.p, l, dv);
+    }
     HIP_CHECK(hipGetLastError());
   }
 
@@ -369,23 +437,34 @@ struct Operator {
   // z = (L L^T)^-1 r through y; partial gets r.z per wave when dotvec is given
   template <typename T>
   void precondition(const T* r, T* y, T* z, const double* dotvec, double* partial,
-                    const mlk::PcgScalars* S) {
+                    const mlk::PcgScalars* S, long long* trace_fw = nullptr,
+                    long long* trace_bw = nullptr) {
     const bool f32 = std::is_same<T, float>::value;
-    const T* fval = f32 ? (const T*)d_fval32.p : (const T*)d_fval.p;
-    const T* bval = f32 ? (const T*)d_bval32.p : (const T*)d_bval.p;
-    const T* fdiag = f32 ? (const T*)d_fdiag32.p : (const T*)d_fdiag.p;
-    const T* bdiag = f32 ? (const T*)d_bdiag32.p : (const T*)d_bdiag.p;
-    int grid = f32 ? tri_grid32 : tri_grid64;
-    int nblocks = (n + 63) / 64;
+    int& grid = f32 ? tri_grid32 : tri_grid64;
     hipEvent_t ev = prof.begin(stream);
-    mlk::k_trisolve<T, false, false><<<grid, 64, 0, stream>>>(
-        n, nblocks, d_foff.p, d_fcol.p, fval, fdiag, r, y, nullptr, nullptr, S, d_flags.p,
-        WAIT_LIMIT);
-    HIP_CHECK(hipGetLastError());
-    mlk::k_trisolve<T, true, true><<<grid, 64, 0, stream>>>(
-        n, nblocks, d_boff.p, d_bcol.p, bval, bdiag, y, z, dotvec, partial, S, d_flags.p,
-        WAIT_LIMIT);
-    HIP_CHECK(hipGetLastError());
+    int nn = n;
+    const double* nodot = nullptr;
+    double* nopart = nullptr;
+    long long limit = wait_ticks;
+    auto run = [&](TriDev& t, const T* rhs, T* out, const double* dv, double* part, bool fwd,
+                   long long* trace) {
+      const T* xv = f32 ? (const T*)t.xval32.p : (const T*)t.xval.p;
+      const T* iv = f32 ? (const T*)t.ival32.p : (const T*)t.ival.p;
+      const T* dg = f32 ? (const T*)t.diag32.p : (const T*)t.diag.p;
+      int nb = t.nblk;
+      if (fwd)
+        launch_dataflow(mlk::k_trisolve<T, false, false>, grid, stream, nn, nb,
+                        (const int*)t.border.p, (const int*)t.bstart.p, (const int*)t.xoff.p, (const int*)t.xcol.p, xv,
+                        (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out, dv,
+                        part, S, d_flags.p, limit, trace);
+      else
+        launch_dataflow(mlk::k_trisolve<T, true, true>, grid, stream, nn, nb,
+                        (const int*)t.border.p, (const int*)t.bstart.p, (const int*)t.xoff.p, (const int*)t.xcol.p, xv,
+                        (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out, dv,
+                        part, S, d_flags.p, limit, trace);
+    };
+    run(fw, r, y, nodot, nopart, true, trace_fw);
+    run(bw, (const T*)y, z, dotvec, partial, false, trace_bw);
     prof.end(f32 ? P_TRI32 : P_TRI64, ev, stream);
   }
 
@@ -442,7 +521,8 @@ struct Operator {
     TM* rlow = m32 ? (TM*)w_r32.p : nullptr;
     const TM* rin = m32 ? (const TM*)w_r32.p : (const TM*)r;
     const int vb = grid_for(n);
-    const int tri_parts = m32 ? tri_grid32 : tri_grid64;
+    // partial sums written by the backward solve: one per wave of its (possibly lowered) grid
+    auto tri_parts = [&]() { return m32 ? tri_grid32 : tri_grid64; };
     double* part = w_partial.p;
     init_scalars(S, maxit, a32);
     mlk::k_fill_f64<<<vb, 256, 0, stream>>>(x, n, 0.0);
@@ -454,7 +534,7 @@ struct Operator {
     fill_sentinel<TM>(y);
     fill_sentinel<TM>(z);
     precondition<TM>(rin, y, z, r, part, S);
-    mlk::k_init_rz<<<1, 256, 0, stream>>>(part, tri_parts, S);
+    mlk::k_init_rz<<<1, 256, 0, stream>>>(part, tri_parts(), S);
     mlk::k_convert<TM, TA><<<vb, 256, 0, stream>>>(n, z, p);
     HIP_CHECK(hipGetLastError());
     *hS = read_scalars(S);
@@ -475,7 +555,7 @@ struct Operator {
           mlk::k_replaced<<<1, 256, 0, stream>>>(part, vb, S);
         }
         precondition<TM>(rin, y, z, r, part, S);
-        mlk::k_beta<<<1, 256, 0, stream>>>(part, tri_parts, S);
+        mlk::k_beta<<<1, 256, 0, stream>>>(part, tri_parts(), S);
         ev = prof.begin(stream);
         mlk::k_update_p<TA, TM><<<vb, 256, 0, stream>>>(n, S, p, z);
         prof.end(P_VEC, ev, stream);
@@ -1030,6 +1110,34 @@ ml_status ml_pcg_factor(ml_pcg* h, double* l, int32_t* shift_index) {
 }
 
 void ml_pcg_destroy(ml_pcg* h) { delete h; }
+
+ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* trace_fw,
+                       int64_t* trace_bw, int64_t capacity) {
+  return guarded([&]() -> int {
+    if (!h || !r || !nblocks) throw ArgError("null argument");
+    Operator& op = h->op;
+    nblocks[0] = op.fw.nblk;
+    nblocks[1] = op.bw.nblk;
+    if (!trace_fw || !trace_bw) return ML_OK;
+    if (capacity < 4 * (int64_t)std::max(op.fw.nblk, op.bw.nblk)) throw ArgError("trace too small");
+    if (!op.factored) {
+      int st = op.factor();
+      if (st != ML_OK) return st;
+    }
+    DBuf<long long> tf, tb;
+    tf.alloc(4 * op.fw.nblk);
+    tb.alloc(4 * op.bw.nblk);
+    HIP_CHECK(hipMemcpyAsync(op.w_r.p, r, sizeof(double) * op.n, hipMemcpyHostToDevice, op.stream));
+    op.fill_sentinel<double>(op.w_y.p);
+    op.fill_sentinel<double>(op.w_z.p);
+    mlk::k_fill_i32<<<1, 64, 0, op.stream>>>(op.d_flags.p, 2, 0);
+    op.precondition<double>(op.w_r.p, op.w_y.p, op.w_z.p, nullptr, nullptr, nullptr, tf.p, tb.p);
+    HIP_CHECK(hipMemcpyAsync(trace_fw, tf.p, sizeof(long long) * 4 * op.fw.nblk, hipMemcpyDeviceToHost, op.stream));
+    HIP_CHECK(hipMemcpyAsync(trace_bw, tb.p, sizeof(long long) * 4 * op.bw.nblk, hipMemcpyDeviceToHost, op.stream));
+    HIP_CHECK(hipStreamSynchronize(op.stream));
+    return ML_OK;
+  });
+}
 
 ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elements, int nen,
                           const int64_t* conn, const double* young, const double* nu,

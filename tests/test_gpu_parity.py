@@ -86,7 +86,8 @@ def test_fp64_solve_matches_oracle_pcg(gpu, name):
     m = gpu.Model(p)
     u, r, info = m.solve()
     assert rel_l2(u, ref["displacements"]) <= 1e-10
-    assert abs(info["iterations"] - ref["iterations"]) <= 3
+    # only the dot products are summed in another order: the iteration count stays close
+    assert abs(info["iterations"] - ref["iterations"]) <= max(3, 0.06 * ref["iterations"])
     # the recursive residual of PCG drifts from b - Ax; the device drifts like the oracle
     assert info["true_residual"] <= 10 * ref["true_residual"] + 1e-12
     assert relative_gap(r, ref["reactions"]) < 1e-8
