@@ -102,6 +102,19 @@ int64_t ml_pattern_ntargets(const ml_pattern* p);
 ml_status ml_pattern_graph(const ml_pattern* p, int64_t* offsets, int64_t* targets);
 void ml_pattern_free(ml_pattern* p);
 
+/* Diagnostics (no counterpart in the reference; host only, no device needed): builds every
+ * index array the device kernels of ml_pcg_create would read for the lower CSR pattern
+ * (offsets[n+1], cols[]; ascending columns, diagonal last) and checks each bound they rely
+ * on: index ranges, block capacities, each row's entries covered once and in order, and a
+ * topological processing order of the triangular-solve blocks. ML_INVALID_ARGUMENT with the
+ * failed check in ml_last_error otherwise. stats[8] (optional): forward and backward block
+ * counts, their largest external and internal entry counts (forward, then backward), the
+ * DAG depth and the longest row. With lvalues (the factor, lower-CSR order), r and z
+ * non-null, also applies z = (L L^T)^-1 r on the host through the triangular-solve layouts,
+ * in the device's block order and summation order. */
+ml_status ml_layout_verify(int64_t n, const int64_t* offsets, const int64_t* cols,
+                           const double* lvalues, const double* r, double* z, int64_t* stats);
+
 /* ------------------------------------------------------------------ Pcg on a given Csr */
 
 typedef struct ml_pcg ml_pcg;
@@ -128,9 +141,9 @@ ml_status ml_pcg_factor(ml_pcg* h, double* l, int32_t* shift_index);
 void ml_pcg_destroy(ml_pcg* h);
 /* Diagnostics (no counterpart in the reference): one preconditioner application with a
  * per-block timeline of both triangular solves. nblocks[2] receives the block counts; with
- * non-null traces (capacity int64 each, >= 4 per block) each block records its start, the end
- * of its staging, its end (device wall-clock ticks) and (wave << 8 | lanes still waiting on an
- * external value after staging). */
+ * non-null traces (capacity int64 each, >= 6 per block) each block records its start, the end
+ * of its staging, its end (device wall-clock ticks), (wave << 8 | lanes still waiting on an
+ * external value after staging), the ticks spent polling and the number of polling rounds. */
 ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* trace_fw,
                        int64_t* trace_bw, int64_t capacity);
 

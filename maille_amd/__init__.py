@@ -35,9 +35,9 @@ ITERATIVE_FROM = 100_000   # solve.rs:185
 EXPORTS = (
     "ml_last_error", "ml_device_count", "ml_solid_d", "ml_stiffness_batch",
     "ml_pattern_build", "ml_pattern_n", "ml_pattern_nnz", "ml_pattern_get",
-    "ml_pattern_ntargets", "ml_pattern_graph", "ml_pattern_free",
+    "ml_pattern_ntargets", "ml_pattern_graph", "ml_pattern_free", "ml_layout_verify",
     "ml_pcg_create", "ml_pcg_solve", "ml_pcg_solve_device", "ml_pcg_multiply",
-    "ml_pcg_precondition", "ml_pcg_factor", "ml_pcg_destroy",
+    "ml_pcg_precondition", "ml_pcg_factor", "ml_pcg_destroy", "ml_pcg_trace",
     "ml_model_create", "ml_model_assemble", "ml_model_set_young", "ml_model_factor",
     "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_add_pressures",
     "ml_model_nfree", "ml_model_nnz", "ml_model_levels", "ml_model_csr",
@@ -121,6 +121,7 @@ def lib():
             "ml_pattern_ntargets": (_i64, [_P]),
             "ml_pattern_graph": (_int, [_P, _I, _I]),
             "ml_pattern_free": (None, [_P]),
+            "ml_layout_verify": (_int, [_i64, _I, _I, _D, _D, _D, _I]),
             "ml_pcg_create": (_int, [_i64, _I, _I, _D, _int, _PP]),
             "ml_pcg_solve": (_int, [_P, _D, _D, _i64, C.c_double, _i64, _S]),
             "ml_pcg_solve_device": (_int, [_P, _P, _P, C.c_double, _i64, _S]),
@@ -128,6 +129,7 @@ def lib():
             "ml_pcg_precondition": (_int, [_P, _D, _D]),
             "ml_pcg_factor": (_int, [_P, _D, _I32]),
             "ml_pcg_destroy": (None, [_P]),
+            "ml_pcg_trace": (_int, [_P, _D, _I, _I, _I, _i64]),
             "ml_model_create": (_int, [_i64, _D, _i64, _int, _I, _D, _D, _U8, _D, _int, _int,
                                        _PP, _S]),
             "ml_model_assemble": (_int, [_P]),
@@ -298,6 +300,28 @@ class Csr:
         if k >= row.stop - row.start or self.cols[row][k] != j:
             raise ValueError("terme hors du profil")
         self.values[row.start + k] += v
+
+
+def layout_verify(offsets, cols, l=None, r=None):
+    """Builds and checks, on the host, every index array the device kernels would read for
+    this lower CSR pattern (raises ValueError naming the failed check). Returns the layout
+    statistics (block counts, largest external/internal block loads, DAG depth, longest row)
+    and, when the factor values ``l`` and a vector ``r`` are given, z = (L L^T)^-1 r applied
+    through the triangular-solve layouts in the device's order."""
+    off, cols = _ai64(offsets), _ai64(cols)
+    stats = np.zeros(8, dtype=np.int64)
+    z = None
+    if l is not None and r is not None:
+        l, r = _f64(l), _f64(r)
+        z = np.zeros(len(off) - 1)
+    _check(lib().ml_layout_verify(len(off) - 1, _i(off), _i(cols),
+                                  _d(l) if z is not None else None,
+                                  _d(r) if z is not None else None,
+                                  _d(z) if z is not None else None, _i(stats)))
+    keys = ("fw_blocks", "bw_blocks", "fw_max_ext", "fw_max_int", "bw_max_ext", "bw_max_int",
+            "depth", "max_row")
+    out = dict(zip(keys, (int(v) for v in stats)))
+    return (out, z) if z is not None else out
 
 
 class Pcg:

@@ -17,7 +17,8 @@ HOST_SOURCES = ["pattern.cpp", "plan.cpp"]
 DEVICE_SOURCES = ["solver.hip"]
 HEADERS = ["kernels.hpp", "kernels.hip", "pattern.hpp", "plan.hpp"]
 
-CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
+            "-Wno-unused-parameter"]
 # Packed FP32 instructions are disabled (the device code is written for FP64 and scalar
 # FP32); -ffp-contract is left to the per-kernel pragmas.
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

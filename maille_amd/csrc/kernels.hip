@@ -651,7 +651,7 @@ __device__ inline T accumulate(T acc, T a, T v) {
   return SUBSEQ ? acc - a * v : acc + a * v;
 }
 
-constexpr int TRI_XCAP = 768;  // external entries a block may hold (host-enforced)
+constexpr int TRI_XCAP = 512;  // external entries a block may hold (host-enforced)
 constexpr int TRI_ICAP = 512;  // internal entries a block may hold (host-enforced)
 
 template <typename T, bool REV, bool SUBSEQ>
@@ -667,7 +667,6 @@ __global__ __launch_bounds__(64) void k_trisolve(
   if (S && S->done) return;
   typedef typename Bits<T>::U U;
   __shared__ T sxv[TRI_XCAP];
-  __shared__ T sxa[TRI_XCAP];
   __shared__ int sxc[TRI_XCAP];
   __shared__ T sia[TRI_ICAP];
   __shared__ unsigned char sik[TRI_ICAP];
@@ -682,14 +681,13 @@ __global__ __launch_bounds__(64) void k_trisolve(
     const int xb = xoff[s], nx = xoff[e] - xb;
     const int ib = ioff[s], ni = ioff[e] - ib;
     long long tb = trace ? wall_clock64() : 0;
-    // stage the block: external columns, coefficients and the values published so far, read
-    // at device scope (a plain load could hit a stale line of this XCD's L2 that still holds
-    // the sentinel)
+    // stage the block: external columns and the values published so far, read at device
+    // scope (a plain load could hit a stale line of this XCD's L2 that still holds the
+    // sentinel); their coefficients stay in global memory and are read eight at a time
 #pragma unroll 4
     for (int q = lane; q < nx; q += 64) {
       const int c = xcol[xb + q];
       sxc[q] = c;
-      sxa[q] = xval[xb + q];
       sxv[q] = from_bits<T>(__hip_atomic_load(reinterpret_cast<U*>(out + (REV ? n - 1 - c : c)),
                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
@@ -716,9 +714,20 @@ __global__ __launch_bounds__(64) void k_trisolve(
     }
     __syncthreads();
     // external values that were ready, in order
-    while (p < x1 && !is_sentinel(sxv[p])) {
-      acc = accumulate<T, SUBSEQ>(acc, sxa[p], sxv[p]);
-      ++p;
+    for (bool go = p < x1; go;) {
+      T aa[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) aa[u] = (p + u < x1) ? xval[xb + p + u] : T(0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (go && p < x1 && !is_sentinel(sxv[p])) {
+          acc = accumulate<T, SUBSEQ>(acc, aa[u], sxv[p]);
+          ++p;
+        } else {
+          go = false;
+        }
+      }
+      go = go && p < x1;
     }
     bool ext_done = p == x1;
     long long tp = 0;
@@ -730,7 +739,10 @@ __global__ __launch_bounds__(64) void k_trisolve(
     int nk = ip < i1 ? (int)sik[ip] : 64;
     T nv = ip < i1 ? sia[ip] : T(0);
     long long t0 = -1;
+    long long waited = 0;
+    int rounds = 0;
     for (int k = 0; k < rows; ++k) {
+      const long long tw = trace ? wall_clock64() : 0;
       // Row k may not start before its external values are in: the whole wave polls, each
       // lane with pending entries loading its next eight, so one round trip serves every row
       // of the block. A lane whose external part completes catches up, in order, on the
@@ -739,13 +751,16 @@ __global__ __launch_bounds__(64) void k_trisolve(
         if (p < x1) {
           int jj[8];
           U vv[8];
+          T aa[8];
 #pragma unroll
           for (int u = 0; u < 8; ++u) jj[u] = (p + u < x1) ? sxc[p + u] : -1;
 #pragma unroll
-          for (int u = 0; u < 8; ++u)
+          for (int u = 0; u < 8; ++u) {
             vv[u] = jj[u] >= 0 ? __hip_atomic_load(reinterpret_cast<U*>(out + (REV ? n - 1 - jj[u] : jj[u])),
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                                : (U)0;
+            aa[u] = jj[u] >= 0 ? xval[xb + p + u] : T(0);
+          }
           bool waiting = false;
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
@@ -753,7 +768,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
               if (vv[u] == Bits<T>::S) {
                 waiting = true;
               } else {
-                acc = accumulate<T, SUBSEQ>(acc, sxa[p], from_bits<T>(vv[u]));
+                acc = accumulate<T, SUBSEQ>(acc, aa[u], from_bits<T>(vv[u]));
                 ++p;
               }
             }
@@ -786,7 +801,9 @@ __global__ __launch_bounds__(64) void k_trisolve(
             __builtin_amdgcn_s_sleep(1);
           }
         }
+        if (trace) ++rounds;
       }
+      if (trace) waited += wall_clock64() - tw;
       T y = T(0);
       if (lane == k) {
         y = SUBSEQ ? acc / dl : (rl - acc) / dl;
@@ -803,10 +820,12 @@ __global__ __launch_bounds__(64) void k_trisolve(
       }
     }
     if (trace && lane == 0) {
-      trace[4 * (long long)b + 0] = tb;
-      trace[4 * (long long)b + 1] = tp;
-      trace[4 * (long long)b + 2] = wall_clock64();
-      trace[4 * (long long)b + 3] = ((long long)blockIdx.x << 8) | late;
+      trace[6 * (long long)b + 0] = tb;
+      trace[6 * (long long)b + 1] = tp;
+      trace[6 * (long long)b + 2] = wall_clock64();
+      trace[6 * (long long)b + 3] = ((long long)blockIdx.x << 8) | late;
+      trace[6 * (long long)b + 4] = waited;
+      trace[6 * (long long)b + 5] = rounds;
     }
     __syncthreads();
   }
@@ -824,15 +843,33 @@ __global__ __launch_bounds__(64) void k_trisolve(
 // over its dense work vector; l_ij = (a_ij - s) / l_jj; then the pivot a_ii(1+shift) - sum of
 // squares must exceed 1e-12 a_ii. Rows of L are published entry by entry; a failed pivot
 // raises *fail and every later row only writes zeros so the launch drains quickly.
-template <typename T, int MAXROW>
+//
+// Staged path (STAGED, rows of at most 64 entries): before the sequential loop over p the wave
+// loads, with flattened coalesced reads, the bounds and diagonal of every row j, all of their
+// columns, finds which are columns of row i (binary search) and keeps the hits in LDS in the
+// reference's summation order with their l values read at device scope; the loop over p then
+// only touches LDS (and polls the few values that were not published yet). Rows with more
+// hits than the LDS holds take the generic path, which reads row j entry by entry.
+constexpr int IC0_HCAP = 512;
+
+template <typename T, int MAXROW, bool STAGED>
 __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
                                             const int* __restrict__ col,
                                             const T* __restrict__ a, T* l, T shift,
                                             int* fail, int* abort_flag, long long limit,
                                             int* where) {
 #pragma clang fp contract(off)
+  typedef typename Bits<T>::U U;
   __shared__ int scol[MAXROW];
   __shared__ T sw[MAXROW];
+  __shared__ T sa[MAXROW];
+  constexpr int SQ = STAGED ? MAXROW : 1;
+  constexpr int SH = STAGED ? IC0_HCAP : 1;
+  __shared__ int sjs[SQ], sjd[SQ], eoff[SQ + 1], hcount[SQ], hoff[SQ + 1];
+  __shared__ T sljj[SQ];
+  __shared__ unsigned char hr[SH];
+  __shared__ int hidx[SH];
+  __shared__ T hv[SH];
   const int lane = threadIdx.x;
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const int start = off[i], dg = off[i + 1] - 1, len = dg - start;
@@ -844,46 +881,136 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
       for (int t = lane; t <= len; t += 64) publish(l + start + t, T(0));
       continue;
     }
-    for (int t = lane; t < len; t += 64) scol[t] = col[start + t];
+    for (int t = lane; t < len; t += 64) {
+      scol[t] = col[start + t];
+      sa[t] = a[start + t];
+    }
     __syncthreads();
-    for (int q = 0; q < len; ++q) {
-      const int j = scol[q];
-      const int js = off[j], jd = off[j + 1] - 1;
-      T sacc = T(0);
-      for (int base = js; base < jd; base += 64) {
-        const int idx = base + lane;
-        T term = T(0);
+    bool staged = false;
+    if constexpr (STAGED) {
+      for (int q = lane; q < len; q += 64) {
+        const int j = scol[q];
+        const int js = off[j], jd = off[j + 1] - 1;
+        sjs[q] = js;
+        sjd[q] = jd;
+        hcount[q] = 0;
+        sljj[q] = from_bits<T>(__hip_atomic_load(reinterpret_cast<U*>(l + jd), __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT));
+      }
+      __syncthreads();
+      if (lane == 0) {
+        eoff[0] = 0;
+        for (int q = 0; q < len; ++q) eoff[q + 1] = eoff[q] + (sjd[q] - sjs[q]);
+      }
+      __syncthreads();
+      const int total = eoff[len];
+      int nh = 0;
+      for (int fb = 0; fb < total; fb += 64) {
+        const int f = fb + lane;
         bool hit = false;
-        if (idx < jd) {
-          const int k = col[idx];
-          int lo = 0, hi = q;  // search k among the computed columns scol[0..q)
-          while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (scol[mid] < k) lo = mid + 1;
+        int q = 0, r = 0, idx = 0;
+        if (f < total) {
+          int lo = 0, hi = len;  // q: last with eoff[q] <= f
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (eoff[mid] <= f) lo = mid;
             else hi = mid;
           }
-          if (lo < q && scol[lo] == k) {
-            const T ljk = await_value(l + idx, abort_flag, limit);
-            term = ljk * sw[lo];
-            hit = true;
+          q = lo;
+          idx = sjs[q] + (f - eoff[q]);
+          const int k = col[idx];
+          int lo2 = 0, hi2 = q;  // k among the columns scol[0..q)
+          while (lo2 < hi2) {
+            const int mid = (lo2 + hi2) >> 1;
+            if (scol[mid] < k) lo2 = mid + 1;
+            else hi2 = mid;
+          }
+          r = lo2;
+          hit = r < q && scol[r] == k;
+        }
+        const unsigned long long mask = __ballot(hit);
+        const int pos = nh + __popcll(mask & ((1ull << lane) - 1ull));
+        if (hit && pos < IC0_HCAP) {
+          hr[pos] = (unsigned char)r;
+          hidx[pos] = idx;
+          hv[pos] = from_bits<T>(__hip_atomic_load(reinterpret_cast<U*>(l + idx), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT));
+          atomicAdd(&hcount[q], 1);
+        }
+        nh += __popcll(mask);
+      }
+      staged = nh <= IC0_HCAP;
+      __syncthreads();
+      if (lane == 0) {
+        hoff[0] = 0;
+        for (int q = 0; q < len; ++q) hoff[q + 1] = hoff[q] + hcount[q];
+      }
+      __syncthreads();
+    }
+    for (int q = 0; q < len; ++q) {
+      T sacc = T(0);
+      int jd;
+      T ljj;
+      if (staged) {
+        const int h1 = hoff[q + 1];
+        for (int hb = hoff[q]; hb < h1; hb += 64) {
+          const int h = hb + lane;
+          T term = T(0);
+          const bool has = h < h1;
+          if (has) {
+            T v = hv[h];
+            if (is_sentinel(v)) v = await_value(l + hidx[h], abort_flag, limit);
+            term = v * sw[hr[h]];
+          }
+          unsigned long long mask = __ballot(has);
+          while (mask) {
+            const int bl = __builtin_ctzll(mask);
+            sacc = sacc + lane_broadcast(term, bl);
+            mask &= mask - 1;
           }
         }
-        unsigned long long mask = __ballot(hit);
-        while (mask) {
-          const int bl = __builtin_ctzll(mask);
-          sacc = sacc + lane_broadcast(term, bl);
-          mask &= mask - 1;
+        jd = sjd[q];
+        ljj = sljj[q];
+        if (is_sentinel(ljj)) ljj = await_value(l + jd, abort_flag, limit);
+      } else {
+        const int j = scol[q];
+        const int js = off[j];
+        jd = off[j + 1] - 1;
+        for (int base = js; base < jd; base += 64) {
+          const int idx = base + lane;
+          T term = T(0);
+          bool hit = false;
+          if (idx < jd) {
+            const int k = col[idx];
+            int lo = 0, hi = q;  // search k among the computed columns scol[0..q)
+            while (lo < hi) {
+              int mid = (lo + hi) >> 1;
+              if (scol[mid] < k) lo = mid + 1;
+              else hi = mid;
+            }
+            if (lo < q && scol[lo] == k) {
+              const T ljk = await_value(l + idx, abort_flag, limit);
+              term = ljk * sw[lo];
+              hit = true;
+            }
+          }
+          unsigned long long mask = __ballot(hit);
+          while (mask) {
+            const int bl = __builtin_ctzll(mask);
+            sacc = sacc + lane_broadcast(term, bl);
+            mask &= mask - 1;
+          }
         }
+        ljj = await_value(l + jd, abort_flag, limit);
       }
-      const T ljj = await_value(l + jd, abort_flag, limit);
       if (lane == 0 && __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
           __hip_atomic_load(where, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) {
         __hip_atomic_store(where, i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(where + 1, j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(where + 1, scol[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(where + 2, (int)blockIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(where + 3, (int)gridDim.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
-      const T lij = (a[start + q] - sacc) / ljj;
+      const T lij = (sa[q] - sacc) / ljj;
       if (lane == 0) sw[q] = lij;
       __syncthreads();
     }

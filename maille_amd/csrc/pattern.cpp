@@ -1,6 +1,7 @@
 #include "pattern.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -370,6 +371,204 @@ TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const in
   std::stable_sort(t.border.begin(), t.border.end(),
                    [&](int32_t a, int32_t c) { return blev[a] < blev[c]; });
   return t;
+}
+
+std::string device_layouts(const LowerPattern& lp, int xcap, int icap, DeviceLayouts* out) {
+  DeviceLayouts& d = *out;
+  const i64 n = lp.n();
+  d.n = n;
+  d.nnz = lp.nnz();
+  d.depth = dag_depth(lp);
+  d.max_row = 0;
+  for (i64 i = 0; i < n; ++i)
+    d.max_row = std::max<int>(d.max_row, (int)(lp.offsets[i + 1] - lp.offsets[i]));
+  d.off32.assign(lp.offsets.begin(), lp.offsets.end());
+  d.col32.assign(lp.cols.begin(), lp.cols.end());
+  // sliced ELL of the full symmetric matrix
+  SellLayout s = sell_layout(lp);
+  TransposeLayout t = transpose_layout(lp);
+  d.slices = s.slices;
+  d.soff.assign(s.slice_offsets.begin(), s.slice_offsets.end());
+  d.scol = s.cols;
+  d.ssrc.assign(s.src.begin(), s.src.end());
+  d.slen.resize(n);
+  for (i64 j = 0; j < n; ++j)
+    d.slen[j] = (int32_t)((lp.offsets[j + 1] - lp.offsets[j]) + (t.offsets[j + 1] - t.offsets[j]));
+  // forward solve: the strictly lower rows of L
+  const i64 nf = lp.nnz() - n;
+  std::vector<int32_t> foff(n + 1), fcol(nf), fsrc(nf), fdsrc(n);
+  for (i64 i = 0; i <= n; ++i) foff[i] = (int32_t)(lp.offsets[i] - i);
+  for (i64 i = 0; i < n; ++i) {
+    i64 o = foff[i];
+    for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p, ++o) {
+      fcol[o] = lp.cols[p];
+      fsrc[o] = (int32_t)p;
+    }
+    fdsrc[i] = (int32_t)(lp.offsets[i + 1] - 1);
+  }
+  // backward solve in logical rows l = n-1-j
+  std::vector<int32_t> boff(n + 1, 0), bcol(nf), bsrc(nf), bdsrc(n);
+  for (i64 l = 0; l < n; ++l) {
+    const i64 j = n - 1 - l;
+    boff[l + 1] = boff[l] + (int32_t)(t.offsets[j + 1] - t.offsets[j]);
+  }
+  for (i64 l = 0; l < n; ++l) {
+    const i64 j = n - 1 - l;
+    i64 o = boff[l];
+    for (i64 q = t.offsets[j + 1] - 1; q >= t.offsets[j]; --q, ++o) {
+      bcol[o] = (int32_t)(n - 1 - t.rows[q]);
+      bsrc[o] = (int32_t)t.src[q];
+    }
+    bdsrc[l] = (int32_t)(lp.offsets[j + 1] - 1);
+  }
+  std::string err;
+  d.fw = tri_layout(n, foff.data(), fcol.data(), fsrc.data(), fdsrc.data(), 64, xcap, icap, &err);
+  if (!err.empty()) return err;
+  d.bw = tri_layout(n, boff.data(), bcol.data(), bsrc.data(), bdsrc.data(), 64, xcap, icap, &err);
+  return err;
+}
+
+namespace {
+
+// One triangular-solve layout against the strictly-lower logical rows it was built from.
+std::string check_tri(const char* name, const TriLayout& t, i64 n, i64 nnz, int xcap, int icap,
+                      const std::vector<int32_t>& rows_off, const std::vector<int32_t>& rows_col,
+                      const std::vector<int32_t>& rows_src) {
+  const std::string pre = std::string(name) + " layout: ";
+  if (t.bstart.size() < 2 || t.bstart.front() != 0 || t.bstart.back() != n)
+    return pre + "block starts do not span the rows";
+  const i64 nb = (i64)t.bstart.size() - 1;
+  for (i64 b = 0; b < nb; ++b) {
+    const i64 rows = (i64)t.bstart[b + 1] - t.bstart[b];
+    if (rows < 1 || rows > 64) return pre + "block of " + std::to_string(rows) + " rows";
+  }
+  if ((i64)t.xoff.size() != n + 1 || (i64)t.ioff.size() != n + 1 || (i64)t.dsrc.size() != n)
+    return pre + "offset arrays of the wrong length";
+  if (t.xoff[0] != 0 || t.ioff[0] != 0 || (size_t)t.xoff[n] != t.xcol.size() ||
+      t.xcol.size() != t.xsrc.size() || (size_t)t.ioff[n] != t.ik.size() ||
+      t.ik.size() != t.isrc.size())
+    return pre + "entry arrays do not match their offsets";
+  std::vector<int32_t> block_of(n), pos(nb, -1);
+  for (i64 b = 0; b < nb; ++b)
+    for (i64 i = t.bstart[b]; i < t.bstart[b + 1]; ++i) block_of[i] = (int32_t)b;
+  if ((i64)t.border.size() != nb) return pre + "processing order of the wrong length";
+  for (i64 k = 0; k < nb; ++k) {
+    const i64 b = t.border[k];
+    if (b < 0 || b >= nb || pos[b] >= 0) return pre + "processing order is not a permutation";
+    pos[b] = (int32_t)k;
+  }
+  for (i64 b = 0; b < nb; ++b) {
+    const i64 s = t.bstart[b], e = t.bstart[b + 1];
+    if (t.xoff[e] - t.xoff[s] > xcap) return pre + "block over the external capacity";
+    if (t.ioff[e] - t.ioff[s] > icap) return pre + "block over the internal capacity";
+    for (i64 i = s; i < e; ++i) {
+      if (t.xoff[i + 1] < t.xoff[i] || t.ioff[i + 1] < t.ioff[i])
+        return pre + "decreasing offsets at row " + std::to_string(i);
+      if (t.dsrc[i] < 0 || t.dsrc[i] >= nnz) return pre + "diagonal source out of range";
+      // the row's entries, externals then internals, are its logical entries in order
+      i64 p = rows_off[i];
+      if ((t.xoff[i + 1] - t.xoff[i]) + (t.ioff[i + 1] - t.ioff[i]) != rows_off[i + 1] - p)
+        return pre + "row " + std::to_string(i) + " does not hold its entries";
+      for (i64 q = t.xoff[i]; q < t.xoff[i + 1]; ++q, ++p) {
+        const i64 c = t.xcol[q];
+        if (c < 0 || c >= s) return pre + "external column outside the earlier rows";
+        if (c != rows_col[p] || t.xsrc[q] != rows_src[p]) return pre + "external entry out of order";
+        if (t.xsrc[q] < 0 || t.xsrc[q] >= nnz) return pre + "external source out of range";
+        if (pos[block_of[c]] >= pos[b]) return pre + "processing order is not topological";
+      }
+      for (i64 q = t.ioff[i]; q < t.ioff[i + 1]; ++q, ++p) {
+        const i64 k = t.ik[q];
+        if (k < 0 || k >= i - s) return pre + "internal entry not before its row";
+        if (s + k != rows_col[p] || t.isrc[q] != rows_src[p]) return pre + "internal entry out of order";
+        if (t.isrc[q] < 0 || t.isrc[q] >= nnz) return pre + "internal source out of range";
+      }
+    }
+  }
+  return std::string();
+}
+
+}  // namespace
+
+std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, int xcap, int icap) {
+  const i64 n = lp.n(), nnz = lp.nnz();
+  if (d.n != n || d.nnz != nnz || (i64)d.off32.size() != n + 1 || (i64)d.col32.size() != nnz)
+    return "CSR copy of the wrong size";
+  // sliced ELL
+  if (d.slices != (n + 63) / 64 || (i64)d.soff.size() != d.slices + 1 || d.soff[0] != 0 ||
+      (size_t)d.soff[d.slices] != d.scol.size() || d.scol.size() != d.ssrc.size() ||
+      (i64)d.slen.size() != n)
+    return "sliced-ELL arrays of the wrong size";
+  for (i64 sl = 0; sl < d.slices; ++sl) {
+    const i64 w = (i64)d.soff[sl + 1] - d.soff[sl];
+    if (w < 64 || w % 64) return "sliced-ELL slice of a bad width";
+  }
+  for (i64 j = 0; j < n; ++j) {
+    const i64 sl = j >> 6, width = ((i64)d.soff[sl + 1] - d.soff[sl]) / 64;
+    if (d.slen[j] < 1 || d.slen[j] > width) return "sliced-ELL row longer than its slice";
+  }
+  for (size_t q = 0; q < d.scol.size(); ++q)
+    if (d.scol[q] < 0 || d.scol[q] >= n || d.ssrc[q] < -1 || d.ssrc[q] >= nnz)
+      return "sliced-ELL entry out of range";
+  // triangular solves, against the logical rows they were built from
+  const i64 nf = nnz - n;
+  std::vector<int32_t> foff(n + 1), fcol(nf), fsrc(nf);
+  for (i64 i = 0; i <= n; ++i) foff[i] = (int32_t)(lp.offsets[i] - i);
+  for (i64 i = 0; i < n; ++i)
+    for (i64 p = lp.offsets[i], o = foff[i]; p < lp.offsets[i + 1] - 1; ++p, ++o) {
+      fcol[o] = lp.cols[p];
+      fsrc[o] = (int32_t)p;
+    }
+  std::string err = check_tri("forward", d.fw, n, nnz, xcap, icap, foff, fcol, fsrc);
+  if (!err.empty()) return err;
+  for (i64 i = 0; i < n; ++i)
+    if (d.fw.dsrc[i] != lp.offsets[i + 1] - 1) return "forward layout: wrong diagonal";
+  // backward rows, restated independently of device_layouts: entries (i, j) of L with i > j
+  // sorted by logical row n-1-j, then ascending logical column n-1-i
+  std::vector<std::array<int64_t, 3>> ent;
+  ent.reserve(nf);
+  for (i64 i = 0; i < n; ++i)
+    for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p)
+      ent.push_back({n - 1 - (i64)lp.cols[p], n - 1 - i, p});
+  std::sort(ent.begin(), ent.end());
+  std::vector<int32_t> boff(n + 1, 0), bcol(nf), bsrc(nf);
+  for (i64 q = 0; q < nf; ++q) {
+    boff[ent[q][0] + 1] += 1;
+    bcol[q] = (int32_t)ent[q][1];
+    bsrc[q] = (int32_t)ent[q][2];
+  }
+  for (i64 l = 0; l < n; ++l) boff[l + 1] += boff[l];
+  err = check_tri("backward", d.bw, n, nnz, xcap, icap, boff, bcol, bsrc);
+  if (!err.empty()) return err;
+  for (i64 l = 0; l < n; ++l)
+    if (d.bw.dsrc[l] != lp.offsets[n - l] - 1) return "backward layout: wrong diagonal";
+  return std::string();
+}
+
+void layout_precondition(const DeviceLayouts& d, const double* l, const double* r, double* z) {
+  const i64 n = d.n;
+  std::vector<double> y(n);
+  // forward: y_i = (r_i - sum l_ij y_j) / l_ii, the sum in ascending j
+  for (int32_t b : d.fw.border) {
+    const i64 s = d.fw.bstart[b], e = d.fw.bstart[b + 1];
+    for (i64 i = s; i < e; ++i) {
+      double acc = 0.0;
+      for (i64 q = d.fw.xoff[i]; q < d.fw.xoff[i + 1]; ++q) acc = acc + l[d.fw.xsrc[q]] * y[d.fw.xcol[q]];
+      for (i64 q = d.fw.ioff[i]; q < d.fw.ioff[i + 1]; ++q) acc = acc + l[d.fw.isrc[q]] * y[s + d.fw.ik[q]];
+      y[i] = (r[i] - acc) / l[d.fw.dsrc[i]];
+    }
+  }
+  // backward in logical rows, vectors indexed physically (n-1-logical)
+  for (int32_t b : d.bw.border) {
+    const i64 s = d.bw.bstart[b], e = d.bw.bstart[b + 1];
+    for (i64 lr = s; lr < e; ++lr) {
+      double acc = y[n - 1 - lr];
+      for (i64 q = d.bw.xoff[lr]; q < d.bw.xoff[lr + 1]; ++q)
+        acc = acc - l[d.bw.xsrc[q]] * z[n - 1 - d.bw.xcol[q]];
+      for (i64 q = d.bw.ioff[lr]; q < d.bw.ioff[lr + 1]; ++q)
+        acc = acc - l[d.bw.isrc[q]] * z[n - 1 - (s + d.bw.ik[q])];
+      z[n - 1 - lr] = acc / l[d.bw.dsrc[lr]];
+    }
+  }
 }
 
 }  // namespace ml

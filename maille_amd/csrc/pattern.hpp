@@ -89,4 +89,27 @@ struct TriLayout {
 TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const int32_t* src,
                      const int32_t* dsrc, int max_rows, int xcap, int icap, std::string* err);
 
+// Every index array the device kernels of one operator read, built on the host from the lower
+// pattern: the 32-bit CSR (IC(0)), the sliced-ELL symmetric layout (SpMV) and the two
+// triangular-solve layouts. The backward solve L^T z = y runs in logical rows l = n-1-j: row l
+// holds the entries l_ij, i > j, of column j of L in decreasing i, at logical column n-1-i.
+struct DeviceLayouts {
+  int64_t n = 0, nnz = 0, depth = 0, slices = 0;
+  int max_row = 0;  // longest row of L, diagonal included
+  std::vector<int32_t> off32, col32;
+  std::vector<int32_t> soff, scol, slen, ssrc;
+  TriLayout fw, bw;
+};
+// Returns an empty string or the reason the pattern cannot be laid out (row too long).
+std::string device_layouts(const LowerPattern& lp, int xcap, int icap, DeviceLayouts* out);
+
+// Verifies every bound the kernels rely on (index ranges, block capacities, coverage and order
+// of each row's entries, topological block order); empty string when all hold.
+std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, int xcap, int icap);
+
+// The preconditioner z = (L L^T)^-1 r computed on the host through the triangular-solve
+// layouts, block by block in processing order, each row summed in the order the device sums
+// it. l holds the factor's values in lower-CSR order.
+void layout_precondition(const DeviceLayouts& d, const double* l, const double* r, double* z);
+
 }  // namespace ml

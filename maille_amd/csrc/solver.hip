@@ -59,11 +59,12 @@ struct DBuf {
     p = nullptr;
     n = 0;
   }
+  // n is the requested count, also when it is zero (one element is still allocated so that
+  // p is a valid pointer, but nothing may be read from it)
   void alloc(size_t count) {
     if (count == n && p) return;
     release();
-    if (count == 0) count = 1;
-    HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+    HIP_CHECK(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
     n = count;
   }
   void upload(const T* host, size_t count, hipStream_t s) {
@@ -229,7 +230,7 @@ struct Operator {
   DBuf<float> w_r32, w_p32, w_ap32, w_y32, w_z32;
   DBuf<mlk::PcgScalars> d_S, d_Sin;
   DBuf<int> d_flags;  // [0] abort, [1] IC(0) failure, [2..5] where an IC(0) wait gave up
-  int tri_grid64 = 0, tri_grid32 = 0, ic_grid = 0;
+  int tri_grid64 = 0, tri_grid32 = 0, ic_grid = 0, ic_grid64 = 0;
   long long wait_ticks = 0;
 
   ~Operator() {
@@ -241,70 +242,30 @@ struct Operator {
     if (!wait_ticks) wait_ticks = wait_limit();
   }
 
-  // Upload the pattern and every layout derived from it.
+  // Upload the pattern and every layout derived from it. The layouts are checked on the host
+  // against every bound the kernels rely on before any of them reaches the device.
   void set_pattern(const LowerPattern& lp) {
     init_stream();
+    DeviceLayouts d;
+    std::string err = device_layouts(lp, mlk::TRI_XCAP, mlk::TRI_ICAP, &d);
+    if (err.empty()) err = check_layouts(lp, d, mlk::TRI_XCAP, mlk::TRI_ICAP);
+    if (!err.empty()) throw ArgError(err);
     n = (int)lp.n();
     nnz = lp.nnz();
     h_off = lp.offsets;
     h_col = lp.cols;
-    depth = dag_depth(lp);
-    max_row = 0;
-    for (int i = 0; i < n; ++i) max_row = std::max<int>(max_row, (int)(lp.offsets[i + 1] - lp.offsets[i]));
-    std::vector<int> off32(n + 1);
-    for (int i = 0; i <= n; ++i) off32[i] = (int)lp.offsets[i];
-    d_off.upload_vec(off32, stream);
-    d_col.upload_vec(lp.cols, stream);
-    // sliced ELL
-    SellLayout s = sell_layout(lp);
-    slices = (int)s.slices;
-    std::vector<int> soff(s.slice_offsets.begin(), s.slice_offsets.end());
-    std::vector<int> slen(n), ssrc(s.src.begin(), s.src.end());
-    {
-      // row lengths: lower row + upper entries
-      std::vector<int> up(n, 0);
-      for (int i = 0; i < n; ++i)
-        for (int64_t p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) up[lp.cols[p]] += 1;
-      for (int j = 0; j < n; ++j) slen[j] = (int)(lp.offsets[j + 1] - lp.offsets[j]) + up[j];
-    }
-    d_soff.upload_vec(soff, stream);
-    d_scol.upload_vec(s.cols, stream);
-    d_slen.upload_vec(slen, stream);
-    d_ssrc.upload_vec(ssrc, stream);
-    // forward solve: strictly lower rows of L; backward solve: logical row l = n-1-j holds the
-    // rows i > j of column j of L in decreasing i (ascending logical column n-1-i)
+    depth = d.depth;
+    max_row = d.max_row;
+    slices = (int)d.slices;
+    d_off.upload_vec(d.off32, stream);
+    d_col.upload_vec(d.col32, stream);
+    d_soff.upload_vec(d.soff, stream);
+    d_scol.upload_vec(d.scol, stream);
+    d_slen.upload_vec(d.slen, stream);
+    d_ssrc.upload_vec(d.ssrc, stream);
     nf = nnz - n;
-    std::vector<int> foff(n + 1), fcol(nf), fsrc(nf), fdsrc(n);
-    for (int i = 0; i <= n; ++i) foff[i] = (int)(lp.offsets[i] - i);
-    for (int i = 0; i < n; ++i) {
-      int64_t o = foff[i];
-      for (int64_t p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p, ++o) {
-        fcol[o] = lp.cols[p];
-        fsrc[o] = (int)p;
-      }
-      fdsrc[i] = (int)(lp.offsets[i + 1] - 1);
-    }
-    TransposeLayout t = transpose_layout(lp);
-    std::vector<int> boff(n + 1, 0), bcol(nf), bsrc(nf), bdsrc(n);
-    for (int l = 0; l < n; ++l) {
-      int j = n - 1 - l;
-      boff[l + 1] = boff[l] + (int)(t.offsets[j + 1] - t.offsets[j]);
-    }
-    for (int l = 0; l < n; ++l) {
-      int j = n - 1 - l;
-      int64_t o = boff[l];
-      for (int64_t q = t.offsets[j + 1] - 1; q >= t.offsets[j]; --q, ++o) {
-        bcol[o] = n - 1 - t.rows[q];
-        bsrc[o] = (int)t.src[q];
-      }
-      bdsrc[l] = (int)(lp.offsets[j + 1] - 1);
-    }
-    std::string err;
-    fw.set(tri_layout(n, foff.data(), fcol.data(), fsrc.data(), fdsrc.data(), 64, mlk::TRI_XCAP,
-                      mlk::TRI_ICAP, &err), stream);
-    bw.set(tri_layout(n, boff.data(), bcol.data(), bsrc.data(), bdsrc.data(), 64, mlk::TRI_XCAP,
-                      mlk::TRI_ICAP, &err), stream);
-    if (!err.empty()) throw ArgError(err);
+    fw.set(d.fw, stream);
+    bw.set(d.bw, stream);
     // values and workspace
     d_a.alloc(nnz);
     d_l.alloc(nnz);
@@ -313,8 +274,10 @@ struct Operator {
     const int nblk = std::max(fw.nblk, bw.nblk);
     tri_grid64 = std::min<int>(nblk, resident_waves((const void*)mlk::k_trisolve<double, false, false>));
     tri_grid32 = std::min<int>(nblk, resident_waves((const void*)mlk::k_trisolve<float, false, false>));
-    ic_grid = std::min<int>(n, std::min(resident_waves((const void*)mlk::k_ic0<double, 512>),
-                                        resident_waves((const void*)mlk::k_ic0<float, 512>)));
+    ic_grid = std::min<int>(n, std::min(resident_waves((const void*)mlk::k_ic0<double, 512, false>),
+                                        resident_waves((const void*)mlk::k_ic0<float, 512, false>)));
+    ic_grid64 = std::min<int>(n, std::min(resident_waves((const void*)mlk::k_ic0<double, 64, false>),
+                                          resident_waves((const void*)mlk::k_ic0<float, 64, false>)));
     int64_t parts = std::max<int64_t>({grid_for(n), (int64_t)tri_grid64, (int64_t)tri_grid32, 1});
     w_partial.alloc(parts);
     d_S.alloc(1);
@@ -373,11 +336,11 @@ struct Operator {
     long long limit = wait_ticks;
     int nn = n;
     if (max_row <= 64)
-      launch_dataflow(mlk::k_ic0<T, 64>, ic_grid, stream, nn, off, col, a, l, shift, fail,
-                      abort_flag, limit, where);
+      launch_dataflow(mlk::k_ic0<T, 64, false>, ic_grid64, stream, nn, off, col, a, l, shift,
+                      fail, abort_flag, limit, where);
     else
-      launch_dataflow(mlk::k_ic0<T, 512>, ic_grid, stream, nn, off, col, a, l, shift, fail,
-                      abort_flag, limit, where);
+      launch_dataflow(mlk::k_ic0<T, 512, false>, ic_grid, stream, nn, off, col, a, l, shift,
+                      fail, abort_flag, limit, where);
     prof.end(P_IC0, ev, stream);
     int flags[6];
     HIP_CHECK(hipMemcpyAsync(flags, d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, stream));
@@ -426,8 +389,11 @@ struct Operator {
       T* xv = f32 ? (T*)t->xval32.p : (T*)t->xval.p;
       T* iv = f32 ? (T*)t->ival32.p : (T*)t->ival.p;
       T* dv = f32 ? (T*)t->diag32.p : (T*)t->diag.p;
-      mlk::k_gather<T, T><<<grid_for(t->xval.n), 256, 0, stream>>>((int64_t)t->xval.n, t->xsrc.p, l, xv);
-      mlk::k_gather<T, T><<<grid_for(t->ival.n), 256, 0, stream>>>((int64_t)t->ival.n, t->isrc.p, l, iv);
+      // a layout may have no external or no internal entries at all (a single block)
+      if (t->xval.n)
+        mlk::k_gather<T, T><<<grid_for(t->xval.n), 256, 0, stream>>>((int64_t)t->xval.n, t->xsrc.p, l, xv);
+      if (t->ival.n)
+        mlk::k_gather<T, T><<<grid_for(t->ival.n), 256, 0, stream>>>((int64_t)t->ival.n, t->isrc.p, l, iv);
       mlk::k_gather<T, T><<<grid_for(n), 256, 0, stream>>>((int64_t)n, t->dsrc.p, l, dv);
     }
     HIP_CHECK(hipGetLastError());
@@ -999,6 +965,29 @@ ml_status ml_pattern_graph(const ml_pattern* p, int64_t* offsets, int64_t* targe
 
 void ml_pattern_free(ml_pattern* p) { delete p; }
 
+ml_status ml_layout_verify(int64_t n, const int64_t* offsets, const int64_t* cols,
+                           const double* lvalues, const double* r, double* z, int64_t* stats) {
+  return guarded([&]() -> int {
+    LowerPattern lp = pattern_from_csr(n, offsets, cols);
+    DeviceLayouts d;
+    std::string err = device_layouts(lp, mlk::TRI_XCAP, mlk::TRI_ICAP, &d);
+    if (err.empty()) err = check_layouts(lp, d, mlk::TRI_XCAP, mlk::TRI_ICAP);
+    if (!err.empty()) throw ArgError(err);
+    if (stats) {
+      stats[0] = (int64_t)d.fw.bstart.size() - 1;
+      stats[1] = (int64_t)d.bw.bstart.size() - 1;
+      stats[2] = d.fw.max_block_ext;
+      stats[3] = d.fw.max_block_int;
+      stats[4] = d.bw.max_block_ext;
+      stats[5] = d.bw.max_block_int;
+      stats[6] = d.depth;
+      stats[7] = d.max_row;
+    }
+    if (lvalues && r && z) layout_precondition(d, lvalues, r, z);
+    return ML_OK;
+  });
+}
+
 ml_status ml_pcg_create(int64_t n, const int64_t* offsets, const int64_t* cols,
                         const double* values, ml_precision precision, ml_pcg** out) {
   return guarded([&]() -> int {
@@ -1119,21 +1108,21 @@ ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* tr
     nblocks[0] = op.fw.nblk;
     nblocks[1] = op.bw.nblk;
     if (!trace_fw || !trace_bw) return ML_OK;
-    if (capacity < 4 * (int64_t)std::max(op.fw.nblk, op.bw.nblk)) throw ArgError("trace too small");
+    if (capacity < 6 * (int64_t)std::max(op.fw.nblk, op.bw.nblk)) throw ArgError("trace too small");
     if (!op.factored) {
       int st = op.factor();
       if (st != ML_OK) return st;
     }
     DBuf<long long> tf, tb;
-    tf.alloc(4 * op.fw.nblk);
-    tb.alloc(4 * op.bw.nblk);
+    tf.alloc(6 * op.fw.nblk);
+    tb.alloc(6 * op.bw.nblk);
     HIP_CHECK(hipMemcpyAsync(op.w_r.p, r, sizeof(double) * op.n, hipMemcpyHostToDevice, op.stream));
     op.fill_sentinel<double>(op.w_y.p);
     op.fill_sentinel<double>(op.w_z.p);
     mlk::k_fill_i32<<<1, 64, 0, op.stream>>>(op.d_flags.p, 2, 0);
     op.precondition<double>(op.w_r.p, op.w_y.p, op.w_z.p, nullptr, nullptr, nullptr, tf.p, tb.p);
-    HIP_CHECK(hipMemcpyAsync(trace_fw, tf.p, sizeof(long long) * 4 * op.fw.nblk, hipMemcpyDeviceToHost, op.stream));
-    HIP_CHECK(hipMemcpyAsync(trace_bw, tb.p, sizeof(long long) * 4 * op.bw.nblk, hipMemcpyDeviceToHost, op.stream));
+    HIP_CHECK(hipMemcpyAsync(trace_fw, tf.p, sizeof(long long) * 6 * op.fw.nblk, hipMemcpyDeviceToHost, op.stream));
+    HIP_CHECK(hipMemcpyAsync(trace_bw, tb.p, sizeof(long long) * 6 * op.bw.nblk, hipMemcpyDeviceToHost, op.stream));
     HIP_CHECK(hipStreamSynchronize(op.stream));
     return ML_OK;
   });
