@@ -197,6 +197,7 @@ def main():
         "modes": {p: {"s_per_step": round(results[p]["s"], 4),
                       "iterations": int(results[p]["info"]["iterations"]),
                       "residual_replacements": int(results[p]["info"]["refinements"]),
+                      "escalated_to_fp64_operator": bool(results[p]["info"]["escalated"]),
                       "true_residual": results[p]["info"]["true_residual"],
                       "rel_err_vs_fp64": results[p]["rel_err_vs_fp64"]} for p in modes},
         "cpu_baseline": cpu,

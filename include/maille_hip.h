@@ -53,7 +53,7 @@ typedef enum ml_solver_kind { ML_AUTO = 0, ML_DIRECT = 1, ML_ITERATIVE = 2 } ml_
 #define ML_MAX_ITERATIONS 10000  /* iterative.rs:11 */
 
 typedef struct ml_solve_info {
-  int64_t iterations;        /* PCG iterations (summed over refinement sweeps in ML_MIXED) */
+  int64_t iterations;        /* PCG iterations (both phases of an escalated ML_MIXED solve) */
   double residual;           /* recursive relative residual |r|/|b| at exit (iterative.rs:246) */
   double true_residual;      /* |b - A x| / |b| computed in FP64 after the solve */
   int32_t shift_index;       /* index into the shift ladder {0,1e-3,1e-2,0.1,1,10} used */
@@ -61,6 +61,11 @@ typedef struct ml_solve_info {
   int64_t bad_element;       /* ML_INVALID_MESH: first offending element, else -1 */
   int32_t mesh_error;        /* ML_INVALID_MESH: 1 degenerate, 2 inverted, 3 curved Tet10 edge,
                                 4 non-finite coordinate, 5 unknown node, 6 bad material */
+  int32_t escalated;         /* ML_MIXED: 1 when a residual replacement stopped halving the FP64
+                                true residual and the solve continued from its iterate with the
+                                FP64 operator (FP32 preconditioner kept), else 0 */
+  int32_t device_retries;    /* dataflow launches (IC(0), triangular solves) that stalled and were
+                                repeated on a halved grid during this call, normally 0 */
   int32_t reserved;
   double seconds[6];         /* device wall time: assemble, factor, solve, reactions, stresses,
                                 host pattern build */
@@ -146,6 +151,11 @@ void ml_pcg_destroy(ml_pcg* h);
  * external value after staging), the ticks spent polling and the number of polling rounds. */
 ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* trace_fw,
                        int64_t* trace_bw, int64_t capacity);
+/* Diagnostics (no counterpart in the reference): the bound of every dataflow wait of this
+ * handle, in device wall-clock ticks (0 restores the default, 2 s). With a tiny bound every
+ * wait gives up at once and every launch still ends normally: the handle repeats the work on
+ * halved grids up to three times, then the call fails with ML_DEVICE_TIMEOUT. */
+ml_status ml_pcg_set_wait_limit(ml_pcg* h, int64_t ticks);
 
 /* ------------------------------------------------------------------ the static model */
 

@@ -38,6 +38,7 @@ EXPORTS = (
     "ml_pattern_ntargets", "ml_pattern_graph", "ml_pattern_free", "ml_layout_verify",
     "ml_pcg_create", "ml_pcg_solve", "ml_pcg_solve_device", "ml_pcg_multiply",
     "ml_pcg_precondition", "ml_pcg_factor", "ml_pcg_destroy", "ml_pcg_trace",
+    "ml_pcg_set_wait_limit",
     "ml_model_create", "ml_model_assemble", "ml_model_set_young", "ml_model_factor",
     "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_add_pressures",
     "ml_model_nfree", "ml_model_nnz", "ml_model_levels", "ml_model_csr",
@@ -82,13 +83,16 @@ class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int64), ("residual", C.c_double),
                 ("true_residual", C.c_double), ("shift_index", C.c_int32),
                 ("refinements", C.c_int32), ("bad_element", C.c_int64),
-                ("mesh_error", C.c_int32), ("reserved", C.c_int32),
+                ("mesh_error", C.c_int32), ("escalated", C.c_int32),
+                ("device_retries", C.c_int32), ("reserved", C.c_int32),
                 ("seconds", C.c_double * 6)]
 
     def as_dict(self):
         return {"iterations": self.iterations, "residual": self.residual,
                 "true_residual": self.true_residual, "shift_index": self.shift_index,
-                "refinements": self.refinements, "seconds": list(self.seconds)}
+                "refinements": self.refinements, "escalated": self.escalated,
+                "device_retries": self.device_retries,
+                "seconds": list(self.seconds)}
 
 
 _P = C.c_void_p
@@ -130,6 +134,7 @@ def lib():
             "ml_pcg_factor": (_int, [_P, _D, _I32]),
             "ml_pcg_destroy": (None, [_P]),
             "ml_pcg_trace": (_int, [_P, _D, _I, _I, _I, _i64]),
+            "ml_pcg_set_wait_limit": (_int, [_P, _i64]),
             "ml_model_create": (_int, [_i64, _D, _i64, _int, _I, _D, _D, _U8, _D, _int, _int,
                                        _PP, _S]),
             "ml_model_assemble": (_int, [_P]),
@@ -364,6 +369,10 @@ class Pcg:
         sh = C.c_int32(-1)
         _check(lib().ml_pcg_factor(self._h, _d(l), C.byref(sh)))
         return l, sh.value
+
+    def set_wait_limit(self, ticks: int = 0):
+        """Bound of the device's dataflow waits in wall-clock ticks (0: the default, 2 s)."""
+        _check(lib().ml_pcg_set_wait_limit(self._h, int(ticks)))
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -66,6 +66,14 @@ __device__ inline float lane_broadcast(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// Orders the calling wave's LDS accesses before and after this point with respect to the other
+// lanes of the same wave (the wave-level counterpart of __syncthreads).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Deterministic block sum (fixed tree), result valid in thread 0.
 template <int BS>
 __device__ inline double block_sum(double v) {
@@ -514,7 +522,19 @@ __global__ __launch_bounds__(256) void k_replaced(const double* __restrict__ par
   double rr = sum_partials(partial, count);
   if (threadIdx.x == 0) {
     const double t = sqrt(rr);
-    if (t > 0.5 * S->prev_true) S->allow = 0;
+    if (t > 0.5 * S->prev_true) {
+      S->allow = 0;
+      if (S->stall_stop && t > S->rscale) {
+        // the low-precision operator no longer reduces the true residual: hand the iterate
+        // back so that the solve continues with the FP64 operator
+        S->rnorm = t;
+        S->updates += 1;
+        S->update = 0;
+        S->done = 1;
+        S->status = 3;
+        return;
+      }
+    }
     S->prev_true = t;
     S->rmax = t;
     S->rnorm = t;
@@ -637,8 +657,10 @@ __global__ __launch_bounds__(256) void k_axpy_add(int n, double* __restrict__ x,
 // Rows are cut on the host into blocks of at most 64 consecutive rows that never straddle a
 // break of the dependency chain (bstart[]); each row's entries are split into those whose
 // column lies before its block (xcol/xval, "external") and those inside it (ik = column minus
-// block start, ival), both stored block after block. Wave w owns blocks w, w+G, ... (the
-// launch is cooperative, so every block of a lower index is finished or in progress).
+// block start, ival), both stored block after block. Wave w of the G-wave grid takes the blocks
+// at positions w, w+G, ... of the host's processing order border[], which is topological (a
+// block's external columns all lie in blocks earlier in border[]): with every wave resident the
+// lowest unfinished block always has its inputs and a running wave, so the solve progresses.
 // Per block: the wave stages the block's entries and the external values already published
 // into LDS with coalesced loads; every lane consumes, in order, the external values that were
 // ready. Then the rows are resolved one after the other: lane k finishes its row (polling any
@@ -660,7 +682,6 @@ __global__ __launch_bounds__(64) void k_trisolve(
     const int* __restrict__ xoff, const int* __restrict__ xcol, const T* __restrict__ xval,
     const int* __restrict__ ioff, const unsigned char* __restrict__ ik,
     const T* __restrict__ ival, const T* __restrict__ diag, const T* __restrict__ rhs, T* out,
-    const double* __restrict__ dotvec, double* __restrict__ partial,
     const PcgScalars* __restrict__ S, int* abort_flag, long long limit,
     long long* __restrict__ trace) {
 #pragma clang fp contract(off)
@@ -672,7 +693,6 @@ __global__ __launch_bounds__(64) void k_trisolve(
   __shared__ unsigned char sik[TRI_ICAP];
   __shared__ T own[64];
   const int lane = threadIdx.x;
-  double dotacc = 0.0;
   for (int kk = blockIdx.x; kk < nblocks; kk += gridDim.x) {
     const int b = border[kk];
     const int s = bstart[b];
@@ -701,7 +721,6 @@ __global__ __launch_bounds__(64) void k_trisolve(
     const int pl = REV ? n - 1 - l : l;
     int p = 0, x1 = 0, ip = 0, i1 = 0;
     T acc = T(0), rl = T(0), dl = T(1);
-    double dv = 0.0;
     if (active) {
       p = xoff[l] - xb;
       x1 = xoff[l + 1] - xb;
@@ -709,10 +728,9 @@ __global__ __launch_bounds__(64) void k_trisolve(
       i1 = ioff[l + 1] - ib;
       rl = rhs[pl];
       dl = diag[l];
-      if (dotvec) dv = dotvec[pl];
       acc = SUBSEQ ? rl : T(0);
     }
-    __syncthreads();
+    wave_sync();
     // external values that were ready, in order
     for (bool go = p < x1; go;) {
       T aa[8];
@@ -809,7 +827,6 @@ __global__ __launch_bounds__(64) void k_trisolve(
         y = SUBSEQ ? acc / dl : (rl - acc) / dl;
         own[k] = y;
         publish(out + pl, y);
-        dotacc += dv * (double)y;
       }
       y = lane_broadcast(y, k);
       if (ext_done && lane > k && nk == k) {
@@ -827,49 +844,28 @@ __global__ __launch_bounds__(64) void k_trisolve(
       trace[6 * (long long)b + 4] = waited;
       trace[6 * (long long)b + 5] = rounds;
     }
-    __syncthreads();
-  }
-  if (partial) {
-    double v = dotacc;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (lane == 0) partial[blockIdx.x] = v;
+    wave_sync();
   }
 }
 
 // ------------------------------------------------------------------ IC(0)
 // Csr::incomplete_cholesky (iterative.rs:146-170), one wavefront per row, rows i = w, w+G, ...
+// (row i depends only on rows j < i, so with every wave resident the launch progresses).
 // Row i: for each entry p (column j, ascending) the sum over row j's entries whose column k is
 // also a computed column of row i is taken in ascending k order, exactly the reference's sum
 // over its dense work vector; l_ij = (a_ij - s) / l_jj; then the pivot a_ii(1+shift) - sum of
 // squares must exceed 1e-12 a_ii. Rows of L are published entry by entry; a failed pivot
 // raises *fail and every later row only writes zeros so the launch drains quickly.
-//
-// Staged path (STAGED, rows of at most 64 entries): before the sequential loop over p the wave
-// loads, with flattened coalesced reads, the bounds and diagonal of every row j, all of their
-// columns, finds which are columns of row i (binary search) and keeps the hits in LDS in the
-// reference's summation order with their l values read at device scope; the loop over p then
-// only touches LDS (and polls the few values that were not published yet). Rows with more
-// hits than the LDS holds take the generic path, which reads row j entry by entry.
-constexpr int IC0_HCAP = 512;
-
-template <typename T, int MAXROW, bool STAGED>
+template <typename T, int MAXROW>
 __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
                                             const int* __restrict__ col,
                                             const T* __restrict__ a, T* l, T shift,
                                             int* fail, int* abort_flag, long long limit,
                                             int* where) {
 #pragma clang fp contract(off)
-  typedef typename Bits<T>::U U;
   __shared__ int scol[MAXROW];
   __shared__ T sw[MAXROW];
   __shared__ T sa[MAXROW];
-  constexpr int SQ = STAGED ? MAXROW : 1;
-  constexpr int SH = STAGED ? IC0_HCAP : 1;
-  __shared__ int sjs[SQ], sjd[SQ], eoff[SQ + 1], hcount[SQ], hoff[SQ + 1];
-  __shared__ T sljj[SQ];
-  __shared__ unsigned char hr[SH];
-  __shared__ int hidx[SH];
-  __shared__ T hv[SH];
   const int lane = threadIdx.x;
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const int start = off[i], dg = off[i + 1] - 1, len = dg - start;
@@ -879,103 +875,16 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
       if (len > MAXROW && lane == 0)
         __hip_atomic_store(fail, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       for (int t = lane; t <= len; t += 64) publish(l + start + t, T(0));
-      continue;
-    }
-    for (int t = lane; t < len; t += 64) {
-      scol[t] = col[start + t];
-      sa[t] = a[start + t];
-    }
-    __syncthreads();
-    bool staged = false;
-    if constexpr (STAGED) {
-      for (int q = lane; q < len; q += 64) {
+    } else {
+      for (int t = lane; t < len; t += 64) {
+        scol[t] = col[start + t];
+        sa[t] = a[start + t];
+      }
+      wave_sync();
+      for (int q = 0; q < len; ++q) {
         const int j = scol[q];
         const int js = off[j], jd = off[j + 1] - 1;
-        sjs[q] = js;
-        sjd[q] = jd;
-        hcount[q] = 0;
-        sljj[q] = from_bits<T>(__hip_atomic_load(reinterpret_cast<U*>(l + jd), __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT));
-      }
-      __syncthreads();
-      if (lane == 0) {
-        eoff[0] = 0;
-        for (int q = 0; q < len; ++q) eoff[q + 1] = eoff[q] + (sjd[q] - sjs[q]);
-      }
-      __syncthreads();
-      const int total = eoff[len];
-      int nh = 0;
-      for (int fb = 0; fb < total; fb += 64) {
-        const int f = fb + lane;
-        bool hit = false;
-        int q = 0, r = 0, idx = 0;
-        if (f < total) {
-          int lo = 0, hi = len;  // q: last with eoff[q] <= f
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (eoff[mid] <= f) lo = mid;
-            else hi = mid;
-          }
-          q = lo;
-          idx = sjs[q] + (f - eoff[q]);
-          const int k = col[idx];
-          int lo2 = 0, hi2 = q;  // k among the columns scol[0..q)
-          while (lo2 < hi2) {
-            const int mid = (lo2 + hi2) >> 1;
-            if (scol[mid] < k) lo2 = mid + 1;
-            else hi2 = mid;
-          }
-          r = lo2;
-          hit = r < q && scol[r] == k;
-        }
-        const unsigned long long mask = __ballot(hit);
-        const int pos = nh + __popcll(mask & ((1ull << lane) - 1ull));
-        if (hit && pos < IC0_HCAP) {
-          hr[pos] = (unsigned char)r;
-          hidx[pos] = idx;
-          hv[pos] = from_bits<T>(__hip_atomic_load(reinterpret_cast<U*>(l + idx), __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT));
-          atomicAdd(&hcount[q], 1);
-        }
-        nh += __popcll(mask);
-      }
-      staged = nh <= IC0_HCAP;
-      __syncthreads();
-      if (lane == 0) {
-        hoff[0] = 0;
-        for (int q = 0; q < len; ++q) hoff[q + 1] = hoff[q] + hcount[q];
-      }
-      __syncthreads();
-    }
-    for (int q = 0; q < len; ++q) {
-      T sacc = T(0);
-      int jd;
-      T ljj;
-      if (staged) {
-        const int h1 = hoff[q + 1];
-        for (int hb = hoff[q]; hb < h1; hb += 64) {
-          const int h = hb + lane;
-          T term = T(0);
-          const bool has = h < h1;
-          if (has) {
-            T v = hv[h];
-            if (is_sentinel(v)) v = await_value(l + hidx[h], abort_flag, limit);
-            term = v * sw[hr[h]];
-          }
-          unsigned long long mask = __ballot(has);
-          while (mask) {
-            const int bl = __builtin_ctzll(mask);
-            sacc = sacc + lane_broadcast(term, bl);
-            mask &= mask - 1;
-          }
-        }
-        jd = sjd[q];
-        ljj = sljj[q];
-        if (is_sentinel(ljj)) ljj = await_value(l + jd, abort_flag, limit);
-      } else {
-        const int j = scol[q];
-        const int js = off[j];
-        jd = off[j + 1] - 1;
+        T sacc = T(0);
         for (int base = js; base < jd; base += 64) {
           const int idx = base + lane;
           T term = T(0);
@@ -1001,27 +910,27 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
             mask &= mask - 1;
           }
         }
-        ljj = await_value(l + jd, abort_flag, limit);
+        const T ljj = await_value(l + jd, abort_flag, limit);
+        if (lane == 0 && __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
+            __hip_atomic_load(where, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) {
+          __hip_atomic_store(where, i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(where + 1, j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(where + 2, (int)blockIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(where + 3, (int)gridDim.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        const T lij = (sa[q] - sacc) / ljj;
+        if (lane == 0) sw[q] = lij;
+        wave_sync();
       }
-      if (lane == 0 && __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
-          __hip_atomic_load(where, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) {
-        __hip_atomic_store(where, i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(where + 1, scol[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(where + 2, (int)blockIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(where + 3, (int)gridDim.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      const T lij = (sa[q] - sacc) / ljj;
-      if (lane == 0) sw[q] = lij;
-      __syncthreads();
+      T squares = T(0);
+      for (int q = 0; q < len; ++q) squares = squares + sw[q] * sw[q];
+      const T pivot = a[dg] * (T(1) + shift) - squares;
+      const bool bad = isnan(pivot) || pivot <= T(1e-12) * a[dg];
+      if (bad && lane == 0) __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int t = lane; t < len; t += 64) publish(l + start + t, sw[t]);
+      if (lane == 0) publish(l + dg, bad ? T(0) : (T)sqrt(pivot));
     }
-    T squares = T(0);
-    for (int q = 0; q < len; ++q) squares = squares + sw[q] * sw[q];
-    const T pivot = a[dg] * (T(1) + shift) - squares;
-    const bool bad = isnan(pivot) || pivot <= T(1e-12) * a[dg];
-    if (bad && lane == 0) __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int t = lane; t < len; t += 64) publish(l + start + t, sw[t]);
-    if (lane == 0) publish(l + dg, bad ? T(0) : (T)sqrt(pivot));
-    __syncthreads();
+    wave_sync();
   }
 }
 

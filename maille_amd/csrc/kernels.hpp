@@ -22,7 +22,7 @@ struct PcgScalars {
   int64_t it;        // completed iterations
   int64_t maxit;
   int32_t done;      // 1 once converged or failed: every later kernel returns at once
-  int32_t status;    // 0 ok, 1 singular, 2 not converged
+  int32_t status;    // 0 ok, 1 singular, 2 not converged, 3 stalled (stall_stop)
   // reliable updates (FP32 operator): the recursive residual is replaced by the FP64 true
   // residual b - A x whenever it has dropped by `delta` since the last replacement
   int32_t update;    // 1 while a replacement is due in the current iteration
@@ -31,6 +31,8 @@ struct PcgScalars {
   double rmax;       // |r| at the last replacement
   double prev_true;  // true |r| at the last replacement
   int64_t updates;   // replacements done
+  int32_t stall_stop;  // 1: stop with status 3 when a replacement fails to halve |b - A x|
+  int32_t pad;
 };
 
 }  // namespace mlk

@@ -1,7 +1,6 @@
 #include "pattern.hpp"
 
 #include <algorithm>
-#include <array>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -522,21 +521,22 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, int xc
   if (!err.empty()) return err;
   for (i64 i = 0; i < n; ++i)
     if (d.fw.dsrc[i] != lp.offsets[i + 1] - 1) return "forward layout: wrong diagonal";
-  // backward rows, restated independently of device_layouts: entries (i, j) of L with i > j
-  // sorted by logical row n-1-j, then ascending logical column n-1-i
-  std::vector<std::array<int64_t, 3>> ent;
-  ent.reserve(nf);
-  for (i64 i = 0; i < n; ++i)
-    for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p)
-      ent.push_back({n - 1 - (i64)lp.cols[p], n - 1 - i, p});
-  std::sort(ent.begin(), ent.end());
+  // backward rows, restated independently of device_layouts: entry (i, j) of L, i > j, goes
+  // to logical row n-1-j; visiting i downwards appends each row's entries in ascending logical
+  // column n-1-i
   std::vector<int32_t> boff(n + 1, 0), bcol(nf), bsrc(nf);
-  for (i64 q = 0; q < nf; ++q) {
-    boff[ent[q][0] + 1] += 1;
-    bcol[q] = (int32_t)ent[q][1];
-    bsrc[q] = (int32_t)ent[q][2];
+  for (i64 i = 0; i < n; ++i)
+    for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) boff[n - lp.cols[p]] += 1;
+  for (i64 r = 0; r < n; ++r) boff[r + 1] += boff[r];
+  {
+    std::vector<int32_t> fill(boff.begin(), boff.end() - 1);
+    for (i64 i = n - 1; i >= 0; --i)
+      for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) {
+        const i64 q = fill[n - 1 - lp.cols[p]]++;
+        bcol[q] = (int32_t)(n - 1 - i);
+        bsrc[q] = (int32_t)p;
+      }
   }
-  for (i64 l = 0; l < n; ++l) boff[l + 1] += boff[l];
   err = check_tri("backward", d.bw, n, nnz, xcap, icap, boff, bcol, bsrc);
   if (!err.empty()) return err;
   for (i64 l = 0; l < n; ++l)

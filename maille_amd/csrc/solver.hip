@@ -132,11 +132,16 @@ int device_cus() {
 // the hardware actually holds (measured on gfx950: 26/CU reported for 6 KB of LDS, 25 hold;
 // 13/CU for 12 KB, 12 hold), and a dataflow kernel whose waves are not all resident waits
 // forever, so one workgroup per CU is kept in reserve.
-int resident_waves(const void* kernel) {
+// Waves of a 64-thread persistent dataflow kernel launched per CU: three quarters of the
+// occupancy the runtime reports. The reported occupancy of LDS-heavy kernels has not always
+// been co-resident on gfx950, and the dataflow kernels need every wave of their grid resident;
+// the margin costs little (their speed hardly changes between 7 and 13 waves per CU). A
+// dataflow launch that stalls anyway ends through its bounded waits, and the caller then
+// repeats the work on a smaller grid (Operator::shrink).
+int dataflow_grid(const void* kernel) {
   int per_cu = 0;
   HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0));
-  per_cu = std::max(1, per_cu - 1);
-  return per_cu * device_cus();
+  return std::max(1, (3 * per_cu) / 4) * device_cus();
 }
 
 // The bound of every dataflow wait: 2 s of the device's steady wall clock (wall_clock64 ticks
@@ -149,12 +154,11 @@ long long wait_limit() {
   return 2000ll * khz;
 }
 
-// Launches a persistent dataflow kernel (64-thread blocks that wait on each other's results)
-// cooperatively, so the runtime guarantees that every block of the grid is resident at once;
-// `grid` is lowered (and remembered) while the device refuses it as too large.
+// Launches a persistent dataflow kernel (64-thread workgroups, one wave each) cooperatively, so
+// that the runtime admits the grid only if it can hold all of it at once; `grid` is lowered
+// (and remembered) while the device refuses it as too large.
 template <typename... Args>
-void launch_dataflow(void (*kernel)(Args...), int& grid, hipStream_t stream,
-                     Args... args) {
+void launch_dataflow(void (*kernel)(Args...), int& grid, hipStream_t stream, Args... args) {
   void* params[] = {(void*)&args...};
   for (;;) {
     hipError_t e = hipLaunchCooperativeKernel((const void*)kernel, dim3(grid), dim3(64), params,
@@ -230,7 +234,11 @@ struct Operator {
   DBuf<float> w_r32, w_p32, w_ap32, w_y32, w_z32;
   DBuf<mlk::PcgScalars> d_S, d_Sin;
   DBuf<int> d_flags;  // [0] abort, [1] IC(0) failure, [2..5] where an IC(0) wait gave up
+  // grids of the dataflow kernels: the full ones, and the ones in use (full >> shrinks)
+  int full_tri64 = 0, full_tri32 = 0, full_ic = 0, full_ic64 = 0;
   int tri_grid64 = 0, tri_grid32 = 0, ic_grid = 0, ic_grid64 = 0;
+  int shrinks = 0;   // grid halvings after a stalled dataflow launch
+  int retries = 0;   // work repeated on a smaller grid (reported in ml_solve_info)
   long long wait_ticks = 0;
 
   ~Operator() {
@@ -272,14 +280,15 @@ struct Operator {
     d_sval.alloc(d_scol.n);
     for (auto* b : {&w_x, &w_r, &w_p, &w_ap, &w_y, &w_z, &w_b, &w_d, &w_ax}) b->alloc(n);
     const int nblk = std::max(fw.nblk, bw.nblk);
-    tri_grid64 = std::min<int>(nblk, resident_waves((const void*)mlk::k_trisolve<double, false, false>));
-    tri_grid32 = std::min<int>(nblk, resident_waves((const void*)mlk::k_trisolve<float, false, false>));
-    ic_grid = std::min<int>(n, std::min(resident_waves((const void*)mlk::k_ic0<double, 512, false>),
-                                        resident_waves((const void*)mlk::k_ic0<float, 512, false>)));
-    ic_grid64 = std::min<int>(n, std::min(resident_waves((const void*)mlk::k_ic0<double, 64, false>),
-                                          resident_waves((const void*)mlk::k_ic0<float, 64, false>)));
-    int64_t parts = std::max<int64_t>({grid_for(n), (int64_t)tri_grid64, (int64_t)tri_grid32, 1});
-    w_partial.alloc(parts);
+    full_tri64 = std::min<int>(nblk, dataflow_grid((const void*)mlk::k_trisolve<double, false, false>));
+    full_tri32 = std::min<int>(nblk, dataflow_grid((const void*)mlk::k_trisolve<float, false, false>));
+    full_ic = std::min<int>(n, std::min(dataflow_grid((const void*)mlk::k_ic0<double, 512>),
+                                        dataflow_grid((const void*)mlk::k_ic0<float, 512>)));
+    full_ic64 = std::min<int>(n, std::min(dataflow_grid((const void*)mlk::k_ic0<double, 64>),
+                                          dataflow_grid((const void*)mlk::k_ic0<float, 64>)));
+    shrinks = 0;
+    set_grids();
+    w_partial.alloc(std::max<int64_t>(grid_for(n), 1));
     d_S.alloc(1);
     d_Sin.alloc(1);
     d_flags.alloc(6);
@@ -336,11 +345,11 @@ struct Operator {
     long long limit = wait_ticks;
     int nn = n;
     if (max_row <= 64)
-      launch_dataflow(mlk::k_ic0<T, 64, false>, ic_grid64, stream, nn, off, col, a, l, shift,
-                      fail, abort_flag, limit, where);
+      launch_dataflow(mlk::k_ic0<T, 64>, ic_grid64, stream, nn, off, col, a, l, shift, fail,
+                      abort_flag, limit, where);
     else
-      launch_dataflow(mlk::k_ic0<T, 512, false>, ic_grid, stream, nn, off, col, a, l, shift,
-                      fail, abort_flag, limit, where);
+      launch_dataflow(mlk::k_ic0<T, 512>, ic_grid, stream, nn, off, col, a, l, shift, fail,
+                      abort_flag, limit, where);
     prof.end(P_IC0, ev, stream);
     int flags[6];
     HIP_CHECK(hipMemcpyAsync(flags, d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, stream));
@@ -356,6 +365,23 @@ struct Operator {
   }
 
   // Pcg::new (iterative.rs:201-207): the first shift of the ladder whose IC(0) succeeds.
+  void set_grids() {
+    tri_grid64 = std::max(1, full_tri64 >> shrinks);
+    tri_grid32 = std::max(1, full_tri32 >> shrinks);
+    ic_grid = std::max(1, full_ic >> shrinks);
+    ic_grid64 = std::max(1, full_ic64 >> shrinks);
+  }
+
+  // After a dataflow launch stalled (its bounded waits expired): halve the dataflow grids so
+  // that the work can be repeated; false once they have been halved three times.
+  bool shrink() {
+    if (shrinks >= 3) return false;
+    ++shrinks;
+    ++retries;
+    set_grids();
+    return true;
+  }
+
   int factor() {
     static const double SHIFTS[6] = {0.0, 1e-3, 1e-2, 1e-1, 1.0, 10.0};
     factored = false;
@@ -363,6 +389,7 @@ struct Operator {
     int st = ML_SINGULAR;
     for (int s = 0; s < 6; ++s) {
       st = run_ic0<double>(d_a.p, d_l.p, SHIFTS[s]);
+      if (st == ML_DEVICE_TIMEOUT && shrink()) { --s; continue; }
       if (st == ML_DEVICE_TIMEOUT) return st;
       if (st == ML_OK) { shift_index = s; break; }
     }
@@ -372,6 +399,7 @@ struct Operator {
       int st32 = ML_SINGULAR;
       for (int s = 0; s < 6; ++s) {
         st32 = run_ic0<float>(d_a32.p, d_l32.p, (float)SHIFTS[s]);
+        if (st32 == ML_DEVICE_TIMEOUT && shrink()) { --s; continue; }
         if (st32 == ML_DEVICE_TIMEOUT) return st32;
         if (st32 == ML_OK) break;
       }
@@ -400,20 +428,16 @@ struct Operator {
   }
 
   // ---------------------------------------------------------------- kernels
-  // z = (L L^T)^-1 r through y; partial gets r.z per wave when dotvec is given
+  // z = (L L^T)^-1 r through y
   template <typename T>
-  void precondition(const T* r, T* y, T* z, const double* dotvec, double* partial,
-                    const mlk::PcgScalars* S, long long* trace_fw = nullptr,
-                    long long* trace_bw = nullptr) {
+  void precondition(const T* r, T* y, T* z, const mlk::PcgScalars* S,
+                    long long* trace_fw = nullptr, long long* trace_bw = nullptr) {
     const bool f32 = std::is_same<T, float>::value;
     int& grid = f32 ? tri_grid32 : tri_grid64;
     hipEvent_t ev = prof.begin(stream);
     int nn = n;
-    const double* nodot = nullptr;
-    double* nopart = nullptr;
     long long limit = wait_ticks;
-    auto run = [&](TriDev& t, const T* rhs, T* out, const double* dv, double* part, bool fwd,
-                   long long* trace) {
+    auto run = [&](TriDev& t, const T* rhs, T* out, bool fwd, long long* trace) {
       const T* xv = f32 ? (const T*)t.xval32.p : (const T*)t.xval.p;
       const T* iv = f32 ? (const T*)t.ival32.p : (const T*)t.ival.p;
       const T* dg = f32 ? (const T*)t.diag32.p : (const T*)t.diag.p;
@@ -421,16 +445,16 @@ struct Operator {
       if (fwd)
         launch_dataflow(mlk::k_trisolve<T, false, false>, grid, stream, nn, nb,
                         (const int*)t.border.p, (const int*)t.bstart.p, (const int*)t.xoff.p, (const int*)t.xcol.p, xv,
-                        (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out, dv,
-                        part, S, d_flags.p, limit, trace);
+                        (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out, S,
+                        d_flags.p, limit, trace);
       else
         launch_dataflow(mlk::k_trisolve<T, true, true>, grid, stream, nn, nb,
                         (const int*)t.border.p, (const int*)t.bstart.p, (const int*)t.xoff.p, (const int*)t.xcol.p, xv,
-                        (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out, dv,
-                        part, S, d_flags.p, limit, trace);
+                        (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out, S,
+                        d_flags.p, limit, trace);
     };
-    run(fw, r, y, nodot, nopart, true, trace_fw);
-    run(bw, (const T*)y, z, dotvec, partial, false, trace_bw);
+    run(fw, r, y, true, trace_fw);
+    run(bw, (const T*)y, z, false, trace_bw);
     prof.end(f32 ? P_TRI32 : P_TRI64, ev, stream);
   }
 
@@ -440,11 +464,12 @@ struct Operator {
     HIP_CHECK(hipGetLastError());
   }
 
-  void init_scalars(mlk::PcgScalars* S, int64_t maxit, bool reliable) {
+  void init_scalars(mlk::PcgScalars* S, int64_t maxit, bool reliable, bool stall_stop) {
     mlk::PcgScalars h;
     std::memset(&h, 0, sizeof(h));
     h.maxit = maxit;
     h.allow = reliable ? 1 : 0;
+    h.stall_stop = stall_stop ? 1 : 0;
     h.delta = 0.1;
     // blocking copy from the stack: the stream must not hold kernels still reading S
     HIP_CHECK(hipStreamSynchronize(stream));
@@ -469,14 +494,18 @@ struct Operator {
     prof.end(f32 ? P_SPMV32 : P_SPMV64, ev, stream);
   }
 
+  static constexpr int PCG_STALLED = 100;  // internal: the FP32 operator stopped helping
+
   // Preconditioned CG (iterative.rs:210-248) on the device. TA: arithmetic of the operator and
   // of p and Ap; TM: arithmetic of the IC(0) factor and z. x, r and every dot product are
   // FP64. With an FP32 operator the recursive residual is replaced by the FP64 true residual
-  // b - A x each time it has dropped tenfold (reliable updates), until replacements stop
-  // helping; convergence is the reference's |r| <= tol |b| on the residual in hand. b, x, r
-  // are device vectors in solver numbering; returns a status and fills *hS.
+  // b - A x each time it has dropped tenfold (reliable updates); when a replacement fails to
+  // halve the true residual the solve returns PCG_STALLED with its iterate, for the caller to
+  // continue with the FP64 operator. Convergence is the reference's |r| <= tol |b| on the
+  // residual in hand. x0 = 0 (the reference), or the x given when `warm` (r = b - A x in FP64).
+  // b, x, r are device vectors in solver numbering; returns a status and fills *hS.
   template <typename TA, typename TM>
-  int pcg(const double* b, double* x, double* r, double tol, int64_t maxit,
+  int pcg(const double* b, double* x, double* r, double tol, int64_t maxit, bool warm,
           mlk::PcgScalars* S, mlk::PcgScalars* hS) {
     const bool a32 = std::is_same<TA, float>::value;
     const bool m32 = std::is_same<TM, float>::value;
@@ -487,20 +516,25 @@ struct Operator {
     TM* rlow = m32 ? (TM*)w_r32.p : nullptr;
     const TM* rin = m32 ? (const TM*)w_r32.p : (const TM*)r;
     const int vb = grid_for(n);
-    // partial sums written by the backward solve: one per wave of its (possibly lowered) grid
-    auto tri_parts = [&]() { return m32 ? tri_grid32 : tri_grid64; };
     double* part = w_partial.p;
-    init_scalars(S, maxit, a32);
-    mlk::k_fill_f64<<<vb, 256, 0, stream>>>(x, n, 0.0);
-    HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);  // abort flag
+    init_scalars(S, maxit, a32, a32);
+    if (warm) {
+      spmv<double>(x, w_ax.p, nullptr, nullptr);
+      mlk::k_residual<<<vb, 256, 0, stream>>>(n, b, w_ax.p, r, part);
+    } else {
+      mlk::k_fill_f64<<<vb, 256, 0, stream>>>(x, n, 0.0);
+      HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    }
     mlk::k_dot<double, double><<<vb, 256, 0, stream>>>(n, b, b, part);
     mlk::k_init_norm<<<1, 256, 0, stream>>>(part, vb, S, tol);
     HIP_CHECK(hipGetLastError());
     if (m32) mlk::k_convert<double, float><<<vb, 256, 0, stream>>>(n, r, (float*)rlow);
     fill_sentinel<TM>(y);
     fill_sentinel<TM>(z);
-    precondition<TM>(rin, y, z, r, part, S);
-    mlk::k_init_rz<<<1, 256, 0, stream>>>(part, tri_parts(), S);
+    precondition<TM>(rin, y, z, S);
+    mlk::k_dot<double, TM><<<vb, 256, 0, stream>>>(n, r, z, part);
+    mlk::k_init_rz<<<1, 256, 0, stream>>>(part, vb, S);
     mlk::k_convert<TM, TA><<<vb, 256, 0, stream>>>(n, z, p);
     HIP_CHECK(hipGetLastError());
     *hS = read_scalars(S);
@@ -520,8 +554,11 @@ struct Operator {
           mlk::k_replace_residual<TM><<<vb, 256, 0, stream>>>(n, S, b, w_ax.p, r, rlow, part);
           mlk::k_replaced<<<1, 256, 0, stream>>>(part, vb, S);
         }
-        precondition<TM>(rin, y, z, r, part, S);
-        mlk::k_beta<<<1, 256, 0, stream>>>(part, tri_parts(), S);
+        precondition<TM>(rin, y, z, S);
+        ev = prof.begin(stream);
+        mlk::k_dot<double, TM><<<vb, 256, 0, stream>>>(n, r, z, part);
+        prof.end(P_VEC, ev, stream);
+        mlk::k_beta<<<1, 256, 0, stream>>>(part, vb, S);
         ev = prof.begin(stream);
         mlk::k_update_p<TA, TM><<<vb, 256, 0, stream>>>(n, S, p, z);
         prof.end(P_VEC, ev, stream);
@@ -534,6 +571,7 @@ struct Operator {
       if (launched > maxit + 8) break;  // cannot happen: k_converged stops at maxit
       batch = std::min<int64_t>(32, batch * 2);
     }
+    if (hS->status == 3) return PCG_STALLED;
     if (hS->status == 1) return ML_SINGULAR;
     if (hS->status == 2) return ML_NOT_CONVERGED;
     return ML_OK;
@@ -560,14 +598,38 @@ struct Operator {
     if (maxit <= 0) maxit = ML_MAX_ITERATIONS;
     mlk::PcgScalars hS;
     int st;
-    if (precision == ML_FP64) st = pcg<double, double>(b, x, w_r.p, tol, maxit, d_S.p, &hS);
-    else if (precision == ML_MIXED) st = pcg<float, float>(b, x, w_r.p, tol, maxit, d_S.p, &hS);
-    else st = pcg<double, float>(b, x, w_r.p, tol, maxit, d_S.p, &hS);
+    int64_t it0 = 0, updates0 = 0;
+    int escalated = 0;
+    const int retries0 = retries;
+    for (;;) {
+      it0 = updates0 = 0;
+      escalated = 0;
+      if (precision == ML_FP64) {
+        st = pcg<double, double>(b, x, w_r.p, tol, maxit, false, d_S.p, &hS);
+      } else if (precision == ML_MIXED) {
+        st = pcg<float, float>(b, x, w_r.p, tol, maxit, false, d_S.p, &hS);
+        if (st == PCG_STALLED) {
+          // continue from the iterate with the FP64 operator (the preconditioner stays FP32)
+          it0 = hS.it;
+          updates0 = hS.updates;
+          escalated = 1;
+          st = pcg<double, float>(b, x, w_r.p, tol, std::max<int64_t>(1, maxit - it0), true,
+                                  d_S.p, &hS);
+        }
+      } else {
+        st = pcg<double, float>(b, x, w_r.p, tol, maxit, false, d_S.p, &hS);
+      }
+      // a stalled triangular solve: solve again from x0 = 0 on smaller grids
+      if (st == ML_DEVICE_TIMEOUT && shrink()) continue;
+      break;
+    }
     if (info) {
-      info->iterations = hS.it;
+      info->device_retries = retries - retries0;
+      info->iterations = it0 + hS.it;
       info->residual = hS.bnorm > 0 ? hS.rnorm / hS.bnorm : 0.0;
       info->shift_index = shift_index;
-      info->refinements = (int32_t)hS.updates;
+      info->refinements = (int32_t)(updates0 + hS.updates);
+      info->escalated = escalated;
       info->true_residual = (st == ML_OK && hS.bnorm > 0) ? true_residual(b, x, hS.bnorm)
                                                           : info->residual;
     }
@@ -683,11 +745,13 @@ struct Model {
 
   int solve_device(const double* f, double* u, double* R, ml_solve_info* info) {
     double t0 = now_s();
+    const int retries0 = op.retries;
     if (!op.factored) {
       int st = op.factor();
       if (info) info->shift_index = op.shift_index;
       if (st != ML_OK) return st;
     }
+    const int factor_retries = op.retries - retries0;
     double t1 = now_s();
     mlk::k_rhs<<<grid_for(plan.ndof), 256, 0, s()>>>(
         plan.ndof, d_rank.p, f, plan.cpl_slot.empty() ? nullptr : d_coupling.p, d_rhs.p);
@@ -702,6 +766,7 @@ struct Model {
     HIP_CHECK(hipStreamSynchronize(s()));
     have_u = true;
     if (info) {
+      info->device_retries += factor_retries;
       info->seconds[1] = t1 - t0;
       info->seconds[2] = t2 - t1;
       info->seconds[3] = now_s() - t2;
@@ -1073,7 +1138,7 @@ ml_status ml_pcg_precondition(ml_pcg* h, const double* r, double* z) {
     op.fill_sentinel<double>(op.w_y.p);
     op.fill_sentinel<double>(op.w_z.p);
     mlk::k_fill_i32<<<1, 64, 0, op.stream>>>(op.d_flags.p, 2, 0);
-    op.precondition<double>(op.w_r.p, op.w_y.p, op.w_z.p, nullptr, nullptr, nullptr);
+    op.precondition<double>(op.w_r.p, op.w_y.p, op.w_z.p, nullptr);
     HIP_CHECK(hipMemcpyAsync(z, op.w_z.p, sizeof(double) * op.n, hipMemcpyDeviceToHost, op.stream));
     HIP_CHECK(hipStreamSynchronize(op.stream));
     if (op.read_flag(0)) return ML_DEVICE_TIMEOUT;
@@ -1100,6 +1165,14 @@ ml_status ml_pcg_factor(ml_pcg* h, double* l, int32_t* shift_index) {
 
 void ml_pcg_destroy(ml_pcg* h) { delete h; }
 
+ml_status ml_pcg_set_wait_limit(ml_pcg* h, int64_t ticks) {
+  return guarded([&]() -> int {
+    if (!h || ticks < 0) throw ArgError("bad arguments");
+    h->op.wait_ticks = ticks > 0 ? ticks : wait_limit();
+    return ML_OK;
+  });
+}
+
 ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* trace_fw,
                        int64_t* trace_bw, int64_t capacity) {
   return guarded([&]() -> int {
@@ -1120,7 +1193,7 @@ ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* tr
     op.fill_sentinel<double>(op.w_y.p);
     op.fill_sentinel<double>(op.w_z.p);
     mlk::k_fill_i32<<<1, 64, 0, op.stream>>>(op.d_flags.p, 2, 0);
-    op.precondition<double>(op.w_r.p, op.w_y.p, op.w_z.p, nullptr, nullptr, nullptr, tf.p, tb.p);
+    op.precondition<double>(op.w_r.p, op.w_y.p, op.w_z.p, nullptr, tf.p, tb.p);
     HIP_CHECK(hipMemcpyAsync(trace_fw, tf.p, sizeof(long long) * 6 * op.fw.nblk, hipMemcpyDeviceToHost, op.stream));
     HIP_CHECK(hipMemcpyAsync(trace_bw, tb.p, sizeof(long long) * 6 * op.bw.nblk, hipMemcpyDeviceToHost, op.stream));
     HIP_CHECK(hipStreamSynchronize(op.stream));
@@ -1178,9 +1251,11 @@ ml_status ml_model_factor(ml_model* h, ml_solve_info* info) {
     if (!h) throw ArgError("null handle");
     if (!h->m.assembled) h->m.assemble();
     double t0 = now_s();
+    const int retries0 = h->m.op.retries;
     int st = h->m.op.factor();
     if (info) {
       info->shift_index = h->m.op.shift_index;
+      info->device_retries = h->m.op.retries - retries0;
       info->seconds[1] = now_s() - t0;
     }
     return st;

@@ -109,7 +109,8 @@ def test_mixed_solve_matches_oracle(gpu, name, precision):
     m = gpu.Model(p, precision=getattr(gpu, precision))
     u, r, info = m.solve()
     print(f"  iterations {info['iterations']} (oracle {ref['iterations']}), replacements "
-          f"{info['refinements']}, true residual {info['true_residual']:.1e}", flush=True)
+          f"{info['refinements']}, escalated {info['escalated']}, true residual "
+          f"{info['true_residual']:.1e}", flush=True)
     assert rel_l2(u, ref["displacements"]) <= 1e-8
 
 
@@ -157,3 +158,24 @@ def test_pressure_loads_match_oracle(gpu):
     f = m.loads()
     g = oracle.add_pressures(p.nodes, p.conn, p.pressures, p.f.copy())
     assert np.array_equal(f, g)
+
+
+def test_stalled_dataflow_waits_end_cleanly(gpu):
+    """A dataflow launch whose waits expire (forced from the host with a one-tick bound, so
+    every wave still ends normally) is repeated on smaller grids and then reported as a
+    device timeout; the handle solves correctly again once the bound is restored."""
+    p = PROBLEMS["jittered_cube_8"]
+    s = _oracle_system(p)
+    off, cols, vals = s.csr()
+    pcg = gpu.Pcg(gpu.Csr(off, cols, vals))
+    b = np.sin(0.37 * np.arange(s.n))
+    x_ref, info_ref = pcg.solve(b, info=True)
+    assert info_ref["device_retries"] == 0
+    pcg.set_wait_limit(1)
+    print("solve with a one-tick wait bound", flush=True)
+    with pytest.raises(gpu.DeviceError, match="device wait expired"):
+        pcg.solve(b)
+    pcg.set_wait_limit(0)
+    x, info = pcg.solve(b, info=True)
+    assert info["device_retries"] == 0
+    assert rel_l2(x, x_ref) <= 1e-10
