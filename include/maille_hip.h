@@ -202,8 +202,11 @@ int64_t ml_model_levels(const ml_model* m);
  * offsets[nfree+1], cols[nnz], values[nnz]; any pointer may be null. */
 ml_status ml_model_csr(ml_model* m, int64_t* order, int64_t* offsets, int64_t* cols,
                        double* values);
-/* Compliance sensitivity of SIMP-type moduli: for C = f.u with u the last solution,
- * dC/dE_e = -u_e^T (K_e/E_e) u_e (the adjoint of compliance is u itself). out[n_elements]. */
+/* Adjoint compliance sensitivity (no counterpart in the reference; the north-star backward()):
+ * for C = f.u with u the last solution of this model and homogeneous essential conditions,
+ * dC/dE_e = -u_e^T (K_e/E_e) u_e, since compliance is self-adjoint (adjoint state = u) and K_e
+ * is linear in E_e. out[n_elements] (host). A SIMP caller chains dE_e/drho_e itself.
+ * ML_INVALID_ARGUMENT before a solve or with nonzero imposed displacements. */
 ml_status ml_model_compliance_gradient(ml_model* m, double* out);
 void ml_model_destroy(ml_model* m);
 

@@ -443,6 +443,12 @@ class Model:
         young = _f64(young)
         _check(lib().ml_model_set_young(self._h, _d(young)))
 
+    def compliance_gradient(self):
+        """dC/dE_e of the compliance C = f.u at the last solution (adjoint: u itself)."""
+        out = np.zeros(self.ne)
+        _check(lib().ml_model_compliance_gradient(self._h, _d(out)))
+        return out
+
     def stresses(self, u):
         u = _f64(u)
         ngp = 1 if self.nen == 4 else 4

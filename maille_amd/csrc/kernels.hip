@@ -404,6 +404,37 @@ __global__ __launch_bounds__(256) void k_reactions(int64_t nfixed, const int* __
   reactions[g] = y - f[g];
 }
 
+// Compliance sensitivity, thread per element: C = f.u with K(E) u = f and homogeneous essential
+// conditions is self-adjoint (the adjoint state is u), and K_e is linear in E_e, so
+// dC/dE_e = -u_e^T (K_e / E_e) u_e.
+template <int NEN>
+__global__ __launch_bounds__(256) void k_compliance_gradient(int64_t ne,
+                                                             const int* __restrict__ conn,
+                                                             const double* __restrict__ ke,
+                                                             const double* __restrict__ young,
+                                                             const double* __restrict__ u,
+                                                             double* __restrict__ out) {
+#pragma clang fp contract(off)
+  constexpr int M = 3 * NEN;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double ue[M];
+#pragma unroll
+  for (int a = 0; a < NEN; ++a) {
+    const int64_t node = conn[e * NEN + a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ue[3 * a + c] = u[3 * node + c];
+  }
+  double s = 0.0;
+  for (int i = 0; i < M; ++i) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) t += ke[((int64_t)i * M + j) * ne + e] * ue[j];
+    s += ue[i] * t;
+  }
+  out[e] = -s / young[e];
+}
+
 // ------------------------------------------------------------------ SpMV (sliced ELL)
 // y = A x (Csr::multiply, iterative.rs:131-142). Lane j of a slice sums row j in the order
 // diagonal, lower entries, upper entries: the reference's accumulation order, so the FP64

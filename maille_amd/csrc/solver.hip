@@ -387,23 +387,24 @@ struct Operator {
     factored = false;
     shift_index = -1;
     int st = ML_SINGULAR;
-    for (int s = 0; s < 6; ++s) {
-      st = run_ic0<double>(d_a.p, d_l.p, SHIFTS[s]);
-      if (st == ML_DEVICE_TIMEOUT && shrink()) { --s; continue; }
-      if (st == ML_DEVICE_TIMEOUT) return st;
-      if (st == ML_OK) { shift_index = s; break; }
-    }
-    if (st != ML_OK) return ML_SINGULAR;
-    gather_factor<double>(d_l.p);
-    if (precision != ML_FP64) {
-      int st32 = ML_SINGULAR;
+    if (precision == ML_FP64) {
       for (int s = 0; s < 6; ++s) {
-        st32 = run_ic0<float>(d_a32.p, d_l32.p, (float)SHIFTS[s]);
-        if (st32 == ML_DEVICE_TIMEOUT && shrink()) { --s; continue; }
-        if (st32 == ML_DEVICE_TIMEOUT) return st32;
-        if (st32 == ML_OK) break;
+        st = run_ic0<double>(d_a.p, d_l.p, SHIFTS[s]);
+        if (st == ML_DEVICE_TIMEOUT && shrink()) { --s; continue; }
+        if (st == ML_DEVICE_TIMEOUT) return st;
+        if (st == ML_OK) { shift_index = s; break; }
       }
-      if (st32 != ML_OK) return ML_SINGULAR;
+      if (st != ML_OK) return ML_SINGULAR;
+      gather_factor<double>(d_l.p);
+    } else {
+      // the mixed modes only apply the FP32 factor: the ladder runs in FP32 arithmetic
+      for (int s = 0; s < 6; ++s) {
+        st = run_ic0<float>(d_a32.p, d_l32.p, (float)SHIFTS[s]);
+        if (st == ML_DEVICE_TIMEOUT && shrink()) { --s; continue; }
+        if (st == ML_DEVICE_TIMEOUT) return st;
+        if (st == ML_OK) { shift_index = s; break; }
+      }
+      if (st != ML_OK) return ML_SINGULAR;
       gather_factor<float>(d_l32.p);
     }
     factored = true;
@@ -647,6 +648,7 @@ struct Model {
   Operator op;
   int ngp = 1;
   DBuf<double> d_xyz, d_young, d_nu, d_ke, d_imposed, d_coupling, d_f, d_u, d_R, d_x, d_rhs;
+  DBuf<double> d_ulast;  // displacements of the last solve (adjoint state of the compliance)
   DBuf<int> d_conn, d_rank, d_cpl_dof, d_rea_dof;
   DBuf<int64_t> d_asm_off, d_cpl_off, d_rea_off, d_rea_item;
   DBuf<uint32_t> d_asm_slot, d_cpl_slot;
@@ -706,6 +708,7 @@ struct Model {
     d_rhs.alloc(plan.nfree);
     d_f.alloc(plan.ndof);
     d_u.alloc(plan.ndof);
+    d_ulast.alloc(plan.ndof);
     d_R.alloc(plan.ndof);
     HIP_CHECK(hipStreamSynchronize(st));
     // the host copies of the big lists are no longer needed
@@ -763,6 +766,8 @@ struct Model {
                                                            d_imposed.p, u);
     HIP_CHECK(hipGetLastError());
     if (R) reactions(u, f, R);
+    if (u != d_ulast.p)
+      HIP_CHECK(hipMemcpyAsync(d_ulast.p, u, sizeof(double) * plan.ndof, hipMemcpyDeviceToDevice, s()));
     HIP_CHECK(hipStreamSynchronize(s()));
     have_u = true;
     if (info) {
@@ -787,6 +792,18 @@ struct Model {
             nfixed, d_rea_dof.p, d_rea_off.p, d_rea_item.p, d_ke.p, d_conn.p, plan.n_elements, u,
             f, R);
     }
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // dC/dE_e of C = f.u at the last solution (homogeneous essential conditions only)
+  void compliance_gradient(double* out_dev) {
+    const int64_t ne = plan.n_elements;
+    if (plan.nen == 4)
+      mlk::k_compliance_gradient<4><<<grid_for(ne), 256, 0, s()>>>(ne, d_conn.p, d_ke.p, d_young.p,
+                                                                  d_ulast.p, out_dev);
+    else
+      mlk::k_compliance_gradient<10><<<grid_for(ne), 256, 0, s()>>>(ne, d_conn.p, d_ke.p, d_young.p,
+                                                                   d_ulast.p, out_dev);
     HIP_CHECK(hipGetLastError());
   }
 
@@ -1155,6 +1172,8 @@ ml_status ml_pcg_factor(ml_pcg* h, double* l, int32_t* shift_index) {
       if (st != ML_OK) return st;
     }
     if (l) {
+      if (op.precision != ML_FP64)  // the FP32 factor, widened
+        mlk::k_convert<float, double><<<grid_for(op.nnz), 256, 0, op.stream>>>(op.nnz, op.d_l32.p, op.d_l.p);
       HIP_CHECK(hipMemcpyAsync(l, op.d_l.p, sizeof(double) * op.nnz, hipMemcpyDeviceToHost, op.stream));
       HIP_CHECK(hipStreamSynchronize(op.stream));
     }
@@ -1323,10 +1342,21 @@ ml_status ml_model_csr(ml_model* h, int64_t* order, int64_t* offsets, int64_t* c
 }
 
 ml_status ml_model_compliance_gradient(ml_model* h, double* out) {
-  (void)h;
-  (void)out;
-  g_last_error = "not implemented";
-  return ML_INVALID_ARGUMENT;
+  return guarded([&]() -> int {
+    if (!h || !out) throw ArgError("null argument");
+    Model& m = h->m;
+    if (!m.have_u) throw ArgError("no solution yet: solve the model first");
+    for (double v : m.imposed)
+      if (v != 0.0)
+        throw ArgError("the compliance gradient needs homogeneous essential conditions");
+    DBuf<double> d_out;
+    d_out.alloc(m.plan.n_elements);
+    m.compliance_gradient(d_out.p);
+    HIP_CHECK(hipMemcpyAsync(out, d_out.p, sizeof(double) * m.plan.n_elements,
+                             hipMemcpyDeviceToHost, m.s()));
+    HIP_CHECK(hipStreamSynchronize(m.s()));
+    return ML_OK;
+  });
 }
 
 ml_status ml_model_add_pressures(ml_model* h, int64_t count, const int64_t* elements,

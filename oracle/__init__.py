@@ -92,6 +92,7 @@ def _declare(L):
         "or_reactions": (None, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _U8, _D, _D,
                                 _D]),
         "or_stresses": (None, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _D, _D]),
+        "or_compliance_gradient": (None, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _D, _D]),
         "or_add_pressures": (None, [C.c_int64, _D, C.c_int, _I, C.c_int64, _I, _I, _D, _D]),
     }
     for name, (res, args) in sig.items():
@@ -463,6 +464,18 @@ def stresses(nodes, conn, young, nu, u):
     out = np.zeros((ne, ngp, 9))
     lib().or_stresses(len(nodes), _d(nodes), ne, nen, _i(conn), _d(young), _d(nu), _d(u),
                       _d(out))
+    return out
+
+
+def compliance_gradient(nodes, conn, young, nu, u):
+    """dC/dE_e = -u_e^T (K_e/E_e) u_e of C = f.u at the solution u (FP64 CPU adjoint)."""
+    nodes, conn = _f64(nodes), _i64(conn)
+    ne, nen = conn.shape
+    young = _f64(np.broadcast_to(young, (ne,)))
+    nu = _f64(np.broadcast_to(nu, (ne,)))
+    out = np.zeros(ne)
+    lib().or_compliance_gradient(len(nodes), _d(nodes), ne, nen, _i(conn), _d(young), _d(nu),
+                                 _d(_f64(u)), _d(out))
     return out
 
 

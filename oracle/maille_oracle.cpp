@@ -969,6 +969,30 @@ void or_stresses(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64* c
   }
 }
 
+// The FP64 CPU adjoint of the compliance C = f.u (no counterpart in the reference, which has
+// no sensitivities): K_e = solid_stiffness (element.rs:223-243) is linear in E_e and the
+// adjoint state of C is u, so dC/dE_e = -u_e^T (K_e / E_e) u_e, summed row by row.
+void or_compliance_gradient(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64* conn,
+                            const double* young, const double* nu, const double* u, double* out) {
+  Mesh m{n_nodes, n_el, nen, xyz, conn, young, nu};
+  const int ml = 3 * nen;
+  std::vector<double> ke(ml * ml);
+  for (i64 e = 0; e < n_el; ++e) {
+    double x[10][3], d[6][6], ue[30];
+    m.coords(e, x);
+    solid_d(young[e], nu[e], d);
+    solid_stiffness(nen, x, d, ke.data());
+    for (int a = 0; a < ml; ++a) ue[a] = u[m.dof(e, a)];
+    double s = 0.0;
+    for (int i = 0; i < ml; ++i) {
+      double t = 0.0;
+      for (int j = 0; j < ml; ++j) t += ke[i * ml + j] * ue[j];
+      s += ue[i] * t;
+    }
+    out[e] = -s / young[e];
+  }
+}
+
 // model.rs:1070-1147 (assemble), the loads part: nodal loads already in f; adds the dead
 // pressure loads (element, face, value) in order through FacePressure::add_load.
 void or_add_pressures(i64 n_nodes, const double* xyz, int nen, const i64* conn, i64 np,
