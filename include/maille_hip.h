@@ -132,7 +132,10 @@ ml_status ml_pcg_create(int64_t n, const int64_t* offsets, const int64_t* cols,
 /* Pcg::solve (iterative.rs:210-248) for nrhs right-hand sides stored one after the other
  * (b[k*n .. k*n+n)), x likewise. tol <= 0 and maxit <= 0 select the reference's 1e-12 and
  * 10000. A zero right-hand side gives exactly x = 0. info may be null; with nrhs > 1 it
- * holds the largest iteration count and residual. */
+ * holds the largest iteration count and residual. With nrhs > 1 the right-hand sides are
+ * solved together, up to 256 per set of launches: one PCG per right-hand side (its own
+ * alpha, beta and stop, as sequential Pcg::solve calls would run them), every SpMV and
+ * triangular solve covering all of them, with an FP64 operator. */
 ml_status ml_pcg_solve(ml_pcg* h, const double* b, double* x, int64_t nrhs, double tol,
                        int64_t maxit, ml_solve_info* info);
 ml_status ml_pcg_solve_device(ml_pcg* h, const double* b_device, double* x_device,
@@ -209,6 +212,30 @@ ml_status ml_model_csr(ml_model* m, int64_t* order, int64_t* offsets, int64_t* c
  * ML_INVALID_ARGUMENT before a solve or with nonzero imposed displacements. */
 ml_status ml_model_compliance_gradient(ml_model* m, double* out);
 void ml_model_destroy(ml_model* m);
+
+/* ------------------------------------------------------------------ batched solves
+ * (no single counterpart in the reference: the equivalent of sequential Model::solve /
+ * Pcg::solve calls, model.rs:548 and iterative.rs:210, run together) */
+
+/* nrhs load cases of one model: f[k*ndof ..] nodal loads of case k, u likewise; infos[nrhs]
+ * (optional) per case. The cases are solved together as ml_pcg_solve does with nrhs > 1;
+ * reactions are not computed. */
+ml_status ml_model_solve_many(ml_model* m, int64_t nrhs, const double* f, double* u,
+                              ml_solve_info* infos);
+
+typedef struct ml_batch ml_batch;
+/* Several models of any sizes (already created; they must outlive the batch and keep their
+ * stiffness while it is used) solved in one set of launches: their K_ff are concatenated
+ * block-diagonally into one operator, each model climbs its own shift ladder in shared IC(0)
+ * launches, and one PCG per model runs with its own scalars. precision: ML_FP64 or
+ * ML_MIXED_IC0. */
+ml_status ml_batch_create(int64_t count, ml_model* const* models, ml_precision precision,
+                          ml_batch** out);
+/* f[k], u[k]: host arrays of 3*n_nodes values of model k; infos[count] optional. The
+ * displacements also become each model's last solution (ml_model_compliance_gradient). */
+ml_status ml_batch_solve(ml_batch* b, const double* const* f, double* const* u,
+                         ml_solve_info* infos);
+void ml_batch_destroy(ml_batch* b);
 
 /* Kernel-duration accounting with HIP events on the model's stream (no counterpart in the
  * reference; for benchmarks). Returns the accumulated milliseconds and launch counts per kind

@@ -43,6 +43,7 @@ EXPORTS = (
     "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_add_pressures",
     "ml_model_nfree", "ml_model_nnz", "ml_model_levels", "ml_model_csr",
     "ml_model_compliance_gradient", "ml_model_destroy", "ml_model_profile",
+    "ml_model_solve_many", "ml_batch_create", "ml_batch_solve", "ml_batch_destroy",
 )
 
 
@@ -151,6 +152,10 @@ def lib():
             "ml_model_compliance_gradient": (_int, [_P, _D]),
             "ml_model_destroy": (None, [_P]),
             "ml_model_profile": (_int, [_P, _int, _D, _I]),
+            "ml_model_solve_many": (_int, [_P, _i64, _D, _D, _S]),
+            "ml_batch_create": (_int, [_i64, C.POINTER(C.c_void_p), _int, _PP]),
+            "ml_batch_solve": (_int, [_P, C.POINTER(_D), C.POINTER(_D), _S]),
+            "ml_batch_destroy": (None, [_P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -443,6 +448,16 @@ class Model:
         young = _f64(young)
         _check(lib().ml_model_set_young(self._h, _d(young)))
 
+    def solve_many(self, F):
+        """Load cases F (rows: nodal loads over all DOFs, without dead pressures) solved
+        together; returns (U, list of info dicts)."""
+        F = np.ascontiguousarray(np.atleast_2d(F), dtype=np.float64)
+        k = F.shape[0]
+        U = np.zeros_like(F)
+        infos = (SolveInfo * k)()
+        _check(lib().ml_model_solve_many(self._h, k, _d(F), _d(U), infos))
+        return U, [i.as_dict() for i in infos]
+
     def compliance_gradient(self):
         """dC/dE_e of the compliance C = f.u at the last solution (adjoint: u itself)."""
         out = np.zeros(self.ne)
@@ -488,4 +503,32 @@ class Model:
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.ml_model_destroy(self._h)
+            self._h = None
+
+
+class Batch:
+    """Several models solved in one set of launches (block-diagonal operator, one PCG per
+    model). The models must stay alive and unchanged while the batch is used."""
+
+    def __init__(self, models, precision: int = FP64):
+        self.models = list(models)
+        arr = (C.c_void_p * len(self.models))(*[m._h.value for m in self.models])
+        h = C.c_void_p()
+        _check(lib().ml_batch_create(len(self.models), arr, precision, C.byref(h)))
+        self._h = h
+
+    def solve(self, loads=None):
+        """loads: one array of nodal loads per model (default: each model's loads());
+        returns (list of displacements, list of info dicts)."""
+        loads = [m.loads() for m in self.models] if loads is None else [_f64(f) for f in loads]
+        us = [np.zeros(m.ndof) for m in self.models]
+        fp = (_D * len(loads))(*[_d(f) for f in loads])
+        up = (_D * len(us))(*[_d(u) for u in us])
+        infos = (SolveInfo * len(us))()
+        _check(lib().ml_batch_solve(self._h, fp, up, infos))
+        return us, [i.as_dict() for i in infos]
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.ml_batch_destroy(self._h)
             self._h = None

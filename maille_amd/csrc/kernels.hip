@@ -469,6 +469,29 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
   }
 }
 
+// y = A x for nrep right-hand sides stored one after the other (x[b n + j]): thread per
+// (b, j), the same row sums as k_spmv.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmv_rep(int n, int nrep, const int* __restrict__ soff,
+                                                  const int* __restrict__ scol,
+                                                  const int* __restrict__ slen,
+                                                  const T* __restrict__ sval,
+                                                  const T* __restrict__ x, T* __restrict__ y) {
+#pragma clang fp contract(off)
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)n * nrep) return;
+  const int j = (int)(g % n);
+  const T* xb = x + (g - j);
+  const int base = soff[j >> 6] + (j & 63);
+  const int len = slen[j];
+  T s = sval[base] * xb[scol[base]];
+  for (int k = 1; k < len; ++k) {
+    const int at = base + (k << 6);
+    s += sval[at] * xb[scol[at]];
+  }
+  y[g] = s;
+}
+
 // ------------------------------------------------------------------ reductions
 template <typename A, typename B>
 __global__ __launch_bounds__(256) void k_dot(int64_t n, const A* __restrict__ a,
@@ -653,6 +676,119 @@ __global__ __launch_bounds__(256) void k_update_p(int n, const PcgScalars* __res
   if (i < n) p[i] = (TV)((double)z[i] + S->beta * (double)p[i]);
 }
 
+// ------------------------------------------------------------------ batched PCG
+// Several independent systems solved together: system s owns the vector entries
+// [seg_off[s], seg_off[s+1]) and its own scalars S[s]; seg_of[i] is the system of entry i.
+// Every system follows iterative.rs:210-248 on its own (its own alpha, beta and stop), so its
+// iterates are those of a solve of that system alone.
+
+// a.b over each system's entries: one workgroup per system, fixed order.
+template <typename A, typename B>
+__global__ __launch_bounds__(256) void k_seg_dot(const int64_t* __restrict__ seg_off,
+                                                 const A* __restrict__ a,
+                                                 const B* __restrict__ b,
+                                                 double* __restrict__ out) {
+  const int s = blockIdx.x;
+  double v = 0.0;
+  for (int64_t i = seg_off[s] + threadIdx.x; i < seg_off[s + 1]; i += 256)
+    v += (double)a[i] * (double)b[i];
+  const double t = block_sum<256>(v);
+  if (threadIdx.x == 0) out[s] = t;
+}
+
+__global__ void k_seg_init(int nseg, const double* __restrict__ bb, PcgScalars* S, double tol,
+                           int64_t maxit) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  PcgScalars& q = S[s];
+  q.bnorm = sqrt(bb[s]);
+  q.rscale = tol * q.bnorm;
+  q.it = 0;
+  q.maxit = maxit;
+  q.status = 0;
+  q.done = q.bnorm == 0.0 ? 1 : 0;  // b == 0: x = 0 exactly
+  q.rnorm = q.bnorm;
+}
+
+__global__ void k_seg_rz(int nseg, const double* __restrict__ rz, PcgScalars* S) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nseg && !S[s].done) S[s].rz = rz[s];
+}
+
+__global__ void k_seg_alpha(int nseg, const double* __restrict__ pap, PcgScalars* S) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg || S[s].done) return;
+  const double curvature = pap[s];
+  if (isnan(curvature) || curvature <= 0.0) {
+    S[s].status = 1;
+    S[s].done = 1;
+    S[s].it += 1;
+  } else {
+    S[s].alpha = S[s].rz / curvature;
+  }
+}
+
+__global__ void k_seg_converged(int nseg, const double* __restrict__ rr, PcgScalars* S) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg || S[s].done) return;
+  S[s].rnorm = sqrt(rr[s]);
+  S[s].it += 1;
+  if (S[s].rnorm <= S[s].rscale) {
+    S[s].done = 1;
+  } else if (S[s].it >= S[s].maxit) {
+    S[s].done = 1;
+    S[s].status = 2;
+  }
+}
+
+__global__ void k_seg_beta(int nseg, const double* __restrict__ rz, PcgScalars* S) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg || S[s].done) return;
+  S[s].beta = rz[s] / S[s].rz;
+  S[s].rz = rz[s];
+}
+
+// x += alpha p; r -= alpha Ap on the systems still running; refills the triangular-solve
+// outputs with the sentinel everywhere.
+template <typename TM>
+__global__ __launch_bounds__(256) void k_seg_update_xr(int64_t n, const int* __restrict__ seg_of,
+                                                       const PcgScalars* __restrict__ S,
+                                                       double* __restrict__ x,
+                                                       double* __restrict__ r,
+                                                       const double* __restrict__ p,
+                                                       const double* __restrict__ ap,
+                                                       TM* __restrict__ r_low,
+                                                       TM* __restrict__ fill_a,
+                                                       TM* __restrict__ fill_b) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  typedef typename Bits<TM>::U U;
+  reinterpret_cast<U*>(fill_a)[i] = Bits<TM>::S;
+  reinterpret_cast<U*>(fill_b)[i] = Bits<TM>::S;
+  const PcgScalars& q = S[seg_of[i]];
+  if (q.done) return;
+  const double alpha = q.alpha;
+  x[i] += alpha * p[i];
+  const double ri = r[i] - alpha * ap[i];
+  r[i] = ri;
+  if (r_low) r_low[i] = (TM)ri;
+}
+
+// p = z + beta p on the systems still running
+template <typename TM>
+__global__ __launch_bounds__(256) void k_seg_update_p(int64_t n, const int* __restrict__ seg_of,
+                                                      const PcgScalars* __restrict__ S,
+                                                      double* __restrict__ p,
+                                                      const TM* __restrict__ z) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const PcgScalars& q = S[seg_of[i]];
+  if (q.done) return;
+  p[i] = (double)z[i] + q.beta * p[i];
+}
+
 // r = b - A x in FP64 (the refinement's true residual), partial r.r
 __global__ __launch_bounds__(256) void k_residual(int n, const double* __restrict__ b,
                                                   const double* __restrict__ ax,
@@ -714,7 +850,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
     const int* __restrict__ ioff, const unsigned char* __restrict__ ik,
     const T* __restrict__ ival, const T* __restrict__ diag, const T* __restrict__ rhs, T* out,
     const PcgScalars* __restrict__ S, int* abort_flag, long long limit,
-    long long* __restrict__ trace) {
+    long long* __restrict__ trace, int nrep) {
 #pragma clang fp contract(off)
   if (S && S->done) return;
   typedef typename Bits<T>::U U;
@@ -724,7 +860,15 @@ __global__ __launch_bounds__(64) void k_trisolve(
   __shared__ unsigned char sik[TRI_ICAP];
   __shared__ T own[64];
   const int lane = threadIdx.x;
-  for (int kk = blockIdx.x; kk < nblocks; kk += gridDim.x) {
+  T* const out0 = out;
+  const T* const rhs0 = rhs;
+  for (int pos = blockIdx.x; pos < nblocks * nrep; pos += gridDim.x) {
+    // positions run block-major over the nrep right-hand sides (stored one after the other),
+    // which keeps the processing order topological for each of them
+    const int kk = pos / nrep;
+    const int rep = pos - kk * nrep;
+    out = out0 + (int64_t)rep * n;
+    rhs = rhs0 + (int64_t)rep * n;
     const int b = border[kk];
     const int s = bstart[b];
     const int e = bstart[b + 1];
@@ -867,7 +1011,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
         nv = ip < i1 ? sia[ip] : T(0);
       }
     }
-    if (trace && lane == 0) {
+    if (trace && lane == 0 && rep == 0) {
       trace[6 * (long long)b + 0] = tb;
       trace[6 * (long long)b + 1] = tp;
       trace[6 * (long long)b + 2] = wall_clock64();
@@ -892,7 +1036,8 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
                                             const int* __restrict__ col,
                                             const T* __restrict__ a, T* l, T shift,
                                             int* fail, int* abort_flag, long long limit,
-                                            int* where) {
+                                            int* where, const int* __restrict__ seg,
+                                            const T* __restrict__ seg_shift) {
 #pragma clang fp contract(off)
   __shared__ int scol[MAXROW];
   __shared__ T sw[MAXROW];
@@ -900,11 +1045,14 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
   const int lane = threadIdx.x;
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const int start = off[i], dg = off[i + 1] - 1, len = dg - start;
+    // batched systems: each has its own shift and failure flag
+    int* const my_fail = seg ? fail + seg[i] : fail;
+    const T my_shift = seg ? seg_shift[seg[i]] : shift;
     const bool failed = __builtin_amdgcn_readfirstlane(
-                            __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0;
+                            __hip_atomic_load(my_fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0;
     if (failed || len > MAXROW) {
       if (len > MAXROW && lane == 0)
-        __hip_atomic_store(fail, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(my_fail, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       for (int t = lane; t <= len; t += 64) publish(l + start + t, T(0));
     } else {
       for (int t = lane; t < len; t += 64) {
@@ -955,9 +1103,9 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
       }
       T squares = T(0);
       for (int q = 0; q < len; ++q) squares = squares + sw[q] * sw[q];
-      const T pivot = a[dg] * (T(1) + shift) - squares;
+      const T pivot = a[dg] * (T(1) + my_shift) - squares;
       const bool bad = isnan(pivot) || pivot <= T(1e-12) * a[dg];
-      if (bad && lane == 0) __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (bad && lane == 0) __hip_atomic_store(my_fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       for (int t = lane; t < len; t += 64) publish(l + start + t, sw[t]);
       if (lane == 0) publish(l + dg, bad ? T(0) : (T)sqrt(pivot));
     }

@@ -236,6 +236,7 @@ struct Operator {
   DBuf<int> d_flags;  // [0] abort, [1] IC(0) failure, [2..5] where an IC(0) wait gave up
   // grids of the dataflow kernels: the full ones, and the ones in use (full >> shrinks)
   int full_tri64 = 0, full_tri32 = 0, full_ic = 0, full_ic64 = 0;
+  int cap_tri64 = 0, cap_tri32 = 0;  // dataflow grids before the cap by block count
   int tri_grid64 = 0, tri_grid32 = 0, ic_grid = 0, ic_grid64 = 0;
   int shrinks = 0;   // grid halvings after a stalled dataflow launch
   int retries = 0;   // work repeated on a smaller grid (reported in ml_solve_info)
@@ -280,8 +281,10 @@ struct Operator {
     d_sval.alloc(d_scol.n);
     for (auto* b : {&w_x, &w_r, &w_p, &w_ap, &w_y, &w_z, &w_b, &w_d, &w_ax}) b->alloc(n);
     const int nblk = std::max(fw.nblk, bw.nblk);
-    full_tri64 = std::min<int>(nblk, dataflow_grid((const void*)mlk::k_trisolve<double, false, false>));
-    full_tri32 = std::min<int>(nblk, dataflow_grid((const void*)mlk::k_trisolve<float, false, false>));
+    cap_tri64 = dataflow_grid((const void*)mlk::k_trisolve<double, false, false>);
+    cap_tri32 = dataflow_grid((const void*)mlk::k_trisolve<float, false, false>);
+    full_tri64 = std::min<int>(nblk, cap_tri64);
+    full_tri32 = std::min<int>(nblk, cap_tri32);
     full_ic = std::min<int>(n, std::min(dataflow_grid((const void*)mlk::k_ic0<double, 512>),
                                         dataflow_grid((const void*)mlk::k_ic0<float, 512>)));
     full_ic64 = std::min<int>(n, std::min(dataflow_grid((const void*)mlk::k_ic0<double, 64>),
@@ -331,25 +334,29 @@ struct Operator {
     return v;
   }
 
+  // One IC(0) factorisation with the given shift; with `seg` (batched systems: the system of
+  // each row), each system uses seg_shift[system] and reports its failure in seg_fail[system].
   template <typename T>
-  int run_ic0(const T* a, T* l, T shift) {
+  int run_ic0(const T* a, T* l, T shift, const int* seg = nullptr, const T* seg_shift = nullptr,
+              int* seg_fail = nullptr, int nseg = 0) {
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 2, 0);
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p + 2, 4, -1);
+    if (seg) mlk::k_fill_i32<<<grid_for(nseg), 256, 0, stream>>>(seg_fail, nseg, 0);
     mlk::k_fill_bits<T><<<grid_for(nnz), 256, 0, stream>>>(l, nnz, mlk::Bits<T>::S);
     hipEvent_t ev = prof.begin(stream);
     const int* off = d_off.p;
     const int* col = d_col.p;
-    int* fail = d_flags.p + 1;
+    int* fail = seg ? seg_fail : d_flags.p + 1;
     int* abort_flag = d_flags.p;
     int* where = d_flags.p + 2;
     long long limit = wait_ticks;
     int nn = n;
     if (max_row <= 64)
       launch_dataflow(mlk::k_ic0<T, 64>, ic_grid64, stream, nn, off, col, a, l, shift, fail,
-                      abort_flag, limit, where);
+                      abort_flag, limit, where, seg, seg_shift);
     else
       launch_dataflow(mlk::k_ic0<T, 512>, ic_grid, stream, nn, off, col, a, l, shift, fail,
-                      abort_flag, limit, where);
+                      abort_flag, limit, where, seg, seg_shift);
     prof.end(P_IC0, ev, stream);
     int flags[6];
     HIP_CHECK(hipMemcpyAsync(flags, d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, stream));
@@ -430,11 +437,15 @@ struct Operator {
 
   // ---------------------------------------------------------------- kernels
   // z = (L L^T)^-1 r through y
+  // With nrep > 1, r, y and z hold nrep vectors one after the other, solved in one launch.
   template <typename T>
   void precondition(const T* r, T* y, T* z, const mlk::PcgScalars* S,
-                    long long* trace_fw = nullptr, long long* trace_bw = nullptr) {
+                    long long* trace_fw = nullptr, long long* trace_bw = nullptr, int nrep = 1) {
     const bool f32 = std::is_same<T, float>::value;
-    int& grid = f32 ? tri_grid32 : tri_grid64;
+    int& grid1 = f32 ? tri_grid32 : tri_grid64;
+    int grid_rep = std::min<int64_t>((int64_t)std::max(fw.nblk, bw.nblk) * nrep,
+                                     std::max(1, (f32 ? cap_tri32 : cap_tri64) >> shrinks));
+    int& grid = nrep > 1 ? grid_rep : grid1;
     hipEvent_t ev = prof.begin(stream);
     int nn = n;
     long long limit = wait_ticks;
@@ -447,12 +458,12 @@ struct Operator {
         launch_dataflow(mlk::k_trisolve<T, false, false>, grid, stream, nn, nb,
                         (const int*)t.border.p, (const int*)t.bstart.p, (const int*)t.xoff.p, (const int*)t.xcol.p, xv,
                         (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out, S,
-                        d_flags.p, limit, trace);
+                        d_flags.p, limit, trace, nrep);
       else
         launch_dataflow(mlk::k_trisolve<T, true, true>, grid, stream, nn, nb,
                         (const int*)t.border.p, (const int*)t.bstart.p, (const int*)t.xoff.p, (const int*)t.xcol.p, xv,
                         (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out, S,
-                        d_flags.p, limit, trace);
+                        d_flags.p, limit, trace, nrep);
     };
     run(fw, r, y, true, trace_fw);
     run(bw, (const T*)y, z, false, trace_bw);
@@ -576,6 +587,182 @@ struct Operator {
     if (hS->status == 1) return ML_SINGULAR;
     if (hS->status == 2) return ML_NOT_CONVERGED;
     return ML_OK;
+  }
+
+  // ---------------------------------------------------------------- batched PCG
+  // Workspace of the batched solves (allocated on first use, grown as needed).
+  DBuf<double> wb_r, wb_p, wb_ap, wb_dot;
+  DBuf<double> wb_y64, wb_z64;
+  DBuf<float> wb_y32, wb_z32, wb_rlow;
+  DBuf<mlk::PcgScalars> wb_S;
+
+  // Per-system shift ladder of a block-diagonal operator (Pcg::new, iterative.rs:201-207, for
+  // each system on its own): one IC(0) launch per round over every system, each at its own
+  // shift; the systems that failed climb one step and the round repeats. shift_idx receives
+  // each system's shift index.
+  template <typename T>
+  int factor_segments(int nseg, const int* seg_of, std::vector<int>* shift_idx) {
+    static const double SHIFTS[6] = {0.0, 1e-3, 1e-2, 1e-1, 1.0, 10.0};
+    const bool f32 = std::is_same<T, float>::value;
+    factored = false;
+    shift_idx->assign(nseg, 0);
+    DBuf<T> d_shift;
+    DBuf<int> d_fail;
+    d_shift.alloc(nseg);
+    d_fail.alloc(nseg);
+    std::vector<T> h_shift(nseg);
+    std::vector<int> h_fail(nseg);
+    T* l = f32 ? (T*)d_l32.p : (T*)d_l.p;
+    const T* a = f32 ? (const T*)d_a32.p : (const T*)d_a.p;
+    for (int round = 0;; ++round) {
+      for (int s = 0; s < nseg; ++s) h_shift[s] = (T)SHIFTS[(*shift_idx)[s]];
+      HIP_CHECK(hipMemcpyAsync(d_shift.p, h_shift.data(), sizeof(T) * nseg, hipMemcpyHostToDevice, stream));
+      int st = run_ic0<T>(a, l, T(0), seg_of, d_shift.p, d_fail.p, nseg);
+      if (st == ML_DEVICE_TIMEOUT && shrink()) continue;
+      if (st != ML_OK) return st;
+      HIP_CHECK(hipMemcpyAsync(h_fail.data(), d_fail.p, sizeof(int) * nseg, hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      bool any = false;
+      for (int s = 0; s < nseg; ++s) {
+        if (h_fail[s] == 2) throw ArgError("matrix row longer than 512 entries");
+        if (h_fail[s]) {
+          any = true;
+          if (++(*shift_idx)[s] >= 6) return ML_SINGULAR;
+        }
+      }
+      if (!any) break;
+    }
+    gather_factor<T>(l);
+    factored = true;
+    return ML_OK;
+  }
+
+  // nseg independent PCG solves (iterative.rs:210-248 each) in one set of launches: either
+  // nrep right-hand sides of this operator stored one after the other (nseg = nrep), or the
+  // systems of this block-diagonal operator (nrep = 1). System s owns the vector entries
+  // [seg_off[s], seg_off[s+1]) and seg_of[i] is the system of entry i. FP64 operator and
+  // iterate; TM is the arithmetic of the factor and the preconditioned residual. The scalars
+  // of every system end in *hS.
+  template <typename TM>
+  int pcg_batch(int nrep, int nseg, const int64_t* seg_off, const int* seg_of, const double* b,
+                double* x, double tol, int64_t maxit, std::vector<mlk::PcgScalars>* hS) {
+    const bool m32 = std::is_same<TM, float>::value;
+    const int64_t N = (int64_t)n * nrep;
+    for (auto* v : {&wb_r, &wb_p, &wb_ap}) if ((int64_t)v->n < N) v->alloc(N);
+    if ((int64_t)wb_dot.n < 4 * nseg) wb_dot.alloc(4 * nseg);
+    if ((int64_t)wb_S.n < nseg) wb_S.alloc(nseg);
+    TM* y;
+    TM* z;
+    TM* rlow = nullptr;
+    if (m32) {
+      for (auto* v : {&wb_y32, &wb_z32, &wb_rlow}) if ((int64_t)v->n < N) v->alloc(N);
+      y = (TM*)wb_y32.p;
+      z = (TM*)wb_z32.p;
+      rlow = (TM*)wb_rlow.p;
+    } else {
+      for (auto* v : {&wb_y64, &wb_z64}) if ((int64_t)v->n < N) v->alloc(N);
+      y = (TM*)wb_y64.p;
+      z = (TM*)wb_z64.p;
+    }
+    double* r = wb_r.p;
+    double* p = wb_p.p;
+    double* ap = wb_ap.p;
+    double* bb = wb_dot.p;
+    double* pap = wb_dot.p + nseg;
+    double* rr = wb_dot.p + 2 * nseg;
+    double* rz = wb_dot.p + 3 * nseg;
+    mlk::PcgScalars* S = wb_S.p;
+    const TM* rin = m32 ? (const TM*)rlow : (const TM*)r;
+    const unsigned vb = grid_for(N), sb = grid_for(nseg);
+    mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);  // abort flag
+    mlk::k_fill_f64<<<vb, 256, 0, stream>>>(x, N, 0.0);
+    HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
+    mlk::k_seg_dot<double, double><<<nseg, 256, 0, stream>>>(seg_off, b, b, bb);
+    mlk::k_seg_init<<<sb, 256, 0, stream>>>(nseg, bb, S, tol, maxit);
+    if (m32) mlk::k_convert<double, float><<<vb, 256, 0, stream>>>(N, r, (float*)rlow);
+    mlk::k_fill_bits<TM><<<vb, 256, 0, stream>>>(y, N, mlk::Bits<TM>::S);
+    mlk::k_fill_bits<TM><<<vb, 256, 0, stream>>>(z, N, mlk::Bits<TM>::S);
+    precondition<TM>(rin, y, z, nullptr, nullptr, nullptr, nrep);
+    mlk::k_seg_dot<double, TM><<<nseg, 256, 0, stream>>>(seg_off, r, z, rz);
+    mlk::k_seg_rz<<<sb, 256, 0, stream>>>(nseg, rz, S);
+    mlk::k_convert<TM, double><<<vb, 256, 0, stream>>>(N, z, p);
+    HIP_CHECK(hipGetLastError());
+    auto spmv_all = [&](const double* in, double* out) {
+      hipEvent_t ev = prof.begin(stream);
+      if (nrep > 1)
+        mlk::k_spmv_rep<double><<<vb, 256, 0, stream>>>(n, nrep, d_soff.p, d_scol.p, d_slen.p,
+                                                        d_sval.p, in, out);
+      else
+        mlk::k_spmv<double><<<vb, 256, 0, stream>>>(n, d_soff.p, d_scol.p, d_slen.p, d_sval.p,
+                                                     in, out, nullptr, nullptr, 0);
+      prof.end(P_SPMV64, ev, stream);
+    };
+    hS->resize(nseg);
+    auto all_done = [&]() {
+      HIP_CHECK(hipMemcpyAsync(hS->data(), S, sizeof(mlk::PcgScalars) * nseg, hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      for (const auto& q : *hS) if (!q.done) return false;
+      return true;
+    };
+    if (read_flag(0)) return ML_DEVICE_TIMEOUT;
+    if (all_done()) return ML_OK;
+    int64_t launched = 0, batch = 4;
+    for (;;) {
+      for (int64_t k = 0; k < batch; ++k) {
+        spmv_all(p, ap);
+        mlk::k_seg_dot<double, double><<<nseg, 256, 0, stream>>>(seg_off, p, ap, pap);
+        mlk::k_seg_alpha<<<sb, 256, 0, stream>>>(nseg, pap, S);
+        hipEvent_t ev = prof.begin(stream);
+        mlk::k_seg_update_xr<TM><<<vb, 256, 0, stream>>>(N, seg_of, S, x, r, p, ap, rlow, y, z);
+        prof.end(P_VEC, ev, stream);
+        mlk::k_seg_dot<double, double><<<nseg, 256, 0, stream>>>(seg_off, r, r, rr);
+        mlk::k_seg_converged<<<sb, 256, 0, stream>>>(nseg, rr, S);
+        precondition<TM>(rin, y, z, nullptr, nullptr, nullptr, nrep);
+        mlk::k_seg_dot<double, TM><<<nseg, 256, 0, stream>>>(seg_off, r, z, rz);
+        mlk::k_seg_beta<<<sb, 256, 0, stream>>>(nseg, rz, S);
+        ev = prof.begin(stream);
+        mlk::k_seg_update_p<TM><<<vb, 256, 0, stream>>>(N, seg_of, S, p, z);
+        prof.end(P_VEC, ev, stream);
+        HIP_CHECK(hipGetLastError());
+      }
+      launched += batch;
+      if (read_flag(0)) return ML_DEVICE_TIMEOUT;
+      if (all_done()) break;
+      if (launched > maxit + 8) break;  // cannot happen: every system stops at maxit
+      batch = std::min<int64_t>(32, batch * 2);
+    }
+    return ML_OK;
+  }
+
+  // The batched solve of nrep right-hand sides b (device, one after the other) of this
+  // operator; per-system status and scalars in *hS. A stalled dataflow launch repeats the
+  // solve on smaller grids.
+  int solve_many(int nrep, const double* b, double* x, double tol, int64_t maxit,
+                 std::vector<mlk::PcgScalars>* hS) {
+    if (!factored) {
+      int st = factor();
+      if (st != ML_OK) return st;
+    }
+    if (tol <= 0) tol = ML_TOLERANCE;
+    if (maxit <= 0) maxit = ML_MAX_ITERATIONS;
+    std::vector<int64_t> off(nrep + 1);
+    std::vector<int> seg((int64_t)n * nrep);
+    for (int k = 0; k <= nrep; ++k) off[k] = (int64_t)k * n;
+    for (int k = 0; k < nrep; ++k) std::fill(seg.begin() + (int64_t)k * n, seg.begin() + (int64_t)(k + 1) * n, k);
+    DBuf<int64_t> d_off_seg;
+    DBuf<int> d_seg;
+    d_off_seg.upload_vec(off, stream);
+    d_seg.upload_vec(seg, stream);
+    int st;
+    for (;;) {
+      st = precision == ML_FP64
+               ? pcg_batch<double>(nrep, nrep, d_off_seg.p, d_seg.p, b, x, tol, maxit, hS)
+               : pcg_batch<float>(nrep, nrep, d_off_seg.p, d_seg.p, b, x, tol, maxit, hS);
+      if (st == ML_DEVICE_TIMEOUT && shrink()) continue;
+      break;
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return st;
   }
 
   // |b - A x| / |b| in FP64 (uses w_ax, w_d as scratch)
@@ -903,6 +1090,18 @@ struct ml_pcg {
 struct ml_model {
   Model m;
 };
+// Several models solved together: their K_ff concatenated block-diagonally into one operator
+// (one launch per kernel for all of them), each model a system of the batched PCG.
+struct ml_batch {
+  std::vector<ml_model*> models;
+  Operator op;
+  int nseg = 0;
+  std::vector<int64_t> row_off;  // model k owns rows [row_off[k], row_off[k+1])
+  std::vector<int> shift_idx;
+  DBuf<int64_t> d_seg_off;
+  DBuf<int> d_seg;
+  DBuf<double> d_rhs, d_x;
+};
 struct ml_pattern {
   Graph graph;
   std::vector<int64_t> order;
@@ -1099,12 +1298,59 @@ ml_status ml_pcg_solve_device(ml_pcg* h, const double* b, double* x, double tol,
   });
 }
 
+namespace {
+
+constexpr int64_t MAX_BATCH_RHS = 256;  // right-hand sides per batched pass
+
+int status_of(const mlk::PcgScalars& q) {
+  return q.status == 1 ? ML_SINGULAR : q.status == 2 ? ML_NOT_CONVERGED : ML_OK;
+}
+
+void info_of(const mlk::PcgScalars& q, int shift_index, ml_solve_info* info) {
+  std::memset(info, 0, sizeof(*info));
+  info->iterations = q.it;
+  info->residual = q.bnorm > 0 ? q.rnorm / q.bnorm : 0.0;
+  info->true_residual = info->residual;
+  info->shift_index = shift_index;
+  info->bad_element = -1;
+}
+
+}  // namespace
+
 ml_status ml_pcg_solve(ml_pcg* h, const double* b, double* x, int64_t nrhs, double tol,
                        int64_t maxit, ml_solve_info* info) {
   return guarded([&]() -> int {
     if (!h || !b || !x || nrhs <= 0) throw ArgError("bad arguments");
     Operator& op = h->op;
     const int n = op.n;
+    if (nrhs > 1) {
+      // batched: up to MAX_BATCH_RHS right-hand sides per set of launches
+      int result = ML_OK;
+      const int retries0 = op.retries;
+      mlk::PcgScalars worst;
+      std::memset(&worst, 0, sizeof(worst));
+      for (int64_t k0 = 0; k0 < nrhs; k0 += MAX_BATCH_RHS) {
+        const int k = (int)std::min(MAX_BATCH_RHS, nrhs - k0);
+        DBuf<double> bd, xd;
+        bd.upload(b + k0 * n, (size_t)k * n, op.stream);
+        xd.alloc((size_t)k * n);
+        std::vector<mlk::PcgScalars> hS;
+        int st = op.solve_many(k, bd.p, xd.p, tol, maxit, &hS);
+        if (st != ML_OK) return st;
+        HIP_CHECK(hipMemcpyAsync(x + k0 * n, xd.p, sizeof(double) * k * n, hipMemcpyDeviceToHost,
+                                 op.stream));
+        HIP_CHECK(hipStreamSynchronize(op.stream));
+        for (const auto& q : hS) {
+          if (status_of(q) != ML_OK && result == ML_OK) result = status_of(q);
+          if (q.it >= worst.it) worst = q;
+        }
+      }
+      if (info) {
+        info_of(worst, op.shift_index, info);
+        info->device_retries = op.retries - retries0;
+      }
+      return result;
+    }
     ml_solve_info worst;
     std::memset(&worst, 0, sizeof(worst));
     int result = ML_OK;
@@ -1373,6 +1619,142 @@ ml_status ml_model_add_pressures(ml_model* h, int64_t count, const int64_t* elem
     return ML_OK;
   });
 }
+
+ml_status ml_model_solve_many(ml_model* h, int64_t nrhs, const double* f, double* u,
+                              ml_solve_info* infos) {
+  return guarded([&]() -> int {
+    if (!h || !f || !u || nrhs <= 0) throw ArgError("bad arguments");
+    Model& m = h->m;
+    if (!m.assembled) m.assemble();
+    Operator& op = m.op;
+    const int64_t ndof = m.plan.ndof, nf = m.plan.nfree;
+    int result = ML_OK;
+    for (int64_t k0 = 0; k0 < nrhs; k0 += MAX_BATCH_RHS) {
+      const int k = (int)std::min(MAX_BATCH_RHS, nrhs - k0);
+      DBuf<double> fd, rhs, xd, ud;
+      fd.upload(f + k0 * ndof, (size_t)k * ndof, op.stream);
+      rhs.alloc((size_t)k * nf);
+      xd.alloc((size_t)k * nf);
+      ud.alloc((size_t)k * ndof);
+      for (int j = 0; j < k; ++j)
+        mlk::k_rhs<<<grid_for(ndof), 256, 0, op.stream>>>(
+            ndof, m.d_rank.p, fd.p + j * ndof, m.plan.cpl_slot.empty() ? nullptr : m.d_coupling.p,
+            rhs.p + j * nf);
+      HIP_CHECK(hipGetLastError());
+      std::vector<mlk::PcgScalars> hS;
+      int st = op.solve_many(k, rhs.p, xd.p, 0, 0, &hS);
+      if (st != ML_OK) return st;
+      for (int j = 0; j < k; ++j)
+        mlk::k_scatter_u<<<grid_for(ndof), 256, 0, op.stream>>>(ndof, m.d_rank.p, xd.p + j * nf,
+                                                                m.d_imposed.p, ud.p + j * ndof);
+      HIP_CHECK(hipMemcpyAsync(u + k0 * ndof, ud.p, sizeof(double) * k * ndof,
+                               hipMemcpyDeviceToHost, op.stream));
+      HIP_CHECK(hipStreamSynchronize(op.stream));
+      for (int j = 0; j < k; ++j) {
+        if (status_of(hS[j]) != ML_OK && result == ML_OK) result = status_of(hS[j]);
+        if (infos) info_of(hS[j], op.shift_index, infos + k0 + j);
+      }
+    }
+    return result;
+  });
+}
+
+ml_status ml_batch_create(int64_t count, ml_model* const* models, ml_precision precision,
+                          ml_batch** out) {
+  return guarded([&]() -> int {
+    if (!out || count <= 0 || !models) throw ArgError("bad arguments");
+    if (precision != ML_FP64 && precision != ML_MIXED_IC0)
+      throw ArgError("batched solves take ML_FP64 or ML_MIXED_IC0");
+    std::unique_ptr<ml_batch> b(new ml_batch);
+    b->models.assign(models, models + count);
+    b->nseg = (int)count;
+    // block-diagonal concatenation of the models' lower patterns
+    LowerPattern cat;
+    cat.offsets.push_back(0);
+    b->row_off.push_back(0);
+    for (ml_model* h : b->models) {
+      if (!h) throw ArgError("null model");
+      Model& m = h->m;
+      if (!m.assembled) m.assemble();
+      const LowerPattern& lp = m.plan.lower;
+      const int64_t r0 = cat.n(), e0 = cat.nnz();
+      for (int64_t i = 0; i < lp.n(); ++i) cat.offsets.push_back(e0 + lp.offsets[i + 1]);
+      for (int32_t c : lp.cols) cat.cols.push_back((int32_t)(c + r0));
+      b->row_off.push_back(cat.n());
+      if (cat.n() >= INT32_MAX || cat.nnz() >= INT32_MAX) throw ArgError("batch too large");
+    }
+    Operator& op = b->op;
+    op.precision = precision;
+    op.set_pattern(cat);
+    int64_t e0 = 0;
+    for (ml_model* h : b->models) {
+      Model& m = h->m;
+      HIP_CHECK(hipStreamSynchronize(m.s()));
+      HIP_CHECK(hipMemcpyAsync(op.d_a.p + e0, m.op.d_a.p, sizeof(double) * m.plan.lower.nnz(),
+                               hipMemcpyDeviceToDevice, op.stream));
+      e0 += m.plan.lower.nnz();
+    }
+    op.values_changed();
+    std::vector<int> seg(cat.n());
+    for (int k = 0; k < b->nseg; ++k)
+      std::fill(seg.begin() + b->row_off[k], seg.begin() + b->row_off[k + 1], k);
+    b->d_seg_off.upload_vec(b->row_off, op.stream);
+    b->d_seg.upload_vec(seg, op.stream);
+    b->d_rhs.alloc(cat.n());
+    b->d_x.alloc(cat.n());
+    int st = precision == ML_FP64 ? op.factor_segments<double>(b->nseg, b->d_seg.p, &b->shift_idx)
+                                  : op.factor_segments<float>(b->nseg, b->d_seg.p, &b->shift_idx);
+    HIP_CHECK(hipStreamSynchronize(op.stream));
+    if (st != ML_OK) return st;
+    *out = b.release();
+    return ML_OK;
+  });
+}
+
+ml_status ml_batch_solve(ml_batch* b, const double* const* f, double* const* u,
+                         ml_solve_info* infos) {
+  return guarded([&]() -> int {
+    if (!b || !f || !u) throw ArgError("null argument");
+    Operator& op = b->op;
+    for (int k = 0; k < b->nseg; ++k) {
+      Model& m = b->models[k]->m;
+      if (!f[k] || !u[k]) throw ArgError("null load or displacement array");
+      HIP_CHECK(hipMemcpyAsync(m.d_f.p, f[k], sizeof(double) * m.plan.ndof,
+                               hipMemcpyHostToDevice, op.stream));
+      mlk::k_rhs<<<grid_for(m.plan.ndof), 256, 0, op.stream>>>(
+          m.plan.ndof, m.d_rank.p, m.d_f.p, m.plan.cpl_slot.empty() ? nullptr : m.d_coupling.p,
+          b->d_rhs.p + b->row_off[k]);
+    }
+    HIP_CHECK(hipGetLastError());
+    std::vector<mlk::PcgScalars> hS;
+    int st;
+    for (;;) {
+      st = op.precision == ML_FP64
+               ? op.pcg_batch<double>(1, b->nseg, b->d_seg_off.p, b->d_seg.p, b->d_rhs.p, b->d_x.p,
+                                      ML_TOLERANCE, ML_MAX_ITERATIONS, &hS)
+               : op.pcg_batch<float>(1, b->nseg, b->d_seg_off.p, b->d_seg.p, b->d_rhs.p, b->d_x.p,
+                                     ML_TOLERANCE, ML_MAX_ITERATIONS, &hS);
+      if (st == ML_DEVICE_TIMEOUT && op.shrink()) continue;
+      break;
+    }
+    if (st != ML_OK) return st;
+    int result = ML_OK;
+    for (int k = 0; k < b->nseg; ++k) {
+      Model& m = b->models[k]->m;
+      mlk::k_scatter_u<<<grid_for(m.plan.ndof), 256, 0, op.stream>>>(
+          m.plan.ndof, m.d_rank.p, b->d_x.p + b->row_off[k], m.d_imposed.p, m.d_ulast.p);
+      HIP_CHECK(hipMemcpyAsync(u[k], m.d_ulast.p, sizeof(double) * m.plan.ndof,
+                               hipMemcpyDeviceToHost, op.stream));
+      m.have_u = true;
+      if (status_of(hS[k]) != ML_OK && result == ML_OK) result = status_of(hS[k]);
+      if (infos) info_of(hS[k], b->shift_idx[k], infos + k);
+    }
+    HIP_CHECK(hipStreamSynchronize(op.stream));
+    return result;
+  });
+}
+
+void ml_batch_destroy(ml_batch* b) { delete b; }
 
 void ml_model_destroy(ml_model* h) { delete h; }
 

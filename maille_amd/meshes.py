@@ -283,3 +283,69 @@ def problem_pressures_arrays(problem: Problem):
         return z, z, np.zeros(0)
     p = np.array(problem.pressures, dtype=np.float64)
     return p[:, 0].astype(np.int64), p[:, 1].astype(np.int64), p[:, 2].copy()
+
+
+# ---------------------------------------------------------------- batched workloads
+def face_traction_loads(nodes, conn, faces, tractions, ndof):
+    """Consistent nodal loads of a traction vector given per face ((element, face) pairs of
+    TET_FACES): A/3 of it at each vertex (Tet4) or mid-node (Tet10), as traction_loads."""
+    quadratic = conn.shape[1] == 10
+    f = np.zeros(ndof)
+    e, fc = faces
+    if len(e) == 0:
+        return f
+    verts = conn[e[:, None], TET_FACES[fc]]  # (nf, 3)
+    p = nodes[verts]
+    cr = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    area = np.sqrt((cr ** 2).sum(axis=1)) / 2.0
+    if quadratic:
+        edge = {tuple(sorted(map(int, ab))): 4 + k for k, ab in enumerate(TET10_EDGES)}
+        mids = np.array([[edge[tuple(sorted((int(f3[a]), int(f3[(a + 1) % 3]))))] for a in range(3)]
+                         for f3 in TET_FACES])
+        carriers = conn[e[:, None], mids[fc]]
+    else:
+        carriers = verts
+    for c in range(3):
+        np.add.at(f, 3 * carriers.ravel() + c, np.repeat(tractions[:, c] * area / 3.0, 3))
+    return f
+
+
+def beam_load_cases(n=(40, 8, 8), size=(200.0, 40.0, 40.0), cases=256, seed=3, young=70000.0,
+                    nu=0.33):
+    """Tet10 beam of n cells (10:2:2 aspect), clamped at x=0. Load case k: on every face of the
+    top surface z = H a traction vector of standard-normal components (SplitMix64 seed
+    ``seed + k``, three draws per face in face order) as consistent nodal loads. Returns the
+    problem (loads of case 0) and the (cases, 3N) load matrix."""
+    nodes, tets = box_grid(n, size)
+    nodes, conn = to_quadratic_tet(nodes, tets)
+    fixed, imposed = clamp(nodes, lambda x: np.abs(x[:, 0]) < 1e-12)
+    top = faces_on(nodes, conn[:, :4], lambda x: np.abs(x[:, 2] - size[2]) < 1e-12)
+    ndof = 3 * len(nodes)
+    F = np.stack([face_traction_loads(nodes, conn, top,
+                                      normal(seed + k, 3 * len(top[0])).reshape(-1, 3), ndof)
+                  for k in range(cases)])
+    ne = len(conn)
+    p = Problem("beam_load_cases", nodes, conn, np.full(ne, young), np.full(ne, nu), fixed,
+                imposed, F[0].copy(), [], {"cells": tuple(n), "cases": cases})
+    return p, F
+
+
+def box_batch(count=64, seed=4, lo=500.0, hi=20000.0):
+    """Tet4 boxes of uneven sizes: N ~ log-uniform in [lo, hi] DOF (seed), cells (nx, m, m) with
+    m = ceil((N/48)^(1/3)) and nx = round(N/(3(m+1)^2)) - 1, unit cells; E ~ U[5e4, 2.5e5] and
+    nu ~ U[0.2, 0.4] per box (seeds seed+1, seed+2); clamped at x=0, uniform traction
+    (0, 0, -1) on the x = nx face."""
+    u = uniform(seed, count)
+    dofs = np.exp(np.log(lo) + (np.log(hi) - np.log(lo)) * u)
+    young = 5e4 + 2e5 * uniform(seed + 1, count)
+    nu = 0.2 + 0.2 * uniform(seed + 2, count)
+    out = []
+    for k in range(count):
+        m = int(np.ceil((dofs[k] / 48.0) ** (1.0 / 3.0)))
+        nx = max(1, int(round(dofs[k] / (3.0 * (m + 1) ** 2))) - 1)
+        size = (float(nx), float(m), float(m))
+        nodes, tets = box_grid((nx, m, m), size)
+        p = _cantilever(f"box_{k}", nodes, tets, size, float(young[k]), float(nu[k]),
+                        (0.0, 0.0, -1.0), (nx, m, m))
+        out.append(p)
+    return out

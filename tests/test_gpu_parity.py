@@ -224,3 +224,57 @@ def test_compliance_gradient_matches_central_differences(gpu):
     mq.solve()
     with pytest.raises(ValueError, match="homogeneous"):
         mq.compliance_gradient()
+
+
+# ------------------------------------------------------------------ batched solves
+def test_many_right_hand_sides_match_separate_solves(gpu):
+    """Pcg::solve on several right-hand sides at once (one PCG per right-hand side, every SpMV
+    and triangular solve covering all of them) against one solve at a time and the oracle's
+    PCG; a zero right-hand side gives exactly x = 0."""
+    p = PROBLEMS["jittered_cube_8"]
+    s = _oracle_system(p)
+    off, cols, vals = s.csr()
+    pcg = gpu.Pcg(gpu.Csr(off, cols, vals))
+    csr = oracle.Csr.from_arrays(off, cols, vals)
+    rng = np.random.default_rng(11)
+    B = rng.normal(size=(5, s.n))
+    B[3] = 0.0
+    X, info = pcg.solve(B, info=True)
+    assert np.all(X[3] == 0.0)
+    for k in (0, 1, 2, 4):
+        x1 = pcg.solve(B[k])
+        assert rel_l2(X[k], x1) <= 1e-10
+        st, xo, it, res, _ = csr.pcg(B[k])
+        assert st == oracle.OK and rel_l2(X[k], xo) <= 1e-10
+
+
+def test_load_cases_of_one_model(gpu):
+    """Model load cases solved together (the beam of the batched workload, at a smaller size)
+    match the model's one-at-a-time solves."""
+    p, F = M.beam_load_cases(n=(12, 3, 3), size=(60.0, 15.0, 15.0), cases=6)
+    m = gpu.Model(p)
+    U, infos = m.solve_many(F)
+    assert len(infos) == 6
+    for k in range(6):
+        u1, _, info1 = m.solve(F[k], reactions=False)
+        assert rel_l2(U[k], u1) <= 1e-10
+        assert abs(infos[k]["iterations"] - info1["iterations"]) <= 3
+
+
+@pytest.mark.parametrize("precision", ["FP64", "MIXED_IC0"])
+def test_batch_of_uneven_models(gpu, precision):
+    """Boxes of uneven sizes and materials solved in one batch (block-diagonal operator, one
+    PCG per model) against each model solved alone and against the oracle's PCG."""
+    boxes = M.box_batch(count=6, lo=300.0, hi=3000.0)
+    models = [gpu.Model(b) for b in boxes]
+    batch = gpu.Batch(models, precision=getattr(gpu, precision))
+    us, infos = batch.solve()
+    bound = 1e-10 if precision == "FP64" else 1e-8
+    for b, m, u, info in zip(boxes, models, us, infos):
+        ref = oracle.model_solve(b, iterative=True)
+        print(f"{b.name}: {b.nfree} free DOF, {info['iterations']} iterations "
+              f"(oracle {ref['iterations']}), shift {info['shift_index']}", flush=True)
+        assert rel_l2(u, ref["displacements"]) <= bound
+        assert info["shift_index"] == ref["shift_index"] or precision != "FP64"
+        u1, _, _ = m.solve(reactions=False)
+        assert rel_l2(u, u1) <= bound
