@@ -69,6 +69,150 @@ def cpu_baseline(problem, iterations, sample_iters):
     }
 
 
+def emit(out):
+    print(json.dumps(out), flush=True)
+
+
+def run_beam_cases(args, ma, meshes):
+    """256 load cases of the Tet10 beam solved together (ml_model_solve_many) against the same
+    model's one-at-a-time solves (the reference's sequential Model::solve calls)."""
+    p, F = meshes.beam_load_cases(cases=args.cases)
+    m = ma.Model(p)
+    m.factor()
+    chosen = m.autotune(nrhs=len(F))
+    log(f"beam: {p.nfree} free DOF, {len(F)} cases; autotuned {chosen}")
+    m.solve_many(F[:8])
+    t0 = time.perf_counter()
+    U, infos = m.solve_many(F)
+    batched = time.perf_counter() - t0
+    seq = min(args.seq, len(F))
+    t0 = time.perf_counter()
+    for k in range(seq):
+        u1, _, _ = m.solve(F[k], reactions=False)
+        err = float(np.linalg.norm(U[k] - u1) / np.linalg.norm(u1))
+    sequential = (time.perf_counter() - t0) / seq
+    emit({"metric": "load cases per second, Tet10 beam 40x8x8 (69,360 free DOF), FP64",
+          "value": round(len(F) / batched, 2), "unit": "cases/s", "n_gpus": 1,
+          "higher_is_better": True, "data": "synthetic", "dtype": "fp64",
+          "config": {"workload": "beam_load_cases", "free_dof": p.nfree, "cases": len(F),
+                     "waves_per_cu": chosen[0], "cases_per_pass": chosen[1]},
+          "batched_s": round(batched, 3), "sequential_s_per_case": round(sequential, 4),
+          "speedup_vs_sequential": round(sequential * len(F) / batched, 2),
+          "iterations_max": max(i["iterations"] for i in infos),
+          "rel_err_last_checked_vs_sequential": err})
+
+
+def run_box_batch(args, ma, meshes):
+    """64 Tet4 boxes of uneven size in one batch (ml_batch) against per-model solves."""
+    boxes = meshes.box_batch(count=args.models)
+    models = [ma.Model(b) for b in boxes]
+    log(f"boxes: {len(boxes)} models, {sum(b.nfree for b in boxes)} free DOF in all")
+    t0 = time.perf_counter()
+    refs = []
+    for m in models:
+        m.factor()
+        u, _, _ = m.solve(reactions=False)
+        refs.append(u)
+    sequential = time.perf_counter() - t0
+    out = {}
+    for name, code in (("fp64", ma.FP64), ("mixed_ic0", ma.MIXED_IC0)):
+        t0 = time.perf_counter()
+        batch = ma.Batch(models, precision=code)
+        setup = time.perf_counter() - t0
+        batch.solve()
+        t0 = time.perf_counter()
+        us, infos = batch.solve()
+        solve = time.perf_counter() - t0
+        err = max(float(np.linalg.norm(u - r) / np.linalg.norm(r)) for u, r in zip(us, refs))
+        out[name] = {"batch_setup_s": round(setup, 3), "batch_solve_s": round(solve, 4),
+                     "max_rel_err_vs_separate_fp64": err,
+                     "iterations_max": max(i["iterations"] for i in infos)}
+        del batch
+    emit({"metric": "models per second, 64 uneven Tet4 boxes (500-20,000 DOF), batched solve",
+          "value": round(len(models) / out["fp64"]["batch_solve_s"], 2), "unit": "models/s",
+          "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "dtype": "fp64",
+          "config": {"workload": "box_batch", "models": len(models),
+                     "free_dof": sum(b.nfree for b in boxes)},
+          "sequential_factor_and_solve_s": round(sequential, 3), "modes": out})
+
+
+def run_tet10(args, ma, meshes):
+    """The Tet10 cantilever (60x12x12 cells, ~225k free DOF): FP64 against the mixed modes."""
+    p = meshes.cantilever_tet10()
+    res = {}
+    u64 = None
+    for name, code in (("fp64", ma.FP64), ("mixed_ic0", ma.MIXED_IC0), ("mixed", ma.MIXED)):
+        m = ma.Model(p, precision=code)
+        m.factor()
+        m.solve(reactions=False)
+        t0 = time.perf_counter()
+        m.assemble()
+        m.factor()
+        u, r, info = m.solve()
+        step = time.perf_counter() - t0
+        if u64 is None:
+            u64 = u
+        f = m.loads()
+        balance = float(np.abs(r.reshape(-1, 3).sum(axis=0) + f.reshape(-1, 3).sum(axis=0)).max()
+                        / np.abs(f.reshape(-1, 3).sum(axis=0)).max())
+        res[name] = {"s_per_step": round(step, 3), "iterations": info["iterations"],
+                     "true_residual": info["true_residual"], "escalated": info["escalated"],
+                     "rel_err_vs_fp64": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
+                     "reaction_balance": balance}
+        log(f"tet10 {name}: {res[name]}")
+        del m
+    best = min((k for k in res if res[k]["rel_err_vs_fp64"] <= 1e-8), key=lambda k: res[k]["s_per_step"])
+    emit({"metric": "Tet10 cantilever solve s (assemble + IC(0) + PCG), FP64 vs mixed",
+          "value": res[best]["s_per_step"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
+          "data": "synthetic", "dtype": best,
+          "config": {"workload": "cantilever_tet10_60x12x12", "free_dof": p.nfree},
+          "modes": res})
+
+
+def run_simp(args, ma, meshes):
+    """Adjoint compliance gradient on the SIMP box (80x40x40 cells, ~403k free DOF): solve,
+    gradient, central differences on random elements; the CPU adjoint timed beside it."""
+    p = meshes.simp_box()
+    m = ma.Model(p)
+    m.factor()
+    t0 = time.perf_counter()
+    u, _, info = m.solve(reactions=False)
+    t_solve = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = m.compliance_gradient()
+    t_grad = time.perf_counter() - t0
+    rng = np.random.default_rng(6)
+    worst = 0.0
+    for e in rng.choice(len(p.conn), args.fd, replace=False):
+        h = 1e-3 * p.young[e]
+        cs = []
+        for sgn in (1.0, -1.0):
+            young = p.young.copy()
+            young[e] += sgn * h
+            m.set_young(young)
+            m.assemble()
+            m.factor()
+            ue, _, _ = m.solve(reactions=False)
+            cs.append(float(p.f @ ue))
+        fd = (cs[0] - cs[1]) / (2 * h)
+        worst = max(worst, abs(fd - g[e]) / abs(g[e]))
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        oracle.compliance_gradient(p.nodes, p.conn, p.young, p.nu, u)
+        cpu = {"value": round(time.perf_counter() - t0, 3), "unit": "s", "cores": 1,
+               "kind": "port", "sample": "the oracle's FP64 CPU adjoint over all elements at the "
+                                         "device solution"}
+    emit({"metric": "adjoint compliance gradient s, SIMP Tet4 box 80x40x40 (403,440 free DOF)",
+          "value": round(t_grad, 4), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+          "data": "synthetic", "dtype": "fp64",
+          "config": {"workload": "simp_box_80x40x40", "free_dof": p.nfree,
+                     "elements": len(p.conn)},
+          "solve_s": round(t_solve, 3), "iterations": info["iterations"],
+          "fd_elements": args.fd, "fd_max_rel_diff": worst, "cpu_baseline": cpu})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +224,13 @@ def main():
                          "displacements stay within 1e-8 of the FP64 drop-in")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=8)
+    ap.add_argument("--workload", choices=["cube", "beam_cases", "box_batch", "tet10", "simp"],
+                    default="cube", help="cube: the headline 3.1M-DOF solve; the others are the "
+                    "batched, Tet10 and adjoint workloads (one JSON line each)")
+    ap.add_argument("--cases", type=int, default=256)
+    ap.add_argument("--seq", type=int, default=8, help="sequential solves timed (beam_cases)")
+    ap.add_argument("--models", type=int, default=64)
+    ap.add_argument("--fd", type=int, default=32, help="elements checked by central differences")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding "
@@ -90,6 +241,11 @@ def main():
 
     import maille_amd as ma
     from maille_amd import meshes
+
+    if args.workload != "cube":
+        {"beam_cases": run_beam_cases, "box_batch": run_box_batch, "tet10": run_tet10,
+         "simp": run_simp}[args.workload](args, ma, meshes)
+        return
 
     t0 = time.time()
     problem = meshes.jittered_cube(args.cells)
@@ -102,8 +258,10 @@ def main():
     for prec in modes:
         t1 = time.time()
         models[prec] = ma.Model(problem, precision=codes[prec])
-        log(f"{prec}: model created in {time.time() - t1:.1f} s (host pattern "
-            f"{models[prec].create_info['seconds'][5]:.1f} s)")
+        chosen = models[prec].autotune()
+        log(f"{prec}: model created and autotuned in {time.time() - t1:.1f} s (host pattern "
+            f"{models[prec].create_info['seconds'][5]:.1f} s; {chosen[0]} waves per CU of "
+            f"{chosen[2]})")
     m0 = models["fp64"]
     f = m0.loads()
     dev = torch.device("cuda:0")

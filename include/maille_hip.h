@@ -237,6 +237,14 @@ ml_status ml_batch_solve(ml_batch* b, const double* const* f, double* const* u,
                          ml_solve_info* infos);
 void ml_batch_destroy(ml_batch* b);
 
+/* Autotuner (no counterpart in the reference): times the triangular solves of this handle's
+ * factor on candidate grids (waves per CU) and, for nrhs > 1, the number of right-hand sides
+ * per batched pass, and keeps the fastest for later solves. Results do not change (the
+ * solves are bit-exact under any schedule). chosen[3] (optional): waves per CU, right-hand
+ * sides per pass, reported occupancy per CU. Factorises first if needed. */
+ml_status ml_pcg_autotune(ml_pcg* h, int64_t nrhs, int32_t* chosen);
+ml_status ml_model_autotune(ml_model* m, int64_t nrhs, int32_t* chosen);
+
 /* Kernel-duration accounting with HIP events on the model's stream (no counterpart in the
  * reference; for benchmarks). Returns the accumulated milliseconds and launch counts per kind
  * (0 SpMV FP64, 1 SpMV FP32, 2 preconditioner FP64 (forward + backward solve), 3

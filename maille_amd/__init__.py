@@ -44,6 +44,7 @@ EXPORTS = (
     "ml_model_nfree", "ml_model_nnz", "ml_model_levels", "ml_model_csr",
     "ml_model_compliance_gradient", "ml_model_destroy", "ml_model_profile",
     "ml_model_solve_many", "ml_batch_create", "ml_batch_solve", "ml_batch_destroy",
+    "ml_pcg_autotune", "ml_model_autotune",
 )
 
 
@@ -156,6 +157,8 @@ def lib():
             "ml_batch_create": (_int, [_i64, C.POINTER(C.c_void_p), _int, _PP]),
             "ml_batch_solve": (_int, [_P, C.POINTER(_D), C.POINTER(_D), _S]),
             "ml_batch_destroy": (None, [_P]),
+            "ml_pcg_autotune": (_int, [_P, _i64, _I32]),
+            "ml_model_autotune": (_int, [_P, _i64, _I32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -375,6 +378,13 @@ class Pcg:
         _check(lib().ml_pcg_factor(self._h, _d(l), C.byref(sh)))
         return l, sh.value
 
+    def autotune(self, nrhs: int = 1):
+        """Times candidate schedules and keeps the fastest; returns (waves per CU, right-hand
+        sides per pass, reported occupancy per CU)."""
+        out = (C.c_int32 * 3)()
+        _check(lib().ml_pcg_autotune(self._h, int(nrhs), out))
+        return tuple(out)
+
     def set_wait_limit(self, ticks: int = 0):
         """Bound of the device's dataflow waits in wall-clock ticks (0: the default, 2 s)."""
         _check(lib().ml_pcg_set_wait_limit(self._h, int(ticks)))
@@ -447,6 +457,13 @@ class Model:
     def set_young(self, young):
         young = _f64(young)
         _check(lib().ml_model_set_young(self._h, _d(young)))
+
+    def autotune(self, nrhs: int = 1):
+        """Times candidate schedules and keeps the fastest; returns (waves per CU, right-hand
+        sides per pass, reported occupancy per CU)."""
+        out = (C.c_int32 * 3)()
+        _check(lib().ml_model_autotune(self._h, int(nrhs), out))
+        return tuple(out)
 
     def solve_many(self, F):
         """Load cases F (rows: nodal loads over all DOFs, without dead pressures) solved

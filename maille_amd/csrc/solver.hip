@@ -138,10 +138,13 @@ int device_cus() {
 // the margin costs little (their speed hardly changes between 7 and 13 waves per CU). A
 // dataflow launch that stalls anyway ends through its bounded waits, and the caller then
 // repeats the work on a smaller grid (Operator::shrink).
-int dataflow_grid(const void* kernel) {
+int occupancy_per_cu(const void* kernel) {
   int per_cu = 0;
   HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0));
-  return std::max(1, (3 * per_cu) / 4) * device_cus();
+  return std::max(1, per_cu);
+}
+int dataflow_grid(const void* kernel) {
+  return std::max(1, (3 * occupancy_per_cu(kernel)) / 4) * device_cus();
 }
 
 // The bound of every dataflow wait: 2 s of the device's steady wall clock (wall_clock64 ticks
@@ -237,6 +240,7 @@ struct Operator {
   // grids of the dataflow kernels: the full ones, and the ones in use (full >> shrinks)
   int full_tri64 = 0, full_tri32 = 0, full_ic = 0, full_ic64 = 0;
   int cap_tri64 = 0, cap_tri32 = 0;  // dataflow grids before the cap by block count
+  int rhs_per_pass = 256;            // right-hand sides per batched pass (autotuned)
   int tri_grid64 = 0, tri_grid32 = 0, ic_grid = 0, ic_grid64 = 0;
   int shrinks = 0;   // grid halvings after a stalled dataflow launch
   int retries = 0;   // work repeated on a smaller grid (reported in ml_solve_info)
@@ -586,6 +590,103 @@ struct Operator {
     if (hS->status == 3) return PCG_STALLED;
     if (hS->status == 1) return ML_SINGULAR;
     if (hS->status == 2) return ML_NOT_CONVERGED;
+    return ML_OK;
+  }
+
+  // ---------------------------------------------------------------- autotuner
+  // Average duration (ms) of `reps` preconditioner applications to w_r (or nrep copies of it)
+  // on the current grids; negative if a launch stalled.
+  template <typename T>
+  double time_precondition(int reps, int nrep, T* r, T* y, T* z) {
+    const int64_t N = (int64_t)n * nrep;
+    hipEvent_t a, b;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);
+    float total = 0.0f;
+    for (int k = 0; k <= reps; ++k) {  // the first application warms up
+      mlk::k_fill_bits<T><<<grid_for(N), 256, 0, stream>>>(y, N, mlk::Bits<T>::S);
+      mlk::k_fill_bits<T><<<grid_for(N), 256, 0, stream>>>(z, N, mlk::Bits<T>::S);
+      HIP_CHECK(hipEventRecord(a, stream));
+      precondition<T>(r, y, z, nullptr, nullptr, nullptr, nrep);
+      HIP_CHECK(hipEventRecord(b, stream));
+      HIP_CHECK(hipEventSynchronize(b));
+      float t = 0.0f;
+      HIP_CHECK(hipEventElapsedTime(&t, a, b));
+      if (k > 0) total += t;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (read_flag(0)) return -1.0;
+    return total / reps;
+  }
+
+  // Picks, by timing, the waves per CU of the triangular solves (single right-hand side) and,
+  // for nrhs > 1, the right-hand sides per batched pass. Results do not change: the solves are
+  // bit-exact under any schedule. chosen[3]: waves per CU, right-hand sides per pass, and the
+  // per-CU occupancy the runtime reports.
+  int autotune(int64_t nrhs, int32_t* chosen) {
+    if (!factored) {
+      int st = factor();
+      if (st != ML_OK) return st;
+    }
+    const bool f32 = precision != ML_FP64;
+    const int occ = f32 ? occupancy_per_cu((const void*)mlk::k_trisolve<float, false, false>)
+                        : occupancy_per_cu((const void*)mlk::k_trisolve<double, false, false>);
+    const int cus = device_cus();
+    const int nblk = std::max(fw.nblk, bw.nblk);
+    const int top = std::max(1, (17 * occ) / 20);  // stay clear of the reported occupancy
+    mlk::k_fill_f64<<<grid_for(n), 256, 0, stream>>>(w_r.p, n, 1.0);
+    if (f32) mlk::k_convert<double, float><<<grid_for(n), 256, 0, stream>>>(n, w_r.p, w_r32.p);
+    double best = 1e300;
+    int best_pc = std::max(1, (3 * occ) / 4);
+    for (int pc = 2; pc <= top; pc += 2) {
+      (f32 ? full_tri32 : full_tri64) = std::min(nblk, pc * cus);
+      shrinks = 0;
+      set_grids();
+      const double t = f32 ? time_precondition<float>(3, 1, w_r32.p, w_y32.p, w_z32.p)
+                           : time_precondition<double>(3, 1, w_r.p, w_y.p, w_z.p);
+      if (t > 0 && t < best) {
+        best = t;
+        best_pc = pc;
+      }
+    }
+    // the replicated solves of batched passes keep their own grid (cap_tri): with many copies
+    // of every block they are bound by throughput, not by the dependency chain
+    (f32 ? full_tri32 : full_tri64) = std::min(nblk, best_pc * cus);
+    shrinks = 0;
+    set_grids();
+    if (nrhs > 1) {
+      // right-hand sides per pass: the cheapest application per right-hand side
+      double best_per = 1e300;
+      int best_k = rhs_per_pass;
+      for (int k : {8, 16, 32, 64, 128, 256}) {
+        if (k > nrhs && k != 8) break;
+        const int64_t N = (int64_t)n * k;
+        DBuf<double> r64, y64, z64;
+        DBuf<float> r32, y32, z32;
+        double t;
+        if (f32) {
+          r32.alloc(N); y32.alloc(N); z32.alloc(N);
+          mlk::k_fill_bits<float><<<grid_for(N), 256, 0, stream>>>(r32.p, N, 0x3f800000u);
+          t = time_precondition<float>(2, k, r32.p, y32.p, z32.p);
+        } else {
+          r64.alloc(N); y64.alloc(N); z64.alloc(N);
+          mlk::k_fill_f64<<<grid_for(N), 256, 0, stream>>>(r64.p, N, 1.0);
+          t = time_precondition<double>(2, k, r64.p, y64.p, z64.p);
+        }
+        if (t > 0 && t / k < best_per) {
+          best_per = t / k;
+          best_k = k;
+        }
+      }
+      rhs_per_pass = best_k;
+    }
+    if (chosen) {
+      chosen[0] = best_pc;
+      chosen[1] = rhs_per_pass;
+      chosen[2] = occ;
+    }
     return ML_OK;
   }
 
@@ -1300,8 +1401,6 @@ ml_status ml_pcg_solve_device(ml_pcg* h, const double* b, double* x, double tol,
 
 namespace {
 
-constexpr int64_t MAX_BATCH_RHS = 256;  // right-hand sides per batched pass
-
 int status_of(const mlk::PcgScalars& q) {
   return q.status == 1 ? ML_SINGULAR : q.status == 2 ? ML_NOT_CONVERGED : ML_OK;
 }
@@ -1329,8 +1428,8 @@ ml_status ml_pcg_solve(ml_pcg* h, const double* b, double* x, int64_t nrhs, doub
       const int retries0 = op.retries;
       mlk::PcgScalars worst;
       std::memset(&worst, 0, sizeof(worst));
-      for (int64_t k0 = 0; k0 < nrhs; k0 += MAX_BATCH_RHS) {
-        const int k = (int)std::min(MAX_BATCH_RHS, nrhs - k0);
+      for (int64_t k0 = 0; k0 < nrhs; k0 += op.rhs_per_pass) {
+        const int k = (int)std::min<int64_t>(op.rhs_per_pass, nrhs - k0);
         DBuf<double> bd, xd;
         bd.upload(b + k0 * n, (size_t)k * n, op.stream);
         xd.alloc((size_t)k * n);
@@ -1629,8 +1728,8 @@ ml_status ml_model_solve_many(ml_model* h, int64_t nrhs, const double* f, double
     Operator& op = m.op;
     const int64_t ndof = m.plan.ndof, nf = m.plan.nfree;
     int result = ML_OK;
-    for (int64_t k0 = 0; k0 < nrhs; k0 += MAX_BATCH_RHS) {
-      const int k = (int)std::min(MAX_BATCH_RHS, nrhs - k0);
+    for (int64_t k0 = 0; k0 < nrhs; k0 += op.rhs_per_pass) {
+      const int k = (int)std::min<int64_t>(op.rhs_per_pass, nrhs - k0);
       DBuf<double> fd, rhs, xd, ud;
       fd.upload(f + k0 * ndof, (size_t)k * ndof, op.stream);
       rhs.alloc((size_t)k * nf);
@@ -1755,6 +1854,21 @@ ml_status ml_batch_solve(ml_batch* b, const double* const* f, double* const* u,
 }
 
 void ml_batch_destroy(ml_batch* b) { delete b; }
+
+ml_status ml_pcg_autotune(ml_pcg* h, int64_t nrhs, int32_t* chosen) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    return h->op.autotune(nrhs, chosen);
+  });
+}
+
+ml_status ml_model_autotune(ml_model* h, int64_t nrhs, int32_t* chosen) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    if (!h->m.assembled) h->m.assemble();
+    return h->m.op.autotune(nrhs, chosen);
+  });
+}
 
 void ml_model_destroy(ml_model* h) { delete h; }
 

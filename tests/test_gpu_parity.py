@@ -278,3 +278,21 @@ def test_batch_of_uneven_models(gpu, precision):
         assert info["shift_index"] == ref["shift_index"] or precision != "FP64"
         u1, _, _ = m.solve(reactions=False)
         assert rel_l2(u, u1) <= bound
+
+
+def test_autotuner_keeps_results(gpu):
+    """The autotuner only changes schedules: the preconditioner and the solution stay bit for
+    bit the same after it has picked its grids."""
+    p = PROBLEMS["jittered_cube_8"]
+    s = _oracle_system(p)
+    off, cols, vals = s.csr()
+    pcg = gpu.Pcg(gpu.Csr(off, cols, vals))
+    r = np.cos(0.11 * np.arange(s.n))
+    b = np.sin(0.37 * np.arange(s.n))
+    z0, x0 = pcg.precondition(r), pcg.solve(b)
+    waves, per_pass, occ = pcg.autotune(nrhs=16)
+    print(f"autotuned: {waves} waves per CU (occupancy {occ}), {per_pass} right-hand sides per pass",
+          flush=True)
+    assert 1 <= waves < occ and per_pass in (8, 16, 32, 64, 128, 256)
+    assert np.array_equal(pcg.precondition(r), z0)
+    assert np.array_equal(pcg.solve(b), x0)
