@@ -821,10 +821,12 @@ __global__ __launch_bounds__(256) void k_axpy_add(int n, double* __restrict__ x,
 //   SUBSEQ = true:  out_l = ((rhs_l - val_1 out_1) - val_2 out_2 ...) / diag_l (backward loop,
 //                   iterative.rs:180-186: subtractions in decreasing row order)
 //
-// Rows are cut on the host into blocks of at most 64 consecutive rows that never straddle a
-// break of the dependency chain (bstart[]); each row's entries are split into those whose
-// column lies before its block (xcol/xval, "external") and those inside it (ik = column minus
-// block start, ival), both stored block after block. Wave w of the G-wave grid takes the blocks
+// Rows are grouped on the host into blocks: chains of at most 64 rows along each row's critical
+// dependency, stored as consecutive slots (bstart[]; brow[] is the row of a slot). Each slot's
+// entries are split into those whose column lies in another block (xcol/xval, "external") and
+// those inside it (ik = position of the row in the block, ival); the internal columns are the
+// row's last, so externals then internals is the row's own order. Both parts are stored slot
+// after slot. Wave w of the G-wave grid takes the blocks
 // at positions w, w+G, ... of the host's processing order border[], which is topological (a
 // block's external columns all lie in blocks earlier in border[]): with every wave resident the
 // lowest unfinished block always has its inputs and a running wave, so the solve progresses.
@@ -846,7 +848,8 @@ constexpr int TRI_ICAP = 512;  // internal entries a block may hold (host-enforc
 template <typename T, bool REV, bool SUBSEQ>
 __global__ __launch_bounds__(64) void k_trisolve(
     int n, int nblocks, const int* __restrict__ border, const int* __restrict__ bstart,
-    const int* __restrict__ xoff, const int* __restrict__ xcol, const T* __restrict__ xval,
+    const int* __restrict__ brow, const int* __restrict__ xoff, const int* __restrict__ xcol,
+    const T* __restrict__ xval,
     const int* __restrict__ ioff, const unsigned char* __restrict__ ik,
     const T* __restrict__ ival, const T* __restrict__ diag, const T* __restrict__ rhs, T* out,
     const PcgScalars* __restrict__ S, int* abort_flag, long long limit,
@@ -892,8 +895,9 @@ __global__ __launch_bounds__(64) void k_trisolve(
       sia[q] = ival[ib + q];
     }
     const bool active = lane < rows;
-    const int l = s + lane;
-    const int pl = REV ? n - 1 - l : l;
+    const int l = s + lane;  // slot
+    const int row = active ? brow[l] : 0;
+    const int pl = REV ? n - 1 - row : row;
     int p = 0, x1 = 0, ip = 0, i1 = 0;
     T acc = T(0), rl = T(0), dl = T(1);
     if (active) {

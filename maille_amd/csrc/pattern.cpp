@@ -282,87 +282,108 @@ i64 dag_depth(const LowerPattern& lp) {
   return depth;
 }
 
-std::vector<int32_t> dataflow_blocks(int64_t n, const int32_t* off, const int32_t* col,
-                                     int max_rows) {
-  std::vector<int32_t> level(n, 0);
-  std::vector<int32_t> starts;
-  starts.reserve(n / 16 + 2);
-  for (int64_t i = 0; i < n; ++i) {
-    int32_t m = 0;
-    for (int64_t p = off[i]; p < off[i + 1]; ++p) m = std::max(m, level[col[p]] + 1);
-    level[i] = m;
-    if (i == 0 || i - starts.back() >= max_rows || level[i] <= level[i - 1])
-      starts.push_back((int32_t)i);
-  }
-  starts.push_back((int32_t)n);
-  return starts;
-}
-
 TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const int32_t* src,
                      const int32_t* dsrc, int max_rows, int xcap, int icap, std::string* err) {
   TriLayout t;
-  std::vector<int32_t> level(n, 0);
-  for (int64_t i = 0; i < n; ++i) {
-    int32_t m = 0;
-    for (int64_t p = off[i]; p < off[i + 1]; ++p) m = std::max(m, level[col[p]] + 1);
-    level[i] = m;
-  }
-  t.bstart.push_back(0);
-  int64_t bx = 0, bi = 0;  // external / internal entries of the open block
+  // chains: rows in increasing order; a row joins the chain of its critical predecessor when
+  // that chain ends in it, has room (rows, external and internal entries), every other
+  // dependency lies in an older chain, and the row's entries inside the chain are the last of
+  // its columns (so a row reads as external entries, then internal ones, in its own order)
+  std::vector<int32_t> chain_of(n, -1), clen, clast, cext, cint;
+  std::vector<double> finish(n, 0.0);
   for (int64_t i = 0; i < n; ++i) {
     const int64_t len = off[i + 1] - off[i];
-    if (len > xcap) {
+    if (len > xcap || len > icap) {
       *err = "matrix row longer than the triangular-solve block capacity";
       return t;
     }
-    if (i > 0) {
-      const int64_t s = t.bstart.back();
-      int64_t nin = 0;
-      for (int64_t p = off[i]; p < off[i + 1]; ++p) nin += col[p] >= s;
-      const bool cut = i - s >= max_rows || level[i] <= level[i - 1] ||
-                       bx + (len - nin) > xcap || bi + nin > icap;
-      if (cut) {
-        t.bstart.push_back((int32_t)i);
-        bx = 0;
-        bi = 0;
+    int32_t best = -1, newest = -1;
+    double bf = -1.0;
+    for (int64_t p = off[i]; p < off[i + 1]; ++p) {
+      const int32_t j = col[p];
+      if (finish[j] > bf) { bf = finish[j]; best = j; }
+      newest = std::max(newest, chain_of[j]);
+    }
+    int32_t ch = -1;
+    if (best >= 0) {
+      const int32_t cb = chain_of[best];
+      if (clast[cb] == best && clen[cb] < max_rows && newest <= cb) {
+        int64_t ext = 0, in = 0;
+        bool suffix = true;
+        for (int64_t p = off[i]; p < off[i + 1]; ++p) {
+          if (chain_of[col[p]] == cb) {
+            ++in;
+          } else {
+            ++ext;
+            if (in > 0) suffix = false;
+          }
+        }
+        if (suffix && cext[cb] + ext <= xcap && cint[cb] + in <= icap) ch = cb;
       }
     }
-    const int64_t s = t.bstart.back();
-    for (int64_t p = off[i]; p < off[i + 1]; ++p) (col[p] >= s ? bi : bx) += 1;
-    t.max_block_ext = std::max<int>(t.max_block_ext, (int)bx);
-    t.max_block_int = std::max<int>(t.max_block_int, (int)bi);
+    if (ch < 0) {
+      ch = (int32_t)clen.size();
+      clen.push_back(0);
+      clast.push_back(-1);
+      cext.push_back(0);
+      cint.push_back(0);
+    }
+    double f = clast[ch] >= 0 ? finish[clast[ch]] + CHAIN_ROW_US : CHAIN_ROW_US;
+    for (int64_t p = off[i]; p < off[i + 1]; ++p) {
+      const int32_t j = col[p];
+      if (chain_of[j] != ch) {
+        f = std::max(f, finish[j] + CHAIN_HOP_US + CHAIN_ROW_US);
+        cext[ch] += 1;
+      } else {
+        cint[ch] += 1;
+      }
+    }
+    finish[i] = f;
+    chain_of[i] = ch;
+    clen[ch] += 1;
+    clast[ch] = (int32_t)i;
   }
-  t.bstart.push_back((int32_t)n);
+  // blocks = chains in creation order; their rows become consecutive slots
+  const int64_t nb = (int64_t)clen.size();
+  t.bstart.assign(nb + 1, 0);
+  for (int64_t b = 0; b < nb; ++b) t.bstart[b + 1] = t.bstart[b] + clen[b];
+  t.brow.resize(n);
+  std::vector<int32_t> pos(n), fill(t.bstart.begin(), t.bstart.end() - 1);
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t q = fill[chain_of[i]]++;
+    t.brow[q] = (int32_t)i;
+    pos[i] = q - t.bstart[chain_of[i]];
+  }
   t.xoff.assign(n + 1, 0);
   t.ioff.assign(n + 1, 0);
-  t.dsrc.assign(dsrc, dsrc + n);
-  for (size_t b = 0; b + 1 < t.bstart.size(); ++b) {
-    const int32_t s = t.bstart[b];
-    for (int32_t i = s; i < t.bstart[b + 1]; ++i) {
-      for (int64_t p = off[i]; p < off[i + 1]; ++p) {
-        if (col[p] < s) {
-          t.xcol.push_back(col[p]);
-          t.xsrc.push_back(src[p]);
-        } else {
-          t.ik.push_back(col[p] - s);
-          t.isrc.push_back(src[p]);
-        }
+  t.dsrc.resize(n);
+  for (int64_t q = 0; q < n; ++q) {
+    const int32_t i = t.brow[q];
+    for (int64_t p = off[i]; p < off[i + 1]; ++p) {
+      if (chain_of[col[p]] != chain_of[i]) {
+        t.xcol.push_back(col[p]);
+        t.xsrc.push_back(src[p]);
+      } else {
+        t.ik.push_back(pos[col[p]]);
+        t.isrc.push_back(src[p]);
       }
-      t.xoff[i + 1] = (int32_t)t.xcol.size();
-      t.ioff[i + 1] = (int32_t)t.ik.size();
     }
+    t.xoff[q + 1] = (int32_t)t.xcol.size();
+    t.ioff[q + 1] = (int32_t)t.ik.size();
+    t.dsrc[q] = dsrc[i];
+  }
+  for (int64_t b = 0; b < nb; ++b) {
+    t.max_block_ext = std::max(t.max_block_ext, t.xoff[t.bstart[b + 1]] - t.xoff[t.bstart[b]]);
+    t.max_block_int = std::max(t.max_block_int, t.ioff[t.bstart[b + 1]] - t.ioff[t.bstart[b]]);
   }
   // Processing order: blocks by their level in the block dependency graph (a block depends
-  // on the blocks holding its external columns), ties in index order. It is a topological
+  // on the blocks holding its external columns), ties in creation order. It is a topological
   // order, so a wave never waits for a block that comes after its own in the sequence.
-  const int64_t nb = (int64_t)t.bstart.size() - 1;
-  std::vector<int32_t> block_of(n), blev(nb, 0);
-  for (int64_t b = 0; b < nb; ++b)
-    for (int32_t i = t.bstart[b]; i < t.bstart[b + 1]; ++i) block_of[i] = (int32_t)b;
+  std::vector<int32_t> blev(nb, 0);
   for (int64_t b = 0; b < nb; ++b) {
     int32_t m = 0;
     for (int32_t q = t.xoff[t.bstart[b]]; q < t.xoff[t.bstart[b + 1]]; ++q)
-      m = std::max(m, blev[block_of[t.xcol[q]]] + 1);
+      m = std::max(m, blev[chain_of[t.xcol[q]]] + 1);
     blev[b] = m;
   }
   t.border.resize(nb);
@@ -435,21 +456,26 @@ std::string check_tri(const char* name, const TriLayout& t, i64 n, i64 nnz, int 
                       const std::vector<int32_t>& rows_src) {
   const std::string pre = std::string(name) + " layout: ";
   if (t.bstart.size() < 2 || t.bstart.front() != 0 || t.bstart.back() != n)
-    return pre + "block starts do not span the rows";
+    return pre + "blocks do not span the rows";
   const i64 nb = (i64)t.bstart.size() - 1;
-  for (i64 b = 0; b < nb; ++b) {
-    const i64 rows = (i64)t.bstart[b + 1] - t.bstart[b];
-    if (rows < 1 || rows > 64) return pre + "block of " + std::to_string(rows) + " rows";
-  }
-  if ((i64)t.xoff.size() != n + 1 || (i64)t.ioff.size() != n + 1 || (i64)t.dsrc.size() != n)
-    return pre + "offset arrays of the wrong length";
+  if ((i64)t.brow.size() != n || (i64)t.xoff.size() != n + 1 || (i64)t.ioff.size() != n + 1 ||
+      (i64)t.dsrc.size() != n)
+    return pre + "slot arrays of the wrong length";
   if (t.xoff[0] != 0 || t.ioff[0] != 0 || (size_t)t.xoff[n] != t.xcol.size() ||
       t.xcol.size() != t.xsrc.size() || (size_t)t.ioff[n] != t.ik.size() ||
       t.ik.size() != t.isrc.size())
     return pre + "entry arrays do not match their offsets";
-  std::vector<int32_t> block_of(n), pos(nb, -1);
-  for (i64 b = 0; b < nb; ++b)
-    for (i64 i = t.bstart[b]; i < t.bstart[b + 1]; ++i) block_of[i] = (int32_t)b;
+  std::vector<int32_t> block_of(n, -1), pos(nb, -1);
+  for (i64 b = 0; b < nb; ++b) {
+    const i64 rows = (i64)t.bstart[b + 1] - t.bstart[b];
+    if (rows < 1 || rows > 64) return pre + "block of " + std::to_string(rows) + " rows";
+    for (i64 q = t.bstart[b]; q < t.bstart[b + 1]; ++q) {
+      const i64 i = t.brow[q];
+      if (i < 0 || i >= n || block_of[i] >= 0) return pre + "slots are not a permutation of the rows";
+      if (q > t.bstart[b] && t.brow[q] <= t.brow[q - 1]) return pre + "block rows not ascending";
+      block_of[i] = (int32_t)b;
+    }
+  }
   if ((i64)t.border.size() != nb) return pre + "processing order of the wrong length";
   for (i64 k = 0; k < nb; ++k) {
     const i64 b = t.border[k];
@@ -460,26 +486,29 @@ std::string check_tri(const char* name, const TriLayout& t, i64 n, i64 nnz, int 
     const i64 s = t.bstart[b], e = t.bstart[b + 1];
     if (t.xoff[e] - t.xoff[s] > xcap) return pre + "block over the external capacity";
     if (t.ioff[e] - t.ioff[s] > icap) return pre + "block over the internal capacity";
-    for (i64 i = s; i < e; ++i) {
-      if (t.xoff[i + 1] < t.xoff[i] || t.ioff[i + 1] < t.ioff[i])
-        return pre + "decreasing offsets at row " + std::to_string(i);
-      if (t.dsrc[i] < 0 || t.dsrc[i] >= nnz) return pre + "diagonal source out of range";
-      // the row's entries, externals then internals, are its logical entries in order
+    for (i64 q = s; q < e; ++q) {
+      const i64 i = t.brow[q], k = q - s;
+      if (t.xoff[q + 1] < t.xoff[q] || t.ioff[q + 1] < t.ioff[q])
+        return pre + "decreasing offsets at slot " + std::to_string(q);
+      if (t.dsrc[q] < 0 || t.dsrc[q] >= nnz) return pre + "diagonal source out of range";
+      // the slot's entries, externals then internals, are its row's logical entries in order
       i64 p = rows_off[i];
-      if ((t.xoff[i + 1] - t.xoff[i]) + (t.ioff[i + 1] - t.ioff[i]) != rows_off[i + 1] - p)
+      if ((t.xoff[q + 1] - t.xoff[q]) + (t.ioff[q + 1] - t.ioff[q]) != rows_off[i + 1] - p)
         return pre + "row " + std::to_string(i) + " does not hold its entries";
-      for (i64 q = t.xoff[i]; q < t.xoff[i + 1]; ++q, ++p) {
-        const i64 c = t.xcol[q];
-        if (c < 0 || c >= s) return pre + "external column outside the earlier rows";
-        if (c != rows_col[p] || t.xsrc[q] != rows_src[p]) return pre + "external entry out of order";
-        if (t.xsrc[q] < 0 || t.xsrc[q] >= nnz) return pre + "external source out of range";
+      for (i64 x = t.xoff[q]; x < t.xoff[q + 1]; ++x, ++p) {
+        const i64 c = t.xcol[x];
+        if (c < 0 || c >= n) return pre + "external column out of range";
+        if (block_of[c] == b) return pre + "external entry inside its own block";
+        if (c != rows_col[p] || t.xsrc[x] != rows_src[p]) return pre + "external entry out of order";
+        if (t.xsrc[x] < 0 || t.xsrc[x] >= nnz) return pre + "external source out of range";
         if (pos[block_of[c]] >= pos[b]) return pre + "processing order is not topological";
       }
-      for (i64 q = t.ioff[i]; q < t.ioff[i + 1]; ++q, ++p) {
-        const i64 k = t.ik[q];
-        if (k < 0 || k >= i - s) return pre + "internal entry not before its row";
-        if (s + k != rows_col[p] || t.isrc[q] != rows_src[p]) return pre + "internal entry out of order";
-        if (t.isrc[q] < 0 || t.isrc[q] >= nnz) return pre + "internal source out of range";
+      for (i64 x = t.ioff[q]; x < t.ioff[q + 1]; ++x, ++p) {
+        const i64 kk = t.ik[x];
+        if (kk < 0 || kk >= k) return pre + "internal entry not before its row";
+        if (t.brow[s + kk] != rows_col[p] || t.isrc[x] != rows_src[p])
+          return pre + "internal entry out of order";
+        if (t.isrc[x] < 0 || t.isrc[x] >= nnz) return pre + "internal source out of range";
       }
     }
   }
@@ -519,8 +548,8 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, int xc
     }
   std::string err = check_tri("forward", d.fw, n, nnz, xcap, icap, foff, fcol, fsrc);
   if (!err.empty()) return err;
-  for (i64 i = 0; i < n; ++i)
-    if (d.fw.dsrc[i] != lp.offsets[i + 1] - 1) return "forward layout: wrong diagonal";
+  for (i64 q = 0; q < n; ++q)
+    if (d.fw.dsrc[q] != lp.offsets[d.fw.brow[q] + 1] - 1) return "forward layout: wrong diagonal";
   // backward rows, restated independently of device_layouts: entry (i, j) of L, i > j, goes
   // to logical row n-1-j; visiting i downwards appends each row's entries in ascending logical
   // column n-1-i
@@ -539,8 +568,8 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, int xc
   }
   err = check_tri("backward", d.bw, n, nnz, xcap, icap, boff, bcol, bsrc);
   if (!err.empty()) return err;
-  for (i64 l = 0; l < n; ++l)
-    if (d.bw.dsrc[l] != lp.offsets[n - l] - 1) return "backward layout: wrong diagonal";
+  for (i64 q = 0; q < n; ++q)
+    if (d.bw.dsrc[q] != lp.offsets[n - d.bw.brow[q]] - 1) return "backward layout: wrong diagonal";
   return std::string();
 }
 
@@ -549,24 +578,27 @@ void layout_precondition(const DeviceLayouts& d, const double* l, const double* 
   std::vector<double> y(n);
   // forward: y_i = (r_i - sum l_ij y_j) / l_ii, the sum in ascending j
   for (int32_t b : d.fw.border) {
-    const i64 s = d.fw.bstart[b], e = d.fw.bstart[b + 1];
-    for (i64 i = s; i < e; ++i) {
+    const i64 s = d.fw.bstart[b];
+    for (i64 q = s; q < d.fw.bstart[b + 1]; ++q) {
       double acc = 0.0;
-      for (i64 q = d.fw.xoff[i]; q < d.fw.xoff[i + 1]; ++q) acc = acc + l[d.fw.xsrc[q]] * y[d.fw.xcol[q]];
-      for (i64 q = d.fw.ioff[i]; q < d.fw.ioff[i + 1]; ++q) acc = acc + l[d.fw.isrc[q]] * y[s + d.fw.ik[q]];
-      y[i] = (r[i] - acc) / l[d.fw.dsrc[i]];
+      for (i64 x = d.fw.xoff[q]; x < d.fw.xoff[q + 1]; ++x) acc = acc + l[d.fw.xsrc[x]] * y[d.fw.xcol[x]];
+      for (i64 x = d.fw.ioff[q]; x < d.fw.ioff[q + 1]; ++x)
+        acc = acc + l[d.fw.isrc[x]] * y[d.fw.brow[s + d.fw.ik[x]]];
+      const i64 i = d.fw.brow[q];
+      y[i] = (r[i] - acc) / l[d.fw.dsrc[q]];
     }
   }
   // backward in logical rows, vectors indexed physically (n-1-logical)
   for (int32_t b : d.bw.border) {
-    const i64 s = d.bw.bstart[b], e = d.bw.bstart[b + 1];
-    for (i64 lr = s; lr < e; ++lr) {
+    const i64 s = d.bw.bstart[b];
+    for (i64 q = s; q < d.bw.bstart[b + 1]; ++q) {
+      const i64 lr = d.bw.brow[q];
       double acc = y[n - 1 - lr];
-      for (i64 q = d.bw.xoff[lr]; q < d.bw.xoff[lr + 1]; ++q)
-        acc = acc - l[d.bw.xsrc[q]] * z[n - 1 - d.bw.xcol[q]];
-      for (i64 q = d.bw.ioff[lr]; q < d.bw.ioff[lr + 1]; ++q)
-        acc = acc - l[d.bw.isrc[q]] * z[n - 1 - (s + d.bw.ik[q])];
-      z[n - 1 - lr] = acc / l[d.bw.dsrc[lr]];
+      for (i64 x = d.bw.xoff[q]; x < d.bw.xoff[q + 1]; ++x)
+        acc = acc - l[d.bw.xsrc[x]] * z[n - 1 - d.bw.xcol[x]];
+      for (i64 x = d.bw.ioff[q]; x < d.bw.ioff[q + 1]; ++x)
+        acc = acc - l[d.bw.isrc[x]] * z[n - 1 - d.bw.brow[s + d.bw.ik[x]]];
+      z[n - 1 - lr] = acc / l[d.bw.dsrc[q]];
     }
   }
 }

@@ -64,26 +64,34 @@ TransposeLayout transpose_layout(const LowerPattern& lp);
 // DAG depth of the lower-triangular solve (longest dependency chain, in rows).
 i64 dag_depth(const LowerPattern& lp);
 
-// Row blocks of the dataflow triangular solve over a strictly-lower layout (off[n+1], col[]):
-// consecutive rows, at most `max_rows` per block, and a new block wherever a row's level (its
-// longest dependency chain) does not exceed the previous row's. Inside a block the rows then
-// form one dependency chain, so solving them one after the other costs no false ordering;
-// across blocks the solve only waits for true dependencies. Returns the block starts, n last.
-std::vector<int32_t> dataflow_blocks(int64_t n, const int32_t* off, const int32_t* col,
-                                     int max_rows);
+// One dataflow triangular solve in logical row order, built from the strictly-lower layout
+// (off[n+1], col[]: a row's entries in the reference's summation order, ascending column),
+// entry sources src into the factor's value array and diagonal sources dsrc.
+//
+// Rows are grouped into chains ("blocks") of at most max_rows rows that one wavefront solves one
+// after the other, so that a dependency inside a chain costs no cross-wave hop. Rows are visited
+// in increasing order; a row joins the chain of its critical predecessor (the dependency
+// expected to finish last, by a model of CHAIN_HOP_US per cross-chain dependency and
+// CHAIN_ROW_US per row) when that chain ends in it, has room (max_rows rows, xcap entries from
+// outside the chain, icap from inside), every other dependency of the row lies in an older
+// chain (creation order is then topological) and the row's dependencies inside the chain are its
+// last columns. Otherwise the row starts a new chain.
+//
+// A block's rows occupy consecutive "slots" (brow: the logical row of a slot, ascending within
+// the block). Each slot's entries are split into the external part (columns in other blocks)
+// and the internal part (the position, 0..63, of a row of the same block); since the internal
+// columns are the row's last, externals then internals is the reference's order. Both parts
+// are stored slot after slot, so a wave loads a whole block with coalesced reads.
+constexpr double CHAIN_HOP_US = 3.0;
+constexpr double CHAIN_ROW_US = 0.25;
 
-// One dataflow triangular solve in logical row order (strictly-lower layout off/col, entry
-// sources src into the factor's value array, diagonal sources dsrc). Rows are cut into blocks
-// as dataflow_blocks does, also capped so a block holds at most xcap entries outside it and
-// icap inside it; each row's entries are split into the external part (columns before the
-// block) and the internal part (columns inside, stored as the 0..63 offset from the block
-// start), both block-contiguous so a wave loads a whole block with coalesced reads.
 struct TriLayout {
-  std::vector<int32_t> bstart;           // blocks + 1
+  std::vector<int32_t> bstart;           // blocks + 1: slot offsets
   std::vector<int32_t> border;           // processing order: blocks sorted by block-DAG level
-  std::vector<int32_t> xoff, xcol, xsrc;  // external entries: rows + 1, entries, sources
-  std::vector<int32_t> ioff, ik, isrc;    // internal entries
-  std::vector<int32_t> dsrc;              // diagonal source per row
+  std::vector<int32_t> brow;             // logical row of each slot
+  std::vector<int32_t> xoff, xcol, xsrc;  // external entries: slots + 1, columns, sources
+  std::vector<int32_t> ioff, ik, isrc;    // internal entries: slots + 1, positions, sources
+  std::vector<int32_t> dsrc;              // diagonal source per slot
   int max_block_ext = 0, max_block_int = 0;
 };
 TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const int32_t* src,

@@ -204,7 +204,7 @@ struct Operator {
   DBuf<float> d_l32;
   struct TriDev {
     int nblk = 0;
-    DBuf<int> border, bstart, xoff, xcol, xsrc, ioff, isrc, dsrc;
+    DBuf<int> border, bstart, brow, xoff, xcol, xsrc, ioff, isrc, dsrc;
     DBuf<unsigned char> ik;
     DBuf<double> xval, ival, diag;
     DBuf<float> xval32, ival32, diag32;
@@ -212,6 +212,7 @@ struct Operator {
       nblk = (int)t.bstart.size() - 1;
       border.upload_vec(t.border, st);
       bstart.upload_vec(t.bstart, st);
+      brow.upload_vec(t.brow, st);
       xoff.upload_vec(t.xoff, st);
       xcol.upload_vec(t.xcol, st);
       xsrc.upload_vec(t.xsrc, st);
@@ -460,12 +461,14 @@ struct Operator {
       int nb = t.nblk;
       if (fwd)
         launch_dataflow(mlk::k_trisolve<T, false, false>, grid, stream, nn, nb,
-                        (const int*)t.border.p, (const int*)t.bstart.p, (const int*)t.xoff.p, (const int*)t.xcol.p, xv,
+                        (const int*)t.border.p, (const int*)t.bstart.p, (const int*)t.brow.p,
+                        (const int*)t.xoff.p, (const int*)t.xcol.p, xv,
                         (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out, S,
                         d_flags.p, limit, trace, nrep);
       else
         launch_dataflow(mlk::k_trisolve<T, true, true>, grid, stream, nn, nb,
-                        (const int*)t.border.p, (const int*)t.bstart.p, (const int*)t.xoff.p, (const int*)t.xcol.p, xv,
+                        (const int*)t.border.p, (const int*)t.bstart.p, (const int*)t.brow.p,
+                        (const int*)t.xoff.p, (const int*)t.xcol.p, xv,
                         (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out, S,
                         d_flags.p, limit, trace, nrep);
     };
