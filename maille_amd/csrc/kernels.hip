@@ -41,17 +41,23 @@ __device__ inline void publish(T* p, T v) {
 // plain load may hit a stale line of this XCD's L2 that still holds the sentinel). The wait
 // is bounded: past `limit` ticks of the device's steady wall clock (or once another wave has
 // given up) it raises *abort and returns 0, so every wavefront reaches the end of the kernel.
+// The clock and the abort flag are read every 16th poll only: each is another round trip, and
+// polls are on the critical path of the dataflow kernels.
 template <typename T>
 __device__ inline T await_value(T* p, int* abort_flag, long long limit) {
   typedef typename Bits<T>::U U;
-  long long t0 = wall_clock64();
-  for (;;) {
+  long long t0 = -1;
+  for (int it = 0;; ++it) {
     U u = __hip_atomic_load(reinterpret_cast<U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (u != Bits<T>::S) return from_bits<T>(u);
-    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return T(0);
-    if (wall_clock64() - t0 > limit) {
-      __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return T(0);
+    if ((it & 15) == 0) {
+      if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return T(0);
+      const long long now = wall_clock64();
+      if (t0 < 0) t0 = now;
+      if (now - t0 > limit) {
+        __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return T(0);
+      }
     }
     __builtin_amdgcn_s_sleep(2);
   }
@@ -832,10 +838,12 @@ __global__ __launch_bounds__(256) void k_axpy_add(int n, double* __restrict__ x,
 // lowest unfinished block always has its inputs and a running wave, so the solve progresses.
 // Per block: the wave stages the block's entries and the external values already published
 // into LDS with coalesced loads; every lane consumes, in order, the external values that were
-// ready. Then the rows are resolved one after the other: lane k finishes its row (polling any
-// late external value, eight device-scope loads per round), publishes and broadcasts it, and
-// each lane whose next entry is that row consumes it at once. Every row is summed in the
-// reference's order, and no in-block step waits on global memory.
+// ready. Then the rows are resolved one after the other: lane k finishes its row, publishes and
+// broadcasts it, and each lane whose next entry is that row consumes it at once. Rows whose
+// external values are all in run back to back in a branch-free loop; a row still missing some
+// makes the wave poll (eight device-scope loads per pending lane per round), and a lane that
+// completes late catches up on the rows solved meanwhile from the lanes holding them. Every
+// row is summed in the reference's order, and no in-block step waits on global memory.
 template <typename T, bool SUBSEQ>
 __device__ inline T accumulate(T acc, T a, T v) {
 #pragma clang fp contract(off)
@@ -859,9 +867,10 @@ __global__ __launch_bounds__(64) void k_trisolve(
   typedef typename Bits<T>::U U;
   __shared__ T sxv[TRI_XCAP];
   __shared__ int sxc[TRI_XCAP];
-  __shared__ T sia[TRI_ICAP];
-  __shared__ unsigned char sik[TRI_ICAP];
-  __shared__ T own[64];
+  // one more internal entry than the layout can hold: position 64 (no row of a block), read
+  // ahead past the end of a lane's list
+  __shared__ T sia[TRI_ICAP + 1];
+  __shared__ unsigned char sik[TRI_ICAP + 1];
   const int lane = threadIdx.x;
   T* const out0 = out;
   const T* const rhs0 = rhs;
@@ -893,6 +902,10 @@ __global__ __launch_bounds__(64) void k_trisolve(
     for (int q = lane; q < ni; q += 64) {
       sik[q] = ik[ib + q];
       sia[q] = ival[ib + q];
+    }
+    if (lane == 0) {
+      sik[TRI_ICAP] = 64;
+      sia[TRI_ICAP] = T(0);
     }
     const bool active = lane < rows;
     const int l = s + lane;  // slot
@@ -926,93 +939,134 @@ __global__ __launch_bounds__(64) void k_trisolve(
       }
       go = go && p < x1;
     }
-    bool ext_done = p == x1;
     long long tp = 0;
     int late = 0;
     if (trace) {
       tp = wall_clock64();
-      late = __popcll(__ballot(active && !ext_done));
+      late = __popcll(__ballot(p < x1));
     }
-    int nk = ip < i1 ? (int)sik[ip] : 64;
-    T nv = ip < i1 ? sia[ip] : T(0);
-    long long t0 = -1;
+    // The lane's next internal entry (nk: the block position of the row it reads, nv: its
+    // coefficient) and the one after, read ahead so that consuming a row never waits on LDS.
+    // A lane whose external part is still pending holds nk = 64, which matches no row, until
+    // it catches up.
+    int nk = 64, nk2 = 64;
+    T nv = T(0), nv2 = T(0);
+    T yown = T(0);  // the lane's solved value, once its row is done
+    auto refill = [&]() {
+      nk = ip < i1 ? (int)sik[ip] : 64;
+      nv = ip < i1 ? sia[ip] : T(0);
+      nk2 = ip + 1 < i1 ? (int)sik[ip + 1] : 64;
+      nv2 = ip + 1 < i1 ? sia[ip + 1] : T(0);
+    };
+    if (p == x1) refill();
+    // Waits of the block are bounded from its start; the clock and the abort flag are read on
+    // the 1st, 17th, 33rd ... poll round of the block only (each is another round trip).
+    const long long t0 = wall_clock64();
+    int polls = 0;
     long long waited = 0;
     int rounds = 0;
-    for (int k = 0; k < rows; ++k) {
-      const long long tw = trace ? wall_clock64() : 0;
-      // Row k may not start before its external values are in: the whole wave polls, each
-      // lane with pending entries loading its next eight, so one round trip serves every row
-      // of the block. A lane whose external part completes catches up, in order, on the
-      // internal entries broadcast while it was waiting.
-      while (__builtin_amdgcn_readlane((int)(p < x1), k)) {
-        if (p < x1) {
-          int jj[8];
-          U vv[8];
-          T aa[8];
+    // lanes whose external part is still pending: none below the current row, whose
+    // predecessors in the block have all been solved
+    unsigned long long pend = __ballot(p < x1);
+    int k = 0;
+    while (k < rows) {
+      if ((pend >> k) & 1ull) {
+        // Row k waits for external values: the whole wave polls, each lane with pending
+        // entries loading its next eight, so one round trip serves every row of the block. A
+        // lane whose external part completes catches up, in order, on the internal entries of
+        // the rows solved while it was waiting.
+        const long long tw = trace ? wall_clock64() : 0;
+        do {
+          const unsigned long long before = pend;
+          if (p < x1) {
+            int jj[8];
+            U vv[8];
+            T aa[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) jj[u] = (p + u < x1) ? sxc[p + u] : -1;
+            for (int u = 0; u < 8; ++u) jj[u] = (p + u < x1) ? sxc[p + u] : -1;
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            vv[u] = jj[u] >= 0 ? __hip_atomic_load(reinterpret_cast<U*>(out + (REV ? n - 1 - jj[u] : jj[u])),
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                               : (U)0;
-            aa[u] = jj[u] >= 0 ? xval[xb + p + u] : T(0);
-          }
-          bool waiting = false;
+            for (int u = 0; u < 8; ++u) {
+              vv[u] = jj[u] >= 0 ? __hip_atomic_load(reinterpret_cast<U*>(out + (REV ? n - 1 - jj[u] : jj[u])),
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                 : (U)0;
+              aa[u] = jj[u] >= 0 ? xval[xb + p + u] : T(0);
+            }
+            bool waiting = false;
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            if (!waiting && jj[u] >= 0) {
-              if (vv[u] == Bits<T>::S) {
-                waiting = true;
-              } else {
-                acc = accumulate<T, SUBSEQ>(acc, aa[u], from_bits<T>(vv[u]));
-                ++p;
+            for (int u = 0; u < 8; ++u) {
+              if (!waiting && jj[u] >= 0) {
+                if (vv[u] == Bits<T>::S) {
+                  waiting = true;
+                } else {
+                  acc = accumulate<T, SUBSEQ>(acc, aa[u], from_bits<T>(vv[u]));
+                  ++p;
+                }
               }
             }
           }
-          if (!ext_done && p == x1) {
-            ext_done = true;
-            while (ip < i1 && (int)sik[ip] < k) {
-              acc = accumulate<T, SUBSEQ>(acc, sia[ip], own[sik[ip]]);
-              ++ip;
+          pend = __ballot(p < x1);
+          if (const unsigned long long caught = before & ~pend) {
+            // lanes whose external part just completed read the rows solved meanwhile from
+            // the lanes that solved them, in order
+            const bool mine = (caught >> lane) & 1ull;
+            for (;;) {
+              const bool need = mine && ip < i1 && (int)sik[ip] < k;
+              if (!__ballot(need)) break;
+              const T v = __shfl(yown, need ? (int)sik[ip] : lane);
+              if (need) {
+                acc = accumulate<T, SUBSEQ>(acc, sia[ip], v);
+                ++ip;
+              }
             }
-            nk = ip < i1 ? (int)sik[ip] : 64;
-            nv = ip < i1 ? sia[ip] : T(0);
+            if (mine) refill();
           }
-        }
-        if (__builtin_amdgcn_readlane((int)(p < x1), k)) {
-          const long long now = wall_clock64();
-          if (t0 < 0) t0 = now;
-          if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-              now - t0 > limit) {
-            // give up: the host reports ML_DEVICE_TIMEOUT; every lane stops waiting
-            __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (!ext_done) {
-              p = x1;
-              ext_done = true;
-              while (ip < i1 && (int)sik[ip] < k) ++ip;
-              nk = ip < i1 ? (int)sik[ip] : 64;
-              nv = ip < i1 ? sia[ip] : T(0);
+          if ((pend >> k) & 1ull) {
+            bool give_up = false;
+            if ((polls++ & 15) == 0)
+              give_up = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                        wall_clock64() - t0 > limit;
+            if (__builtin_amdgcn_readfirstlane((int)give_up)) {
+              // give up: the host reports ML_DEVICE_TIMEOUT; every lane stops waiting
+              __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              if (p < x1) {
+                p = x1;
+                while (ip < i1 && (int)sik[ip] < k) ++ip;
+                refill();
+              }
+              pend = 0;
+            } else {
+              __builtin_amdgcn_s_sleep(1);
             }
-          } else {
-            __builtin_amdgcn_s_sleep(1);
           }
+          if (trace) ++rounds;
+        } while ((pend >> k) & 1ull);
+        if (trace) waited += wall_clock64() - tw;
+      }
+      // rows k .. kend-1 have all their external values: solve them back to back. Loads
+      // the polls left in flight are drained first, so that the row loop only ever waits on
+      // its own read-ahead, issued a row earlier. The loop has no branch but the one around
+      // the division: every lane takes the same steps, selecting on whether it reads row k.
+      const int kend = pend ? min(rows, (int)__builtin_ctzll(pend)) : rows;
+      __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
+      for (; k < kend; ++k) {
+        T y = acc;
+        if (lane == k) {
+          y = SUBSEQ ? acc / dl : (rl - acc) / dl;
+          yown = y;
+          publish(out + pl, y);
         }
-        if (trace) ++rounds;
-      }
-      if (trace) waited += wall_clock64() - tw;
-      T y = T(0);
-      if (lane == k) {
-        y = SUBSEQ ? acc / dl : (rl - acc) / dl;
-        own[k] = y;
-        publish(out + pl, y);
-      }
-      y = lane_broadcast(y, k);
-      if (ext_done && lane > k && nk == k) {
-        acc = accumulate<T, SUBSEQ>(acc, nv, y);
-        ++ip;
-        nk = ip < i1 ? (int)sik[ip] : 64;
-        nv = ip < i1 ? sia[ip] : T(0);
+        y = lane_broadcast(y, k);
+        const bool hit = nk == k;
+        const T upd = accumulate<T, SUBSEQ>(acc, nv, y);
+        acc = hit ? upd : acc;
+        ip += hit ? 1 : 0;
+        nk = hit ? nk2 : nk;
+        nv = hit ? nv2 : nv;
+        // the entry after next (the same one again when the lane did not advance), or the
+        // terminal position 64 past the end of the list
+        const int q = ip + 1 < i1 ? ip + 1 : TRI_ICAP;
+        nk2 = sik[q];
+        nv2 = sia[q];
       }
     }
     if (trace && lane == 0 && rep == 0) {
