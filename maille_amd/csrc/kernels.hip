@@ -852,21 +852,26 @@ __device__ inline T accumulate(T acc, T a, T v) {
 
 constexpr int TRI_XCAP = 512;  // external entries a block may hold (host-enforced)
 constexpr int TRI_ICAP = 512;  // internal entries a block may hold (host-enforced)
+constexpr int TRI_UCAP = 256;  // distinct external columns a block may hold (host-enforced)
 
-template <typename T, bool REV, bool SUBSEQ>
+// TRACE: for diagnostics, each block records its timeline in trace[6 b .. 6 b + 5] and the
+// publish time of each slot's row in trace[6 nblocks + slot] (first right-hand side only).
+template <typename T, bool REV, bool SUBSEQ, bool TRACE = false>
 __global__ __launch_bounds__(64) void k_trisolve(
     int n, int nblocks, const int* __restrict__ border, const int* __restrict__ bstart,
-    const int* __restrict__ brow, const int* __restrict__ xoff, const int* __restrict__ xcol,
-    const T* __restrict__ xval,
-    const int* __restrict__ ioff, const unsigned char* __restrict__ ik,
+    const int* __restrict__ brow, const int* __restrict__ xoff, const int* __restrict__ uoff,
+    const int* __restrict__ ucol, const unsigned char* __restrict__ xu,
+    const T* __restrict__ xval, const int* __restrict__ ioff, const unsigned char* __restrict__ ik,
     const T* __restrict__ ival, const T* __restrict__ diag, const T* __restrict__ rhs, T* out,
     const PcgScalars* __restrict__ S, int* abort_flag, long long limit,
     long long* __restrict__ trace, int nrep) {
 #pragma clang fp contract(off)
   if (S && S->done) return;
   typedef typename Bits<T>::U U;
-  __shared__ T sxv[TRI_XCAP];
-  __shared__ int sxc[TRI_XCAP];
+  __shared__ T suv[TRI_UCAP];    // values of the distinct external columns (sentinel: not yet)
+  __shared__ int suc[TRI_UCAP];  // their physical indices
+  __shared__ T sxa[TRI_XCAP];    // external coefficients
+  __shared__ unsigned char sxu[TRI_XCAP];  // their columns, as indices into suv
   // one more internal entry than the layout can hold: position 64 (no row of a block), read
   // ahead past the end of a lane's list
   __shared__ T sia[TRI_ICAP + 1];
@@ -887,16 +892,23 @@ __global__ __launch_bounds__(64) void k_trisolve(
     const int rows = e - s;
     const int xb = xoff[s], nx = xoff[e] - xb;
     const int ib = ioff[s], ni = ioff[e] - ib;
-    long long tb = trace ? wall_clock64() : 0;
-    // stage the block: external columns and the values published so far, read at device
-    // scope (a plain load could hit a stale line of this XCD's L2 that still holds the
-    // sentinel); their coefficients stay in global memory and are read eight at a time
+    const int ub = uoff[b], nu = uoff[b + 1] - ub;
+    long long tb = TRACE ? wall_clock64() : 0;
+    // stage the block: its distinct external columns with the values published so far, read
+    // at device scope (a plain load could hit a stale line of this XCD's L2 that still holds
+    // the sentinel), the external entries (coefficient, column index) and the internal ones
+#pragma unroll 4
+    for (int q = lane; q < nu; q += 64) {
+      const int c = ucol[ub + q];
+      const int pc = REV ? n - 1 - c : c;
+      suc[q] = pc;
+      suv[q] = from_bits<T>(__hip_atomic_load(reinterpret_cast<U*>(out + pc), __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT));
+    }
 #pragma unroll 4
     for (int q = lane; q < nx; q += 64) {
-      const int c = xcol[xb + q];
-      sxc[q] = c;
-      sxv[q] = from_bits<T>(__hip_atomic_load(reinterpret_cast<U*>(out + (REV ? n - 1 - c : c)),
-                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      sxu[q] = xu[xb + q];
+      sxa[q] = xval[xb + q];
     }
 #pragma unroll 4
     for (int q = lane; q < ni; q += 64) {
@@ -923,25 +935,35 @@ __global__ __launch_bounds__(64) void k_trisolve(
       acc = SUBSEQ ? rl : T(0);
     }
     wave_sync();
-    // external values that were ready, in order
-    for (bool go = p < x1; go;) {
-      T aa[8];
+    // Each lane consumes its external entries, in order, as far as their values are in; the
+    // entries are looked up four at a time so that the LDS reads overlap.
+    auto advance = [&]() {
+      for (bool go = p < x1; go;) {
+        int ju[4];
+        T aa[4], vv[4];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) aa[u] = (p + u < x1) ? xval[xb + p + u] : T(0);
+        for (int u = 0; u < 4; ++u) ju[u] = p + u < x1 ? (int)sxu[p + u] : -1;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (go && p < x1 && !is_sentinel(sxv[p])) {
-          acc = accumulate<T, SUBSEQ>(acc, aa[u], sxv[p]);
-          ++p;
-        } else {
-          go = false;
+        for (int u = 0; u < 4; ++u) {
+          vv[u] = ju[u] >= 0 ? suv[ju[u]] : T(0);
+          aa[u] = ju[u] >= 0 ? sxa[p + u] : T(0);
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (go && ju[u] >= 0 && !is_sentinel(vv[u])) {
+            acc = accumulate<T, SUBSEQ>(acc, aa[u], vv[u]);
+            ++p;
+          } else {
+            go = false;
+          }
+        }
+        go = go && p < x1;
       }
-      go = go && p < x1;
-    }
+    };
+    advance();
     long long tp = 0;
     int late = 0;
-    if (trace) {
+    if (TRACE) {
       tp = wall_clock64();
       late = __popcll(__ballot(p < x1));
     }
@@ -975,35 +997,20 @@ __global__ __launch_bounds__(64) void k_trisolve(
         // entries loading its next eight, so one round trip serves every row of the block. A
         // lane whose external part completes catches up, in order, on the internal entries of
         // the rows solved while it was waiting.
-        const long long tw = trace ? wall_clock64() : 0;
+        const long long tw = TRACE ? wall_clock64() : 0;
         do {
           const unsigned long long before = pend;
-          if (p < x1) {
-            int jj[8];
-            U vv[8];
-            T aa[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) jj[u] = (p + u < x1) ? sxc[p + u] : -1;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              vv[u] = jj[u] >= 0 ? __hip_atomic_load(reinterpret_cast<U*>(out + (REV ? n - 1 - jj[u] : jj[u])),
-                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                 : (U)0;
-              aa[u] = jj[u] >= 0 ? xval[xb + p + u] : T(0);
-            }
-            bool waiting = false;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              if (!waiting && jj[u] >= 0) {
-                if (vv[u] == Bits<T>::S) {
-                  waiting = true;
-                } else {
-                  acc = accumulate<T, SUBSEQ>(acc, aa[u], from_bits<T>(vv[u]));
-                  ++p;
-                }
-              }
+          // one round trip: every distinct column of the block still missing is loaded again
+#pragma unroll 2
+          for (int q = lane; q < nu; q += 64) {
+            if (is_sentinel(suv[q])) {
+              const U u = __hip_atomic_load(reinterpret_cast<U*>(out + suc[q]), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+              if (u != Bits<T>::S) suv[q] = from_bits<T>(u);
             }
           }
+          wave_sync();
+          advance();
           pend = __ballot(p < x1);
           if (const unsigned long long caught = before & ~pend) {
             // lanes whose external part just completed read the rows solved meanwhile from
@@ -1038,9 +1045,9 @@ __global__ __launch_bounds__(64) void k_trisolve(
               __builtin_amdgcn_s_sleep(1);
             }
           }
-          if (trace) ++rounds;
+          if (TRACE) ++rounds;
         } while ((pend >> k) & 1ull);
-        if (trace) waited += wall_clock64() - tw;
+        if (TRACE) waited += wall_clock64() - tw;
       }
       // rows k .. kend-1 have all their external values: solve them back to back. Loads
       // the polls left in flight are drained first, so that the row loop only ever waits on
@@ -1054,6 +1061,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
           y = SUBSEQ ? acc / dl : (rl - acc) / dl;
           yown = y;
           publish(out + pl, y);
+          if (TRACE && rep == 0) trace[6 * (long long)nblocks + l] = wall_clock64();
         }
         y = lane_broadcast(y, k);
         const bool hit = nk == k;
@@ -1069,7 +1077,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
         nv2 = sia[q];
       }
     }
-    if (trace && lane == 0 && rep == 0) {
+    if (TRACE && lane == 0 && rep == 0) {
       trace[6 * (long long)b + 0] = tb;
       trace[6 * (long long)b + 1] = tp;
       trace[6 * (long long)b + 2] = wall_clock64();

@@ -283,17 +283,24 @@ i64 dag_depth(const LowerPattern& lp) {
 }
 
 TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const int32_t* src,
-                     const int32_t* dsrc, int max_rows, int xcap, int icap, std::string* err) {
+                     const int32_t* dsrc, int max_rows, const TriCaps& caps, std::string* err) {
   TriLayout t;
+  const int xcap = caps.xcap, icap = caps.icap, ucap = caps.ucap;
+  if (ucap < 1 || ucap > 256) {
+    *err = "distinct-column capacity outside 1..256";
+    return t;
+  }
   // chains: rows in increasing order; a row joins the chain of its critical predecessor when
   // that chain ends in it, has room (rows, external and internal entries), every other
   // dependency lies in an older chain, and the row's entries inside the chain are the last of
   // its columns (so a row reads as external entries, then internal ones, in its own order)
   std::vector<int32_t> chain_of(n, -1), clen, clast, cext, cint;
+  std::vector<std::vector<int32_t>> cdist;  // per chain: its distinct outside columns, ascending
+  std::vector<int32_t> merged;
   std::vector<double> finish(n, 0.0);
   for (int64_t i = 0; i < n; ++i) {
     const int64_t len = off[i + 1] - off[i];
-    if (len > xcap || len > icap) {
+    if (len > xcap || len > icap || len > ucap) {
       *err = "matrix row longer than the triangular-solve block capacity";
       return t;
     }
@@ -318,7 +325,15 @@ TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const in
             if (in > 0) suffix = false;
           }
         }
-        if (suffix && cext[cb] + ext <= xcap && cint[cb] + in <= icap) ch = cb;
+        if (suffix && cext[cb] + ext <= xcap && cint[cb] + in <= icap) {
+          // a row's columns are distinct, so its new outside columns are those not listed yet
+          const std::vector<int32_t>& dl = cdist[cb];
+          int64_t fresh = 0;
+          for (int64_t p = off[i]; p < off[i + 1]; ++p)
+            if (chain_of[col[p]] != cb && !std::binary_search(dl.begin(), dl.end(), col[p]))
+              ++fresh;
+          if ((int64_t)dl.size() + fresh <= ucap) ch = cb;
+        }
       }
     }
     if (ch < 0) {
@@ -327,16 +342,26 @@ TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const in
       clast.push_back(-1);
       cext.push_back(0);
       cint.push_back(0);
+      cdist.emplace_back();
     }
     double f = clast[ch] >= 0 ? finish[clast[ch]] + CHAIN_ROW_US : CHAIN_ROW_US;
+    merged.clear();
     for (int64_t p = off[i]; p < off[i + 1]; ++p) {
       const int32_t j = col[p];
       if (chain_of[j] != ch) {
         f = std::max(f, finish[j] + CHAIN_HOP_US + CHAIN_ROW_US);
         cext[ch] += 1;
+        merged.push_back(j);  // ascending: the row's columns are
       } else {
         cint[ch] += 1;
       }
+    }
+    if (!merged.empty()) {
+      std::vector<int32_t>& dl = cdist[ch];
+      const size_t old_size = dl.size();
+      dl.insert(dl.end(), merged.begin(), merged.end());
+      std::inplace_merge(dl.begin(), dl.begin() + old_size, dl.end());
+      dl.erase(std::unique(dl.begin(), dl.end()), dl.end());
     }
     finish[i] = f;
     chain_of[i] = ch;
@@ -372,9 +397,30 @@ TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const in
     t.ioff[q + 1] = (int32_t)t.ik.size();
     t.dsrc[q] = dsrc[i];
   }
+  cdist.clear();
+  cdist.shrink_to_fit();
+  // each block's distinct outside columns in order of first use, and every external entry's
+  // index in that list
+  t.uoff.assign(nb + 1, 0);
+  t.xu.resize(t.xcol.size());
+  std::vector<int32_t> uidx(n, -1);
+  for (int64_t b = 0; b < nb; ++b) {
+    const int32_t ub = (int32_t)t.ucol.size();
+    for (int32_t x = t.xoff[t.bstart[b]]; x < t.xoff[t.bstart[b + 1]]; ++x) {
+      const int32_t c = t.xcol[x];
+      if (uidx[c] < 0) {
+        uidx[c] = (int32_t)t.ucol.size() - ub;
+        t.ucol.push_back(c);
+      }
+      t.xu[x] = (uint8_t)uidx[c];
+    }
+    for (int32_t u = ub; u < (int32_t)t.ucol.size(); ++u) uidx[t.ucol[u]] = -1;
+    t.uoff[b + 1] = (int32_t)t.ucol.size();
+  }
   for (int64_t b = 0; b < nb; ++b) {
     t.max_block_ext = std::max(t.max_block_ext, t.xoff[t.bstart[b + 1]] - t.xoff[t.bstart[b]]);
     t.max_block_int = std::max(t.max_block_int, t.ioff[t.bstart[b + 1]] - t.ioff[t.bstart[b]]);
+    t.max_block_cols = std::max(t.max_block_cols, t.uoff[b + 1] - t.uoff[b]);
   }
   // Processing order: blocks by their level in the block dependency graph (a block depends
   // on the blocks holding its external columns), ties in creation order. It is a topological
@@ -393,7 +439,7 @@ TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const in
   return t;
 }
 
-std::string device_layouts(const LowerPattern& lp, int xcap, int icap, DeviceLayouts* out) {
+std::string device_layouts(const LowerPattern& lp, const TriCaps& caps, DeviceLayouts* out) {
   DeviceLayouts& d = *out;
   const i64 n = lp.n();
   d.n = n;
@@ -442,16 +488,16 @@ std::string device_layouts(const LowerPattern& lp, int xcap, int icap, DeviceLay
     bdsrc[l] = (int32_t)(lp.offsets[j + 1] - 1);
   }
   std::string err;
-  d.fw = tri_layout(n, foff.data(), fcol.data(), fsrc.data(), fdsrc.data(), 64, xcap, icap, &err);
+  d.fw = tri_layout(n, foff.data(), fcol.data(), fsrc.data(), fdsrc.data(), 64, caps, &err);
   if (!err.empty()) return err;
-  d.bw = tri_layout(n, boff.data(), bcol.data(), bsrc.data(), bdsrc.data(), 64, xcap, icap, &err);
+  d.bw = tri_layout(n, boff.data(), bcol.data(), bsrc.data(), bdsrc.data(), 64, caps, &err);
   return err;
 }
 
 namespace {
 
 // One triangular-solve layout against the strictly-lower logical rows it was built from.
-std::string check_tri(const char* name, const TriLayout& t, i64 n, i64 nnz, int xcap, int icap,
+std::string check_tri(const char* name, const TriLayout& t, i64 n, i64 nnz, const TriCaps& caps,
                       const std::vector<int32_t>& rows_off, const std::vector<int32_t>& rows_col,
                       const std::vector<int32_t>& rows_src) {
   const std::string pre = std::string(name) + " layout: ";
@@ -465,6 +511,10 @@ std::string check_tri(const char* name, const TriLayout& t, i64 n, i64 nnz, int 
       t.xcol.size() != t.xsrc.size() || (size_t)t.ioff[n] != t.ik.size() ||
       t.ik.size() != t.isrc.size())
     return pre + "entry arrays do not match their offsets";
+  if ((i64)t.uoff.size() != nb + 1 || t.uoff[0] != 0 || (size_t)t.uoff[nb] != t.ucol.size() ||
+      t.xu.size() != t.xcol.size())
+    return pre + "distinct-column arrays do not match their offsets";
+  if (caps.ucap > 256) return pre + "distinct-column capacity over 256";
   std::vector<int32_t> block_of(n, -1), pos(nb, -1);
   for (i64 b = 0; b < nb; ++b) {
     const i64 rows = (i64)t.bstart[b + 1] - t.bstart[b];
@@ -482,10 +532,19 @@ std::string check_tri(const char* name, const TriLayout& t, i64 n, i64 nnz, int 
     if (b < 0 || b >= nb || pos[b] >= 0) return pre + "processing order is not a permutation";
     pos[b] = (int32_t)k;
   }
+  std::vector<char> listed(n, 0);
   for (i64 b = 0; b < nb; ++b) {
     const i64 s = t.bstart[b], e = t.bstart[b + 1];
-    if (t.xoff[e] - t.xoff[s] > xcap) return pre + "block over the external capacity";
-    if (t.ioff[e] - t.ioff[s] > icap) return pre + "block over the internal capacity";
+    if (t.xoff[e] - t.xoff[s] > caps.xcap) return pre + "block over the external capacity";
+    if (t.ioff[e] - t.ioff[s] > caps.icap) return pre + "block over the internal capacity";
+    const i64 ub = t.uoff[b], nu = (i64)t.uoff[b + 1] - ub;
+    if (nu < 0 || nu > caps.ucap) return pre + "block over the distinct-column capacity";
+    for (i64 u = ub; u < ub + nu; ++u) {
+      const i64 c = t.ucol[u];
+      if (c < 0 || c >= n || listed[c]) return pre + "distinct columns out of range or repeated";
+      listed[c] = 1;
+    }
+    for (i64 u = ub; u < ub + nu; ++u) listed[t.ucol[u]] = 0;
     for (i64 q = s; q < e; ++q) {
       const i64 i = t.brow[q], k = q - s;
       if (t.xoff[q + 1] < t.xoff[q] || t.ioff[q + 1] < t.ioff[q])
@@ -500,6 +559,8 @@ std::string check_tri(const char* name, const TriLayout& t, i64 n, i64 nnz, int 
         if (c < 0 || c >= n) return pre + "external column out of range";
         if (block_of[c] == b) return pre + "external entry inside its own block";
         if (c != rows_col[p] || t.xsrc[x] != rows_src[p]) return pre + "external entry out of order";
+        if (t.xu[x] >= nu || t.ucol[ub + t.xu[x]] != c)
+          return pre + "external entry names the wrong distinct column";
         if (t.xsrc[x] < 0 || t.xsrc[x] >= nnz) return pre + "external source out of range";
         if (pos[block_of[c]] >= pos[b]) return pre + "processing order is not topological";
       }
@@ -517,7 +578,7 @@ std::string check_tri(const char* name, const TriLayout& t, i64 n, i64 nnz, int 
 
 }  // namespace
 
-std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, int xcap, int icap) {
+std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, const TriCaps& caps) {
   const i64 n = lp.n(), nnz = lp.nnz();
   if (d.n != n || d.nnz != nnz || (i64)d.off32.size() != n + 1 || (i64)d.col32.size() != nnz)
     return "CSR copy of the wrong size";
@@ -546,7 +607,7 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, int xc
       fcol[o] = lp.cols[p];
       fsrc[o] = (int32_t)p;
     }
-  std::string err = check_tri("forward", d.fw, n, nnz, xcap, icap, foff, fcol, fsrc);
+  std::string err = check_tri("forward", d.fw, n, nnz, caps, foff, fcol, fsrc);
   if (!err.empty()) return err;
   for (i64 q = 0; q < n; ++q)
     if (d.fw.dsrc[q] != lp.offsets[d.fw.brow[q] + 1] - 1) return "forward layout: wrong diagonal";
@@ -566,7 +627,7 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, int xc
         bsrc[q] = (int32_t)p;
       }
   }
-  err = check_tri("backward", d.bw, n, nnz, xcap, icap, boff, bcol, bsrc);
+  err = check_tri("backward", d.bw, n, nnz, caps, boff, bcol, bsrc);
   if (!err.empty()) return err;
   for (i64 q = 0; q < n; ++q)
     if (d.bw.dsrc[q] != lp.offsets[n - d.bw.brow[q]] - 1) return "backward layout: wrong diagonal";

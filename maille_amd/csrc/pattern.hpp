@@ -73,17 +73,28 @@ i64 dag_depth(const LowerPattern& lp);
 // in increasing order; a row joins the chain of its critical predecessor (the dependency
 // expected to finish last, by a model of CHAIN_HOP_US per cross-chain dependency and
 // CHAIN_ROW_US per row) when that chain ends in it, has room (max_rows rows, xcap entries from
-// outside the chain, icap from inside), every other dependency of the row lies in an older
-// chain (creation order is then topological) and the row's dependencies inside the chain are its
-// last columns. Otherwise the row starts a new chain.
+// outside the chain, icap from inside, ucap distinct outside columns), every other dependency
+// of the row lies in an older chain (creation order is then topological) and the row's
+// dependencies inside the chain are its last columns. Otherwise the row starts a new chain.
 //
 // A block's rows occupy consecutive "slots" (brow: the logical row of a slot, ascending within
 // the block). Each slot's entries are split into the external part (columns in other blocks)
 // and the internal part (the position, 0..63, of a row of the same block); since the internal
 // columns are the row's last, externals then internals is the reference's order. Both parts
 // are stored slot after slot, so a wave loads a whole block with coalesced reads.
+//
+// The rows of a chain are mesh neighbours and share most of their external columns: each block
+// also lists its distinct external columns (ucol, in order of first use) and each external
+// entry names its column by its index in that list (xu), so that a wave waits on, and loads,
+// every external value once.
 constexpr double CHAIN_HOP_US = 3.0;
 constexpr double CHAIN_ROW_US = 0.25;
+
+// Per-block capacities of a triangular-solve layout (the kernels stage each part in LDS):
+// external entries, internal entries and distinct external columns (at most 256: xu is a byte).
+struct TriCaps {
+  int xcap, icap, ucap;
+};
 
 struct TriLayout {
   std::vector<int32_t> bstart;           // blocks + 1: slot offsets
@@ -92,10 +103,12 @@ struct TriLayout {
   std::vector<int32_t> xoff, xcol, xsrc;  // external entries: slots + 1, columns, sources
   std::vector<int32_t> ioff, ik, isrc;    // internal entries: slots + 1, positions, sources
   std::vector<int32_t> dsrc;              // diagonal source per slot
-  int max_block_ext = 0, max_block_int = 0;
+  std::vector<int32_t> uoff, ucol;        // distinct external columns: blocks + 1, columns
+  std::vector<uint8_t> xu;                // per external entry: its column's index in ucol
+  int max_block_ext = 0, max_block_int = 0, max_block_cols = 0;
 };
 TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const int32_t* src,
-                     const int32_t* dsrc, int max_rows, int xcap, int icap, std::string* err);
+                     const int32_t* dsrc, int max_rows, const TriCaps& caps, std::string* err);
 
 // Every index array the device kernels of one operator read, built on the host from the lower
 // pattern: the 32-bit CSR (IC(0)), the sliced-ELL symmetric layout (SpMV) and the two
@@ -109,11 +122,11 @@ struct DeviceLayouts {
   TriLayout fw, bw;
 };
 // Returns an empty string or the reason the pattern cannot be laid out (row too long).
-std::string device_layouts(const LowerPattern& lp, int xcap, int icap, DeviceLayouts* out);
+std::string device_layouts(const LowerPattern& lp, const TriCaps& caps, DeviceLayouts* out);
 
 // Verifies every bound the kernels rely on (index ranges, block capacities, coverage and order
 // of each row's entries, topological block order); empty string when all hold.
-std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, int xcap, int icap);
+std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, const TriCaps& caps);
 
 // The preconditioner z = (L L^T)^-1 r computed on the host through the triangular-solve
 // layouts, block by block in processing order, each row summed in the order the device sums

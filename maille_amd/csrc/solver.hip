@@ -79,6 +79,9 @@ struct DBuf {
 
 inline unsigned grid_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
+// per-block capacities of the triangular-solve layouts, as the kernel's LDS holds them
+constexpr TriCaps kTriCaps = {mlk::TRI_XCAP, mlk::TRI_ICAP, mlk::TRI_UCAP};
+
 // Kernel-duration accounting with HIP events on the handle's stream (enabled on demand).
 enum ProfKind { P_SPMV64, P_SPMV32, P_TRI64, P_TRI32, P_IC0, P_STIFF, P_ASM, P_VEC, P_COUNT };
 
@@ -204,8 +207,8 @@ struct Operator {
   DBuf<float> d_l32;
   struct TriDev {
     int nblk = 0;
-    DBuf<int> border, bstart, brow, xoff, xcol, xsrc, ioff, isrc, dsrc;
-    DBuf<unsigned char> ik;
+    DBuf<int> border, bstart, brow, xoff, xsrc, ioff, isrc, dsrc, uoff, ucol;
+    DBuf<unsigned char> ik, xu;
     DBuf<double> xval, ival, diag;
     DBuf<float> xval32, ival32, diag32;
     void set(const TriLayout& t, hipStream_t st) {
@@ -214,8 +217,10 @@ struct Operator {
       bstart.upload_vec(t.bstart, st);
       brow.upload_vec(t.brow, st);
       xoff.upload_vec(t.xoff, st);
-      xcol.upload_vec(t.xcol, st);
       xsrc.upload_vec(t.xsrc, st);
+      uoff.upload_vec(t.uoff, st);
+      ucol.upload_vec(t.ucol, st);
+      xu.upload_vec(t.xu, st);
       ioff.upload_vec(t.ioff, st);
       isrc.upload_vec(t.isrc, st);
       dsrc.upload_vec(t.dsrc, st);
@@ -261,8 +266,8 @@ struct Operator {
   void set_pattern(const LowerPattern& lp) {
     init_stream();
     DeviceLayouts d;
-    std::string err = device_layouts(lp, mlk::TRI_XCAP, mlk::TRI_ICAP, &d);
-    if (err.empty()) err = check_layouts(lp, d, mlk::TRI_XCAP, mlk::TRI_ICAP);
+    std::string err = device_layouts(lp, kTriCaps, &d);
+    if (err.empty()) err = check_layouts(lp, d, kTriCaps);
     if (!err.empty()) throw ArgError(err);
     n = (int)lp.n();
     nnz = lp.nnz();
@@ -459,18 +464,17 @@ struct Operator {
       const T* iv = f32 ? (const T*)t.ival32.p : (const T*)t.ival.p;
       const T* dg = f32 ? (const T*)t.diag32.p : (const T*)t.diag.p;
       int nb = t.nblk;
+      auto go = [&](auto kernel) {
+        launch_dataflow(kernel, grid, stream, nn, nb, (const int*)t.border.p,
+                        (const int*)t.bstart.p, (const int*)t.brow.p, (const int*)t.xoff.p,
+                        (const int*)t.uoff.p, (const int*)t.ucol.p, (const unsigned char*)t.xu.p,
+                        xv, (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out,
+                        S, d_flags.p, limit, trace, nrep);
+      };
       if (fwd)
-        launch_dataflow(mlk::k_trisolve<T, false, false>, grid, stream, nn, nb,
-                        (const int*)t.border.p, (const int*)t.bstart.p, (const int*)t.brow.p,
-                        (const int*)t.xoff.p, (const int*)t.xcol.p, xv,
-                        (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out, S,
-                        d_flags.p, limit, trace, nrep);
+        trace ? go(mlk::k_trisolve<T, false, false, true>) : go(mlk::k_trisolve<T, false, false>);
       else
-        launch_dataflow(mlk::k_trisolve<T, true, true>, grid, stream, nn, nb,
-                        (const int*)t.border.p, (const int*)t.bstart.p, (const int*)t.brow.p,
-                        (const int*)t.xoff.p, (const int*)t.xcol.p, xv,
-                        (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out, S,
-                        d_flags.p, limit, trace, nrep);
+        trace ? go(mlk::k_trisolve<T, true, true, true>) : go(mlk::k_trisolve<T, true, true>);
     };
     run(fw, r, y, true, trace_fw);
     run(bw, (const T*)y, z, false, trace_bw);
@@ -1355,8 +1359,8 @@ ml_status ml_layout_verify(int64_t n, const int64_t* offsets, const int64_t* col
   return guarded([&]() -> int {
     LowerPattern lp = pattern_from_csr(n, offsets, cols);
     DeviceLayouts d;
-    std::string err = device_layouts(lp, mlk::TRI_XCAP, mlk::TRI_ICAP, &d);
-    if (err.empty()) err = check_layouts(lp, d, mlk::TRI_XCAP, mlk::TRI_ICAP);
+    std::string err = device_layouts(lp, kTriCaps, &d);
+    if (err.empty()) err = check_layouts(lp, d, kTriCaps);
     if (!err.empty()) throw ArgError(err);
     if (stats) {
       stats[0] = (int64_t)d.fw.bstart.size() - 1;
@@ -1548,21 +1552,24 @@ ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* tr
     nblocks[0] = op.fw.nblk;
     nblocks[1] = op.bw.nblk;
     if (!trace_fw || !trace_bw) return ML_OK;
-    if (capacity < 6 * (int64_t)std::max(op.fw.nblk, op.bw.nblk)) throw ArgError("trace too small");
+    if (capacity < 6 * (int64_t)std::max(op.fw.nblk, op.bw.nblk) + op.n)
+      throw ArgError("trace too small");
     if (!op.factored) {
       int st = op.factor();
       if (st != ML_OK) return st;
     }
     DBuf<long long> tf, tb;
-    tf.alloc(6 * op.fw.nblk);
-    tb.alloc(6 * op.bw.nblk);
+    tf.alloc(6 * op.fw.nblk + op.n);
+    tb.alloc(6 * op.bw.nblk + op.n);
     HIP_CHECK(hipMemcpyAsync(op.w_r.p, r, sizeof(double) * op.n, hipMemcpyHostToDevice, op.stream));
     op.fill_sentinel<double>(op.w_y.p);
     op.fill_sentinel<double>(op.w_z.p);
     mlk::k_fill_i32<<<1, 64, 0, op.stream>>>(op.d_flags.p, 2, 0);
     op.precondition<double>(op.w_r.p, op.w_y.p, op.w_z.p, nullptr, tf.p, tb.p);
-    HIP_CHECK(hipMemcpyAsync(trace_fw, tf.p, sizeof(long long) * 6 * op.fw.nblk, hipMemcpyDeviceToHost, op.stream));
-    HIP_CHECK(hipMemcpyAsync(trace_bw, tb.p, sizeof(long long) * 6 * op.bw.nblk, hipMemcpyDeviceToHost, op.stream));
+    HIP_CHECK(hipMemcpyAsync(trace_fw, tf.p, sizeof(long long) * (6 * op.fw.nblk + op.n),
+                             hipMemcpyDeviceToHost, op.stream));
+    HIP_CHECK(hipMemcpyAsync(trace_bw, tb.p, sizeof(long long) * (6 * op.bw.nblk + op.n),
+                             hipMemcpyDeviceToHost, op.stream));
     HIP_CHECK(hipStreamSynchronize(op.stream));
     return ML_OK;
   });
