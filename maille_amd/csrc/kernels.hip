@@ -1000,7 +1000,9 @@ __global__ __launch_bounds__(64) void k_trisolve(
         const long long tw = TRACE ? wall_clock64() : 0;
         do {
           const unsigned long long before = pend;
-          // one round trip: every distinct column of the block still missing is loaded again
+          // One round trip: every distinct column of the block still missing is loaded again
+          // (rows after k included: their values then wait in LDS, which beats a round trip
+          // of their own later on).
 #pragma unroll 2
           for (int q = lane; q < nu; q += 64) {
             if (is_sentinel(suv[q])) {

@@ -227,14 +227,21 @@ struct Operator {
       std::vector<unsigned char> k8(t.ik.begin(), t.ik.end());
       ik.upload_vec(k8, st);
       HIP_CHECK(hipStreamSynchronize(st));
-      xval.alloc(t.xcol.size());
-      ival.alloc(t.ik.size());
-      diag.alloc(t.dsrc.size());
+      n_ext = t.xcol.size();
+      n_int = t.ik.size();
+      n_slots = t.dsrc.size();
+    }
+    // the factor's values in the layout, in the precision the handle applies it
+    size_t n_ext = 0, n_int = 0, n_slots = 0;
+    void alloc64() {
+      xval.alloc(n_ext);
+      ival.alloc(n_int);
+      diag.alloc(n_slots);
     }
     void alloc32() {
-      xval32.alloc(xval.n);
-      ival32.alloc(ival.n);
-      diag32.alloc(diag.n);
+      xval32.alloc(n_ext);
+      ival32.alloc(n_int);
+      diag32.alloc(n_slots);
     }
   } fw, bw;
   int64_t nf = 0;  // strictly lower entries
@@ -285,6 +292,10 @@ struct Operator {
     nf = nnz - n;
     fw.set(d.fw, stream);
     bw.set(d.bw, stream);
+    if (precision == ML_FP64) {
+      fw.alloc64();
+      bw.alloc64();
+    }
     // values and workspace
     d_a.alloc(nnz);
     d_l.alloc(nnz);
@@ -436,10 +447,10 @@ struct Operator {
       T* iv = f32 ? (T*)t->ival32.p : (T*)t->ival.p;
       T* dv = f32 ? (T*)t->diag32.p : (T*)t->diag.p;
       // a layout may have no external or no internal entries at all (a single block)
-      if (t->xval.n)
-        mlk::k_gather<T, T><<<grid_for(t->xval.n), 256, 0, stream>>>((int64_t)t->xval.n, t->xsrc.p, l, xv);
-      if (t->ival.n)
-        mlk::k_gather<T, T><<<grid_for(t->ival.n), 256, 0, stream>>>((int64_t)t->ival.n, t->isrc.p, l, iv);
+      if (t->n_ext)
+        mlk::k_gather<T, T><<<grid_for(t->n_ext), 256, 0, stream>>>((int64_t)t->n_ext, t->xsrc.p, l, xv);
+      if (t->n_int)
+        mlk::k_gather<T, T><<<grid_for(t->n_int), 256, 0, stream>>>((int64_t)t->n_int, t->isrc.p, l, iv);
       mlk::k_gather<T, T><<<grid_for(n), 256, 0, stream>>>((int64_t)n, t->dsrc.p, l, dv);
     }
     HIP_CHECK(hipGetLastError());
@@ -484,6 +495,24 @@ struct Operator {
   template <typename T>
   void fill_sentinel(T* p) {
     mlk::k_fill_bits<T><<<grid_for(n), 256, 0, stream>>>(p, n, mlk::Bits<T>::S);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // The handle's preconditioner, w_z = (L L^T)^-1 w_r: the FP64 factor, or for the mixed
+  // precisions the FP32 one that their iterations apply (w_r narrowed, the result widened).
+  void apply_precondition(long long* trace_fw = nullptr, long long* trace_bw = nullptr) {
+    mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 2, 0);
+    if (precision == ML_FP64) {
+      fill_sentinel<double>(w_y.p);
+      fill_sentinel<double>(w_z.p);
+      precondition<double>(w_r.p, w_y.p, w_z.p, nullptr, trace_fw, trace_bw);
+    } else {
+      mlk::k_convert<double, float><<<grid_for(n), 256, 0, stream>>>(n, w_r.p, w_r32.p);
+      fill_sentinel<float>(w_y32.p);
+      fill_sentinel<float>(w_z32.p);
+      precondition<float>(w_r32.p, w_y32.p, w_z32.p, nullptr, trace_fw, trace_bw);
+      mlk::k_convert<float, double><<<grid_for(n), 256, 0, stream>>>(n, w_z32.p, w_z.p);
+    }
     HIP_CHECK(hipGetLastError());
   }
 
@@ -1504,10 +1533,7 @@ ml_status ml_pcg_precondition(ml_pcg* h, const double* r, double* z) {
       if (st != ML_OK) return st;
     }
     HIP_CHECK(hipMemcpyAsync(op.w_r.p, r, sizeof(double) * op.n, hipMemcpyHostToDevice, op.stream));
-    op.fill_sentinel<double>(op.w_y.p);
-    op.fill_sentinel<double>(op.w_z.p);
-    mlk::k_fill_i32<<<1, 64, 0, op.stream>>>(op.d_flags.p, 2, 0);
-    op.precondition<double>(op.w_r.p, op.w_y.p, op.w_z.p, nullptr);
+    op.apply_precondition();
     HIP_CHECK(hipMemcpyAsync(z, op.w_z.p, sizeof(double) * op.n, hipMemcpyDeviceToHost, op.stream));
     HIP_CHECK(hipStreamSynchronize(op.stream));
     if (op.read_flag(0)) return ML_DEVICE_TIMEOUT;
@@ -1562,10 +1588,7 @@ ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* tr
     tf.alloc(6 * op.fw.nblk + op.n);
     tb.alloc(6 * op.bw.nblk + op.n);
     HIP_CHECK(hipMemcpyAsync(op.w_r.p, r, sizeof(double) * op.n, hipMemcpyHostToDevice, op.stream));
-    op.fill_sentinel<double>(op.w_y.p);
-    op.fill_sentinel<double>(op.w_z.p);
-    mlk::k_fill_i32<<<1, 64, 0, op.stream>>>(op.d_flags.p, 2, 0);
-    op.precondition<double>(op.w_r.p, op.w_y.p, op.w_z.p, nullptr, tf.p, tb.p);
+    op.apply_precondition(tf.p, tb.p);
     HIP_CHECK(hipMemcpyAsync(trace_fw, tf.p, sizeof(long long) * (6 * op.fw.nblk + op.n),
                              hipMemcpyDeviceToHost, op.stream));
     HIP_CHECK(hipMemcpyAsync(trace_bw, tb.p, sizeof(long long) * (6 * op.bw.nblk + op.n),

@@ -77,6 +77,47 @@ def test_spmv_factor_and_preconditioner(gpu, name):
     assert np.array_equal(pcg.precondition(r), csr.precondition(s.l_values(), r))
 
 
+def _precondition_f32(off, cols, l, r):
+    """iterative.rs:173-188 in single precision: every product, sum and quotient rounded to
+    float32, forward sums in ascending column order, backward subtractions in decreasing row
+    order."""
+    n = len(off) - 1
+    l = l.astype(np.float32)
+    y = np.zeros(n, dtype=np.float32)
+    for i in range(n):
+        acc = np.float32(0.0)
+        for p in range(off[i], off[i + 1] - 1):
+            acc = np.float32(acc + np.float32(l[p] * y[cols[p]]))
+        y[i] = np.float32(np.float32(np.float32(r[i]) - acc) / l[off[i + 1] - 1])
+    below = [[] for _ in range(n)]  # column j: (i, l_ij), i > j, ascending i
+    for i in range(n):
+        for p in range(off[i], off[i + 1] - 1):
+            below[cols[p]].append((i, l[p]))
+    z = np.zeros(n, dtype=np.float32)
+    for j in range(n - 1, -1, -1):
+        acc = y[j]
+        for i, v in reversed(below[j]):
+            acc = np.float32(acc - np.float32(v * z[i]))
+        z[j] = np.float32(acc / l[off[j + 1] - 1])
+    return z
+
+
+@pytest.mark.parametrize("precision", ["MIXED", "MIXED_IC0"])
+def test_mixed_preconditioner_applies_the_fp32_factor(gpu, precision):
+    """A mixed handle's preconditioner is its FP32 factor applied in FP32 arithmetic: bit for
+    bit the float32 restatement of the reference's two triangular loops, on every call."""
+    p = PROBLEMS["jittered_cube_8"]
+    s = _oracle_system(p)
+    off, cols, vals = s.csr()
+    print(f"{precision} preconditioner n={s.n}", flush=True)
+    pcg = gpu.Pcg(gpu.Csr(off, cols, vals), precision=getattr(gpu, precision))
+    r = np.cos(0.11 * np.arange(s.n))
+    z = pcg.precondition(r)
+    l, _ = pcg.factor()  # the FP32 factor, widened
+    assert np.array_equal(z, _precondition_f32(off, cols, l, r).astype(np.float64))
+    assert np.array_equal(pcg.precondition(r), z)
+
+
 @pytest.mark.parametrize("name", list(PROBLEMS))
 def test_fp64_solve_matches_oracle_pcg(gpu, name):
     p = PROBLEMS[name]
