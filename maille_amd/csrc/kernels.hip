@@ -854,8 +854,8 @@ constexpr int TRI_XCAP = 512;  // external entries a block may hold (host-enforc
 constexpr int TRI_ICAP = 512;  // internal entries a block may hold (host-enforced)
 constexpr int TRI_UCAP = 256;  // distinct external columns a block may hold (host-enforced)
 
-// TRACE: for diagnostics, each block records its timeline in trace[6 b .. 6 b + 5] and the
-// publish time of each slot's row in trace[6 nblocks + slot] (first right-hand side only).
+// TRACE: for diagnostics, each block records its timeline in trace[8 b .. 8 b + 7] and the
+// publish time of each slot's row in trace[8 nblocks + slot] (first right-hand side only).
 template <typename T, bool REV, bool SUBSEQ, bool TRACE = false>
 __global__ __launch_bounds__(64) void k_trisolve(
     int n, int nblocks, const int* __restrict__ border, const int* __restrict__ bstart,
@@ -876,6 +876,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
   // ahead past the end of a lane's list
   __shared__ T sia[TRI_ICAP + 1];
   __shared__ unsigned char sik[TRI_ICAP + 1];
+  __shared__ T own[64];  // the rows of the block solved so far, copied when polling starts
   const int lane = threadIdx.x;
   T* const out0 = out;
   const T* const rhs0 = rhs;
@@ -936,20 +937,22 @@ __global__ __launch_bounds__(64) void k_trisolve(
     }
     wave_sync();
     // Each lane consumes its external entries, in order, as far as their values are in; the
-    // entries are looked up four at a time so that the LDS reads overlap.
+    // entries are looked up eight at a time so that the LDS reads overlap. The index of the
+    // column a lane waits on is kept, so that a lane still waiting costs one LDS read.
+    int wu = p < x1 ? (int)sxu[p] : 0;
     auto advance = [&]() {
-      for (bool go = p < x1; go;) {
-        int ju[4];
-        T aa[4], vv[4];
+      for (bool go = p < x1 && !is_sentinel(suv[wu]); go;) {
+        int ju[8];
+        T aa[8], vv[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) ju[u] = p + u < x1 ? (int)sxu[p + u] : -1;
+        for (int u = 0; u < 8; ++u) ju[u] = p + u < x1 ? (int)sxu[p + u] : -1;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
           vv[u] = ju[u] >= 0 ? suv[ju[u]] : T(0);
           aa[u] = ju[u] >= 0 ? sxa[p + u] : T(0);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
           if (go && ju[u] >= 0 && !is_sentinel(vv[u])) {
             acc = accumulate<T, SUBSEQ>(acc, aa[u], vv[u]);
             ++p;
@@ -959,6 +962,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
         }
         go = go && p < x1;
       }
+      wu = p < x1 ? (int)sxu[p] : 0;
     };
     advance();
     long long tp = 0;
@@ -985,7 +989,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
     // the 1st, 17th, 33rd ... poll round of the block only (each is another round trip).
     const long long t0 = wall_clock64();
     int polls = 0;
-    long long waited = 0;
+    long long waited = 0, t_load = 0, t_adv = 0;
     int rounds = 0;
     // lanes whose external part is still pending: none below the current row, whose
     // predecessors in the block have all been solved
@@ -998,8 +1002,10 @@ __global__ __launch_bounds__(64) void k_trisolve(
         // lane whose external part completes catches up, in order, on the internal entries of
         // the rows solved while it was waiting.
         const long long tw = TRACE ? wall_clock64() : 0;
+        own[lane] = yown;  // read after the round's wave_sync()
         do {
           const unsigned long long before = pend;
+          const long long ta = TRACE ? wall_clock64() : 0;
           // One round trip: every distinct column of the block still missing is loaded again
           // (rows after k included: their values then wait in LDS, which beats a round trip
           // of their own later on).
@@ -1012,23 +1018,41 @@ __global__ __launch_bounds__(64) void k_trisolve(
             }
           }
           wave_sync();
+          const long long tb2 = TRACE ? wall_clock64() : 0;
           advance();
-          pend = __ballot(p < x1);
-          if (const unsigned long long caught = before & ~pend) {
-            // lanes whose external part just completed read the rows solved meanwhile from
-            // the lanes that solved them, in order
-            const bool mine = (caught >> lane) & 1ull;
-            for (;;) {
-              const bool need = mine && ip < i1 && (int)sik[ip] < k;
-              if (!__ballot(need)) break;
-              const T v = __shfl(yown, need ? (int)sik[ip] : lane);
-              if (need) {
-                acc = accumulate<T, SUBSEQ>(acc, sia[ip], v);
-                ++ip;
-              }
-            }
-            if (mine) refill();
+          if (TRACE) {
+            __builtin_amdgcn_s_waitcnt(0);
+            const long long tc = wall_clock64();
+            t_load += tb2 - ta;
+            t_adv += tc - tb2;
           }
+          if (((before >> lane) & 1ull) && p == x1) {
+            // the lane's external part just completed: it consumes, in order, its internal
+            // entries on the rows solved meanwhile, four at a time
+            for (bool go = ip < i1; go;) {
+              int kq[4];
+              T aa[4], vv[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) kq[u] = ip + u < i1 ? (int)sik[ip + u] : 64;
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                vv[u] = kq[u] < k ? own[kq[u]] : T(0);
+                aa[u] = kq[u] < k ? sia[ip + u] : T(0);
+              }
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                if (go && kq[u] < k) {
+                  acc = accumulate<T, SUBSEQ>(acc, aa[u], vv[u]);
+                  ++ip;
+                } else {
+                  go = false;
+                }
+              }
+              go = go && ip < i1;
+            }
+            refill();
+          }
+          pend = __ballot(p < x1);
           if ((pend >> k) & 1ull) {
             bool give_up = false;
             if ((polls++ & 15) == 0)
@@ -1063,7 +1087,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
           y = SUBSEQ ? acc / dl : (rl - acc) / dl;
           yown = y;
           publish(out + pl, y);
-          if (TRACE && rep == 0) trace[6 * (long long)nblocks + l] = wall_clock64();
+          if (TRACE && rep == 0) trace[8 * (long long)nblocks + l] = wall_clock64();
         }
         y = lane_broadcast(y, k);
         const bool hit = nk == k;
@@ -1080,12 +1104,14 @@ __global__ __launch_bounds__(64) void k_trisolve(
       }
     }
     if (TRACE && lane == 0 && rep == 0) {
-      trace[6 * (long long)b + 0] = tb;
-      trace[6 * (long long)b + 1] = tp;
-      trace[6 * (long long)b + 2] = wall_clock64();
-      trace[6 * (long long)b + 3] = ((long long)blockIdx.x << 8) | late;
-      trace[6 * (long long)b + 4] = waited;
-      trace[6 * (long long)b + 5] = rounds;
+      trace[8 * (long long)b + 0] = tb;
+      trace[8 * (long long)b + 1] = tp;
+      trace[8 * (long long)b + 2] = wall_clock64();
+      trace[8 * (long long)b + 3] = ((long long)blockIdx.x << 8) | late;
+      trace[8 * (long long)b + 4] = waited;
+      trace[8 * (long long)b + 5] = rounds;
+      trace[8 * (long long)b + 6] = t_load;
+      trace[8 * (long long)b + 7] = t_adv;
     }
     wave_sync();
   }
