@@ -223,6 +223,9 @@ def main():
                     help="mode(s) to time; 'best' times all three and reports the fastest whose "
                          "displacements stay within 1e-8 of the FP64 drop-in")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="end with a plain 256 MiB read + 256 MiB write kernel, the reference "
+                         "against which HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled")
     ap.add_argument("--cpu-iters", type=int, default=8)
     ap.add_argument("--workload", choices=["cube", "beam_cases", "box_batch", "tet10", "simp"],
                     default="cube", help="cube: the headline 3.1M-DOF solve; the others are the "
@@ -325,7 +328,8 @@ def main():
     d = kernels[dominant]
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": d["achieved_GBs"],
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None}
+                "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dominant, f"jittered_cube_{args.cells}", head)}
     p64 = results["fp64"]["prof"]
     spmv64_ms, spmv64_cnt = p64["spmv64"]
     spmv64_avg = spmv64_ms / max(spmv64_cnt, 1)
@@ -362,6 +366,30 @@ def main():
         "speedup_vs_cpu": round(cpu["value"] / per_step, 1) if cpu else None,
     }
     print(json.dumps(out), flush=True)
+    if args.copy_probe:
+        copy_probe(torch)
+
+
+def copy_probe(torch):
+    """One elementwise kernel that reads 256 MiB and writes 256 MiB: what HBM counters read for
+    it scales their readings for the solve's kernels."""
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    log("copy probe: 268435456 bytes read, 268435456 bytes written")
+
+
+def measured_traffic(kernel, workload, precision):
+    """HBM bytes per launch of `kernel` from the committed counter passes (profiles/), when
+    they were taken on this workload and precision; None otherwise."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    return rec.get("bytes_per_launch")
 
 
 if __name__ == "__main__":

@@ -854,8 +854,8 @@ constexpr int TRI_XCAP = 512;  // external entries a block may hold (host-enforc
 constexpr int TRI_ICAP = 512;  // internal entries a block may hold (host-enforced)
 constexpr int TRI_UCAP = 256;  // distinct external columns a block may hold (host-enforced)
 
-// TRACE: for diagnostics, each block records its timeline in trace[8 b .. 8 b + 7] and the
-// publish time of each slot's row in trace[8 nblocks + slot] (first right-hand side only).
+// TRACE: for diagnostics, each block records its timeline in trace[10 b .. 10 b + 9] and the
+// publish time of each slot's row in trace[10 nblocks + slot] (first right-hand side only).
 template <typename T, bool REV, bool SUBSEQ, bool TRACE = false>
 __global__ __launch_bounds__(64) void k_trisolve(
     int n, int nblocks, const int* __restrict__ border, const int* __restrict__ bstart,
@@ -990,6 +990,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
     const long long t0 = wall_clock64();
     int polls = 0;
     long long waited = 0, t_load = 0, t_adv = 0;
+    int n_polled = 0;
     int rounds = 0;
     // lanes whose external part is still pending: none below the current row, whose
     // predecessors in the block have all been solved
@@ -1012,6 +1013,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
 #pragma unroll 2
           for (int q = lane; q < nu; q += 64) {
             if (is_sentinel(suv[q])) {
+              if (TRACE) ++n_polled;
               const U u = __hip_atomic_load(reinterpret_cast<U*>(out + suc[q]), __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_AGENT);
               if (u != Bits<T>::S) suv[q] = from_bits<T>(u);
@@ -1087,7 +1089,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
           y = SUBSEQ ? acc / dl : (rl - acc) / dl;
           yown = y;
           publish(out + pl, y);
-          if (TRACE && rep == 0) trace[8 * (long long)nblocks + l] = wall_clock64();
+          if (TRACE && rep == 0) trace[10 * (long long)nblocks + l] = wall_clock64();
         }
         y = lane_broadcast(y, k);
         const bool hit = nk == k;
@@ -1103,15 +1105,21 @@ __global__ __launch_bounds__(64) void k_trisolve(
         nv2 = sia[q];
       }
     }
+    if (TRACE) {
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) n_polled += __shfl_xor(n_polled, o);
+    }
     if (TRACE && lane == 0 && rep == 0) {
-      trace[8 * (long long)b + 0] = tb;
-      trace[8 * (long long)b + 1] = tp;
-      trace[8 * (long long)b + 2] = wall_clock64();
-      trace[8 * (long long)b + 3] = ((long long)blockIdx.x << 8) | late;
-      trace[8 * (long long)b + 4] = waited;
-      trace[8 * (long long)b + 5] = rounds;
-      trace[8 * (long long)b + 6] = t_load;
-      trace[8 * (long long)b + 7] = t_adv;
+      trace[10 * (long long)b + 0] = tb;
+      trace[10 * (long long)b + 1] = tp;
+      trace[10 * (long long)b + 2] = wall_clock64();
+      trace[10 * (long long)b + 3] = ((long long)blockIdx.x << 8) | late;
+      trace[10 * (long long)b + 4] = waited;
+      trace[10 * (long long)b + 5] = rounds;
+      trace[10 * (long long)b + 6] = t_load;
+      trace[10 * (long long)b + 7] = t_adv;
+      trace[10 * (long long)b + 8] = n_polled;
+      trace[10 * (long long)b + 9] = nu;
     }
     wave_sync();
   }

@@ -399,8 +399,9 @@ TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const in
   }
   cdist.clear();
   cdist.shrink_to_fit();
-  // each block's distinct outside columns in order of first use, and every external entry's
-  // index in that list
+  // each block's distinct outside columns, ascending (neighbouring lanes then load from the
+  // same cache lines, which a wave's load merges), and every external entry's index in that
+  // list
   t.uoff.assign(nb + 1, 0);
   t.xu.resize(t.xcol.size());
   std::vector<int32_t> uidx(n, -1);
@@ -409,11 +410,14 @@ TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const in
     for (int32_t x = t.xoff[t.bstart[b]]; x < t.xoff[t.bstart[b + 1]]; ++x) {
       const int32_t c = t.xcol[x];
       if (uidx[c] < 0) {
-        uidx[c] = (int32_t)t.ucol.size() - ub;
+        uidx[c] = 0;
         t.ucol.push_back(c);
       }
-      t.xu[x] = (uint8_t)uidx[c];
     }
+    std::sort(t.ucol.begin() + ub, t.ucol.end());
+    for (int32_t u = ub; u < (int32_t)t.ucol.size(); ++u) uidx[t.ucol[u]] = u - ub;
+    for (int32_t x = t.xoff[t.bstart[b]]; x < t.xoff[t.bstart[b + 1]]; ++x)
+      t.xu[x] = (uint8_t)uidx[t.xcol[x]];
     for (int32_t u = ub; u < (int32_t)t.ucol.size(); ++u) uidx[t.ucol[u]] = -1;
     t.uoff[b + 1] = (int32_t)t.ucol.size();
   }

@@ -84,7 +84,7 @@ i64 dag_depth(const LowerPattern& lp);
 // are stored slot after slot, so a wave loads a whole block with coalesced reads.
 //
 // The rows of a chain are mesh neighbours and share most of their external columns: each block
-// also lists its distinct external columns (ucol, in order of first use) and each external
+// also lists its distinct external columns (ucol, ascending) and each external
 // entry names its column by its index in that list (xu), so that a wave waits on, and loads,
 // every external value once.
 constexpr double CHAIN_HOP_US = 3.0;

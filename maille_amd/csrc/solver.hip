@@ -1578,20 +1578,20 @@ ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* tr
     nblocks[0] = op.fw.nblk;
     nblocks[1] = op.bw.nblk;
     if (!trace_fw || !trace_bw) return ML_OK;
-    if (capacity < 8 * (int64_t)std::max(op.fw.nblk, op.bw.nblk) + op.n)
+    if (capacity < 10 * (int64_t)std::max(op.fw.nblk, op.bw.nblk) + op.n)
       throw ArgError("trace too small");
     if (!op.factored) {
       int st = op.factor();
       if (st != ML_OK) return st;
     }
     DBuf<long long> tf, tb;
-    tf.alloc(8 * op.fw.nblk + op.n);
-    tb.alloc(8 * op.bw.nblk + op.n);
+    tf.alloc(10 * op.fw.nblk + op.n);
+    tb.alloc(10 * op.bw.nblk + op.n);
     HIP_CHECK(hipMemcpyAsync(op.w_r.p, r, sizeof(double) * op.n, hipMemcpyHostToDevice, op.stream));
     op.apply_precondition(tf.p, tb.p);
-    HIP_CHECK(hipMemcpyAsync(trace_fw, tf.p, sizeof(long long) * (8 * op.fw.nblk + op.n),
+    HIP_CHECK(hipMemcpyAsync(trace_fw, tf.p, sizeof(long long) * (10 * op.fw.nblk + op.n),
                              hipMemcpyDeviceToHost, op.stream));
-    HIP_CHECK(hipMemcpyAsync(trace_bw, tb.p, sizeof(long long) * (8 * op.bw.nblk + op.n),
+    HIP_CHECK(hipMemcpyAsync(trace_bw, tb.p, sizeof(long long) * (10 * op.bw.nblk + op.n),
                              hipMemcpyDeviceToHost, op.stream));
     HIP_CHECK(hipStreamSynchronize(op.stream));
     return ML_OK;
