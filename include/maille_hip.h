@@ -149,12 +149,13 @@ ml_status ml_pcg_factor(ml_pcg* h, double* l, int32_t* shift_index);
 void ml_pcg_destroy(ml_pcg* h);
 /* Diagnostics (no counterpart in the reference): one preconditioner application with a
  * per-block timeline of both triangular solves. nblocks[2] receives the block counts; with
- * non-null traces (capacity int64 each, >= 8 per block + n) each block b records in
- * trace[8 b ..] its start, the end of its staging, its end (device wall-clock ticks), (wave << 8
- * | lanes still waiting on an external value after staging), the ticks spent polling, the
- * number of polling rounds, and of the polling ticks those spent loading and those spent
- * consuming what arrived; trace[8 nblocks + s] is the publish time of the row of slot s (the
- * block layout's s-th row, in logical row order of that solve). */
+ * non-null traces (capacity int64 each, >= 10 per block + n) each block b records in
+ * trace[10 b ..] its start, the end of its staging, its end (device wall-clock ticks), (wave <<
+ * 8 | lanes still waiting on an external value after staging), the ticks spent polling, the
+ * number of polling rounds, of the polling ticks those spent loading and those spent consuming
+ * what arrived, the loads the polls issued and the block's distinct external columns;
+ * trace[10 nblocks + s] is the publish time of the row of slot s (the block layout's s-th row,
+ * in logical row order of that solve). */
 ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* trace_fw,
                        int64_t* trace_bw, int64_t capacity);
 /* Diagnostics (no counterpart in the reference): the bound of every dataflow wait of this
