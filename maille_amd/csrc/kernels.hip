@@ -829,21 +829,27 @@ __global__ __launch_bounds__(256) void k_axpy_add(int n, double* __restrict__ x,
 //
 // Rows are grouped on the host into blocks: chains of at most 64 rows along each row's critical
 // dependency, stored as consecutive slots (bstart[]; brow[] is the row of a slot). Each slot's
-// entries are split into those whose column lies in another block (xcol/xval, "external") and
+// entries are split into those whose column lies in another block ("external": a coefficient
+// and the index xu of its column in the block's list ucol of distinct external columns) and
 // those inside it (ik = position of the row in the block, ival); the internal columns are the
 // row's last, so externals then internals is the row's own order. Both parts are stored slot
-// after slot. Wave w of the G-wave grid takes the blocks
-// at positions w, w+G, ... of the host's processing order border[], which is topological (a
-// block's external columns all lie in blocks earlier in border[]): with every wave resident the
-// lowest unfinished block always has its inputs and a running wave, so the solve progresses.
-// Per block: the wave stages the block's entries and the external values already published
-// into LDS with coalesced loads; every lane consumes, in order, the external values that were
-// ready. Then the rows are resolved one after the other: lane k finishes its row, publishes and
-// broadcasts it, and each lane whose next entry is that row consumes it at once. Rows whose
-// external values are all in run back to back in a branch-free loop; a row still missing some
-// makes the wave poll (eight device-scope loads per pending lane per round), and a lane that
-// completes late catches up on the rows solved meanwhile from the lanes holding them. Every
-// row is summed in the reference's order, and no in-block step waits on global memory.
+// after slot.
+//
+// Workgroup w of the G-workgroup grid takes the blocks at positions w, w+G, ... of the host's
+// processing order border[], which is topological (a block's external columns all lie in blocks
+// earlier in border[]): with every workgroup resident the lowest unfinished block always has
+// its inputs and a running workgroup, so the solve progresses. A workgroup is three waves:
+//  - two pollers, half a round apart, load at device scope the block's distinct external
+//    columns that are still missing, round after round, and drop each value into LDS as it
+//    arrives;
+//  - the solver never waits on global memory: every lane consumes, in order, its external
+//    values as they appear in LDS, and the rows are resolved one after the other. Lane k
+//    finishes its row, publishes and broadcasts it, and each lane whose next internal entry is
+//    that row consumes it at once. Rows whose external values are all in run back to back in a
+//    branch-free loop; a lane that completes late catches up on the rows solved meanwhile.
+// Splitting the roles keeps the publishing stores of the solver out of the pollers' waits (one
+// counter covers both loads and stores of a wave) and keeps the polls going while rows run.
+// Every row is summed in the reference's order.
 template <typename T, bool SUBSEQ>
 __device__ inline T accumulate(T acc, T a, T v) {
 #pragma clang fp contract(off)
@@ -853,11 +859,22 @@ __device__ inline T accumulate(T acc, T a, T v) {
 constexpr int TRI_XCAP = 512;  // external entries a block may hold (host-enforced)
 constexpr int TRI_ICAP = 512;  // internal entries a block may hold (host-enforced)
 constexpr int TRI_UCAP = 256;  // distinct external columns a block may hold (host-enforced)
+constexpr int TRI_THREADS = 192;  // three waves: the solver (lanes 0-63) and two pollers
+
+// LDS words shared by the waves of a workgroup, read and written whole.
+template <typename U>
+__device__ inline U lds_load(const U* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename U>
+__device__ inline void lds_store(U* p, U v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // TRACE: for diagnostics, each block records its timeline in trace[10 b .. 10 b + 9] and the
 // publish time of each slot's row in trace[10 nblocks + slot] (first right-hand side only).
 template <typename T, bool REV, bool SUBSEQ, bool TRACE = false>
-__global__ __launch_bounds__(64) void k_trisolve(
+__global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
     int n, int nblocks, const int* __restrict__ border, const int* __restrict__ bstart,
     const int* __restrict__ brow, const int* __restrict__ xoff, const int* __restrict__ uoff,
     const int* __restrict__ ucol, const unsigned char* __restrict__ xu,
@@ -868,7 +885,7 @@ __global__ __launch_bounds__(64) void k_trisolve(
 #pragma clang fp contract(off)
   if (S && S->done) return;
   typedef typename Bits<T>::U U;
-  __shared__ T suv[TRI_UCAP];    // values of the distinct external columns (sentinel: not yet)
+  __shared__ U suv[TRI_UCAP];    // values of the distinct external columns (sentinel: not yet)
   __shared__ int suc[TRI_UCAP];  // their physical indices
   __shared__ T sxa[TRI_XCAP];    // external coefficients
   __shared__ unsigned char sxu[TRI_XCAP];  // their columns, as indices into suv
@@ -876,8 +893,12 @@ __global__ __launch_bounds__(64) void k_trisolve(
   // ahead past the end of a lane's list
   __shared__ T sia[TRI_ICAP + 1];
   __shared__ unsigned char sik[TRI_ICAP + 1];
-  __shared__ T own[64];  // the rows of the block solved so far, copied when polling starts
-  const int lane = threadIdx.x;
+  __shared__ T own[64];     // the rows of the block solved so far, copied when the solver waits
+  __shared__ int s_state;   // 0: rows running; 1: rows done; 2: a wait gave up
+  __shared__ long long s_poll[3];  // TRACE: the poller's rounds, load ticks and loads
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;  // 0: the solver; 1, 2: the pollers
+  const bool poller = wv > 0;
   T* const out0 = out;
   const T* const rhs0 = rhs;
   for (int pos = blockIdx.x; pos < nblocks * nrep; pos += gridDim.x) {
@@ -894,234 +915,265 @@ __global__ __launch_bounds__(64) void k_trisolve(
     const int xb = xoff[s], nx = xoff[e] - xb;
     const int ib = ioff[s], ni = ioff[e] - ib;
     const int ub = uoff[b], nu = uoff[b + 1] - ub;
-    long long tb = TRACE ? wall_clock64() : 0;
-    // stage the block: its distinct external columns with the values published so far, read
-    // at device scope (a plain load could hit a stale line of this XCD's L2 that still holds
-    // the sentinel), the external entries (coefficient, column index) and the internal ones
-#pragma unroll 4
-    for (int q = lane; q < nu; q += 64) {
-      const int c = ucol[ub + q];
-      const int pc = REV ? n - 1 - c : c;
-      suc[q] = pc;
-      suv[q] = from_bits<T>(__hip_atomic_load(reinterpret_cast<U*>(out + pc), __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT));
-    }
-#pragma unroll 4
-    for (int q = lane; q < nx; q += 64) {
-      sxu[q] = xu[xb + q];
-      sxa[q] = xval[xb + q];
-    }
-#pragma unroll 4
-    for (int q = lane; q < ni; q += 64) {
-      sik[q] = ik[ib + q];
-      sia[q] = ival[ib + q];
-    }
-    if (lane == 0) {
-      sik[TRI_ICAP] = 64;
-      sia[TRI_ICAP] = T(0);
-    }
-    const bool active = lane < rows;
-    const int l = s + lane;  // slot
-    const int row = active ? brow[l] : 0;
-    const int pl = REV ? n - 1 - row : row;
-    int p = 0, x1 = 0, ip = 0, i1 = 0;
-    T acc = T(0), rl = T(0), dl = T(1);
-    if (active) {
-      p = xoff[l] - xb;
-      x1 = xoff[l + 1] - xb;
-      ip = ioff[l] - ib;
-      i1 = ioff[l + 1] - ib;
-      rl = rhs[pl];
-      dl = diag[l];
-      acc = SUBSEQ ? rl : T(0);
-    }
-    wave_sync();
-    // Each lane consumes its external entries, in order, as far as their values are in; the
-    // entries are looked up eight at a time so that the LDS reads overlap. The index of the
-    // column a lane waits on is kept, so that a lane still waiting costs one LDS read.
-    int wu = p < x1 ? (int)sxu[p] : 0;
-    auto advance = [&]() {
-      for (bool go = p < x1 && !is_sentinel(suv[wu]); go;) {
-        int ju[8];
-        T aa[8], vv[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) ju[u] = p + u < x1 ? (int)sxu[p + u] : -1;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          vv[u] = ju[u] >= 0 ? suv[ju[u]] : T(0);
-          aa[u] = ju[u] >= 0 ? sxa[p + u] : T(0);
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          if (go && ju[u] >= 0 && !is_sentinel(vv[u])) {
-            acc = accumulate<T, SUBSEQ>(acc, aa[u], vv[u]);
-            ++p;
-          } else {
-            go = false;
-          }
-        }
-        go = go && p < x1;
-      }
-      wu = p < x1 ? (int)sxu[p] : 0;
-    };
-    advance();
-    long long tp = 0;
-    int late = 0;
-    if (TRACE) {
-      tp = wall_clock64();
-      late = __popcll(__ballot(p < x1));
-    }
-    // The lane's next internal entry (nk: the block position of the row it reads, nv: its
-    // coefficient) and the one after, read ahead so that consuming a row never waits on LDS.
-    // A lane whose external part is still pending holds nk = 64, which matches no row, until
-    // it catches up.
-    int nk = 64, nk2 = 64;
-    T nv = T(0), nv2 = T(0);
-    T yown = T(0);  // the lane's solved value, once its row is done
-    auto refill = [&]() {
-      nk = ip < i1 ? (int)sik[ip] : 64;
-      nv = ip < i1 ? sia[ip] : T(0);
-      nk2 = ip + 1 < i1 ? (int)sik[ip + 1] : 64;
-      nv2 = ip + 1 < i1 ? sia[ip + 1] : T(0);
-    };
-    if (p == x1) refill();
-    // Waits of the block are bounded from its start; the clock and the abort flag are read on
-    // the 1st, 17th, 33rd ... poll round of the block only (each is another round trip).
+    const long long tb = TRACE ? wall_clock64() : 0;
+    // Waits of the block are bounded from its start.
     const long long t0 = wall_clock64();
-    int polls = 0;
-    long long waited = 0, t_load = 0, t_adv = 0;
-    int n_polled = 0;
-    int rounds = 0;
-    // lanes whose external part is still pending: none below the current row, whose
-    // predecessors in the block have all been solved
-    unsigned long long pend = __ballot(p < x1);
-    int k = 0;
-    while (k < rows) {
-      if ((pend >> k) & 1ull) {
-        // Row k waits for external values: the whole wave polls, each lane with pending
-        // entries loading its next eight, so one round trip serves every row of the block. A
-        // lane whose external part completes catches up, in order, on the internal entries of
-        // the rows solved while it was waiting.
-        const long long tw = TRACE ? wall_clock64() : 0;
-        own[lane] = yown;  // read after the round's wave_sync()
-        do {
-          const unsigned long long before = pend;
-          const long long ta = TRACE ? wall_clock64() : 0;
-          // One round trip: every distinct column of the block still missing is loaded again
-          // (rows after k included: their values then wait in LDS, which beats a round trip
-          // of their own later on).
+    // per-lane state of the solver
+    int p = 0, x1 = 0, ip = 0, i1 = 0, pl = 0, l = 0;
+    T acc = T(0), rl = T(0), dl = T(1);
+    // stage the block: the poller the distinct external columns with the values published so
+    // far, read at device scope (a plain load could hit a stale line of this XCD's L2 that
+    // still holds the sentinel); the solver the external entries (coefficient, column index)
+    // and the internal ones
+    if (wv == 1) {
+#pragma unroll 4
+      for (int q = lane; q < nu; q += 64) {
+        const int c = ucol[ub + q];
+        const int pc = REV ? n - 1 - c : c;
+        suc[q] = pc;
+        suv[q] = __hip_atomic_load(reinterpret_cast<U*>(out + pc), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+      }
+    } else if (wv == 0) {
+#pragma unroll 4
+      for (int q = lane; q < nx; q += 64) {
+        sxu[q] = xu[xb + q];
+        sxa[q] = xval[xb + q];
+      }
+#pragma unroll 4
+      for (int q = lane; q < ni; q += 64) {
+        sik[q] = ik[ib + q];
+        sia[q] = ival[ib + q];
+      }
+      if (lane == 0) {
+        sik[TRI_ICAP] = 64;
+        sia[TRI_ICAP] = T(0);
+        s_state = 0;
+      }
+      if (lane < rows) {
+        l = s + lane;  // slot
+        const int row = brow[l];
+        pl = REV ? n - 1 - row : row;
+        p = xoff[l] - xb;
+        x1 = xoff[l + 1] - xb;
+        ip = ioff[l] - ib;
+        i1 = ioff[l + 1] - ib;
+        rl = rhs[pl];
+        dl = diag[l];
+        acc = SUBSEQ ? rl : T(0);
+      }
+    }
+    __syncthreads();
+    if (poller) {
+      // One round trip per round: every distinct column still missing is loaded again, until
+      // all are in or the rows are done. The clock and the abort flag are read on the 1st,
+      // 17th, 33rd ... round only (each is another round trip).
+      long long t_load = 0;
+      int rounds = 0, n_polled = 0;
+      // the second poller starts half a round later, so that between them a missing column is
+      // looked at twice per round trip
+      if (wv == 2) __builtin_amdgcn_s_sleep(16);
+      for (int it = 0;; ++it) {
+        const long long ta = TRACE ? wall_clock64() : 0;
+        bool missing = false;
 #pragma unroll 2
-          for (int q = lane; q < nu; q += 64) {
-            if (is_sentinel(suv[q])) {
-              if (TRACE) ++n_polled;
-              const U u = __hip_atomic_load(reinterpret_cast<U*>(out + suc[q]), __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-              if (u != Bits<T>::S) suv[q] = from_bits<T>(u);
-            }
+        for (int q = lane; q < nu; q += 64) {
+          if (lds_load(&suv[q]) == Bits<T>::S) {
+            if (TRACE) ++n_polled;
+            const U u = __hip_atomic_load(reinterpret_cast<U*>(out + suc[q]), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+            if (u != Bits<T>::S)
+              lds_store(&suv[q], u);
+            else
+              missing = true;
           }
-          wave_sync();
-          const long long tb2 = TRACE ? wall_clock64() : 0;
-          advance();
-          if (TRACE) {
-            __builtin_amdgcn_s_waitcnt(0);
-            const long long tc = wall_clock64();
-            t_load += tb2 - ta;
-            t_adv += tc - tb2;
-          }
-          if (((before >> lane) & 1ull) && p == x1) {
-            // the lane's external part just completed: it consumes, in order, its internal
-            // entries on the rows solved meanwhile, four at a time
-            for (bool go = ip < i1; go;) {
-              int kq[4];
-              T aa[4], vv[4];
-#pragma unroll
-              for (int u = 0; u < 4; ++u) kq[u] = ip + u < i1 ? (int)sik[ip + u] : 64;
-#pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                vv[u] = kq[u] < k ? own[kq[u]] : T(0);
-                aa[u] = kq[u] < k ? sia[ip + u] : T(0);
-              }
-#pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                if (go && kq[u] < k) {
-                  acc = accumulate<T, SUBSEQ>(acc, aa[u], vv[u]);
-                  ++ip;
-                } else {
-                  go = false;
-                }
-              }
-              go = go && ip < i1;
-            }
-            refill();
-          }
-          pend = __ballot(p < x1);
-          if ((pend >> k) & 1ull) {
-            bool give_up = false;
-            if ((polls++ & 15) == 0)
-              give_up = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-                        wall_clock64() - t0 > limit;
-            if (__builtin_amdgcn_readfirstlane((int)give_up)) {
-              // give up: the host reports ML_DEVICE_TIMEOUT; every lane stops waiting
-              __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              if (p < x1) {
-                p = x1;
-                while (ip < i1 && (int)sik[ip] < k) ++ip;
-                refill();
-              }
-              pend = 0;
-            } else {
-              __builtin_amdgcn_s_sleep(1);
-            }
-          }
-          if (TRACE) ++rounds;
-        } while ((pend >> k) & 1ull);
-        if (TRACE) waited += wall_clock64() - tw;
-      }
-      // rows k .. kend-1 have all their external values: solve them back to back. Loads
-      // the polls left in flight are drained first, so that the row loop only ever waits on
-      // its own read-ahead, issued a row earlier. The loop has no branch but the one around
-      // the division: every lane takes the same steps, selecting on whether it reads row k.
-      const int kend = pend ? min(rows, (int)__builtin_ctzll(pend)) : rows;
-      __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
-      for (; k < kend; ++k) {
-        T y = acc;
-        if (lane == k) {
-          y = SUBSEQ ? acc / dl : (rl - acc) / dl;
-          yown = y;
-          publish(out + pl, y);
-          if (TRACE && rep == 0) trace[10 * (long long)nblocks + l] = wall_clock64();
         }
-        y = lane_broadcast(y, k);
-        const bool hit = nk == k;
-        const T upd = accumulate<T, SUBSEQ>(acc, nv, y);
-        acc = hit ? upd : acc;
-        ip += hit ? 1 : 0;
-        nk = hit ? nk2 : nk;
-        nv = hit ? nv2 : nv;
-        // the entry after next (the same one again when the lane did not advance), or the
-        // terminal position 64 past the end of the list
-        const int q = ip + 1 < i1 ? ip + 1 : TRI_ICAP;
-        nk2 = sik[q];
-        nv2 = sia[q];
+        if (TRACE) {
+          t_load += wall_clock64() - ta;
+          ++rounds;
+        }
+        if (!__ballot(missing) || lds_load(&s_state) != 0) break;
+        if ((it & 15) == 0) {
+          const bool give_up =
+              __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+              wall_clock64() - t0 > limit;
+          if (__builtin_amdgcn_readfirstlane((int)give_up)) {
+            // give up: the host reports ML_DEVICE_TIMEOUT; the solver stops waiting
+            __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) lds_store(&s_state, 2);
+            break;
+          }
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (TRACE) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) n_polled += __shfl_xor(n_polled, o);
+        if (wv == 1 && lane == 0) {
+          s_poll[0] = rounds;
+          s_poll[1] = t_load;
+          s_poll[2] = n_polled;
+        }
+      }
+    } else {
+      // Each lane consumes its external entries, in order, as far as their values are in; the
+      // entries are looked up eight at a time so that the LDS reads overlap. The index of the
+      // column a lane waits on is kept, so that a lane still waiting costs one LDS read.
+      int wu = p < x1 ? (int)sxu[p] : 0;
+      auto advance = [&]() {
+        for (bool go = p < x1 && lds_load(&suv[wu]) != Bits<T>::S; go;) {
+          int ju[8];
+          T aa[8];
+          U vv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) ju[u] = p + u < x1 ? (int)sxu[p + u] : -1;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            vv[u] = ju[u] >= 0 ? lds_load(&suv[ju[u]]) : Bits<T>::S;
+            aa[u] = ju[u] >= 0 ? sxa[p + u] : T(0);
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            if (go && ju[u] >= 0 && vv[u] != Bits<T>::S) {
+              acc = accumulate<T, SUBSEQ>(acc, aa[u], from_bits<T>(vv[u]));
+              ++p;
+            } else {
+              go = false;
+            }
+          }
+          go = go && p < x1;
+        }
+        wu = p < x1 ? (int)sxu[p] : 0;
+      };
+      advance();
+      long long tp = 0;
+      int late = 0;
+      if (TRACE) {
+        tp = wall_clock64();
+        late = __popcll(__ballot(p < x1));
+      }
+      // The lane's next internal entry (nk: the block position of the row it reads, nv: its
+      // coefficient) and the one after, read ahead so that consuming a row never waits on LDS.
+      // A lane whose external part is still pending holds nk = 64, which matches no row, until
+      // it catches up.
+      int nk = 64, nk2 = 64;
+      T nv = T(0), nv2 = T(0);
+      T yown = T(0);  // the lane's solved value, once its row is done
+      auto refill = [&]() {
+        nk = ip < i1 ? (int)sik[ip] : 64;
+        nv = ip < i1 ? sia[ip] : T(0);
+        nk2 = ip + 1 < i1 ? (int)sik[ip + 1] : 64;
+        nv2 = ip + 1 < i1 ? sia[ip + 1] : T(0);
+      };
+      if (p == x1) refill();
+      long long waited = 0;
+      int spins = 0;
+      // lanes whose external part is still pending: none below the current row, whose
+      // predecessors in the block have all been solved
+      unsigned long long pend = __ballot(p < x1);
+      int k = 0;
+      while (k < rows) {
+        if ((pend >> k) & 1ull) {
+          // Row k waits for external values: the lanes take in what the poller has dropped
+          // into LDS, and a lane whose external part completes catches up, in order, on its
+          // internal entries of the rows solved so far (four at a time, from own[]).
+          const long long tw = TRACE ? wall_clock64() : 0;
+          own[lane] = yown;
+          wave_sync();
+          do {
+            const unsigned long long before = pend;
+            advance();
+            if (((before >> lane) & 1ull) && p == x1) {
+              for (bool go = ip < i1; go;) {
+                int kq[4];
+                T aa[4], vv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) kq[u] = ip + u < i1 ? (int)sik[ip + u] : 64;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                  vv[u] = kq[u] < k ? own[kq[u]] : T(0);
+                  aa[u] = kq[u] < k ? sia[ip + u] : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                  if (go && kq[u] < k) {
+                    acc = accumulate<T, SUBSEQ>(acc, aa[u], vv[u]);
+                    ++ip;
+                  } else {
+                    go = false;
+                  }
+                }
+                go = go && ip < i1;
+              }
+              refill();
+            }
+            pend = __ballot(p < x1);
+            if ((pend >> k) & 1ull) {
+              // the poller gave up, or (every 16th spin) this wait has outlived the bound
+              bool give_up = lds_load(&s_state) == 2;
+              if (!give_up && (spins & 15) == 0) give_up = wall_clock64() - t0 > limit;
+              ++spins;
+              if (__builtin_amdgcn_readfirstlane((int)give_up)) {
+                // the host reports ML_DEVICE_TIMEOUT; every lane stops waiting
+                __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (p < x1) {
+                  p = x1;
+                  while (ip < i1 && (int)sik[ip] < k) ++ip;
+                  refill();
+                }
+                pend = 0;
+              } else {
+                __builtin_amdgcn_s_sleep(1);
+              }
+            }
+          } while ((pend >> k) & 1ull);
+          if (TRACE) waited += wall_clock64() - tw;
+        }
+        // Rows k .. kend-1 have all their external values: solve them back to back. Loads left
+        // in flight are drained first, so that the row loop only ever waits on its own
+        // read-ahead, issued a row earlier. The loop has no branch but the one around the
+        // division: every lane takes the same steps, selecting on whether it reads row k.
+        const int kend = pend ? min(rows, (int)__builtin_ctzll(pend)) : rows;
+        __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
+        for (; k < kend; ++k) {
+          T y = acc;
+          if (lane == k) {
+            y = SUBSEQ ? acc / dl : (rl - acc) / dl;
+            yown = y;
+            publish(out + pl, y);
+            if (TRACE && rep == 0) trace[10 * (long long)nblocks + l] = wall_clock64();
+          }
+          y = lane_broadcast(y, k);
+          const bool hit = nk == k;
+          const T upd = accumulate<T, SUBSEQ>(acc, nv, y);
+          acc = hit ? upd : acc;
+          ip += hit ? 1 : 0;
+          nk = hit ? nk2 : nk;
+          nv = hit ? nv2 : nv;
+          // the entry after next (the same one again when the lane did not advance), or the
+          // terminal position 64 past the end of the list
+          const int q = ip + 1 < i1 ? ip + 1 : TRI_ICAP;
+          nk2 = sik[q];
+          nv2 = sia[q];
+        }
+      }
+      if (lane == 0) lds_store(&s_state, 1);  // the poller stops
+      if (TRACE && lane == 0 && rep == 0) {
+        trace[10 * (long long)b + 0] = tb;
+        trace[10 * (long long)b + 1] = tp;
+        trace[10 * (long long)b + 3] = ((long long)blockIdx.x << 8) | late;
+        trace[10 * (long long)b + 4] = waited;
+        trace[10 * (long long)b + 7] = spins;
+        trace[10 * (long long)b + 9] = nu;
       }
     }
-    if (TRACE) {
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) n_polled += __shfl_xor(n_polled, o);
-    }
-    if (TRACE && lane == 0 && rep == 0) {
-      trace[10 * (long long)b + 0] = tb;
-      trace[10 * (long long)b + 1] = tp;
+    __syncthreads();
+    if (TRACE && threadIdx.x == 0 && rep == 0) {
       trace[10 * (long long)b + 2] = wall_clock64();
-      trace[10 * (long long)b + 3] = ((long long)blockIdx.x << 8) | late;
-      trace[10 * (long long)b + 4] = waited;
-      trace[10 * (long long)b + 5] = rounds;
-      trace[10 * (long long)b + 6] = t_load;
-      trace[10 * (long long)b + 7] = t_adv;
-      trace[10 * (long long)b + 8] = n_polled;
-      trace[10 * (long long)b + 9] = nu;
+      trace[10 * (long long)b + 5] = s_poll[0];
+      trace[10 * (long long)b + 6] = s_poll[1];
+      trace[10 * (long long)b + 8] = s_poll[2];
     }
-    wave_sync();
   }
 }
 

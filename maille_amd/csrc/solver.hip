@@ -130,24 +130,19 @@ int device_cus() {
   return cus;
 }
 
-// Waves of a persistent 64-thread kernel that are resident at once on the whole device. The
-// occupancy the runtime reports for LDS-bound kernels can be one workgroup per CU more than
-// the hardware actually holds (measured on gfx950: 26/CU reported for 6 KB of LDS, 25 hold;
-// 13/CU for 12 KB, 12 hold), and a dataflow kernel whose waves are not all resident waits
-// forever, so one workgroup per CU is kept in reserve.
-// Waves of a 64-thread persistent dataflow kernel launched per CU: three quarters of the
-// occupancy the runtime reports. The reported occupancy of LDS-heavy kernels has not always
-// been co-resident on gfx950, and the dataflow kernels need every wave of their grid resident;
-// the margin costs little (their speed hardly changes between 7 and 13 waves per CU). A
+// Workgroups of a persistent dataflow kernel launched per CU: three quarters of the occupancy
+// the runtime reports. The reported occupancy of LDS-heavy kernels has not always been
+// co-resident on gfx950, and the dataflow kernels need every workgroup of their grid resident;
+// the margin costs little (their speed hardly changes between 7 and 13 workgroups per CU). A
 // dataflow launch that stalls anyway ends through its bounded waits, and the caller then
 // repeats the work on a smaller grid (Operator::shrink).
-int occupancy_per_cu(const void* kernel) {
+int occupancy_per_cu(const void* kernel, int threads) {
   int per_cu = 0;
-  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0));
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
   return std::max(1, per_cu);
 }
-int dataflow_grid(const void* kernel) {
-  return std::max(1, (3 * occupancy_per_cu(kernel)) / 4) * device_cus();
+int dataflow_grid(const void* kernel, int threads) {
+  return std::max(1, (3 * occupancy_per_cu(kernel, threads)) / 4) * device_cus();
 }
 
 // The bound of every dataflow wait: 2 s of the device's steady wall clock (wall_clock64 ticks
@@ -160,15 +155,16 @@ long long wait_limit() {
   return 2000ll * khz;
 }
 
-// Launches a persistent dataflow kernel (64-thread workgroups, one wave each) cooperatively, so
-// that the runtime admits the grid only if it can hold all of it at once; `grid` is lowered
-// (and remembered) while the device refuses it as too large.
+// Launches a persistent dataflow kernel of `threads`-thread workgroups cooperatively, so that
+// the runtime admits the grid only if it can hold all of it at once; `grid` is lowered (and
+// remembered) while the device refuses it as too large.
 template <typename... Args>
-void launch_dataflow(void (*kernel)(Args...), int& grid, hipStream_t stream, Args... args) {
+void launch_dataflow(void (*kernel)(Args...), int& grid, int threads, hipStream_t stream,
+                     Args... args) {
   void* params[] = {(void*)&args...};
   for (;;) {
-    hipError_t e = hipLaunchCooperativeKernel((const void*)kernel, dim3(grid), dim3(64), params,
-                                              0, stream);
+    hipError_t e = hipLaunchCooperativeKernel((const void*)kernel, dim3(grid), dim3(threads),
+                                              params, 0, stream);
     if (e == hipSuccess) return;
     (void)hipGetLastError();
     if (e != hipErrorCooperativeLaunchTooLarge || grid <= 1)
@@ -302,14 +298,14 @@ struct Operator {
     d_sval.alloc(d_scol.n);
     for (auto* b : {&w_x, &w_r, &w_p, &w_ap, &w_y, &w_z, &w_b, &w_d, &w_ax}) b->alloc(n);
     const int nblk = std::max(fw.nblk, bw.nblk);
-    cap_tri64 = dataflow_grid((const void*)mlk::k_trisolve<double, false, false>);
-    cap_tri32 = dataflow_grid((const void*)mlk::k_trisolve<float, false, false>);
+    cap_tri64 = dataflow_grid((const void*)mlk::k_trisolve<double, false, false>, mlk::TRI_THREADS);
+    cap_tri32 = dataflow_grid((const void*)mlk::k_trisolve<float, false, false>, mlk::TRI_THREADS);
     full_tri64 = std::min<int>(nblk, cap_tri64);
     full_tri32 = std::min<int>(nblk, cap_tri32);
-    full_ic = std::min<int>(n, std::min(dataflow_grid((const void*)mlk::k_ic0<double, 512>),
-                                        dataflow_grid((const void*)mlk::k_ic0<float, 512>)));
-    full_ic64 = std::min<int>(n, std::min(dataflow_grid((const void*)mlk::k_ic0<double, 64>),
-                                          dataflow_grid((const void*)mlk::k_ic0<float, 64>)));
+    full_ic = std::min<int>(n, std::min(dataflow_grid((const void*)mlk::k_ic0<double, 512>, 64),
+                                        dataflow_grid((const void*)mlk::k_ic0<float, 512>, 64)));
+    full_ic64 = std::min<int>(n, std::min(dataflow_grid((const void*)mlk::k_ic0<double, 64>, 64),
+                                          dataflow_grid((const void*)mlk::k_ic0<float, 64>, 64)));
     shrinks = 0;
     set_grids();
     w_partial.alloc(std::max<int64_t>(grid_for(n), 1));
@@ -373,10 +369,10 @@ struct Operator {
     long long limit = wait_ticks;
     int nn = n;
     if (max_row <= 64)
-      launch_dataflow(mlk::k_ic0<T, 64>, ic_grid64, stream, nn, off, col, a, l, shift, fail,
+      launch_dataflow(mlk::k_ic0<T, 64>, ic_grid64, 64, stream, nn, off, col, a, l, shift, fail,
                       abort_flag, limit, where, seg, seg_shift);
     else
-      launch_dataflow(mlk::k_ic0<T, 512>, ic_grid, stream, nn, off, col, a, l, shift, fail,
+      launch_dataflow(mlk::k_ic0<T, 512>, ic_grid, 64, stream, nn, off, col, a, l, shift, fail,
                       abort_flag, limit, where, seg, seg_shift);
     prof.end(P_IC0, ev, stream);
     int flags[6];
@@ -476,7 +472,7 @@ struct Operator {
       const T* dg = f32 ? (const T*)t.diag32.p : (const T*)t.diag.p;
       int nb = t.nblk;
       auto go = [&](auto kernel) {
-        launch_dataflow(kernel, grid, stream, nn, nb, (const int*)t.border.p,
+        launch_dataflow(kernel, grid, mlk::TRI_THREADS, stream, nn, nb, (const int*)t.border.p,
                         (const int*)t.bstart.p, (const int*)t.brow.p, (const int*)t.xoff.p,
                         (const int*)t.uoff.p, (const int*)t.ucol.p, (const unsigned char*)t.xu.p,
                         xv, (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out,
@@ -667,8 +663,9 @@ struct Operator {
       if (st != ML_OK) return st;
     }
     const bool f32 = precision != ML_FP64;
-    const int occ = f32 ? occupancy_per_cu((const void*)mlk::k_trisolve<float, false, false>)
-                        : occupancy_per_cu((const void*)mlk::k_trisolve<double, false, false>);
+    const int occ =
+        f32 ? occupancy_per_cu((const void*)mlk::k_trisolve<float, false, false>, mlk::TRI_THREADS)
+            : occupancy_per_cu((const void*)mlk::k_trisolve<double, false, false>, mlk::TRI_THREADS);
     const int cus = device_cus();
     const int nblk = std::max(fw.nblk, bw.nblk);
     const int top = std::max(1, (17 * occ) / 20);  // stay clear of the reported occupancy
