@@ -1178,15 +1178,19 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
 }
 
 // ------------------------------------------------------------------ IC(0)
-// Csr::incomplete_cholesky (iterative.rs:146-170), one wavefront per row, rows i = w, w+G, ...
-// (row i depends only on rows j < i, so with every wave resident the launch progresses).
+// Csr::incomplete_cholesky (iterative.rs:146-170), one wavefront per row. Wave w of the G-wave
+// grid takes the rows at positions w, w+G, ... of order[], the rows sorted by their level in the
+// dependency DAG: a row only reads rows at earlier positions, so with every wave resident the
+// launch progresses, and the rows a wave takes one after the other are a few levels apart, so
+// it seldom holds a row long before that row's inputs exist.
 // Row i: for each entry p (column j, ascending) the sum over row j's entries whose column k is
 // also a computed column of row i is taken in ascending k order, exactly the reference's sum
 // over its dense work vector; l_ij = (a_ij - s) / l_jj; then the pivot a_ii(1+shift) - sum of
 // squares must exceed 1e-12 a_ii. Rows of L are published entry by entry; a failed pivot
 // raises *fail and every later row only writes zeros so the launch drains quickly.
 template <typename T, int MAXROW>
-__global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
+__global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ order,
+                                            const int* __restrict__ off,
                                             const int* __restrict__ col,
                                             const T* __restrict__ a, T* l, T shift,
                                             int* fail, int* abort_flag, long long limit,
@@ -1197,7 +1201,8 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ off,
   __shared__ T sw[MAXROW];
   __shared__ T sa[MAXROW];
   const int lane = threadIdx.x;
-  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+  for (int pos = blockIdx.x; pos < n; pos += gridDim.x) {
+    const int i = order[pos];
     const int start = off[i], dg = off[i + 1] - 1, len = dg - start;
     // batched systems: each has its own shift and failure flag
     int* const my_fail = seg ? fail + seg[i] : fail;

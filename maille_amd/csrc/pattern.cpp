@@ -454,6 +454,22 @@ std::string device_layouts(const LowerPattern& lp, const TriCaps& caps, DeviceLa
     d.max_row = std::max<int>(d.max_row, (int)(lp.offsets[i + 1] - lp.offsets[i]));
   d.off32.assign(lp.offsets.begin(), lp.offsets.end());
   d.col32.assign(lp.cols.begin(), lp.cols.end());
+  {
+    // rows by DAG level (counting sort: stable, so ascending row within a level)
+    std::vector<int32_t> level(n, 0);
+    int32_t top = 0;
+    for (i64 i = 0; i < n; ++i) {
+      int32_t m = 0;
+      for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) m = std::max(m, level[lp.cols[p]] + 1);
+      level[i] = m;
+      top = std::max(top, m);
+    }
+    std::vector<int64_t> at(top + 2, 0);
+    for (i64 i = 0; i < n; ++i) at[level[i] + 1] += 1;
+    for (int32_t v = 0; v <= top; ++v) at[v + 1] += at[v];
+    d.ic_order.resize(n);
+    for (i64 i = 0; i < n; ++i) d.ic_order[at[level[i]]++] = (int32_t)i;
+  }
   // sliced ELL of the full symmetric matrix
   SellLayout s = sell_layout(lp);
   TransposeLayout t = transpose_layout(lp);
@@ -586,6 +602,19 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, const 
   const i64 n = lp.n(), nnz = lp.nnz();
   if (d.n != n || d.nnz != nnz || (i64)d.off32.size() != n + 1 || (i64)d.col32.size() != nnz)
     return "CSR copy of the wrong size";
+  // IC(0) order: a permutation in which every row comes after the rows it reads
+  {
+    if ((i64)d.ic_order.size() != n) return "IC(0) order of the wrong length";
+    std::vector<int32_t> at(n, -1);
+    for (i64 k = 0; k < n; ++k) {
+      const i64 i = d.ic_order[k];
+      if (i < 0 || i >= n || at[i] >= 0) return "IC(0) order is not a permutation";
+      at[i] = (int32_t)k;
+    }
+    for (i64 i = 0; i < n; ++i)
+      for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p)
+        if (at[lp.cols[p]] >= at[i]) return "IC(0) order is not topological";
+  }
   // sliced ELL
   if (d.slices != (n + 63) / 64 || (i64)d.soff.size() != d.slices + 1 || d.soff[0] != 0 ||
       (size_t)d.soff[d.slices] != d.scol.size() || d.scol.size() != d.ssrc.size() ||

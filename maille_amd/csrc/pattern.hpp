@@ -118,6 +118,10 @@ struct DeviceLayouts {
   int64_t n = 0, nnz = 0, depth = 0, slices = 0;
   int max_row = 0;  // longest row of L, diagonal included
   std::vector<int32_t> off32, col32;
+  // IC(0) processing order: the rows by level in the dependency DAG of L (a row's level is one
+  // more than the highest level among its columns), ascending row within a level; it is
+  // topological, and neighbouring positions are mostly independent rows
+  std::vector<int32_t> ic_order;
   std::vector<int32_t> soff, scol, slen, ssrc;
   TriLayout fw, bw;
 };

@@ -191,6 +191,7 @@ struct Operator {
   std::vector<int64_t> h_off;
   std::vector<int32_t> h_col;
   DBuf<int> d_off, d_col;
+  DBuf<int> d_icorder;   // IC(0) processing order (rows by dependency level)
   DBuf<double> d_a;      // values
   DBuf<float> d_a32;
   // sliced ELL of the full symmetric matrix
@@ -281,6 +282,7 @@ struct Operator {
     slices = (int)d.slices;
     d_off.upload_vec(d.off32, stream);
     d_col.upload_vec(d.col32, stream);
+    d_icorder.upload_vec(d.ic_order, stream);
     d_soff.upload_vec(d.soff, stream);
     d_scol.upload_vec(d.scol, stream);
     d_slen.upload_vec(d.slen, stream);
@@ -368,12 +370,13 @@ struct Operator {
     int* where = d_flags.p + 2;
     long long limit = wait_ticks;
     int nn = n;
+    const int* order = d_icorder.p;
     if (max_row <= 64)
-      launch_dataflow(mlk::k_ic0<T, 64>, ic_grid64, 64, stream, nn, off, col, a, l, shift, fail,
-                      abort_flag, limit, where, seg, seg_shift);
+      launch_dataflow(mlk::k_ic0<T, 64>, ic_grid64, 64, stream, nn, order, off, col, a, l, shift,
+                      fail, abort_flag, limit, where, seg, seg_shift);
     else
-      launch_dataflow(mlk::k_ic0<T, 512>, ic_grid, 64, stream, nn, off, col, a, l, shift, fail,
-                      abort_flag, limit, where, seg, seg_shift);
+      launch_dataflow(mlk::k_ic0<T, 512>, ic_grid, 64, stream, nn, order, off, col, a, l, shift,
+                      fail, abort_flag, limit, where, seg, seg_shift);
     prof.end(P_IC0, ev, stream);
     int flags[6];
     HIP_CHECK(hipMemcpyAsync(flags, d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, stream));
