@@ -462,7 +462,23 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
     const int base = soff[j >> 6] + (j & 63);
     const int len = slen[j];
     T s = sval[base] * x[scol[base]];
-    for (int k = 1; k < len; ++k) {
+    // eight entries at a time: their columns and values are loaded together, then the eight
+    // gathers, so that a wave keeps many loads in flight; the sum stays in entry order
+    int k = 1;
+    for (; k + 8 <= len; k += 8) {
+      int c[8];
+      T v[8], xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        c[u] = scol[base + ((k + u) << 6)];
+        v[u] = sval[base + ((k + u) << 6)];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xv[u] = x[c[u]];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u] * xv[u];
+    }
+    for (; k < len; ++k) {
       const int at = base + (k << 6);
       s += sval[at] * x[scol[at]];
     }
@@ -491,7 +507,21 @@ __global__ __launch_bounds__(256) void k_spmv_rep(int n, int nrep, const int* __
   const int base = soff[j >> 6] + (j & 63);
   const int len = slen[j];
   T s = sval[base] * xb[scol[base]];
-  for (int k = 1; k < len; ++k) {
+  int k = 1;
+  for (; k + 8 <= len; k += 8) {
+    int c[8];
+    T v[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      c[u] = scol[base + ((k + u) << 6)];
+      v[u] = sval[base + ((k + u) << 6)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = xb[c[u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u] * xv[u];
+  }
+  for (; k < len; ++k) {
     const int at = base + (k << 6);
     s += sval[at] * xb[scol[at]];
   }
