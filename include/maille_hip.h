@@ -150,12 +150,12 @@ void ml_pcg_destroy(ml_pcg* h);
 /* Diagnostics (no counterpart in the reference): one preconditioner application with a
  * per-block timeline of both triangular solves. nblocks[2] receives the block counts; with
  * non-null traces (capacity int64 each, >= 10 per block + n) each block b records in
- * trace[10 b ..] its start, the end of its staging, its end (device wall-clock ticks), (wave <<
- * 8 | lanes still waiting on an external value after staging), the ticks spent polling, the
- * number of polling rounds, of the polling ticks those spent loading and those spent consuming
- * what arrived, the loads the polls issued and the block's distinct external columns;
- * trace[10 nblocks + s] is the publish time of the row of slot s (the block layout's s-th row,
- * in logical row order of that solve). */
+ * trace[10 b ..] its start, the end of its staging, its end (device wall-clock ticks),
+ * (workgroup << 8 | rows still waiting on an external value after staging), the ticks the
+ * solving wave spent waiting, the first polling wave's rounds and its ticks in loads, the
+ * solving wave's waiting iterations, the loads the first polling wave issued and the block's
+ * distinct external columns; trace[10 nblocks + s] is the publish time of the row of slot s (the
+ * block layout's s-th row, in logical row order of that solve). */
 ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* trace_fw,
                        int64_t* trace_bw, int64_t capacity);
 /* Diagnostics (no counterpart in the reference): the bound of every dataflow wait of this
