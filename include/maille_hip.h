@@ -197,6 +197,11 @@ ml_status ml_model_solve_device(ml_model* m, const double* f_device, double* u_d
 /* element_stresses for every element (model.rs:596-598, element.rs:270-302): out[n_elements
  * x ngp x 9] = x, y, z, sxx, syy, szz, tyz, txz, txy per Gauss point (1 for Tet4, 4 Tet10). */
 ml_status ml_model_stresses(ml_model* m, const double* u, double* out);
+/* nodal_stresses (results.rs:138-166, smooth :184-221): the Gauss-point stresses of u
+ * extrapolated to each element's nodes and averaged over the elements sharing each node, every
+ * sum in the reference's element and Gauss-point order; out[n_nodes x 9] = x, y, z of the node,
+ * sxx, syy, szz, tyz, txz, txy (zeros at a node no element uses). */
+ml_status ml_model_nodal_stresses(ml_model* m, const double* u, double* out);
 /* face_pressure + FacePressure::add_load (assembly.rs:638-674) for dead pressures
  * (element, face, value), added to f[3*n_nodes] in the given order. Host. */
 ml_status ml_model_add_pressures(ml_model* m, int64_t count, const int64_t* elements,

@@ -40,7 +40,8 @@ EXPORTS = (
     "ml_pcg_precondition", "ml_pcg_factor", "ml_pcg_destroy", "ml_pcg_trace",
     "ml_pcg_set_wait_limit",
     "ml_model_create", "ml_model_assemble", "ml_model_set_young", "ml_model_factor",
-    "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_add_pressures",
+    "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_nodal_stresses",
+    "ml_model_add_pressures",
     "ml_model_nfree", "ml_model_nnz", "ml_model_levels", "ml_model_csr",
     "ml_model_compliance_gradient", "ml_model_destroy", "ml_model_profile",
     "ml_model_solve_many", "ml_batch_create", "ml_batch_solve", "ml_batch_destroy",
@@ -145,6 +146,7 @@ def lib():
             "ml_model_solve": (_int, [_P, _D, _D, _D, _S]),
             "ml_model_solve_device": (_int, [_P, _P, _P, _P, _S]),
             "ml_model_stresses": (_int, [_P, _D, _D]),
+            "ml_model_nodal_stresses": (_int, [_P, _D, _D]),
             "ml_model_add_pressures": (_int, [_P, _i64, _I, _I, _D, _D]),
             "ml_model_nfree": (_i64, [_P]),
             "ml_model_nnz": (_i64, [_P]),
@@ -411,6 +413,7 @@ class Model:
         fixed = np.ascontiguousarray(problem.fixed, dtype=np.uint8)
         imposed = _f64(problem.imposed)
         self.ndof = 3 * len(nodes)
+        self.n_nodes = len(nodes)
         self.ne, self.nen = ne, nen
         h = C.c_void_p()
         si = SolveInfo()
@@ -486,6 +489,14 @@ class Model:
         ngp = 1 if self.nen == 4 else 4
         out = np.zeros((self.ne, ngp, 9))
         _check(lib().ml_model_stresses(self._h, _d(u), _d(out)))
+        return out
+
+    def nodal_stresses(self, u):
+        """results.rs:138-166 (nodal_stresses): per node x, y, z, then sxx, syy, szz, tyz, txz,
+        txy averaged over the elements at the node, shape (n_nodes, 9)."""
+        u = _f64(u)
+        out = np.zeros((self.n_nodes, 9))
+        _check(lib().ml_model_nodal_stresses(self._h, _d(u), _d(out)))
         return out
 
     def nfree(self):

@@ -441,6 +441,58 @@ __global__ __launch_bounds__(256) void k_compliance_gradient(int64_t ne,
   out[e] = -s / young[e];
 }
 
+// Node-averaged stresses (nodal_stresses + smooth, results.rs:138-166,184-221), thread per node:
+// its incidences (element e, local node k), in element order, each add the element's Gauss-point
+// values extrapolated to node k, Gauss point after Gauss point (s += w * v); the sum is divided
+// by the number of incidences (an isolated node keeps zeros). Tet4: one Gauss point, weight 1.
+// Tet10: TETRA_4 point g lies near vertex g, and a vertex takes (3 sqrt5 + 1)/4 of its own point
+// and -(sqrt5 - 1)/4 of the others, an edge mid-node (sqrt5 + 1)/4 of its two end points' and
+// -(sqrt5 - 1)/4 of the others (results.rs:113-133). out per node: x, y, z, then the six
+// components in the Gauss-point layout's order.
+template <int NEN>
+__device__ inline double extrapolation_weight(int k, int g) {
+  constexpr double VERTEX = 1.9270509831248424, MIDDLE = 0.8090169943749475,
+                   OTHER = -0.30901699437494745;
+  if (NEN == 4) return 1.0;
+  if (k < 4) return k == g ? VERTEX : OTHER;
+  // mid-node k: edge (0-1), (1-2), (2-0), (0-3), (1-3), (2-3) for k = 4 .. 9
+  const int a = (k == 4 || k == 6 || k == 7) ? 0 : (k == 5 || k == 8) ? 1 : 2;
+  const int b = k == 4 ? 1 : k == 5 ? 2 : k == 6 ? 2 : 3;
+  return (g == a || g == b) ? MIDDLE : OTHER;
+}
+
+template <int NEN>
+__global__ __launch_bounds__(256) void k_nodal_stresses(int64_t n_nodes,
+                                                        const double* __restrict__ xyz,
+                                                        const int64_t* __restrict__ inc_off,
+                                                        const int* __restrict__ inc_item,
+                                                        const double* __restrict__ gp,
+                                                        double* __restrict__ out) {
+#pragma clang fp contract(off)
+  constexpr int NGP = NEN == 4 ? 1 : 4;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int64_t t0 = inc_off[n], t1 = inc_off[n + 1];
+  for (int64_t t = t0; t < t1; ++t) {
+    const int it = inc_item[t];
+    const int64_t e = it / NEN;
+    const int k = it - (int)e * NEN;
+    const double* p = gp + e * NGP * 9;
+#pragma unroll
+    for (int g = 0; g < NGP; ++g) {
+      const double w = extrapolation_weight<NEN>(k, g);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s[c] += w * p[g * 9 + 3 + c];
+    }
+  }
+  const double cnt = (double)(t1 > t0 ? t1 - t0 : 1);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) out[9 * n + d] = xyz[3 * n + d];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) out[9 * n + 3 + c] = s[c] / cnt;
+}
+
 // ------------------------------------------------------------------ SpMV (sliced ELL)
 // y = A x (Csr::multiply, iterative.rs:131-142). Lane j of a slice sums row j in the order
 // diagonal, lower entries, upper entries: the reference's accumulation order, so the FP64

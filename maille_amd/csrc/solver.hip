@@ -973,6 +973,10 @@ struct Model {
   int ngp = 1;
   DBuf<double> d_xyz, d_young, d_nu, d_ke, d_imposed, d_coupling, d_f, d_u, d_R, d_x, d_rhs;
   DBuf<double> d_ulast;  // displacements of the last solve (adjoint state of the compliance)
+  // node -> (element, local node) incidences in element order, built on first use
+  DBuf<int64_t> d_inc_off;
+  DBuf<int> d_inc_item;
+  bool have_inc = false;
   DBuf<int> d_conn, d_rank, d_cpl_dof, d_rea_dof;
   DBuf<int64_t> d_asm_off, d_cpl_off, d_rea_off, d_rea_item;
   DBuf<uint32_t> d_asm_slot, d_cpl_slot;
@@ -1128,6 +1132,32 @@ struct Model {
     else
       mlk::k_compliance_gradient<10><<<grid_for(ne), 256, 0, s()>>>(ne, d_conn.p, d_ke.p, d_young.p,
                                                                    d_ulast.p, out_dev);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // Gauss-point stresses (gp_dev: n_elements x ngp x 9) averaged at the nodes (n_nodes x 9).
+  void nodal_stresses(const double* gp_dev, double* out_dev) {
+    const int64_t nn = mesh.n_nodes, ne = plan.n_elements;
+    const int nen = plan.nen;
+    if (!have_inc) {
+      if (ne * nen > INT32_MAX) throw ArgError("too many element nodes for the incidence list");
+      std::vector<int64_t> off(nn + 1, 0);
+      for (int64_t t = 0; t < ne * nen; ++t) off[conn[t] + 1] += 1;
+      for (int64_t v = 0; v < nn; ++v) off[v + 1] += off[v];
+      std::vector<int> item(ne * nen);
+      std::vector<int64_t> at(off.begin(), off.end() - 1);
+      for (int64_t t = 0; t < ne * nen; ++t) item[at[conn[t]]++] = (int)t;  // element order
+      d_inc_off.upload_vec(off, s());
+      d_inc_item.upload_vec(item, s());
+      HIP_CHECK(hipStreamSynchronize(s()));
+      have_inc = true;
+    }
+    if (nen == 4)
+      mlk::k_nodal_stresses<4><<<grid_for(nn), 256, 0, s()>>>(nn, d_xyz.p, d_inc_off.p,
+                                                                d_inc_item.p, gp_dev, out_dev);
+    else
+      mlk::k_nodal_stresses<10><<<grid_for(nn), 256, 0, s()>>>(nn, d_xyz.p, d_inc_off.p,
+                                                                 d_inc_item.p, gp_dev, out_dev);
     HIP_CHECK(hipGetLastError());
   }
 
@@ -1694,6 +1724,23 @@ ml_status ml_model_stresses(ml_model* h, const double* u, double* out) {
     d_out.alloc(count);
     m.stresses(m.d_u.p, d_out.p);
     HIP_CHECK(hipMemcpyAsync(out, d_out.p, count * sizeof(double), hipMemcpyDeviceToHost, m.s()));
+    HIP_CHECK(hipStreamSynchronize(m.s()));
+    return ML_OK;
+  });
+}
+
+ml_status ml_model_nodal_stresses(ml_model* h, const double* u, double* out) {
+  return guarded([&]() -> int {
+    if (!h || !u || !out) throw ArgError("null argument");
+    Model& m = h->m;
+    HIP_CHECK(hipMemcpyAsync(m.d_u.p, u, sizeof(double) * m.plan.ndof, hipMemcpyHostToDevice, m.s()));
+    DBuf<double> d_gp, d_out;
+    d_gp.alloc((size_t)m.plan.n_elements * m.ngp * 9);
+    d_out.alloc((size_t)m.mesh.n_nodes * 9);
+    m.stresses(m.d_u.p, d_gp.p);
+    m.nodal_stresses(d_gp.p, d_out.p);
+    HIP_CHECK(hipMemcpyAsync(out, d_out.p, sizeof(double) * 9 * m.mesh.n_nodes,
+                             hipMemcpyDeviceToHost, m.s()));
     HIP_CHECK(hipStreamSynchronize(m.s()));
     return ML_OK;
   });

@@ -93,6 +93,7 @@ def _declare(L):
                                 _D]),
         "or_stresses": (None, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _D, _D]),
         "or_compliance_gradient": (None, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _D, _D]),
+        "or_nodal_stresses": (None, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D]),
         "or_add_pressures": (None, [C.c_int64, _D, C.c_int, _I, C.c_int64, _I, _I, _D, _D]),
     }
     for name, (res, args) in sig.items():
@@ -464,6 +465,17 @@ def stresses(nodes, conn, young, nu, u):
     out = np.zeros((ne, ngp, 9))
     lib().or_stresses(len(nodes), _d(nodes), ne, nen, _i(conn), _d(young), _d(nu), _d(u),
                       _d(out))
+    return out
+
+
+def nodal_stresses(nodes, conn, gp):
+    """results.rs:138-166,184-221 — Gauss-point stresses gp (ne, ngp, 9) extrapolated to the nodes
+    and averaged over the elements sharing each node, shape (n_nodes, 9): node position, then the
+    six components in gp's order."""
+    nodes, conn, gp = _f64(nodes), _i64(conn), _f64(gp)
+    ne, nen = conn.shape
+    out = np.zeros((len(nodes), 9))
+    lib().or_nodal_stresses(len(nodes), _d(nodes), ne, nen, _i(conn), _d(gp), _d(out))
     return out
 
 

@@ -969,6 +969,48 @@ void or_stresses(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64* c
   }
 }
 
+// results.rs:184-221 (smooth) with :138-166 (nodal_stresses) on tetrahedra. The Gauss-point
+// values gp[e][g][3..8] are extrapolated to each element's nodes and summed per node over the
+// elements in element order and, inside an element, over the Gauss points in order (s += w * v),
+// then divided by the number of elements at the node (an isolated node keeps zeros). Tet4: one
+// Gauss point, weight 1 (results.rs:96-97). Tet10: results.rs:113-133, the TETRA_4 point g lying
+// near vertex g: a vertex takes (3 sqrt5 + 1)/4 of its own point and -(sqrt5 - 1)/4 of the
+// others, an edge mid-node (sqrt5 + 1)/4 of its two end points' and -(sqrt5 - 1)/4 of the others.
+// out[n][0..2] = the node's position, out[n][3..8] the six components in gp's order.
+static const double TET10_VERTEX = 1.9270509831248424;
+static const double TET10_MIDDLE = 0.8090169943749475;
+static const double TET10_OTHER = -0.30901699437494745;
+
+static double extrapolation_weight(int nen, int k, int g) {
+  if (nen == 4) return 1.0;
+  if (k < 4) return k == g ? TET10_VERTEX : TET10_OTHER;
+  const int a = TET10_EDGES[k - 4][0], b = TET10_EDGES[k - 4][1];
+  return (g == a || g == b) ? TET10_MIDDLE : TET10_OTHER;
+}
+
+void or_nodal_stresses(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64* conn,
+                       const double* gp, double* out) {
+  const int ngp = nen == 4 ? 1 : 4;
+  std::vector<double> sum(6 * n_nodes, 0.0);
+  std::vector<i64> count(n_nodes, 0);
+  for (i64 e = 0; e < n_el; ++e) {
+    const double* points = gp + e * ngp * 9;
+    for (int k = 0; k < nen; ++k) {
+      const i64 n = conn[e * nen + k];
+      for (int g = 0; g < ngp; ++g) {
+        const double w = extrapolation_weight(nen, k, g);
+        for (int c = 0; c < 6; ++c) sum[6 * n + c] += w * points[g * 9 + 3 + c];
+      }
+      count[n] += 1;
+    }
+  }
+  for (i64 n = 0; n < n_nodes; ++n) {
+    for (int d = 0; d < 3; ++d) out[9 * n + d] = xyz[3 * n + d];
+    const double c = (double)std::max<i64>(count[n], 1);
+    for (int j = 0; j < 6; ++j) out[9 * n + 3 + j] = sum[6 * n + j] / c;
+  }
+}
+
 // The FP64 CPU adjoint of the compliance C = f.u (no counterpart in the reference, which has
 // no sensitivities): K_e = solid_stiffness (element.rs:223-243) is linear in E_e and the
 // adjoint state of C is u, so dC/dE_e = -u_e^T (K_e / E_e) u_e, summed row by row.

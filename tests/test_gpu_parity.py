@@ -155,6 +155,20 @@ def test_mixed_solve_matches_oracle(gpu, name, precision):
     assert rel_l2(u, ref["displacements"]) <= 1e-8
 
 
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_stresses_and_nodal_stresses_are_bit_exact(gpu, name):
+    """The Gauss-point stresses (element.rs:270-302) and the node-averaged stresses
+    (results.rs:138-166,184-221) of one displacement field, on the device and in the oracle,
+    bit for bit."""
+    p = PROBLEMS[name]
+    print(f"stresses {name}", flush=True)
+    m = gpu.Model(p)
+    u = np.random.default_rng(11).standard_normal(m.ndof) * 1e-3
+    gp_ref = oracle.stresses(p.nodes, p.conn, p.young, p.nu, u)
+    assert np.array_equal(m.stresses(u), gp_ref)
+    assert np.array_equal(m.nodal_stresses(u), oracle.nodal_stresses(p.nodes, p.conn, gp_ref))
+
+
 def test_iterative_matches_direct_on_reference_cantilever(gpu):
     """tests/iterative.rs:69-83 with the device PCG against the oracle's skyline Cholesky."""
     p = PROBLEMS["reference_cantilever"]

@@ -619,3 +619,62 @@ def test_compliance_gradient_matches_central_differences():
         assert abs(fd - g[e]) <= 1e-6 * abs(g[e]), (e, fd, g[e])
     # compliance is the sum of the element energies: C = -sum_e E_e dC/dE_e
     assert abs(-(p.young * g).sum() - p.f @ u) <= 1e-10 * abs(p.f @ u)
+
+
+# ------------------------------------------------------------------ nodal stresses
+TET10_REFERENCE_NODES = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (0.5, 0, 0),
+                                  (0.5, 0.5, 0), (0, 0.5, 0), (0, 0, 0.5), (0.5, 0, 0.5),
+                                  (0, 0.5, 0.5)], dtype=float)
+
+
+def test_tet10_extrapolation_reproduces_a_linear_function_at_the_ten_nodes():
+    """results.rs:263-285, through nodal_stresses on the reference Tet10 (one element, so no
+    averaging): f = 2 + 3 xi - 5 eta + 7 zeta known at the TETRA_4 points (quadrature.rs:108-138,
+    point g near vertex g) comes out at the ten nodes, in each of the six components."""
+    a, b = 0.585410196624968500, 0.138196601125010500
+    points = [(b, b, b), (a, b, b), (b, a, b), (b, b, a)]
+    f = lambda x, y, z: 2.0 + 3.0 * x - 5.0 * y + 7.0 * z
+    scale = np.array([1.0, -2.0, 3.0, 0.5, 4.0, -7.0])
+    gp = np.zeros((1, 4, 9))
+    for g, p in enumerate(points):
+        gp[0, g, :3] = p
+        gp[0, g, 3:] = f(*p) * scale
+    out = oracle.nodal_stresses(TET10_REFERENCE_NODES, np.arange(10)[None, :], gp)
+    assert np.array_equal(out[:, :3], TET10_REFERENCE_NODES)
+    for k, node in enumerate(TET10_REFERENCE_NODES):
+        assert np.abs(out[k, 3:] - f(*node) * scale).max() < 1e-12, k
+
+
+def test_nodal_stresses_average_over_the_elements_at_a_node():
+    """results.rs:200-220: each node takes the mean over the elements that use it (here two Tet4
+    sharing a face, each with its own constant stress); a node no element uses keeps zeros."""
+    nodes = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 1), (5, 5, 5)], float)
+    conn = np.array([[0, 1, 2, 3], [1, 2, 3, 4]])
+    s0 = np.array([1.0, 2.0, 3.0, 4.0, 5.0, 6.0])
+    s1 = np.array([-3.0, 0.25, 7.0, 1.5, -2.0, 10.0])
+    gp = np.zeros((2, 1, 9))
+    gp[0, 0, 3:], gp[1, 0, 3:] = s0, s1
+    out = oracle.nodal_stresses(nodes, conn, gp)
+    assert np.array_equal(out[:, :3], nodes)
+    assert np.array_equal(out[0, 3:], s0) and np.array_equal(out[4, 3:], s1)
+    for n in (1, 2, 3):
+        assert np.array_equal(out[n, 3:], (s0 + s1) / 2)
+    assert np.array_equal(out[5, 3:], np.zeros(6))
+
+
+@pytest.mark.parametrize("nen", [4, 10])
+def test_nodal_stresses_of_a_linear_field_are_d_eps(nen):
+    """A linear displacement field on a box mesh: every Gauss point carries D eps
+    (element.rs:840-860), so every node, after extrapolation and averaging, carries it too."""
+    nodes, tets = M.box_grid((3, 2, 2), (3.0, 2.0, 2.0))
+    if nen == 10:
+        nodes, tets = M.to_quadratic_tet(nodes, tets)
+    eps = np.array([1e-3, -2e-3, 0.5e-3, 3e-3, -1e-3, 2e-3])
+    grad = np.array([[eps[0], eps[5] / 2, eps[4] / 2],
+                     [eps[5] / 2, eps[1], eps[3] / 2],
+                     [eps[4] / 2, eps[3] / 2, eps[2]]])
+    u = (nodes @ grad.T).ravel()
+    gp = oracle.stresses(nodes, tets, E, NU, u)
+    out = oracle.nodal_stresses(nodes, tets, gp)
+    sig = oracle.solid_d(E, NU) @ eps
+    assert np.abs(out[:, 3:] - sig).max() < 1e-9 * np.abs(sig).max()
