@@ -213,6 +213,55 @@ def run_simp(args, ma, meshes):
           "fd_elements": args.fd, "fd_max_rel_diff": worst, "cpu_baseline": cpu})
 
 
+def run_stresses(args, ma, meshes):
+    """Post-processing of the cube's solution (jittered cube, 6M Tet4 at 100^3): Gauss-point
+    stresses and their average at the nodes, timed with device events; the oracle on one core
+    beside it, over the whole mesh."""
+    p = meshes.jittered_cube(args.cells)
+    m = ma.Model(p)
+    m.factor()
+    u, _, info = m.solve(reactions=False)
+    out = m.nodal_stresses(u)  # the first call builds the node incidence list
+    m.profile(enable=1)
+    reps = 5
+    for _ in range(reps):
+        out = m.nodal_stresses(u)
+    pr = m.profile(enable=0)
+    st_ms = pr["stresses"][0] / pr["stresses"][1]
+    nd_ms = pr["nodal"][0] / pr["nodal"][1]
+    ne, nen = p.conn.shape
+    nn = len(p.nodes)
+    ngp = 1 if nen == 4 else 4
+    # algorithmic bytes, each array once: Gauss-point stresses read connectivity (int32), the
+    # element's coordinates and displacements, E and nu, and write 9 words per Gauss point; the
+    # averaging reads the six stress words of each Gauss point, the incidence list, the node
+    # positions, and writes 9 words per node
+    st_bytes = ne * (4 * nen + 16 + ngp * 72) + nn * (24 + 24)
+    nd_bytes = ne * ngp * 48 + 8 * (nn + 1) + 4 * ne * nen + nn * (24 + 72)
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        gp = oracle.stresses(p.nodes, p.conn, p.young, p.nu, u)
+        ref = oracle.nodal_stresses(p.nodes, p.conn, gp)
+        cpu = {"value": round(time.perf_counter() - t0, 3), "unit": "s", "cores": 1,
+               "kind": "port", "sample": "the oracle's Gauss-point stresses and nodal averaging "
+                                         "over the whole mesh at the device solution"}
+        cpu["bit_exact"] = bool(np.array_equal(out, ref))
+    total = st_ms + nd_ms
+    emit({"metric": "Gauss-point + nodal stresses ms, jittered cube (Tet4)",
+          "value": round(total, 4), "unit": "ms", "n_gpus": 1, "higher_is_better": False,
+          "data": "synthetic", "dtype": "fp64",
+          "config": {"workload": f"jittered_cube_{args.cells}_stresses", "elements": int(ne),
+                     "nodes": int(nn)},
+          "kernels": {"stresses": {"avg_ms": round(st_ms, 4), "bytes": st_bytes,
+                                   "achieved_GBs": round(st_bytes / (st_ms * 1e-3) / 1e9, 1)},
+                      "nodal": {"avg_ms": round(nd_ms, 4), "bytes": nd_bytes,
+                                "achieved_GBs": round(nd_bytes / (nd_ms * 1e-3) / 1e9, 1)}},
+          "solve_iterations": info["iterations"], "cpu_baseline": cpu,
+          "speedup_vs_cpu": round(cpu["value"] * 1e3 / total, 1) if cpu else None})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,7 +276,8 @@ def main():
                     help="end with a plain 256 MiB read + 256 MiB write kernel, the reference "
                          "against which HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled")
     ap.add_argument("--cpu-iters", type=int, default=8)
-    ap.add_argument("--workload", choices=["cube", "beam_cases", "box_batch", "tet10", "simp"],
+    ap.add_argument("--workload", choices=["cube", "beam_cases", "box_batch", "tet10", "simp",
+                                           "stresses"],
                     default="cube", help="cube: the headline 3.1M-DOF solve; the others are the "
                     "batched, Tet10 and adjoint workloads (one JSON line each)")
     ap.add_argument("--cases", type=int, default=256)
@@ -247,7 +297,7 @@ def main():
 
     if args.workload != "cube":
         {"beam_cases": run_beam_cases, "box_batch": run_box_batch, "tet10": run_tet10,
-         "simp": run_simp}[args.workload](args, ma, meshes)
+         "simp": run_simp, "stresses": run_stresses}[args.workload](args, ma, meshes)
         return
 
     t0 = time.time()

@@ -521,11 +521,11 @@ class Model:
     def profile(self, enable=-1):
         """Kernel durations (ms) and launch counts accumulated since the last reset; enable
         1/0 resets and switches the accounting on/off."""
-        ms = np.zeros(8)
-        cnt = np.zeros(8, dtype=np.int64)
+        ms = np.zeros(10)
+        cnt = np.zeros(10, dtype=np.int64)
         _check(lib().ml_model_profile(self._h, enable, _d(ms), _i(cnt)))
         names = ("spmv64", "spmv32", "precond64", "precond32", "ic0", "stiffness",
-                 "assembly", "vector")
+                 "assembly", "vector", "stresses", "nodal")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(names)}
 
     def __del__(self):

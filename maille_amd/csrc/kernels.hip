@@ -474,7 +474,35 @@ __global__ __launch_bounds__(256) void k_nodal_stresses(int64_t n_nodes,
   if (n >= n_nodes) return;
   double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int64_t t0 = inc_off[n], t1 = inc_off[n + 1];
-  for (int64_t t = t0; t < t1; ++t) {
+  // four incidences at a time: their items, then all their Gauss-point values, are loaded
+  // before the sums take them in order (Tet4; one at a time for the four-point Tet10)
+  constexpr int U = NEN == 4 ? 4 : 1;
+  int64_t t = t0;
+  for (; t + U <= t1; t += U) {
+    int it[U];
+    double v[U][NGP][6];
+#pragma unroll
+    for (int u = 0; u < U; ++u) it[u] = inc_item[t + u];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double* p = gp + (int64_t)(it[u] / NEN) * NGP * 9;
+#pragma unroll
+      for (int g = 0; g < NGP; ++g)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v[u][g][c] = p[g * 9 + 3 + c];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = it[u] % NEN;
+#pragma unroll
+      for (int g = 0; g < NGP; ++g) {
+        const double w = extrapolation_weight<NEN>(k, g);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s[c] += w * v[u][g][c];
+      }
+    }
+  }
+  for (; t < t1; ++t) {
     const int it = inc_item[t];
     const int64_t e = it / NEN;
     const int k = it - (int)e * NEN;

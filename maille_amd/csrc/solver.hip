@@ -83,7 +83,9 @@ inline unsigned grid_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 
 constexpr TriCaps kTriCaps = {mlk::TRI_XCAP, mlk::TRI_ICAP, mlk::TRI_UCAP};
 
 // Kernel-duration accounting with HIP events on the handle's stream (enabled on demand).
-enum ProfKind { P_SPMV64, P_SPMV32, P_TRI64, P_TRI32, P_IC0, P_STIFF, P_ASM, P_VEC, P_COUNT };
+enum ProfKind {
+  P_SPMV64, P_SPMV32, P_TRI64, P_TRI32, P_IC0, P_STIFF, P_ASM, P_VEC, P_STRESS, P_NODAL, P_COUNT
+};
 
 struct Profiler {
   bool on = false;
@@ -1152,23 +1154,27 @@ struct Model {
       HIP_CHECK(hipStreamSynchronize(s()));
       have_inc = true;
     }
+    hipEvent_t ev = op.prof.begin(s());
     if (nen == 4)
       mlk::k_nodal_stresses<4><<<grid_for(nn), 256, 0, s()>>>(nn, d_xyz.p, d_inc_off.p,
                                                                 d_inc_item.p, gp_dev, out_dev);
     else
       mlk::k_nodal_stresses<10><<<grid_for(nn), 256, 0, s()>>>(nn, d_xyz.p, d_inc_off.p,
                                                                  d_inc_item.p, gp_dev, out_dev);
+    op.prof.end(P_NODAL, ev, s());
     HIP_CHECK(hipGetLastError());
   }
 
   void stresses(const double* u_dev, double* out_dev) {
     const int64_t ne = plan.n_elements;
+    hipEvent_t ev = op.prof.begin(s());
     if (plan.nen == 4)
       mlk::k_stresses<4><<<grid_for(ne), 256, 0, s()>>>(ne, d_xyz.p, d_conn.p, d_young.p, d_nu.p,
                                                        u_dev, out_dev);
     else
       mlk::k_stresses<10><<<grid_for(4 * ne), 256, 0, s()>>>(ne, d_xyz.p, d_conn.p, d_young.p,
                                                             d_nu.p, u_dev, out_dev);
+    op.prof.end(P_STRESS, ev, s());
     HIP_CHECK(hipGetLastError());
   }
 };
