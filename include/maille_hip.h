@@ -206,6 +206,11 @@ ml_status ml_model_nodal_stresses(ml_model* m, const double* u, double* out);
  * (element, face, value), added to f[3*n_nodes] in the given order. Host. */
 ml_status ml_model_add_pressures(ml_model* m, int64_t count, const int64_t* elements,
                                  const int64_t* faces, const double* values, double* f);
+/* edge_load + FacePressure::add_load (assembly.rs:680-710, 638-644; model.rs:1123-1127) for dead
+ * line loads (element, edge 0-5 as mesh.rs:204-211, q[3] per unit length), added to f[3*n_nodes]
+ * in the given order after the pressures, as Model::assemble does. Host. */
+ml_status ml_model_add_edge_loads(ml_model* m, int64_t count, const int64_t* elements,
+                                  const int64_t* edges, const double* q, double* f);
 /* Sizes: free DOFs, stored lower-triangle entries, DAG depth of the IC(0) factor. */
 int64_t ml_model_nfree(const ml_model* m);
 int64_t ml_model_nnz(const ml_model* m);

@@ -41,7 +41,7 @@ EXPORTS = (
     "ml_pcg_set_wait_limit",
     "ml_model_create", "ml_model_assemble", "ml_model_set_young", "ml_model_factor",
     "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_nodal_stresses",
-    "ml_model_add_pressures",
+    "ml_model_add_pressures", "ml_model_add_edge_loads",
     "ml_model_nfree", "ml_model_nnz", "ml_model_levels", "ml_model_csr",
     "ml_model_compliance_gradient", "ml_model_destroy", "ml_model_profile",
     "ml_model_solve_many", "ml_batch_create", "ml_batch_solve", "ml_batch_destroy",
@@ -148,6 +148,7 @@ def lib():
             "ml_model_stresses": (_int, [_P, _D, _D]),
             "ml_model_nodal_stresses": (_int, [_P, _D, _D]),
             "ml_model_add_pressures": (_int, [_P, _i64, _I, _I, _D, _D]),
+            "ml_model_add_edge_loads": (_int, [_P, _i64, _I, _I, _D, _D]),
             "ml_model_nfree": (_i64, [_P]),
             "ml_model_nnz": (_i64, [_P]),
             "ml_model_levels": (_i64, [_P]),
@@ -424,13 +425,20 @@ class Model:
         self.create_info = si.as_dict()
 
     def loads(self):
-        """Model::assemble's F (model.rs:1089-1121): nodal loads plus dead pressures."""
+        """Model::assemble's F (model.rs:1089-1127): nodal loads, then dead pressures, then dead
+        line loads on edges."""
         f = _f64(self.problem.f).copy()
         p = self.problem.pressures
         if p:
             arr = np.array(p, dtype=np.float64)
             e, fc, v = _ai64(arr[:, 0]), _ai64(arr[:, 1]), _f64(arr[:, 2])
             _check(lib().ml_model_add_pressures(self._h, len(e), _i(e), _i(fc), _d(v), _d(f)))
+        el = getattr(self.problem, "edge_loads", None)
+        if el:
+            e = _ai64([l[0] for l in el])
+            ed = _ai64([l[1] for l in el])
+            q = _f64(np.array([l[2] for l in el], dtype=np.float64).reshape(-1, 3))
+            _check(lib().ml_model_add_edge_loads(self._h, len(e), _i(e), _i(ed), _d(q), _d(f)))
         return f
 
     def solve(self, f=None, reactions=True):

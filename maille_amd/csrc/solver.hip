@@ -1181,6 +1181,42 @@ struct Model {
 
 }  // namespace ml
 
+// edge_load + FacePressure::add_load (assembly.rs:680-710, 638-644) for one tetrahedron edge on
+// the host: f_a = q int N_a ds over the straight edge between its two vertices (TET10_EDGES,
+// mid-node 4 + edge on a Tet10; mesh.rs:204-211), LINE_2 with line2 or LINE_3 with line3
+// (quadrature.rs:62-90, shape.rs:115-125), terms q_c * w * length * N_a added point after point.
+void edge_load_add(int nen, const int64_t* enodes, const double* xyz, int edge, const double* q,
+                   double* f) {
+  static const int EDGES[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+  static const double S2[2] = {0.2113248654051871, 0.7886751345948129}, W2[2] = {0.5, 0.5};
+  static const double S3[3] = {0.1127016653792583, 0.5, 0.8872983346207417};
+  static const double W3[3] = {5.0 / 18.0, 4.0 / 9.0, 5.0 / 18.0};
+  const int pos[3] = {EDGES[edge][0], EDGES[edge][1], 4 + edge};
+  const int m = nen == 4 ? 2 : 3;
+  const double* c0 = xyz + 3 * enodes[pos[0]];
+  const double* c1 = xyz + 3 * enodes[pos[1]];
+  double len2 = 0.0;
+  for (int c = 0; c < 3; ++c) len2 += (c1[c] - c0[c]) * (c1[c] - c0[c]);
+  const double length = std::sqrt(len2);
+  double forces[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int g = 0; g < m; ++g) {
+    const double sv = m == 2 ? S2[g] : S3[g], w = m == 2 ? W2[g] : W3[g];
+    double na[3];
+    if (m == 2) {
+      na[0] = 1.0 - sv;
+      na[1] = sv;
+    } else {
+      na[0] = (1.0 - sv) * (1.0 - 2.0 * sv);
+      na[1] = sv * (2.0 * sv - 1.0);
+      na[2] = 4.0 * sv * (1.0 - sv);
+    }
+    for (int a = 0; a < m; ++a)
+      for (int c = 0; c < 3; ++c) forces[3 * a + c] += q[c] * w * length * na[a];
+  }
+  for (int a = 0; a < m; ++a)
+    for (int c = 0; c < 3; ++c) f[3 * enodes[pos[a]] + c] += forces[3 * a + c];
+}
+
 // face_rule + face_pressure + FacePressure::add_load (assembly.rs:527-593, 638-674) for one
 // tetrahedron face on the host: f_a = -p sum_g w_g 2A N_a n, n outward (centroid test),
 // TRIANGLE_3 on a Tet4 face, TRIANGLE_6 on a Tet10 face, added to f in face-node order.
@@ -1801,6 +1837,24 @@ ml_status ml_model_add_pressures(ml_model* h, int64_t count, const int64_t* elem
       face_pressure_add(m.plan.nen, m.conn.data() + elements[k] * m.plan.nen, m.nodes.data(),
                         (int)faces[k], values[k], f);
     }
+    return ML_OK;
+  });
+}
+
+ml_status ml_model_add_edge_loads(ml_model* h, int64_t count, const int64_t* elements,
+                                  const int64_t* edges, const double* q, double* f) {
+  return guarded([&]() -> int {
+    if (!h || (count > 0 && (!elements || !edges || !q)) || !f) throw ArgError("null argument");
+    Model& m = h->m;
+    for (int64_t k = 0; k < count; ++k) {
+      if (elements[k] < 0 || elements[k] >= m.plan.n_elements || edges[k] < 0 || edges[k] > 5)
+        throw ArgError("line load on an edge that does not exist");
+      for (int c = 0; c < 3; ++c)
+        if (!std::isfinite(q[3 * k + c])) throw ArgError("line load not finite");
+    }
+    for (int64_t k = 0; k < count; ++k)
+      edge_load_add(m.plan.nen, m.conn.data() + elements[k] * m.plan.nen, m.nodes.data(),
+                    (int)edges[k], q + 3 * k, f);
     return ML_OK;
   });
 }

@@ -28,7 +28,8 @@ TRACTION_FACES = (
 class Problem:
     """A static linear-elastic problem in the reference's terms (model.rs:230-247): nodes,
     Tet4/Tet10 connectivity with a material per element, fixed DOFs with imposed values,
-    nodal loads already summed into ``f`` and dead face pressures (element, face, value)."""
+    nodal loads already summed into ``f``, dead face pressures (element, face, value) and dead
+    line loads on edges (element, edge, (qx, qy, qz))."""
 
     name: str
     nodes: np.ndarray          # (N, 3) float64
@@ -40,6 +41,7 @@ class Problem:
     f: np.ndarray              # (3N,) float64, nodal loads
     pressures: list = field(default_factory=list)  # [(element, face, value)]
     meta: dict = field(default_factory=dict)
+    edge_loads: list = field(default_factory=list)  # [(element, edge, (qx, qy, qz))]
 
     @property
     def nen(self) -> int:

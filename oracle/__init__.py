@@ -94,6 +94,7 @@ def _declare(L):
         "or_stresses": (None, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _D, _D]),
         "or_compliance_gradient": (None, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _D, _D]),
         "or_nodal_stresses": (None, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D]),
+        "or_add_edge_loads": (None, [C.c_int64, _D, C.c_int, _I, C.c_int64, _I, _I, _D, _D]),
         "or_add_pressures": (None, [C.c_int64, _D, C.c_int, _I, C.c_int64, _I, _I, _D, _D]),
     }
     for name, (res, args) in sig.items():
@@ -440,6 +441,19 @@ def add_pressures(nodes, conn, pressures, f):
     return f
 
 
+def add_edge_loads(nodes, conn, loads, f):
+    """model.rs:1123-1127 — dead line loads (element, edge, (qx, qy, qz)) added to f in order."""
+    if len(loads) == 0:
+        return f
+    nodes, conn = _f64(nodes), _i64(conn)
+    le = _i64([l[0] for l in loads])
+    ld = _i64([l[1] for l in loads])
+    lq = _f64([l[2] for l in loads]).reshape(-1, 3)
+    lib().or_add_edge_loads(len(nodes), _d(nodes), conn.shape[1], _i(conn), len(loads), _i(le),
+                            _i(ld), _d(lq), _d(f))
+    return f
+
+
 def reactions(nodes, conn, young, nu, fixed, f, u):
     """model.rs:586-593 — R = K u - F on the fixed DOFs."""
     nodes, conn = _f64(nodes), _i64(conn)
@@ -493,10 +507,11 @@ def compliance_gradient(nodes, conn, young, nu, u):
 
 def model_solve(model, iterative=True, tol=TOLERANCE, maxit=MAX_ITERATIONS):
     """model.rs:548-610 — the whole static solve on a ``maille_amd.meshes.Problem``-like object
-    (nodes, conn, young, nu, fixed, imposed, f with nodal loads, pressures). Returns a dict
+    (nodes, conn, young, nu, fixed, imposed, f with nodal loads, pressures, edge loads). Returns a dict
     with displacements, reactions, stresses, iterations, residual, status."""
     f = model.f.copy()
     add_pressures(model.nodes, model.conn, model.pressures, f)
+    add_edge_loads(model.nodes, model.conn, getattr(model, "edge_loads", []), f)
     sys_ = System(model.nodes, model.conn, model.young, model.nu, model.fixed,
                   model.imposed, iterative=iterative)
     st = sys_.factorize()

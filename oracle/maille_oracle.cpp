@@ -1053,4 +1053,49 @@ void or_add_pressures(i64 n_nodes, const double* xyz, int nen, const i64* conn, 
   }
 }
 
+// assembly.rs:680-710 (edge_load) with FacePressure::add_load (:638-644), model.rs:1123-1127:
+// the dead line load q (force per unit length) on edge `edge` of a tetrahedron, f_a = q int N_a ds:
+// the edge's nodes are TET10_EDGES[edge] (and mid-node 4 + edge on a Tet10, mesh.rs:204-211),
+// its length that of the straight segment between its two vertices, LINE_2 with line2 (two
+// nodes) or LINE_3 with line3 (three nodes, quadrature.rs:62-90, shape.rs:115-125); each term
+// q_c * w * length * N_a is added rule point after rule point, then into f node after node.
+void or_add_edge_loads(i64 n_nodes, const double* xyz, int nen, const i64* conn, i64 nl,
+                       const i64* l_el, const i64* l_edge, const double* q, double* f) {
+  (void)n_nodes;
+  static const double LINE_2_S[2] = {0.2113248654051871, 0.7886751345948129};
+  static const double LINE_2_W[2] = {0.5, 0.5};
+  static const double LINE_3_S[3] = {0.1127016653792583, 0.5, 0.8872983346207417};
+  static const double LINE_3_W[3] = {5.0 / 18.0, 4.0 / 9.0, 5.0 / 18.0};
+  for (i64 k = 0; k < nl; ++k) {
+    const i64 e = l_el[k];
+    const int edge = (int)l_edge[k];
+    int pos[3] = {TET10_EDGES[edge][0], TET10_EDGES[edge][1], 4 + edge};
+    const int m = nen == 4 ? 2 : 3;
+    const double* c0 = xyz + 3 * conn[e * nen + pos[0]];
+    const double* c1 = xyz + 3 * conn[e * nen + pos[1]];
+    double len2 = 0.0;
+    for (int c = 0; c < 3; ++c) len2 += (c1[c] - c0[c]) * (c1[c] - c0[c]);
+    const double length = std::sqrt(len2);
+    double forces[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int np = m == 2 ? 2 : 3;
+    for (int g = 0; g < np; ++g) {
+      const double sv = m == 2 ? LINE_2_S[g] : LINE_3_S[g];
+      const double w = m == 2 ? LINE_2_W[g] : LINE_3_W[g];
+      double na[3];
+      if (m == 2) {
+        na[0] = 1.0 - sv;
+        na[1] = sv;
+      } else {
+        na[0] = (1.0 - sv) * (1.0 - 2.0 * sv);
+        na[1] = sv * (2.0 * sv - 1.0);
+        na[2] = 4.0 * sv * (1.0 - sv);
+      }
+      for (int a = 0; a < m; ++a)
+        for (int c = 0; c < 3; ++c) forces[3 * a + c] += q[3 * k + c] * w * length * na[a];
+    }
+    for (int a = 0; a < m; ++a)
+      for (int c = 0; c < 3; ++c) f[3 * conn[e * nen + pos[a]] + c] += forces[3 * a + c];
+  }
+}
+
 }  // extern "C"

@@ -215,6 +215,32 @@ def test_pressure_loads_match_oracle(gpu):
     assert np.array_equal(f, g)
 
 
+@pytest.mark.parametrize("name", ["jittered_cube_8", "cantilever_tet10_small"])
+def test_edge_loads_match_oracle(gpu, name):
+    """edge_load (assembly.rs:680-710) on the edges of the x = max end, added after the
+    pressures as Model::assemble does (model.rs:1113-1127): the library's F equals the oracle's
+    bit for bit, and the FP64 solve under it stays within 1e-10 of the oracle's PCG."""
+    import dataclasses
+    base = PROBLEMS[name]
+    xmax = base.nodes[:, 0].max()
+    loads = []
+    for e, row in enumerate(base.conn):
+        for k, (a, b) in enumerate([(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]):
+            if abs(base.nodes[row[a], 0] - xmax) < 1e-9 and abs(base.nodes[row[b], 0] - xmax) < 1e-9:
+                loads.append((e, k, (0.5, -1.25, 2.0)))
+    assert loads
+    p = dataclasses.replace(base, edge_loads=loads)
+    print(f"edge loads {name}: {len(loads)}", flush=True)
+    m = gpu.Model(p)
+    f = m.loads()
+    g = oracle.add_edge_loads(p.nodes, p.conn, loads,
+                              oracle.add_pressures(p.nodes, p.conn, p.pressures, p.f.copy()))
+    assert np.array_equal(f, g)
+    u, _, _ = m.solve()
+    ref = oracle.model_solve(p, iterative=True)
+    assert rel_l2(u, ref["displacements"]) <= 1e-10
+
+
 def test_stalled_dataflow_waits_end_cleanly(gpu):
     """A dataflow launch whose waits expire (forced from the host with a one-tick bound, so
     every wave still ends normally) is repeated on smaller grids and then reported as a

@@ -678,3 +678,62 @@ def test_nodal_stresses_of_a_linear_field_are_d_eps(nen):
     out = oracle.nodal_stresses(nodes, tets, gp)
     sig = oracle.solid_d(E, NU) @ eps
     assert np.abs(out[:, 3:] - sig).max() < 1e-9 * np.abs(sig).max()
+
+
+# ------------------------------------------------------------------ edge loads
+TET_EDGES = [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]
+
+
+def _edges_where(nodes, conn, on):
+    """tests/edge_load.rs:22-39: the (element, edge) pairs all of whose nodes satisfy `on`, once
+    per pair of vertices, in element order."""
+    seen, found = set(), []
+    nen = conn.shape[1]
+    for i, e in enumerate(conn):
+        for k, (a, b) in enumerate(TET_EDGES):
+            ids = [e[a], e[b]] + ([e[4 + k]] if nen == 10 else [])
+            key = (min(e[a], e[b]), max(e[a], e[b]))
+            if all(on(nodes[v]) for v in ids) and key not in seen:
+                seen.add(key)
+                found.append((i, k))
+    return found
+
+
+@pytest.mark.parametrize("nen", [4, 10])
+def test_the_resultant_on_a_tetrahedron_edge_splits_as_the_shape_functions(nen):
+    """tests/edge_load.rs:115-138 (Tet4: q L / 2 on each end) and :330-360 (Tet10: q L times 1/6,
+    1/6, 2/3 on the two ends and the mid-node), every edge of the first element of a 1x1x1 box
+    of 1 x 2 x 2."""
+    nodes, tets = M.box_grid((1, 1, 1), (1.0, 2.0, 2.0))
+    if nen == 10:
+        nodes, tets = M.to_quadratic_tet(nodes, tets)
+    q = np.array([1.0, -2.0, 3.0])
+    share = [0.5, 0.5] if nen == 4 else [1 / 6, 1 / 6, 2 / 3]
+    for k, (a, b) in enumerate(TET_EDGES):
+        f = oracle.add_edge_loads(nodes, tets, [(0, k, tuple(q))], np.zeros(3 * len(nodes)))
+        ids = [tets[0][a], tets[0][b]] + ([tets[0][4 + k]] if nen == 10 else [])
+        length = np.sqrt(((nodes[ids[1]] - nodes[ids[0]]) ** 2).sum())
+        for v, w in zip(ids, share):
+            assert np.allclose(f[3 * v:3 * v + 3], q * length * w, rtol=1e-12, atol=0), k
+        assert np.allclose(f.reshape(-1, 3).sum(axis=0), q * length, rtol=1e-12, atol=0)
+        others = np.setdiff1d(np.arange(len(nodes)), ids)
+        assert not f.reshape(-1, 3)[others].any()
+
+
+@pytest.mark.parametrize("quadratic", [False, True])
+def test_the_edge_load_resultant_is_q_times_the_rim_length(quadratic):
+    """tests/edge_load.rs:291-328 (small strains): q along x on the four edges of the face x = 1
+    of a unit cube on rollers; the x reactions sum to -4 q."""
+    def fixed(p, c):
+        return 0.0 if abs(p[c]) < 1e-9 else None
+
+    prob = _box_problem((2, 2, 2), (1.0, 1.0, 1.0), quadratic, fixed)
+    end = lambda x: abs(x[0] - 1.0) < 1e-9
+    rim = []
+    for side in (lambda x: abs(x[1]) < 1e-9, lambda x: abs(x[1] - 1.0) < 1e-9,
+                 lambda x: abs(x[2]) < 1e-9, lambda x: abs(x[2] - 1.0) < 1e-9):
+        rim += _edges_where(prob.nodes, prob.conn, lambda x, s=side: end(x) and s(x))
+    q = 25.0
+    prob.edge_loads = [(e, k, (q, 0.0, 0.0)) for e, k in rim]
+    r = oracle.model_solve(prob, iterative=False)
+    assert abs(r["reactions"][0::3].sum() + 4.0 * q) < 1e-9 * (1.0 + 4.0 * q)
