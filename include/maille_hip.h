@@ -35,7 +35,8 @@ typedef enum ml_status {
   ML_OUT_OF_PATTERN = 4,      /* Csr::add outside the profile (iterative.rs:126 panics) */
   ML_HIP_ERROR = 5,           /* a HIP runtime call failed; see ml_last_error() */
   ML_INVALID_MESH = 6,        /* Mesh::new refusals (mesh.rs:257-308), see ml_solve_info */
-  ML_DEVICE_TIMEOUT = 7       /* a dataflow kernel's bounded wait expired (never expected) */
+  ML_DEVICE_TIMEOUT = 7,      /* a dataflow kernel's bounded wait expired (never expected) */
+  ML_INVALID_FILE = 8         /* an input file refused by ml_msh_read (IoError, error.rs:9-65) */
 } ml_status;
 
 /* Arithmetic of the solve. ML_FP64 is the drop-in (the reference computes in f64).
@@ -266,6 +267,54 @@ ml_status ml_model_autotune(ml_model* m, int64_t nrhs, int32_t* chosen);
  * 8 Gauss-point stresses, 9 nodal stress averaging) into ms[10] and counts[10] (either may be
  * null), then resets and enables (1) or disables (0) accounting; enable < 0 only reads. */
 ml_status ml_model_profile(ml_model* m, int enable, double* ms, int64_t* counts);
+
+/* ------------------------------------------------------------------ Gmsh .msh input */
+/* The mesh half of maille-io's input (crates/maille-io/src/msh): a Gmsh 4.1 ASCII file read into
+ * nodes, cells in Maille's numbering (Tet10 mid-nodes 8 and 9 swapped, numbering.rs:11-15) with
+ * positive orientation (numbering.rs:20-46), and the physical groups as named sets. Host only;
+ * no device is needed. */
+typedef struct ml_msh ml_msh;
+
+/* Refusals of ml_msh_read, one per reader variant of IoError (error.rs:9-65). */
+typedef enum ml_msh_error {
+  ML_MSH_SYNTAX = 1,                   /* IoError::Syntax: malformed line */
+  ML_MSH_MISSING_SECTION = 2,          /* IoError::MissingSection */
+  ML_MSH_UNSUPPORTED_VERSION = 3,      /* IoError::UnsupportedVersion: not 4.1 */
+  ML_MSH_BINARY = 4,                   /* IoError::BinaryFormat */
+  ML_MSH_UNSUPPORTED_ELEMENT = 5,      /* IoError::UnsupportedElement: a Gmsh type not read */
+  ML_MSH_NO_CELLS = 6,                 /* IoError::NoCells: no triangle and no tetrahedron */
+  ML_MSH_MIXED_CELL_KINDS = 7,         /* IoError::MixedCellKinds: two element orders */
+  ML_MSH_NON_PLANAR = 8,               /* IoError::NonPlanar: triangles off z = 0 */
+  ML_MSH_DUPLICATE_SET_NAME = 9,       /* IoError::DuplicateSetName */
+  ML_MSH_DUPLICATE_NODE_TAG = 10,      /* IoError::DuplicateNodeTag */
+  ML_MSH_UNKNOWN_NODE = 11,            /* IoError::UnknownNode */
+  ML_MSH_BOUNDARY_FACE_NOT_FOUND = 12, /* IoError::BoundaryFaceNotFound */
+  ML_MSH_BOUNDARY_EDGE_NOT_FOUND = 13  /* IoError::BoundaryEdgeNotFound */
+} ml_msh_error;
+
+/* msh::read (reader.rs:23-39) of text[length] (no terminating NUL needed). On a refused file
+ * returns ML_INVALID_FILE with *error_kind (ml_msh_error) and *error_line (1-based, 0 when the
+ * refusal has no line) set and the message in ml_last_error(); error_kind and error_line may be
+ * null. */
+ml_status ml_msh_read(const char* text, int64_t length, ml_msh** out, int32_t* error_kind,
+                      int64_t* error_line);
+/* sizes[5]: nodes, nodes per cell (3 Tri3, 6 Tri6, 4 Tet4, 10 Tet10), cells, sets, dimension. */
+ml_status ml_msh_sizes(const ml_msh* m, int64_t* sizes);
+/* MshFile::nodes and cells (mod.rs:111-120): xyz[3*nodes], cells[cells*nodes_per_cell]; either
+ * pointer may be null. */
+ml_status ml_msh_geometry(const ml_msh* m, double* xyz, int64_t* cells);
+/* Set k of MshFile::sets (mod.rs:97-109), sets in byte order of their names: counts[5] = nodes,
+ * faces, edges, elements, name length in bytes; the name (UTF-8, as written between the quotes)
+ * is copied with a terminating NUL into name[capacity] when name is not null, and
+ * ML_INVALID_ARGUMENT is returned when it does not fit. */
+ml_status ml_msh_set_info(const ml_msh* m, int64_t k, char* name, int64_t capacity,
+                          int64_t* counts);
+/* The sorted lists of set k: nodes[], faces[2*count] and edges[2*count] as (element, local
+ * index) pairs (faces as Element::face_nodes, edges as Element::edge_nodes), elements[]; any
+ * pointer may be null. */
+ml_status ml_msh_set_items(const ml_msh* m, int64_t k, int64_t* nodes, int64_t* faces,
+                           int64_t* edges, int64_t* elements);
+void ml_msh_destroy(ml_msh* m);
 
 #ifdef __cplusplus
 }

@@ -25,7 +25,7 @@ _LIB_PATH = os.path.join(_HERE, "libmaille_hip.so")
 _lib = None
 
 ML_OK, ML_SINGULAR, ML_NOT_CONVERGED, ML_INVALID_ARGUMENT, ML_OUT_OF_PATTERN, ML_HIP_ERROR, \
-    ML_INVALID_MESH, ML_DEVICE_TIMEOUT = range(8)
+    ML_INVALID_MESH, ML_DEVICE_TIMEOUT, ML_INVALID_FILE = range(9)
 FP64, MIXED, MIXED_IC0 = 0, 1, 2
 AUTO, DIRECT, ITERATIVE = 0, 1, 2
 TOLERANCE = 1e-12          # iterative.rs:9
@@ -46,6 +46,8 @@ EXPORTS = (
     "ml_model_compliance_gradient", "ml_model_destroy", "ml_model_profile",
     "ml_model_solve_many", "ml_batch_create", "ml_batch_solve", "ml_batch_destroy",
     "ml_pcg_autotune", "ml_model_autotune",
+    "ml_msh_read", "ml_msh_sizes", "ml_msh_geometry", "ml_msh_set_info", "ml_msh_set_items",
+    "ml_msh_destroy",
 )
 
 
@@ -162,6 +164,12 @@ def lib():
             "ml_batch_destroy": (None, [_P]),
             "ml_pcg_autotune": (_int, [_P, _i64, _I32]),
             "ml_model_autotune": (_int, [_P, _i64, _I32]),
+            "ml_msh_read": (_int, [_P, _i64, _PP, _I32, _I]),
+            "ml_msh_sizes": (_int, [_P, _I]),
+            "ml_msh_geometry": (_int, [_P, _D, _I]),
+            "ml_msh_set_info": (_int, [_P, _i64, C.c_char_p, _i64, _I]),
+            "ml_msh_set_items": (_int, [_P, _i64, _I, _I, _I, _I]),
+            "ml_msh_destroy": (None, [_P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

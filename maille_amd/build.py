@@ -13,9 +13,9 @@ LIB = os.path.join(HERE, "libmaille_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HOST_SOURCES = ["pattern.cpp", "plan.cpp"]
+HOST_SOURCES = ["msh.cpp", "pattern.cpp", "plan.cpp"]
 DEVICE_SOURCES = ["solver.hip"]
-HEADERS = ["kernels.hpp", "kernels.hip", "pattern.hpp", "plan.hpp"]
+HEADERS = ["kernels.hpp", "kernels.hip", "msh.hpp", "pattern.hpp", "plan.hpp"]
 
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
             "-Wno-unused-parameter"]

@@ -18,6 +18,7 @@
 
 #include "../../include/maille_hip.h"
 #include "kernels.hip"
+#include "msh.hpp"
 #include "pattern.hpp"
 #include "plan.hpp"
 
@@ -1299,6 +1300,9 @@ struct ml_pcg {
 struct ml_model {
   Model m;
 };
+struct ml_msh {
+  msh::File f;
+};
 // Several models solved together: their K_ff concatenated block-diagonally into one operator
 // (one launch per kernel for all of them), each model a system of the batched PCG.
 struct ml_batch {
@@ -2014,7 +2018,8 @@ void ml_model_destroy(ml_model* h) { delete h; }
 
 // Profiling hooks for the benchmark: accumulate kernel durations measured with HIP events on
 // the handle's stream. kinds: 0 spmv64, 1 spmv32, 2 precondition64 (fwd+bwd), 3
-// precondition32, 4 ic0, 5 stiffness, 6 assembly, 7 vector updates.
+// precondition32, 4 ic0, 5 stiffness, 6 assembly, 7 vector updates, 8 Gauss-point stresses,
+// 9 nodal averaging.
 ml_status ml_model_profile(ml_model* h, int enable, double* ms, int64_t* counts) {
   return guarded([&]() -> int {
     if (!h) throw ArgError("null handle");
@@ -2031,5 +2036,107 @@ ml_status ml_model_profile(ml_model* h, int enable, double* ms, int64_t* counts)
     return ML_OK;
   });
 }
+
+// ------------------------------------------------------------------ Gmsh .msh input
+ml_status ml_msh_read(const char* text, int64_t length, ml_msh** out, int32_t* error_kind,
+                      int64_t* error_line) {
+  if (error_kind) *error_kind = 0;
+  if (error_line) *error_line = 0;
+  return guarded([&]() -> int {
+    if (!out || length < 0 || (length > 0 && !text)) throw ArgError("bad msh arguments");
+    *out = nullptr;
+    try {
+      auto* h = new ml_msh;
+      try {
+        h->f = msh::read(text, (size_t)length);
+      } catch (...) {
+        delete h;
+        throw;
+      }
+      *out = h;
+    } catch (const msh::Error& e) {
+      if (error_kind) *error_kind = e.kind;
+      if (error_line) *error_line = e.line;
+      g_last_error = e.line > 0 ? "line " + std::to_string(e.line) + ": " + e.what() : e.what();
+      return ML_INVALID_FILE;
+    }
+    return ML_OK;
+  });
+}
+
+ml_status ml_msh_sizes(const ml_msh* m, int64_t* sizes) {
+  if (!m || !sizes) return ML_INVALID_ARGUMENT;
+  sizes[0] = m->f.n_nodes();
+  sizes[1] = m->f.nen;
+  sizes[2] = m->f.n_cells();
+  sizes[3] = (int64_t)m->f.sets.size();
+  sizes[4] = m->f.dim;
+  return ML_OK;
+}
+
+ml_status ml_msh_geometry(const ml_msh* m, double* xyz, int64_t* cells) {
+  if (!m) return ML_INVALID_ARGUMENT;
+  if (xyz) std::copy(m->f.xyz.begin(), m->f.xyz.end(), xyz);
+  if (cells) std::copy(m->f.cells.begin(), m->f.cells.end(), cells);
+  return ML_OK;
+}
+
+static const msh::Set* msh_set(const ml_msh* m, int64_t k, const std::string** name) {
+  if (!m || k < 0 || k >= (int64_t)m->f.sets.size()) return nullptr;
+  auto it = m->f.sets.begin();
+  std::advance(it, k);
+  if (name) *name = &it->first;
+  return &it->second;
+}
+
+ml_status ml_msh_set_info(const ml_msh* m, int64_t k, char* name, int64_t capacity,
+                          int64_t* counts) {
+  const std::string* nm = nullptr;
+  const msh::Set* s = msh_set(m, k, &nm);
+  if (!s) {
+    g_last_error = "no such set";
+    return ML_INVALID_ARGUMENT;
+  }
+  if (counts) {
+    counts[0] = (int64_t)s->nodes.size();
+    counts[1] = (int64_t)s->faces.size();
+    counts[2] = (int64_t)s->edges.size();
+    counts[3] = (int64_t)s->elements.size();
+    counts[4] = (int64_t)nm->size();
+  }
+  if (name) {
+    if (capacity < (int64_t)nm->size() + 1) {
+      g_last_error = "set name buffer too small";
+      return ML_INVALID_ARGUMENT;
+    }
+    std::memcpy(name, nm->data(), nm->size());
+    name[nm->size()] = 0;
+  }
+  return ML_OK;
+}
+
+ml_status ml_msh_set_items(const ml_msh* m, int64_t k, int64_t* nodes, int64_t* faces,
+                           int64_t* edges, int64_t* elements) {
+  const msh::Set* s = msh_set(m, k, nullptr);
+  if (!s) {
+    g_last_error = "no such set";
+    return ML_INVALID_ARGUMENT;
+  }
+  if (nodes) std::copy(s->nodes.begin(), s->nodes.end(), nodes);
+  if (elements) std::copy(s->elements.begin(), s->elements.end(), elements);
+  if (faces)
+    for (size_t i = 0; i < s->faces.size(); ++i) {
+      faces[2 * i] = s->faces[i].first;
+      faces[2 * i + 1] = s->faces[i].second;
+    }
+  if (edges)
+    for (size_t i = 0; i < s->edges.size(); ++i) {
+      edges[2 * i] = s->edges[i].first;
+      edges[2 * i + 1] = s->edges[i].second;
+    }
+  return ML_OK;
+}
+
+void ml_msh_destroy(ml_msh* m) { delete m; }
 
 }  // extern "C"

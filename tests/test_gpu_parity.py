@@ -241,6 +241,89 @@ def test_edge_loads_match_oracle(gpu, name):
     assert rel_l2(u, ref["displacements"]) <= 1e-10
 
 
+BEAM_TOML = """
+units = "N, mm, MPa"
+mesh = "beam.msh"
+[materials.alu]
+young = 70e3
+poisson = 0.33
+[[sections]]
+set = "solide"
+material = "alu"
+[[bcs]]
+set = "encastrement"
+dof = "ux"
+value = 0.0
+[[bcs]]
+set = "encastrement"
+dof = "uy"
+value = 0.0
+[[bcs]]
+set = "encastrement"
+dof = "uz"
+value = 0.0
+[[pressures]]
+set = "bout"
+value = 2.5
+[[edge_loads]]
+set = "arete"
+value = [0.0, -4.0, 1.0]
+[[loads]]
+set = "coin"
+dof = "uz"
+value = -50.0
+"""
+
+
+def test_model_files_solve_on_the_device(gpu, tmp_path):
+    """A model file and its .msh read and built by the library (maille_amd.model_file,
+    build.rs:27-235) and solved on the device: the reference's example cube in traction gives
+    the analytic field (tests/model.rs:194-228), and a Tet10 beam written Gmsh's way (half the
+    cells reversed, shuffled tags, three volume blocks) with a clamped face, an end pressure,
+    a line load on an end edge and a point load matches the oracle's solve of the same Problem
+    within 1e-10."""
+    import os
+    import shutil
+
+    import msh_cases
+    from maille_amd import model_file as mf
+    from test_model_file import CUBE
+
+    golden = os.path.join(os.path.dirname(__file__), "golden", "cube.msh")
+    shutil.copy(golden, tmp_path / "cube.msh")
+    (tmp_path / "cube.toml").write_text(CUBE)
+    print("model file: cube.toml", flush=True)
+    p, u, r, info = mf.solve_file(str(tmp_path / "cube.toml"))
+    u = u.reshape(-1, 3)
+    x = p.nodes
+    assert np.abs(u[:, 0] - 0.01 * x[:, 0]).max() < 1e-12
+    assert np.abs(u[:, 1] + 0.25 * 0.01 * x[:, 1]).max() < 1e-12
+    assert np.abs(u[:, 2] + 0.25 * 0.01 * x[:, 2]).max() < 1e-12
+
+    beam = M.cantilever_tet10((12, 3, 3), (60.0, 15.0, 15.0))
+    nodes, cells = beam.nodes, beam.conn
+    face = lambda pred: [tuple(int(v) for v in cells[e][list(M.TET_FACES[f])])
+                         for e, f in zip(*M.faces_on(nodes, cells, pred))]
+    edge = set()
+    for c in cells:
+        for a, b in M.TET10_EDGES:
+            pa, pb = nodes[c[a]], nodes[c[b]]
+            if max(pa[0], pb[0]) == 60.0 and min(pa[0], pb[0]) == 60.0 and pa[1] == pb[1] == 0.0:
+                edge.add(tuple(sorted((int(c[a]), int(c[b])))))
+    corner = [int(np.argmax((nodes == [60.0, 15.0, 15.0]).all(axis=1)))]
+    groups = [("solide", 3, None), ("encastrement", 2, face(lambda q: q[:, 0] == 0.0)),
+              ("bout", 2, face(lambda q: q[:, 0] == 60.0)), ("arete", 1, sorted(edge)),
+              ("coin", 0, corner)]
+    (tmp_path / "beam.msh").write_text(msh_cases.write_msh(nodes, cells, groups, seed=2))
+    (tmp_path / "beam.toml").write_text(BEAM_TOML)
+    print("model file: beam.toml", flush=True)
+    p = mf.load(str(tmp_path / "beam.toml"))
+    assert np.array_equal(p.conn, cells) and len(p.pressures) and len(p.edge_loads) == 3
+    u, _, info = gpu.Model(p).solve()
+    ref = oracle.model_solve(p, iterative=True)
+    assert rel_l2(u, ref["displacements"]) <= 1e-10
+
+
 def test_stalled_dataflow_waits_end_cleanly(gpu):
     """A dataflow launch whose waits expire (forced from the host with a one-tick bound, so
     every wave still ends normally) is repeated on smaller grids and then reported as a
