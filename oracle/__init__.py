@@ -18,6 +18,8 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 OK, SINGULAR, NOT_CONVERGED, INVALID, OUT_OF_PATTERN = range(5)
+NEWTON_NOT_CONVERGED, PLASTIC_COLLAPSE = 9, 10  # MailleError::{NotConverged, PlasticCollapse}
+NEWTON_ITERATIONS = 25  # model.rs:1222
 TOLERANCE = 1e-12  # iterative.rs:9
 MAX_ITERATIONS = 10_000  # iterative.rs:11
 SHIFTS = (0.0, 1e-3, 1e-2, 1e-1, 1.0, 10.0)  # iterative.rs:14
@@ -96,6 +98,13 @@ def _declare(L):
         "or_nodal_stresses": (None, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D]),
         "or_add_edge_loads": (None, [C.c_int64, _D, C.c_int, _I, C.c_int64, _I, _I, _D, _D]),
         "or_add_pressures": (None, [C.c_int64, _D, C.c_int, _I, C.c_int64, _I, _I, _D, _D]),
+        "or_von_mises": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, _D, _D, _D,
+                                   _D, _D]),
+        "or_solid_response": (C.c_int, [C.c_int, _D, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, _D, _D, _D, _D, _D, _D]),
+        "or_quasi_static": (C.c_int, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _D, _D, _U8,
+                                      _D, _D, C.c_double, C.c_int64, C.c_int64, _D, _D, C.c_int,
+                                      _D, _D, _D, _D, _D, _I, _I, _I, _D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -527,3 +536,66 @@ def model_solve(model, iterative=True, tol=TOLERANCE, maxit=MAX_ITERATIONS):
                                      model.fixed, f, u)
         out["stresses"] = stresses(model.nodes, model.conn, model.young, model.nu, u)
     return out
+
+
+# ------------------------------------------------------------------ quasi-static (plasticity)
+def von_mises(young, nu, yield_stress, hardening, strain, prev=None):
+    """plasticity.rs:45-112 — one Gauss point: (stress[6], state[7] = plastic strain + p,
+    tangent 6x6, yielding)."""
+    strain = _f64(strain)
+    prev = np.zeros(7) if prev is None else _f64(prev)
+    stress, state, tan = np.zeros(6), np.zeros(7), np.zeros((6, 6))
+    y = lib().or_von_mises(young, nu, yield_stress, hardening, _d(strain), _d(prev), _d(stress),
+                           _d(state), _d(tan))
+    return stress, state, tan, bool(y)
+
+
+def solid_response(x, young, nu, yield_stress, hardening, ue, prev=None):
+    """element.rs:395-452 — one tetrahedron: (fint[m], tangent m x m with tangent[j][i] at [j, i],
+    Gauss-point stresses (ngp, 9), end states (ngp, 7), yielding)."""
+    x = _f64(x)
+    nen = len(x)
+    m, ngp = 3 * nen, (1 if nen == 4 else 4)
+    prev = np.zeros((ngp, 7)) if prev is None else _f64(prev)
+    fint, tan = np.zeros(m), np.zeros((m, m))
+    out, states = np.zeros((ngp, 9)), np.zeros((ngp, 7))
+    y = lib().or_solid_response(nen, _d(x), young, nu, yield_stress, hardening, _d(_f64(ue)),
+                                _d(prev), _d(fint), _d(tan), _d(out), _d(states))
+    return fint, tan, out, states, bool(y)
+
+
+def quasi_static(nodes, conn, young, nu, yield_stress, hardening, fixed, imposed, f,
+                 time_period, increments, amplitude, iterative=True):
+    """model.rs:715-1022 — Model::solve_quasi_static for small strains on tetrahedra (no contact,
+    follower pressures or temperatures). ``amplitude`` = [(t, lambda), ...] starting at (0, 0).
+    Returns a dict: status, u / reactions (n+1, ndof), stresses (n+1, ne, ngp, 9), p (n+1, ne,
+    ngp), residuals (list per increment), newton / cuts / pcg counters, fail_increment."""
+    nodes, conn = _f64(nodes), _i64(conn)
+    ne, nen = conn.shape
+    ngp = 1 if nen == 4 else 4
+    bc = lambda a: _f64(np.broadcast_to(a, (ne,)))
+    young, nu, yield_stress, hardening = bc(young), bc(nu), bc(yield_stress), bc(hardening)
+    ndof = 3 * len(nodes)
+    fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+    imposed, f = _f64(imposed), _f64(f)
+    amp = np.asarray(amplitude, dtype=np.float64).reshape(-1, 2)
+    at, av = _f64(amp[:, 0]), _f64(amp[:, 1])
+    n = int(increments)
+    u = np.zeros((n + 1, ndof))
+    r = np.zeros((n + 1, ndof))
+    s = np.zeros((n + 1, ne, ngp, 9))
+    p = np.zeros((n + 1, ne, ngp))
+    res = np.zeros((n, NEWTON_ITERATIONS + 1))
+    res_len = np.zeros(n, dtype=np.int64)
+    counters = np.zeros(3, dtype=np.int64)
+    fail = np.zeros(1, dtype=np.int64)
+    times = np.zeros(3)
+    st = lib().or_quasi_static(len(nodes), _d(nodes), ne, nen, _i(conn), _d(young), _d(nu),
+                               _d(yield_stress), _d(hardening), fixed.ctypes.data_as(_U8),
+                               _d(imposed), _d(f), float(time_period), n, len(at), _d(at),
+                               _d(av), 1 if iterative else 0, _d(u), _d(r), _d(s), _d(p),
+                               _d(res), _i(res_len), _i(counters), _i(fail), _d(times))
+    return {"status": st, "u": u, "reactions": r, "stresses": s, "p": p,
+            "residuals": [res[k, :res_len[k]].copy() for k in range(n)],
+            "newton": int(counters[0]), "cuts": int(counters[1]), "pcg": int(counters[2]),
+            "fail_increment": int(fail[0]), "times": times}

@@ -232,6 +232,128 @@ void solid_stresses(int nen, const double (*x)[3], const double d[6][6], const d
   }
 }
 
+// ---------------------------------------------------------------- plasticity.rs
+// plasticity.rs:11 (YIELD_TOLERANCE)
+const double YIELD_TOLERANCE = 1e-10;
+
+// plasticity.rs:14-21 (PlasticState): plastic strain (Voigt, engineering shears) and p
+struct PlasticState { double ep[6]; double p; };
+
+// plasticity.rs:115-126 (contract): shears count twice
+double contract6(const double* a, const double* b) {
+  double s = 0.0;
+  for (int k = 0; k < 6; ++k) s += k < 3 ? a[k] * b[k] : 2.0 * a[k] * b[k];
+  return s;
+}
+
+// plasticity.rs:45-112 (VonMises::update): closed-form radial return from `prev`, stress, end
+// state, consistent tangent (6x6) and whether the point yielded.
+bool von_mises_update(double young, double poisson, double yield0, double hardening,
+                      const double strain[6], const PlasticState& prev, double stress[6],
+                      PlasticState& state, double tangent[6][6]) {
+  const double g = young / (2.0 * (1.0 + poisson));
+  const double bulk = young / (3.0 * (1.0 - 2.0 * poisson));
+  double elastic[6];
+  for (int k = 0; k < 6; ++k) elastic[k] = strain[k] - prev.ep[k];
+  const double volume = elastic[0] + elastic[1] + elastic[2];
+  double trial[6];
+  for (int k = 0; k < 6; ++k)
+    trial[k] = k < 3 ? 2.0 * g * (elastic[k] - volume / 3.0) : g * elastic[k];
+  const double root_3_2 = std::sqrt(1.5);
+  const double norm = std::sqrt(contract6(trial, trial));
+  const double q = root_3_2 * norm;
+  const double yield_stress = yield0 + hardening * prev.p;
+  const double excess = q - yield_stress;
+  double theta = 1.0, theta_bar = 0.0, dp = 0.0;
+  if (excess > YIELD_TOLERANCE * yield_stress) {
+    dp = excess / (3.0 * g + hardening);
+    theta = 1.0 - 3.0 * g * dp / q;
+    theta_bar = 3.0 * g / (3.0 * g + hardening) - (1.0 - theta);
+  }
+  double n[6];
+  for (int k = 0; k < 6; ++k) n[k] = dp > 0.0 ? trial[k] / norm : 0.0;
+  state = prev;
+  for (int k = 0; k < 6; ++k) {
+    const double tensor = root_3_2 * dp * n[k];
+    state.ep[k] += k < 3 ? tensor : 2.0 * tensor;
+  }
+  state.p += dp;
+  for (int k = 0; k < 6; ++k) stress[k] = theta * trial[k] + (k < 3 ? bulk * volume : 0.0);
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      const double delta = i == j ? 1.0 : 0.0;
+      double deviatoric = 0.0;
+      if (i < 3 && j < 3) deviatoric = delta - 1.0 / 3.0;
+      else if (i >= 3 && j >= 3 && i == j) deviatoric = 0.5;
+      const double volumetric = (i < 3 && j < 3) ? bulk : 0.0;
+      tangent[i][j] =
+          volumetric + 2.0 * g * theta * deviatoric - 2.0 * g * theta_bar * n[i] * n[j];
+    }
+  return dp > 0.0;
+}
+
+// element.rs:338-358 (solid_gauss) + :395-452 (solid_response, elastoplastic_response) for one
+// tetrahedron: internal force fint[m], tangent tan[j*m+i] = tangent[j][i] (row-major), Gauss-point
+// stresses out[q*9..] = x, y, z, s0..s5 (Voigt), end states, yielding.
+bool solid_response(int nen, const double (*x)[3], double young, double poisson, double yield0,
+                    double hardening, const double* ue, const PlasticState* prev, double* fint,
+                    double* tan, double* out, PlasticState* states) {
+  const int m = 3 * nen;
+  int ngp;
+  const TetGP* rule = rule_of(nen, &ngp);
+  std::fill(fint, fint + m, 0.0);
+  std::fill(tan, tan + m * m, 0.0);
+  bool yielding = false;
+  double n[10], dn[10][3], g[10][3];
+  for (int q = 0; q < ngp; ++q) {
+    tet_shape(nen, rule[q].xi, rule[q].eta, rule[q].zeta, n, dn);
+    const double det = mapping3d(nen, x, dn, g);
+    const double weight = rule[q].w * std::fabs(det);
+    double pos[3];
+    for (int k = 0; k < 3; ++k) {
+      double s = 0.0;
+      for (int a = 0; a < nen; ++a) s += n[a] * x[a][k];
+      pos[k] = s;
+    }
+    double strain[6];
+    for (int r = 0; r < 6; ++r) {
+      double s = 0.0;
+      for (int k = 0; k < m; ++k) {
+        double col[6];
+        b_column(g, k, col);
+        s += col[r] * ue[k];
+      }
+      strain[r] = s;
+    }
+    double stress[6], c[6][6];
+    yielding |= von_mises_update(young, poisson, yield0, hardening, strain, prev[q], stress,
+                                 states[q], c);
+    for (int i = 0; i < m; ++i) {
+      double bi[6], cb[6];
+      b_column(g, i, bi);
+      double f = 0.0;
+      for (int r = 0; r < 6; ++r) f += bi[r] * stress[r];
+      fint[i] += weight * f;
+      for (int r = 0; r < 6; ++r) {
+        double s = 0.0;
+        for (int cc = 0; cc < 6; ++cc) s += c[r][cc] * bi[cc];
+        cb[r] = s;
+      }
+      for (int j = 0; j < m; ++j) {
+        double bj[6];
+        b_column(g, j, bj);
+        double s = 0.0;
+        for (int r = 0; r < 6; ++r) s += bj[r] * cb[r];
+        tan[j * m + i] += weight * s;
+      }
+    }
+    double* o = out + 9 * q;
+    for (int k = 0; k < 3; ++k) o[k] = pos[k];
+    for (int k = 0; k < 6; ++k) o[3 + k] = stress[k];
+  }
+  return yielding;
+}
+
 // mesh.rs:184-198 (face_positions, tet part)
 int face_positions(int nen, int face, int* pos) {
   const int* f = TET_FACES[face];
@@ -641,6 +763,7 @@ struct System {
   std::vector<double> coupling;
   std::vector<double> imposed;
   std::vector<char> fixed;
+  const double* tangents = nullptr;  // n_el x m x m row-major element matrices, else stiffness
   int shift_index = -1;
   bool factored = false;
   double t_pattern = 0, t_assembly = 0, t_factor = 0, t_solve = 0;
@@ -687,7 +810,8 @@ int system_build(System& s) {
   s.coupling.assign(s.count, 0.0);
   std::vector<double> ke(ml * ml);
   for (i64 e = 0; e < m.n_el; ++e) {
-    m.stiffness(e, ke.data());
+    if (s.tangents) std::copy(s.tangents + e * ml * ml, s.tangents + (e + 1) * ml * ml, ke.begin());
+    else m.stiffness(e, ke.data());
     for (int li = 0; li < ml; ++li) {
       i64 a = s.free_of[m.dof(e, li)];
       if (a < 0) continue;
@@ -741,6 +865,156 @@ int system_solve(System& s, const double* f, double* u, double tol, i64 maxit, i
   s.t_solve = now() - t0;
   return st;
 }
+
+// ---------------------------------------------------------------- model.rs: quasi-static
+// model.rs:1220-1224
+const double NEWTON_TOLERANCE = 1e-8;
+const int NEWTON_ITERATIONS = 25;
+const int MAX_CUTS = 5;
+enum QsStatus { NEWTON_NOT_CONVERGED = 9, PLASTIC_COLLAPSE = 10 };
+
+// model.rs:181-247 (Amplitude::at): piecewise linear, constant after the last point
+double amplitude_at(const std::vector<double>& ts, const std::vector<double>& vs, double t) {
+  size_t after = 0;
+  while (after < ts.size() && ts[after] <= t) ++after;  // partition_point(ti <= t)
+  if (after >= 1 && after < ts.size()) {
+    double t0 = ts[after - 1], v0 = vs[after - 1], t1 = ts[after], v1 = vs[after];
+    return v0 + (v1 - v0) * (t - t0) / (t1 - t0);
+  }
+  if (after >= 1) return vs[after - 1];
+  if (!ts.empty()) return vs[0];
+  return 0.0;
+}
+
+// model.rs:1236-1238 (norm): sequential sum of squares
+double norm_seq(const double* v, i64 n, const char* mask, bool want) {
+  double s = 0.0;
+  for (i64 k = 0; k < n; ++k)
+    if (!mask || (mask[k] != 0) == want) s += v[k] * v[k];
+  return std::sqrt(s);
+}
+
+// Model::solve_quasi_static for small strains on tetrahedra without contact, follower pressures
+// or temperatures (model.rs:732-1022): the state of one run.
+struct QuasiStatic {
+  Mesh mesh;
+  const double* yield0;     // n_el (infinite: elastic)
+  const double* hardening;  // n_el
+  std::vector<char> fixed;
+  std::vector<double> imposed, f;
+  std::vector<double> amp_t, amp_v;
+  bool iterative = true;
+  int ngp = 1;
+  i64 ndof = 0;
+  // QuasiStaticState (model.rs:1226-1234)
+  std::vector<double> u;
+  std::vector<PlasticState> committed;
+  double scale = 0.0;
+  // the last equilibrium: responses, reactions, residuals
+  std::vector<double> stresses, p, reactions, residuals;
+  std::vector<PlasticState> states;
+  i64 newton_total = 0, cuts_total = 0, pcg_total = 0;
+  double t_build = 0, t_solve = 0, t_resp = 0;
+
+  // model.rs:1058-1067 (responses) with model.rs:1241-1250 (internal_force)
+  bool responses(const std::vector<double>& uu, std::vector<double>& fint_g,
+                 std::vector<double>* tangents) {
+    const int nen = mesh.nen, m = 3 * nen;
+    std::vector<double> fint(m), tan(m * m), out(9 * ngp), ue(m);
+    stresses.assign(mesh.n_el * ngp * 9, 0.0);
+    p.assign(mesh.n_el * ngp, 0.0);
+    states.assign(mesh.n_el * ngp, PlasticState{});
+    if (tangents) tangents->assign(mesh.n_el * m * m, 0.0);
+    fint_g.assign(ndof, 0.0);
+    bool yielding = false;
+    for (i64 e = 0; e < mesh.n_el; ++e) {
+      double x[10][3];
+      mesh.coords(e, x);
+      for (int k = 0; k < m; ++k) ue[k] = uu[mesh.dof(e, k)];
+      yielding |= solid_response(nen, x, mesh.young[e], mesh.nu[e], yield0[e], hardening[e],
+                                 ue.data(), &committed[e * ngp], fint.data(), tan.data(),
+                                 out.data(), &states[e * ngp]);
+      std::copy(out.begin(), out.end(), stresses.begin() + e * ngp * 9);
+      for (int q = 0; q < ngp; ++q) p[e * ngp + q] = states[e * ngp + q].p;
+      if (tangents) std::copy(tan.begin(), tan.end(), tangents->begin() + e * m * m);
+      for (int k = 0; k < m; ++k) fint_g[mesh.dof(e, k)] += fint[k];
+    }
+    return yielding;
+  }
+
+  // model.rs:923-1022 (equilibrium) from this->u: status; on success this->u is the equilibrium
+  int equilibrium(std::vector<double>& uu, double lambda, double& sc) {
+    residuals.clear();
+    std::vector<double> fint, tangents, residual(ndof), external(ndof), du(ndof), rhs(ndof);
+    for (;;) {
+      double t0 = now();
+      bool yielding = responses(uu, fint, &tangents);
+      t_resp += now() - t0;
+      for (i64 g = 0; g < ndof; ++g) external[g] = lambda * (f[g] + 0.0) + 0.0;
+      for (i64 g = 0; g < ndof; ++g) residual[g] = fint[g] - external[g];
+      sc = std::fmax(std::fmax(sc, norm_seq(fint.data(), ndof, nullptr, false)),
+                     norm_seq(external.data(), ndof, nullptr, false));
+      const double free_norm = norm_seq(residual.data(), ndof, fixed.data(), false);
+      const double relative = sc > 0.0 ? free_norm / sc : free_norm;
+      if (!std::isfinite(relative)) return NEWTON_NOT_CONVERGED;
+      residuals.push_back(relative);
+      if (residuals.size() > 1 && relative <= NEWTON_TOLERANCE) {
+        reactions.assign(ndof, 0.0);
+        for (i64 g = 0; g < ndof; ++g)
+          if (fixed[g]) reactions[g] = residual[g];
+        return OK;
+      }
+      if ((int)residuals.size() > NEWTON_ITERATIONS) return NEWTON_NOT_CONVERGED;
+      for (i64 g = 0; g < ndof; ++g) du[g] = fixed[g] ? lambda * imposed[g] - uu[g] : 0.0;
+      System sys;
+      sys.mesh = mesh;
+      sys.iterative = iterative;
+      sys.ndof = ndof;
+      sys.fixed = fixed;
+      sys.imposed = du;
+      sys.tangents = tangents.data();
+      double t1 = now();
+      int st = system_build(sys);
+      if (st != OK) return st;
+      st = system_factorize(sys);
+      t_build += now() - t1;
+      if (st == SINGULAR && yielding) return PLASTIC_COLLAPSE;
+      if (st != OK) return st;
+      for (i64 g = 0; g < ndof; ++g) rhs[g] = -residual[g];
+      i64 iters = 0;
+      double resid = 0.0;
+      t1 = now();
+      st = system_solve(sys, rhs.data(), du.data(), TOLERANCE, MAX_ITERATIONS, &iters, &resid);
+      t_solve += now() - t1;
+      pcg_total += iters;
+      ++newton_total;
+      if (st != OK) return st;
+      for (i64 g = 0; g < ndof; ++g) uu[g] += du[g];
+    }
+  }
+
+  // model.rs:800-857 (advance)
+  int advance(double t0, double t1, int cuts) {
+    std::vector<double> trial = u;
+    double sc = scale;
+    int st = equilibrium(trial, amplitude_at(amp_t, amp_v, t1), sc);
+    if (st == OK) {
+      u = trial;
+      scale = sc;
+      committed = states;
+      return OK;
+    }
+    if ((st == NEWTON_NOT_CONVERGED || st == SINGULAR || st == PLASTIC_COLLAPSE) &&
+        cuts < MAX_CUTS) {
+      const double middle = (t0 + t1) / 2.0;
+      ++cuts_total;
+      st = advance(t0, middle, cuts + 1);
+      if (st != OK) return st;
+      return advance(middle, t1, cuts + 1);
+    }
+    return st;
+  }
+};
 
 }  // namespace
 
@@ -926,6 +1200,98 @@ void or_system_times(void* h, double* t) {
   t[0] = s->t_pattern; t[1] = s->t_assembly; t[2] = s->t_factor; t[3] = s->t_solve;
 }
 void or_system_free(void* h) { delete (System*)h; }
+
+// model.rs:715-797 (solve_quasi_static) for small strains on tetrahedra. Per element: E, nu,
+// yield stress (infinite: elastic) and hardening. Per DOF: fixed, BC value, load F. Amplitude
+// points (t, lambda). Outputs for increments 0..n (arrays sized (n+1) x ...): u, reactions,
+// Gauss-point stresses (9 each) and p; residuals per increment n x 26 with their counts;
+// counters[0..3] = Newton iterations, increment cuts, PCG iterations (all attempts); on a typed
+// failure *fail_increment is the step increment that failed.
+int or_quasi_static(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64* conn,
+                    const double* young, const double* nu, const double* yield0,
+                    const double* hardening, const uint8_t* fixed, const double* imposed,
+                    const double* f, double time_period, i64 increments, i64 npts,
+                    const double* amp_t, const double* amp_v, int iterative, double* u_out,
+                    double* r_out, double* s_out, double* p_out, double* res_out, i64* res_len,
+                    i64* counters, i64* fail_increment, double* times) {
+  QuasiStatic qs;
+  qs.mesh = Mesh{n_nodes, n_el, nen, xyz, conn, young, nu};
+  qs.yield0 = yield0;
+  qs.hardening = hardening;
+  qs.ndof = 3 * n_nodes;
+  qs.ngp = nen == 4 ? 1 : 4;
+  qs.iterative = iterative != 0;
+  qs.fixed.assign(fixed, fixed + qs.ndof);
+  qs.imposed.assign(imposed, imposed + qs.ndof);
+  qs.f.assign(f, f + qs.ndof);
+  qs.amp_t.assign(amp_t, amp_t + npts);
+  qs.amp_v.assign(amp_v, amp_v + npts);
+  qs.u.assign(qs.ndof, 0.0);
+  qs.committed.assign(n_el * qs.ngp, PlasticState{});
+  const i64 ns = n_el * qs.ngp;
+  auto record = [&](i64 k) {
+    std::copy(qs.u.begin(), qs.u.end(), u_out + k * qs.ndof);
+    if (r_out) std::copy(qs.reactions.begin(), qs.reactions.end(), r_out + k * qs.ndof);
+    if (s_out) std::copy(qs.stresses.begin(), qs.stresses.end(), s_out + k * ns * 9);
+    if (p_out) std::copy(qs.p.begin(), qs.p.end(), p_out + k * ns);
+  };
+  std::vector<double> fint;
+  qs.responses(qs.u, fint, nullptr);  // model.rs:767 (at_rest)
+  qs.reactions.assign(qs.ndof, 0.0);
+  record(0);
+  *fail_increment = -1;
+  const double dt = time_period / (double)increments;  // Step::time_step (model.rs:171-173)
+  int st = OK;
+  for (i64 inc = 1; inc <= increments; ++inc) {
+    const double time = (double)inc * dt;
+    const double start = time - dt;
+    st = qs.advance(start, time, 0);
+    if (st != OK) { *fail_increment = inc; break; }
+    record(inc);
+    res_len[inc - 1] = (i64)qs.residuals.size();
+    std::copy(qs.residuals.begin(), qs.residuals.end(), res_out + (inc - 1) * (NEWTON_ITERATIONS + 1));
+  }
+  counters[0] = qs.newton_total;
+  counters[1] = qs.cuts_total;
+  counters[2] = qs.pcg_total;
+  if (times) { times[0] = qs.t_resp; times[1] = qs.t_build; times[2] = qs.t_solve; }
+  return st;
+}
+
+// plasticity.rs:45-112 for the unit tests: stress[6], state[7] (ep, p), tangent[36]; returns yielding
+int or_von_mises(double young, double poisson, double yield0, double hardening,
+                 const double* strain, const double* prev, double* stress, double* state,
+                 double* tangent) {
+  PlasticState a, b;
+  std::copy(prev, prev + 6, a.ep);
+  a.p = prev[6];
+  double c[6][6];
+  bool y = von_mises_update(young, poisson, yield0, hardening, strain, a, stress, b, c);
+  std::copy(b.ep, b.ep + 6, state);
+  state[6] = b.p;
+  std::memcpy(tangent, c, sizeof(c));
+  return y ? 1 : 0;
+}
+
+// element.rs:395-452 for one element: fint[m], tangent[m*m] (row-major tangent[j][i]),
+// out[ngp*9], states in/out [ngp*7]; returns yielding
+int or_solid_response(int nen, const double* x, double young, double poisson, double yield0,
+                      double hardening, const double* ue, const double* prev, double* fint,
+                      double* tan, double* out, double* states) {
+  int ngp = nen == 4 ? 1 : 4;
+  PlasticState a[4], b[4];
+  for (int q = 0; q < ngp; ++q) {
+    std::copy(prev + 7 * q, prev + 7 * q + 6, a[q].ep);
+    a[q].p = prev[7 * q + 6];
+  }
+  bool y = solid_response(nen, (const double (*)[3])x, young, poisson, yield0, hardening, ue, a,
+                          fint, tan, out, b);
+  for (int q = 0; q < ngp; ++q) {
+    std::copy(b[q].ep, b[q].ep + 6, states + 7 * q);
+    states[7 * q + 6] = b[q].p;
+  }
+  return y ? 1 : 0;
+}
 
 // model.rs:586-593 + :1353-1369 (product): reactions R = K u - F on the fixed DOFs, 0 elsewhere
 void or_reactions(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64* conn,
