@@ -36,7 +36,11 @@ typedef enum ml_status {
   ML_HIP_ERROR = 5,           /* a HIP runtime call failed; see ml_last_error() */
   ML_INVALID_MESH = 6,        /* Mesh::new refusals (mesh.rs:257-308), see ml_solve_info */
   ML_DEVICE_TIMEOUT = 7,      /* a dataflow kernel's bounded wait expired (never expected) */
-  ML_INVALID_FILE = 8         /* an input file refused by ml_msh_read (IoError, error.rs:9-65) */
+  ML_INVALID_FILE = 8,        /* an input file refused by ml_msh_read (IoError, error.rs:9-65) */
+  ML_NEWTON_NOT_CONVERGED = 9,/* MailleError::NotConverged (error.rs:158-163): Newton-Raphson
+                                 failed even on the smallest cut of an increment */
+  ML_PLASTIC_COLLAPSE = 10    /* MailleError::PlasticCollapse (error.rs:164-168): the tangent
+                                 stopped being positive definite while points yield */
 } ml_status;
 
 /* Arithmetic of the solve. ML_FP64 is the drop-in (the reference computes in f64).
@@ -227,6 +231,45 @@ ml_status ml_model_csr(ml_model* m, int64_t* order, int64_t* offsets, int64_t* c
  * ML_INVALID_ARGUMENT before a solve or with nonzero imposed displacements. */
 ml_status ml_model_compliance_gradient(ml_model* m, double* out);
 void ml_model_destroy(ml_model* m);
+
+/* ------------------------------------------------------------------ quasi-static plasticity */
+
+typedef struct ml_quasi_static_info {
+  int64_t newton_iterations;  /* linear solves of every equilibrium attempt, cuts included */
+  int64_t cuts;               /* increment halvings (Event::Cut, model.rs:840-851) */
+  int64_t pcg_iterations;     /* PCG iterations of all those solves */
+  int64_t failed_increment;   /* on ML_NEWTON_NOT_CONVERGED / ML_PLASTIC_COLLAPSE / a solver
+                                 error: the Step increment (1-based) that failed, else -1 */
+  double failed_time;         /* its end time (the `time` of the error) */
+  int32_t device_retries;     /* stalled dataflow launches repeated on smaller grids */
+  int32_t reserved;
+  double seconds[4];          /* element responses + residuals, assembly + IC(0), PCG, total */
+} ml_quasi_static_info;
+
+/* Model::solve_quasi_static (model.rs:715-797) for small strains on this model's tetrahedra:
+ * von Mises plasticity with linear isotropic hardening (plasticity.rs:34-112; per element
+ * yield_stress > 0, infinite for an elastic element, and hardening >= 0, material.rs:117-130),
+ * loads f[3*n_nodes] and the model's BC values scaled by the amplitude lambda(t) (npoints
+ * points amp_t/amp_v, the first (0, 0), times increasing: Amplitude::new, model.rs:191-213) over
+ * `increments` equal increments of time_period (Step::quasi_static, model.rs:136-150). Each
+ * increment is an equilibrium solved by Newton-Raphson on the device (model.rs:923-1022: at
+ * least one iteration, converged at relative residual <= 1e-8, at most 25), cut in halves up
+ * to 5 times when Newton fails (model.rs:800-857). Outputs for increments 0..n (increment 0 is
+ * the mesh at rest), host arrays, any may be null:
+ *   u[(n+1) x 3*n_nodes], reactions[(n+1) x 3*n_nodes] (F_int - lambda F on the fixed DOFs),
+ *   stresses[(n+1) x n_elements x ngp x 9] (x, y, z, sxx, syy, szz, tyz, txz, txy per Gauss
+ *   point), p[(n+1) x n_elements x ngp] (equivalent plastic strain),
+ *   residuals[n x 26] with residual_counts[n] (Increment::residuals of the increment's last
+ *   equilibrium). The norms behind the residuals are fixed-order tree sums, so they agree with
+ *   the reference's sequential sums to rounding.
+ * The model's matrix holds tangents afterwards; a later ml_model_solve reassembles K. */
+ml_status ml_model_solve_quasi_static(ml_model* m, const double* yield_stress,
+                                      const double* hardening, const double* f,
+                                      double time_period, int64_t increments, int64_t npoints,
+                                      const double* amp_t, const double* amp_v, double* u,
+                                      double* reactions, double* stresses, double* p,
+                                      double* residuals, int64_t* residual_counts,
+                                      ml_quasi_static_info* info);
 
 /* ------------------------------------------------------------------ batched solves
  * (no single counterpart in the reference: the equivalent of sequential Model::solve /

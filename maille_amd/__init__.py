@@ -25,7 +25,8 @@ _LIB_PATH = os.path.join(_HERE, "libmaille_hip.so")
 _lib = None
 
 ML_OK, ML_SINGULAR, ML_NOT_CONVERGED, ML_INVALID_ARGUMENT, ML_OUT_OF_PATTERN, ML_HIP_ERROR, \
-    ML_INVALID_MESH, ML_DEVICE_TIMEOUT, ML_INVALID_FILE = range(9)
+    ML_INVALID_MESH, ML_DEVICE_TIMEOUT, ML_INVALID_FILE, ML_NEWTON_NOT_CONVERGED, \
+    ML_PLASTIC_COLLAPSE = range(11)
 FP64, MIXED, MIXED_IC0 = 0, 1, 2
 AUTO, DIRECT, ITERATIVE = 0, 1, 2
 TOLERANCE = 1e-12          # iterative.rs:9
@@ -47,7 +48,7 @@ EXPORTS = (
     "ml_model_solve_many", "ml_batch_create", "ml_batch_solve", "ml_batch_destroy",
     "ml_pcg_autotune", "ml_model_autotune",
     "ml_msh_read", "ml_msh_sizes", "ml_msh_geometry", "ml_msh_set_info", "ml_msh_set_items",
-    "ml_msh_destroy",
+    "ml_msh_destroy", "ml_model_solve_quasi_static",
 )
 
 
@@ -80,8 +81,41 @@ class InvalidMesh(MailleError):
         self.element = element
 
 
+class NotConverged(MailleError):
+    """MailleError::NotConverged { increment, time } (error.rs:158-163): Newton-Raphson failed
+    on the smallest cut of an increment of a quasi-static Step."""
+
+    def __init__(self, increment, time):
+        super().__init__(f"Newton-Raphson did not converge at increment {increment} (t = {time})")
+        self.increment = increment
+        self.time = time
+
+
+class PlasticCollapse(MailleError):
+    """MailleError::PlasticCollapse { increment, time } (error.rs:164-168): the tangent stopped
+    being positive definite while Gauss points yield (a load beyond the limit load)."""
+
+    def __init__(self, increment, time):
+        super().__init__(f"plastic collapse at increment {increment} (t = {time})")
+        self.increment = increment
+        self.time = time
+
+
 class DeviceError(MailleError):
     """A HIP failure or an expired device wait."""
+
+
+class QuasiStaticInfo(C.Structure):
+    _fields_ = [("newton_iterations", C.c_int64), ("cuts", C.c_int64),
+                ("pcg_iterations", C.c_int64), ("failed_increment", C.c_int64),
+                ("failed_time", C.c_double), ("device_retries", C.c_int32),
+                ("reserved", C.c_int32), ("seconds", C.c_double * 4)]
+
+    def as_dict(self):
+        return {"newton_iterations": self.newton_iterations, "cuts": self.cuts,
+                "pcg_iterations": self.pcg_iterations, "failed_increment": self.failed_increment,
+                "failed_time": self.failed_time, "device_retries": self.device_retries,
+                "seconds": list(self.seconds)}
 
 
 class SolveInfo(C.Structure):
@@ -170,6 +204,9 @@ def lib():
             "ml_msh_set_info": (_int, [_P, _i64, C.c_char_p, _i64, _I]),
             "ml_msh_set_items": (_int, [_P, _i64, _I, _I, _I, _I]),
             "ml_msh_destroy": (None, [_P]),
+            "ml_model_solve_quasi_static": (_int, [_P, _D, _D, _D, C.c_double, _i64, _i64, _D,
+                                                   _D, _D, _D, _D, _D, _D, _I,
+                                                   C.POINTER(QuasiStaticInfo)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -185,9 +222,13 @@ def _check(status, info=None):
     if status == ML_SINGULAR:
         raise Singular("singular matrix")
     if status == ML_NOT_CONVERGED:
-        it = info.iterations if info is not None else -1
-        res = info.residual if info is not None else float("nan")
+        it = getattr(info, "iterations", getattr(info, "pcg_iterations", -1))
+        res = getattr(info, "residual", float("nan"))
         raise IterativeNotConverged(it, res)
+    if status in (ML_NEWTON_NOT_CONVERGED, ML_PLASTIC_COLLAPSE):
+        inc = getattr(info, "failed_increment", -1)
+        t = getattr(info, "failed_time", float("nan"))
+        raise (NotConverged if status == ML_NEWTON_NOT_CONVERGED else PlasticCollapse)(inc, t)
     if status == ML_INVALID_MESH and info is not None:
         raise InvalidMesh(info.mesh_error, info.bad_element)
     msg = lib().ml_last_error().decode(errors="replace")

@@ -326,6 +326,247 @@ __global__ __launch_bounds__(256) void k_stresses(int64_t ne, const double* __re
   }
 }
 
+// ------------------------------------------------------------------ elasto-plastic response
+// VonMises::update (plasticity.rs:45-112): closed-form radial return from the state (ep, p) at
+// the start of the step, for the total strain `strain`. Writes the stress, the end state and
+// the consistent tangent dsigma/deps; returns whether the point yielded. Infinite yield stress:
+// elastic. Shear strains are engineering shears, shear stresses tensor components.
+__device__ inline bool von_mises_update(double young, double poisson, double yield0,
+                                        double hardening, const double (&strain)[6],
+                                        const double (&ep0)[6], double p0, double (&stress)[6],
+                                        double (&ep)[6], double& p, double (&c)[6][6]) {
+#pragma clang fp contract(off)
+  const double g = young / (2.0 * (1.0 + poisson));
+  const double bulk = young / (3.0 * (1.0 - 2.0 * poisson));
+  double el[6];
+  for (int k = 0; k < 6; ++k) el[k] = strain[k] - ep0[k];
+  const double volume = el[0] + el[1] + el[2];
+  double trial[6];
+  for (int k = 0; k < 6; ++k) trial[k] = k < 3 ? 2.0 * g * (el[k] - volume / 3.0) : g * el[k];
+  const double root_3_2 = sqrt(1.5);
+  double contract = 0.0;  // s : s, shears counted twice (plasticity.rs:115-126)
+  for (int k = 0; k < 6; ++k) contract += k < 3 ? trial[k] * trial[k] : 2.0 * trial[k] * trial[k];
+  const double norm = sqrt(contract);
+  const double q = root_3_2 * norm;
+  const double ys = yield0 + hardening * p0;
+  const double excess = q - ys;
+  double theta = 1.0, theta_bar = 0.0, dp = 0.0;
+  if (excess > 1e-10 * ys) {  // YIELD_TOLERANCE (plasticity.rs:11)
+    dp = excess / (3.0 * g + hardening);
+    theta = 1.0 - 3.0 * g * dp / q;
+    theta_bar = 3.0 * g / (3.0 * g + hardening) - (1.0 - theta);
+  }
+  double n[6];
+  for (int k = 0; k < 6; ++k) n[k] = dp > 0.0 ? trial[k] / norm : 0.0;
+  for (int k = 0; k < 6; ++k) {
+    const double tensor = root_3_2 * dp * n[k];
+    ep[k] = ep0[k] + (k < 3 ? tensor : 2.0 * tensor);
+  }
+  p = p0 + dp;
+  for (int k = 0; k < 6; ++k) stress[k] = theta * trial[k] + (k < 3 ? bulk * volume : 0.0);
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      const double delta = i == j ? 1.0 : 0.0;
+      const double dev = (i < 3 && j < 3) ? delta - 1.0 / 3.0 : ((i >= 3 && i == j) ? 0.5 : 0.0);
+      const double vol = (i < 3 && j < 3) ? bulk : 0.0;
+      c[i][j] = vol + 2.0 * g * theta * dev - 2.0 * g * theta_bar * n[i] * n[j];
+    }
+  return dp > 0.0;
+}
+
+// solid_response (element.rs:395-452) on the global displacements u, from the committed states,
+// one thread per (element e, column i) like k_stiffness: each thread repeats the Gauss-point
+// strain and return mapping (cheap beside the column) and accumulates, Gauss point after Gauss
+// point in the reference's order, column i of the tangent (tangent[j][i] = sum w b_j.(C b_i),
+// written to slot (j*M + i)*ne + e, the stiffness layout the gather assembly reads) and the
+// internal force fint[i] = sum w b_i.sigma (slot i*ne + e). The thread of column 0 writes the
+// end states (component k of point q at (k*NG + q)*ne + e; 6 plastic strains, then p), the
+// Gauss-point records (position, then the Voigt stress: 9 doubles at (e*NG + q)*9) and, if a
+// point yielded, raises *yielding.
+template <int NEN>
+__global__ __launch_bounds__(256) void k_response(
+    int64_t ne, const double* __restrict__ xyz, const int* __restrict__ conn,
+    const double* __restrict__ young, const double* __restrict__ nu,
+    const double* __restrict__ yield0, const double* __restrict__ hardening,
+    const double* __restrict__ u, const double* __restrict__ committed,
+    double* __restrict__ ke, double* __restrict__ fint, double* __restrict__ trial,
+    double* __restrict__ records, int* __restrict__ yielding) {
+#pragma clang fp contract(off)
+  constexpr int M = 3 * NEN, NG = Rule<NEN>::N;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)M * ne) return;
+  const int i = (int)(t / ne);
+  const int64_t e = t - (int64_t)i * ne;
+  double x[NEN][3], ue[M];
+  for (int a = 0; a < NEN; ++a) {
+    const int64_t node = conn[e * NEN + a];
+    for (int c = 0; c < 3; ++c) {
+      x[a][c] = xyz[3 * node + c];
+      ue[3 * a + c] = u[3 * node + c];
+    }
+  }
+  const double E = young[e], v = nu[e], sy = yield0[e], h = hardening[e];
+  double acc[M];
+  for (int j = 0; j < M; ++j) acc[j] = 0.0;
+  double fi = 0.0;
+  bool yielded = false;
+  for (int q = 0; q < NG; ++q) {
+    double xi, eta, zeta, w;
+    gauss_point(NEN, q, xi, eta, zeta, w);
+    double g[NEN][3];
+    const double det = tet_mapping<NEN>(x, xi, eta, zeta, g);
+    const double weight = w * fabs(det);
+    double strain[6];
+    for (int r = 0; r < 6; ++r) {
+      double s = 0.0;
+      for (int k = 0; k < M; ++k) {
+        double col[6];
+        b_column<NEN>(g, k, col);
+        s += col[r] * ue[k];
+      }
+      strain[r] = s;
+    }
+    double ep0[6], ep[6], p, stress[6], c[6][6];
+    for (int k = 0; k < 6; ++k) ep0[k] = committed[((int64_t)k * NG + q) * ne + e];
+    const double p0 = committed[((int64_t)6 * NG + q) * ne + e];
+    yielded |= von_mises_update(E, v, sy, h, strain, ep0, p0, stress, ep, p, c);
+    double bi[6], cb[6];
+    b_column<NEN>(g, i, bi);
+    double f = 0.0;
+    for (int r = 0; r < 6; ++r) f += bi[r] * stress[r];
+    fi += weight * f;
+    for (int r = 0; r < 6; ++r) {
+      double s = 0.0;
+      for (int k = 0; k < 6; ++k) s += c[r][k] * bi[k];
+      cb[r] = s;
+    }
+    for (int j = 0; j < M; ++j) {
+      double bj[6];
+      b_column<NEN>(g, j, bj);
+      double s = 0.0;
+      for (int r = 0; r < 6; ++r) s += bj[r] * cb[r];
+      acc[j] += weight * s;
+    }
+    if (i == 0) {
+      for (int k = 0; k < 6; ++k) trial[((int64_t)k * NG + q) * ne + e] = ep[k];
+      trial[((int64_t)6 * NG + q) * ne + e] = p;
+      double n[NEN];
+      const double l[4] = {1.0 - xi - eta - zeta, xi, eta, zeta};
+      if constexpr (NEN == 4) {
+        for (int a = 0; a < 4; ++a) n[a] = l[a];
+      } else {
+        const int ed[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+        for (int a = 0; a < 4; ++a) n[a] = l[a] * (2.0 * l[a] - 1.0);
+        for (int k = 0; k < 6; ++k) n[4 + k] = 4.0 * l[ed[k][0]] * l[ed[k][1]];
+      }
+      double* o = records + (e * NG + q) * 9;
+      for (int k = 0; k < 3; ++k) {
+        double s = 0.0;
+        for (int a = 0; a < NEN; ++a) s += n[a] * x[a][k];
+        o[k] = s;
+      }
+      for (int k = 0; k < 6; ++k) o[3 + k] = stress[k];
+    }
+  }
+  for (int j = 0; j < M; ++j) ke[((int64_t)j * M + i) * ne + e] = acc[j];
+  fint[(int64_t)i * ne + e] = fi;
+  if (i == 0 && yielded) *yielding = 1;
+}
+
+// internal_force (model.rs:1241-1250), thread per node: F_int[3n + c] sums the node's element
+// forces over its incidences in element order (f[g] += v, starting from 0.0).
+template <int NEN>
+__global__ __launch_bounds__(256) void k_internal_force(int64_t n_nodes, int64_t ne,
+                                                        const int64_t* __restrict__ inc_off,
+                                                        const int* __restrict__ inc_item,
+                                                        const double* __restrict__ fint,
+                                                        double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n_nodes) return;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int64_t k = inc_off[v]; k < inc_off[v + 1]; ++k) {
+    const int64_t t = inc_item[k];
+    const int64_t e = t / NEN;
+    const int a = (int)(t - e * NEN);
+    for (int c = 0; c < 3; ++c) s[c] += fint[(int64_t)(3 * a + c) * ne + e];
+  }
+  for (int c = 0; c < 3; ++c) out[3 * v + c] = s[c];
+}
+
+// The Newton residual of equilibrium (model.rs:950-967), thread per DOF: external = lambda (F + 0)
+// + 0 (no follower or contact force on this path), r = F_int - external, -r for the solve, and
+// per-block sums of F_int^2, external^2 and r^2 over the free DOFs (fixed-tree, deterministic)
+// for the three norms; the reactions are r on the fixed DOFs (model.rs:981-986).
+__global__ __launch_bounds__(256) void k_newton_residual(int64_t ndof, double lambda,
+                                                         const double* __restrict__ f,
+                                                         const double* __restrict__ fint,
+                                                         const int* __restrict__ rank_of_dof,
+                                                         double* __restrict__ neg_r,
+                                                         double* __restrict__ reactions,
+                                                         double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0.0, b = 0.0, c = 0.0;
+  if (g < ndof) {
+    const double ext = lambda * (f[g] + 0.0) + 0.0;
+    const double fi = fint[g];
+    const double r = fi - ext;
+    neg_r[g] = -r;
+    const bool free_dof = rank_of_dof[g] >= 0;
+    reactions[g] = free_dof ? 0.0 : r;
+    a = fi * fi;
+    b = ext * ext;
+    c = free_dof ? r * r : 0.0;
+  }
+  a = block_sum<256>(a);
+  b = block_sum<256>(b);
+  c = block_sum<256>(c);
+  if (threadIdx.x == 0) {
+    partial[3 * blockIdx.x] = a;
+    partial[3 * blockIdx.x + 1] = b;
+    partial[3 * blockIdx.x + 2] = c;
+  }
+}
+
+// One block: the three sums of k_newton_residual's partials, in a fixed order.
+__global__ __launch_bounds__(256) void k_sum3(const double* __restrict__ partial, int count,
+                                              double* __restrict__ out) {
+  for (int w = 0; w < 3; ++w) {
+    double v = 0.0;
+    for (int k = threadIdx.x; k < count; k += 256) v += partial[3 * k + w];
+    v = block_sum<256>(v);
+    if (threadIdx.x == 0) out[w] = v;
+    __syncthreads();
+  }
+}
+
+// du at the start of a Newton iteration (model.rs:998-1004): lambda * BC value - u on the fixed
+// DOFs, zero elsewhere.
+__global__ __launch_bounds__(256) void k_newton_du(int64_t ndof, double lambda,
+                                                   const int* __restrict__ rank_of_dof,
+                                                   const double* __restrict__ imposed,
+                                                   const double* __restrict__ u,
+                                                   double* __restrict__ du) {
+#pragma clang fp contract(off)
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ndof) return;
+  du[g] = rank_of_dof[g] < 0 ? lambda * imposed[g] - u[g] : 0.0;
+}
+
+// FreeSystem::solve's scatter into du (model.rs:1509-1514) and u += du (model.rs:1017-1019).
+__global__ __launch_bounds__(256) void k_newton_update(int64_t ndof,
+                                                       const int* __restrict__ rank_of_dof,
+                                                       const double* __restrict__ x,
+                                                       const double* __restrict__ du,
+                                                       double* __restrict__ u) {
+#pragma clang fp contract(off)
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ndof) return;
+  const int r = rank_of_dof[g];
+  u[g] += r >= 0 ? x[r] : du[g];
+}
+
 // ------------------------------------------------------------------ assembly
 // Csr::add in the reference's scatter order (model.rs:1480-1492): entry q sums its element
 // contributions in element order, starting from 0.0.

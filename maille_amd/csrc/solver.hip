@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -1139,7 +1140,8 @@ struct Model {
   }
 
   // Gauss-point stresses (gp_dev: n_elements x ngp x 9) averaged at the nodes (n_nodes x 9).
-  void nodal_stresses(const double* gp_dev, double* out_dev) {
+  // node -> (element, local node) incidences in element order (t = e*nen + a)
+  void ensure_incidences() {
     const int64_t nn = mesh.n_nodes, ne = plan.n_elements;
     const int nen = plan.nen;
     if (!have_inc) {
@@ -1155,6 +1157,12 @@ struct Model {
       HIP_CHECK(hipStreamSynchronize(s()));
       have_inc = true;
     }
+  }
+
+  void nodal_stresses(const double* gp_dev, double* out_dev) {
+    const int64_t nn = mesh.n_nodes;
+    const int nen = plan.nen;
+    ensure_incidences();
     hipEvent_t ev = op.prof.begin(s());
     if (nen == 4)
       mlk::k_nodal_stresses<4><<<grid_for(nn), 256, 0, s()>>>(nn, d_xyz.p, d_inc_off.p,
@@ -1177,6 +1185,217 @@ struct Model {
                                                             d_nu.p, u_dev, out_dev);
     op.prof.end(P_STRESS, ev, s());
     HIP_CHECK(hipGetLastError());
+  }
+};
+
+template <class T>
+void swap_bufs(DBuf<T>& a, DBuf<T>& b) {
+  std::swap(a.p, b.p);
+  std::swap(a.n, b.n);
+}
+
+// Model::solve_quasi_static (model.rs:715-797) for small strains on tetrahedra without contact,
+// follower pressures or temperatures: increments of a Step, each brought to equilibrium by
+// Newton-Raphson (equilibrium, model.rs:923-1022) and cut in two halves, at most MAX_CUTS deep,
+// when Newton fails (advance, model.rs:800-857). Each Newton iteration runs on the device: the
+// element responses (k_response: return mapping, consistent tangent, internal forces), the
+// residual and its norms, the gather assembly of the tangents into the model's pattern, IC(0)
+// with the shift ladder and the PCG solve. The pattern, RCM numbering and assembly plan are the
+// model's: FreeSystem::new rebuilds the same ones every iteration (same element DOFs, same fixed
+// set), so only the values change.
+struct QuasiStatic {
+  static constexpr double NEWTON_TOLERANCE = 1e-8;  // model.rs:1220
+  static constexpr int NEWTON_ITERATIONS = 25;       // model.rs:1222
+  static constexpr int MAX_CUTS = 5;                 // model.rs:1224
+
+  Model& m;
+  std::vector<double> amp_t, amp_v;
+  int ngp;
+  int64_t ne, ndof, nn;
+  unsigned nblk;
+  DBuf<double> d_yield, d_hard, d_committed, d_trial, d_fint_e, d_fint, d_negr, d_react,
+      d_records, d_du, d_u, d_ut, d_f, d_partial, d_sums;
+  DBuf<int> d_flag;
+  double scale = 0.0;
+  std::vector<double> residuals;
+  int64_t newton = 0, cuts = 0, pcg_iterations = 0;
+  int device_retries = 0;
+  double t_resp = 0, t_factor = 0, t_solve = 0;
+
+  QuasiStatic(Model& model, const double* yield0, const double* hardening, const double* f)
+      : m(model) {
+    ne = m.plan.n_elements;
+    ndof = m.plan.ndof;
+    nn = m.mesh.n_nodes;
+    ngp = m.plan.nen == 4 ? 1 : 4;
+    nblk = grid_for(ndof);
+    hipStream_t st = m.s();
+    d_yield.upload(yield0, ne, st);
+    d_hard.upload(hardening, ne, st);
+    d_f.upload(f, ndof, st);
+    d_committed.alloc((size_t)7 * ngp * ne);
+    d_trial.alloc((size_t)7 * ngp * ne);
+    d_fint_e.alloc((size_t)3 * m.plan.nen * ne);
+    d_records.alloc((size_t)9 * ngp * ne);
+    for (auto* b : {&d_fint, &d_negr, &d_react, &d_du, &d_u, &d_ut}) b->alloc(ndof);
+    d_partial.alloc((size_t)3 * nblk);
+    d_sums.alloc(3);
+    d_flag.alloc(1);
+    mlk::k_fill_f64<<<grid_for(d_committed.n), 256, 0, st>>>(d_committed.p, d_committed.n, 0.0);
+    mlk::k_fill_f64<<<grid_for(ndof), 256, 0, st>>>(d_u.p, ndof, 0.0);
+    HIP_CHECK(hipGetLastError());
+    m.ensure_incidences();
+    // the model's K values are replaced by tangents from here on
+    m.assembled = false;
+    m.op.factored = false;
+    m.have_u = false;
+  }
+
+  // Amplitude::at (model.rs:236-246)
+  double amplitude(double t) const {
+    size_t after = 0;
+    while (after < amp_t.size() && amp_t[after] <= t) ++after;
+    if (after >= 1 && after < amp_t.size())
+      return amp_v[after - 1] +
+             (amp_v[after] - amp_v[after - 1]) * (t - amp_t[after - 1]) / (amp_t[after] - amp_t[after - 1]);
+    if (after >= 1) return amp_v[after - 1];
+    return amp_v.empty() ? 0.0 : amp_v[0];
+  }
+
+  // responses (model.rs:1058-1067) at u from the committed states, then internal_force
+  // (model.rs:1241-1250): tangents into the model's Ke array, trial states, Gauss-point records.
+  void responses(const double* u) {
+    hipStream_t st = m.s();
+    mlk::k_fill_i32<<<1, 256, 0, st>>>(d_flag.p, 1, 0);
+    hipEvent_t ev = m.op.prof.begin(st);
+    if (m.plan.nen == 4)
+      mlk::k_response<4><<<grid_for(12 * ne), 256, 0, st>>>(
+          ne, m.d_xyz.p, m.d_conn.p, m.d_young.p, m.d_nu.p, d_yield.p, d_hard.p, u,
+          d_committed.p, m.d_ke.p, d_fint_e.p, d_trial.p, d_records.p, d_flag.p);
+    else
+      mlk::k_response<10><<<grid_for(30 * ne), 256, 0, st>>>(
+          ne, m.d_xyz.p, m.d_conn.p, m.d_young.p, m.d_nu.p, d_yield.p, d_hard.p, u,
+          d_committed.p, m.d_ke.p, d_fint_e.p, d_trial.p, d_records.p, d_flag.p);
+    HIP_CHECK(hipGetLastError());
+    m.op.prof.end(P_STIFF, ev, st);
+    if (m.plan.nen == 4)
+      mlk::k_internal_force<4><<<grid_for(nn), 256, 0, st>>>(nn, ne, m.d_inc_off.p,
+                                                            m.d_inc_item.p, d_fint_e.p, d_fint.p);
+    else
+      mlk::k_internal_force<10><<<grid_for(nn), 256, 0, st>>>(nn, ne, m.d_inc_off.p,
+                                                             m.d_inc_item.p, d_fint_e.p, d_fint.p);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // equilibrium (model.rs:923-1022) from u (device, updated in place)
+  int equilibrium(double* u, double lambda, double& sc) {
+    residuals.clear();
+    hipStream_t st = m.s();
+    for (;;) {
+      double t0 = now_s();
+      responses(u);
+      mlk::k_newton_residual<<<nblk, 256, 0, st>>>(ndof, lambda, d_f.p, d_fint.p, m.d_rank.p,
+                                                  d_negr.p, d_react.p, d_partial.p);
+      mlk::k_sum3<<<1, 256, 0, st>>>(d_partial.p, (int)nblk, d_sums.p);
+      HIP_CHECK(hipGetLastError());
+      double sums[3];
+      int yielding = 0;
+      HIP_CHECK(hipMemcpyAsync(sums, d_sums.p, sizeof(sums), hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipMemcpyAsync(&yielding, d_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      t_resp += now_s() - t0;
+      sc = std::fmax(std::fmax(sc, std::sqrt(sums[0])), std::sqrt(sums[1]));
+      const double free_norm = std::sqrt(sums[2]);
+      const double relative = sc > 0.0 ? free_norm / sc : free_norm;
+      if (!std::isfinite(relative)) return ML_NEWTON_NOT_CONVERGED;
+      residuals.push_back(relative);
+      if (residuals.size() > 1 && relative <= NEWTON_TOLERANCE) return ML_OK;
+      if ((int)residuals.size() > NEWTON_ITERATIONS) return ML_NEWTON_NOT_CONVERGED;
+      // K_ff du_f = -r_f - K_fc du_c with du_c = lambda * BC value - u_c
+      t0 = now_s();
+      mlk::k_newton_du<<<grid_for(ndof), 256, 0, st>>>(ndof, lambda, m.d_rank.p, m.d_imposed.p,
+                                                       u, d_du.p);
+      hipEvent_t ev = m.op.prof.begin(st);
+      mlk::k_assemble<<<grid_for(m.op.nnz), 256, 0, st>>>(m.op.nnz, m.d_asm_off.p,
+                                                          m.d_asm_slot.p, m.d_ke.p, m.op.d_a.p);
+      HIP_CHECK(hipGetLastError());
+      m.op.prof.end(P_ASM, ev, st);
+      const bool coupled = !m.plan.cpl_slot.empty();
+      if (coupled) {
+        mlk::k_coupling<<<grid_for(m.plan.nfree), 256, 0, st>>>(
+            m.plan.nfree, m.d_cpl_off.p, m.d_cpl_slot.p, m.d_cpl_dof.p, m.d_ke.p, d_du.p,
+            m.d_coupling.p);
+        HIP_CHECK(hipGetLastError());
+      }
+      m.op.values_changed();
+      const int retries0 = m.op.retries;
+      int s = m.op.factor();
+      device_retries += m.op.retries - retries0;
+      t_factor += now_s() - t0;
+      if (s == ML_SINGULAR && yielding) return ML_PLASTIC_COLLAPSE;
+      if (s != ML_OK) return s;
+      t0 = now_s();
+      mlk::k_rhs<<<grid_for(ndof), 256, 0, st>>>(ndof, m.d_rank.p, d_negr.p,
+                                                 coupled ? m.d_coupling.p : nullptr, m.d_rhs.p);
+      HIP_CHECK(hipGetLastError());
+      ml_solve_info info;
+      std::memset(&info, 0, sizeof(info));
+      s = m.op.solve(m.d_rhs.p, m.d_x.p, 0, 0, &info);
+      pcg_iterations += info.iterations;
+      device_retries += info.device_retries;
+      ++newton;
+      t_solve += now_s() - t0;
+      if (s != ML_OK) return s;
+      mlk::k_newton_update<<<grid_for(ndof), 256, 0, st>>>(ndof, m.d_rank.p, m.d_x.p, d_du.p, u);
+      HIP_CHECK(hipGetLastError());
+    }
+  }
+
+  // advance (model.rs:800-857): the state changes only when an equilibrium converges
+  int advance(double t0, double t1, int depth) {
+    HIP_CHECK(hipMemcpyAsync(d_ut.p, d_u.p, sizeof(double) * ndof, hipMemcpyDeviceToDevice, m.s()));
+    double sc = scale;
+    int st = equilibrium(d_ut.p, amplitude(t1), sc);
+    if (st == ML_OK) {
+      swap_bufs(d_u, d_ut);
+      swap_bufs(d_committed, d_trial);
+      scale = sc;
+      return ML_OK;
+    }
+    if ((st == ML_NEWTON_NOT_CONVERGED || st == ML_SINGULAR || st == ML_PLASTIC_COLLAPSE) &&
+        depth < MAX_CUTS) {
+      const double middle = (t0 + t1) / 2.0;
+      ++cuts;
+      st = advance(t0, middle, depth + 1);
+      if (st != ML_OK) return st;
+      return advance(middle, t1, depth + 1);
+    }
+    return st;
+  }
+
+  // increment k's field into the caller's host arrays (any may be null): u, reactions, the
+  // Gauss-point records and p per Gauss point (layout (element, Gauss point))
+  void record(int64_t k, double* u, double* R, double* records, double* p) {
+    hipStream_t st = m.s();
+    if (u) HIP_CHECK(hipMemcpyAsync(u + k * ndof, d_u.p, sizeof(double) * ndof, hipMemcpyDeviceToHost, st));
+    if (R) {
+      if (k == 0) std::fill(R, R + ndof, 0.0);
+      else HIP_CHECK(hipMemcpyAsync(R + k * ndof, d_react.p, sizeof(double) * ndof, hipMemcpyDeviceToHost, st));
+    }
+    const size_t ns = (size_t)ngp * ne;
+    if (records)
+      HIP_CHECK(hipMemcpyAsync(records + k * ns * 9, d_records.p, sizeof(double) * ns * 9,
+                               hipMemcpyDeviceToHost, st));
+    std::vector<double> pq;
+    if (p) {
+      pq.resize(ns);
+      HIP_CHECK(hipMemcpyAsync(pq.data(), d_committed.p + 6 * ns, sizeof(double) * ns,
+                               hipMemcpyDeviceToHost, st));
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    if (p)
+      for (int64_t e = 0; e < ne; ++e)
+        for (int q = 0; q < ngp; ++q) p[k * ns + e * ngp + q] = pq[(size_t)q * ne + e];
   }
 };
 
@@ -1827,6 +2046,80 @@ ml_status ml_model_compliance_gradient(ml_model* h, double* out) {
                              hipMemcpyDeviceToHost, m.s()));
     HIP_CHECK(hipStreamSynchronize(m.s()));
     return ML_OK;
+  });
+}
+
+ml_status ml_model_solve_quasi_static(ml_model* h, const double* yield_stress,
+                                      const double* hardening, const double* f,
+                                      double time_period, int64_t increments, int64_t npoints,
+                                      const double* amp_t, const double* amp_v, double* u,
+                                      double* reactions, double* stresses, double* p,
+                                      double* residuals, int64_t* residual_counts,
+                                      ml_quasi_static_info* info) {
+  return guarded([&]() -> int {
+    if (!h || !yield_stress || !hardening || !f || !amp_t || !amp_v) throw ArgError("null argument");
+    Model& m = h->m;
+    const int64_t ne = m.plan.n_elements, ndof = m.plan.ndof;
+    if (!(std::isfinite(time_period) && time_period > 0.0))
+      throw ArgError("the Step's time period must be finite and > 0");
+    if (increments <= 0) throw ArgError("a Step needs at least one increment");
+    if (npoints <= 0) throw ArgError("amplitude point 0 is invalid");
+    for (int64_t k = 0; k < npoints; ++k) {
+      bool ok = std::isfinite(amp_t[k]) && std::isfinite(amp_v[k]);
+      const double eps = std::numeric_limits<double>::epsilon();
+      ok = ok && (k == 0 ? (std::fabs(amp_t[k]) <= eps && std::fabs(amp_v[k]) <= eps)
+                         : amp_t[k] > amp_t[k - 1]);
+      if (!ok) throw ArgError("amplitude point " + std::to_string(k) + " is invalid");
+    }
+    for (int64_t e = 0; e < ne; ++e) {
+      if (!(yield_stress[e] > 0.0) || std::isnan(yield_stress[e]))
+        throw ArgError("yield stress must be > 0 (infinite: elastic)");
+      if (!(std::isfinite(hardening[e]) && hardening[e] >= 0.0))
+        throw ArgError("hardening must be finite and >= 0");
+    }
+    for (int64_t g = 0; g < ndof; ++g)
+      if (!std::isfinite(f[g])) throw ArgError("loads must be finite");
+    if (info) {
+      std::memset(info, 0, sizeof(*info));
+      info->failed_increment = -1;
+    }
+    const double t_start = now_s();
+    QuasiStatic qs(m, yield_stress, hardening, f);
+    qs.amp_t.assign(amp_t, amp_t + npoints);
+    qs.amp_v.assign(amp_v, amp_v + npoints);
+    qs.responses(qs.d_u.p);  // model.rs:767 (the mesh at rest)
+    qs.record(0, u, reactions, stresses, p);
+    const double dt = time_period / (double)increments;  // Step::time_step (model.rs:171-173)
+    int st = ML_OK;
+    for (int64_t inc = 1; inc <= increments; ++inc) {
+      const double time = (double)inc * dt;
+      const double start = time - dt;
+      st = qs.advance(start, time, 0);
+      if (st != ML_OK) {
+        if (info) {
+          info->failed_increment = inc;
+          info->failed_time = time;
+        }
+        break;
+      }
+      qs.record(inc, u, reactions, stresses, p);
+      if (residual_counts) residual_counts[inc - 1] = (int64_t)qs.residuals.size();
+      if (residuals)
+        std::copy(qs.residuals.begin(), qs.residuals.end(),
+                  residuals + (inc - 1) * (QuasiStatic::NEWTON_ITERATIONS + 1));
+    }
+    HIP_CHECK(hipStreamSynchronize(m.s()));
+    if (info) {
+      info->newton_iterations = qs.newton;
+      info->cuts = qs.cuts;
+      info->pcg_iterations = qs.pcg_iterations;
+      info->device_retries = qs.device_retries;
+      info->seconds[0] = qs.t_resp;
+      info->seconds[1] = qs.t_factor;
+      info->seconds[2] = qs.t_solve;
+      info->seconds[3] = now_s() - t_start;
+    }
+    return st;
   });
 }
 

@@ -83,3 +83,66 @@ def von_mises_of(stresses) -> np.ndarray:
     a, b, c = s[..., 0], s[..., 1], s[..., 2]
     shear = s[..., 5] * s[..., 5] + s[..., 3] * s[..., 3] + s[..., 4] * s[..., 4]
     return np.sqrt(np.maximum(a * a + b * b + c * c - a * b - b * c - c * a + 3.0 * shear, 0.0))
+
+
+@dataclass
+class Increment:
+    """results.rs Increment for a quasi-static Step: time, displacements and reactions over all
+    DOFs, Gauss-point stresses (ne, ngp, 9: position, then sxx, syy, szz, tyz, txz, txy), the
+    equivalent plastic strain p (ne, ngp) and the relative residuals of the increment's Newton
+    iterations (empty for the mesh at rest)."""
+
+    time: float
+    displacements: np.ndarray
+    reactions: np.ndarray
+    stresses: np.ndarray
+    equivalent_plastic_strains: np.ndarray
+    residuals: np.ndarray
+
+
+def solve_quasi_static(model, step: Step, amplitude: Amplitude, yield_stress, hardening,
+                       f=None, stresses: bool = True):
+    """Model::solve_quasi_static (model.rs:715-797) on the device for ``model`` (a
+    ``maille_amd.Model``): per-element yield stress (``np.inf`` for an elastic element) and
+    hardening, loads ``f`` (default: the model's nodal loads, pressures and edge loads) and the
+    model's BC values scaled by ``amplitude``. Returns (increments, info); raises
+    ``NotConverged`` / ``PlasticCollapse`` like the reference."""
+    from . import (QuasiStaticInfo, _check, _d, _f64, _i, lib)
+    ne, nen, ndof = model.ne, model.nen, model.ndof
+    ngp = 1 if nen == 4 else 4
+    sy = _f64(np.broadcast_to(np.asarray(yield_stress, dtype=np.float64), (ne,)))
+    hd = _f64(np.broadcast_to(np.asarray(hardening, dtype=np.float64), (ne,)))
+    if np.any(np.isnan(sy)) or np.any(sy <= 0):
+        raise PlasticityInputError("yield stress must be > 0 (infinite: elastic)")
+    if np.any(~np.isfinite(hd)) or np.any(hd < 0):
+        raise PlasticityInputError("hardening must be finite and >= 0")
+    f = model.loads() if f is None else _f64(f)
+    n = int(step.increments)
+    at = _f64([t for t, _ in amplitude.points])
+    av = _f64([v for _, v in amplitude.points])
+    u = np.zeros((n + 1, ndof))
+    r = np.zeros((n + 1, ndof))
+    s = np.zeros((n + 1, ne, ngp, 9)) if stresses else None
+    p = np.zeros((n + 1, ne, ngp))
+    res = np.zeros((n, NEWTON_ITERATIONS + 1))
+    cnt = np.zeros(n, dtype=np.int64)
+    info = QuasiStaticInfo()
+    st = lib().ml_model_solve_quasi_static(
+        model._h, _d(sy), _d(hd), _d(f), float(step.time_period), n, len(at), _d(at), _d(av),
+        _d(u), _d(r), _d(s) if stresses else None, _d(p), _d(res), _i(cnt), C.byref(info))
+    try:
+        _check(st, info)
+    except Exception as err:
+        # every failure names the Step increment it ended on, as NotConverged and
+        # PlasticCollapse do in the reference
+        err.increment = info.failed_increment
+        err.cuts = info.cuts
+        raise
+    dt = step.time_step
+    out = []
+    for k in range(n + 1):
+        out.append(Increment(time=(k * dt), displacements=u[k], reactions=r[k],
+                             stresses=s[k] if stresses else None,
+                             equivalent_plastic_strains=p[k],
+                             residuals=res[k - 1, :cnt[k - 1]].copy() if k else np.zeros(0)))
+    return out, info.as_dict()
