@@ -262,6 +262,86 @@ def run_stresses(args, ma, meshes):
           "speedup_vs_cpu": round(cpu["value"] * 1e3 / total, 1) if cpu else None})
 
 
+def run_plastic(args, ma, meshes):
+    """Quasi-static von Mises plasticity (Model::solve_quasi_static) on a Tet4 bar 120x24x24
+    (414,720 elements, 224,375 free DOF) bent by a ramped tip displacement over 5 increments:
+    every Newton iteration's element responses, tangent assembly, IC(0) and PCG on the device.
+    The oracle's Newton (same algorithm, one core) runs the first increment, and its measured
+    costs per Newton iteration (responses, FreeSystem build with IC(0)) and per PCG iteration
+    are scaled to the device run's counts."""
+    from maille_amd import plasticity as P
+    p = meshes.plastic_bar()
+    codes = {"fp64": ma.FP64, "mixed": ma.MIXED, "mixed_ic0": ma.MIXED_IC0}
+    prec = "fp64" if args.precision == "best" else args.precision
+    m = ma.Model(p, precision=codes[prec])
+    sy, hd = 250.0, 2000.0
+    step, amp = P.Step(1.0, args.increments), P.Amplitude.ramp(1.0)
+    incs, _ = P.solve_quasi_static(m, P.Step(1.0, 1), P.Amplitude([(0.0, 0.0), (1.0, 1e-3)]),
+                                   sy, hd, stresses=False)  # warm-up: an elastic increment
+    m.profile(enable=1)
+    t0 = time.perf_counter()
+    incs, info = P.solve_quasi_static(m, step, amp, sy, hd, stresses=False)
+    wall = time.perf_counter() - t0
+    pr = m.profile(enable=0)
+    ne, nen = p.conn.shape
+    ngp = 1 if nen == 4 else 4
+    mm = 3 * nen
+    # k_response's algorithmic bytes per element, each array once: connectivity (int32),
+    # coordinates and displacements of its nodes, E, nu, yield stress, hardening, the committed
+    # states (7 words per Gauss point) in; the tangent (m^2), internal force (m), trial states
+    # and Gauss-point records (9 words per point) out
+    resp_bytes = ne * (4 * nen + 48 * nen + 32 + 56 * ngp + 8 * mm * mm + 8 * mm + 56 * ngp
+                       + 72 * ngp)
+    resp_ms = pr["stiffness"][0] / max(pr["stiffness"][1], 1)
+    tri = pr["precond64"] if prec == "fp64" else pr["precond32"]
+    kernels = {
+        "response": {"avg_ms": round(resp_ms, 4), "launches": pr["stiffness"][1],
+                     "bytes_per_launch": resp_bytes,
+                     "achieved_GBs": round(resp_bytes / (resp_ms * 1e-3) / 1e9, 1)},
+        "assembly": {"avg_ms": round(pr["assembly"][0] / max(pr["assembly"][1], 1), 4)},
+        "ic0": {"avg_ms": round(pr["ic0"][0] / max(pr["ic0"][1], 1), 3), "launches": pr["ic0"][1]},
+        "precondition": {"avg_ms": round(tri[0] / max(tri[1], 1), 4), "launches": tri[1],
+                         "share_of_run": round(tri[0] * 1e-3 / wall, 3)},
+        "spmv": {"avg_ms": round((pr["spmv64"][0] + pr["spmv32"][0]) /
+                                 max(pr["spmv64"][1] + pr["spmv32"][1], 1), 4)},
+    }
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        first = oracle.quasi_static(p.nodes, p.conn, p.young, p.nu, sy, hd, p.fixed, p.imposed,
+                                    p.f, step.time_step, 1, [(0.0, 0.0), (1.0, 1.0)],
+                                    iterative=True)
+        t_resp, t_build, t_solve = first["times"]
+        nwt = max(first["newton"], 1)
+        per_newton = (t_resp + t_build) / nwt
+        per_pcg = t_solve / max(first["pcg"], 1)
+        est = per_newton * info["newton_iterations"] + per_pcg * info["pcg_iterations"]
+        gap = float(np.linalg.norm(incs[1].displacements - first["u"][1]) /
+                    np.linalg.norm(first["u"][1]))
+        cpu = {"value": round(est, 2), "unit": "s", "cores": 1, "kind": "port",
+               "sample": (f"oracle Newton on the first increment ({first['newton']} Newton, "
+                          f"{first['pcg']} PCG iterations measured: {per_newton:.2f} s per Newton "
+                          f"iteration for responses, FreeSystem build and IC(0), "
+                          f"{per_pcg * 1e3:.1f} ms per PCG iteration), scaled to the device "
+                          f"run's counts"),
+               "first_increment_rel_l2": gap}
+    emit({"metric": "quasi-static von Mises Newton-Raphson s, Tet4 bar 120x24x24, 5 increments",
+          "value": round(wall, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+          "data": "synthetic", "dtype": "fp64" if prec == "fp64" else f"fp64+{prec}",
+          "config": {"workload": "plastic_bar_120x24x24", "free_dof": p.nfree,
+                     "elements": int(ne), "increments": args.increments,
+                     "yield_stress": sy, "hardening": hd},
+          "newton_iterations": info["newton_iterations"], "cuts": info["cuts"],
+          "pcg_iterations": info["pcg_iterations"],
+          "residuals": [[float(f"{r:.3e}") for r in inc.residuals] for inc in incs[1:]],
+          "max_p": float(incs[-1].equivalent_plastic_strains.max()),
+          "phases_s": {"responses": round(info["seconds"][0], 3),
+                       "assembly_ic0": round(info["seconds"][1], 3),
+                       "pcg": round(info["seconds"][2], 3)},
+          "kernels": kernels, "cpu_baseline": cpu,
+          "speedup_vs_cpu": round(cpu["value"] / wall, 1) if cpu else None})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,13 +357,14 @@ def main():
                          "against which HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled")
     ap.add_argument("--cpu-iters", type=int, default=8)
     ap.add_argument("--workload", choices=["cube", "beam_cases", "box_batch", "tet10", "simp",
-                                           "stresses"],
+                                           "stresses", "plastic"],
                     default="cube", help="cube: the headline 3.1M-DOF solve; the others are the "
                     "batched, Tet10 and adjoint workloads (one JSON line each)")
     ap.add_argument("--cases", type=int, default=256)
     ap.add_argument("--seq", type=int, default=8, help="sequential solves timed (beam_cases)")
     ap.add_argument("--models", type=int, default=64)
     ap.add_argument("--fd", type=int, default=32, help="elements checked by central differences")
+    ap.add_argument("--increments", type=int, default=5, help="Step increments (plastic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding "
@@ -297,7 +378,8 @@ def main():
 
     if args.workload != "cube":
         {"beam_cases": run_beam_cases, "box_batch": run_box_batch, "tet10": run_tet10,
-         "simp": run_simp, "stresses": run_stresses}[args.workload](args, ma, meshes)
+         "simp": run_simp, "stresses": run_stresses,
+         "plastic": run_plastic}[args.workload](args, ma, meshes)
         return
 
     t0 = time.time()

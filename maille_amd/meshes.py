@@ -351,3 +351,23 @@ def box_batch(count=64, seed=4, lo=500.0, hi=20000.0):
                         (0.0, 0.0, -1.0), (nx, m, m))
         out.append(p)
     return out
+
+
+def plastic_bar(n=(120, 24, 24), size=(600.0, 120.0, 120.0), tip=7.0, young=210000.0, nu=0.3):
+    """A Tet4 bar clamped at x = 0 (all DOFs) and bent by an imposed tip displacement u_z = tip
+    on the x = Lx face (about three times the first-yield deflection 2 sy L^2 / (3 E h) of a
+    steel bar with sy = 250 MPa): a plastic zone grows from the clamp as the amplitude ramps up.
+    Elastic constants per element; the plasticity parameters are the caller's."""
+    nodes, conn = box_grid(n, size)
+    nn = len(nodes)
+    fixed = np.zeros(3 * nn, dtype=bool)
+    imposed = np.zeros(3 * nn)
+    x0 = np.abs(nodes[:, 0]) < 1e-9 * size[0]
+    x1 = np.abs(nodes[:, 0] - size[0]) < 1e-9 * size[0]
+    for c in range(3):
+        fixed[3 * np.nonzero(x0)[0] + c] = True
+    fixed[3 * np.nonzero(x1)[0] + 2] = True
+    imposed[3 * np.nonzero(x1)[0] + 2] = tip
+    ne = len(conn)
+    return Problem(f"plastic_bar_{n[0]}x{n[1]}x{n[2]}", nodes, conn, np.full(ne, young),
+                   np.full(ne, nu), fixed, imposed, np.zeros(3 * nn))
