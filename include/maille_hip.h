@@ -306,7 +306,8 @@ ml_status ml_model_autotune(ml_model* m, int64_t nrhs, int32_t* chosen);
 /* Kernel-duration accounting with HIP events on the model's stream (no counterpart in the
  * reference; for benchmarks). Returns the accumulated milliseconds and launch counts per kind
  * (0 SpMV FP64, 1 SpMV FP32, 2 preconditioner FP64 (forward + backward solve), 3
- * preconditioner FP32, 4 IC(0), 5 element stiffness, 6 gather assembly, 7 PCG vector updates,
+ * preconditioner FP32, 4 IC(0), 5 element stiffness or elasto-plastic response, 6 gather
+ * assembly, 7 PCG vector updates,
  * 8 Gauss-point stresses, 9 nodal stress averaging) into ms[10] and counts[10] (either may be
  * null), then resets and enables (1) or disables (0) accounting; enable < 0 only reads. */
 ml_status ml_model_profile(ml_model* m, int enable, double* ms, int64_t* counts);

@@ -2311,7 +2311,8 @@ void ml_model_destroy(ml_model* h) { delete h; }
 
 // Profiling hooks for the benchmark: accumulate kernel durations measured with HIP events on
 // the handle's stream. kinds: 0 spmv64, 1 spmv32, 2 precondition64 (fwd+bwd), 3
-// precondition32, 4 ic0, 5 stiffness, 6 assembly, 7 vector updates, 8 Gauss-point stresses,
+// precondition32, 4 ic0, 5 stiffness or elasto-plastic response, 6 assembly, 7 vector updates,
+// 8 Gauss-point stresses,
 // 9 nodal averaging.
 ml_status ml_model_profile(ml_model* h, int enable, double* ms, int64_t* counts) {
   return guarded([&]() -> int {
