@@ -271,6 +271,47 @@ ml_status ml_model_solve_quasi_static(ml_model* m, const double* yield_stress,
                                       double* residuals, int64_t* residual_counts,
                                       ml_quasi_static_info* info);
 
+/* ------------------------------------------------------------------ transient heat conduction */
+
+typedef struct ml_thermal ml_thermal;
+
+/* The thermal model of Model::solve_step (model.rs:620-706) on tetrahedra: one DOF (temperature)
+ * per node (model.rs:1074-1078). Per element: conductivity k > 0 and rho*c > 0 (density times
+ * specific heat, material.rs:21-30); convection faces (element, face 0-3 as
+ * Element::face_nodes, film h, sink temperature) whose h int N N^T is added to the element's K
+ * (thermal_matrix, model.rs:1320-1326, assembly.rs:598-624); fixed[n_nodes] (0/1) with the BC
+ * temperatures imposed[n_nodes]. Validates the mesh, builds the free numbering, RCM pattern and
+ * assembly plan on the host and the conduction and consistent capacity matrices (TETRA_4 /
+ * TETRA_14 for the capacity, assembly.rs:413-431) on the device. */
+ml_status ml_thermal_create(int64_t n_nodes, const double* nodes, int64_t n_elements, int nen,
+                            const int64_t* conn, const double* conductivity, const double* rho_c,
+                            int64_t nconv, const int64_t* conv_elements, const int64_t* conv_faces,
+                            const double* film, const double* sink, const uint8_t* fixed,
+                            const double* imposed, ml_precision precision, ml_thermal** out,
+                            ml_solve_info* info);
+/* Model::assemble's thermal F (model.rs:1089-1127) after the nodal heat loads already in
+ * f[n_nodes]: the heat sources (element, q per unit volume; element_heat_source,
+ * assembly.rs:465-484) in order, then the convection loads h T_inf int N, faces grouped per
+ * element in element order. Pass the same convection faces as to ml_thermal_create. Host. */
+ml_status ml_thermal_add_loads(ml_thermal* h, int64_t nsources, const int64_t* elements,
+                               const double* values, int64_t nconv, const int64_t* conv_elements,
+                               const int64_t* conv_faces, const double* film, const double* sink,
+                               double* f);
+/* solve_step_with (model.rs:628-706): `increments` equal steps of time_period with the theta
+ * method, theta in [0.5, 1] (Step::with_theta, model.rs:152-159): (C/dt + theta K) T_{n+1} =
+ * (C/dt - (1 - theta) K) T_n + F, the left matrix assembled and factored (IC(0), shift ladder) on
+ * the device once per (dt, theta), each step a device PCG solve. initial[n_nodes]: the starting
+ * field (the BC temperatures replace it at the fixed nodes). Outputs for increments 0..n, host
+ * arrays: temperatures[(n+1) x n_nodes]; reactions[(n+1) x n_nodes] (C dT/dt + K(theta T_{n+1} +
+ * (1 - theta) T_n) - F on the fixed nodes) or null; fluxes[(n+1) x n_elements x ngp x 6] (x, y,
+ * z, qx, qy, qz with q = -k grad T at the Gauss points, element.rs:478-499) or null.
+ * info->iterations sums the PCG iterations of the steps. */
+ml_status ml_thermal_solve_step(ml_thermal* h, double time_period, int64_t increments,
+                                double theta, const double* f, const double* initial,
+                                double* temperatures, double* reactions, double* fluxes,
+                                ml_solve_info* info);
+void ml_thermal_destroy(ml_thermal* h);
+
 /* ------------------------------------------------------------------ batched solves
  * (no single counterpart in the reference: the equivalent of sequential Model::solve /
  * Pcg::solve calls, model.rs:548 and iterative.rs:210, run together) */

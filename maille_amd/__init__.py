@@ -49,6 +49,7 @@ EXPORTS = (
     "ml_pcg_autotune", "ml_model_autotune",
     "ml_msh_read", "ml_msh_sizes", "ml_msh_geometry", "ml_msh_set_info", "ml_msh_set_items",
     "ml_msh_destroy", "ml_model_solve_quasi_static",
+    "ml_thermal_create", "ml_thermal_add_loads", "ml_thermal_solve_step", "ml_thermal_destroy",
 )
 
 
@@ -204,6 +205,12 @@ def lib():
             "ml_msh_set_info": (_int, [_P, _i64, C.c_char_p, _i64, _I]),
             "ml_msh_set_items": (_int, [_P, _i64, _I, _I, _I, _I]),
             "ml_msh_destroy": (None, [_P]),
+            "ml_thermal_create": (_int, [_i64, _D, _i64, _int, _I, _D, _D, _i64, _I, _I, _D, _D,
+                                         _U8, _D, _int, _PP, _S]),
+            "ml_thermal_add_loads": (_int, [_P, _i64, _I, _D, _i64, _I, _I, _D, _D, _D]),
+            "ml_thermal_solve_step": (_int, [_P, C.c_double, _i64, C.c_double, _D, _D, _D, _D,
+                                             _D, _S]),
+            "ml_thermal_destroy": (None, [_P]),
             "ml_model_solve_quasi_static": (_int, [_P, _D, _D, _D, C.c_double, _i64, _i64, _D,
                                                    _D, _D, _D, _D, _D, _D, _I,
                                                    C.POINTER(QuasiStaticInfo)]),

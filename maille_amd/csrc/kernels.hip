@@ -567,6 +567,214 @@ __global__ __launch_bounds__(256) void k_newton_update(int64_t ndof,
   u[g] += r >= 0 ? x[r] : du[g];
 }
 
+// ------------------------------------------------------------------ transient heat conduction
+// TETRA_14 (quadrature.rs:139-177), the rule of the Tet10 capacity
+__device__ inline void tetra14_point(int q, double& xi, double& eta, double& zeta, double& w) {
+  const double A1 = 0.09273525031089142, W1 = 0.01224884051939372;
+  const double A2 = 0.3108859192633008, W2 = 0.01878132095300283;
+  const double B = 0.04550370412564851, W3 = 0.00709100346284674;
+  const double C1 = 1.0 - 3.0 * A1, C2 = 1.0 - 3.0 * A2, D = 0.5 - B;
+  if (q < 8) {
+    const double a = q < 4 ? A1 : A2, c = q < 4 ? C1 : C2;
+    const int r = q & 3;
+    xi = r == 1 ? c : a;
+    eta = r == 2 ? c : a;
+    zeta = r == 3 ? c : a;
+    w = q < 4 ? W1 : W2;
+    return;
+  }
+  const double pts[6][3] = {{B, B, D}, {B, D, B}, {D, B, B}, {D, D, B}, {D, B, D}, {B, D, D}};
+  xi = pts[q - 8][0];
+  eta = pts[q - 8][1];
+  zeta = pts[q - 8][2];
+  w = W3;
+}
+
+// TETRA_4 at point q (Tet4's capacity rule)
+__device__ inline void tetra4_point(int q, double& xi, double& eta, double& zeta, double& w) {
+  gauss_point(10, q, xi, eta, zeta, w);
+}
+
+template <int NEN>
+__device__ inline void tet_values(double xi, double eta, double zeta, double (&n)[NEN]) {
+#pragma clang fp contract(off)
+  const double l[4] = {1.0 - xi - eta - zeta, xi, eta, zeta};
+  if constexpr (NEN == 4) {
+    for (int a = 0; a < 4; ++a) n[a] = l[a];
+  } else {
+    const int ed[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+    for (int a = 0; a < 4; ++a) n[a] = l[a] * (2.0 * l[a] - 1.0);
+    for (int k = 0; k < 6; ++k) n[4 + k] = 4.0 * l[ed[k][0]] * l[ed[k][1]];
+  }
+}
+
+// conduction (element.rs:450-462, the element's own rule) and consistent capacity
+// (element.rs:465-477 with TETRA_4 for Tet4, TETRA_14 for Tet10; assembly.rs:392-431), one
+// thread per (element e, row i): K[i][j] += w|det| k (g_i . g_j), C[i][j] += w|det| rho_c N_i N_j,
+// Gauss point after Gauss point. Entry (i, j) at slot (i*NEN + j)*ne + e.
+template <int NEN>
+__global__ __launch_bounds__(256) void k_thermal_matrices(int64_t ne, const double* __restrict__ xyz,
+                                                          const int* __restrict__ conn,
+                                                          const double* __restrict__ k,
+                                                          const double* __restrict__ rho_c,
+                                                          double* __restrict__ kmat,
+                                                          double* __restrict__ cmat) {
+#pragma clang fp contract(off)
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)NEN * ne) return;
+  const int i = (int)(t / ne);
+  const int64_t e = t - (int64_t)i * ne;
+  double x[NEN][3];
+  for (int a = 0; a < NEN; ++a) {
+    const int64_t node = conn[e * NEN + a];
+    for (int c = 0; c < 3; ++c) x[a][c] = xyz[3 * node + c];
+  }
+  const double ke = k[e], rc = rho_c[e];
+  double acc[NEN];
+  for (int j = 0; j < NEN; ++j) acc[j] = 0.0;
+  for (int q = 0; q < Rule<NEN>::N; ++q) {
+    double xi, eta, zeta, w;
+    gauss_point(NEN, q, xi, eta, zeta, w);
+    double g[NEN][3];
+    const double weight = w * fabs(tet_mapping<NEN>(x, xi, eta, zeta, g));
+    for (int j = 0; j < NEN; ++j) {
+      double s = 0.0;
+      for (int c = 0; c < 3; ++c) s += g[i][c] * g[j][c];
+      acc[j] += weight * ke * s;
+    }
+  }
+  for (int j = 0; j < NEN; ++j) kmat[((int64_t)i * NEN + j) * ne + e] = acc[j];
+  for (int j = 0; j < NEN; ++j) acc[j] = 0.0;
+  constexpr int NC = NEN == 4 ? 4 : 14;
+  for (int q = 0; q < NC; ++q) {
+    double xi, eta, zeta, w;
+    if constexpr (NEN == 4) tetra4_point(q, xi, eta, zeta, w);
+    else tetra14_point(q, xi, eta, zeta, w);
+    double g[NEN][3], n[NEN];
+    const double weight = w * fabs(tet_mapping<NEN>(x, xi, eta, zeta, g));
+    tet_values<NEN>(xi, eta, zeta, n);
+    for (int j = 0; j < NEN; ++j) acc[j] += weight * rc * n[i] * n[j];
+  }
+  for (int j = 0; j < NEN; ++j) cmat[((int64_t)i * NEN + j) * ne + e] = acc[j];
+}
+
+// Convection face matrices added into K (thermal_matrix, model.rs:1320-1326): element e adds its
+// list (slot, value) in order, the faces in the order the model lists them.
+__global__ __launch_bounds__(256) void k_add_faces(int64_t ne, const int64_t* __restrict__ off,
+                                                   const int64_t* __restrict__ slot,
+                                                   const double* __restrict__ val,
+                                                   double* __restrict__ kmat) {
+#pragma clang fp contract(off)
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  for (int64_t c = off[e]; c < off[e + 1]; ++c) kmat[slot[c]] += val[c];
+}
+
+// model.rs:652-663 (combine): left = C/dt + theta K, right = C/dt + (theta - 1) K
+__global__ __launch_bounds__(256) void k_theta_combine(int64_t n, double dt, double s_left,
+                                                       double s_right,
+                                                       const double* __restrict__ kmat,
+                                                       const double* __restrict__ cmat,
+                                                       double* __restrict__ left,
+                                                       double* __restrict__ right) {
+#pragma clang fp contract(off)
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const double c = cmat[q], k = kmat[q];
+  left[q] = c / dt + s_left * k;
+  right[q] = c / dt + s_right * k;
+}
+
+// product (model.rs:1353-1369) over every DOF, then + F (model.rs:681-682), thread per node: the
+// node's incidences (element e, local node a) in element order each add sum_b M[a][b] T[node b].
+template <int NEN>
+__global__ __launch_bounds__(256) void k_thermal_carry(int64_t n_nodes, int64_t ne,
+                                                       const int64_t* __restrict__ inc_off,
+                                                       const int* __restrict__ inc_item,
+                                                       const int* __restrict__ conn,
+                                                       const double* __restrict__ mat,
+                                                       const double* __restrict__ temp,
+                                                       const double* __restrict__ f,
+                                                       double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n_nodes) return;
+  double y = 0.0;
+  for (int64_t k = inc_off[v]; k < inc_off[v + 1]; ++k) {
+    const int64_t t = inc_item[k];
+    const int64_t e = t / NEN;
+    const int a = (int)(t - e * NEN);
+    double s = 0.0;
+    for (int b = 0; b < NEN; ++b) s += mat[((int64_t)a * NEN + b) * ne + e] * temp[conn[e * NEN + b]];
+    y += s;
+  }
+  out[v] = y + f[v];
+}
+
+// Reactions of a step (model.rs:684-692): product of the left matrix over the fixed DOFs (the
+// rows (e, a) of fixed DOF k in element order), minus the step's right-hand side.
+template <int NEN>
+__global__ __launch_bounds__(256) void k_thermal_reactions(int64_t nfixed, const int* __restrict__ rdof,
+                                                           const int64_t* __restrict__ roff,
+                                                           const int64_t* __restrict__ item,
+                                                           const int* __restrict__ conn, int64_t ne,
+                                                           const double* __restrict__ left,
+                                                           const double* __restrict__ temp,
+                                                           const double* __restrict__ rhs,
+                                                           double* __restrict__ reactions) {
+#pragma clang fp contract(off)
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nfixed) return;
+  double y = 0.0;
+  for (int64_t c = roff[k]; c < roff[k + 1]; ++c) {
+    const int64_t e = item[c] / NEN;
+    const int a = (int)(item[c] - e * NEN);
+    double s = 0.0;
+    for (int b = 0; b < NEN; ++b) s += left[((int64_t)a * NEN + b) * ne + e] * temp[conn[e * NEN + b]];
+    y += s;
+  }
+  const int g = rdof[k];
+  reactions[g] = y - rhs[g];
+}
+
+// heat_fluxes (element.rs:478-499), thread per (element, Gauss point): x, y, z, then
+// q = -k grad T.
+template <int NEN>
+__global__ __launch_bounds__(256) void k_heat_flux(int64_t ne, const double* __restrict__ xyz,
+                                                   const int* __restrict__ conn,
+                                                   const double* __restrict__ k,
+                                                   const double* __restrict__ temp,
+                                                   double* __restrict__ out) {
+#pragma clang fp contract(off)
+  constexpr int NG = Rule<NEN>::N;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ne * NG) return;
+  const int64_t e = t / NG;
+  const int q = (int)(t - e * NG);
+  double x[NEN][3], te[NEN];
+  for (int a = 0; a < NEN; ++a) {
+    const int64_t node = conn[e * NEN + a];
+    for (int c = 0; c < 3; ++c) x[a][c] = xyz[3 * node + c];
+    te[a] = temp[node];
+  }
+  double xi, eta, zeta, w;
+  gauss_point(NEN, q, xi, eta, zeta, w);
+  double g[NEN][3], n[NEN];
+  tet_mapping<NEN>(x, xi, eta, zeta, g);
+  tet_values<NEN>(xi, eta, zeta, n);
+  double* o = out + t * 6;
+  for (int c = 0; c < 3; ++c) {
+    double s = 0.0;
+    for (int a = 0; a < NEN; ++a) s += n[a] * x[a][c];
+    o[c] = s;
+  }
+  for (int c = 0; c < 3; ++c) {
+    double s = 0.0;
+    for (int a = 0; a < NEN; ++a) s += g[a][c] * te[a];
+    o[3 + c] = -k[e] * s;
+  }
+}
+
 // ------------------------------------------------------------------ assembly
 // Csr::add in the reference's scatter order (model.rs:1480-1492): entry q sums its element
 // contributions in element order, starting from 0.0.

@@ -34,6 +34,12 @@ void parallel_for(i64 n, F body) {
   for (auto& th : pool) th.join();
 }
 
+// element_dofs (assembly.rs:20-32, Tet branch): [3n, 3n+1, 3n+2] per node, or the element's
+// nodes when the model carries one DOF per node (temperature, model.rs:1074-1078)
+inline i64 dof_of(const MeshView& m, i64 e, int local) {
+  return m.dpn == 3 ? 3 * m.conn[e * m.nen + local / 3] + local % 3 : m.conn[e * m.nen + local];
+}
+
 }  // namespace
 
 int check_mesh(const MeshView& m, i64* where) {
@@ -83,8 +89,9 @@ int build_plan(const MeshView& m, const uint8_t* fixed, const double* imposed, P
   p.n_nodes = m.n_nodes;
   p.n_elements = m.n_elements;
   p.nen = m.nen;
-  p.m = 3 * m.nen;
-  p.ndof = 3 * m.n_nodes;
+  p.dpn = m.dpn;
+  p.m = m.dpn * m.nen;
+  p.ndof = m.dpn * m.n_nodes;
   const int mm = p.m;
   const i64 ne = m.n_elements;
   if (p.ndof >= (i64)INT32_MAX || (i64)mm * mm * ne >= (i64)UINT32_MAX) {
@@ -105,7 +112,7 @@ int build_plan(const MeshView& m, const uint8_t* fixed, const double* imposed, P
   std::vector<i64> eoff(ne + 1, 0);
   for (i64 e = 0; e < ne; ++e) {
     i64 c = 0;
-    for (int a = 0; a < mm; ++a) c += !fixed[3 * m.conn[e * m.nen + a / 3] + a % 3];
+    for (int a = 0; a < mm; ++a) c += !fixed[dof_of(m, e, a)];
     eoff[e + 1] = eoff[e] + c;
   }
   std::vector<i64> eit(eoff[ne]);
@@ -113,7 +120,7 @@ int build_plan(const MeshView& m, const uint8_t* fixed, const double* imposed, P
     for (i64 e = b; e < en; ++e) {
       i64 o = eoff[e];
       for (int a = 0; a < mm; ++a) {
-        i64 f = free_of[3 * m.conn[e * m.nen + a / 3] + a % 3];
+        i64 f = free_of[dof_of(m, e, a)];
         if (f >= 0) eit[o++] = f;
       }
     }
@@ -148,8 +155,8 @@ int build_plan(const MeshView& m, const uint8_t* fixed, const double* imposed, P
       int k = 0;
       for (int li = 0; li < mm; ++li)
         for (int lj = li; lj < mm; ++lj, ++k) {
-          int32_t ra = p.rank_of_dof[3 * m.conn[e * m.nen + li / 3] + li % 3];
-          int32_t rb = p.rank_of_dof[3 * m.conn[e * m.nen + lj / 3] + lj % 3];
+          int32_t ra = p.rank_of_dof[dof_of(m, e, li)];
+          int32_t rb = p.rank_of_dof[dof_of(m, e, lj)];
           if (ra < 0 || rb < 0) continue;
           // the reference adds ke[row][col] where row is the DOF of larger rank
           i64 q = p.lower.find(std::max(ra, rb), std::min(ra, rb));
@@ -192,10 +199,10 @@ int build_plan(const MeshView& m, const uint8_t* fixed, const double* imposed, P
   if (any_imposed) {
     for (i64 e = 0; e < ne; ++e)
       for (int li = 0; li < mm; ++li) {
-        int32_t ra = p.rank_of_dof[3 * m.conn[e * m.nen + li / 3] + li % 3];
+        int32_t ra = p.rank_of_dof[dof_of(m, e, li)];
         if (ra < 0) continue;
         for (int lj = 0; lj < mm; ++lj) {
-          i64 gj = 3 * m.conn[e * m.nen + lj / 3] + lj % 3;
+          i64 gj = dof_of(m, e, lj);
           if (fixed[gj]) p.cpl_off[ra + 1] += 1;
         }
       }
@@ -205,10 +212,10 @@ int build_plan(const MeshView& m, const uint8_t* fixed, const double* imposed, P
     std::vector<int64_t> cursor(p.cpl_off.begin(), p.cpl_off.end() - 1);
     for (i64 e = 0; e < ne; ++e)
       for (int li = 0; li < mm; ++li) {
-        int32_t ra = p.rank_of_dof[3 * m.conn[e * m.nen + li / 3] + li % 3];
+        int32_t ra = p.rank_of_dof[dof_of(m, e, li)];
         if (ra < 0) continue;
         for (int lj = 0; lj < mm; ++lj) {
-          i64 gj = 3 * m.conn[e * m.nen + lj / 3] + lj % 3;
+          i64 gj = dof_of(m, e, lj);
           if (!fixed[gj]) continue;
           i64 c = cursor[ra]++;
           p.cpl_slot[c] = (uint32_t)((i64)(li * mm + lj) * ne + e);
@@ -227,7 +234,7 @@ int build_plan(const MeshView& m, const uint8_t* fixed, const double* imposed, P
   p.rea_off.assign(p.rea_dof.size() + 1, 0);
   for (i64 e = 0; e < ne; ++e)
     for (int li = 0; li < mm; ++li) {
-      int32_t k = fixed_index[3 * m.conn[e * m.nen + li / 3] + li % 3];
+      int32_t k = fixed_index[dof_of(m, e, li)];
       if (k >= 0) p.rea_off[k + 1] += 1;
     }
   for (size_t k = 0; k < p.rea_dof.size(); ++k) p.rea_off[k + 1] += p.rea_off[k];
@@ -236,7 +243,7 @@ int build_plan(const MeshView& m, const uint8_t* fixed, const double* imposed, P
     std::vector<int64_t> cursor(p.rea_off.begin(), p.rea_off.end() - 1);
     for (i64 e = 0; e < ne; ++e)
       for (int li = 0; li < mm; ++li) {
-        int32_t k = fixed_index[3 * m.conn[e * m.nen + li / 3] + li % 3];
+        int32_t k = fixed_index[dof_of(m, e, li)];
         if (k >= 0) p.rea_item[cursor[k]++] = e * mm + li;
       }
   }

@@ -18,6 +18,7 @@ struct MeshView {
   const i64* conn = nullptr;      // n_elements x nen
   const double* young = nullptr;
   const double* nu = nullptr;
+  int dpn = 3;                    // DOFs per node: 3 (displacements) or 1 (temperature)
 };
 
 // mesh.rs:257-348 + material.rs:145-153: returns 0 or a mesh error code (see maille_hip.h)
@@ -26,7 +27,8 @@ int check_mesh(const MeshView& m, i64* where);
 
 struct Plan {
   i64 n_nodes = 0, n_elements = 0, ndof = 0, nfree = 0;
-  int nen = 0, m = 0;  // m = 3 nen
+  int nen = 0, m = 0;  // m = dpn nen
+  int dpn = 3;
   std::vector<int32_t> rank_of_dof;  // global DOF -> solver row (RCM rank) or -1 if fixed
   std::vector<i64> order;            // RCM order over free indices (order[new] = old)
   LowerPattern lower;
@@ -48,7 +50,7 @@ struct Plan {
   double seconds = 0.0;
 };
 
-// FreeSystem::new up to the scatter (model.rs:1442-1499), on the host. fixed: 3 n_nodes flags.
+// FreeSystem::new up to the scatter (model.rs:1442-1499), on the host. fixed: dpn n_nodes flags.
 // Returns 0 or an ml_status code with a message in *err.
 int build_plan(const MeshView& m, const uint8_t* fixed, const double* imposed, Plan& plan,
                std::string* err);

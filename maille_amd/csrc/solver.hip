@@ -1510,6 +1510,308 @@ void face_pressure_add(int nen, const int64_t* enodes, const double* xyz, int fa
     for (int c = 0; c < 3; ++c) f[3 * enodes[pos[a]] + c] += forces[3 * a + c];
 }
 
+// tet4 / tet10 values and Mapping3d's determinant at (xi, eta, zeta) on the host (shape.rs:95-190)
+void tet_point_host(int nen, const double (*x)[3], double xi, double eta, double zeta, double* n,
+                    double* det) {
+  const double dl[4][3] = {{-1.0, -1.0, -1.0}, {1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+  const double l[4] = {1.0 - xi - eta - zeta, xi, eta, zeta};
+  const int ed[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+  double dn[10][3];
+  if (nen == 4) {
+    for (int a = 0; a < 4; ++a) {
+      n[a] = l[a];
+      for (int c = 0; c < 3; ++c) dn[a][c] = dl[a][c];
+    }
+  } else {
+    for (int a = 0; a < 4; ++a) {
+      n[a] = l[a] * (2.0 * l[a] - 1.0);
+      for (int c = 0; c < 3; ++c) dn[a][c] = (4.0 * l[a] - 1.0) * dl[a][c];
+    }
+    for (int k2 = 0; k2 < 6; ++k2) {
+      n[4 + k2] = 4.0 * l[ed[k2][0]] * l[ed[k2][1]];
+      for (int c = 0; c < 3; ++c)
+        dn[4 + k2][c] = 4.0 * (l[ed[k2][0]] * dl[ed[k2][1]][c] + l[ed[k2][1]] * dl[ed[k2][0]][c]);
+    }
+  }
+  double j[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int a = 0; a < nen; ++a)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) j[r][c] += dn[a][r] * x[a][c];
+  double d = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+    d += j[0][c] * (j[1][c1] * j[2][c2] - j[1][c2] * j[2][c1]);
+  }
+  *det = d;
+}
+
+// face_convection (assembly.rs:598-624) on the face rule of assembly.rs:527-593 (TRIANGLE_3 on a
+// Tet4 face, TRIANGLE_6 on a Tet10 face, weights w 2A), on the host: the face's node positions
+// in the element, matrix[a][b] = sum film w N_a N_b and load[a] = sum film sink w N_a.
+int face_convection_terms(int nen, const int64_t* enodes, const double* xyz, int face,
+                          double film, double sink, int* pos, double* matrix, double* load) {
+  static const int FACES[4][3] = {{0, 1, 2}, {0, 3, 1}, {1, 3, 2}, {2, 3, 0}};
+  static const int EDGES[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+  auto mid = [](int a, int b) {
+    for (int k = 0; k < 6; ++k)
+      if ((EDGES[k][0] == a && EDGES[k][1] == b) || (EDGES[k][0] == b && EDGES[k][1] == a))
+        return 4 + k;
+    return 4;
+  };
+  const int* fc = FACES[face];
+  pos[0] = fc[0]; pos[1] = fc[1]; pos[2] = fc[2];
+  int nn = 3;
+  if (nen == 10) {
+    pos[3] = mid(fc[0], fc[1]); pos[4] = mid(fc[1], fc[2]); pos[5] = mid(fc[2], fc[0]);
+    nn = 6;
+  }
+  auto X = [&](int local, int k) { return xyz[3 * enodes[local] + k]; };
+  double u[3], v[3];
+  for (int k = 0; k < 3; ++k) {
+    u[k] = X(pos[1], k) - X(pos[0], k);
+    v[k] = X(pos[2], k) - X(pos[0], k);
+  }
+  double cross[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+  double sq = 0.0;
+  for (int k = 0; k < 3; ++k) sq += cross[k] * cross[k];
+  const double twice_area = std::sqrt(sq);
+  const double A6 = 0.445948490915965, WA = 0.223381589678011 / 2.0;
+  const double B6 = 0.091576213509771, WB = 0.109951743655322 / 2.0;
+  const double tri3[3][3] = {{1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0},
+                             {2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0},
+                             {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0}};
+  const double tri6[6][3] = {{A6, A6, WA}, {1.0 - 2.0 * A6, A6, WA}, {A6, 1.0 - 2.0 * A6, WA},
+                             {B6, B6, WB}, {1.0 - 2.0 * B6, B6, WB}, {B6, 1.0 - 2.0 * B6, WB}};
+  std::fill(matrix, matrix + nn * nn, 0.0);
+  std::fill(load, load + nn, 0.0);
+  const int ng = nn == 3 ? 3 : 6;
+  for (int g = 0; g < ng; ++g) {
+    const double* gp = nn == 3 ? tri3[g] : tri6[g];
+    const double w = gp[2] * twice_area;
+    const double xi = gp[0], eta = gp[1];
+    double nv[6];
+    if (nn == 3) {
+      nv[0] = 1.0 - xi - eta; nv[1] = xi; nv[2] = eta;
+    } else {
+      const double l1 = 1.0 - xi - eta, l2 = xi, l3 = eta;
+      nv[0] = l1 * (2.0 * l1 - 1.0);
+      nv[1] = l2 * (2.0 * l2 - 1.0);
+      nv[2] = l3 * (2.0 * l3 - 1.0);
+      nv[3] = 4.0 * l1 * l2;
+      nv[4] = 4.0 * l2 * l3;
+      nv[5] = 4.0 * l3 * l1;
+    }
+    for (int a = 0; a < nn; ++a) {
+      load[a] += film * sink * w * nv[a];
+      for (int b = 0; b < nn; ++b) matrix[a * nn + b] += film * w * nv[a] * nv[b];
+    }
+  }
+  return nn;
+}
+
+namespace ml {
+// Model::solve_step_with (model.rs:628-706) for a thermal model on tetrahedra: one DOF
+// (temperature) per node, the theta method (C/dt + theta K) T_{n+1} = (C/dt - (1-theta) K) T_n + F
+// with K = conduction + convection faces (thermal_matrix, model.rs:1320-1326) and the consistent
+// capacity C. The element matrices, the left matrix's gather assembly and its IC(0) are built on
+// the device once per (dt, theta); every step is a carried-term product, a PCG solve on the
+// device and the step's reactions and heat fluxes.
+struct Thermal {
+  MeshView mesh;
+  std::vector<double> nodes, k, rho_c, zeros, imposed;
+  std::vector<int64_t> conn;
+  std::vector<uint8_t> fixed;
+  Plan plan;
+  Operator op;
+  DBuf<double> d_xyz, d_k, d_rhoc, d_kmat, d_cmat, d_left, d_right, d_imposed, d_coupling, d_f,
+      d_t, d_carry, d_x, d_rhs, d_R, d_q, d_fval;
+  DBuf<int> d_conn, d_rank, d_rea_dof, d_inc_item;
+  DBuf<int64_t> d_asm_off, d_rea_off, d_rea_item, d_cpl_off, d_inc_off, d_foff, d_fslot;
+  DBuf<uint32_t> d_asm_slot, d_cpl_slot;
+  DBuf<int> d_cpl_dof;
+  double dt = 0.0, theta = 0.0;
+  bool factored = false;
+
+  hipStream_t s() { return op.stream; }
+
+  int create(int64_t n_nodes, const double* xyz, int64_t ne, int nen, const int64_t* cn,
+             const double* kk, const double* rc, int64_t nconv, const int64_t* conv_el,
+             const int64_t* conv_face, const double* film, const double* sink,
+             const uint8_t* fx, const double* imp, int precision, ml_solve_info* info) {
+    if (nen != 4 && nen != 10) throw ArgError("nen must be 4 (Tet4) or 10 (Tet10)");
+    nodes.assign(xyz, xyz + 3 * n_nodes);
+    conn.assign(cn, cn + ne * nen);
+    k.assign(kk, kk + ne);
+    rho_c.assign(rc, rc + ne);
+    zeros.assign(ne, 0.0);
+    fixed.assign(fx, fx + n_nodes);
+    imposed.assign(n_nodes, 0.0);
+    if (imp) imposed.assign(imp, imp + n_nodes);
+    for (int64_t e = 0; e < ne; ++e)
+      if (!(std::isfinite(rc[e]) && rc[e] > 0.0))
+        throw ArgError("density and specific heat must be given (rho c > 0) for every element");
+    // conductivity plays the part of E in the mesh checks (finite, > 0), with nu = 0
+    mesh = MeshView{n_nodes, ne, nen, nodes.data(), conn.data(), k.data(), zeros.data(), 1};
+    i64 where = -1;
+    int merr = check_mesh(mesh, &where);
+    if (info) { info->mesh_error = merr; info->bad_element = where; }
+    if (merr) return ML_INVALID_MESH;
+    std::string err;
+    if (build_plan(mesh, fixed.data(), imposed.data(), plan, &err)) throw ArgError(err);
+    if (info) info->seconds[5] = plan.seconds;
+    op.precision = precision;
+    op.set_pattern(plan.lower);
+    hipStream_t st = s();
+    d_xyz.upload_vec(nodes, st);
+    std::vector<int> c32(conn.begin(), conn.end());
+    d_conn.upload_vec(c32, st);
+    d_k.upload_vec(k, st);
+    d_rhoc.upload_vec(rho_c, st);
+    d_imposed.upload_vec(imposed, st);
+    d_rank.upload_vec(plan.rank_of_dof, st);
+    d_asm_off.upload_vec(plan.asm_off, st);
+    d_asm_slot.upload_vec(plan.asm_slot, st);
+    d_cpl_off.upload_vec(plan.cpl_off, st);
+    d_cpl_slot.upload_vec(plan.cpl_slot, st);
+    d_cpl_dof.upload_vec(plan.cpl_dof, st);
+    d_rea_dof.upload_vec(plan.rea_dof, st);
+    d_rea_off.upload_vec(plan.rea_off, st);
+    d_rea_item.upload_vec(plan.rea_item, st);
+    // node -> (element, local node) incidences in element order
+    std::vector<int64_t> off(n_nodes + 1, 0);
+    for (int64_t t = 0; t < ne * nen; ++t) off[conn[t] + 1] += 1;
+    for (int64_t v = 0; v < n_nodes; ++v) off[v + 1] += off[v];
+    std::vector<int> item(ne * nen);
+    std::vector<int64_t> at(off.begin(), off.end() - 1);
+    for (int64_t t = 0; t < ne * nen; ++t) item[at[conn[t]]++] = (int)t;
+    d_inc_off.upload_vec(off, st);
+    d_inc_item.upload_vec(item, st);
+    // convection faces grouped per element, in the given order: (K slot, value) additions
+    std::vector<int64_t> foff(ne + 1, 0), fslot;
+    std::vector<double> fval;
+    {
+      std::vector<std::vector<int64_t>> per(ne);
+      for (int64_t c = 0; c < nconv; ++c) {
+        if (conv_el[c] < 0 || conv_el[c] >= ne || conv_face[c] < 0 || conv_face[c] > 3)
+          throw ArgError("convection face out of range");
+        if (!(std::isfinite(film[c]) && std::isfinite(sink[c])))
+          throw ArgError("convection film and sink must be finite");
+        per[conv_el[c]].push_back(c);
+      }
+      int pos[6];
+      double mat[36], load[6];
+      for (int64_t e = 0; e < ne; ++e) {
+        for (int64_t c : per[e]) {
+          const int nn = face_convection_terms(nen, conn.data() + e * nen, nodes.data(),
+                                               (int)conv_face[c], film[c], sink[c], pos, mat, load);
+          for (int a = 0; a < nn; ++a)
+            for (int b = 0; b < nn; ++b) {
+              fslot.push_back(((int64_t)pos[a] * nen + pos[b]) * ne + e);
+              fval.push_back(mat[a * nn + b]);
+            }
+        }
+        foff[e + 1] = (int64_t)fslot.size();
+      }
+    }
+    d_foff.upload_vec(foff, st);
+    d_fslot.upload_vec(fslot, st);
+    d_fval.upload_vec(fval, st);
+    const size_t mm = (size_t)nen * nen * ne;
+    for (auto* b : {&d_kmat, &d_cmat, &d_left, &d_right}) b->alloc(mm);
+    d_coupling.alloc(plan.nfree);
+    d_x.alloc(plan.nfree);
+    d_rhs.alloc(plan.nfree);
+    for (auto* b : {&d_f, &d_t, &d_carry, &d_R}) b->alloc(n_nodes);
+    d_q.alloc((size_t)6 * (nen == 4 ? 1 : 4) * ne);
+    if (nen == 4)
+      mlk::k_thermal_matrices<4><<<grid_for(4 * ne), 256, 0, st>>>(ne, d_xyz.p, d_conn.p, d_k.p,
+                                                                  d_rhoc.p, d_kmat.p, d_cmat.p);
+    else
+      mlk::k_thermal_matrices<10><<<grid_for(10 * ne), 256, 0, st>>>(ne, d_xyz.p, d_conn.p, d_k.p,
+                                                                    d_rhoc.p, d_kmat.p, d_cmat.p);
+    HIP_CHECK(hipGetLastError());
+    if (!fslot.empty()) {
+      mlk::k_add_faces<<<grid_for(ne), 256, 0, st>>>(ne, d_foff.p, d_fslot.p, d_fval.p, d_kmat.p);
+      HIP_CHECK(hipGetLastError());
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<uint32_t>().swap(plan.asm_slot);
+    return ML_OK;
+  }
+
+  // FreeSystem::new on the left matrix (model.rs:664-671), once per (dt, theta)
+  int prepare(double step_dt, double step_theta) {
+    if (factored && step_dt == dt && step_theta == theta) return ML_OK;
+    factored = false;
+    hipStream_t st = s();
+    const int64_t ne = plan.n_elements, mm = (int64_t)plan.nen * plan.nen * ne;
+    mlk::k_theta_combine<<<grid_for(mm), 256, 0, st>>>(mm, step_dt, step_theta, step_theta - 1.0,
+                                                       d_kmat.p, d_cmat.p, d_left.p, d_right.p);
+    mlk::k_assemble<<<grid_for(op.nnz), 256, 0, st>>>(op.nnz, d_asm_off.p, d_asm_slot.p, d_left.p,
+                                                      op.d_a.p);
+    HIP_CHECK(hipGetLastError());
+    if (!plan.cpl_slot.empty()) {
+      mlk::k_coupling<<<grid_for(plan.nfree), 256, 0, st>>>(plan.nfree, d_cpl_off.p, d_cpl_slot.p,
+                                                            d_cpl_dof.p, d_left.p, d_imposed.p,
+                                                            d_coupling.p);
+      HIP_CHECK(hipGetLastError());
+    }
+    op.values_changed();
+    const int r = op.factor();
+    if (r != ML_OK) return r;
+    dt = step_dt;
+    theta = step_theta;
+    factored = true;
+    return ML_OK;
+  }
+
+  void fluxes(double* out_dev) {
+    const int64_t ne = plan.n_elements;
+    if (plan.nen == 4)
+      mlk::k_heat_flux<4><<<grid_for(ne), 256, 0, s()>>>(ne, d_xyz.p, d_conn.p, d_k.p, d_t.p, out_dev);
+    else
+      mlk::k_heat_flux<10><<<grid_for(4 * ne), 256, 0, s()>>>(ne, d_xyz.p, d_conn.p, d_k.p, d_t.p,
+                                                              out_dev);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // one step from d_t: carried + F, the solve, the new temperatures, reactions into d_R
+  int step(ml_solve_info* info) {
+    hipStream_t st = s();
+    const int64_t nn = plan.n_nodes, ne = plan.n_elements;
+    if (plan.nen == 4)
+      mlk::k_thermal_carry<4><<<grid_for(nn), 256, 0, st>>>(nn, ne, d_inc_off.p, d_inc_item.p,
+                                                           d_conn.p, d_right.p, d_t.p, d_f.p,
+                                                           d_carry.p);
+    else
+      mlk::k_thermal_carry<10><<<grid_for(nn), 256, 0, st>>>(nn, ne, d_inc_off.p, d_inc_item.p,
+                                                            d_conn.p, d_right.p, d_t.p, d_f.p,
+                                                            d_carry.p);
+    mlk::k_rhs<<<grid_for(nn), 256, 0, st>>>(nn, d_rank.p, d_carry.p,
+                                             plan.cpl_slot.empty() ? nullptr : d_coupling.p,
+                                             d_rhs.p);
+    HIP_CHECK(hipGetLastError());
+    const int r = op.solve(d_rhs.p, d_x.p, 0, 0, info);
+    if (r != ML_OK) return r;
+    mlk::k_scatter_u<<<grid_for(nn), 256, 0, st>>>(nn, d_rank.p, d_x.p, d_t.p, d_t.p);
+    mlk::k_fill_f64<<<grid_for(nn), 256, 0, st>>>(d_R.p, nn, 0.0);
+    const int64_t nfixed = (int64_t)plan.rea_dof.size();
+    if (nfixed) {
+      if (plan.nen == 4)
+        mlk::k_thermal_reactions<4><<<grid_for(nfixed), 256, 0, st>>>(
+            nfixed, d_rea_dof.p, d_rea_off.p, d_rea_item.p, d_conn.p, ne, d_left.p, d_t.p,
+            d_carry.p, d_R.p);
+      else
+        mlk::k_thermal_reactions<10><<<grid_for(nfixed), 256, 0, st>>>(
+            nfixed, d_rea_dof.p, d_rea_off.p, d_rea_item.p, d_conn.p, ne, d_left.p, d_t.p,
+            d_carry.p, d_R.p);
+    }
+    HIP_CHECK(hipGetLastError());
+    return ML_OK;
+  }
+};
+}  // namespace ml
+
 // ====================================================================== C ABI
 using namespace ml;
 
@@ -2122,6 +2424,163 @@ ml_status ml_model_solve_quasi_static(ml_model* h, const double* yield_stress,
     return st;
   });
 }
+
+// ------------------------------------------------------------------ transient heat conduction
+struct ml_thermal {
+  Thermal t;
+};
+
+ml_status ml_thermal_create(int64_t n_nodes, const double* nodes, int64_t n_elements, int nen,
+                            const int64_t* conn, const double* conductivity, const double* rho_c,
+                            int64_t nconv, const int64_t* conv_elements, const int64_t* conv_faces,
+                            const double* film, const double* sink, const uint8_t* fixed,
+                            const double* imposed, ml_precision precision, ml_thermal** out,
+                            ml_solve_info* info) {
+  return guarded([&]() -> int {
+    if (!out || !nodes || !conn || !conductivity || !rho_c || !fixed) throw ArgError("null argument");
+    if (nconv > 0 && (!conv_elements || !conv_faces || !film || !sink)) throw ArgError("null argument");
+    if (precision < ML_FP64 || precision > ML_MIXED_IC0) throw ArgError("bad precision");
+    if (info) {
+      std::memset(info, 0, sizeof(*info));
+      info->bad_element = -1;
+    }
+    std::unique_ptr<ml_thermal> h(new ml_thermal);
+    int st = h->t.create(n_nodes, nodes, n_elements, nen, conn, conductivity, rho_c, nconv,
+                         conv_elements, conv_faces, film, sink, fixed, imposed, precision, info);
+    if (st != ML_OK) return st;
+    *out = h.release();
+    return ML_OK;
+  });
+}
+
+ml_status ml_thermal_add_loads(ml_thermal* h, int64_t nsources, const int64_t* elements,
+                               const double* values, int64_t nconv, const int64_t* conv_elements,
+                               const int64_t* conv_faces, const double* film, const double* sink,
+                               double* f) {
+  return guarded([&]() -> int {
+    if (!h || !f) throw ArgError("null argument");
+    Thermal& t = h->t;
+    const int nen = t.plan.nen;
+    const int64_t ne = t.plan.n_elements;
+    // assembly.rs:465-484 (element_heat_source): f_i += w |det J| N_i q on the element's rule
+    for (int64_t s2 = 0; s2 < nsources; ++s2) {
+      const int64_t e = elements[s2];
+      if (e < 0 || e >= ne) throw ArgError("heat source element out of range");
+      if (!std::isfinite(values[s2])) throw ArgError("heat source must be finite");
+      const int64_t* en = t.conn.data() + e * nen;
+      double x[10][3];
+      for (int a = 0; a < nen; ++a)
+        for (int c = 0; c < 3; ++c) x[a][c] = t.nodes[3 * en[a] + c];
+      double fe[10] = {0};
+      const int ng = nen == 4 ? 1 : 4;
+      const double A = 0.585410196624968500, B = 0.138196601125010500;
+      for (int q = 0; q < ng; ++q) {
+        double xi = 0.25, eta = 0.25, zeta = 0.25, w = 1.0 / 6.0;
+        if (nen == 10) {
+          xi = q == 1 ? A : B; eta = q == 2 ? A : B; zeta = q == 3 ? A : B; w = 1.0 / 24.0;
+        }
+        double n[10], det;
+        tet_point_host(nen, x, xi, eta, zeta, n, &det);
+        const double weight = w * std::fabs(det);
+        for (int a = 0; a < nen; ++a) fe[a] += weight * n[a] * values[s2];
+      }
+      for (int a = 0; a < nen; ++a) f[en[a]] += fe[a];
+    }
+    // model.rs:1122-1126: convection loads, faces grouped per element in element order
+    std::vector<int64_t> order(nconv);
+    for (int64_t c = 0; c < nconv; ++c) {
+      order[c] = c;
+      if (conv_elements[c] < 0 || conv_elements[c] >= ne || conv_faces[c] < 0 || conv_faces[c] > 3)
+        throw ArgError("convection face out of range");
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return conv_elements[a] < conv_elements[b]; });
+    int pos[6];
+    double mat[36], load[6];
+    for (int64_t c : order) {
+      const int64_t* en = t.conn.data() + conv_elements[c] * nen;
+      const int nn = face_convection_terms(nen, en, t.nodes.data(), (int)conv_faces[c], film[c],
+                                           sink[c], pos, mat, load);
+      for (int a = 0; a < nn; ++a) f[en[pos[a]]] += load[a];
+    }
+    return ML_OK;
+  });
+}
+
+ml_status ml_thermal_solve_step(ml_thermal* h, double time_period, int64_t increments,
+                                double theta, const double* f, const double* initial,
+                                double* temperatures, double* reactions, double* fluxes,
+                                ml_solve_info* info) {
+  return guarded([&]() -> int {
+    if (!h || !f || !initial || !temperatures) throw ArgError("null argument");
+    if (!(std::isfinite(time_period) && time_period > 0.0))
+      throw ArgError("the Step's time period must be finite and > 0");
+    if (increments <= 0) throw ArgError("a Step needs at least one increment");
+    if (!(theta >= 0.5 && theta <= 1.0)) throw ArgError("theta must lie in [0.5, 1]");
+    Thermal& t = h->t;
+    const int64_t nn = t.plan.n_nodes, ne = t.plan.n_elements;
+    for (int64_t v = 0; v < nn; ++v)
+      if (!std::isfinite(initial[v])) throw ArgError("initial temperatures must be finite");
+    if (info) {
+      std::memset(info, 0, sizeof(*info));
+      info->bad_element = -1;
+    }
+    const double t0 = now_s();
+    const double dt = time_period / (double)increments;  // Step::time_step (model.rs:171-173)
+    int st = t.prepare(dt, theta);
+    if (info) info->shift_index = t.op.shift_index;
+    if (st != ML_OK) return st;
+    const double t1 = now_s();
+    hipStream_t s = t.s();
+    // model.rs:666-669: the initial field, the BC values at the fixed nodes
+    std::vector<double> t_init(initial, initial + nn);
+    for (int64_t v = 0; v < nn; ++v)
+      if (t.fixed[v]) t_init[v] = t.imposed[v];
+    t.d_t.upload_vec(t_init, s);
+    t.d_f.upload(f, nn, s);
+    const size_t nq = (size_t)6 * (t.plan.nen == 4 ? 1 : 4) * ne;
+    auto record = [&](int64_t k) {
+      HIP_CHECK(hipMemcpyAsync(temperatures + k * nn, t.d_t.p, sizeof(double) * nn,
+                               hipMemcpyDeviceToHost, s));
+      if (reactions) {
+        if (k == 0) std::fill(reactions, reactions + nn, 0.0);
+        else HIP_CHECK(hipMemcpyAsync(reactions + k * nn, t.d_R.p, sizeof(double) * nn,
+                                      hipMemcpyDeviceToHost, s));
+      }
+      if (fluxes) {
+        t.fluxes(t.d_q.p);
+        HIP_CHECK(hipMemcpyAsync(fluxes + k * nq, t.d_q.p, sizeof(double) * nq,
+                                 hipMemcpyDeviceToHost, s));
+      }
+    };
+    record(0);
+    int64_t iters = 0;
+    for (int64_t k = 1; k <= increments; ++k) {
+      ml_solve_info si;
+      std::memset(&si, 0, sizeof(si));
+      st = t.step(&si);
+      iters += si.iterations;
+      if (info) {
+        info->device_retries += si.device_retries;
+        info->residual = si.residual;
+      }
+      if (st != ML_OK) {
+        if (info) info->iterations = iters;
+        return st;
+      }
+      record(k);
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (info) {
+      info->iterations = iters;
+      info->seconds[1] = t1 - t0;
+      info->seconds[2] = now_s() - t1;
+    }
+    return ML_OK;
+  });
+}
+
+void ml_thermal_destroy(ml_thermal* h) { delete h; }
 
 ml_status ml_model_add_pressures(ml_model* h, int64_t count, const int64_t* elements,
                                  const int64_t* faces, const double* values, double* f) {

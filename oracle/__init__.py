@@ -102,6 +102,15 @@ def _declare(L):
                                    _D, _D]),
         "or_solid_response": (C.c_int, [C.c_int, _D, C.c_double, C.c_double, C.c_double,
                                         C.c_double, _D, _D, _D, _D, _D, _D]),
+        "or_thermal_element": (None, [C.c_int, _D, C.c_double, C.c_double, C.c_double, _D, _D,
+                                      _D]),
+        "or_thermal_loads": (None, [C.c_int64, _D, C.c_int, _I, C.c_int64, _I, _D, C.c_int64, _I,
+                                    _I, _D, _D, _D]),
+        "or_transient": (C.c_int, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, C.c_int64, _I,
+                                   _I, _D, _D, _U8, _D, _D, _D, C.c_double, C.c_int64,
+                                   C.c_double, C.c_int, _D, _D, _D, _I, _D]),
+        "or_thermal_steady": (C.c_int, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, C.c_int64,
+                                        _I, _I, _D, _D, _U8, _D, _D, C.c_int, _D, _D]),
         "or_quasi_static": (C.c_int, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _D, _D, _U8,
                                       _D, _D, C.c_double, C.c_int64, C.c_int64, _D, _D, C.c_int,
                                       _D, _D, _D, _D, _D, _I, _I, _I, _D]),
@@ -599,3 +608,79 @@ def quasi_static(nodes, conn, young, nu, yield_stress, hardening, fixed, imposed
             "residuals": [res[k, :res_len[k]].copy() for k in range(n)],
             "newton": int(counters[0]), "cuts": int(counters[1]), "pcg": int(counters[2]),
             "fail_increment": int(fail[0]), "times": times}
+
+
+# ------------------------------------------------------------------ transient heat conduction
+def thermal_element(x, k, rho_c, q=0.0):
+    """element.rs:450-477 + assembly.rs:465-484 — (conduction, capacity, nodal source) of one
+    tetrahedron."""
+    x = _f64(x)
+    nen = len(x)
+    ke, ce, fe = np.zeros((nen, nen)), np.zeros((nen, nen)), np.zeros(nen)
+    lib().or_thermal_element(nen, _d(x), k, rho_c, q, _d(ke), _d(ce), _d(fe))
+    return ke, ce, fe
+
+
+def _conv_arrays(convection):
+    conv = list(convection or [])
+    ce = _i64([c[0] for c in conv]) if conv else np.zeros(1, dtype=np.int64)
+    cf = _i64([c[1] for c in conv]) if conv else np.zeros(1, dtype=np.int64)
+    film = _f64([c[2] for c in conv]) if conv else np.zeros(1)
+    sink = _f64([c[3] for c in conv]) if conv else np.zeros(1)
+    return len(conv), ce, cf, film, sink
+
+
+def thermal_loads(nodes, conn, f, sources=(), convection=()):
+    """model.rs:1089-1127 (thermal part) — f (nodal heat loads) plus the heat sources
+    [(element, q)] in order and the convection loads [(element, face, film, sink)] grouped per
+    element."""
+    nodes, conn = _f64(nodes), _i64(conn)
+    f = _f64(f).copy()
+    src = list(sources or [])
+    se = _i64([s_[0] for s_ in src]) if src else np.zeros(1, dtype=np.int64)
+    sq = _f64([s_[1] for s_ in src]) if src else np.zeros(1)
+    nc, ce, cf, film, sink = _conv_arrays(convection)
+    lib().or_thermal_loads(len(nodes), _d(nodes), conn.shape[1], _i(conn), len(src), _i(se),
+                           _d(sq), nc, _i(ce), _i(cf), _d(film), _d(sink), _d(f))
+    return f
+
+
+def thermal_steady(nodes, conn, k, fixed, imposed, f, convection=(), iterative=True):
+    """Model::solve for a thermal model (model.rs:548-610): (status, temperatures, reactions)."""
+    nodes, conn = _f64(nodes), _i64(conn)
+    ne, nen = conn.shape
+    k = _f64(np.broadcast_to(k, (ne,)))
+    nc, ce, cf, film, sink = _conv_arrays(convection)
+    fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+    t, r = np.zeros(len(nodes)), np.zeros(len(nodes))
+    st = lib().or_thermal_steady(len(nodes), _d(nodes), ne, nen, _i(conn), _d(k), nc, _i(ce),
+                                 _i(cf), _d(film), _d(sink), fixed.ctypes.data_as(_U8),
+                                 _d(_f64(imposed)), _d(_f64(f)), 1 if iterative else 0, _d(t),
+                                 _d(r))
+    return st, t, r
+
+
+def transient(nodes, conn, k, rho_c, fixed, imposed, f, initial, time_period, increments,
+              theta=1.0, convection=(), iterative=True):
+    """Model::solve_step (model.rs:620-706) on tetrahedra: dict with status, temperatures and
+    reactions ((n+1), n_nodes), heat fluxes ((n+1), ne, ngp, 6), pcg iterations, times."""
+    nodes, conn = _f64(nodes), _i64(conn)
+    ne, nen = conn.shape
+    ngp = 1 if nen == 4 else 4
+    k = _f64(np.broadcast_to(k, (ne,)))
+    rho_c = _f64(np.broadcast_to(rho_c, (ne,)))
+    nc, ce, cf, film, sink = _conv_arrays(convection)
+    fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+    nn, n = len(nodes), int(increments)
+    t = np.zeros((n + 1, nn))
+    r = np.zeros((n + 1, nn))
+    q = np.zeros((n + 1, ne, ngp, 6))
+    it = np.zeros(1, dtype=np.int64)
+    times = np.zeros(2)
+    st = lib().or_transient(nn, _d(nodes), ne, nen, _i(conn), _d(k), _d(rho_c), nc, _i(ce),
+                            _i(cf), _d(film), _d(sink), fixed.ctypes.data_as(_U8),
+                            _d(_f64(imposed)), _d(_f64(f)), _d(_f64(initial)),
+                            float(time_period), n, float(theta), 1 if iterative else 0, _d(t),
+                            _d(r), _d(q), _i(it), _d(times))
+    return {"status": st, "temperatures": t, "reactions": r, "fluxes": q, "pcg": int(it[0]),
+            "times": times}

@@ -50,6 +50,20 @@ const TetGP TETRA_4[4] = {{TET_B, TET_B, TET_B, 1.0 / 24.0},
                           {TET_A, TET_B, TET_B, 1.0 / 24.0},
                           {TET_B, TET_A, TET_B, 1.0 / 24.0},
                           {TET_B, TET_B, TET_A, 1.0 / 24.0}};
+// quadrature.rs:139-177 (TETRA_14): degree-5 rule for the Tet10 capacity
+const double T14_A1 = 0.09273525031089142, T14_W1 = 0.01224884051939372;
+const double T14_A2 = 0.3108859192633008, T14_W2 = 0.01878132095300283;
+const double T14_B = 0.04550370412564851, T14_W3 = 0.00709100346284674;
+const double T14_C1 = 1.0 - 3.0 * T14_A1, T14_C2 = 1.0 - 3.0 * T14_A2, T14_D = 0.5 - T14_B;
+const TetGP TETRA_14[14] = {
+    {T14_A1, T14_A1, T14_A1, T14_W1}, {T14_C1, T14_A1, T14_A1, T14_W1},
+    {T14_A1, T14_C1, T14_A1, T14_W1}, {T14_A1, T14_A1, T14_C1, T14_W1},
+    {T14_A2, T14_A2, T14_A2, T14_W2}, {T14_C2, T14_A2, T14_A2, T14_W2},
+    {T14_A2, T14_C2, T14_A2, T14_W2}, {T14_A2, T14_A2, T14_C2, T14_W2},
+    {T14_B, T14_B, T14_D, T14_W3},    {T14_B, T14_D, T14_B, T14_W3},
+    {T14_D, T14_B, T14_B, T14_W3},    {T14_D, T14_D, T14_B, T14_W3},
+    {T14_D, T14_B, T14_D, T14_W3},    {T14_B, T14_D, T14_D, T14_W3}};
+
 // quadrature.rs:19-35 (TRIANGLE_3)
 const TriGP TRIANGLE_3[3] = {{1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0},
                              {2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0},
@@ -407,6 +421,106 @@ int face_pressure(int nen, const double (*xe)[3], int face, double value, int* p
   return nn;
 }
 
+// ---------------------------------------------------------------- thermal elements
+// element.rs:450-462 (conduction) on the element's rule (assembly.rs:392-403):
+// ke[i*nen + j] += w|det| k (g_i . g_j)
+void conduction(int nen, const double (*x)[3], double k, double* ke) {
+  int ngp;
+  const TetGP* rule = rule_of(nen, &ngp);
+  std::fill(ke, ke + nen * nen, 0.0);
+  double n[10], dn[10][3], g[10][3];
+  for (int q = 0; q < ngp; ++q) {
+    tet_shape(nen, rule[q].xi, rule[q].eta, rule[q].zeta, n, dn);
+    const double weight = rule[q].w * std::fabs(mapping3d(nen, x, dn, g));
+    for (int i = 0; i < nen; ++i)
+      for (int j = 0; j < nen; ++j) {
+        double s = 0.0;
+        for (int c = 0; c < 3; ++c) s += g[i][c] * g[j][c];
+        ke[i * nen + j] += weight * k * s;
+      }
+  }
+}
+
+// element.rs:465-477 (capacity) with TETRA_4 (Tet4) or TETRA_14 (Tet10), assembly.rs:413-431
+void capacity(int nen, const double (*x)[3], double rho_c, double* ce) {
+  const TetGP* rule = nen == 4 ? TETRA_4 : TETRA_14;
+  const int ngp = nen == 4 ? 4 : 14;
+  std::fill(ce, ce + nen * nen, 0.0);
+  double n[10], dn[10][3], g[10][3];
+  for (int q = 0; q < ngp; ++q) {
+    tet_shape(nen, rule[q].xi, rule[q].eta, rule[q].zeta, n, dn);
+    const double weight = rule[q].w * std::fabs(mapping3d(nen, x, dn, g));
+    for (int i = 0; i < nen; ++i)
+      for (int j = 0; j < nen; ++j) ce[i * nen + j] += weight * rho_c * n[i] * n[j];
+  }
+}
+
+// assembly.rs:465-484 (element_heat_source, nodal_source): f_i += w|det| N_i q
+void nodal_source(int nen, const double (*x)[3], double q, double* f) {
+  int ngp;
+  const TetGP* rule = rule_of(nen, &ngp);
+  std::fill(f, f + nen, 0.0);
+  double n[10], dn[10][3], g[10][3];
+  for (int p = 0; p < ngp; ++p) {
+    tet_shape(nen, rule[p].xi, rule[p].eta, rule[p].zeta, n, dn);
+    const double weight = rule[p].w * std::fabs(mapping3d(nen, x, dn, g));
+    for (int i = 0; i < nen; ++i) f[i] += weight * n[i] * q;
+  }
+}
+
+// assembly.rs:598-624 (face_convection) on the face rule of assembly.rs:527-593: matrix
+// (face nodes x face nodes, row-major) += film w N_a N_b, load += film sink w N_a
+int face_convection(int nen, const double (*xe)[3], int face, double film, double sink, int* pos,
+                    double* matrix, double* load) {
+  int nn = face_positions(nen, face, pos);
+  double c[6][3];
+  for (int a = 0; a < nn; ++a)
+    for (int k = 0; k < 3; ++k) c[a][k] = xe[pos[a]][k];
+  double u[3], v[3];
+  for (int k = 0; k < 3; ++k) { u[k] = c[1][k] - c[0][k]; v[k] = c[2][k] - c[0][k]; }
+  double cross[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2],
+                     u[0] * v[1] - u[1] * v[0]};
+  double sq = 0.0;
+  for (int k = 0; k < 3; ++k) sq += cross[k] * cross[k];
+  const double twice_area = std::sqrt(sq);
+  const int ngp = (nn == 3) ? 3 : 6;
+  const TriGP* rule = (nn == 3) ? TRIANGLE_3 : TRIANGLE_6;
+  std::fill(matrix, matrix + nn * nn, 0.0);
+  std::fill(load, load + nn, 0.0);
+  for (int q = 0; q < ngp; ++q) {
+    const double w = rule[q].w * twice_area;
+    double nv[6];
+    tri_values(nn, rule[q].xi, rule[q].eta, nv);
+    for (int a = 0; a < nn; ++a) {
+      load[a] += film * sink * w * nv[a];
+      for (int b = 0; b < nn; ++b) matrix[a * nn + b] += film * w * nv[a] * nv[b];
+    }
+  }
+  return nn;
+}
+
+// element.rs:478-499 (heat_fluxes): per Gauss point x, y, z, qx, qy, qz with q = -k grad T
+void heat_fluxes(int nen, const double (*x)[3], double k, const double* te, double* out) {
+  int ngp;
+  const TetGP* rule = rule_of(nen, &ngp);
+  double n[10], dn[10][3], g[10][3];
+  for (int q = 0; q < ngp; ++q) {
+    tet_shape(nen, rule[q].xi, rule[q].eta, rule[q].zeta, n, dn);
+    mapping3d(nen, x, dn, g);
+    double* o = out + 6 * q;
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      for (int a = 0; a < nen; ++a) s += n[a] * x[a][c];
+      o[c] = s;
+    }
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      for (int a = 0; a < nen; ++a) s += g[a][c] * te[a];
+      o[3 + c] = -k * s;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- solve.rs: dot
 // solve.rs:100-110: 8 lanes over chunks of 8, tail summed separately
 double dot8(const double* a, const double* b, i64 n) {
@@ -747,8 +861,11 @@ struct Mesh {
     solid_d(young[e], nu[e], d);
     solid_stiffness(nen, x, d, ke);
   }
-  // assembly.rs:20-32 (element_dofs, Tet branch)
-  i64 dof(i64 e, int local) const { return 3 * conn[e * nen + local / 3] + local % 3; }
+  int dpn = 3;  // DOFs per node: 3 (displacements) or 1 (temperature, model.rs:1074-1078)
+  // assembly.rs:20-32 (element_dofs, Tet branch), or the element's nodes in thermal analysis
+  i64 dof(i64 e, int local) const {
+    return dpn == 3 ? 3 * conn[e * nen + local / 3] + local % 3 : conn[e * nen + local];
+  }
 };
 
 struct System {
@@ -776,7 +893,7 @@ double now() {
 // model.rs:1442-1499 (FreeSystem::new) up to the factorisation
 int system_build(System& s) {
   const Mesh& m = s.mesh;
-  const int ml = 3 * m.nen;
+  const int ml = m.dpn * m.nen;
   double t0 = now();
   s.free_of.assign(s.ndof, -1);
   s.count = 0;
@@ -1013,6 +1130,66 @@ struct QuasiStatic {
       return advance(middle, t1, cuts + 1);
     }
     return st;
+  }
+};
+
+// ---------------------------------------------------------------- model.rs: transient
+// model.rs:1353-1369 (product): y[g] += sum_b M_e[a][b] x[dof_b] over the elements in order,
+// for the rows a whose DOF is in `rows`; elements with no DOF in rows are skipped
+void product(const Mesh& m, const std::vector<double>& mats, const double* x, const char* rows,
+             i64 ndof, double* y) {
+  const int ml = m.dpn * m.nen;
+  std::fill(y, y + ndof, 0.0);
+  for (i64 e = 0; e < m.n_el; ++e) {
+    bool any = false;
+    for (int a = 0; a < ml; ++a) any = any || rows[m.dof(e, a)];
+    if (!any) continue;
+    for (int a = 0; a < ml; ++a) {
+      const i64 g = m.dof(e, a);
+      if (!rows[g]) continue;
+      double s = 0.0;
+      for (int b = 0; b < ml; ++b) s += mats[(e * ml + a) * ml + b] * x[m.dof(e, b)];
+      y[g] += s;
+    }
+  }
+}
+
+// Model::solve_step_with (model.rs:628-706) for tetrahedra: the theta method
+// (C/dt + theta K) T_{n+1} = (C/dt - (1 - theta) K) T_n + F, K with the convection faces
+// (thermal_matrix, model.rs:1320-1326), the left matrix factored once.
+struct Transient {
+  Mesh mesh;  // dpn = 1; young = conductivity per element
+  const double* rho_c;
+  std::vector<double> kmat, cmat, left, right;  // n_el x nen x nen
+  std::vector<double> f;
+  std::vector<char> fixed;
+  std::vector<double> imposed;
+  double t_build = 0, t_solve = 0;
+  i64 pcg_total = 0;
+
+  // model.rs:1080-1086 + 1320-1326: conduction, then each convection face of the element in
+  // the order given, added into it
+  void matrices(i64 nconv, const i64* conv_el, const i64* conv_face, const double* film,
+                const double* sink) {
+    const int nen = mesh.nen;
+    kmat.assign(mesh.n_el * nen * nen, 0.0);
+    cmat.assign(mesh.n_el * nen * nen, 0.0);
+    double x[10][3];
+    for (i64 e = 0; e < mesh.n_el; ++e) {
+      mesh.coords(e, x);
+      conduction(nen, x, mesh.young[e], &kmat[e * nen * nen]);
+      capacity(nen, x, rho_c[e], &cmat[e * nen * nen]);
+    }
+    int pos[6];
+    double mat[36], load[6];
+    // faces grouped per element in the order given (model.rs:1081-1086)
+    for (i64 c = 0; c < nconv; ++c) {
+      const i64 e = conv_el[c];
+      mesh.coords(e, x);
+      const int nn = face_convection(nen, x, (int)conv_face[c], film[c], sink[c], pos, mat, load);
+      for (int a = 0; a < nn; ++a)
+        for (int b = 0; b < nn; ++b) kmat[(e * nen + pos[a]) * nen + pos[b]] += mat[a * nn + b];
+    }
   }
 };
 
@@ -1256,6 +1433,152 @@ int or_quasi_static(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64
   counters[2] = qs.pcg_total;
   if (times) { times[0] = qs.t_resp; times[1] = qs.t_build; times[2] = qs.t_solve; }
   return st;
+}
+
+// model.rs:1089-1127 (assemble, thermal part): F = nodal heat loads (already in f), then the
+// heat sources (element, q) in order, then the convection loads h T_inf int N, faces grouped
+// per element in element order (model.rs:1122-1126).
+void or_thermal_loads(i64 n_nodes, const double* xyz, int nen, const i64* conn, i64 nsrc,
+                      const i64* src_el, const double* src_q, i64 nconv, const i64* conv_el,
+                      const i64* conv_face, const double* film, const double* sink, double* f) {
+  Mesh m{n_nodes, 0, nen, xyz, conn, nullptr, nullptr};
+  double x[10][3], fe[10];
+  for (i64 k = 0; k < nsrc; ++k) {
+    m.coords(src_el[k], x);
+    nodal_source(nen, x, src_q[k], fe);
+    for (int a = 0; a < nen; ++a) f[conn[src_el[k] * nen + a]] += fe[a];
+  }
+  std::vector<i64> order(nconv);
+  for (i64 c = 0; c < nconv; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](i64 a, i64 b) { return conv_el[a] < conv_el[b]; });
+  int pos[6];
+  double mat[36], load[6];
+  for (i64 c : order) {
+    m.coords(conv_el[c], x);
+    const int nn = face_convection(nen, x, (int)conv_face[c], film[c], sink[c], pos, mat, load);
+    for (int a = 0; a < nn; ++a) f[conn[conv_el[c] * nen + pos[a]]] += load[a];
+  }
+}
+
+// Model::solve_step (model.rs:620-706) for tetrahedra. Per element: conductivity k and rho*c;
+// convection faces (element, face, film, sink); per node: fixed, BC value, F (thermal loads),
+// initial temperature. Outputs for increments 0..n: temperatures, reactions ((n+1) x n_nodes),
+// heat fluxes ((n+1) x n_el x ngp x 6: x, y, z, qx, qy, qz). times[0..1]: build+factor, steps.
+int or_transient(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64* conn,
+                 const double* k, const double* rho_c, i64 nconv, const i64* conv_el,
+                 const i64* conv_face, const double* film, const double* sink,
+                 const uint8_t* fixed, const double* imposed, const double* f,
+                 const double* initial, double time_period, i64 increments, double theta,
+                 int iterative, double* t_out, double* r_out, double* q_out, i64* pcg_iters,
+                 double* times) {
+  Transient tr;
+  tr.mesh = Mesh{n_nodes, n_el, nen, xyz, conn, k, nullptr};
+  tr.mesh.dpn = 1;
+  tr.rho_c = rho_c;
+  tr.matrices(nconv, conv_el, conv_face, film, sink);
+  const double dt = time_period / (double)increments;  // Step::time_step
+  const i64 mm = (i64)nen * nen;
+  tr.left.resize(n_el * mm);
+  tr.right.resize(n_el * mm);
+  for (i64 q = 0; q < n_el * mm; ++q) {  // model.rs:652-663 (combine)
+    tr.left[q] = tr.cmat[q] / dt + theta * tr.kmat[q];
+    tr.right[q] = tr.cmat[q] / dt + (theta - 1.0) * tr.kmat[q];
+  }
+  std::vector<double> t(initial, initial + n_nodes);
+  for (i64 g = 0; g < n_nodes; ++g)
+    if (fixed[g]) t[g] = imposed[g];
+  System sys;
+  sys.mesh = tr.mesh;
+  sys.iterative = iterative != 0;
+  sys.ndof = n_nodes;
+  sys.fixed.assign(fixed, fixed + n_nodes);
+  sys.imposed = t;
+  sys.tangents = tr.left.data();
+  double t0 = now();
+  int st = system_build(sys);
+  if (st == OK) st = system_factorize(sys);
+  if (times) times[0] = now() - t0;
+  if (st != OK) return st;
+  const int ngp = nen == 4 ? 1 : 4;
+  auto fluxes = [&](i64 inc) {
+    if (!q_out) return;
+    double x[10][3], te[10];
+    for (i64 e = 0; e < n_el; ++e) {
+      tr.mesh.coords(e, x);
+      for (int a = 0; a < nen; ++a) te[a] = t[conn[e * nen + a]];
+      heat_fluxes(nen, x, k[e], te, q_out + ((inc * n_el + e) * ngp) * 6);
+    }
+  };
+  std::copy(t.begin(), t.end(), t_out);
+  if (r_out) std::fill(r_out, r_out + n_nodes, 0.0);
+  fluxes(0);
+  std::vector<char> all(n_nodes, 1);
+  std::vector<double> carried(n_nodes), rhs(n_nodes), react(n_nodes), u(n_nodes);
+  t0 = now();
+  *pcg_iters = 0;
+  for (i64 inc = 1; inc <= increments; ++inc) {
+    product(tr.mesh, tr.right, t.data(), all.data(), n_nodes, carried.data());
+    for (i64 g = 0; g < n_nodes; ++g) rhs[g] = carried[g] + f[g];
+    i64 iters = 0;
+    double resid = 0.0;
+    st = system_solve(sys, rhs.data(), u.data(), TOLERANCE, MAX_ITERATIONS, &iters, &resid);
+    *pcg_iters += iters;
+    if (st != OK) return st;
+    for (i64 g = 0; g < n_nodes; ++g)
+      if (!fixed[g]) t[g] = u[g];
+    product(tr.mesh, tr.left, t.data(), sys.fixed.data(), n_nodes, react.data());
+    for (i64 g = 0; g < n_nodes; ++g)
+      if (fixed[g]) react[g] -= rhs[g];
+    std::copy(t.begin(), t.end(), t_out + inc * n_nodes);
+    if (r_out) std::copy(react.begin(), react.end(), r_out + inc * n_nodes);
+    fluxes(inc);
+  }
+  if (times) times[1] = now() - t0;
+  return OK;
+}
+
+// Model::solve for a thermal model (model.rs:548-610, Analysis::Thermal): K T = F with K the
+// conduction + convection matrices, the BC temperatures at the fixed nodes; reactions K T - F
+// on the fixed nodes (model.rs:586-593).
+int or_thermal_steady(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64* conn,
+                      const double* k, i64 nconv, const i64* conv_el, const i64* conv_face,
+                      const double* film, const double* sink, const uint8_t* fixed,
+                      const double* imposed, const double* f, int iterative, double* t,
+                      double* r) {
+  Transient tr;
+  tr.mesh = Mesh{n_nodes, n_el, nen, xyz, conn, k, nullptr};
+  tr.mesh.dpn = 1;
+  std::vector<double> ones(n_el, 1.0);
+  tr.rho_c = ones.data();
+  tr.matrices(nconv, conv_el, conv_face, film, sink);
+  System sys;
+  sys.mesh = tr.mesh;
+  sys.iterative = iterative != 0;
+  sys.ndof = n_nodes;
+  sys.fixed.assign(fixed, fixed + n_nodes);
+  sys.imposed.assign(imposed, imposed + n_nodes);
+  sys.tangents = tr.kmat.data();
+  int st = system_build(sys);
+  if (st == OK) st = system_factorize(sys);
+  if (st != OK) return st;
+  i64 iters = 0;
+  double resid = 0.0;
+  st = system_solve(sys, f, t, TOLERANCE, MAX_ITERATIONS, &iters, &resid);
+  if (st != OK) return st;
+  product(tr.mesh, tr.kmat, t, sys.fixed.data(), n_nodes, r);
+  for (i64 g = 0; g < n_nodes; ++g)
+    if (fixed[g]) r[g] -= f[g];
+  return OK;
+}
+
+// element matrices for the unit tests: conduction k (nen x nen), capacity, nodal source
+void or_thermal_element(int nen, const double* x, double k, double rho_c, double q, double* ke,
+                        double* ce, double* fe) {
+  const double (*xe)[3] = (const double (*)[3])x;
+  conduction(nen, xe, k, ke);
+  capacity(nen, xe, rho_c, ce);
+  nodal_source(nen, xe, q, fe);
 }
 
 // plasticity.rs:45-112 for the unit tests: stress[6], state[7] (ep, p), tangent[36]; returns yielding
