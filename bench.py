@@ -342,6 +342,59 @@ def run_plastic(args, ma, meshes):
           "speedup_vs_cpu": round(cpu["value"] / wall, 1) if cpu else None})
 
 
+def run_transient(args, ma, meshes):
+    """Transient heat conduction (Model::solve_step, implicit Euler) on a Tet4 cube of
+    60^3 cells (1,296,000 elements, 226,981 nodes): T = 100 on x = 0, convection (h = 10, sink 0)
+    on x = 1, from T = 0, 50 steps of 0.002 (k = 1, rho c = 1). The left matrix is assembled and
+    factored once; each step is a device PCG solve from zero, as Pcg::solve starts. The oracle
+    (one core) runs the same model for 3 steps; its build and per-step costs are scaled to 50."""
+    from maille_amd import thermal as T
+    n = args.cells if args.cells != 100 else 60
+    nodes, conn = meshes.box_grid((n, n, n), (1.0, 1.0, 1.0))
+    nn, ne = len(nodes), len(conn)
+    fixed = (np.abs(nodes[:, 0]) < 1e-12).astype(np.uint8)
+    imposed = np.where(fixed, 100.0, 0.0)
+    faces = ((0, 1, 2), (0, 3, 1), (1, 3, 2), (2, 3, 0))
+    on = np.abs(nodes[:, 0] - 1.0) < 1e-12
+    conv = [(e, fc, 10.0, 0.0) for e in np.nonzero(on[conn].sum(axis=1) >= 3)[0]
+            for fc, vs in enumerate(faces) if on[conn[e, list(vs)]].all()]
+    steps, period = args.increments if args.increments != 5 else 50, 0.1
+    codes = {"fp64": ma.FP64, "mixed": ma.MIXED, "mixed_ic0": ma.MIXED_IC0}
+    prec = "fp64" if args.precision == "best" else args.precision
+    t0 = time.perf_counter()
+    m = T.ThermalModel(nodes, conn, 1.0, 1.0, fixed, imposed, convection=conv,
+                       precision=codes[prec])
+    t_create = time.perf_counter() - t0
+    zero = np.zeros(nn)
+    m.solve_step(T.Step(period / steps, 1), zero, fluxes=False)  # warm-up: the same dt, factored
+    t0 = time.perf_counter()
+    incs, info = m.solve_step(T.Step(period, steps), zero, fluxes=False)
+    wall = time.perf_counter() - t0
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        f = oracle.thermal_loads(nodes, conn, zero, (), conv)
+        sample = 3
+        run = oracle.transient(nodes, conn, 1.0, 1.0, fixed, imposed, f, zero, period / steps * sample,
+                               sample, convection=conv)
+        t_build, t_steps = run["times"]
+        est = t_build + t_steps / sample * steps
+        gap = float(np.abs(incs[sample].temperatures - run["temperatures"][sample]).max() / 100.0)
+        cpu = {"value": round(est, 2), "unit": "s", "cores": 1, "kind": "port",
+               "sample": (f"oracle on the same model: FreeSystem build with IC(0) {t_build:.2f} s "
+                          f"and {sample} steps ({t_steps / sample:.2f} s each, {run['pcg']} PCG "
+                          f"iterations) measured, scaled to {steps} steps"),
+               "step3_max_abs_diff_over_100": gap}
+    emit({"metric": "transient heat conduction s, Tet4 cube 60^3, 50 implicit Euler steps",
+          "value": round(wall, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+          "data": "synthetic", "dtype": "fp64" if prec == "fp64" else f"fp64+{prec}",
+          "config": {"workload": f"thermal_cube_{n}", "nodes": int(nn), "elements": int(ne),
+                     "steps": steps, "convection_faces": len(conv)},
+          "ms_per_step": round(wall * 1e3 / steps, 3), "pcg_iterations": info["iterations"],
+          "create_s": round(t_create, 2), "factor_s": round(info["seconds"][1], 3),
+          "cpu_baseline": cpu, "speedup_vs_cpu": round(cpu["value"] / wall, 1) if cpu else None})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -357,7 +410,7 @@ def main():
                          "against which HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled")
     ap.add_argument("--cpu-iters", type=int, default=8)
     ap.add_argument("--workload", choices=["cube", "beam_cases", "box_batch", "tet10", "simp",
-                                           "stresses", "plastic"],
+                                           "stresses", "plastic", "transient"],
                     default="cube", help="cube: the headline 3.1M-DOF solve; the others are the "
                     "batched, Tet10 and adjoint workloads (one JSON line each)")
     ap.add_argument("--cases", type=int, default=256)
@@ -379,7 +432,7 @@ def main():
     if args.workload != "cube":
         {"beam_cases": run_beam_cases, "box_batch": run_box_batch, "tet10": run_tet10,
          "simp": run_simp, "stresses": run_stresses,
-         "plastic": run_plastic}[args.workload](args, ma, meshes)
+         "plastic": run_plastic, "transient": run_transient}[args.workload](args, ma, meshes)
         return
 
     t0 = time.time()
