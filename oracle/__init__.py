@@ -111,6 +111,13 @@ def _declare(L):
                                    C.c_double, C.c_int, _D, _D, _D, _I, _D]),
         "or_thermal_steady": (C.c_int, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, C.c_int64,
                                         _I, _I, _D, _D, _U8, _D, _D, C.c_int, _D, _D]),
+        "or_large_response": (None, [C.c_int, _D, C.c_int, C.c_double, C.c_double, _D, _D, _D,
+                                     _D]),
+        "or_follower_pressure": (C.c_int, [C.c_int, _I, _D, C.c_int, C.c_double, _D, _INT, _D,
+                                           _D]),
+        "or_large_strain": (C.c_int, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _U8,
+                                      C.c_int64, _I, _I, _D, _U8, _D, _D, C.c_double, C.c_int64,
+                                      C.c_int64, _D, _D, C.c_int, _D, _D, _D, _D, _I, _I, _I]),
         "or_quasi_static": (C.c_int, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, _D, _D, _D, _U8,
                                       _D, _D, C.c_double, C.c_int64, C.c_int64, _D, _D, C.c_int,
                                       _D, _D, _D, _D, _D, _I, _I, _I, _D]),
@@ -684,3 +691,68 @@ def transient(nodes, conn, k, rho_c, fixed, imposed, f, initial, time_period, in
                             _d(r), _d(q), _i(it), _d(times))
     return {"status": st, "temperatures": t, "reactions": r, "fluxes": q, "pcg": int(it[0]),
             "times": times}
+
+
+# ------------------------------------------------------------------ large strains
+def large_strain(nodes, conn, young, nu, fixed, imposed, f, time_period, increments, amplitude,
+                 neo_hookean=False, pressures=(), iterative=True):
+    """model.rs:732-1022 with Kinematics::Large (finite_strain.rs, follower_pressure
+    assembly.rs:732-862) on tetrahedra. ``neo_hookean``: bool or per-element flags;
+    ``pressures`` [(element, face, value)] follow the faces. Returns a dict like quasi_static
+    (stresses are Cauchy stresses at the current Gauss-point positions)."""
+    nodes, conn = _f64(nodes), _i64(conn)
+    ne, nen = conn.shape
+    ngp = 1 if nen == 4 else 4
+    young = _f64(np.broadcast_to(young, (ne,)))
+    nu = _f64(np.broadcast_to(nu, (ne,)))
+    neo = np.ascontiguousarray(np.broadcast_to(np.asarray(neo_hookean, dtype=np.uint8), (ne,)))
+    pr = list(pressures)
+    pe = _i64([q[0] for q in pr]) if pr else np.zeros(1, dtype=np.int64)
+    pf = _i64([q[1] for q in pr]) if pr else np.zeros(1, dtype=np.int64)
+    pv = _f64([q[2] for q in pr]) if pr else np.zeros(1)
+    ndof = 3 * len(nodes)
+    fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+    amp = np.asarray(amplitude, dtype=np.float64).reshape(-1, 2)
+    at, av = _f64(amp[:, 0]), _f64(amp[:, 1])
+    n = int(increments)
+    u = np.zeros((n + 1, ndof))
+    r = np.zeros((n + 1, ndof))
+    s = np.zeros((n + 1, ne, ngp, 9))
+    res = np.zeros((n, NEWTON_ITERATIONS + 1))
+    res_len = np.zeros(n, dtype=np.int64)
+    counters = np.zeros(3, dtype=np.int64)
+    fail = np.zeros(1, dtype=np.int64)
+    st = lib().or_large_strain(len(nodes), _d(nodes), ne, nen, _i(conn), _d(young), _d(nu),
+                               neo.ctypes.data_as(_U8), len(pr), _i(pe), _i(pf), _d(pv),
+                               fixed.ctypes.data_as(_U8), _d(_f64(imposed)), _d(_f64(f)),
+                               float(time_period), n, len(at), _d(at), _d(av),
+                               1 if iterative else 0, _d(u), _d(r), _d(s), _d(res), _i(res_len),
+                               _i(counters), _i(fail))
+    return {"status": st, "u": u, "reactions": r, "stresses": s,
+            "residuals": [res[k, :res_len[k]].copy() for k in range(n)],
+            "newton": int(counters[0]), "cuts": int(counters[1]), "pcg": int(counters[2]),
+            "fail_increment": int(fail[0])}
+
+
+def large_response(x, young, nu, ue, neo_hookean=False):
+    """finite_strain.rs:119-214 — one tetrahedron: (fint, tangent[row, col], stresses (ngp, 9))."""
+    x = _f64(x)
+    nen = len(x)
+    m, ngp = 3 * nen, (1 if nen == 4 else 4)
+    lam = young * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    mu = young / (2.0 * (1.0 + nu))
+    fint, tan, out = np.zeros(m), np.zeros((m, m)), np.zeros((ngp, 9))
+    lib().or_large_response(nen, _d(x), 1 if neo_hookean else 0, lam, mu, _d(_f64(ue)), _d(fint),
+                            _d(tan), _d(out))
+    return fint, tan, out
+
+
+def follower_pressure(nodes, element_nodes, face, value, u):
+    """assembly.rs:732-862 — (positions, forces[3m], stiffness 3m x 3m) of a follower pressure on
+    a tetrahedron face at the displacements u."""
+    nodes, en = _f64(nodes), _i64(element_nodes)
+    pos = (C.c_int * 6)()
+    forces, stiff = np.zeros(18), np.zeros(324)
+    m = lib().or_follower_pressure(len(en), _i(en), _d(nodes), face, value, _d(_f64(u)), pos,
+                                   _d(forces), _d(stiff))
+    return list(pos)[:m], forces[:3 * m], stiff[:9 * m * m].reshape(3 * m, 3 * m)

@@ -421,6 +421,246 @@ int face_pressure(int nen, const double (*xe)[3], int face, double value, int* p
   return nn;
 }
 
+// ---------------------------------------------------------------- finite_strain.rs
+typedef double Mat3[3][3];
+
+// finite_strain.rs:89-117 (mul, transpose, det, inverse)
+void mat_mul(const Mat3 a, const Mat3 b, Mat3 out) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += a[i][k] * b[k][j];
+      out[i][j] = s;
+    }
+}
+double mat_det(const Mat3 a) {
+  return a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+         a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+         a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+}
+void mat_inverse(const Mat3 a, Mat3 out) {
+  const double d = mat_det(a);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      // cofactor(j, i)
+      const int r0 = (j + 1) % 3, r1 = (j + 2) % 3, c0 = (i + 1) % 3, c1 = (i + 2) % 3;
+      out[i][j] = (a[r0][c0] * a[r1][c1] - a[r0][c1] * a[r1][c0]) / d;
+    }
+}
+inline double kd(int a, int b) { return a == b ? 1.0 : 0.0; }
+
+// finite_strain.rs:39-86 (LargeStrainLaw::evaluate): second Piola-Kirchhoff stress and the
+// material modulus; neo = Neo-Hookean, else Saint-Venant-Kirchhoff
+void law_evaluate(bool neo, double lambda, double mu, const Mat3 f, Mat3 s, double* modulus) {
+  Mat3 ft, c;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) ft[i][j] = f[j][i];
+  mat_mul(ft, f, c);
+  if (!neo) {
+    Mat3 e;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) e[i][j] = 0.5 * (c[i][j] - kd(i, j));
+    const double trace = e[0][0] + e[1][1] + e[2][2];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) s[i][j] = 2.0 * mu * e[i][j] + lambda * trace * kd(i, j);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l)
+            modulus[((i * 3 + j) * 3 + k) * 3 + l] =
+                lambda * kd(i, j) * kd(k, l) + mu * (kd(i, k) * kd(j, l) + kd(i, l) * kd(j, k));
+    return;
+  }
+  const double log_j = std::log(mat_det(f));
+  Mat3 inv;
+  mat_inverse(c, inv);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) s[i][j] = mu * (kd(i, j) - inv[i][j]) + lambda * log_j * inv[i][j];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l)
+          modulus[((i * 3 + j) * 3 + k) * 3 + l] =
+              lambda * inv[i][j] * inv[k][l] +
+              (mu - lambda * log_j) * (inv[i][k] * inv[j][l] + inv[i][l] * inv[j][k]);
+}
+
+// finite_strain.rs:119-214 (large_strain_response) for a tetrahedron on its element rule
+// (element.rs:338-358): internal force, tangent tan[row*m + col], Cauchy stresses at the current
+// Gauss-point positions out[q*9..] = x, y, z, sxx, syy, szz, tyz, txz, txy.
+void large_strain_response(int nen, const double (*x)[3], bool neo, double lambda, double mu,
+                           const double* ue, double* fint, double* tan, double* out) {
+  const int m = 3 * nen;
+  int ngp;
+  const TetGP* rule = rule_of(nen, &ngp);
+  std::fill(fint, fint + m, 0.0);
+  std::fill(tan, tan + m * m, 0.0);
+  double n[10], dn[10][3], g[10][3], modulus[81];
+  for (int q = 0; q < ngp; ++q) {
+    tet_shape(nen, rule[q].xi, rule[q].eta, rule[q].zeta, n, dn);
+    const double weight = rule[q].w * std::fabs(mapping3d(nen, x, dn, g));
+    double pos[3];
+    for (int c = 0; c < 3; ++c) {
+      double sm = 0.0;
+      for (int a = 0; a < nen; ++a) sm += n[a] * x[a][c];
+      pos[c] = sm;
+    }
+    Mat3 f;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) f[i][j] = i == j ? 1.0 : 0.0;
+    for (int a = 0; a < nen; ++a)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) f[i][j] += ue[3 * a + i] * g[a][j];
+    Mat3 ft, s, pk1;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) ft[i][j] = f[j][i];
+    law_evaluate(neo, lambda, mu, f, s, modulus);
+    mat_mul(f, s, pk1);
+    double a4[81];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) {
+            double material = 0.0;
+            for (int ii = 0; ii < 3; ++ii)
+              for (int kk = 0; kk < 3; ++kk)
+                material += f[i][ii] * modulus[((ii * 3 + j) * 3 + kk) * 3 + l] * f[k][kk];
+            const double geometric = i == k ? s[j][l] : 0.0;
+            a4[((i * 3 + j) * 3 + k) * 3 + l] = geometric + material;
+          }
+    for (int na = 0; na < nen; ++na)
+      for (int i = 0; i < 3; ++i) {
+        const int row = 3 * na + i;
+        double fsum = 0.0;
+        for (int j = 0; j < 3; ++j) fsum += pk1[i][j] * g[na][j];
+        fint[row] += weight * fsum;
+        for (int nb = 0; nb < nen; ++nb)
+          for (int k = 0; k < 3; ++k) {
+            double v = 0.0;
+            for (int j = 0; j < 3; ++j)
+              for (int l = 0; l < 3; ++l) v += g[na][j] * a4[((i * 3 + j) * 3 + k) * 3 + l] * g[nb][l];
+            tan[row * m + 3 * nb + k] += weight * v;
+          }
+      }
+    const double jdet = mat_det(f);
+    Mat3 sigma;
+    mat_mul(pk1, ft, sigma);
+    double* o = out + 9 * q;
+    for (int c = 0; c < 3; ++c) {
+      double sm = 0.0;
+      for (int a = 0; a < nen; ++a) sm += n[a] * ue[3 * a + c];
+      o[c] = pos[c] + sm;
+    }
+    o[3] = sigma[0][0] / jdet;
+    o[4] = sigma[1][1] / jdet;
+    o[5] = sigma[2][2] / jdet;
+    o[6] = sigma[1][2] / jdet;
+    o[7] = sigma[0][2] / jdet;
+    o[8] = sigma[0][1] / jdet;
+  }
+}
+
+// assembly.rs:727-730 (levi_civita)
+double levi_civita(int i, int j, int k) {
+  const double a = i, b = j, c = k;
+  return (a - b) * (b - c) * (c - a) / 2.0;
+}
+
+// assembly.rs:732-862 (follower_pressure), tetrahedron face: forces[3 m] per unit lambda and
+// stiffness[3m x 3m] of the pressure `value` on `face` at the displacements u (global).
+// Returns the face node count m; pos[] their positions in the element.
+int follower_pressure(int nen, const i64* enodes, const double* xyz, int face, double value,
+                      const double* u, int* pos, double* forces, double* stiffness) {
+  const int m = face_positions(nen, face, pos);
+  double ref[6][3], cur[6][3], xe[10][3];
+  for (int a = 0; a < nen; ++a)
+    for (int c = 0; c < 3; ++c) xe[a][c] = xyz[3 * enodes[a] + c];
+  for (int a = 0; a < m; ++a)
+    for (int c = 0; c < 3; ++c) {
+      ref[a][c] = xe[pos[a]][c];
+      cur[a][c] = ref[a][c] + u[3 * enodes[pos[a]] + c];
+    }
+  // face_rule's outward normal (assembly.rs:555-584) on the reference coordinates
+  double uu[3], vv[3];
+  for (int k = 0; k < 3; ++k) { uu[k] = ref[1][k] - ref[0][k]; vv[k] = ref[2][k] - ref[0][k]; }
+  double cross[3] = {uu[1] * vv[2] - uu[2] * vv[1], uu[2] * vv[0] - uu[0] * vv[2],
+                     uu[0] * vv[1] - uu[1] * vv[0]};
+  double sq = 0.0;
+  for (int k = 0; k < 3; ++k) sq += cross[k] * cross[k];
+  const double twice_area = std::sqrt(sq);
+  double outward[3];
+  for (int k = 0; k < 3; ++k)
+    outward[k] = (ref[0][k] + ref[1][k] + ref[2][k]) / 3.0 -
+                 (xe[0][k] + xe[1][k] + xe[2][k] + xe[3][k]) / 4.0;
+  double dotn = 0.0;
+  for (int k = 0; k < 3; ++k) dotn += cross[k] * outward[k];
+  const double nsign = dotn < 0.0 ? -1.0 : 1.0;
+  double normal[3];
+  for (int k = 0; k < 3; ++k) normal[k] = nsign * cross[k] / twice_area;
+  const int ngp = m == 3 ? 3 : 6;
+  const TriGP* rule = m == 3 ? TRIANGLE_3 : TRIANGLE_6;
+  double nv[6][6], dn[6][6][2];
+  for (int q = 0; q < ngp; ++q) {
+    const double xi = rule[q].xi, eta = rule[q].eta;
+    if (m == 3) {
+      nv[q][0] = 1.0 - xi - eta; nv[q][1] = xi; nv[q][2] = eta;
+      const double d[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+      for (int a = 0; a < 3; ++a) { dn[q][a][0] = d[a][0]; dn[q][a][1] = d[a][1]; }
+    } else {
+      const double l1 = 1.0 - xi - eta, l2 = xi, l3 = eta;
+      nv[q][0] = l1 * (2.0 * l1 - 1.0);
+      nv[q][1] = l2 * (2.0 * l2 - 1.0);
+      nv[q][2] = l3 * (2.0 * l3 - 1.0);
+      nv[q][3] = 4.0 * l1 * l2;
+      nv[q][4] = 4.0 * l2 * l3;
+      nv[q][5] = 4.0 * l3 * l1;
+      const double d[6][2] = {{1.0 - 4.0 * l1, 1.0 - 4.0 * l1}, {4.0 * l2 - 1.0, 0.0},
+                              {0.0, 4.0 * l3 - 1.0},           {4.0 * (l1 - l2), -4.0 * l2},
+                              {4.0 * l3, 4.0 * l2},            {-4.0 * l3, 4.0 * (l1 - l3)}};
+      for (int a = 0; a < 6; ++a) { dn[q][a][0] = d[a][0]; dn[q][a][1] = d[a][1]; }
+    }
+  }
+  auto tangents = [&](const double (*xx)[3], int q, double* a, double* b) {
+    for (int c = 0; c < 3; ++c) {
+      double s0 = 0.0, s1 = 0.0;
+      for (int k = 0; k < m; ++k) s0 += dn[q][k][0] * xx[k][c];
+      for (int k = 0; k < m; ++k) s1 += dn[q][k][1] * xx[k][c];
+      a[c] = s0;
+      b[c] = s1;
+    }
+  };
+  auto area_vector = [](const double* a, const double* b, double* out) {
+    out[0] = a[1] * b[2] - a[2] * b[1];
+    out[1] = a[2] * b[0] - a[0] * b[2];
+    out[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  double a0[3], b0[3], av[3];
+  tangents(ref, 0, a0, b0);
+  area_vector(a0, b0, av);
+  double along = 0.0;
+  for (int c = 0; c < 3; ++c) along += av[c] * normal[c];
+  const double sign = along < 0.0 ? -1.0 : 1.0;
+  std::fill(forces, forces + 3 * m, 0.0);
+  std::fill(stiffness, stiffness + 9 * m * m, 0.0);
+  for (int q = 0; q < ngp; ++q) {
+    double xi[3], eta[3], area[3];
+    tangents(cur, q, xi, eta);
+    area_vector(xi, eta, area);
+    const double w = -value * sign * rule[q].w;
+    for (int a = 0; a < m; ++a)
+      for (int c = 0; c < 3; ++c) forces[3 * a + c] += w * nv[q][a] * area[c];
+    for (int b = 0; b < m; ++b)
+      for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 3; ++k) {
+          double d = 0.0;
+          for (int l = 0; l < 3; ++l)
+            d += levi_civita(c, k, l) * dn[q][b][0] * eta[l] + levi_civita(c, l, k) * xi[l] * dn[q][b][1];
+          for (int a = 0; a < m; ++a) stiffness[(3 * a + c) * 3 * m + 3 * b + k] += w * nv[q][a] * d;
+        }
+  }
+  return m;
+}
+
 // ---------------------------------------------------------------- thermal elements
 // element.rs:450-462 (conduction) on the element's rule (assembly.rs:392-403):
 // ke[i*nen + j] += w|det| k (g_i . g_j)
@@ -1017,6 +1257,14 @@ struct QuasiStatic {
   Mesh mesh;
   const double* yield0;     // n_el (infinite: elastic)
   const double* hardening;  // n_el
+  // Kinematics::Large (total Lagrangian, finite_strain.rs): per element 1 = Neo-Hookean, 0 =
+  // Saint-Venant-Kirchhoff; follower pressures (element, face, value), model.rs:868-903
+  bool large = false;
+  const uint8_t* neo = nullptr;
+  i64 npres = 0;
+  const i64* pres_el = nullptr;
+  const i64* pres_face = nullptr;
+  const double* pres_val = nullptr;
   std::vector<char> fixed;
   std::vector<double> imposed, f;
   std::vector<double> amp_t, amp_v;
@@ -1048,9 +1296,16 @@ struct QuasiStatic {
       double x[10][3];
       mesh.coords(e, x);
       for (int k = 0; k < m; ++k) ue[k] = uu[mesh.dof(e, k)];
-      yielding |= solid_response(nen, x, mesh.young[e], mesh.nu[e], yield0[e], hardening[e],
-                                 ue.data(), &committed[e * ngp], fint.data(), tan.data(),
-                                 out.data(), &states[e * ngp]);
+      if (large) {  // element_large_response (assembly.rs:347-355); states stay default
+        const double E = mesh.young[e], v = mesh.nu[e];
+        const double lam = E * v / ((1.0 + v) * (1.0 - 2.0 * v)), mu = E / (2.0 * (1.0 + v));
+        large_strain_response(nen, x, neo && neo[e], lam, mu, ue.data(), fint.data(), tan.data(),
+                              out.data());
+      } else {
+        yielding |= solid_response(nen, x, mesh.young[e], mesh.nu[e], yield0[e], hardening[e],
+                                   ue.data(), &committed[e * ngp], fint.data(), tan.data(),
+                                   out.data(), &states[e * ngp]);
+      }
       std::copy(out.begin(), out.end(), stresses.begin() + e * ngp * 9);
       for (int q = 0; q < ngp; ++q) p[e * ngp + q] = states[e * ngp + q].p;
       if (tangents) std::copy(tan.begin(), tan.end(), tangents->begin() + e * m * m);
@@ -1066,8 +1321,29 @@ struct QuasiStatic {
     for (;;) {
       double t0 = now();
       bool yielding = responses(uu, fint, &tangents);
+      // model.rs:868-903 (follower_force): forces per unit lambda, tangent -= lambda sym(K_p)
+      std::vector<double> follower(ndof, 0.0);
+      if (large) {
+        const int nen = mesh.nen, m = 3 * nen;
+        int pos[6];
+        double pf[18], pk[324];
+        for (i64 k = 0; k < npres; ++k) {
+          const i64 e = pres_el[k];
+          const int nf = follower_pressure(nen, mesh.conn + e * nen, mesh.xyz, (int)pres_face[k],
+                                           pres_val[k], uu.data(), pos, pf, pk);
+          for (int a = 0; a < nf; ++a)
+            for (int c = 0; c < 3; ++c) follower[mesh.dof(e, 3 * pos[a] + c)] += pf[3 * a + c];
+          const int n3 = 3 * nf;
+          double* te = &tangents[e * m * m];
+          for (int r = 0; r < n3; ++r)
+            for (int c = 0; c < n3; ++c) {
+              const double sym = 0.5 * (pk[r * n3 + c] + pk[c * n3 + r]);
+              te[(3 * pos[r / 3] + r % 3) * m + 3 * pos[c / 3] + c % 3] -= lambda * sym;
+            }
+        }
+      }
       t_resp += now() - t0;
-      for (i64 g = 0; g < ndof; ++g) external[g] = lambda * (f[g] + 0.0) + 0.0;
+      for (i64 g = 0; g < ndof; ++g) external[g] = lambda * (f[g] + follower[g]) + 0.0;
       for (i64 g = 0; g < ndof; ++g) residual[g] = fint[g] - external[g];
       sc = std::fmax(std::fmax(sc, norm_seq(fint.data(), ndof, nullptr, false)),
                      norm_seq(external.data(), ndof, nullptr, false));
@@ -1579,6 +1855,78 @@ void or_thermal_element(int nen, const double* x, double k, double rho_c, double
   conduction(nen, xe, k, ke);
   capacity(nen, xe, rho_c, ce);
   nodal_source(nen, xe, q, fe);
+}
+
+// Model::solve_quasi_static with Kinematics::Large (model.rs:732-1022 with element_large_response
+// and follower_force) on tetrahedra: per element E, nu and the law (neo[e] = 1: Neo-Hookean, else
+// Saint-Venant-Kirchhoff); follower pressures (element, face, value); outputs as or_quasi_static
+// (stresses are Cauchy stresses at the current Gauss-point positions; p stays 0).
+int or_large_strain(i64 n_nodes, const double* xyz, i64 n_el, int nen, const i64* conn,
+                    const double* young, const double* nu, const uint8_t* neo, i64 npres,
+                    const i64* pres_el, const i64* pres_face, const double* pres_val,
+                    const uint8_t* fixed, const double* imposed, const double* f,
+                    double time_period, i64 increments, i64 npts, const double* amp_t,
+                    const double* amp_v, int iterative, double* u_out, double* r_out,
+                    double* s_out, double* res_out, i64* res_len, i64* counters,
+                    i64* fail_increment) {
+  std::vector<double> inf(n_el, INFINITY), zero(n_el, 0.0);
+  QuasiStatic qs;
+  qs.mesh = Mesh{n_nodes, n_el, nen, xyz, conn, young, nu};
+  qs.yield0 = inf.data();
+  qs.hardening = zero.data();
+  qs.large = true;
+  qs.neo = neo;
+  qs.npres = npres;
+  qs.pres_el = pres_el;
+  qs.pres_face = pres_face;
+  qs.pres_val = pres_val;
+  qs.ndof = 3 * n_nodes;
+  qs.ngp = nen == 4 ? 1 : 4;
+  qs.iterative = iterative != 0;
+  qs.fixed.assign(fixed, fixed + qs.ndof);
+  qs.imposed.assign(imposed, imposed + qs.ndof);
+  qs.f.assign(f, f + qs.ndof);
+  qs.amp_t.assign(amp_t, amp_t + npts);
+  qs.amp_v.assign(amp_v, amp_v + npts);
+  qs.u.assign(qs.ndof, 0.0);
+  qs.committed.assign(n_el * qs.ngp, PlasticState{});
+  const i64 ns = n_el * qs.ngp;
+  auto record = [&](i64 k) {
+    std::copy(qs.u.begin(), qs.u.end(), u_out + k * qs.ndof);
+    if (r_out) std::copy(qs.reactions.begin(), qs.reactions.end(), r_out + k * qs.ndof);
+    if (s_out) std::copy(qs.stresses.begin(), qs.stresses.end(), s_out + k * ns * 9);
+  };
+  std::vector<double> fint;
+  qs.responses(qs.u, fint, nullptr);
+  qs.reactions.assign(qs.ndof, 0.0);
+  record(0);
+  *fail_increment = -1;
+  const double dt = time_period / (double)increments;
+  int st = OK;
+  for (i64 inc = 1; inc <= increments; ++inc) {
+    const double time = (double)inc * dt;
+    st = qs.advance(time - dt, time, 0);
+    if (st != OK) { *fail_increment = inc; break; }
+    record(inc);
+    res_len[inc - 1] = (i64)qs.residuals.size();
+    std::copy(qs.residuals.begin(), qs.residuals.end(), res_out + (inc - 1) * (NEWTON_ITERATIONS + 1));
+  }
+  counters[0] = qs.newton_total;
+  counters[1] = qs.cuts_total;
+  counters[2] = qs.pcg_total;
+  return st;
+}
+
+// finite_strain.rs:119-214 for one element (unit tests): fint[m], tan[m*m], out[ngp*9]
+void or_large_response(int nen, const double* x, int neo, double lambda, double mu,
+                       const double* ue, double* fint, double* tan, double* out) {
+  large_strain_response(nen, (const double (*)[3])x, neo != 0, lambda, mu, ue, fint, tan, out);
+}
+
+// assembly.rs:732-862 for one face (unit tests): returns m; pos[6], forces[3m], stiffness[9m^2]
+int or_follower_pressure(int nen, const i64* enodes, const double* xyz, int face, double value,
+                         const double* u, int* pos, double* forces, double* stiffness) {
+  return follower_pressure(nen, enodes, xyz, face, value, u, pos, forces, stiffness);
 }
 
 // plasticity.rs:45-112 for the unit tests: stress[6], state[7] (ep, p), tangent[36]; returns yielding
