@@ -271,6 +271,25 @@ ml_status ml_model_solve_quasi_static(ml_model* m, const double* yield_stress,
                                       double* residuals, int64_t* residual_counts,
                                       ml_quasi_static_info* info);
 
+/* Model::solve_quasi_static with Kinematics::Large (model.rs:732-1022, with_kinematics) on this
+ * model's tetrahedra: total-Lagrangian elements (element_large_response, finite_strain.rs:119-214)
+ * with the Saint-Venant-Kirchhoff law, or the compressible Neo-Hookean law where neo_hookean[e]
+ * is 1 (Hyperelasticity::NeoHookean, material.rs:36-42; null: all Saint-Venant-Kirchhoff), Lame
+ * constants from the model's E and nu; follower pressures (element, face 0-3, value) whose forces
+ * and symmetrised stiffness follow the deformed faces (follower_pressure, assembly.rs:732-862;
+ * follower_force, model.rs:868-903); loads f (dead: nodal loads and edge loads, no pressures) and
+ * BC values scaled by the amplitude, Newton-Raphson and increment cutting as
+ * ml_model_solve_quasi_static. Outputs as there: u, reactions, stresses (Cauchy stresses at the
+ * current Gauss-point positions), residuals / residual_counts; any may be null. The follower
+ * terms are computed on the host from the iterate (one copy of u per Newton iteration). */
+ml_status ml_model_solve_large_strain(ml_model* m, const uint8_t* neo_hookean, const double* f,
+                                      int64_t npressures, const int64_t* pres_elements,
+                                      const int64_t* pres_faces, const double* pres_values,
+                                      double time_period, int64_t increments, int64_t npoints,
+                                      const double* amp_t, const double* amp_v, double* u,
+                                      double* reactions, double* stresses, double* residuals,
+                                      int64_t* residual_counts, ml_quasi_static_info* info);
+
 /* ------------------------------------------------------------------ transient heat conduction */
 
 typedef struct ml_thermal ml_thermal;

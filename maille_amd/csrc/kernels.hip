@@ -473,6 +473,170 @@ __global__ __launch_bounds__(256) void k_response(
   if (i == 0 && yielded) *yielding = 1;
 }
 
+// large_strain_response (finite_strain.rs:119-214) with LargeStrainLaw::evaluate (:39-86), one
+// thread per (element e, column c = 3 nb + k) like k_response: per Gauss point, F = I + sum u_a
+// (x) grad N_a, the second Piola-Kirchhoff stress S and modulus of the law (Saint-Venant-Kirchhoff,
+// or Neo-Hookean when neo[e]), P = F S and A_ijkl = delta_ik S_jl + sum F_i,ii C_ii,j,kk,l F_k,kk
+// for the thread's k; column c of the tangent (rows (na, i): sum_j sum_l g_na,j A_ijkl g_nb,l) to
+// slot (row*M + c)*ne + e, and fint[c] = sum w P_k,j g_nb,j. The thread of column 0 writes zero
+// states and the Cauchy stress P F^T / J at the current Gauss-point position.
+template <int NEN>
+__global__ __launch_bounds__(256) void k_large_response(
+    int64_t ne, const double* __restrict__ xyz, const int* __restrict__ conn,
+    const double* __restrict__ young, const double* __restrict__ nu,
+    const unsigned char* __restrict__ neo, const double* __restrict__ u,
+    double* __restrict__ ke, double* __restrict__ fint, double* __restrict__ trial,
+    double* __restrict__ records) {
+#pragma clang fp contract(off)
+  constexpr int M = 3 * NEN, NG = Rule<NEN>::N;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)M * ne) return;
+  const int col = (int)(t / ne);
+  const int64_t e = t - (int64_t)col * ne;
+  const int nb = col / 3, kc = col % 3;
+  double x[NEN][3], ue[M];
+  for (int a = 0; a < NEN; ++a) {
+    const int64_t node = conn[e * NEN + a];
+    for (int c = 0; c < 3; ++c) {
+      x[a][c] = xyz[3 * node + c];
+      ue[3 * a + c] = u[3 * node + c];
+    }
+  }
+  const double E = young[e], v = nu[e];
+  const double lambda = E * v / ((1.0 + v) * (1.0 - 2.0 * v)), mu = E / (2.0 * (1.0 + v));
+  const bool nh = neo && neo[e];
+  double acc[M];
+  for (int r = 0; r < M; ++r) acc[r] = 0.0;
+  double fi = 0.0;
+  auto kd = [](int a, int b) { return a == b ? 1.0 : 0.0; };
+  for (int q = 0; q < NG; ++q) {
+    double xi, eta, zeta, w;
+    gauss_point(NEN, q, xi, eta, zeta, w);
+    double g[NEN][3];
+    const double weight = w * fabs(tet_mapping<NEN>(x, xi, eta, zeta, g));
+    double f[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) f[i][j] = i == j ? 1.0 : 0.0;
+    for (int a = 0; a < NEN; ++a)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) f[i][j] += ue[3 * a + i] * g[a][j];
+    double c[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) s += f[k][i] * f[k][j];
+        c[i][j] = s;
+      }
+    const double jdet = f[0][0] * (f[1][1] * f[2][2] - f[1][2] * f[2][1]) -
+                        f[0][1] * (f[1][0] * f[2][2] - f[1][2] * f[2][0]) +
+                        f[0][2] * (f[1][0] * f[2][1] - f[1][1] * f[2][0]);
+    double s2[3][3], inv[3][3], log_j = 0.0;
+    if (nh) {
+      log_j = log(jdet);
+      const double dc = c[0][0] * (c[1][1] * c[2][2] - c[1][2] * c[2][1]) -
+                        c[0][1] * (c[1][0] * c[2][2] - c[1][2] * c[2][0]) +
+                        c[0][2] * (c[1][0] * c[2][1] - c[1][1] * c[2][0]);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          const int r0 = (j + 1) % 3, r1 = (j + 2) % 3, c0 = (i + 1) % 3, c1 = (i + 2) % 3;
+          inv[i][j] = (c[r0][c0] * c[r1][c1] - c[r0][c1] * c[r1][c0]) / dc;
+        }
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) s2[i][j] = mu * (kd(i, j) - inv[i][j]) + lambda * log_j * inv[i][j];
+    } else {
+      double ee[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) ee[i][j] = 0.5 * (c[i][j] - kd(i, j));
+      const double trace = ee[0][0] + ee[1][1] + ee[2][2];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) s2[i][j] = 2.0 * mu * ee[i][j] + lambda * trace * kd(i, j);
+    }
+    auto modulus = [&](int i, int j, int k, int l) {
+      if (nh)
+        return lambda * inv[i][j] * inv[k][l] +
+               (mu - lambda * log_j) * (inv[i][k] * inv[j][l] + inv[i][l] * inv[j][k]);
+      return lambda * kd(i, j) * kd(k, l) + mu * (kd(i, k) * kd(j, l) + kd(i, l) * kd(j, k));
+    };
+    double pk1[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) s += f[i][k] * s2[k][j];
+        pk1[i][j] = s;
+      }
+    // A_{i j kc l} for this thread's kc
+    double a4[3][3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int l = 0; l < 3; ++l) {
+          double material = 0.0;
+          for (int ii = 0; ii < 3; ++ii)
+            for (int kk = 0; kk < 3; ++kk) material += f[i][ii] * modulus(ii, j, kk, l) * f[kc][kk];
+          const double geometric = i == kc ? s2[j][l] : 0.0;
+          a4[i][j][l] = geometric + material;
+        }
+    {
+      double s = 0.0;
+      for (int j = 0; j < 3; ++j) s += pk1[kc][j] * g[nb][j];
+      fi += weight * s;
+    }
+    for (int na = 0; na < NEN; ++na)
+      for (int i = 0; i < 3; ++i) {
+        double vv = 0.0;
+        for (int j = 0; j < 3; ++j)
+          for (int l = 0; l < 3; ++l) vv += g[na][j] * a4[i][j][l] * g[nb][l];
+        acc[3 * na + i] += weight * vv;
+      }
+    if (col == 0) {
+      for (int k = 0; k < 7; ++k) trial[((int64_t)k * NG + q) * ne + e] = 0.0;
+      double n[NEN];
+      const double l4[4] = {1.0 - xi - eta - zeta, xi, eta, zeta};
+      if constexpr (NEN == 4) {
+        for (int a = 0; a < 4; ++a) n[a] = l4[a];
+      } else {
+        const int ed[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+        for (int a = 0; a < 4; ++a) n[a] = l4[a] * (2.0 * l4[a] - 1.0);
+        for (int k = 0; k < 6; ++k) n[4 + k] = 4.0 * l4[ed[k][0]] * l4[ed[k][1]];
+      }
+      double sigma[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double s = 0.0;
+          for (int k = 0; k < 3; ++k) s += pk1[i][k] * f[j][k];
+          sigma[i][j] = s;
+        }
+      double* o = records + (e * NG + q) * 9;
+      for (int k = 0; k < 3; ++k) {
+        double p0 = 0.0;
+        for (int a = 0; a < NEN; ++a) p0 += n[a] * x[a][k];
+        double s = 0.0;
+        for (int a = 0; a < NEN; ++a) s += n[a] * ue[3 * a + k];
+        o[k] = p0 + s;
+      }
+      o[3] = sigma[0][0] / jdet;
+      o[4] = sigma[1][1] / jdet;
+      o[5] = sigma[2][2] / jdet;
+      o[6] = sigma[1][2] / jdet;
+      o[7] = sigma[0][2] / jdet;
+      o[8] = sigma[0][1] / jdet;
+    }
+  }
+  for (int r = 0; r < M; ++r) ke[((int64_t)r * M + col) * ne + e] = acc[r];
+  fint[(int64_t)col * ne + e] = fi;
+}
+
+// Element tangents corrected in place, group after group (a group is one element's list of
+// (slot, value) additions in the caller's order): ke[slot] += value.
+__global__ __launch_bounds__(256) void k_add_groups(int64_t ngroups, const int64_t* __restrict__ off,
+                                                    const int64_t* __restrict__ slot,
+                                                    const double* __restrict__ val,
+                                                    double* __restrict__ ke) {
+#pragma clang fp contract(off)
+  const int64_t gidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gidx >= ngroups) return;
+  for (int64_t c = off[gidx]; c < off[gidx + 1]; ++c) ke[slot[c]] += val[c];
+}
+
 // internal_force (model.rs:1241-1250), thread per node: F_int[3n + c] sums the node's element
 // forces over its incidences in element order (f[g] += v, starting from 0.0).
 template <int NEN>
@@ -494,12 +658,14 @@ __global__ __launch_bounds__(256) void k_internal_force(int64_t n_nodes, int64_t
   for (int c = 0; c < 3; ++c) out[3 * v + c] = s[c];
 }
 
-// The Newton residual of equilibrium (model.rs:950-967), thread per DOF: external = lambda (F + 0)
-// + 0 (no follower or contact force on this path), r = F_int - external, -r for the solve, and
+// The Newton residual of equilibrium (model.rs:950-967), thread per DOF: external = lambda (F +
+// follower) + 0 (follower forces per unit lambda under large strains, else 0; no contact force on
+// this path), r = F_int - external, -r for the solve, and
 // per-block sums of F_int^2, external^2 and r^2 over the free DOFs (fixed-tree, deterministic)
 // for the three norms; the reactions are r on the fixed DOFs (model.rs:981-986).
 __global__ __launch_bounds__(256) void k_newton_residual(int64_t ndof, double lambda,
                                                          const double* __restrict__ f,
+                                                         const double* __restrict__ follower,
                                                          const double* __restrict__ fint,
                                                          const int* __restrict__ rank_of_dof,
                                                          double* __restrict__ neg_r,
@@ -509,7 +675,7 @@ __global__ __launch_bounds__(256) void k_newton_residual(int64_t ndof, double la
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double a = 0.0, b = 0.0, c = 0.0;
   if (g < ndof) {
-    const double ext = lambda * (f[g] + 0.0) + 0.0;
+    const double ext = lambda * (f[g] + (follower ? follower[g] : 0.0)) + 0.0;
     const double fi = fint[g];
     const double r = fi - ext;
     neg_r[g] = -r;

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -1188,6 +1189,129 @@ struct Model {
   }
 };
 
+// follower_pressure (assembly.rs:732-862) for a tetrahedron face, on the host: positions of the
+// face's nodes in the element, forces[3m] per unit pressure factor and stiffness[3m x 3m]
+// (row-major) of the pressure `value` at the displacements u. The face's outward normal comes
+// from face_rule on the reference coordinates (assembly.rs:555-584); n dS = x,xi x x,eta on the
+// current ones, with TRIANGLE_3 / TRIANGLE_6 weights and tri3 / tri6 derivatives (shape.rs:1-47).
+int follower_terms(int nen, const int64_t* en, const double* xyz, int face, double value,
+                   const double* u, int* pos, double* forces, double* stiffness) {
+  static const int FACES[4][3] = {{0, 1, 2}, {0, 3, 1}, {1, 3, 2}, {2, 3, 0}};
+  static const int EDGES[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+  auto mid = [](int a, int b) {
+    for (int k = 0; k < 6; ++k)
+      if ((EDGES[k][0] == a && EDGES[k][1] == b) || (EDGES[k][0] == b && EDGES[k][1] == a))
+        return 4 + k;
+    return 4;
+  };
+  const int* fc = FACES[face];
+  pos[0] = fc[0]; pos[1] = fc[1]; pos[2] = fc[2];
+  int m = 3;
+  if (nen == 10) {
+    pos[3] = mid(fc[0], fc[1]); pos[4] = mid(fc[1], fc[2]); pos[5] = mid(fc[2], fc[0]);
+    m = 6;
+  }
+  double ref[6][3], cur[6][3];
+  for (int a = 0; a < m; ++a)
+    for (int c = 0; c < 3; ++c) {
+      ref[a][c] = xyz[3 * en[pos[a]] + c];
+      cur[a][c] = ref[a][c] + u[3 * en[pos[a]] + c];
+    }
+  double e1[3], e2[3];
+  for (int k = 0; k < 3; ++k) { e1[k] = ref[1][k] - ref[0][k]; e2[k] = ref[2][k] - ref[0][k]; }
+  double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                  e1[0] * e2[1] - e1[1] * e2[0]};
+  double sq = 0.0;
+  for (int k = 0; k < 3; ++k) sq += cr[k] * cr[k];
+  const double twice_area = std::sqrt(sq);
+  double outward[3];
+  for (int k = 0; k < 3; ++k)
+    outward[k] = (ref[0][k] + ref[1][k] + ref[2][k]) / 3.0 -
+                 (xyz[3 * en[0] + k] + xyz[3 * en[1] + k] + xyz[3 * en[2] + k] + xyz[3 * en[3] + k]) / 4.0;
+  double dn0 = 0.0;
+  for (int k = 0; k < 3; ++k) dn0 += cr[k] * outward[k];
+  const double nsign = dn0 < 0.0 ? -1.0 : 1.0;
+  double normal[3];
+  for (int k = 0; k < 3; ++k) normal[k] = nsign * cr[k] / twice_area;
+  const double A6 = 0.445948490915965, WA = 0.223381589678011 / 2.0;
+  const double B6 = 0.091576213509771, WB = 0.109951743655322 / 2.0;
+  const double tri3[3][3] = {{1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0},
+                             {2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0},
+                             {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0}};
+  const double tri6[6][3] = {{A6, A6, WA}, {1.0 - 2.0 * A6, A6, WA}, {A6, 1.0 - 2.0 * A6, WA},
+                             {B6, B6, WB}, {1.0 - 2.0 * B6, B6, WB}, {B6, 1.0 - 2.0 * B6, WB}};
+  const int ng = m == 3 ? 3 : 6;
+  double nv[6][6], dn[6][6][2], wq[6];
+  for (int q = 0; q < ng; ++q) {
+    const double* gp = m == 3 ? tri3[q] : tri6[q];
+    const double xi = gp[0], eta = gp[1];
+    wq[q] = gp[2];
+    if (m == 3) {
+      nv[q][0] = 1.0 - xi - eta; nv[q][1] = xi; nv[q][2] = eta;
+      dn[q][0][0] = -1.0; dn[q][0][1] = -1.0;
+      dn[q][1][0] = 1.0; dn[q][1][1] = 0.0;
+      dn[q][2][0] = 0.0; dn[q][2][1] = 1.0;
+    } else {
+      const double l1 = 1.0 - xi - eta, l2 = xi, l3 = eta;
+      nv[q][0] = l1 * (2.0 * l1 - 1.0);
+      nv[q][1] = l2 * (2.0 * l2 - 1.0);
+      nv[q][2] = l3 * (2.0 * l3 - 1.0);
+      nv[q][3] = 4.0 * l1 * l2;
+      nv[q][4] = 4.0 * l2 * l3;
+      nv[q][5] = 4.0 * l3 * l1;
+      dn[q][0][0] = 1.0 - 4.0 * l1; dn[q][0][1] = 1.0 - 4.0 * l1;
+      dn[q][1][0] = 4.0 * l2 - 1.0; dn[q][1][1] = 0.0;
+      dn[q][2][0] = 0.0; dn[q][2][1] = 4.0 * l3 - 1.0;
+      dn[q][3][0] = 4.0 * (l1 - l2); dn[q][3][1] = -4.0 * l2;
+      dn[q][4][0] = 4.0 * l3; dn[q][4][1] = 4.0 * l2;
+      dn[q][5][0] = -4.0 * l3; dn[q][5][1] = 4.0 * (l1 - l3);
+    }
+  }
+  auto tangents = [&](const double (*xx)[3], int q, double* a, double* b) {
+    for (int c = 0; c < 3; ++c) {
+      double s0 = 0.0, s1 = 0.0;
+      for (int k = 0; k < m; ++k) s0 += dn[q][k][0] * xx[k][c];
+      for (int k = 0; k < m; ++k) s1 += dn[q][k][1] * xx[k][c];
+      a[c] = s0;
+      b[c] = s1;
+    }
+  };
+  auto area_vector = [](const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  auto levi = [](int i, int j, int k) {
+    const double a = i, b = j, c = k;
+    return (a - b) * (b - c) * (c - a) / 2.0;
+  };
+  double ta[3], tb[3], av[3];
+  tangents(ref, 0, ta, tb);
+  area_vector(ta, tb, av);
+  double along = 0.0;
+  for (int c = 0; c < 3; ++c) along += av[c] * normal[c];
+  const double sign = along < 0.0 ? -1.0 : 1.0;
+  std::fill(forces, forces + 3 * m, 0.0);
+  std::fill(stiffness, stiffness + 9 * m * m, 0.0);
+  for (int q = 0; q < ng; ++q) {
+    double xi[3], eta[3], area[3];
+    tangents(cur, q, xi, eta);
+    area_vector(xi, eta, area);
+    const double w = -value * sign * wq[q];
+    for (int a = 0; a < m; ++a)
+      for (int c = 0; c < 3; ++c) forces[3 * a + c] += w * nv[q][a] * area[c];
+    for (int b = 0; b < m; ++b)
+      for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 3; ++k) {
+          double d = 0.0;
+          for (int l = 0; l < 3; ++l)
+            d += levi(c, k, l) * dn[q][b][0] * eta[l] + levi(c, l, k) * xi[l] * dn[q][b][1];
+          for (int a = 0; a < m; ++a) stiffness[(3 * a + c) * 3 * m + 3 * b + k] += w * nv[q][a] * d;
+        }
+  }
+  return m;
+}
+
 template <class T>
 void swap_bufs(DBuf<T>& a, DBuf<T>& b) {
   std::swap(a.p, b.p);
@@ -1216,6 +1340,16 @@ struct QuasiStatic {
   DBuf<double> d_yield, d_hard, d_committed, d_trial, d_fint_e, d_fint, d_negr, d_react,
       d_records, d_du, d_u, d_ut, d_f, d_partial, d_sums;
   DBuf<int> d_flag;
+  // Kinematics::Large: per-element law (1 Neo-Hookean, 0 Saint-Venant-Kirchhoff) and follower
+  // pressures (element, face, value), whose forces and stiffness follow u (follower_force,
+  // model.rs:868-903)
+  bool large = false;
+  DBuf<unsigned char> d_neo;
+  std::vector<int64_t> pres_el, pres_face;
+  std::vector<double> pres_val;
+  DBuf<double> d_follow, d_gval;
+  DBuf<int64_t> d_goff, d_gslot;
+  std::vector<double> h_u;
   double scale = 0.0;
   std::vector<double> residuals;
   int64_t newton = 0, cuts = 0, pcg_iterations = 0;
@@ -1268,7 +1402,16 @@ struct QuasiStatic {
     hipStream_t st = m.s();
     mlk::k_fill_i32<<<1, 256, 0, st>>>(d_flag.p, 1, 0);
     hipEvent_t ev = m.op.prof.begin(st);
-    if (m.plan.nen == 4)
+    if (large) {
+      if (m.plan.nen == 4)
+        mlk::k_large_response<4><<<grid_for(12 * ne), 256, 0, st>>>(
+            ne, m.d_xyz.p, m.d_conn.p, m.d_young.p, m.d_nu.p, d_neo.p, u, m.d_ke.p, d_fint_e.p,
+            d_trial.p, d_records.p);
+      else
+        mlk::k_large_response<10><<<grid_for(30 * ne), 256, 0, st>>>(
+            ne, m.d_xyz.p, m.d_conn.p, m.d_young.p, m.d_nu.p, d_neo.p, u, m.d_ke.p, d_fint_e.p,
+            d_trial.p, d_records.p);
+    } else if (m.plan.nen == 4)
       mlk::k_response<4><<<grid_for(12 * ne), 256, 0, st>>>(
           ne, m.d_xyz.p, m.d_conn.p, m.d_young.p, m.d_nu.p, d_yield.p, d_hard.p, u,
           d_committed.p, m.d_ke.p, d_fint_e.p, d_trial.p, d_records.p, d_flag.p);
@@ -1287,6 +1430,71 @@ struct QuasiStatic {
     HIP_CHECK(hipGetLastError());
   }
 
+  // follower_force (model.rs:868-903) at u: the follower forces per unit lambda into d_follow
+  // (pressure after pressure, as the reference adds them) and -lambda sym(K_p) added into each
+  // element's tangent, its pressures in order. The terms are computed on the host from u.
+  void follower(const double* u, double lambda) {
+    hipStream_t st = m.s();
+    const int nen = m.plan.nen, mm = 3 * nen;
+    const int64_t ne = m.plan.n_elements;
+    h_u.resize(ndof);
+    HIP_CHECK(hipMemcpyAsync(h_u.data(), u, sizeof(double) * ndof, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<double> fol(ndof, 0.0);
+    // per element (in element order), its pressures' tangent corrections in pressure order
+    std::vector<std::vector<int64_t>> by_el_idx;
+    std::vector<int64_t> els;
+    {
+      std::vector<int64_t> order(pres_el.size());
+      for (size_t k = 0; k < order.size(); ++k) order[k] = (int64_t)k;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int64_t a, int64_t b) { return pres_el[a] < pres_el[b]; });
+      for (int64_t k : order) {
+        if (els.empty() || els.back() != pres_el[k]) {
+          els.push_back(pres_el[k]);
+          by_el_idx.emplace_back();
+        }
+        by_el_idx.back().push_back(k);
+      }
+    }
+    std::vector<int> npos(pres_el.size());
+    std::vector<std::array<int, 6>> pos(pres_el.size());
+    std::vector<std::array<double, 18>> forces(pres_el.size());
+    std::vector<std::array<double, 324>> stiff(pres_el.size());
+    for (size_t k = 0; k < pres_el.size(); ++k)
+      npos[k] = follower_terms(nen, m.conn.data() + pres_el[k] * nen, m.nodes.data(),
+                               (int)pres_face[k], pres_val[k], h_u.data(), pos[k].data(),
+                               forces[k].data(), stiff[k].data());
+    for (size_t k = 0; k < pres_el.size(); ++k) {
+      const int64_t* en = m.conn.data() + pres_el[k] * nen;
+      for (int a = 0; a < npos[k]; ++a)
+        for (int c = 0; c < 3; ++c) fol[3 * en[pos[k][a]] + c] += forces[k][3 * a + c];
+    }
+    std::vector<int64_t> goff(1, 0), gslot;
+    std::vector<double> gval;
+    for (size_t gi = 0; gi < els.size(); ++gi) {
+      const int64_t e = els[gi];
+      for (int64_t k : by_el_idx[gi]) {
+        const int n3 = 3 * npos[k];
+        for (int r = 0; r < n3; ++r)
+          for (int c = 0; c < n3; ++c) {
+            const double sym = 0.5 * (stiff[k][r * n3 + c] + stiff[k][c * n3 + r]);
+            const int row = 3 * pos[k][r / 3] + r % 3, col = 3 * pos[k][c / 3] + c % 3;
+            gslot.push_back(((int64_t)row * mm + col) * ne + e);
+            gval.push_back(-(lambda * sym));
+          }
+      }
+      goff.push_back((int64_t)gslot.size());
+    }
+    d_follow.upload_vec(fol, st);
+    d_goff.upload_vec(goff, st);
+    d_gslot.upload_vec(gslot, st);
+    d_gval.upload_vec(gval, st);
+    mlk::k_add_groups<<<grid_for((int64_t)els.size()), 256, 0, st>>>((int64_t)els.size(), d_goff.p,
+                                                                    d_gslot.p, d_gval.p, m.d_ke.p);
+    HIP_CHECK(hipGetLastError());
+  }
+
   // equilibrium (model.rs:923-1022) from u (device, updated in place)
   int equilibrium(double* u, double lambda, double& sc) {
     residuals.clear();
@@ -1294,8 +1502,11 @@ struct QuasiStatic {
     for (;;) {
       double t0 = now_s();
       responses(u);
-      mlk::k_newton_residual<<<nblk, 256, 0, st>>>(ndof, lambda, d_f.p, d_fint.p, m.d_rank.p,
-                                                  d_negr.p, d_react.p, d_partial.p);
+      const bool follows = large && !pres_el.empty();
+      if (follows) follower(u, lambda);
+      mlk::k_newton_residual<<<nblk, 256, 0, st>>>(ndof, lambda, d_f.p,
+                                                  follows ? d_follow.p : nullptr, d_fint.p,
+                                                  m.d_rank.p, d_negr.p, d_react.p, d_partial.p);
       mlk::k_sum3<<<1, 256, 0, st>>>(d_partial.p, (int)nblk, d_sums.p);
       HIP_CHECK(hipGetLastError());
       double sums[3];
@@ -2581,6 +2792,89 @@ ml_status ml_thermal_solve_step(ml_thermal* h, double time_period, int64_t incre
 }
 
 void ml_thermal_destroy(ml_thermal* h) { delete h; }
+
+ml_status ml_model_solve_large_strain(ml_model* h, const uint8_t* neo_hookean, const double* f,
+                                      int64_t npressures, const int64_t* pres_elements,
+                                      const int64_t* pres_faces, const double* pres_values,
+                                      double time_period, int64_t increments, int64_t npoints,
+                                      const double* amp_t, const double* amp_v, double* u,
+                                      double* reactions, double* stresses, double* residuals,
+                                      int64_t* residual_counts, ml_quasi_static_info* info) {
+  return guarded([&]() -> int {
+    if (!h || !f || !amp_t || !amp_v) throw ArgError("null argument");
+    if (npressures > 0 && (!pres_elements || !pres_faces || !pres_values))
+      throw ArgError("null argument");
+    Model& m = h->m;
+    const int64_t ne = m.plan.n_elements, ndof = m.plan.ndof;
+    if (!(std::isfinite(time_period) && time_period > 0.0))
+      throw ArgError("the Step's time period must be finite and > 0");
+    if (increments <= 0) throw ArgError("a Step needs at least one increment");
+    if (npoints <= 0) throw ArgError("amplitude point 0 is invalid");
+    for (int64_t k = 0; k < npoints; ++k) {
+      bool ok = std::isfinite(amp_t[k]) && std::isfinite(amp_v[k]);
+      const double eps = std::numeric_limits<double>::epsilon();
+      ok = ok && (k == 0 ? (std::fabs(amp_t[k]) <= eps && std::fabs(amp_v[k]) <= eps)
+                         : amp_t[k] > amp_t[k - 1]);
+      if (!ok) throw ArgError("amplitude point " + std::to_string(k) + " is invalid");
+    }
+    for (int64_t g = 0; g < ndof; ++g)
+      if (!std::isfinite(f[g])) throw ArgError("loads must be finite");
+    for (int64_t k = 0; k < npressures; ++k) {
+      if (pres_elements[k] < 0 || pres_elements[k] >= ne || pres_faces[k] < 0 || pres_faces[k] > 3)
+        throw ArgError("pressure face out of range");
+      if (!std::isfinite(pres_values[k])) throw ArgError("pressure must be finite");
+    }
+    if (info) {
+      std::memset(info, 0, sizeof(*info));
+      info->failed_increment = -1;
+    }
+    const double t_start = now_s();
+    std::vector<double> inf(ne, std::numeric_limits<double>::infinity()), zero(ne, 0.0);
+    QuasiStatic qs(m, inf.data(), zero.data(), f);
+    qs.large = true;
+    std::vector<unsigned char> neo(ne, 0);
+    if (neo_hookean)
+      for (int64_t e = 0; e < ne; ++e) neo[e] = neo_hookean[e] ? 1 : 0;
+    qs.d_neo.upload_vec(neo, m.s());
+    qs.pres_el.assign(pres_elements, pres_elements + npressures);
+    qs.pres_face.assign(pres_faces, pres_faces + npressures);
+    qs.pres_val.assign(pres_values, pres_values + npressures);
+    qs.amp_t.assign(amp_t, amp_t + npoints);
+    qs.amp_v.assign(amp_v, amp_v + npoints);
+    qs.responses(qs.d_u.p);  // model.rs:767 (the mesh at rest)
+    qs.record(0, u, reactions, stresses, nullptr);
+    const double dt = time_period / (double)increments;
+    int st = ML_OK;
+    for (int64_t inc = 1; inc <= increments; ++inc) {
+      const double time = (double)inc * dt;
+      st = qs.advance(time - dt, time, 0);
+      if (st != ML_OK) {
+        if (info) {
+          info->failed_increment = inc;
+          info->failed_time = time;
+        }
+        break;
+      }
+      qs.record(inc, u, reactions, stresses, nullptr);
+      if (residual_counts) residual_counts[inc - 1] = (int64_t)qs.residuals.size();
+      if (residuals)
+        std::copy(qs.residuals.begin(), qs.residuals.end(),
+                  residuals + (inc - 1) * (QuasiStatic::NEWTON_ITERATIONS + 1));
+    }
+    HIP_CHECK(hipStreamSynchronize(m.s()));
+    if (info) {
+      info->newton_iterations = qs.newton;
+      info->cuts = qs.cuts;
+      info->pcg_iterations = qs.pcg_iterations;
+      info->device_retries = qs.device_retries;
+      info->seconds[0] = qs.t_resp;
+      info->seconds[1] = qs.t_factor;
+      info->seconds[2] = qs.t_solve;
+      info->seconds[3] = now_s() - t_start;
+    }
+    return st;
+  });
+}
 
 ml_status ml_model_add_pressures(ml_model* h, int64_t count, const int64_t* elements,
                                  const int64_t* faces, const double* values, double* f) {

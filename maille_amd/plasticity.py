@@ -146,3 +146,55 @@ def solve_quasi_static(model, step: Step, amplitude: Amplitude, yield_stress, ha
                              equivalent_plastic_strains=p[k],
                              residuals=res[k - 1, :cnt[k - 1]].copy() if k else np.zeros(0)))
     return out, info.as_dict()
+
+
+def solve_large_strain(model, step: Step, amplitude: Amplitude, neo_hookean=False, pressures=(),
+                       f=None):
+    """Model::solve_quasi_static with Kinematics::Large (model.rs:732-1022) on the device for
+    ``model`` (a ``maille_amd.Model``): Saint-Venant-Kirchhoff elements, or Neo-Hookean where
+    ``neo_hookean`` (bool or per element) says so; ``pressures`` [(element, face, value)] follow
+    the deformed faces. ``f`` defaults to the problem's dead loads: nodal loads and edge loads
+    (pressures are follower loads here, as model.rs:1108-1114 leaves them out of F). Returns
+    (increments, info); stresses are Cauchy stresses at the current Gauss-point positions."""
+    from . import (QuasiStaticInfo, _ai64, _check, _d, _f64, _i, _U8, lib)
+    ne, nen, ndof = model.ne, model.nen, model.ndof
+    ngp = 1 if nen == 4 else 4
+    neo = np.ascontiguousarray(np.broadcast_to(np.asarray(neo_hookean, dtype=np.uint8), (ne,)))
+    if f is None:
+        f = _f64(model.problem.f).copy()
+        el = getattr(model.problem, "edge_loads", None)
+        if el:
+            e = _ai64([q[0] for q in el])
+            ed = _ai64([q[1] for q in el])
+            q = _f64(np.array([q[2] for q in el], dtype=np.float64).reshape(-1, 3))
+            _check(lib().ml_model_add_edge_loads(model._h, len(e), _i(e), _i(ed), _d(q), _d(f)))
+    else:
+        f = _f64(f)
+    pr = list(pressures)
+    pe = _ai64([q[0] for q in pr]) if pr else np.zeros(1, dtype=np.int64)
+    pf = _ai64([q[1] for q in pr]) if pr else np.zeros(1, dtype=np.int64)
+    pv = _f64([q[2] for q in pr]) if pr else np.zeros(1)
+    n = int(step.increments)
+    at = _f64([t for t, _ in amplitude.points])
+    av = _f64([v for _, v in amplitude.points])
+    u = np.zeros((n + 1, ndof))
+    r = np.zeros((n + 1, ndof))
+    s = np.zeros((n + 1, ne, ngp, 9))
+    res = np.zeros((n, NEWTON_ITERATIONS + 1))
+    cnt = np.zeros(n, dtype=np.int64)
+    info = QuasiStaticInfo()
+    st = lib().ml_model_solve_large_strain(
+        model._h, neo.ctypes.data_as(_U8), _d(f), len(pr), _i(pe), _i(pf), _d(pv),
+        float(step.time_period), n, len(at), _d(at), _d(av), _d(u), _d(r), _d(s), _d(res),
+        _i(cnt), C.byref(info))
+    try:
+        _check(st, info)
+    except Exception as err:
+        err.increment = info.failed_increment
+        err.cuts = info.cuts
+        raise
+    dt = step.time_step
+    return ([Increment(time=k * dt, displacements=u[k], reactions=r[k], stresses=s[k],
+                       equivalent_plastic_strains=np.zeros((ne, ngp)),
+                       residuals=res[k - 1, :cnt[k - 1]].copy() if k else np.zeros(0))
+             for k in range(n + 1)], info.as_dict())
