@@ -395,6 +395,90 @@ def run_transient(args, ma, meshes):
           "cpu_baseline": cpu, "speedup_vs_cpu": round(cpu["value"] / wall, 1) if cpu else None})
 
 
+def run_large(args, ma, meshes):
+    """Large strains (Kinematics::Large) with follower pressures: a Neo-Hookean Tet4 block of
+    40x40x40 cells (384,000 elements, 203,043 free DOF), symmetry supports on x = 0, y = 0,
+    z = 0 and a follower pressure on its three other faces ramped to the value that compresses
+    it to a stretch of 0.9, in 4 increments; every Newton iteration's element responses,
+    follower terms, assembly, IC(0) and PCG on the device. The oracle's Newton runs the first
+    increment; its costs per Newton and per PCG iteration are scaled to the device run."""
+    from maille_amd import plasticity as P
+    n = 40
+    size = 1.0
+    nodes, conn = meshes.box_grid((n, n, n), (size, size, size))
+    ne = len(conn)
+    young, nu = 210000.0, 0.3
+    lam = young * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = young / (2 * (1 + nu))
+    stretch = 0.9
+    jj = stretch ** 3
+    p = -(mu * (stretch * stretch - 1.0) + lam * np.log(jj)) / jj
+    fixed = np.zeros(3 * len(nodes), dtype=bool)
+    for c in range(3):
+        fixed[3 * np.nonzero(np.abs(nodes[:, c]) < 1e-12)[0] + c] = True
+    faces = ((0, 1, 2), (0, 3, 1), (1, 3, 2), (2, 3, 0))
+    pressures = []
+    for c in range(3):
+        on = np.abs(nodes[:, c] - size) < 1e-12
+        for e in np.nonzero(on[conn].sum(axis=1) >= 3)[0]:
+            for fc, vs in enumerate(faces):
+                if on[conn[e, list(vs)]].all():
+                    pressures.append((int(e), fc, p))
+    prob = meshes.Problem("block", nodes, conn, np.full(ne, young), np.full(ne, nu), fixed,
+                          np.zeros(3 * len(nodes)), np.zeros(3 * len(nodes)))
+    m = ma.Model(prob)
+    step, amp = P.Step(1.0, 4), P.Amplitude.ramp(1.0)
+    P.solve_large_strain(m, P.Step(1.0, 1), P.Amplitude([(0.0, 0.0), (1.0, 1e-3)]),
+                         neo_hookean=True, pressures=pressures)  # warm-up
+    m.profile(enable=1)
+    t0 = time.perf_counter()
+    incs, info = P.solve_large_strain(m, step, amp, neo_hookean=True, pressures=pressures)
+    wall = time.perf_counter() - t0
+    pr = m.profile(enable=0)
+    resp_ms = pr["stiffness"][0] / max(pr["stiffness"][1], 1)
+    tri = pr["precond64"]
+    u = incs[-1].displacements.reshape(-1, 3)
+    hom = float(np.abs(u - (stretch - 1.0) * nodes).max())
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        t1 = time.perf_counter()
+        # the device run's first increment: lambda = 0.25 at t = 0.25
+        first = oracle.large_strain(nodes, conn, young, nu, fixed, np.zeros(3 * len(nodes)),
+                                    np.zeros(3 * len(nodes)), step.time_step, 1,
+                                    [(0.0, 0.0), (step.time_step, 0.25)], neo_hookean=True,
+                                    pressures=pressures)
+        t_first = time.perf_counter() - t1
+        per_newton = t_first / max(first["newton"], 1)
+        est = per_newton * info["newton_iterations"]
+        cpu = {"value": round(est, 2), "unit": "s", "cores": 1, "kind": "port",
+               "sample": (f"oracle Newton on the first increment ({first['newton']} Newton, "
+                          f"{first['pcg']} PCG iterations, {t_first:.1f} s), scaled per Newton "
+                          f"iteration to the device run's {info['newton_iterations']}"),
+               "first_increment_rel_l2": float(np.linalg.norm(incs[1].displacements - first["u"][1])
+                                               / np.linalg.norm(first["u"][1]))}
+    emit({"metric": "large-strain Neo-Hookean Newton-Raphson with follower pressures s, Tet4 "
+                    "block 40^3, 4 increments",
+          "value": round(wall, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+          "data": "synthetic", "dtype": "fp64",
+          "config": {"workload": "neo_hookean_block_40", "elements": int(ne),
+                     "free_dof": prob.nfree, "follower_faces": len(pressures),
+                     "pressure": round(p, 3), "stretch": stretch},
+          "newton_iterations": info["newton_iterations"], "cuts": info["cuts"],
+          "pcg_iterations": info["pcg_iterations"],
+          "residuals": [[float(f"{r:.3e}") for r in inc.residuals] for inc in incs[1:]],
+          "max_dev_from_homogeneous": hom,
+          "phases_s": {"responses_follower": round(info["seconds"][0], 3),
+                       "assembly_ic0": round(info["seconds"][1], 3),
+                       "pcg": round(info["seconds"][2], 3)},
+          "kernels": {"large_response": {"avg_ms": round(resp_ms, 4),
+                                         "launches": pr["stiffness"][1]},
+                      "precondition": {"avg_ms": round(tri[0] / max(tri[1], 1), 4),
+                                       "launches": tri[1],
+                                       "share_of_run": round(tri[0] * 1e-3 / wall, 3)}},
+          "cpu_baseline": cpu, "speedup_vs_cpu": round(cpu["value"] / wall, 1) if cpu else None})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -410,7 +494,7 @@ def main():
                          "against which HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled")
     ap.add_argument("--cpu-iters", type=int, default=8)
     ap.add_argument("--workload", choices=["cube", "beam_cases", "box_batch", "tet10", "simp",
-                                           "stresses", "plastic", "transient"],
+                                           "stresses", "plastic", "transient", "large"],
                     default="cube", help="cube: the headline 3.1M-DOF solve; the others are the "
                     "batched, Tet10 and adjoint workloads (one JSON line each)")
     ap.add_argument("--cases", type=int, default=256)
@@ -432,7 +516,8 @@ def main():
     if args.workload != "cube":
         {"beam_cases": run_beam_cases, "box_batch": run_box_batch, "tet10": run_tet10,
          "simp": run_simp, "stresses": run_stresses,
-         "plastic": run_plastic, "transient": run_transient}[args.workload](args, ma, meshes)
+         "plastic": run_plastic, "transient": run_transient,
+         "large": run_large}[args.workload](args, ma, meshes)
         return
 
     t0 = time.time()
