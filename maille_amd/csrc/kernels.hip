@@ -123,6 +123,12 @@ __global__ void k_gather(int64_t n, const int* __restrict__ src, const S* __rest
   }
 }
 
+// 1 / (double)d of each FP32 diagonal entry of a triangular-solve layout
+__global__ void k_recip64(int64_t n, const float* __restrict__ d, double* __restrict__ r) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = 1.0 / (double)d[i];
+}
+
 template <typename S, typename D>
 __global__ void k_convert(int64_t n, const S* __restrict__ from, D* __restrict__ to) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1604,9 +1610,9 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
     const int* __restrict__ brow, const int* __restrict__ xoff, const int* __restrict__ uoff,
     const int* __restrict__ ucol, const unsigned char* __restrict__ xu,
     const T* __restrict__ xval, const int* __restrict__ ioff, const unsigned char* __restrict__ ik,
-    const T* __restrict__ ival, const T* __restrict__ diag, const T* __restrict__ rhs, T* out,
-    const PcgScalars* __restrict__ S, int* abort_flag, long long limit,
-    long long* __restrict__ trace, int nrep) {
+    const T* __restrict__ ival, const T* __restrict__ diag, const double* __restrict__ rdiag,
+    const T* __restrict__ rhs, T* out, const PcgScalars* __restrict__ S, int* abort_flag,
+    long long limit, long long* __restrict__ trace, int nrep) {
 #pragma clang fp contract(off)
   if (S && S->done) return;
   typedef typename Bits<T>::U U;
@@ -1646,6 +1652,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
     // per-lane state of the solver
     int p = 0, x1 = 0, ip = 0, i1 = 0, pl = 0, l = 0;
     T acc = T(0), rl = T(0), dl = T(1);
+    double rd = 1.0;  // FP32: 1 / (double)diag, see the row loop
     // stage the block: the poller the distinct external columns with the values published so
     // far, read at device scope (a plain load could hit a stale line of this XCD's L2 that
     // still holds the sentinel); the solver the external entries (coefficient, column index)
@@ -1685,6 +1692,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
         i1 = ioff[l + 1] - ib;
         rl = rhs[pl];
         dl = diag[l];
+        if constexpr (std::is_same<T, float>::value) rd = rdiag[l];
         acc = SUBSEQ ? rl : T(0);
       }
     }
@@ -1863,7 +1871,16 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
         for (; k < kend; ++k) {
           T y = acc;
           if (lane == k) {
-            y = SUBSEQ ? acc / dl : (rl - acc) / dl;
+            // FP32: the quotient num / d is RN32(RN64(num * RN64(1 / d))). That product is
+            // within 2^-52 (relative) of num / d, and a quotient of two binary32 numbers is
+            // never a binary32 rounding midpoint (that needs 25 significant bits) and never
+            // within 2^-49 of one, so both round to the same binary32 value (results in the
+            // normal range). Three dependent operations replace the division sequence on the
+            // row chain.
+            if constexpr (std::is_same<T, float>::value)
+              y = (float)((double)(SUBSEQ ? acc : rl - acc) * rd);
+            else
+              y = SUBSEQ ? acc / dl : (rl - acc) / dl;
             yown = y;
             publish(out + pl, y);
             if (TRACE && rep == 0) trace[10 * (long long)nblocks + l] = wall_clock64();

@@ -213,6 +213,7 @@ struct Operator {
     DBuf<unsigned char> ik, xu;
     DBuf<double> xval, ival, diag;
     DBuf<float> xval32, ival32, diag32;
+    DBuf<double> rdiag32;  // 1 / (double)diag32, what the FP32 solve multiplies by
     void set(const TriLayout& t, hipStream_t st) {
       nblk = (int)t.bstart.size() - 1;
       border.upload_vec(t.border, st);
@@ -244,6 +245,7 @@ struct Operator {
       xval32.alloc(n_ext);
       ival32.alloc(n_int);
       diag32.alloc(n_slots);
+      rdiag32.alloc(n_slots);
     }
   } fw, bw;
   int64_t nf = 0;  // strictly lower entries
@@ -456,6 +458,8 @@ struct Operator {
       if (t->n_int)
         mlk::k_gather<T, T><<<grid_for(t->n_int), 256, 0, stream>>>((int64_t)t->n_int, t->isrc.p, l, iv);
       mlk::k_gather<T, T><<<grid_for(n), 256, 0, stream>>>((int64_t)n, t->dsrc.p, l, dv);
+      if (f32)
+        mlk::k_recip64<<<grid_for(n), 256, 0, stream>>>((int64_t)n, t->diag32.p, t->rdiag32.p);
     }
     HIP_CHECK(hipGetLastError());
   }
@@ -483,7 +487,8 @@ struct Operator {
         launch_dataflow(kernel, grid, mlk::TRI_THREADS, stream, nn, nb, (const int*)t.border.p,
                         (const int*)t.bstart.p, (const int*)t.brow.p, (const int*)t.xoff.p,
                         (const int*)t.uoff.p, (const int*)t.ucol.p, (const unsigned char*)t.xu.p,
-                        xv, (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rhs, out,
+                        xv, (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg,
+                        f32 ? (const double*)t.rdiag32.p : (const double*)nullptr, rhs, out,
                         S, d_flags.p, limit, trace, nrep);
       };
       if (fwd)
