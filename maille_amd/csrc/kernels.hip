@@ -1691,8 +1691,10 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
         ip = ioff[l] - ib;
         i1 = ioff[l + 1] - ib;
         rl = rhs[pl];
-        dl = diag[l];
-        if constexpr (std::is_same<T, float>::value) rd = rdiag[l];
+        if constexpr (std::is_same<T, float>::value)
+          rd = rdiag[l];  // the FP32 rows divide through the reciprocal only
+        else
+          dl = diag[l];
         acc = SUBSEQ ? rl : T(0);
       }
     }
