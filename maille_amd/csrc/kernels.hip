@@ -183,7 +183,17 @@ __device__ inline double tet_mapping(const double (&x)[NEN][3], double xi, doubl
 // Column i of B (element.rs:209-220), Voigt (xx, yy, zz, yz, xz, xy).
 template <int NEN>
 __device__ inline void b_column(const double (&g)[NEN][3], int i, double (&col)[6]) {
-  const double gx = g[i / 3][0], gy = g[i / 3][1], gz = g[i / 3][2];
+  // the node's gradient picked by selection: a runtime index into g would move the whole
+  // array to scratch memory
+  const int node = i / 3;
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+  for (int a = 0; a < NEN; ++a)
+    if (a == node) {
+      gx = g[a][0];
+      gy = g[a][1];
+      gz = g[a][2];
+    }
   const int c = i % 3;
   col[0] = c == 0 ? gx : 0.0;
   col[1] = c == 1 ? gy : 0.0;
@@ -191,6 +201,27 @@ __device__ inline void b_column(const double (&g)[NEN][3], int i, double (&col)[
   col[3] = c == 0 ? 0.0 : (c == 1 ? gz : gy);
   col[4] = c == 0 ? gz : (c == 1 ? 0.0 : gx);
   col[5] = c == 0 ? gy : (c == 1 ? gx : 0.0);
+}
+
+// Row `a` of a per-node array picked by selection (no runtime index into register arrays).
+template <int NEN>
+__device__ inline void pick_row(const double (&g)[NEN][3], int a, double (&out)[3]) {
+  out[0] = out[1] = out[2] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NEN; ++k)
+    if (k == a) {
+      out[0] = g[k][0];
+      out[1] = g[k][1];
+      out[2] = g[k][2];
+    }
+}
+template <int NEN>
+__device__ inline double pick(const double (&v)[NEN], int a) {
+  double out = 0.0;
+#pragma unroll
+  for (int k = 0; k < NEN; ++k)
+    if (k == a) out = v[k];
+  return out;
 }
 
 // solid_d (element.rs:193-206)
@@ -581,16 +612,19 @@ __global__ __launch_bounds__(256) void k_large_response(
           const double geometric = i == kc ? s2[j][l] : 0.0;
           a4[i][j][l] = geometric + material;
         }
+    double gb[3];
+    pick_row<NEN>(g, nb, gb);
     {
       double s = 0.0;
-      for (int j = 0; j < 3; ++j) s += pk1[kc][j] * g[nb][j];
+      for (int j = 0; j < 3; ++j) s += pk1[kc][j] * gb[j];
       fi += weight * s;
     }
+#pragma unroll
     for (int na = 0; na < NEN; ++na)
       for (int i = 0; i < 3; ++i) {
         double vv = 0.0;
         for (int j = 0; j < 3; ++j)
-          for (int l = 0; l < 3; ++l) vv += g[na][j] * a4[i][j][l] * g[nb][l];
+          for (int l = 0; l < 3; ++l) vv += g[na][j] * a4[i][j][l] * gb[l];
         acc[3 * na + i] += weight * vv;
       }
     if (col == 0) {
@@ -809,9 +843,12 @@ __global__ __launch_bounds__(256) void k_thermal_matrices(int64_t ne, const doub
     gauss_point(NEN, q, xi, eta, zeta, w);
     double g[NEN][3];
     const double weight = w * fabs(tet_mapping<NEN>(x, xi, eta, zeta, g));
+    double gi[3];
+    pick_row<NEN>(g, i, gi);
+#pragma unroll
     for (int j = 0; j < NEN; ++j) {
       double s = 0.0;
-      for (int c = 0; c < 3; ++c) s += g[i][c] * g[j][c];
+      for (int c = 0; c < 3; ++c) s += gi[c] * g[j][c];
       acc[j] += weight * ke * s;
     }
   }
@@ -825,7 +862,9 @@ __global__ __launch_bounds__(256) void k_thermal_matrices(int64_t ne, const doub
     double g[NEN][3], n[NEN];
     const double weight = w * fabs(tet_mapping<NEN>(x, xi, eta, zeta, g));
     tet_values<NEN>(xi, eta, zeta, n);
-    for (int j = 0; j < NEN; ++j) acc[j] += weight * rc * n[i] * n[j];
+    const double ni = pick<NEN>(n, i);
+#pragma unroll
+    for (int j = 0; j < NEN; ++j) acc[j] += weight * rc * ni * n[j];
   }
   for (int j = 0; j < NEN; ++j) cmat[((int64_t)i * NEN + j) * ne + e] = acc[j];
 }
@@ -1053,6 +1092,7 @@ __global__ __launch_bounds__(256) void k_compliance_gradient(int64_t ne,
     for (int c = 0; c < 3; ++c) ue[3 * a + c] = u[3 * node + c];
   }
   double s = 0.0;
+#pragma unroll
   for (int i = 0; i < M; ++i) {
     double t = 0.0;
 #pragma unroll
