@@ -111,6 +111,8 @@ def _declare(L):
                                    C.c_double, C.c_int, _D, _D, _D, _I, _D]),
         "or_thermal_steady": (C.c_int, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, C.c_int64,
                                         _I, _I, _D, _D, _U8, _D, _D, C.c_int, _D, _D]),
+        "or_system_pcg_parallel": (C.c_double, [_P, _D, C.c_int64, C.c_int, _D]),
+        "or_omp_threads": (C.c_int, []),
         "or_large_response": (None, [C.c_int, _D, C.c_int, C.c_double, C.c_double, _D, _D, _D,
                                      _D]),
         "or_follower_pressure": (C.c_int, [C.c_int, _I, _D, C.c_int, C.c_double, _D, _INT, _D,
@@ -440,6 +442,14 @@ class System:
         a = Csr.from_arrays(off, cols, vals)
         nb = np.linalg.norm(b)
         return float(np.linalg.norm(b - a.multiply(x)) / (nb if nb > 0 else 1.0))
+
+    def pcg_parallel(self, f, iters, threads=0):
+        """(wall seconds per PCG iteration, |r|/|b| after them) of the multi-core (OpenMP)
+        variant on this factored system: a timing baseline, not a parity reference (its sums
+        run in other orders)."""
+        rr = np.zeros(1)
+        t = lib().or_system_pcg_parallel(self._h, _d(_f64(f)), int(iters), int(threads), _d(rr))
+        return float(t), float(rr[0])
 
     def times(self):
         t = np.zeros(4)

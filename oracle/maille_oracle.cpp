@@ -32,6 +32,9 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef int64_t i64;
 
@@ -1962,6 +1965,161 @@ int or_solid_response(int nen, const double* x, double young, double poisson, do
     states[7 * q + 6] = b[q].p;
   }
   return y ? 1 : 0;
+}
+
+// The multi-core CPU baseline (timing only, not a parity reference): the same IC(0)-PCG on the
+// factored System, with OpenMP over the host cores. SpMV row-parallel on the full symmetric
+// matrix, forward and backward triangular solves level-scheduled (rows of one dependency level in
+// parallel, each row summed as the reference sums it), dot products and vector updates parallel.
+// Runs `iters` PCG iterations on the right-hand side of f (no stop test) and returns the wall
+// time per iteration (and, in *rrel, |r|/|b| at the end); threads <= 0 uses the OpenMP default.
+double or_system_pcg_parallel(void* h, const double* f, i64 iters, int threads, double* rrel) {
+  System* s = (System*)h;
+  if (!s->factored || !s->iterative) return -1.0;
+  const Csr& a = s->csr;
+  const i64 n = a.n();
+  const std::vector<double>& l = s->l;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#endif
+  // full symmetric rows for the SpMV
+  std::vector<i64> foff(n + 1, 0);
+  for (i64 i = 0; i < n; ++i)
+    for (i64 p = a.offsets[i]; p < a.offsets[i + 1]; ++p) {
+      foff[i + 1] += 1;
+      if (a.cols[p] != i) foff[a.cols[p] + 1] += 1;
+    }
+  for (i64 i = 0; i < n; ++i) foff[i + 1] += foff[i];
+  std::vector<i64> fcol(foff[n]), cur(foff.begin(), foff.end() - 1);
+  std::vector<double> fval(foff[n]);
+  // upper rows (L^T) for the backward solve: row j holds l_ij for i > j
+  std::vector<i64> uoff(n + 1, 0);
+  for (i64 i = 0; i < n; ++i)
+    for (i64 p = a.offsets[i]; p < a.offsets[i + 1] - 1; ++p) uoff[a.cols[p] + 1] += 1;
+  for (i64 i = 0; i < n; ++i) uoff[i + 1] += uoff[i];
+  std::vector<i64> ucol(uoff[n]), ucur(uoff.begin(), uoff.end() - 1);
+  std::vector<double> uval(uoff[n]);
+  for (i64 i = 0; i < n; ++i)
+    for (i64 p = a.offsets[i]; p < a.offsets[i + 1]; ++p) {
+      const i64 j = a.cols[p];
+      fcol[cur[i]] = j;
+      fval[cur[i]++] = a.values[p];
+      if (j != i) {
+        fcol[cur[j]] = i;
+        fval[cur[j]++] = a.values[p];
+        ucol[ucur[j]] = i;
+        uval[ucur[j]++] = l[p];
+      }
+    }
+  // dependency levels of the forward (L) and backward (L^T) solves
+  std::vector<i64> lev_f(n, 0), lev_b(n, 0);
+  i64 depth_f = 0, depth_b = 0;
+  for (i64 i = 0; i < n; ++i) {
+    i64 d = 0;
+    for (i64 p = a.offsets[i]; p < a.offsets[i + 1] - 1; ++p) d = std::max(d, lev_f[a.cols[p]] + 1);
+    lev_f[i] = d;
+    depth_f = std::max(depth_f, d + 1);
+  }
+  for (i64 i = n - 1; i >= 0; --i) {
+    i64 d = 0;
+    for (i64 p = uoff[i]; p < uoff[i + 1]; ++p) d = std::max(d, lev_b[ucol[p]] + 1);
+    lev_b[i] = d;
+    depth_b = std::max(depth_b, d + 1);
+  }
+  auto bucket = [&](const std::vector<i64>& lev, i64 depth, std::vector<i64>& off,
+                    std::vector<i64>& rows) {
+    off.assign(depth + 1, 0);
+    for (i64 i = 0; i < n; ++i) off[lev[i] + 1] += 1;
+    for (i64 d = 0; d < depth; ++d) off[d + 1] += off[d];
+    rows.resize(n);
+    std::vector<i64> c(off.begin(), off.end() - 1);
+    for (i64 i = 0; i < n; ++i) rows[c[lev[i]]++] = i;
+  };
+  std::vector<i64> fo, fr, bo, br;
+  bucket(lev_f, depth_f, fo, fr);
+  bucket(lev_b, depth_b, bo, br);
+  std::vector<double> b(n), x(n, 0.0), r(n), z(n), y(n), pv(n), ap(n);
+  for (i64 k = 0; k < n; ++k) b[k] = -s->coupling[k];
+  for (i64 g = 0; g < s->ndof; ++g)
+    if (s->free_of[g] >= 0) b[s->rank[s->free_of[g]]] += f[g];
+  double rz = 0.0, pap = 0.0, rz_next = 0.0;
+  const double t0 = now();
+#pragma omp parallel
+  {
+    auto precondition = [&]() {
+      for (i64 d = 0; d < depth_f; ++d) {
+#pragma omp for schedule(static)
+        for (i64 q = fo[d]; q < fo[d + 1]; ++q) {
+          const i64 i = fr[q];
+          double sum = 0.0;
+          const i64 e = a.offsets[i + 1] - 1;
+          for (i64 p = a.offsets[i]; p < e; ++p) sum += l[p] * y[a.cols[p]];
+          y[i] = (r[i] - sum) / l[e];
+        }
+      }
+      for (i64 d = 0; d < depth_b; ++d) {
+#pragma omp for schedule(static)
+        for (i64 q = bo[d]; q < bo[d + 1]; ++q) {
+          const i64 i = br[q];
+          double v = y[i];
+          for (i64 p = uoff[i + 1] - 1; p >= uoff[i]; --p) v -= uval[p] * z[ucol[p]];
+          z[i] = v / l[a.offsets[i + 1] - 1];
+        }
+      }
+    };
+#pragma omp for schedule(static)
+    for (i64 k = 0; k < n; ++k) r[k] = b[k];
+    precondition();
+#pragma omp for schedule(static) reduction(+ : rz)
+    for (i64 k = 0; k < n; ++k) {
+      pv[k] = z[k];
+      rz += r[k] * z[k];
+    }
+    for (i64 it = 0; it < iters; ++it) {
+#pragma omp single
+      {
+        pap = 0.0;
+        rz_next = 0.0;
+      }
+#pragma omp for schedule(static) reduction(+ : pap)
+      for (i64 i = 0; i < n; ++i) {
+        double sum = 0.0;
+        for (i64 p = foff[i]; p < foff[i + 1]; ++p) sum += fval[p] * pv[fcol[p]];
+        ap[i] = sum;
+        pap += pv[i] * sum;
+      }
+      const double alpha = rz / pap;
+#pragma omp for schedule(static)
+      for (i64 k = 0; k < n; ++k) {
+        x[k] += alpha * pv[k];
+        r[k] -= alpha * ap[k];
+      }
+      precondition();
+#pragma omp for schedule(static) reduction(+ : rz_next)
+      for (i64 k = 0; k < n; ++k) rz_next += r[k] * z[k];
+      const double beta = rz_next / rz;
+#pragma omp for schedule(static)
+      for (i64 k = 0; k < n; ++k) pv[k] = z[k] + beta * pv[k];
+#pragma omp single
+      rz = rz_next;
+    }
+  }
+  const double per_iter = (now() - t0) / (double)std::max<i64>(iters, 1);
+  if (rrel) {  // |r| / |b| after the iterations, to check the variant converges as the serial one
+    double rr = 0.0, bb = 0.0;
+    for (i64 k = 0; k < n; ++k) { rr += r[k] * r[k]; bb += b[k] * b[k]; }
+    *rrel = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+  }
+  return per_iter;
+}
+
+// OpenMP threads the parallel baseline would use (1 without OpenMP)
+int or_omp_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }
 
 // model.rs:586-593 + :1353-1369 (product): reactions R = K u - F on the fixed DOFs, 0 elsewhere
