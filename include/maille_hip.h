@@ -224,7 +224,7 @@ int64_t ml_model_levels(const ml_model* m);
  * offsets[nfree+1], cols[nnz], values[nnz]; any pointer may be null. */
 ml_status ml_model_csr(ml_model* m, int64_t* order, int64_t* offsets, int64_t* cols,
                        double* values);
-/* Adjoint compliance sensitivity (no counterpart in the reference; the north-star backward()):
+/* Adjoint compliance sensitivity (no counterpart in the reference; the backward() of the path):
  * for C = f.u with u the last solution of this model and homogeneous essential conditions,
  * dC/dE_e = -u_e^T (K_e/E_e) u_e, since compliance is self-adjoint (adjoint state = u) and K_e
  * is linear in E_e. out[n_elements] (host). A SIMP caller chains dE_e/drho_e itself.

@@ -23,8 +23,20 @@
 //   skyline Cholesky .............. numpy dense solve; reference 4x4 known answer
 //   Model::solve .................. patch test, uniaxial traction, Timoshenko cantilever,
 //                                   hydrostatic cube (reference integration tests)
+//   nodal stresses, edge loads .... the reference's results.rs and edge-load tests restated
+//   von Mises, solid_response ..... plasticity.rs unit tests restated, tangents against finite
+//                                   differences (tests/test_plasticity_oracle.py)
+//   quasi-static Newton ........... tests/plasticity.rs restated: uniaxial load/unload closed
+//                                   form, strain history vs the material point, elastic in one
+//                                   iteration, quadratic convergence, limit load, PCG = skyline
+//   large strains, follower ....... tests/finite_strain.rs restated (rigid rotations, closed
+//                                   forms of both laws, follower compression, cuts), element
+//                                   tangent and follower stiffness against finite differences
+//   thermal, theta method ......... element closed forms; tests/transient.rs restated (Fourier
+//                                   series, amplification factor, steady state, long time)
 // Parity unpinned by any reference fixture (the reference ships none and cannot run here):
-// the exact RCM permutation, exact PCG iteration counts and bitwise CSR values.
+// the exact RCM permutation, exact PCG and Newton iteration counts and bitwise CSR values.
+// The multi-core PCG variant (or_system_pcg_parallel) is a timing baseline only.
 
 #include <algorithm>
 #include <chrono>

@@ -2,7 +2,7 @@
 C ABI. Bit-exact where the device restates the reference's arithmetic operation for operation
 (element stiffness, assembled K_ff, SpMV, IC(0) factor, triangular solves, reactions,
 stresses); displacements of the PCG solve within 1e-10 relative L2 of the oracle's PCG (FP64)
-and 1e-8 (mixed), the north-star tolerances."""
+and 1e-8 (mixed), the parity tolerances of the solve path."""
 import numpy as np
 import pytest
 
@@ -143,7 +143,7 @@ def test_fp64_solve_matches_oracle_pcg(gpu, name):
 @pytest.mark.parametrize("name", list(PROBLEMS))
 def test_mixed_solve_matches_oracle(gpu, name, precision):
     """FP32 operator and/or preconditioner with FP64 iterate, residual and dots: within
-    1e-8 relative L2 of the oracle's FP64 PCG (the north-star bound for mixed precision)."""
+    1e-8 relative L2 of the oracle's FP64 PCG (the bound for mixed precision)."""
     p = PROBLEMS[name]
     print(f"{precision} solve {name}", flush=True)
     ref = oracle.model_solve(p, iterative=True)

@@ -594,7 +594,7 @@ def test_box_grid_matches_a_literal_loop_restatement():
         assert [mids[(min(t[a], t[b]), max(t[a], t[b]))] for a, b in M.TET10_EDGES] == q[e, 4:].tolist()
 
 
-# ------------------------------------------------------------------ adjoint (north-star pattern 3)
+# ------------------------------------------------------------------ adjoint compliance gradient
 def test_compliance_gradient_matches_central_differences():
     """The CPU adjoint dC/dE_e = -u_e^T (K_e/E_e) u_e against central differences of the
     compliance C = f.u computed with the skyline direct solver (solve.rs:10-139), on a small
