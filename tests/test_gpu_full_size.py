@@ -1,0 +1,84 @@
+"""The benchmark cube at its full size on the device (jittered Tet4 cube 100^3: 6,000,000
+elements, 3,060,300 free DOF): the assembled K_ff, the SpMV, the IC(0) factor and the
+preconditioner application equal the oracle's bit for bit, and the solves meet size-independent
+properties (true residual, balance of reactions against the applied load, mixed precision
+against FP64). The oracle's full PCG would take minutes on one core, so the solution itself is
+checked through these properties at this size and against the oracle's PCG at the small sizes
+of the other tests."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from maille_amd import meshes as M
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[full size] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def cube():
+    t = time.time()
+    p = M.jittered_cube(100)
+    say(f"mesh {len(p.conn)} elements in {time.time() - t:.1f} s")
+    return p
+
+
+def test_full_size_operator_factor_and_preconditioner_are_bit_exact(gpu, cube):
+    p = cube
+    t = time.time()
+    s = oracle.System(p.nodes, p.conn, p.young, p.nu, p.fixed, p.imposed)
+    assert s.factorize() == oracle.OK
+    off, cols, vals = s.csr()
+    say(f"oracle system n={s.n} nnz={len(cols)} in {time.time() - t:.1f} s")
+    t = time.time()
+    m = gpu.Model(p)
+    order, doff, dcols, dvals = m.csr()
+    say(f"device model in {time.time() - t:.1f} s")
+    assert np.array_equal(order, s.order())
+    assert np.array_equal(doff, off) and np.array_equal(dcols, cols)
+    assert np.array_equal(dvals, vals)  # bit-exact assembly of 69.3M entries
+    del m, order, doff, dcols, dvals
+    t = time.time()
+    pcg = gpu.Pcg(gpu.Csr(off, cols, vals))
+    a = oracle.Csr.from_arrays(off, cols, vals)
+    x = np.sin(0.37 * np.arange(s.n))
+    assert np.array_equal(pcg.multiply(x), a.multiply(x))  # bit-exact SpMV
+    l, shift = pcg.factor()
+    assert shift == s.shift_index
+    lv = s.l_values()
+    assert np.array_equal(l, lv)  # bit-exact IC(0)
+    r = np.cos(0.11 * np.arange(s.n))
+    assert np.array_equal(pcg.precondition(r), a.precondition(lv, r))
+    say(f"SpMV, factor and preconditioner compared in {time.time() - t:.1f} s")
+
+
+def test_full_size_solutions_meet_the_size_independent_properties(gpu, cube):
+    p = cube
+    t = time.time()
+    m = gpu.Model(p)
+    f = m.loads()
+    u, r, info = m.solve(f)
+    say(f"FP64 solve: {info['iterations']} iterations, true residual {info['true_residual']:.2e} "
+        f"in {time.time() - t:.1f} s")
+    assert info["true_residual"] <= 1e-10
+    # equilibrium: the reactions on the clamped face carry the applied pressure load
+    total_f = f.reshape(-1, 3).sum(axis=0)
+    total_r = r.reshape(-1, 3).sum(axis=0)
+    assert np.abs(total_f + total_r).max() <= 1e-9 * np.abs(total_f).max()
+    assert np.abs(r[~p.fixed]).max() == 0.0
+    # the load is a unit pressure on the unit top face: 1 N downwards in all
+    assert abs(total_f[2] + 1.0) < 1e-12 and np.abs(total_f[:2]).max() < 1e-12
+    t = time.time()
+    mm = gpu.Model(p, precision=gpu.MIXED_IC0)
+    um, _, im = mm.solve(f, reactions=False)
+    say(f"mixed_ic0 solve: {im['iterations']} iterations in {time.time() - t:.1f} s")
+    assert rel_l2(um, u) <= 1e-8
+    # compression: every top node moves down
+    top = np.abs(p.nodes[:, 2] - 1.0) < 1e-12
+    assert (u.reshape(-1, 3)[top, 2] < 0).all()
