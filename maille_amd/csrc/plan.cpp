@@ -103,11 +103,7 @@ int build_plan(const MeshView& m, const uint8_t* fixed, const double* imposed, P
   i64 count = 0;
   for (i64 g = 0; g < p.ndof; ++g)
     if (!fixed[g]) free_of[g] = count++;
-  p.nfree = count;
-  if (count == 0) {
-    *err = "every DOF is fixed";
-    return 3;
-  }
+  p.nfree = count;  // may be 0: every DOF imposed, as FreeSystem::new allows
   // model.rs:1455-1458: per-element free DOFs in element_dofs order (assembly.rs:20-32)
   std::vector<i64> eoff(ne + 1, 0);
   for (i64 e = 0; e < ne; ++e) {

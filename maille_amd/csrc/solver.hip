@@ -1017,11 +1017,15 @@ struct Model {
     if (st) throw ArgError(err);
     if (info) info->seconds[5] = plan.seconds;
     op.precision = precision;
-    op.set_pattern(plan.lower);
+    // with every DOF imposed there is no free system: u is the imposed field and only the
+    // reactions are computed (FreeSystem::new and ::solve with zero unknowns)
+    if (plan.nfree > 0) op.set_pattern(plan.lower);
+    else op.init_stream();
     upload();
     assemble();
     return ML_OK;
   }
+  bool all_fixed() const { return plan.nfree == 0; }
 
   void upload() {
     hipStream_t st = s();
@@ -1069,6 +1073,10 @@ struct Model {
   // model.rs:1480-1492 on the device: Ke of every element, gathered into K_ff and K_fc u_c.
   void assemble() {
     stiffness();
+    if (all_fixed()) {
+      assembled = true;
+      return;
+    }
     hipEvent_t ev = op.prof.begin(s());
     mlk::k_assemble<<<grid_for(op.nnz), 256, 0, s()>>>(op.nnz, d_asm_off.p, d_asm_slot.p, d_ke.p,
                                                        op.d_a.p);
@@ -1087,6 +1095,17 @@ struct Model {
   int solve_device(const double* f, double* u, double* R, ml_solve_info* info) {
     double t0 = now_s();
     const int retries0 = op.retries;
+    if (all_fixed()) {
+      mlk::k_scatter_u<<<grid_for(plan.ndof), 256, 0, s()>>>(plan.ndof, d_rank.p, d_x.p,
+                                                             d_imposed.p, u);
+      HIP_CHECK(hipGetLastError());
+      if (R) reactions(u, f, R);
+      if (u != d_ulast.p)
+        HIP_CHECK(hipMemcpyAsync(d_ulast.p, u, sizeof(double) * plan.ndof, hipMemcpyDeviceToDevice, s()));
+      HIP_CHECK(hipStreamSynchronize(s()));
+      have_u = true;
+      return ML_OK;
+    }
     if (!op.factored) {
       int st = op.factor();
       if (info) info->shift_index = op.shift_index;
@@ -1531,6 +1550,12 @@ struct QuasiStatic {
       t0 = now_s();
       mlk::k_newton_du<<<grid_for(ndof), 256, 0, st>>>(ndof, lambda, m.d_rank.p, m.d_imposed.p,
                                                        u, d_du.p);
+      if (m.all_fixed()) {  // no free DOF: the step is the imposed increment
+        ++newton;
+        mlk::k_newton_update<<<grid_for(ndof), 256, 0, st>>>(ndof, m.d_rank.p, m.d_x.p, d_du.p, u);
+        HIP_CHECK(hipGetLastError());
+        continue;
+      }
       hipEvent_t ev = m.op.prof.begin(st);
       mlk::k_assemble<<<grid_for(m.op.nnz), 256, 0, st>>>(m.op.nnz, m.d_asm_off.p,
                                                           m.d_asm_slot.p, m.d_ke.p, m.op.d_a.p);
@@ -1874,6 +1899,7 @@ struct Thermal {
     if (merr) return ML_INVALID_MESH;
     std::string err;
     if (build_plan(mesh, fixed.data(), imposed.data(), plan, &err)) throw ArgError(err);
+    if (plan.nfree == 0) throw ArgError("every temperature is fixed: nothing to solve for");
     if (info) info->seconds[5] = plan.seconds;
     op.precision = precision;
     op.set_pattern(plan.lower);
@@ -2460,6 +2486,7 @@ ml_status ml_model_factor(ml_model* h, ml_solve_info* info) {
   return guarded([&]() -> int {
     if (!h) throw ArgError("null handle");
     if (!h->m.assembled) h->m.assemble();
+    if (h->m.all_fixed()) return ML_OK;  // no free system to factor
     double t0 = now_s();
     const int retries0 = h->m.op.retries;
     int st = h->m.op.factor();
@@ -2919,6 +2946,7 @@ ml_status ml_model_solve_many(ml_model* h, int64_t nrhs, const double* f, double
   return guarded([&]() -> int {
     if (!h || !f || !u || nrhs <= 0) throw ArgError("bad arguments");
     Model& m = h->m;
+    if (m.all_fixed()) throw ArgError("every DOF is fixed: nothing to solve for");
     if (!m.assembled) m.assemble();
     Operator& op = m.op;
     const int64_t ndof = m.plan.ndof, nf = m.plan.nfree;
@@ -2959,6 +2987,9 @@ ml_status ml_batch_create(int64_t count, ml_model* const* models, ml_precision p
     if (!out || count <= 0 || !models) throw ArgError("bad arguments");
     if (precision != ML_FP64 && precision != ML_MIXED_IC0)
       throw ArgError("batched solves take ML_FP64 or ML_MIXED_IC0");
+    for (int64_t k = 0; k < count; ++k)
+      if (!models[k] || models[k]->m.all_fixed())
+        throw ArgError("every model of a batch needs a free DOF");
     std::unique_ptr<ml_batch> b(new ml_batch);
     b->models.assign(models, models + count);
     b->nseg = (int)count;
@@ -3061,6 +3092,7 @@ ml_status ml_model_autotune(ml_model* h, int64_t nrhs, int32_t* chosen) {
   return guarded([&]() -> int {
     if (!h) throw ArgError("null handle");
     if (!h->m.assembled) h->m.assemble();
+    if (h->m.all_fixed()) return ML_OK;  // nothing to tune
     return h->m.op.autotune(nrhs, chosen);
   });
 }

@@ -105,3 +105,26 @@ def test_a_small_load_gives_the_linear_solution_on_the_device(gpu):
     assert np.abs(incs[1].displacements - u_lin).max() < 1e-5 * eps * L
     u_again, _, _ = model.solve()
     assert rel_l2(u_again, u_lin) <= 1e-12
+
+
+@pytest.mark.parametrize("neo", [False, True], ids=["SVK", "NeoHookean"])
+@pytest.mark.parametrize("mesh", [0, 1], ids=["Tet4", "Tet10"])
+def test_every_node_imposed_rigid_rotation_and_stretch(gpu, mesh, neo):
+    """tests/finite_strain.rs:122-185 and :480-496 on the device: every DOF imposed (no free
+    system, as FreeSystem::new allows); a rigid rotation leaves no stress and no reaction, a
+    prescribed stretch gives the oracle's stresses."""
+    from test_large_strain_oracle import imposed_map, rotation
+    name, nodes, conn = solid_meshes()[mesh]
+    fixed, imposed = imposed_map(nodes, rotation)
+    incs, info = device(gpu, nodes, conn, fixed, imposed, 1, neo)
+    assert np.abs(incs[1].stresses[..., 3:]).max() < 1e-10 * E
+    assert np.abs(incs[1].reactions).max() < 1e-10 * E
+    lat, sxx = svk_uniaxial(1.4)
+    fixed, imposed = imposed_map(nodes, lambda n: [1.4 * n[0], lat * n[1], lat * n[2]])
+    incs, info = device(gpu, nodes, conn, fixed, imposed, 1, neo)
+    run = reference(nodes, conn, fixed, imposed, 1, neo)
+    assert np.array_equal(incs[1].displacements, run["u"][1])
+    assert np.abs(incs[1].stresses - run["stresses"][1]).max() <= 1e-9 * abs(sxx)
+    assert len(incs[1].residuals) == len(run["residuals"][0]) == 2
+    if not neo:
+        assert np.abs(incs[1].stresses[..., 3] - sxx).max() < 1e-9 * sxx

@@ -460,3 +460,22 @@ def test_autotuner_keeps_results(gpu):
     assert 1 <= waves < occ and per_pass in (8, 16, 32, 64, 128, 256)
     assert np.array_equal(pcg.precondition(r), z0)
     assert np.array_equal(pcg.solve(b), x0)
+
+
+def test_a_model_with_every_dof_imposed(gpu):
+    """Model::solve when every DOF is imposed (FreeSystem::new with no unknowns): u is the
+    imposed field and the reactions K u - F equal the oracle's bit for bit."""
+    p = M.cantilever_tet4((4, 2, 2), (20.0, 10.0, 10.0))
+    strain = np.array([1e-3, -2e-4, 3e-4, 1e-4, -5e-4, 2e-4])
+    x = p.nodes
+    u0 = np.stack([strain[0] * x[:, 0] + strain[5] / 2 * x[:, 1] + strain[4] / 2 * x[:, 2],
+                   strain[5] / 2 * x[:, 0] + strain[1] * x[:, 1] + strain[3] / 2 * x[:, 2],
+                   strain[4] / 2 * x[:, 0] + strain[3] / 2 * x[:, 1] + strain[2] * x[:, 2]],
+                  axis=1).ravel()
+    import dataclasses
+    q = dataclasses.replace(p, fixed=np.ones(p.ndof, dtype=bool), imposed=u0, pressures=[])
+    m = gpu.Model(q)
+    u, r, info = m.solve()
+    assert np.array_equal(u, u0) and info["iterations"] == 0
+    ref = oracle.reactions(q.nodes, q.conn, q.young, q.nu, q.fixed, q.f, u0)
+    assert np.array_equal(r, ref)
