@@ -329,6 +329,13 @@ ml_status ml_thermal_solve_step(ml_thermal* h, double time_period, int64_t incre
                                 double theta, const double* f, const double* initial,
                                 double* temperatures, double* reactions, double* fluxes,
                                 ml_solve_info* info);
+/* Model::solve for the thermal model (Analysis::Thermal, model.rs:548-610): K T = F with K the
+ * conduction and convection matrices, the BC temperatures at the fixed nodes, on the device
+ * (the handle's left matrix becomes K and is factored; a later ml_thermal_solve_step refactors).
+ * temperatures[n_nodes]; reactions[n_nodes] (K T - F on the fixed nodes) and
+ * fluxes[n_elements x ngp x 6] may be null. */
+ml_status ml_thermal_solve_steady(ml_thermal* h, const double* f, double* temperatures,
+                                  double* reactions, double* fluxes, ml_solve_info* info);
 void ml_thermal_destroy(ml_thermal* h);
 
 /* ------------------------------------------------------------------ batched solves

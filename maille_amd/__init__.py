@@ -50,7 +50,7 @@ EXPORTS = (
     "ml_msh_read", "ml_msh_sizes", "ml_msh_geometry", "ml_msh_set_info", "ml_msh_set_items",
     "ml_msh_destroy", "ml_model_solve_quasi_static",
     "ml_thermal_create", "ml_thermal_add_loads", "ml_thermal_solve_step", "ml_thermal_destroy",
-    "ml_model_solve_large_strain",
+    "ml_model_solve_large_strain", "ml_thermal_solve_steady",
 )
 
 
@@ -212,6 +212,7 @@ def lib():
             "ml_thermal_solve_step": (_int, [_P, C.c_double, _i64, C.c_double, _D, _D, _D, _D,
                                              _D, _S]),
             "ml_thermal_destroy": (None, [_P]),
+            "ml_thermal_solve_steady": (_int, [_P, _D, _D, _D, _D, _S]),
             "ml_model_solve_large_strain": (_int, [_P, _U8, _D, _i64, _I, _I, _D, C.c_double,
                                                    _i64, _i64, _D, _D, _D, _D, _D, _D, _I,
                                                    C.POINTER(QuasiStaticInfo)]),

@@ -2823,6 +2823,76 @@ ml_status ml_thermal_solve_step(ml_thermal* h, double time_period, int64_t incre
   });
 }
 
+ml_status ml_thermal_solve_steady(ml_thermal* h, const double* f, double* temperatures,
+                                  double* reactions, double* fluxes, ml_solve_info* info) {
+  return guarded([&]() -> int {
+    if (!h || !f || !temperatures) throw ArgError("null argument");
+    Thermal& t = h->t;
+    const int64_t nn = t.plan.n_nodes, ne = t.plan.n_elements;
+    if (info) {
+      std::memset(info, 0, sizeof(*info));
+      info->bad_element = -1;
+    }
+    // K T = F (Model::solve, Analysis::Thermal): the left matrix with C / dt = 0 and theta = 1
+    // is K itself (0 + 1 K rounds to K)
+    const double t0 = now_s();
+    int st = t.prepare(std::numeric_limits<double>::infinity(), 1.0);
+    if (info) info->shift_index = t.op.shift_index;
+    if (st != ML_OK) return st;
+    const double t1 = now_s();
+    hipStream_t s = t.s();
+    t.d_t.upload_vec(t.imposed, s);  // the BC temperatures at the fixed nodes
+    t.d_f.upload(f, nn, s);
+    // the step's carried term is (C/dt - 0 K) T = 0: F alone on the right
+    mlk::k_fill_f64<<<grid_for(nn), 256, 0, s>>>(t.d_carry.p, nn, 0.0);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(t.d_carry.p, t.d_f.p, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
+    ml_solve_info si;
+    std::memset(&si, 0, sizeof(si));
+    mlk::k_rhs<<<grid_for(nn), 256, 0, s>>>(nn, t.d_rank.p, t.d_carry.p,
+                                            t.plan.cpl_slot.empty() ? nullptr : t.d_coupling.p,
+                                            t.d_rhs.p);
+    HIP_CHECK(hipGetLastError());
+    st = t.op.solve(t.d_rhs.p, t.d_x.p, 0, 0, &si);
+    if (info) {
+      info->iterations = si.iterations;
+      info->residual = si.residual;
+      info->true_residual = si.true_residual;
+      info->device_retries = si.device_retries;
+    }
+    if (st != ML_OK) return st;
+    mlk::k_scatter_u<<<grid_for(nn), 256, 0, s>>>(nn, t.d_rank.p, t.d_x.p, t.d_t.p, t.d_t.p);
+    mlk::k_fill_f64<<<grid_for(nn), 256, 0, s>>>(t.d_R.p, nn, 0.0);
+    const int64_t nfixed = (int64_t)t.plan.rea_dof.size();
+    if (nfixed) {
+      if (t.plan.nen == 4)
+        mlk::k_thermal_reactions<4><<<grid_for(nfixed), 256, 0, s>>>(
+            nfixed, t.d_rea_dof.p, t.d_rea_off.p, t.d_rea_item.p, t.d_conn.p, ne, t.d_left.p,
+            t.d_t.p, t.d_carry.p, t.d_R.p);
+      else
+        mlk::k_thermal_reactions<10><<<grid_for(nfixed), 256, 0, s>>>(
+            nfixed, t.d_rea_dof.p, t.d_rea_off.p, t.d_rea_item.p, t.d_conn.p, ne, t.d_left.p,
+            t.d_t.p, t.d_carry.p, t.d_R.p);
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(temperatures, t.d_t.p, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+    if (reactions)
+      HIP_CHECK(hipMemcpyAsync(reactions, t.d_R.p, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+    if (fluxes) {
+      t.fluxes(t.d_q.p);
+      HIP_CHECK(hipMemcpyAsync(fluxes, t.d_q.p,
+                               sizeof(double) * 6 * (t.plan.nen == 4 ? 1 : 4) * ne,
+                               hipMemcpyDeviceToHost, s));
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (info) {
+      info->seconds[1] = t1 - t0;
+      info->seconds[2] = now_s() - t1;
+    }
+    return ML_OK;
+  });
+}
+
 void ml_thermal_destroy(ml_thermal* h) { delete h; }
 
 ml_status ml_model_solve_large_strain(ml_model* h, const uint8_t* neo_hookean, const double* f,

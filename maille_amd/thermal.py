@@ -117,6 +117,20 @@ class ThermalModel:
                                   heat_fluxes=q[k] if fluxes else None) for k in range(n + 1)],
                 si.as_dict())
 
+    def solve(self, fluxes: bool = True):
+        """Model::solve for the thermal model (model.rs:548-610): K T = F on the device.
+        Returns (ThermalIncrement at time 0 with the steady temperatures, reactions and heat
+        fluxes, solve info)."""
+        f = self.assemble_loads()
+        ngp = 1 if self.nen == 4 else 4
+        t = np.zeros(self.n_nodes)
+        r = np.zeros(self.n_nodes)
+        q = np.zeros((self.ne, ngp, 6)) if fluxes else None
+        si = SolveInfo()
+        _check(lib().ml_thermal_solve_steady(self._h, _d(f), _d(t), _d(r),
+                                             _d(q) if fluxes else None, C.byref(si)), si)
+        return ThermalIncrement(time=0.0, temperatures=t, reactions=r, heat_fluxes=q), si.as_dict()
+
     def __del__(self):
         if getattr(self, "_h", None) and getattr(self, "_lib", None) is not None:
             self._lib.ml_thermal_destroy(self._h)

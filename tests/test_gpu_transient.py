@@ -106,3 +106,27 @@ def test_convection_and_sources_on_a_tet10_slab_match_the_oracle(gpu):
     run = oracle.transient(nodes, conn, K, RHO_C, fixed, imposed, f, zero, 0.5, 10, theta=0.75,
                            convection=conv)
     compare(run, incs, 100.0)
+
+
+@pytest.mark.parametrize("name", ["Tet4", "Tet10"])
+def test_steady_thermal_solve_matches_the_oracle(gpu, name):
+    """Model::solve of the thermal model on the device: the walled slab with sources and a
+    convection face, against the oracle's steady solve; a following Step from that field stays
+    on it (the reuse of one handle for both)."""
+    nodes, conn, fixed, imposed, f = walled(name)
+    conv = convection_at(nodes, conn, L, 3.0, 15.0)
+    fixed2 = fixed.copy()
+    fixed2[np.abs(nodes[:, 0] - L) < 1e-12] = 0
+    sources = [(e, 300.0) for e in range(len(conn))]
+    m = T.ThermalModel(nodes, conn, K, RHO_C, fixed2, imposed, heat_sources=sources,
+                       convection=conv)
+    f2 = oracle.thermal_loads(nodes, conn, np.zeros(len(nodes)), sources, conv)
+    inc, info = m.solve()
+    st, t_ref, r_ref = oracle.thermal_steady(nodes, conn, K, fixed2, imposed, f2, convection=conv)
+    assert st == oracle.OK
+    assert np.abs(inc.temperatures - t_ref).max() <= 1e-10 * np.abs(t_ref).max()
+    assert np.abs(inc.reactions - r_ref).max() <= 1e-8 * np.abs(r_ref).max()
+    assert info["iterations"] > 0
+    incs, _ = m.solve_step(T.Step(1.0, 4).with_theta(0.5), inc.temperatures)
+    for i in incs[1:]:
+        assert np.abs(i.temperatures - inc.temperatures).max() < 1e-9 * np.abs(t_ref).max()
