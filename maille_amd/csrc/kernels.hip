@@ -123,8 +123,9 @@ __global__ void k_gather(int64_t n, const int* __restrict__ src, const S* __rest
   }
 }
 
-// 1 / (double)d of each FP32 diagonal entry of a triangular-solve layout
-__global__ void k_recip64(int64_t n, const float* __restrict__ d, double* __restrict__ r) {
+// 1 / (double)d, correctly rounded, of each diagonal entry of a triangular-solve layout
+template <typename T>
+__global__ void k_recip64(int64_t n, const T* __restrict__ d, double* __restrict__ r) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) r[i] = 1.0 / (double)d[i];
 }
@@ -1731,10 +1732,8 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
         ip = ioff[l] - ib;
         i1 = ioff[l + 1] - ib;
         rl = rhs[pl];
-        if constexpr (std::is_same<T, float>::value)
-          rd = rdiag[l];  // the FP32 rows divide through the reciprocal only
-        else
-          dl = diag[l];
+        rd = rdiag[l];
+        if constexpr (!std::is_same<T, float>::value) dl = diag[l];  // FP64: for the corrections
         acc = SUBSEQ ? rl : T(0);
       }
     }
@@ -1919,10 +1918,21 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
             // within 2^-49 of one, so both round to the same binary32 value (results in the
             // normal range). Three dependent operations replace the division sequence on the
             // row chain.
-            if constexpr (std::is_same<T, float>::value)
+            // FP64: q = num * RN(1/d), then two exact-remainder corrections
+            // q += fma(-q, d, num) * RN(1/d). After the first, q is within one ulp of num / d;
+            // with a faithful q and the correctly rounded reciprocal, the second gives the
+            // correctly rounded quotient (Markstein's theorem; normal range). Five dependent
+            // operations replace the division sequence.
+            if constexpr (std::is_same<T, float>::value) {
               y = (float)((double)(SUBSEQ ? acc : rl - acc) * rd);
-            else
-              y = SUBSEQ ? acc / dl : (rl - acc) / dl;
+            } else {
+              const double num = SUBSEQ ? acc : rl - acc;
+              double q = num * rd;
+              double r = fma(-q, dl, num);
+              q = fma(r, rd, q);
+              r = fma(-q, dl, num);
+              y = fma(r, rd, q);
+            }
             yown = y;
             publish(out + pl, y);
             if (TRACE && rep == 0) trace[10 * (long long)nblocks + l] = wall_clock64();

@@ -213,7 +213,7 @@ struct Operator {
     DBuf<unsigned char> ik, xu;
     DBuf<double> xval, ival, diag;
     DBuf<float> xval32, ival32, diag32;
-    DBuf<double> rdiag32;  // 1 / (double)diag32, what the FP32 solve multiplies by
+    DBuf<double> rdiag32, rdiag64;  // 1 / (double)diag, what the solves multiply by
     void set(const TriLayout& t, hipStream_t st) {
       nblk = (int)t.bstart.size() - 1;
       border.upload_vec(t.border, st);
@@ -240,6 +240,7 @@ struct Operator {
       xval.alloc(n_ext);
       ival.alloc(n_int);
       diag.alloc(n_slots);
+      rdiag64.alloc(n_slots);
     }
     void alloc32() {
       xval32.alloc(n_ext);
@@ -459,7 +460,11 @@ struct Operator {
         mlk::k_gather<T, T><<<grid_for(t->n_int), 256, 0, stream>>>((int64_t)t->n_int, t->isrc.p, l, iv);
       mlk::k_gather<T, T><<<grid_for(n), 256, 0, stream>>>((int64_t)n, t->dsrc.p, l, dv);
       if (f32)
-        mlk::k_recip64<<<grid_for(n), 256, 0, stream>>>((int64_t)n, t->diag32.p, t->rdiag32.p);
+        mlk::k_recip64<float><<<grid_for(n), 256, 0, stream>>>((int64_t)n, t->diag32.p,
+                                                               t->rdiag32.p);
+      else
+        mlk::k_recip64<double><<<grid_for(n), 256, 0, stream>>>((int64_t)n, t->diag.p,
+                                                                t->rdiag64.p);
     }
     HIP_CHECK(hipGetLastError());
   }
@@ -488,7 +493,7 @@ struct Operator {
                         (const int*)t.bstart.p, (const int*)t.brow.p, (const int*)t.xoff.p,
                         (const int*)t.uoff.p, (const int*)t.ucol.p, (const unsigned char*)t.xu.p,
                         xv, (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg,
-                        f32 ? (const double*)t.rdiag32.p : (const double*)nullptr, rhs, out,
+                        f32 ? (const double*)t.rdiag32.p : (const double*)t.rdiag64.p, rhs, out,
                         S, d_flags.p, limit, trace, nrep);
       };
       if (fwd)
