@@ -18,7 +18,13 @@ MAX_CUTS = 5             # model.rs:1224
 
 class PlasticityInputError(ValueError):
     """MailleError's input refusals of the quasi-static path (NoIncrements, InvalidAmplitude,
-    NonPositive / NegativeProperty of the plasticity parameters)."""
+    NonPositive / NegativeProperty of the plasticity parameters). ``kind`` names the variant;
+    ``point`` is the offending amplitude point for InvalidAmplitude."""
+
+    def __init__(self, message, kind="invalid_property", point=None):
+        super().__init__(message)
+        self.kind = kind
+        self.point = point
 
 
 @dataclass(frozen=True)
@@ -33,7 +39,7 @@ class Step:
         if not (np.isfinite(self.time_period) and self.time_period > 0):
             raise PlasticityInputError(f"time period must be finite and > 0: {self.time_period}")
         if int(self.increments) <= 0:
-            raise PlasticityInputError("a Step needs at least one increment")
+            raise PlasticityInputError("a Step needs at least one increment", "no_increments")
 
     @property
     def time_step(self) -> float:
@@ -48,13 +54,14 @@ class Amplitude:
     def __init__(self, points):
         pts = [(float(t), float(v)) for t, v in points]
         if not pts:
-            raise PlasticityInputError("amplitude point 0 is invalid")
+            raise PlasticityInputError("amplitude point 0 is invalid", "invalid_amplitude", 0)
         eps = np.finfo(np.float64).eps
         for k, (t, v) in enumerate(pts):
             ok = np.isfinite(t) and np.isfinite(v)
             ok = ok and ((abs(t) <= eps and abs(v) <= eps) if k == 0 else t > pts[k - 1][0])
             if not ok:
-                raise PlasticityInputError(f"amplitude point {k} is invalid")
+                raise PlasticityInputError(f"amplitude point {k} is invalid",
+                                           "invalid_amplitude", k)
         self.points = pts
 
     @classmethod
