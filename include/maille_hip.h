@@ -39,8 +39,11 @@ typedef enum ml_status {
   ML_INVALID_FILE = 8,        /* an input file refused by ml_msh_read (IoError, error.rs:9-65) */
   ML_NEWTON_NOT_CONVERGED = 9,/* MailleError::NotConverged (error.rs:158-163): Newton-Raphson
                                  failed even on the smallest cut of an increment */
-  ML_PLASTIC_COLLAPSE = 10    /* MailleError::PlasticCollapse (error.rs:164-168): the tangent
+  ML_PLASTIC_COLLAPSE = 10,   /* MailleError::PlasticCollapse (error.rs:164-168): the tangent
                                  stopped being positive definite while points yield */
+  ML_DIRECT_REQUIRED = 11     /* ml_model_create with ML_AUTO below ML_ITERATIVE_FROM free DOFs:
+                                 SolverKind::Auto takes the direct skyline there (solve.rs:
+                                 185-194), the caller's own arm; no handle was created */
 } ml_status;
 
 /* Arithmetic of the solve. ML_FP64 is the drop-in (the reference computes in f64).
@@ -51,7 +54,9 @@ typedef enum ml_status {
  * ML_MIXED_IC0: FP64 SpMV and PCG with an FP32 IC(0) factor and FP32 triangular solves. */
 typedef enum ml_precision { ML_FP64 = 0, ML_MIXED = 1, ML_MIXED_IC0 = 2 } ml_precision;
 
-/* SolverKind (solve.rs:171-195). ML_AUTO is iterative from ML_ITERATIVE_FROM free DOFs. */
+/* SolverKind (solve.rs:171-195). The library is the iterative arm: ML_ITERATIVE always runs the
+ * device IC(0)-PCG, ML_AUTO runs it from ML_ITERATIVE_FROM free DOFs on and is answered with
+ * ML_DIRECT_REQUIRED below (where Auto is Direct), ML_DIRECT is refused. */
 typedef enum ml_solver_kind { ML_AUTO = 0, ML_DIRECT = 1, ML_ITERATIVE = 2 } ml_solver_kind;
 #define ML_ITERATIVE_FROM 100000 /* solve.rs:185 */
 #define ML_TOLERANCE 1e-12       /* iterative.rs:9 */
@@ -63,9 +68,11 @@ typedef struct ml_solve_info {
   double true_residual;      /* |b - A x| / |b| computed in FP64 after the solve */
   int32_t shift_index;       /* index into the shift ladder {0,1e-3,1e-2,0.1,1,10} used */
   int32_t refinements;       /* FP64 true-residual replacements (ML_MIXED), else 0 */
-  int64_t bad_element;       /* ML_INVALID_MESH: first offending element, else -1 */
+  int64_t bad_element;       /* ML_INVALID_MESH: the first offending element (the node for kinds
+                                4 and 5, as NonFiniteCoordinate and UnknownNode carry it), else -1 */
   int32_t mesh_error;        /* ML_INVALID_MESH: 1 degenerate, 2 inverted, 3 curved Tet10 edge,
-                                4 non-finite coordinate, 5 unknown node, 6 bad material */
+                                4 non-finite coordinate, 5 unknown node, 6 bad material,
+                                7 empty mesh (mesh.rs:258-300, material.rs:57-60,145-153) */
   int32_t escalated;         /* ML_MIXED: 1 when a residual replacement stopped halving the FP64
                                 true residual and the solve continued from its iterate with the
                                 FP64 operator (FP32 preconditioner kept), else 0 */
@@ -178,8 +185,11 @@ typedef struct ml_model ml_model;
  * young/nu per element (material.rs:16-34), fixed[3*n_nodes] (0/1) with imposed values
  * (model.rs:1133-1139). Validates the mesh (mesh.rs:257-348), builds free numbering, Graph,
  * RCM and the CSR pattern on the host, uploads and assembles K_ff on the device.
- * solver: ML_AUTO/ML_ITERATIVE run the device IC(0)-PCG; ML_DIRECT is refused
- * (ML_INVALID_ARGUMENT: the skyline Cholesky is not part of this library). */
+ * solver: ML_ITERATIVE runs the device IC(0)-PCG; ML_AUTO does so from ML_ITERATIVE_FROM free
+ * DOFs on and returns ML_DIRECT_REQUIRED below (solve.rs:185-194: Auto is Direct there; the
+ * check follows the host free numbering, before any device work); ML_DIRECT is refused
+ * (ML_INVALID_ARGUMENT: the skyline Cholesky is not part of this library). A refused mesh
+ * returns ML_INVALID_MESH with info->mesh_error and info->bad_element set. */
 ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elements, int nen,
                           const int64_t* conn, const double* young, const double* nu,
                           const uint8_t* fixed, const double* imposed, ml_solver_kind solver,
@@ -301,7 +311,9 @@ typedef struct ml_thermal ml_thermal;
  * (thermal_matrix, model.rs:1320-1326, assembly.rs:598-624); fixed[n_nodes] (0/1) with the BC
  * temperatures imposed[n_nodes]. Validates the mesh, builds the free numbering, RCM pattern and
  * assembly plan on the host and the conduction and consistent capacity matrices (TETRA_4 /
- * TETRA_14 for the capacity, assembly.rs:413-431) on the device. */
+ * TETRA_14 for the capacity, assembly.rs:413-431) on the device. A model whose every
+ * temperature is fixed has no free system and is refused (ML_INVALID_ARGUMENT), unlike the
+ * mechanical ml_model_create, which follows FreeSystem::new in accepting zero unknowns. */
 ml_status ml_thermal_create(int64_t n_nodes, const double* nodes, int64_t n_elements, int nen,
                             const int64_t* conn, const double* conductivity, const double* rho_c,
                             int64_t nconv, const int64_t* conv_elements, const int64_t* conv_faces,
@@ -344,7 +356,7 @@ void ml_thermal_destroy(ml_thermal* h);
 
 /* nrhs load cases of one model: f[k*ndof ..] nodal loads of case k, u likewise; infos[nrhs]
  * (optional) per case. The cases are solved together as ml_pcg_solve does with nrhs > 1;
- * reactions are not computed. */
+ * reactions are not computed. A model without free DOFs is refused (ML_INVALID_ARGUMENT). */
 ml_status ml_model_solve_many(ml_model* m, int64_t nrhs, const double* f, double* u,
                               ml_solve_info* infos);
 

@@ -15,8 +15,11 @@ the calls that compute raise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import itertools
 import os
+import weakref
 
 import numpy as np
 
@@ -26,7 +29,7 @@ _lib = None
 
 ML_OK, ML_SINGULAR, ML_NOT_CONVERGED, ML_INVALID_ARGUMENT, ML_OUT_OF_PATTERN, ML_HIP_ERROR, \
     ML_INVALID_MESH, ML_DEVICE_TIMEOUT, ML_INVALID_FILE, ML_NEWTON_NOT_CONVERGED, \
-    ML_PLASTIC_COLLAPSE = range(11)
+    ML_PLASTIC_COLLAPSE, ML_DIRECT_REQUIRED = range(12)
 FP64, MIXED, MIXED_IC0 = 0, 1, 2
 AUTO, DIRECT, ITERATIVE = 0, 1, 2
 TOLERANCE = 1e-12          # iterative.rs:9
@@ -101,6 +104,11 @@ class PlasticCollapse(MailleError):
         super().__init__(f"plastic collapse at increment {increment} (t = {time})")
         self.increment = increment
         self.time = time
+
+
+class DirectRequired(MailleError):
+    """SolverKind::Auto below ITERATIVE_FROM free DOFs (solve.rs:185-194) is the direct skyline
+    solver, which is not part of this library (the caller takes its own Direct arm)."""
 
 
 class DeviceError(MailleError):
@@ -228,6 +236,53 @@ def lib():
     return _lib
 
 
+# Live handles, freed explicitly: close() (or a with-block) frees one, and at interpreter exit
+# every handle still open is freed, newest first (a Batch before the Models it holds), before
+# the interpreter tears down and while the HIP runtime and any tool attached to it are alive.
+_live = weakref.WeakValueDictionary()
+_serial = itertools.count()
+
+
+class _Handle:
+    """A library handle with deterministic release."""
+
+    _destroy = ""  # name of the library function that frees the handle
+
+    def _register(self, handle):
+        self._h = handle
+        self._key = next(_serial)
+        _live[self._key] = self
+
+    def close(self):
+        """Frees the handle's device memory and stream now; later calls raise."""
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            getattr(_lib, self._destroy)(h)
+        self._h = None
+        _live.pop(getattr(self, "_key", None), None)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
+    def __del__(self):
+        self.close()
+
+
+def close_all():
+    """Frees every handle still open, newest first."""
+    for key in sorted(list(_live.keys()), reverse=True):
+        obj = _live.get(key)
+        if obj is not None:
+            obj.close()
+
+
+atexit.register(close_all)
+
+
 def _check(status, info=None):
     if status == ML_OK:
         return
@@ -241,6 +296,8 @@ def _check(status, info=None):
         inc = getattr(info, "failed_increment", -1)
         t = getattr(info, "failed_time", float("nan"))
         raise (NotConverged if status == ML_NEWTON_NOT_CONVERGED else PlasticCollapse)(inc, t)
+    if status == ML_DIRECT_REQUIRED:
+        raise DirectRequired(lib().ml_last_error().decode(errors="replace"))
     if status == ML_INVALID_MESH and info is not None:
         raise InvalidMesh(info.mesh_error, info.bad_element)
     msg = lib().ml_last_error().decode(errors="replace")
@@ -298,8 +355,10 @@ def solid_stiffness(coords: np.ndarray, young: float, nu: float) -> np.ndarray:
 
 
 # ------------------------------------------------------------------ graph, RCM, CSR pattern
-class Graph:
+class Graph(_Handle):
     """iterative.rs:19-77 — DOF adjacency, built with its RCM order and lower pattern."""
+
+    _destroy = "ml_pattern_free"
 
     def __init__(self, n, elements):
         lens = [len(e) for e in elements]
@@ -308,7 +367,7 @@ class Graph:
         items = np.array([v for e in elements for v in e] or [0], dtype=np.int64)
         h = C.c_void_p()
         _check(lib().ml_pattern_build(n, len(elements), _i(off), _i(items), C.byref(h)))
-        self._h = h
+        self._register(h)
         self.n = n
 
     @classmethod
@@ -340,10 +399,6 @@ class Graph:
         _check(lib().ml_pattern_get(self._h, None, _i(off), _i(cols)))
         return off, cols
 
-    def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
-            _lib.ml_pattern_free(self._h)
-            self._h = None
 
 
 def reverse_cuthill_mckee(graph: Graph) -> np.ndarray:
@@ -401,15 +456,17 @@ def layout_verify(offsets, cols, l=None, r=None):
     return (out, z) if z is not None else out
 
 
-class Pcg:
+class Pcg(_Handle):
     """IC(0)-preconditioned CG on the device (iterative.rs:194-248). ``precision`` FP64 is the
     drop-in; MIXED runs an FP32 operator inside FP64 refinement."""
+
+    _destroy = "ml_pcg_destroy"
 
     def __init__(self, a: Csr, precision: int = FP64):
         h = C.c_void_p()
         _check(lib().ml_pcg_create(a.n(), _i(a.offsets), _i(a.cols), _d(a.values), precision,
                                    C.byref(h)))
-        self._h = h
+        self._register(h)
         self.n = a.n()
         self.nnz = len(a.cols)
 
@@ -453,19 +510,19 @@ class Pcg:
         """Bound of the device's dataflow waits in wall-clock ticks (0: the default, 2 s)."""
         _check(lib().ml_pcg_set_wait_limit(self._h, int(ticks)))
 
-    def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
-            _lib.ml_pcg_destroy(self._h)
-            self._h = None
 
 
 # ------------------------------------------------------------------ the static model
-class Model:
+class Model(_Handle):
     """Model::new + Model::solve (model.rs:251-548) for a Tet4/Tet10 mesh on the device.
     ``problem`` is a ``maille_amd.meshes.Problem`` (nodes, conn, young, nu, fixed, imposed,
-    nodal loads f, dead pressures)."""
+    nodal loads f, dead pressures). ``solver`` ITERATIVE (the default) always runs the device
+    PCG, as ``with_solver(Iterative)``; AUTO raises DirectRequired below ITERATIVE_FROM free
+    DOFs, where the reference's Auto is the direct solver."""
 
-    def __init__(self, problem, precision: int = FP64, solver: int = AUTO):
+    _destroy = "ml_model_destroy"
+
+    def __init__(self, problem, precision: int = FP64, solver: int = ITERATIVE):
         self.problem = problem
         nodes = _f64(problem.nodes)
         conn = _ai64(problem.conn)
@@ -482,7 +539,7 @@ class Model:
         _check(lib().ml_model_create(len(nodes), _d(nodes), ne, nen, _i(conn), _d(young),
                                      _d(nu), fixed.ctypes.data_as(_U8), _d(imposed), solver,
                                      precision, C.byref(h), C.byref(si)), si)
-        self._h = h
+        self._register(h)
         self.create_info = si.as_dict()
 
     def loads(self):
@@ -597,22 +654,20 @@ class Model:
                  "assembly", "vector", "stresses", "nodal")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(names)}
 
-    def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
-            _lib.ml_model_destroy(self._h)
-            self._h = None
 
 
-class Batch:
+class Batch(_Handle):
     """Several models solved in one set of launches (block-diagonal operator, one PCG per
     model). The models must stay alive and unchanged while the batch is used."""
+
+    _destroy = "ml_batch_destroy"
 
     def __init__(self, models, precision: int = FP64):
         self.models = list(models)
         arr = (C.c_void_p * len(self.models))(*[m._h.value for m in self.models])
         h = C.c_void_p()
         _check(lib().ml_batch_create(len(self.models), arr, precision, C.byref(h)))
-        self._h = h
+        self._register(h)
 
     def solve(self, loads=None):
         """loads: one array of nodal loads per model (default: each model's loads());
@@ -624,8 +679,3 @@ class Batch:
         infos = (SolveInfo * len(us))()
         _check(lib().ml_batch_solve(self._h, fp, up, infos))
         return us, [i.as_dict() for i in infos]
-
-    def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
-            _lib.ml_batch_destroy(self._h)
-            self._h = None

@@ -56,7 +56,7 @@ int check_mesh(const MeshView& m, i64* where) {
   for (i64 e = 0; e < m.n_elements; ++e) {
     const i64* t = m.conn + e * m.nen;
     for (int a = 0; a < m.nen; ++a)
-      if (t[a] < 0 || t[a] >= m.n_nodes) { *where = e; return 5; }
+      if (t[a] < 0 || t[a] >= m.n_nodes) { *where = t[a]; return 5; }  // UnknownNode { node }
     // mesh.rs:329-348 (check_tetrahedron)
     const double* c[10];
     for (int a = 0; a < m.nen; ++a) c[a] = m.nodes + 3 * t[a];

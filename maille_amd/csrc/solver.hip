@@ -1002,7 +1002,7 @@ struct Model {
 
   int create(int64_t n_nodes, const double* xyz, int64_t ne, int nen_, const int64_t* cn,
              const double* E, const double* v, const uint8_t* fx, const double* imp,
-             int precision, ml_solve_info* info) {
+             int precision, int solver, ml_solve_info* info) {
     if (nen_ != 4 && nen_ != 10) throw ArgError("nen must be 4 (Tet4) or 10 (Tet10)");
     nodes.assign(xyz, xyz + 3 * n_nodes);
     conn.assign(cn, cn + ne * nen_);
@@ -1021,6 +1021,13 @@ struct Model {
     int st = build_plan(mesh, fixed.data(), imposed.data(), plan, &err);
     if (st) throw ArgError(err);
     if (info) info->seconds[5] = plan.seconds;
+    // SolverKind::Auto (solve.rs:185-194) is Direct below ITERATIVE_FROM free DOFs: the
+    // caller's skyline arm, not this library's
+    if (solver == ML_AUTO && plan.nfree < ML_ITERATIVE_FROM) {
+      g_last_error = "ML_AUTO with " + std::to_string(plan.nfree) + " free DOFs (< " +
+                     std::to_string(ML_ITERATIVE_FROM) + "): SolverKind::Auto is Direct here";
+      return ML_DIRECT_REQUIRED;
+    }
     op.precision = precision;
     // with every DOF imposed there is no free system: u is the imposed field and only the
     // reactions are computed (FreeSystem::new and ::solve with zero unknowns)
@@ -1098,6 +1105,11 @@ struct Model {
   }
 
   int solve_device(const double* f, double* u, double* R, ml_solve_info* info) {
+    if (info) {
+      std::memset(info, 0, sizeof(*info));
+      info->shift_index = -1;  // no factorisation (every DOF imposed) until one runs
+      info->bad_element = -1;
+    }
     double t0 = now_s();
     const int retries0 = op.retries;
     if (all_fixed()) {
@@ -2457,7 +2469,7 @@ ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elemen
     }
     std::unique_ptr<ml_model> h(new ml_model);
     int st = h->m.create(n_nodes, nodes, n_elements, nen, conn, young, nu, fixed, imposed,
-                         precision, info);
+                         precision, solver, info);
     if (st != ML_OK) return st;
     *out = h.release();
     return ML_OK;

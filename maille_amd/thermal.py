@@ -14,7 +14,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import FP64, SolveInfo, _ai64, _check, _d, _f64, _i, _U8, lib
+from . import FP64, SolveInfo, _ai64, _check, _d, _f64, _Handle, _i, _U8, lib
 from .plasticity import PlasticityInputError
 
 
@@ -54,7 +54,11 @@ class ThermalIncrement:
     heat_fluxes: np.ndarray
 
 
-class ThermalModel:
+class ThermalModel(_Handle):
+    """The thermal model on the device (one temperature per node); close() frees it."""
+
+    _destroy = "ml_thermal_destroy"
+
     def __init__(self, nodes, conn, conductivity, rho_c, fixed, imposed, loads=None,
                  heat_sources=(), convection=(), precision: int = FP64):
         nodes = _f64(nodes)
@@ -76,8 +80,7 @@ class ThermalModel:
                                        nc, _i(ce), _i(cf), _d(film), _d(sink),
                                        fixed.ctypes.data_as(_U8), _d(imposed), precision,
                                        C.byref(h), C.byref(si)), si)
-        self._h = h
-        self._lib = lib()
+        self._register(h)
 
     def _conv(self):
         conv = self.convection
@@ -130,8 +133,3 @@ class ThermalModel:
         _check(lib().ml_thermal_solve_steady(self._h, _d(f), _d(t), _d(r),
                                              _d(q) if fluxes else None, C.byref(si)), si)
         return ThermalIncrement(time=0.0, temperatures=t, reactions=r, heat_fluxes=q), si.as_dict()
-
-    def __del__(self):
-        if getattr(self, "_h", None) and getattr(self, "_lib", None) is not None:
-            self._lib.ml_thermal_destroy(self._h)
-            self._h = None
