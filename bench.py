@@ -40,6 +40,13 @@ def spmv_bytes(n, nnz_full, val):
     return nnz_full * (val + 4) + (n + 1) * 4 + 2 * n * val
 
 
+def sweep_bytes(n, nnz_strict):
+    """Algorithmic bytes of one Jacobi sweep (k_sweep): FP32 coefficients and int32 columns of
+    the strict triangle, row lengths, and the FP64 right-hand side, reciprocal diagonal,
+    previous iterate and result."""
+    return nnz_strict * (4 + 4) + n * (4 + 8 + 8 + 8 + 8)
+
+
 def trisolve_bytes(n, nnz_l, val):
     """Algorithmic bytes of one triangular solve with L (or L^T): entries + row pointers +
     right-hand side + solution."""
@@ -498,14 +505,19 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=100)
-    ap.add_argument("--precision", choices=["best", "mixed", "mixed_ic0", "fp64"], default="best",
-                    help="mode(s) to time; 'best' times all three and reports the fastest whose "
-                         "displacements stay within 1e-8 of the FP64 drop-in")
+    ap.add_argument("--precision", choices=["best", "mixed", "mixed_ic0", "mixed_jacobi", "fp64"],
+                    default="best",
+                    help="mode(s) to time; 'best' compares fp64, mixed_ic0 and mixed_jacobi and "
+                         "reports the fastest whose displacements stay within 1e-8 of the FP64 "
+                         "drop-in")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a plain 256 MiB read + 256 MiB write kernel, the reference "
                          "against which HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled")
-    ap.add_argument("--cpu-iters", type=int, default=8)
+    ap.add_argument("--cpu-iters", type=int, default=0,
+                    help="PCG iterations of the CPU sample (0: a tenth of the FP64 count)")
+    ap.add_argument("--compare-steps", type=int, default=2,
+                    help="steps timed per mode to pick the headline mode")
     ap.add_argument("--workload", choices=["cube", "beam_cases", "box_batch", "tet10", "simp",
                                            "stresses", "plastic", "transient", "large"],
                     default="cube", help="cube: the headline 3.1M-DOF solve; the others are the "
@@ -537,17 +549,25 @@ def main():
     problem = meshes.jittered_cube(args.cells)
     log(f"mesh: {len(problem.conn)} elements, {problem.nfree} free DOF ({time.time() - t0:.1f} s)")
 
-    codes = {"fp64": ma.FP64, "mixed": ma.MIXED, "mixed_ic0": ma.MIXED_IC0}
-    modes = ["fp64", "mixed", "mixed_ic0"] if args.precision == "best" else \
+    codes = {"fp64": ma.FP64, "mixed": ma.MIXED, "mixed_ic0": ma.MIXED_IC0,
+             "mixed_jacobi": ma.MIXED_JACOBI}
+    modes = ["fp64", "mixed_ic0", "mixed_jacobi"] if args.precision == "best" else \
         (["fp64"] if args.precision == "fp64" else ["fp64", args.precision])
-    models = {}
+    models, create = {}, {}
     for prec in modes:
-        t1 = time.time()
+        t1 = time.perf_counter()
         models[prec] = ma.Model(problem, precision=codes[prec])
+        t_create = time.perf_counter() - t1
         chosen = models[prec].autotune()
-        log(f"{prec}: model created and autotuned in {time.time() - t1:.1f} s (host pattern "
-            f"{models[prec].create_info['seconds'][5]:.1f} s; {chosen[0]} waves per CU of "
-            f"{chosen[2]})")
+        create[prec] = {"create_s": round(t_create, 3),
+                        "host_pattern_s": round(models[prec].create_info["seconds"][5], 3),
+                        "autotune_s": round(time.perf_counter() - t1 - t_create, 3),
+                        "waves_per_cu": chosen[0], "reported_occupancy": chosen[2],
+                        "sweeps": models[prec].sweeps()}
+        log(f"{prec}: model created in {t_create:.1f} s (host pattern "
+            f"{create[prec]['host_pattern_s']:.1f} s), autotuned in "
+            f"{create[prec]['autotune_s']:.1f} s: {chosen[0]} waves per CU of {chosen[2]}, "
+            f"{create[prec]['sweeps']} Jacobi sweeps")
     m0 = models["fp64"]
     f = m0.loads()
     dev = torch.device("cuda:0")
@@ -561,40 +581,55 @@ def main():
         m.factor()
         return m.solve_device(f_dev.data_ptr(), u_dev[prec].data_ptr(), r_dev.data_ptr())
 
-    results = {}
-    for prec in modes:
+    def timed(prec, warmup, steps):
         m = models[prec]
-        for _ in range(args.warmup):
-            step(prec)
+        retries = 0
+        for _ in range(warmup):
+            retries += step(prec)["device_retries"]
         torch.cuda.synchronize()
         m.profile(enable=1)
         torch.cuda.synchronize()
         t_start = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             info = step(prec)
+            retries += info["device_retries"]
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t_start
-        results[prec] = {"s": elapsed / args.steps, "elapsed": elapsed, "info": info,
-                         "prof": m.profile(enable=0)}
-        log(f"{prec}: {elapsed / args.steps:.4f} s per step, {info['iterations']} iterations, "
-            f"true residual {info['true_residual']:.2e}; kernels {results[prec]['prof']}")
+        return {"s": elapsed / steps, "elapsed": elapsed, "info": info,
+                "prof": m.profile(enable=0), "device_retries": retries, "steps": steps}
+
+    # every mode is compared on a few steps; the fastest whose displacements stay within 1e-8
+    # of the FP64 drop-in is then timed over exactly --steps steps
+    compare = {}
+    for prec in modes:
+        compare[prec] = timed(prec, 1, args.compare_steps)
+        log(f"{prec}: {compare[prec]['s']:.4f} s per step over {args.compare_steps}, "
+            f"{compare[prec]['info']['iterations']} iterations, true residual "
+            f"{compare[prec]['info']['true_residual']:.2e}, retries "
+            f"{compare[prec]['device_retries']}")
     u64 = u_dev["fp64"].cpu().numpy()
     for prec in modes:
         u = u_dev[prec].cpu().numpy()
-        results[prec]["rel_err_vs_fp64"] = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
-    valid = [p for p in modes if results[p]["rel_err_vs_fp64"] <= 1e-8]
-    head = min(valid, key=lambda p: results[p]["s"])
-    res = results[head]
+        compare[prec]["rel_err_vs_fp64"] = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
+    valid = [p for p in modes if compare[p]["rel_err_vs_fp64"] <= 1e-8]
+    head = min(valid, key=lambda p: compare[p]["s"])
+    res = timed(head, args.warmup, args.steps)
+    log(f"headline {head}: {res['s']:.4f} s per step over {args.steps} steps; kernels "
+        f"{res['prof']}")
+    u = u_dev[head].cpu().numpy()
+    res["rel_err_vs_fp64"] = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
     per_step, elapsed, info, prof = res["s"], res["elapsed"], res["info"], res["prof"]
 
     n, nnz_l = m0.nfree(), m0.nnz()
     nnz_full = 2 * nnz_l - n
+    sweeps = models[head].sweeps()
     aval = 4 if head == "mixed" else 8
     mval = 8 if head == "fp64" else 4
     kinds = {"spmv": "spmv32" if head == "mixed" else "spmv64",
              "precondition": "precond64" if head == "fp64" else "precond32"}
-    bytes_per = {"spmv": spmv_bytes(n, nnz_full, aval),
-                 "precondition": 2 * trisolve_bytes(n, nnz_l, mval)}
+    pre_bytes = (2 * sweeps * sweep_bytes(n, nnz_l - n) if sweeps
+                 else 2 * trisolve_bytes(n, nnz_l, mval))
+    bytes_per = {"spmv": spmv_bytes(n, nnz_full, aval), "precondition": pre_bytes}
     kernels = {}
     for name, kind in kinds.items():
         ms, cnt = prof[kind]
@@ -604,6 +639,9 @@ def main():
                          "bytes_per_launch": bytes_per[name],
                          "achieved_GBs": round(achieved, 1),
                          "share_of_step": round(ms / 1e3 / elapsed, 3)}
+    kernels["precondition"]["form"] = (f"{sweeps} Jacobi sweeps per triangular system "
+                                       f"(k_sweep, {2 * sweeps} launches)" if sweeps else
+                                       "exact dataflow triangular solves (k_trisolve, 2 launches)")
     ic_ms, ic_cnt = prof["ic0"]
     kernels["ic0_factor"] = {"ms_per_step": round(ic_ms / args.steps, 2),
                              "share_of_step": round(ic_ms / 1e3 / elapsed, 3)}
@@ -613,15 +651,48 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(dominant, f"jittered_cube_{args.cells}", head)}
-    p64 = results["fp64"]["prof"]
+    if roofline["traffic"] is not None:
+        roofline["traffic_source"] = ("replayed from the committed FETCH_SIZE/WRITE_SIZE passes "
+                                      f"profiles/traffic_{dominant}_jittered_cube_{args.cells}_"
+                                      f"{head}.json, not measured in this run")
+    p64 = compare["fp64"]["prof"]
     spmv64_ms, spmv64_cnt = p64["spmv64"]
     spmv64_avg = spmv64_ms / max(spmv64_cnt, 1)
     spmv64_gbs = spmv_bytes(n, nnz_full, 8) / (spmv64_avg * 1e-3) / 1e9 if spmv64_cnt else 0.0
 
+    # the whole Model::solve (model.rs:548-610) of the headline mode: model creation (mesh
+    # checks, free numbering, Graph, RCM, pattern on the host; upload, element stiffness and
+    # assembly on the device), IC(0), PCG, reactions, Gauss-point stresses
+    mh = models[head]
+    gp = torch.zeros(len(problem.conn) * 9, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    mh.assemble()  # element stiffness + assembly (synchronised)
+    t2 = time.perf_counter()
+    mh.factor()
+    t3 = time.perf_counter()
+    e2e = mh.solve_device(f_dev.data_ptr(), u_dev[head].data_ptr(), r_dev.data_ptr())
+    t4 = time.perf_counter()
+    mh.stresses_device(u_dev[head].data_ptr(), gp.data_ptr())
+    torch.cuda.synchronize()
+    t5 = time.perf_counter()
+    del gp
+    end_to_end = {
+        "create_s": create[head]["create_s"], "host_pattern_s": create[head]["host_pattern_s"],
+        "assemble_s": round(t2 - t1, 4), "factor_s": round(t3 - t2, 4),
+        "solve_s": round(e2e["seconds"][2], 4), "reactions_s": round(e2e["seconds"][3], 4),
+        "stresses_s": round(t5 - t4, 4),
+        "note": "create = mesh checks, free numbering, Graph, RCM and pattern on the host, "
+                "upload, stiffness and assembly on the device (includes assemble_s once)",
+    }
+    end_to_end["total_s"] = round(create[head]["create_s"] + (t5 - t2), 3)
+
     cpu = None
     if not args.no_cpu:
         t_cpu = time.time()
-        cpu = cpu_baseline(problem, results["fp64"]["info"]["iterations"], args.cpu_iters)
+        iters = compare["fp64"]["info"]["iterations"]
+        sample = args.cpu_iters or max(8, -(-iters // 10))
+        cpu = cpu_baseline(problem, iters, sample)
         log(f"cpu baseline (oracle, 1 core): {cpu['value']} s estimated; "
             f"{cpu['multicore']['cores']} threads: {cpu['multicore']['value']} s "
             f"({time.time() - t_cpu:.1f} s sample)")
@@ -630,27 +701,40 @@ def main():
         "metric": METRIC, "value": round(per_step, 4), "unit": "s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per_step * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp64": "fp64", "mixed": "fp32+fp64", "mixed_ic0": "fp64+fp32-ic0"}[head],
+        "dtype": {"fp64": "fp64", "mixed": "fp32+fp64", "mixed_ic0": "fp64+fp32-ic0",
+                  "mixed_jacobi": "fp64+fp32-ic0-jacobi"}[head],
         "data": "synthetic",
         "config": {"workload": f"jittered_cube_{args.cells}", "free_dof": n, "nnz_L": nnz_l,
-                   "elements": len(problem.conn), "precision": head,
+                   "elements": len(problem.conn), "precision": head, "sweeps": sweeps,
                    "step": "assemble + IC(0) + PCG to 1e-12 + reactions, inputs in HBM",
                    "dag_depth": m0.levels()},
         "iterations": int(info["iterations"]), "true_residual": info["true_residual"],
+        "rel_err_vs_fp64": res["rel_err_vs_fp64"],
+        "device_retries": int(res["device_retries"]),
         "roofline": roofline, "kernels": kernels,
         "spmv64": {"avg_ms": round(spmv64_avg, 4), "achieved_GBs": round(spmv64_gbs, 1),
                    "frac": round(spmv64_gbs / HBM_PEAK_GBS, 4)},
-        "modes": {p: {"s_per_step": round(results[p]["s"], 4),
-                      "iterations": int(results[p]["info"]["iterations"]),
-                      "residual_replacements": int(results[p]["info"]["refinements"]),
-                      "escalated_to_fp64_operator": bool(results[p]["info"]["escalated"]),
-                      "true_residual": results[p]["info"]["true_residual"],
-                      "rel_err_vs_fp64": results[p]["rel_err_vs_fp64"]} for p in modes},
+        "modes": {p: {"s_per_step": round(compare[p]["s"], 4),
+                      "steps": compare[p]["steps"],
+                      "iterations": int(compare[p]["info"]["iterations"]),
+                      "sweeps": create[p]["sweeps"],
+                      "residual_replacements": int(compare[p]["info"]["refinements"]),
+                      "escalated_to_fp64_operator": bool(compare[p]["info"]["escalated"]),
+                      "true_residual": compare[p]["info"]["true_residual"],
+                      "rel_err_vs_fp64": compare[p]["rel_err_vs_fp64"],
+                      "device_retries": int(compare[p]["device_retries"]),
+                      "setup": create[p]} for p in modes},
+        "end_to_end": end_to_end,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(cpu["value"] / per_step, 1) if cpu else None,
         "speedup_vs_multicore_cpu": round(cpu["multicore"]["value"] / per_step, 1) if cpu else None,
     }
     print(json.dumps(out), flush=True)
+    # deterministic teardown: every device handle is freed before the interpreter exits
+    for m in models.values():
+        m.close()
+    del f_dev, u_dev, r_dev
+    torch.cuda.synchronize()
     if args.copy_probe:
         copy_probe(torch)
 

@@ -47,12 +47,23 @@ typedef enum ml_status {
 } ml_status;
 
 /* Arithmetic of the solve. ML_FP64 is the drop-in (the reference computes in f64).
- * ML_MIXED: FP32 SpMV and FP32 IC(0)/triangular solves inside one PCG whose iterate x,
- * residual r, alpha/beta dot products and refinement stay FP64: the recursive residual is
- * replaced by the FP64 true residual b - A x each time it has dropped tenfold (reliable
- * updates), until a replacement stops reducing it.
- * ML_MIXED_IC0: FP64 SpMV and PCG with an FP32 IC(0) factor and FP32 triangular solves. */
-typedef enum ml_precision { ML_FP64 = 0, ML_MIXED = 1, ML_MIXED_IC0 = 2 } ml_precision;
+ * The mixed modes factor IC(0) in FP32 (the shift ladder in FP32 arithmetic) and store the
+ * factor in FP32; they apply it with FP64 vectors and sums (each coefficient widened exactly),
+ * which is the reference's FP64 triangular solve of the FP32-rounded factor:
+ * ML_MIXED: FP32 SpMV inside one PCG whose iterate x, residual r, alpha/beta dot products and
+ * refinement stay FP64: the recursive residual is replaced by the FP64 true residual b - A x
+ * each time it has dropped tenfold (reliable updates), until a replacement stops reducing it.
+ * ML_MIXED_IC0: FP64 SpMV and PCG, exact triangular solves of the FP32 factor.
+ * ML_MIXED_JACOBI: FP64 SpMV and PCG; the FP32 factor applied by k Jacobi sweeps per
+ * triangular system instead of exact solves (bandwidth-bound instead of latency-bound; no
+ * counterpart in the reference): y = D^-1 r, then k times y <- D^-1 (r - N y) with N the
+ * strict lower part of L, and the transposed iteration on L^T z = y, so that the
+ * preconditioner stays symmetric. k is 4 until ml_model_autotune / ml_pcg_autotune times
+ * whole solves and keeps the fastest k (0, the exact solves, included), or the caller sets it
+ * (ml_pcg_set_sweeps). */
+typedef enum ml_precision {
+  ML_FP64 = 0, ML_MIXED = 1, ML_MIXED_IC0 = 2, ML_MIXED_JACOBI = 3
+} ml_precision;
 
 /* SolverKind (solve.rs:171-195). The library is the iterative arm: ML_ITERATIVE always runs the
  * device IC(0)-PCG, ML_AUTO runs it from ML_ITERATIVE_FROM free DOFs on and is answered with
@@ -123,7 +134,8 @@ void ml_pattern_free(ml_pattern* p);
  * index array the device kernels of ml_pcg_create would read for the lower CSR pattern
  * (offsets[n+1], cols[]; ascending columns, diagonal last) and checks each bound they rely
  * on: index ranges, block capacities, each row's entries covered once and in order, and a
- * topological processing order of the triangular-solve blocks. ML_INVALID_ARGUMENT with the
+ * topological processing order of the triangular-solve blocks; likewise the two Jacobi-sweep
+ * layouts (ML_MIXED_JACOBI). ML_INVALID_ARGUMENT with the
  * failed check in ml_last_error otherwise. stats[8] (optional): forward and backward block
  * counts, their largest external and internal entry counts (forward, then backward), the
  * DAG depth and the longest row. With lvalues (the factor, lower-CSR order), r and z
@@ -175,6 +187,12 @@ ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* tr
  * wait gives up at once and every launch still ends normally: the handle repeats the work on
  * halved grids up to three times, then the call fails with ML_DEVICE_TIMEOUT. */
 ml_status ml_pcg_set_wait_limit(ml_pcg* h, int64_t ticks);
+/* No counterpart in the reference: the Jacobi sweeps per triangular system with which a mixed
+ * handle applies its FP32 factor (0..64; 0: the exact triangular solves). An FP64 handle
+ * takes 0 only (ML_INVALID_ARGUMENT otherwise). Batched solves (nrhs > 1, ml_batch) always
+ * use the exact solves. ml_pcg_sweeps returns the current count. */
+ml_status ml_pcg_set_sweeps(ml_pcg* h, int sweeps);
+int ml_pcg_sweeps(const ml_pcg* h);
 
 /* ------------------------------------------------------------------ the static model */
 
@@ -194,6 +212,9 @@ ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elemen
                           const int64_t* conn, const double* young, const double* nu,
                           const uint8_t* fixed, const double* imposed, ml_solver_kind solver,
                           ml_precision precision, ml_model** out, ml_solve_info* info);
+/* ml_pcg_set_sweeps / ml_pcg_sweeps for the model's operator. */
+ml_status ml_model_set_sweeps(ml_model* m, int sweeps);
+int ml_model_sweeps(const ml_model* m);
 /* Recompute the element stiffnesses and K_ff values on the device (after ml_model_set_young). */
 ml_status ml_model_assemble(ml_model* m);
 /* Replace the per-element Young's moduli (n_elements values, host) without touching the
@@ -212,6 +233,8 @@ ml_status ml_model_solve_device(ml_model* m, const double* f_device, double* u_d
 /* element_stresses for every element (model.rs:596-598, element.rs:270-302): out[n_elements
  * x ngp x 9] = x, y, z, sxx, syy, szz, tyz, txz, txy per Gauss point (1 for Tet4, 4 Tet10). */
 ml_status ml_model_stresses(ml_model* m, const double* u, double* out);
+/* Same with device buffers (u of 3*n_nodes doubles, out of n_elements x ngp x 9). */
+ml_status ml_model_stresses_device(ml_model* m, const double* u_device, double* out_device);
 /* nodal_stresses (results.rs:138-166, smooth :184-221): the Gauss-point stresses of u
  * extrapolated to each element's nodes and averaged over the elements sharing each node, every
  * sum in the reference's element and Gauss-point order; out[n_nodes x 9] = x, y, z of the node,

@@ -30,7 +30,7 @@ _lib = None
 ML_OK, ML_SINGULAR, ML_NOT_CONVERGED, ML_INVALID_ARGUMENT, ML_OUT_OF_PATTERN, ML_HIP_ERROR, \
     ML_INVALID_MESH, ML_DEVICE_TIMEOUT, ML_INVALID_FILE, ML_NEWTON_NOT_CONVERGED, \
     ML_PLASTIC_COLLAPSE, ML_DIRECT_REQUIRED = range(12)
-FP64, MIXED, MIXED_IC0 = 0, 1, 2
+FP64, MIXED, MIXED_IC0, MIXED_JACOBI = 0, 1, 2, 3
 AUTO, DIRECT, ITERATIVE = 0, 1, 2
 TOLERANCE = 1e-12          # iterative.rs:9
 MAX_ITERATIONS = 10_000    # iterative.rs:11
@@ -42,9 +42,11 @@ EXPORTS = (
     "ml_pattern_ntargets", "ml_pattern_graph", "ml_pattern_free", "ml_layout_verify",
     "ml_pcg_create", "ml_pcg_solve", "ml_pcg_solve_device", "ml_pcg_multiply",
     "ml_pcg_precondition", "ml_pcg_factor", "ml_pcg_destroy", "ml_pcg_trace",
-    "ml_pcg_set_wait_limit",
+    "ml_pcg_set_wait_limit", "ml_pcg_set_sweeps", "ml_pcg_sweeps", "ml_model_set_sweeps",
+    "ml_model_sweeps",
     "ml_model_create", "ml_model_assemble", "ml_model_set_young", "ml_model_factor",
-    "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_nodal_stresses",
+    "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_stresses_device",
+    "ml_model_nodal_stresses",
     "ml_model_add_pressures", "ml_model_add_edge_loads",
     "ml_model_nfree", "ml_model_nnz", "ml_model_levels", "ml_model_csr",
     "ml_model_compliance_gradient", "ml_model_destroy", "ml_model_profile",
@@ -184,6 +186,10 @@ def lib():
             "ml_pcg_destroy": (None, [_P]),
             "ml_pcg_trace": (_int, [_P, _D, _I, _I, _I, _i64]),
             "ml_pcg_set_wait_limit": (_int, [_P, _i64]),
+            "ml_pcg_set_sweeps": (_int, [_P, _int]),
+            "ml_pcg_sweeps": (_int, [_P]),
+            "ml_model_set_sweeps": (_int, [_P, _int]),
+            "ml_model_sweeps": (_int, [_P]),
             "ml_model_create": (_int, [_i64, _D, _i64, _int, _I, _D, _D, _U8, _D, _int, _int,
                                        _PP, _S]),
             "ml_model_assemble": (_int, [_P]),
@@ -192,6 +198,7 @@ def lib():
             "ml_model_solve": (_int, [_P, _D, _D, _D, _S]),
             "ml_model_solve_device": (_int, [_P, _P, _P, _P, _S]),
             "ml_model_stresses": (_int, [_P, _D, _D]),
+            "ml_model_stresses_device": (_int, [_P, _P, _P]),
             "ml_model_nodal_stresses": (_int, [_P, _D, _D]),
             "ml_model_add_pressures": (_int, [_P, _i64, _I, _I, _D, _D]),
             "ml_model_add_edge_loads": (_int, [_P, _i64, _I, _I, _D, _D]),
@@ -510,6 +517,14 @@ class Pcg(_Handle):
         """Bound of the device's dataflow waits in wall-clock ticks (0: the default, 2 s)."""
         _check(lib().ml_pcg_set_wait_limit(self._h, int(ticks)))
 
+    def set_sweeps(self, sweeps: int):
+        """Jacobi sweeps per triangular system with which a mixed handle applies its FP32
+        factor (0: the exact triangular solves)."""
+        _check(lib().ml_pcg_set_sweeps(self._h, int(sweeps)))
+
+    def sweeps(self) -> int:
+        return lib().ml_pcg_sweeps(self._h)
+
 
 
 # ------------------------------------------------------------------ the static model
@@ -589,10 +604,18 @@ class Model(_Handle):
 
     def autotune(self, nrhs: int = 1):
         """Times candidate schedules and keeps the fastest; returns (waves per CU, right-hand
-        sides per pass, reported occupancy per CU)."""
+        sides per pass, reported occupancy per CU). A MIXED_JACOBI model also keeps the number
+        of Jacobi sweeps whose whole solve was fastest (``sweeps()``)."""
         out = (C.c_int32 * 3)()
         _check(lib().ml_model_autotune(self._h, int(nrhs), out))
         return tuple(out)
+
+    def set_sweeps(self, sweeps: int):
+        """Jacobi sweeps per triangular system for a mixed model's FP32 factor (0: exact)."""
+        _check(lib().ml_model_set_sweeps(self._h, int(sweeps)))
+
+    def sweeps(self) -> int:
+        return lib().ml_model_sweeps(self._h)
 
     def solve_many(self, F):
         """Load cases F (rows: nodal loads over all DOFs, without dead pressures) solved
@@ -616,6 +639,10 @@ class Model(_Handle):
         out = np.zeros((self.ne, ngp, 9))
         _check(lib().ml_model_stresses(self._h, _d(u), _d(out)))
         return out
+
+    def stresses_device(self, u_ptr, out_ptr):
+        """Gauss-point stresses with device buffers (integer device addresses)."""
+        _check(lib().ml_model_stresses_device(self._h, u_ptr, out_ptr))
 
     def nodal_stresses(self, u):
         """results.rs:138-166 (nodal_stresses): per node x, y, z, then sxx, syy, szz, tyz, txz,

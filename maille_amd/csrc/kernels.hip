@@ -113,6 +113,18 @@ __global__ void k_fill_i32(int* p, int64_t n, int v) {
   if (i < n) p[i] = v;
 }
 
+// Values uniform in [-1, 1) from SplitMix64 of seed + index: a fixed pseudo-random vector.
+__global__ void k_fill_hash(double* p, int64_t n, unsigned long long seed) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    p[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
 template <typename S, typename D>
 __global__ void k_gather(int64_t n, const int* __restrict__ src, const S* __restrict__ from,
                          D* __restrict__ to) {
@@ -1270,6 +1282,67 @@ __global__ __launch_bounds__(256) void k_spmv_rep(int n, int nrep, const int* __
   y[g] = s;
 }
 
+// ------------------------------------------------------------------ Jacobi sweeps
+// One Jacobi sweep of a triangular system of the IC(0) factor (no counterpart in the
+// reference, which solves it exactly): out_i = (rhs_i - sum_j m_ij in_j) * rd_i over the
+// strict triangle m of row i (l_ij, j < i, for L y = r; l_ji, j > i, for L^T z = y), with
+// rd = RN64(1 / l_ii). The sum runs sequentially in ascending j from 0, with the factor's FP32
+// coefficients widened exactly and FP64 vectors and arithmetic (contraction off, so a float64
+// restatement reproduces it bit for bit). FIRST: the previous iterate is in = rhs * rd, formed
+// as it is read. Lane i of a 64-row slice owns row i; entries are column-major in the slice.
+// With `partial`, the block's share of dot.out (the PCG's r.z) is written, in a fixed tree.
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ soff,
+                                               const int* __restrict__ scol,
+                                               const int* __restrict__ slen,
+                                               const float* __restrict__ sval,
+                                               const double* __restrict__ rd,
+                                               const double* __restrict__ rhs,
+                                               const double* __restrict__ in,
+                                               double* __restrict__ out,
+                                               const double* __restrict__ dot,
+                                               double* __restrict__ partial,
+                                               const PcgScalars* __restrict__ S) {
+#pragma clang fp contract(off)
+  if (S && S->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double dv = 0.0;
+  if (i < n) {
+    const int base = soff[i >> 6] + (i & 63);
+    const int len = slen[i];
+    double s = 0.0;
+    int k = 0;
+    // eight entries at a time: columns and coefficients, then the eight gathers, so that a
+    // wave keeps many loads in flight; the sum stays in entry order
+    for (; k + 8 <= len; k += 8) {
+      int c[8];
+      float v[8];
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        c[u] = scol[base + ((k + u) << 6)];
+        v[u] = sval[base + ((k + u) << 6)];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = FIRST ? rhs[c[u]] * rd[c[u]] : in[c[u]];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s = s + (double)v[u] * x[u];
+    }
+    for (; k < len; ++k) {
+      const int at = base + (k << 6);
+      const int c = scol[at];
+      s = s + (double)sval[at] * (FIRST ? rhs[c] * rd[c] : in[c]);
+    }
+    const double y = (rhs[i] - s) * rd[i];
+    out[i] = y;
+    if (dot) dv = dot[i] * y;
+  }
+  if (partial) {
+    const double b = block_sum<256>(dv);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+  }
+}
+
 // ------------------------------------------------------------------ reductions
 template <typename A, typename B>
 __global__ __launch_bounds__(256) void k_dot(int64_t n, const A* __restrict__ a,
@@ -1416,7 +1489,7 @@ __global__ __launch_bounds__(256) void k_beta(const double* __restrict__ partial
 }
 
 // x += alpha p; r -= alpha Ap (iterative.rs:229-232); partial r.r; fills the two triangular
-// solve outputs with the sentinel for the coming preconditioner application.
+// solve outputs (when given) with the sentinel for the coming preconditioner application.
 template <typename TV, typename TM>
 __global__ __launch_bounds__(256) void k_update_xr(int n, const PcgScalars* __restrict__ S,
                                                    double* __restrict__ x, double* __restrict__ r,
@@ -1437,8 +1510,8 @@ __global__ __launch_bounds__(256) void k_update_xr(int n, const PcgScalars* __re
     if (r_low) r_low[i] = (TM)ri;
     v = ri * ri;
     typedef typename Bits<TM>::U U;
-    reinterpret_cast<U*>(fill_a)[i] = Bits<TM>::S;
-    reinterpret_cast<U*>(fill_b)[i] = Bits<TM>::S;
+    if (fill_a) reinterpret_cast<U*>(fill_a)[i] = Bits<TM>::S;
+    if (fill_b) reinterpret_cast<U*>(fill_b)[i] = Bits<TM>::S;
   }
   double s = block_sum<256>(v);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -1645,13 +1718,16 @@ __device__ inline void lds_store(U* p, U v) {
 
 // TRACE: for diagnostics, each block records its timeline in trace[10 b .. 10 b + 9] and the
 // publish time of each slot's row in trace[10 nblocks + slot] (first right-hand side only).
-template <typename T, bool REV, bool SUBSEQ, bool TRACE = false>
+// T is the arithmetic of the vectors and sums, TL that of the stored coefficients (FP32 for the
+// mixed modes: each coefficient is widened exactly to T as it is used, so the solve is the
+// reference's FP64 solve applied to the FP32-rounded factor).
+template <typename T, bool REV, bool SUBSEQ, bool TRACE = false, typename TL = T>
 __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
     int n, int nblocks, const int* __restrict__ border, const int* __restrict__ bstart,
     const int* __restrict__ brow, const int* __restrict__ xoff, const int* __restrict__ uoff,
     const int* __restrict__ ucol, const unsigned char* __restrict__ xu,
-    const T* __restrict__ xval, const int* __restrict__ ioff, const unsigned char* __restrict__ ik,
-    const T* __restrict__ ival, const T* __restrict__ diag, const double* __restrict__ rdiag,
+    const TL* __restrict__ xval, const int* __restrict__ ioff, const unsigned char* __restrict__ ik,
+    const TL* __restrict__ ival, const TL* __restrict__ diag, const double* __restrict__ rdiag,
     const T* __restrict__ rhs, T* out, const PcgScalars* __restrict__ S, int* abort_flag,
     long long limit, long long* __restrict__ trace, int nrep) {
 #pragma clang fp contract(off)
@@ -1659,11 +1735,11 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
   typedef typename Bits<T>::U U;
   __shared__ U suv[TRI_UCAP];    // values of the distinct external columns (sentinel: not yet)
   __shared__ int suc[TRI_UCAP];  // their physical indices
-  __shared__ T sxa[TRI_XCAP];    // external coefficients
+  __shared__ TL sxa[TRI_XCAP];   // external coefficients
   __shared__ unsigned char sxu[TRI_XCAP];  // their columns, as indices into suv
   // one more internal entry than the layout can hold: position 64 (no row of a block), read
   // ahead past the end of a lane's list
-  __shared__ T sia[TRI_ICAP + 1];
+  __shared__ TL sia[TRI_ICAP + 1];
   __shared__ unsigned char sik[TRI_ICAP + 1];
   __shared__ T own[64];     // the rows of the block solved so far, copied when the solver waits
   __shared__ int s_state;   // 0: rows running; 1: rows done; 2: a wait gave up
@@ -1720,7 +1796,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
       }
       if (lane == 0) {
         sik[TRI_ICAP] = 64;
-        sia[TRI_ICAP] = T(0);
+        sia[TRI_ICAP] = TL(0);
         s_state = 0;
       }
       if (lane < rows) {
@@ -1733,7 +1809,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
         i1 = ioff[l + 1] - ib;
         rl = rhs[pl];
         rd = rdiag[l];
-        if constexpr (!std::is_same<T, float>::value) dl = diag[l];  // FP64: for the corrections
+        if constexpr (!std::is_same<T, float>::value) dl = (T)diag[l];  // FP64: the corrections
         acc = SUBSEQ ? rl : T(0);
       }
     }
@@ -1804,7 +1880,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             vv[u] = ju[u] >= 0 ? lds_load(&suv[ju[u]]) : Bits<T>::S;
-            aa[u] = ju[u] >= 0 ? sxa[p + u] : T(0);
+            aa[u] = ju[u] >= 0 ? (T)sxa[p + u] : T(0);
           }
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
@@ -1831,13 +1907,15 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
       // A lane whose external part is still pending holds nk = 64, which matches no row, until
       // it catches up.
       int nk = 64, nk2 = 64;
-      T nv = T(0), nv2 = T(0);
+      // kept in the stored precision and widened where used, so that the read-ahead never
+      // waits on its LDS load inside the row loop
+      TL nv = TL(0), nv2 = TL(0);
       T yown = T(0);  // the lane's solved value, once its row is done
       auto refill = [&]() {
         nk = ip < i1 ? (int)sik[ip] : 64;
-        nv = ip < i1 ? sia[ip] : T(0);
+        nv = ip < i1 ? sia[ip] : TL(0);
         nk2 = ip + 1 < i1 ? (int)sik[ip + 1] : 64;
-        nv2 = ip + 1 < i1 ? sia[ip + 1] : T(0);
+        nv2 = ip + 1 < i1 ? sia[ip + 1] : TL(0);
       };
       if (p == x1) refill();
       long long waited = 0;
@@ -1866,7 +1944,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                   vv[u] = kq[u] < k ? own[kq[u]] : T(0);
-                  aa[u] = kq[u] < k ? sia[ip + u] : T(0);
+                  aa[u] = kq[u] < k ? (T)sia[ip + u] : T(0);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -1939,7 +2017,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
           }
           y = lane_broadcast(y, k);
           const bool hit = nk == k;
-          const T upd = accumulate<T, SUBSEQ>(acc, nv, y);
+          const T upd = accumulate<T, SUBSEQ>(acc, (T)nv, y);
           acc = hit ? upd : acc;
           ip += hit ? 1 : 0;
           nk = hit ? nk2 : nk;

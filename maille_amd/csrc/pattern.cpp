@@ -667,6 +667,103 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, const 
   return std::string();
 }
 
+namespace {
+
+// Sliced ELL of rows whose entries are listed by `row_entries(i, put)` (put(col, src) in the
+// row's order).
+template <class F>
+std::string sweep_sell(i64 n, const std::vector<i64>& len, F row_entries, SweepLayout* out) {
+  SweepLayout& s = *out;
+  s.slices = (n + 63) / 64;
+  s.soff.assign(s.slices + 1, 0);
+  std::vector<i64> off(s.slices + 1, 0);
+  for (i64 sl = 0; sl < s.slices; ++sl) {
+    i64 w = 1;  // at least one column per slice, so that every slice has a valid base
+    for (i64 j = sl * 64; j < std::min(n, (sl + 1) * 64); ++j) w = std::max(w, len[j]);
+    off[sl + 1] = off[sl] + w * 64;
+  }
+  if (off[s.slices] >= (i64)INT32_MAX) return "strict triangle too large for 32-bit offsets";
+  s.soff.assign(off.begin(), off.end());
+  s.scol.assign(off[s.slices], 0);
+  s.ssrc.assign(off[s.slices], -1);
+  s.slen.resize(n);
+  parallel_ranges(n, [&](i64 b, i64 e) {
+    for (i64 i = b; i < e; ++i) {
+      const i64 base = off[i >> 6] + (i & 63), w = (off[(i >> 6) + 1] - off[i >> 6]) / 64;
+      i64 k = 0;
+      row_entries(i, [&](i64 col, i64 src) {
+        s.scol[base + (k << 6)] = (int32_t)col;
+        s.ssrc[base + (k << 6)] = (int32_t)src;
+        ++k;
+      });
+      s.slen[i] = (int32_t)k;
+      for (; k < w; ++k) s.scol[base + (k << 6)] = (int32_t)i;
+    }
+  });
+  return std::string();
+}
+
+}  // namespace
+
+std::string sweep_layouts(const LowerPattern& lp, SweepLayout* lower, SweepLayout* upper) {
+  const i64 n = lp.n();
+  if (lp.nnz() >= (i64)INT32_MAX) return "factor too large for 32-bit sources";
+  std::vector<i64> len(n);
+  for (i64 i = 0; i < n; ++i) len[i] = lp.offsets[i + 1] - lp.offsets[i] - 1;
+  std::string err = sweep_sell(n, len, [&](i64 i, auto put) {
+    for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) put(lp.cols[p], p);
+  }, lower);
+  if (!err.empty()) return err;
+  TransposeLayout t = transpose_layout(lp);
+  for (i64 j = 0; j < n; ++j) len[j] = t.offsets[j + 1] - t.offsets[j];
+  err = sweep_sell(n, len, [&](i64 j, auto put) {
+    for (i64 q = t.offsets[j]; q < t.offsets[j + 1]; ++q) put(t.rows[q], t.src[q]);
+  }, upper);
+  if (!err.empty()) return err;
+  for (SweepLayout* s : {lower, upper}) {
+    s->dsrc.resize(n);
+    for (i64 i = 0; i < n; ++i) s->dsrc[i] = (int32_t)(lp.offsets[i + 1] - 1);
+  }
+  return std::string();
+}
+
+std::string check_sweep_layouts(const LowerPattern& lp, const SweepLayout& lower,
+                                const SweepLayout& upper) {
+  const i64 n = lp.n(), nnz = lp.nnz();
+  // upper rows restated: column j of L, ascending row
+  std::vector<std::vector<std::pair<i64, i64>>> cols(n);
+  for (i64 i = 0; i < n; ++i)
+    for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) cols[lp.cols[p]].push_back({i, p});
+  for (int u = 0; u < 2; ++u) {
+    const SweepLayout& s = u ? upper : lower;
+    if (s.slices != (n + 63) / 64 || (i64)s.soff.size() != s.slices + 1 || s.soff[0] != 0 ||
+        (size_t)s.soff[s.slices] != s.scol.size() || s.scol.size() != s.ssrc.size() ||
+        (i64)s.slen.size() != n || (i64)s.dsrc.size() != n)
+      return "sweep layout arrays of the wrong size";
+    for (i64 sl = 0; sl < s.slices; ++sl) {
+      const i64 w = (i64)s.soff[sl + 1] - s.soff[sl];
+      if (w < 64 || w % 64) return "sweep layout slice of a bad width";
+    }
+    for (size_t q = 0; q < s.scol.size(); ++q)
+      if (s.scol[q] < 0 || s.scol[q] >= n || s.ssrc[q] < -1 || s.ssrc[q] >= nnz)
+        return "sweep layout entry out of range";
+    for (i64 i = 0; i < n; ++i) {
+      const i64 sl = i >> 6, base = s.soff[sl] + (i & 63), w = ((i64)s.soff[sl + 1] - s.soff[sl]) / 64;
+      if (s.slen[i] < 0 || s.slen[i] > w) return "sweep layout row longer than its slice";
+      if (s.dsrc[i] != lp.offsets[i + 1] - 1) return "sweep layout: wrong diagonal";
+      const i64 want = u ? (i64)cols[i].size() : lp.offsets[i + 1] - lp.offsets[i] - 1;
+      if (s.slen[i] != want) return "sweep layout row of the wrong length";
+      for (i64 k = 0; k < s.slen[i]; ++k) {
+        const i64 at = base + (k << 6);
+        const i64 c = u ? cols[i][k].first : lp.cols[lp.offsets[i] + k];
+        const i64 src = u ? cols[i][k].second : lp.offsets[i] + k;
+        if (s.scol[at] != c || s.ssrc[at] != src) return "sweep layout entry out of order";
+      }
+    }
+  }
+  return std::string();
+}
+
 void layout_precondition(const DeviceLayouts& d, const double* l, const double* r, double* z) {
   const i64 n = d.n;
   std::vector<double> y(n);

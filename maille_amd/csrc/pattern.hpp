@@ -132,6 +132,23 @@ std::string device_layouts(const LowerPattern& lp, const TriCaps& caps, DeviceLa
 // of each row's entries, topological block order); empty string when all hold.
 std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, const TriCaps& caps);
 
+// Sliced ELL (64 rows per slice, entries column-major within a slice) of one strict triangle
+// of the factor, for its Jacobi sweeps: row i holds the strictly lower entries l_ij, j < i, in
+// ascending j (lower), or the strictly upper entries of L^T, l_ji for j > i in ascending j
+// (upper). src is each entry's index in the lower-CSR value array; a padded position repeats
+// the row's own column with src -1 and is never read (len bounds each row).
+struct SweepLayout {
+  int64_t slices = 0;
+  std::vector<int32_t> soff;  // slices + 1, in entries
+  std::vector<int32_t> scol, ssrc, slen, dsrc;  // dsrc[i]: the diagonal l_ii in lower-CSR order
+};
+// Both triangles' layouts; an empty string or the reason they do not fit 32-bit offsets.
+std::string sweep_layouts(const LowerPattern& lp, SweepLayout* lower, SweepLayout* upper);
+// Verifies every index the sweep kernel reads and that each row lists exactly its strict
+// triangle, in ascending column; empty string when all hold.
+std::string check_sweep_layouts(const LowerPattern& lp, const SweepLayout& lower,
+                                const SweepLayout& upper);
+
 // The preconditioner z = (L L^T)^-1 r computed on the host through the triangular-solve
 // layouts, block by block in processing order, each row summed in the order the device sums
 // it. l holds the factor's values in lower-CSR order.

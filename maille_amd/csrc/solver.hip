@@ -252,9 +252,28 @@ struct Operator {
   int64_t nf = 0;  // strictly lower entries
   // workspace
   DBuf<double> w_x, w_r, w_p, w_ap, w_y, w_z, w_b, w_d, w_ax, w_partial;
-  DBuf<float> w_r32, w_p32, w_ap32, w_y32, w_z32;
+  DBuf<float> w_p32, w_ap32;  // p and Ap of the FP32 operator (ML_MIXED)
   DBuf<mlk::PcgScalars> d_S, d_Sin;
   DBuf<int> d_flags;  // [0] abort, [1] IC(0) failure, [2..5] where an IC(0) wait gave up
+  // Jacobi sweeps of the FP32 factor (ML_MIXED_JACOBI, or a mixed handle given sweeps > 0):
+  // sliced ELL of the strict lower triangle and of the strict upper one (rows of L^T), their
+  // FP32 coefficients gathered after each factorisation, and RN64(1 / l_ii)
+  struct SweepDev {
+    DBuf<int> soff, scol, slen, ssrc;
+    DBuf<float> sval;
+    void set(const SweepLayout& l, hipStream_t st) {
+      soff.upload_vec(l.soff, st);
+      scol.upload_vec(l.scol, st);
+      slen.upload_vec(l.slen, st);
+      ssrc.upload_vec(l.ssrc, st);
+      sval.alloc(l.scol.size());
+    }
+  } sw_lo, sw_up;
+  DBuf<int> d_sw_dsrc;
+  DBuf<float> d_sw_diag;
+  DBuf<double> d_sw_rd, w_s1, w_s2;
+  bool have_sweep = false;
+  int sweeps = 0;  // Jacobi sweeps per triangular system; 0: the exact dataflow solves
   // grids of the dataflow kernels: the full ones, and the ones in use (full >> shrinks)
   int full_tri64 = 0, full_tri32 = 0, full_ic = 0, full_ic64 = 0;
   int cap_tri64 = 0, cap_tri32 = 0;  // dataflow grids before the cap by block count
@@ -309,7 +328,8 @@ struct Operator {
     for (auto* b : {&w_x, &w_r, &w_p, &w_ap, &w_y, &w_z, &w_b, &w_d, &w_ax}) b->alloc(n);
     const int nblk = std::max(fw.nblk, bw.nblk);
     cap_tri64 = dataflow_grid((const void*)mlk::k_trisolve<double, false, false>, mlk::TRI_THREADS);
-    cap_tri32 = dataflow_grid((const void*)mlk::k_trisolve<float, false, false>, mlk::TRI_THREADS);
+    cap_tri32 = dataflow_grid((const void*)mlk::k_trisolve<double, false, false, false, float>,
+                              mlk::TRI_THREADS);
     full_tri64 = std::min<int>(nblk, cap_tri64);
     full_tri32 = std::min<int>(nblk, cap_tri32);
     full_ic = std::min<int>(n, std::min(dataflow_grid((const void*)mlk::k_ic0<double, 512>, 64),
@@ -326,9 +346,51 @@ struct Operator {
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p + 2, 4, -1);
     HIP_CHECK(hipGetLastError());
     if (precision != ML_FP64) alloc_mixed();
+    have_sweep = false;
+    sweeps = 0;
+    if (precision == ML_MIXED_JACOBI) set_sweeps(4);
     factored = false;
     // the host layouts above are temporaries: wait for their uploads before they go away
     HIP_CHECK(hipStreamSynchronize(stream));
+  }
+
+  // Jacobi sweeps per triangular system for the FP32 factor (0: exact dataflow solves); the
+  // sweep layouts are built and checked on the host the first time sweeps are asked for.
+  void set_sweeps(int k) {
+    if (k < 0 || k > 64) throw ArgError("sweeps must be in 0..64");
+    if (k > 0 && precision == ML_FP64)
+      throw ArgError("the FP64 drop-in applies its factor exactly (sweeps need a mixed handle)");
+    if (k > 0 && !have_sweep) {
+      LowerPattern lp;
+      lp.offsets = h_off;
+      lp.cols = h_col;
+      SweepLayout lo, up;
+      std::string err = sweep_layouts(lp, &lo, &up);
+      if (err.empty()) err = check_sweep_layouts(lp, lo, up);
+      if (!err.empty()) throw ArgError(err);
+      sw_lo.set(lo, stream);
+      sw_up.set(up, stream);
+      d_sw_dsrc.upload_vec(lo.dsrc, stream);
+      d_sw_diag.alloc(n);
+      d_sw_rd.alloc(n);
+      w_s1.alloc(n);
+      w_s2.alloc(n);
+      HIP_CHECK(hipStreamSynchronize(stream));  // the host layouts go out of scope
+      have_sweep = true;
+      if (factored) gather_sweeps();
+    }
+    sweeps = k;
+  }
+
+  // the FP32 factor's values into the sweep layouts, and the reciprocals of its diagonal
+  void gather_sweeps() {
+    for (SweepDev* t : {&sw_lo, &sw_up})
+      mlk::k_gather<float, float><<<grid_for(t->scol.n), 256, 0, stream>>>(
+          (int64_t)t->scol.n, t->ssrc.p, d_l32.p, t->sval.p);
+    mlk::k_gather<float, float><<<grid_for(n), 256, 0, stream>>>((int64_t)n, d_sw_dsrc.p, d_l32.p,
+                                                                 d_sw_diag.p);
+    mlk::k_recip64<float><<<grid_for(n), 256, 0, stream>>>((int64_t)n, d_sw_diag.p, d_sw_rd.p);
+    HIP_CHECK(hipGetLastError());
   }
 
   void alloc_mixed() {
@@ -337,7 +399,8 @@ struct Operator {
     d_sval32.alloc(d_scol.n);
     fw.alloc32();
     bw.alloc32();
-    for (auto* b : {&w_r32, &w_p32, &w_ap32, &w_y32, &w_z32}) b->alloc(n);
+    if (precision == ML_MIXED)
+      for (auto* b : {&w_p32, &w_ap32}) b->alloc(n);
   }
 
   // After d_a holds the lower values: refresh the SpMV copies.
@@ -441,6 +504,7 @@ struct Operator {
       }
       if (st != ML_OK) return ML_SINGULAR;
       gather_factor<float>(d_l32.p);
+      if (have_sweep) gather_sweeps();
     }
     factored = true;
     return ML_OK;
@@ -470,40 +534,96 @@ struct Operator {
   }
 
   // ---------------------------------------------------------------- kernels
-  // z = (L L^T)^-1 r through y
-  // With nrep > 1, r, y and z hold nrep vectors one after the other, solved in one launch.
-  template <typename T>
-  void precondition(const T* r, T* y, T* z, const mlk::PcgScalars* S,
+  // z = (L L^T)^-1 r through y by the two dataflow triangular solves, with FP64 vectors and
+  // sums: the FP64 factor (drop-in), or the mixed handles' FP32 factor, each coefficient
+  // widened as it is used. With nrep > 1, r, y and z hold nrep vectors one after the other,
+  // solved in one launch.
+  void precondition(const double* r, double* y, double* z, const mlk::PcgScalars* S,
                     long long* trace_fw = nullptr, long long* trace_bw = nullptr, int nrep = 1) {
-    const bool f32 = std::is_same<T, float>::value;
-    int& grid1 = f32 ? tri_grid32 : tri_grid64;
+    const bool c32 = precision != ML_FP64;
+    int& grid1 = c32 ? tri_grid32 : tri_grid64;
     int grid_rep = std::min<int64_t>((int64_t)std::max(fw.nblk, bw.nblk) * nrep,
-                                     std::max(1, (f32 ? cap_tri32 : cap_tri64) >> shrinks));
+                                     std::max(1, (c32 ? cap_tri32 : cap_tri64) >> shrinks));
     int& grid = nrep > 1 ? grid_rep : grid1;
     hipEvent_t ev = prof.begin(stream);
     int nn = n;
     long long limit = wait_ticks;
-    auto run = [&](TriDev& t, const T* rhs, T* out, bool fwd, long long* trace) {
-      const T* xv = f32 ? (const T*)t.xval32.p : (const T*)t.xval.p;
-      const T* iv = f32 ? (const T*)t.ival32.p : (const T*)t.ival.p;
-      const T* dg = f32 ? (const T*)t.diag32.p : (const T*)t.diag.p;
+    auto run = [&](TriDev& t, const double* rhs, double* out, bool fwd, long long* trace) {
       int nb = t.nblk;
-      auto go = [&](auto kernel) {
+      auto go = [&](auto kernel, auto* xv, auto* iv, auto* dg, const double* rd) {
         launch_dataflow(kernel, grid, mlk::TRI_THREADS, stream, nn, nb, (const int*)t.border.p,
                         (const int*)t.bstart.p, (const int*)t.brow.p, (const int*)t.xoff.p,
                         (const int*)t.uoff.p, (const int*)t.ucol.p, (const unsigned char*)t.xu.p,
-                        xv, (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg,
-                        f32 ? (const double*)t.rdiag32.p : (const double*)t.rdiag64.p, rhs, out,
-                        S, d_flags.p, limit, trace, nrep);
+                        xv, (const int*)t.ioff.p, (const unsigned char*)t.ik.p, iv, dg, rd, rhs,
+                        out, S, d_flags.p, limit, trace, nrep);
       };
-      if (fwd)
-        trace ? go(mlk::k_trisolve<T, false, false, true>) : go(mlk::k_trisolve<T, false, false>);
-      else
-        trace ? go(mlk::k_trisolve<T, true, true, true>) : go(mlk::k_trisolve<T, true, true>);
+      if (c32) {
+        const float* xv = t.xval32.p;
+        const float* iv = t.ival32.p;
+        const float* dg = t.diag32.p;
+        const double* rd = t.rdiag32.p;
+        if (fwd)
+          trace ? go(mlk::k_trisolve<double, false, false, true, float>, xv, iv, dg, rd)
+                : go(mlk::k_trisolve<double, false, false, false, float>, xv, iv, dg, rd);
+        else
+          trace ? go(mlk::k_trisolve<double, true, true, true, float>, xv, iv, dg, rd)
+                : go(mlk::k_trisolve<double, true, true, false, float>, xv, iv, dg, rd);
+      } else {
+        const double* xv = t.xval.p;
+        const double* iv = t.ival.p;
+        const double* dg = t.diag.p;
+        const double* rd = t.rdiag64.p;
+        if (fwd)
+          trace ? go(mlk::k_trisolve<double, false, false, true>, xv, iv, dg, rd)
+                : go(mlk::k_trisolve<double, false, false>, xv, iv, dg, rd);
+        else
+          trace ? go(mlk::k_trisolve<double, true, true, true>, xv, iv, dg, rd)
+                : go(mlk::k_trisolve<double, true, true>, xv, iv, dg, rd);
+      }
     };
     run(fw, r, y, true, trace_fw);
-    run(bw, (const T*)y, z, false, trace_bw);
-    prof.end(f32 ? P_TRI32 : P_TRI64, ev, stream);
+    run(bw, (const double*)y, z, false, trace_bw);
+    prof.end(c32 ? P_TRI32 : P_TRI64, ev, stream);
+  }
+
+  // z ~ (L L^T)^-1 r by `sweeps` Jacobi sweeps of L y = r from y = D^-1 r, then as many of
+  // L^T z = y from z = D^-1 y (the transposed iteration, so the preconditioner stays
+  // symmetric), on the FP32 factor with FP64 vectors. With `dot`, the last sweep also leaves
+  // per-block partial sums of dot.z in `partial` (the PCG's r.z).
+  void apply_sweeps(const double* r, double* z, const mlk::PcgScalars* S,
+                    const double* dot = nullptr, double* partial = nullptr) {
+    hipEvent_t ev = prof.begin(stream);
+    const unsigned g = grid_for(n);
+    const int k = sweeps;
+    double* a = w_s1.p;
+    double* b = w_s2.p;
+    auto sweep = [&](const SweepDev& t, bool first, const double* rhs, const double* in,
+                     double* out, bool last) {
+      const double* dv = last ? dot : nullptr;
+      double* pv = last ? partial : nullptr;
+      if (first)
+        mlk::k_sweep<true><<<g, 256, 0, stream>>>(n, t.soff.p, t.scol.p, t.slen.p, t.sval.p,
+                                                  d_sw_rd.p, rhs, nullptr, out, dv, pv, S);
+      else
+        mlk::k_sweep<false><<<g, 256, 0, stream>>>(n, t.soff.p, t.scol.p, t.slen.p, t.sval.p,
+                                                   d_sw_rd.p, rhs, in, out, dv, pv, S);
+    };
+    // forward, ping-pong between a and b; y ends in a
+    sweep(sw_lo, true, r, nullptr, a, false);
+    for (int m = 2; m <= k; ++m) {
+      sweep(sw_lo, false, r, a, b, false);
+      std::swap(a, b);
+    }
+    // backward from y = a; the sweeps alternate between b and z so that the last lands in z
+    const double* y = a;
+    const double* prev = nullptr;
+    for (int m = 1; m <= k; ++m) {
+      double* out = ((k - m) & 1) ? b : z;
+      sweep(sw_up, m == 1, y, prev, out, m == k);
+      prev = out;
+    }
+    HIP_CHECK(hipGetLastError());
+    prof.end(P_TRI32, ev, stream);
   }
 
   template <typename T>
@@ -512,20 +632,17 @@ struct Operator {
     HIP_CHECK(hipGetLastError());
   }
 
-  // The handle's preconditioner, w_z = (L L^T)^-1 w_r: the FP64 factor, or for the mixed
-  // precisions the FP32 one that their iterations apply (w_r narrowed, the result widened).
+  // The handle's preconditioner, w_z = M^-1 w_r, as its PCG applies it: the exact solves with
+  // the FP64 factor or the mixed handles' FP32 one, or the handle's Jacobi sweeps. A trace is
+  // taken of the exact solves.
   void apply_precondition(long long* trace_fw = nullptr, long long* trace_bw = nullptr) {
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 2, 0);
-    if (precision == ML_FP64) {
+    if (sweeps > 0 && !trace_fw && !trace_bw) {
+      apply_sweeps(w_r.p, w_z.p, nullptr);
+    } else {
       fill_sentinel<double>(w_y.p);
       fill_sentinel<double>(w_z.p);
-      precondition<double>(w_r.p, w_y.p, w_z.p, nullptr, trace_fw, trace_bw);
-    } else {
-      mlk::k_convert<double, float><<<grid_for(n), 256, 0, stream>>>(n, w_r.p, w_r32.p);
-      fill_sentinel<float>(w_y32.p);
-      fill_sentinel<float>(w_z32.p);
-      precondition<float>(w_r32.p, w_y32.p, w_z32.p, nullptr, trace_fw, trace_bw);
-      mlk::k_convert<float, double><<<grid_for(n), 256, 0, stream>>>(n, w_z32.p, w_z.p);
+      precondition(w_r.p, w_y.p, w_z.p, nullptr, trace_fw, trace_bw);
     }
     HIP_CHECK(hipGetLastError());
   }
@@ -563,8 +680,9 @@ struct Operator {
   static constexpr int PCG_STALLED = 100;  // internal: the FP32 operator stopped helping
 
   // Preconditioned CG (iterative.rs:210-248) on the device. TA: arithmetic of the operator and
-  // of p and Ap; TM: arithmetic of the IC(0) factor and z. x, r and every dot product are
-  // FP64. With an FP32 operator the recursive residual is replaced by the FP64 true residual
+  // of p and Ap; TM: arithmetic of z (FP64: the mixed handles store their factor in FP32 but
+  // apply it with FP64 vectors and sums, since FP32 intermediate vectors of the triangular
+  // solves cost PCG iterations). x, r and every dot product are FP64. With an FP32 operator the recursive residual is replaced by the FP64 true residual
   // b - A x each time it has dropped tenfold (reliable updates); when a replacement fails to
   // halve the true residual the solve returns PCG_STALLED with its iterate, for the caller to
   // continue with the FP64 operator. Convergence is the reference's |r| <= tol |b| on the
@@ -574,15 +692,16 @@ struct Operator {
   int pcg(const double* b, double* x, double* r, double tol, int64_t maxit, bool warm,
           mlk::PcgScalars* S, mlk::PcgScalars* hS) {
     const bool a32 = std::is_same<TA, float>::value;
-    const bool m32 = std::is_same<TM, float>::value;
+    static_assert(std::is_same<TM, double>::value, "the preconditioner runs on FP64 vectors");
     TA* p = a32 ? (TA*)w_p32.p : (TA*)w_p.p;
     TA* ap = a32 ? (TA*)w_ap32.p : (TA*)w_ap.p;
-    TM* y = m32 ? (TM*)w_y32.p : (TM*)w_y.p;
-    TM* z = m32 ? (TM*)w_z32.p : (TM*)w_z.p;
-    TM* rlow = m32 ? (TM*)w_r32.p : nullptr;
-    const TM* rin = m32 ? (const TM*)w_r32.p : (const TM*)r;
+    TM* y = w_y.p;
+    TM* z = w_z.p;
+    TM* rlow = nullptr;
+    const TM* rin = r;
     const int vb = grid_for(n);
     double* part = w_partial.p;
+    const bool jac = sweeps > 0;
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);  // abort flag
     init_scalars(S, maxit, a32, a32);
     if (warm) {
@@ -595,11 +714,14 @@ struct Operator {
     mlk::k_dot<double, double><<<vb, 256, 0, stream>>>(n, b, b, part);
     mlk::k_init_norm<<<1, 256, 0, stream>>>(part, vb, S, tol);
     HIP_CHECK(hipGetLastError());
-    if (m32) mlk::k_convert<double, float><<<vb, 256, 0, stream>>>(n, r, (float*)rlow);
-    fill_sentinel<TM>(y);
-    fill_sentinel<TM>(z);
-    precondition<TM>(rin, y, z, S);
-    mlk::k_dot<double, TM><<<vb, 256, 0, stream>>>(n, r, z, part);
+    if (jac) {
+      apply_sweeps(rin, z, S, r, part);
+    } else {
+      fill_sentinel<TM>(y);
+      fill_sentinel<TM>(z);
+      precondition(rin, y, z, S);
+      mlk::k_dot<double, TM><<<vb, 256, 0, stream>>>(n, r, z, part);
+    }
     mlk::k_init_rz<<<1, 256, 0, stream>>>(part, vb, S);
     mlk::k_convert<TM, TA><<<vb, 256, 0, stream>>>(n, z, p);
     HIP_CHECK(hipGetLastError());
@@ -612,7 +734,9 @@ struct Operator {
         spmv<TA>(p, ap, part, S);
         mlk::k_alpha<<<1, 256, 0, stream>>>(part, vb, S);
         hipEvent_t ev = prof.begin(stream);
-        mlk::k_update_xr<TA, TM><<<vb, 256, 0, stream>>>(n, S, x, r, p, ap, rlow, y, z, part);
+        mlk::k_update_xr<TA, TM><<<vb, 256, 0, stream>>>(n, S, x, r, p, ap, rlow,
+                                                         jac ? nullptr : y, jac ? nullptr : z,
+                                                         part);
         prof.end(P_VEC, ev, stream);
         mlk::k_converged<<<1, 256, 0, stream>>>(part, vb, S);
         if (a32) {
@@ -620,10 +744,14 @@ struct Operator {
           mlk::k_replace_residual<TM><<<vb, 256, 0, stream>>>(n, S, b, w_ax.p, r, rlow, part);
           mlk::k_replaced<<<1, 256, 0, stream>>>(part, vb, S);
         }
-        precondition<TM>(rin, y, z, S);
-        ev = prof.begin(stream);
-        mlk::k_dot<double, TM><<<vb, 256, 0, stream>>>(n, r, z, part);
-        prof.end(P_VEC, ev, stream);
+        if (jac) {
+          apply_sweeps(rin, z, S, r, part);
+        } else {
+          precondition(rin, y, z, S);
+          ev = prof.begin(stream);
+          mlk::k_dot<double, TM><<<vb, 256, 0, stream>>>(n, r, z, part);
+          prof.end(P_VEC, ev, stream);
+        }
         mlk::k_beta<<<1, 256, 0, stream>>>(part, vb, S);
         ev = prof.begin(stream);
         mlk::k_update_p<TA, TM><<<vb, 256, 0, stream>>>(n, S, p, z);
@@ -646,8 +774,8 @@ struct Operator {
   // ---------------------------------------------------------------- autotuner
   // Average duration (ms) of `reps` preconditioner applications to w_r (or nrep copies of it)
   // on the current grids; negative if a launch stalled.
-  template <typename T>
-  double time_precondition(int reps, int nrep, T* r, T* y, T* z) {
+  double time_precondition(int reps, int nrep, double* r, double* y, double* z) {
+    typedef double T;
     const int64_t N = (int64_t)n * nrep;
     hipEvent_t a, b;
     HIP_CHECK(hipEventCreate(&a));
@@ -658,7 +786,7 @@ struct Operator {
       mlk::k_fill_bits<T><<<grid_for(N), 256, 0, stream>>>(y, N, mlk::Bits<T>::S);
       mlk::k_fill_bits<T><<<grid_for(N), 256, 0, stream>>>(z, N, mlk::Bits<T>::S);
       HIP_CHECK(hipEventRecord(a, stream));
-      precondition<T>(r, y, z, nullptr, nullptr, nullptr, nrep);
+      precondition(r, y, z, nullptr, nullptr, nullptr, nrep);
       HIP_CHECK(hipEventRecord(b, stream));
       HIP_CHECK(hipEventSynchronize(b));
       float t = 0.0f;
@@ -672,31 +800,33 @@ struct Operator {
   }
 
   // Picks, by timing, the waves per CU of the triangular solves (single right-hand side) and,
-  // for nrhs > 1, the right-hand sides per batched pass. Results do not change: the solves are
+  // for nrhs > 1, the right-hand sides per pass. Results do not change: the solves are
   // bit-exact under any schedule. chosen[3]: waves per CU, right-hand sides per pass, and the
-  // per-CU occupancy the runtime reports.
+  // per-CU occupancy the runtime reports. An ML_MIXED_JACOBI handle then times whole solves of
+  // a fixed pseudo-random right-hand side with 0 (the exact solves), 2, 3, 4, 6 and 8 sweeps
+  // and keeps the fastest that converges (which changes its iterates, within the solve's
+  // tolerance).
   int autotune(int64_t nrhs, int32_t* chosen) {
     if (!factored) {
       int st = factor();
       if (st != ML_OK) return st;
     }
-    const bool f32 = precision != ML_FP64;
+    const bool c32 = precision != ML_FP64;
     const int occ =
-        f32 ? occupancy_per_cu((const void*)mlk::k_trisolve<float, false, false>, mlk::TRI_THREADS)
+        c32 ? occupancy_per_cu((const void*)mlk::k_trisolve<double, false, false, false, float>,
+                               mlk::TRI_THREADS)
             : occupancy_per_cu((const void*)mlk::k_trisolve<double, false, false>, mlk::TRI_THREADS);
     const int cus = device_cus();
     const int nblk = std::max(fw.nblk, bw.nblk);
     const int top = std::max(1, (17 * occ) / 20);  // stay clear of the reported occupancy
     mlk::k_fill_f64<<<grid_for(n), 256, 0, stream>>>(w_r.p, n, 1.0);
-    if (f32) mlk::k_convert<double, float><<<grid_for(n), 256, 0, stream>>>(n, w_r.p, w_r32.p);
     double best = 1e300;
     int best_pc = std::max(1, (3 * occ) / 4);
     for (int pc = 2; pc <= top; pc += 2) {
-      (f32 ? full_tri32 : full_tri64) = std::min(nblk, pc * cus);
+      (c32 ? full_tri32 : full_tri64) = std::min(nblk, pc * cus);
       shrinks = 0;
       set_grids();
-      const double t = f32 ? time_precondition<float>(3, 1, w_r32.p, w_y32.p, w_z32.p)
-                           : time_precondition<double>(3, 1, w_r.p, w_y.p, w_z.p);
+      const double t = time_precondition(3, 1, w_r.p, w_y.p, w_z.p);
       if (t > 0 && t < best) {
         best = t;
         best_pc = pc;
@@ -704,7 +834,7 @@ struct Operator {
     }
     // the replicated solves of batched passes keep their own grid (cap_tri): with many copies
     // of every block they are bound by throughput, not by the dependency chain
-    (f32 ? full_tri32 : full_tri64) = std::min(nblk, best_pc * cus);
+    (c32 ? full_tri32 : full_tri64) = std::min(nblk, best_pc * cus);
     shrinks = 0;
     set_grids();
     if (nrhs > 1) {
@@ -715,23 +845,39 @@ struct Operator {
         if (k > nrhs && k != 8) break;
         const int64_t N = (int64_t)n * k;
         DBuf<double> r64, y64, z64;
-        DBuf<float> r32, y32, z32;
-        double t;
-        if (f32) {
-          r32.alloc(N); y32.alloc(N); z32.alloc(N);
-          mlk::k_fill_bits<float><<<grid_for(N), 256, 0, stream>>>(r32.p, N, 0x3f800000u);
-          t = time_precondition<float>(2, k, r32.p, y32.p, z32.p);
-        } else {
-          r64.alloc(N); y64.alloc(N); z64.alloc(N);
-          mlk::k_fill_f64<<<grid_for(N), 256, 0, stream>>>(r64.p, N, 1.0);
-          t = time_precondition<double>(2, k, r64.p, y64.p, z64.p);
-        }
+        r64.alloc(N); y64.alloc(N); z64.alloc(N);
+        mlk::k_fill_f64<<<grid_for(N), 256, 0, stream>>>(r64.p, N, 1.0);
+        const double t = time_precondition(2, k, r64.p, y64.p, z64.p);
         if (t > 0 && t / k < best_per) {
           best_per = t / k;
           best_k = k;
         }
       }
       rhs_per_pass = best_k;
+    }
+    if (precision == ML_MIXED_JACOBI) {
+      DBuf<double> tb, tx;
+      tb.alloc(n);
+      tx.alloc(n);
+      mlk::k_fill_hash<<<grid_for(n), 256, 0, stream>>>(tb.p, n, 0x6d61696c6c65ull);
+      HIP_CHECK(hipGetLastError());
+      double best_t = 1e300;
+      int best_k = sweeps;
+      for (int k : {0, 2, 3, 4, 6, 8}) {
+        set_sweeps(k);
+        mlk::PcgScalars hS;
+        HIP_CHECK(hipStreamSynchronize(stream));
+        const double t0 = now_s();
+        const int st = pcg<double, double>(tb.p, tx.p, w_r.p, ML_TOLERANCE, ML_MAX_ITERATIONS,
+                                           false, d_S.p, &hS);
+        const double t = now_s() - t0;
+        if (st == ML_DEVICE_TIMEOUT) return st;
+        if (st == ML_OK && t < best_t) {
+          best_t = t;
+          best_k = k;
+        }
+      }
+      set_sweeps(best_k);
     }
     if (chosen) {
       chosen[0] = best_pc;
@@ -745,7 +891,6 @@ struct Operator {
   // Workspace of the batched solves (allocated on first use, grown as needed).
   DBuf<double> wb_r, wb_p, wb_ap, wb_dot;
   DBuf<double> wb_y64, wb_z64;
-  DBuf<float> wb_y32, wb_z32, wb_rlow;
   DBuf<mlk::PcgScalars> wb_S;
 
   // Per-system shift ladder of a block-diagonal operator (Pcg::new, iterative.rs:201-207, for
@@ -792,30 +937,21 @@ struct Operator {
   // nseg independent PCG solves (iterative.rs:210-248 each) in one set of launches: either
   // nrep right-hand sides of this operator stored one after the other (nseg = nrep), or the
   // systems of this block-diagonal operator (nrep = 1). System s owns the vector entries
-  // [seg_off[s], seg_off[s+1]) and seg_of[i] is the system of entry i. FP64 operator and
-  // iterate; TM is the arithmetic of the factor and the preconditioned residual. The scalars
-  // of every system end in *hS.
+  // [seg_off[s], seg_off[s+1]) and seg_of[i] is the system of entry i. FP64 operator,
+  // iterate and preconditioner vectors (the factor's coefficients in the handle's precision).
+  // The scalars of every system end in *hS.
   template <typename TM>
   int pcg_batch(int nrep, int nseg, const int64_t* seg_off, const int* seg_of, const double* b,
                 double* x, double tol, int64_t maxit, std::vector<mlk::PcgScalars>* hS) {
-    const bool m32 = std::is_same<TM, float>::value;
+    static_assert(std::is_same<TM, double>::value, "the preconditioner runs on FP64 vectors");
     const int64_t N = (int64_t)n * nrep;
     for (auto* v : {&wb_r, &wb_p, &wb_ap}) if ((int64_t)v->n < N) v->alloc(N);
     if ((int64_t)wb_dot.n < 4 * nseg) wb_dot.alloc(4 * nseg);
     if ((int64_t)wb_S.n < nseg) wb_S.alloc(nseg);
-    TM* y;
-    TM* z;
+    for (auto* v : {&wb_y64, &wb_z64}) if ((int64_t)v->n < N) v->alloc(N);
+    TM* y = wb_y64.p;
+    TM* z = wb_z64.p;
     TM* rlow = nullptr;
-    if (m32) {
-      for (auto* v : {&wb_y32, &wb_z32, &wb_rlow}) if ((int64_t)v->n < N) v->alloc(N);
-      y = (TM*)wb_y32.p;
-      z = (TM*)wb_z32.p;
-      rlow = (TM*)wb_rlow.p;
-    } else {
-      for (auto* v : {&wb_y64, &wb_z64}) if ((int64_t)v->n < N) v->alloc(N);
-      y = (TM*)wb_y64.p;
-      z = (TM*)wb_z64.p;
-    }
     double* r = wb_r.p;
     double* p = wb_p.p;
     double* ap = wb_ap.p;
@@ -824,17 +960,16 @@ struct Operator {
     double* rr = wb_dot.p + 2 * nseg;
     double* rz = wb_dot.p + 3 * nseg;
     mlk::PcgScalars* S = wb_S.p;
-    const TM* rin = m32 ? (const TM*)rlow : (const TM*)r;
+    const TM* rin = r;
     const unsigned vb = grid_for(N), sb = grid_for(nseg);
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);  // abort flag
     mlk::k_fill_f64<<<vb, 256, 0, stream>>>(x, N, 0.0);
     HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
     mlk::k_seg_dot<double, double><<<nseg, 256, 0, stream>>>(seg_off, b, b, bb);
     mlk::k_seg_init<<<sb, 256, 0, stream>>>(nseg, bb, S, tol, maxit);
-    if (m32) mlk::k_convert<double, float><<<vb, 256, 0, stream>>>(N, r, (float*)rlow);
     mlk::k_fill_bits<TM><<<vb, 256, 0, stream>>>(y, N, mlk::Bits<TM>::S);
     mlk::k_fill_bits<TM><<<vb, 256, 0, stream>>>(z, N, mlk::Bits<TM>::S);
-    precondition<TM>(rin, y, z, nullptr, nullptr, nullptr, nrep);
+    precondition(rin, y, z, nullptr, nullptr, nullptr, nrep);
     mlk::k_seg_dot<double, TM><<<nseg, 256, 0, stream>>>(seg_off, r, z, rz);
     mlk::k_seg_rz<<<sb, 256, 0, stream>>>(nseg, rz, S);
     mlk::k_convert<TM, double><<<vb, 256, 0, stream>>>(N, z, p);
@@ -869,7 +1004,7 @@ struct Operator {
         prof.end(P_VEC, ev, stream);
         mlk::k_seg_dot<double, double><<<nseg, 256, 0, stream>>>(seg_off, r, r, rr);
         mlk::k_seg_converged<<<sb, 256, 0, stream>>>(nseg, rr, S);
-        precondition<TM>(rin, y, z, nullptr, nullptr, nullptr, nrep);
+        precondition(rin, y, z, nullptr, nullptr, nullptr, nrep);
         mlk::k_seg_dot<double, TM><<<nseg, 256, 0, stream>>>(seg_off, r, z, rz);
         mlk::k_seg_beta<<<sb, 256, 0, stream>>>(nseg, rz, S);
         ev = prof.begin(stream);
@@ -907,9 +1042,9 @@ struct Operator {
     d_seg.upload_vec(seg, stream);
     int st;
     for (;;) {
-      st = precision == ML_FP64
-               ? pcg_batch<double>(nrep, nrep, d_off_seg.p, d_seg.p, b, x, tol, maxit, hS)
-               : pcg_batch<float>(nrep, nrep, d_off_seg.p, d_seg.p, b, x, tol, maxit, hS);
+      // batched passes apply the factor by the exact solves (FP32 coefficients for the mixed
+      // handles), whose replicated blocks fill the device
+      st = pcg_batch<double>(nrep, nrep, d_off_seg.p, d_seg.p, b, x, tol, maxit, hS);
       if (st == ML_DEVICE_TIMEOUT && shrink()) continue;
       break;
     }
@@ -947,17 +1082,18 @@ struct Operator {
       if (precision == ML_FP64) {
         st = pcg<double, double>(b, x, w_r.p, tol, maxit, false, d_S.p, &hS);
       } else if (precision == ML_MIXED) {
-        st = pcg<float, float>(b, x, w_r.p, tol, maxit, false, d_S.p, &hS);
+        st = pcg<float, double>(b, x, w_r.p, tol, maxit, false, d_S.p, &hS);
         if (st == PCG_STALLED) {
           // continue from the iterate with the FP64 operator (the preconditioner stays FP32)
           it0 = hS.it;
           updates0 = hS.updates;
           escalated = 1;
-          st = pcg<double, float>(b, x, w_r.p, tol, std::max<int64_t>(1, maxit - it0), true,
-                                  d_S.p, &hS);
+          st = pcg<double, double>(b, x, w_r.p, tol, std::max<int64_t>(1, maxit - it0), true,
+                                   d_S.p, &hS);
         }
       } else {
-        st = pcg<double, float>(b, x, w_r.p, tol, maxit, false, d_S.p, &hS);
+        // ML_MIXED_IC0 and ML_MIXED_JACOBI: FP64 operator, the FP32 factor
+        st = pcg<double, double>(b, x, w_r.p, tol, maxit, false, d_S.p, &hS);
       }
       // a stalled triangular solve: solve again from x0 = 0 on smaller grids
       if (st == ML_DEVICE_TIMEOUT && shrink()) continue;
@@ -2246,6 +2382,9 @@ ml_status ml_layout_verify(int64_t n, const int64_t* offsets, const int64_t* col
     DeviceLayouts d;
     std::string err = device_layouts(lp, kTriCaps, &d);
     if (err.empty()) err = check_layouts(lp, d, kTriCaps);
+    SweepLayout lo, up;
+    if (err.empty()) err = sweep_layouts(lp, &lo, &up);
+    if (err.empty()) err = check_sweep_layouts(lp, lo, up);
     if (!err.empty()) throw ArgError(err);
     if (stats) {
       stats[0] = (int64_t)d.fw.bstart.size() - 1;
@@ -2266,7 +2405,7 @@ ml_status ml_pcg_create(int64_t n, const int64_t* offsets, const int64_t* cols,
                         const double* values, ml_precision precision, ml_pcg** out) {
   return guarded([&]() -> int {
     if (!out || !values) throw ArgError("null argument");
-    if (precision < ML_FP64 || precision > ML_MIXED_IC0) throw ArgError("bad precision");
+    if (precision < ML_FP64 || precision > ML_MIXED_JACOBI) throw ArgError("bad precision");
     LowerPattern lp = pattern_from_csr(n, offsets, cols);
     std::unique_ptr<ml_pcg> h(new ml_pcg);
     h->op.precision = precision;
@@ -2426,6 +2565,30 @@ ml_status ml_pcg_set_wait_limit(ml_pcg* h, int64_t ticks) {
   });
 }
 
+ml_status ml_pcg_set_sweeps(ml_pcg* h, int sweeps) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    h->op.set_sweeps(sweeps);
+    return ML_OK;
+  });
+}
+
+int ml_pcg_sweeps(const ml_pcg* h) { return h ? h->op.sweeps : 0; }
+
+ml_status ml_model_set_sweeps(ml_model* h, int sweeps) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    if (h->m.all_fixed()) {
+      if (sweeps < 0 || sweeps > 64) throw ArgError("sweeps must be in 0..64");
+      return ML_OK;  // no free system
+    }
+    h->m.op.set_sweeps(sweeps);
+    return ML_OK;
+  });
+}
+
+int ml_model_sweeps(const ml_model* h) { return h ? h->m.op.sweeps : 0; }
+
 ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* trace_fw,
                        int64_t* trace_bw, int64_t capacity) {
   return guarded([&]() -> int {
@@ -2462,7 +2625,7 @@ ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elemen
     if (!out || !nodes || !conn || !young || !nu || !fixed) throw ArgError("null argument");
     if (solver == ML_DIRECT)
       throw ArgError("the direct (skyline) solver is not part of the device library");
-    if (precision < ML_FP64 || precision > ML_MIXED_IC0) throw ArgError("bad precision");
+    if (precision < ML_FP64 || precision > ML_MIXED_JACOBI) throw ArgError("bad precision");
     if (info) {
       std::memset(info, 0, sizeof(*info));
       info->bad_element = -1;
@@ -2551,6 +2714,16 @@ ml_status ml_model_stresses(ml_model* h, const double* u, double* out) {
     d_out.alloc(count);
     m.stresses(m.d_u.p, d_out.p);
     HIP_CHECK(hipMemcpyAsync(out, d_out.p, count * sizeof(double), hipMemcpyDeviceToHost, m.s()));
+    HIP_CHECK(hipStreamSynchronize(m.s()));
+    return ML_OK;
+  });
+}
+
+ml_status ml_model_stresses_device(ml_model* h, const double* u, double* out) {
+  return guarded([&]() -> int {
+    if (!h || !u || !out) throw ArgError("null argument");
+    Model& m = h->m;
+    m.stresses(u, out);
     HIP_CHECK(hipStreamSynchronize(m.s()));
     return ML_OK;
   });
@@ -2699,7 +2872,7 @@ ml_status ml_thermal_create(int64_t n_nodes, const double* nodes, int64_t n_elem
   return guarded([&]() -> int {
     if (!out || !nodes || !conn || !conductivity || !rho_c || !fixed) throw ArgError("null argument");
     if (nconv > 0 && (!conv_elements || !conv_faces || !film || !sink)) throw ArgError("null argument");
-    if (precision < ML_FP64 || precision > ML_MIXED_IC0) throw ArgError("bad precision");
+    if (precision < ML_FP64 || precision > ML_MIXED_JACOBI) throw ArgError("bad precision");
     if (info) {
       std::memset(info, 0, sizeof(*info));
       info->bad_element = -1;
@@ -3141,11 +3314,8 @@ ml_status ml_batch_solve(ml_batch* b, const double* const* f, double* const* u,
     std::vector<mlk::PcgScalars> hS;
     int st;
     for (;;) {
-      st = op.precision == ML_FP64
-               ? op.pcg_batch<double>(1, b->nseg, b->d_seg_off.p, b->d_seg.p, b->d_rhs.p, b->d_x.p,
-                                      ML_TOLERANCE, ML_MAX_ITERATIONS, &hS)
-               : op.pcg_batch<float>(1, b->nseg, b->d_seg_off.p, b->d_seg.p, b->d_rhs.p, b->d_x.p,
-                                     ML_TOLERANCE, ML_MAX_ITERATIONS, &hS);
+      st = op.pcg_batch<double>(1, b->nseg, b->d_seg_off.p, b->d_seg.p, b->d_rhs.p, b->d_x.p,
+                                ML_TOLERANCE, ML_MAX_ITERATIONS, &hS);
       if (st == ML_DEVICE_TIMEOUT && op.shrink()) continue;
       break;
     }

@@ -77,45 +77,62 @@ def test_spmv_factor_and_preconditioner(gpu, name):
     assert np.array_equal(pcg.precondition(r), csr.precondition(s.l_values(), r))
 
 
-def _precondition_f32(off, cols, l, r):
-    """iterative.rs:173-188 in single precision: every product, sum and quotient rounded to
-    float32, forward sums in ascending column order, backward subtractions in decreasing row
-    order."""
-    n = len(off) - 1
-    l = l.astype(np.float32)
-    y = np.zeros(n, dtype=np.float32)
-    for i in range(n):
-        acc = np.float32(0.0)
-        for p in range(off[i], off[i + 1] - 1):
-            acc = np.float32(acc + np.float32(l[p] * y[cols[p]]))
-        y[i] = np.float32(np.float32(np.float32(r[i]) - acc) / l[off[i + 1] - 1])
-    below = [[] for _ in range(n)]  # column j: (i, l_ij), i > j, ascending i
-    for i in range(n):
-        for p in range(off[i], off[i + 1] - 1):
-            below[cols[p]].append((i, l[p]))
-    z = np.zeros(n, dtype=np.float32)
-    for j in range(n - 1, -1, -1):
-        acc = y[j]
-        for i, v in reversed(below[j]):
-            acc = np.float32(acc - np.float32(v * z[i]))
-        z[j] = np.float32(acc / l[off[j + 1] - 1])
-    return z
-
-
-@pytest.mark.parametrize("precision", ["MIXED", "MIXED_IC0"])
+@pytest.mark.parametrize("precision", ["MIXED", "MIXED_IC0", "MIXED_JACOBI"])
 def test_mixed_preconditioner_applies_the_fp32_factor(gpu, precision):
-    """A mixed handle's preconditioner is its FP32 factor applied in FP32 arithmetic: bit for
-    bit the float32 restatement of the reference's two triangular loops, on every call."""
+    """A mixed handle stores its factor in FP32 and applies it with FP64 vectors and sums: its
+    exact preconditioner is, bit for bit, the oracle's FP64 precondition (iterative.rs:
+    173-188) of the FP32 factor widened, on every call."""
     p = PROBLEMS["jittered_cube_8"]
     s = _oracle_system(p)
     off, cols, vals = s.csr()
     print(f"{precision} preconditioner n={s.n}", flush=True)
     pcg = gpu.Pcg(gpu.Csr(off, cols, vals), precision=getattr(gpu, precision))
+    pcg.set_sweeps(0)
     r = np.cos(0.11 * np.arange(s.n))
     z = pcg.precondition(r)
     l, _ = pcg.factor()  # the FP32 factor, widened
-    assert np.array_equal(z, _precondition_f32(off, cols, l, r).astype(np.float64))
+    assert np.array_equal(l.astype(np.float32).astype(np.float64), l)
+    csr = oracle.Csr.from_arrays(off, cols, vals)
+    assert np.array_equal(z, csr.precondition(l, r))
     assert np.array_equal(pcg.precondition(r), z)
+    # the FP32 factor is the FP64 one to FP32 accuracy (it is factored in FP32 arithmetic)
+    assert s.factorize() == oracle.OK
+    l64 = s.l_values()
+    assert np.abs(l - l64).max() <= 1e-5 * np.abs(l64).max()
+
+
+@pytest.mark.parametrize("sweeps", [1, 2, 3, 4, 7])
+def test_jacobi_sweeps_match_their_restatement(gpu, sweeps):
+    """ML_MIXED_JACOBI's preconditioner: `sweeps` Jacobi sweeps of L y = r and of L^T z = y on
+    the FP32 factor, bit for bit the float64 restatement, on every call; more sweeps approach
+    the exact triangular solves."""
+    from sweeps import jacobi_precondition
+    p = PROBLEMS["jittered_cube_8"]
+    s = _oracle_system(p)
+    off, cols, vals = s.csr()
+    print(f"jacobi sweeps={sweeps} n={s.n}", flush=True)
+    pcg = gpu.Pcg(gpu.Csr(off, cols, vals), precision=gpu.MIXED_JACOBI)
+    assert pcg.sweeps() == 4
+    pcg.set_sweeps(sweeps)
+    assert pcg.sweeps() == sweeps
+    r = np.cos(0.11 * np.arange(s.n))
+    z = pcg.precondition(r)
+    l, _ = pcg.factor()
+    assert np.array_equal(z, jacobi_precondition(off, cols, l, r, sweeps))
+    assert np.array_equal(pcg.precondition(r), z)
+    exact = oracle.Csr.from_arrays(off, cols, vals).precondition(l, r)
+    far = rel_l2(jacobi_precondition(off, cols, l, r, 1), exact)
+    assert rel_l2(jacobi_precondition(off, cols, l, r, 12), exact) < far
+
+
+def test_fp64_handle_refuses_sweeps(gpu):
+    p = PROBLEMS["reference_cantilever"]
+    s = _oracle_system(p)
+    off, cols, vals = s.csr()
+    pcg = gpu.Pcg(gpu.Csr(off, cols, vals))
+    pcg.set_sweeps(0)
+    with pytest.raises(ValueError, match="exactly"):
+        pcg.set_sweeps(2)
 
 
 @pytest.mark.parametrize("name", list(PROBLEMS))
@@ -139,7 +156,7 @@ def test_fp64_solve_matches_oracle_pcg(gpu, name):
     assert np.array_equal(st[..., :3], ref["stresses"][..., :3])
 
 
-@pytest.mark.parametrize("precision", ["MIXED", "MIXED_IC0"])
+@pytest.mark.parametrize("precision", ["MIXED", "MIXED_IC0", "MIXED_JACOBI"])
 @pytest.mark.parametrize("name", list(PROBLEMS))
 def test_mixed_solve_matches_oracle(gpu, name, precision):
     """FP32 operator and/or preconditioner with FP64 iterate, residual and dots: within
