@@ -40,11 +40,11 @@ def spmv_bytes(n, nnz_full, val):
     return nnz_full * (val + 4) + (n + 1) * 4 + 2 * n * val
 
 
-def sweep_bytes(n, nnz_strict):
-    """Algorithmic bytes of one Jacobi sweep (k_sweep): FP32 coefficients and int32 columns of
-    the strict triangle, row lengths, and the FP64 right-hand side, reciprocal diagonal,
-    previous iterate and result."""
-    return nnz_strict * (4 + 4) + n * (4 + 8 + 8 + 8 + 8)
+def sweep_bytes(n, nnz_strict, col_bytes=4):
+    """Algorithmic bytes of one Jacobi sweep (k_sweep): FP32 coefficients and the columns
+    (int32, or 16-bit distances from the row) of the strict triangle, row lengths, and the
+    FP64 right-hand side, reciprocal diagonal, previous iterate and result."""
+    return nnz_strict * (4 + col_bytes) + n * (4 + 8 + 8 + 8 + 8)
 
 
 def trisolve_bytes(n, nnz_l, val):
@@ -627,7 +627,8 @@ def main():
     mval = 8 if head == "fp64" else 4
     kinds = {"spmv": "spmv32" if head == "mixed" else "spmv64",
              "precondition": "precond64" if head == "fp64" else "precond32"}
-    pre_bytes = (2 * sweeps * sweep_bytes(n, nnz_l - n) if sweeps
+    col_bytes = models[head].sweep_column_bytes() or 4
+    pre_bytes = (2 * sweeps * sweep_bytes(n, nnz_l - n, col_bytes) if sweeps
                  else 2 * trisolve_bytes(n, nnz_l, mval))
     bytes_per = {"spmv": spmv_bytes(n, nnz_full, aval), "precondition": pre_bytes}
     kernels = {}
@@ -640,7 +641,8 @@ def main():
                          "achieved_GBs": round(achieved, 1),
                          "share_of_step": round(ms / 1e3 / elapsed, 3)}
     kernels["precondition"]["form"] = (f"{sweeps} Jacobi sweeps per triangular system "
-                                       f"(k_sweep, {2 * sweeps} launches)" if sweeps else
+                                       f"(k_sweep, {2 * sweeps} launches, {col_bytes}-byte "
+                                       f"columns)" if sweeps else
                                        "exact dataflow triangular solves (k_trisolve, 2 launches)")
     ic_ms, ic_cnt = prof["ic0"]
     kernels["ic0_factor"] = {"ms_per_step": round(ic_ms / args.steps, 2),

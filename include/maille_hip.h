@@ -215,6 +215,9 @@ ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elemen
 /* ml_pcg_set_sweeps / ml_pcg_sweeps for the model's operator. */
 ml_status ml_model_set_sweeps(ml_model* m, int sweeps);
 int ml_model_sweeps(const ml_model* m);
+/* Bytes per column index the sweeps read (2: 16-bit distances from the row, when every
+ * |i - j| of the factor fits; 4: int32), 0 before any sweep layout is built. */
+int ml_model_sweep_column_bytes(const ml_model* m);
 /* Recompute the element stiffnesses and K_ff values on the device (after ml_model_set_young). */
 ml_status ml_model_assemble(ml_model* m);
 /* Replace the per-element Young's moduli (n_elements values, host) without touching the

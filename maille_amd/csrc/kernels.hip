@@ -63,6 +63,28 @@ __device__ inline T await_value(T* p, int* abort_flag, long long limit) {
   }
 }
 
+// Co-residency probe of a persistent kernel's grid (a launch that does nothing else): thread
+// 0 of every workgroup counts it in probe[0], then waits, at most `limit` ticks of the steady
+// wall clock, until all gridDim.x workgroups have arrived; one that gives up sets probe[1]. The
+// other waves wait at the barrier meanwhile, so each workgroup holds the resources of the
+// kernel that calls this (its registers and LDS) while it is counted. Every wait ends, so the
+// launch always drains: workgroups that were not resident run once others have exited.
+__device__ inline void residency_probe(int* probe, long long limit) {
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(probe, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const long long t0 = wall_clock64();
+    while (__hip_atomic_load(probe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+           (int)gridDim.x) {
+      if (wall_clock64() - t0 > limit) {
+        __hip_atomic_store(probe + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(8);
+    }
+  }
+  __syncthreads();
+}
+
 __device__ inline double lane_broadcast(double v, int lane) {
   int lo = __builtin_amdgcn_readlane((int)(__double_as_longlong(v) & 0xffffffffll), lane);
   int hi = __builtin_amdgcn_readlane((int)(__double_as_longlong(v) >> 32), lane);
@@ -1290,10 +1312,13 @@ __global__ __launch_bounds__(256) void k_spmv_rep(int n, int nrep, const int* __
 // coefficients widened exactly and FP64 vectors and arithmetic (contraction off, so a float64
 // restatement reproduces it bit for bit). FIRST: the previous iterate is in = rhs * rd, formed
 // as it is read. Lane i of a 64-row slice owns row i; entries are column-major in the slice.
-// With `partial`, the block's share of dot.out (the PCG's r.z) is written, in a fixed tree.
-template <bool FIRST>
+// D16: columns are read as 16-bit distances from the row, j = i + sign * d (sign -1 for the
+// lower triangle, +1 for the upper one), else as int32 indices. With `partial`, the block's
+// share of dot.out (the PCG's r.z) is written, in a fixed tree.
+template <bool FIRST, bool D16>
 __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ soff,
                                                const int* __restrict__ scol,
+                                               const uint16_t* __restrict__ sdel, int sign,
                                                const int* __restrict__ slen,
                                                const float* __restrict__ sval,
                                                const double* __restrict__ rd,
@@ -1310,6 +1335,7 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
   if (i < n) {
     const int base = soff[i >> 6] + (i & 63);
     const int len = slen[i];
+    auto column = [&](int at) { return D16 ? i + sign * (int)sdel[at] : scol[at]; };
     double s = 0.0;
     int k = 0;
     // eight entries at a time: columns and coefficients, then the eight gathers, so that a
@@ -1320,7 +1346,7 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
       double x[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        c[u] = scol[base + ((k + u) << 6)];
+        c[u] = column(base + ((k + u) << 6));
         v[u] = sval[base + ((k + u) << 6)];
       }
 #pragma unroll
@@ -1330,7 +1356,7 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
     }
     for (; k < len; ++k) {
       const int at = base + (k << 6);
-      const int c = scol[at];
+      const int c = column(at);
       s = s + (double)sval[at] * (FIRST ? rhs[c] * rd[c] : in[c]);
     }
     const double y = (rhs[i] - s) * rd[i];
@@ -1731,6 +1757,10 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
     const T* __restrict__ rhs, T* out, const PcgScalars* __restrict__ S, int* abort_flag,
     long long limit, long long* __restrict__ trace, int nrep) {
 #pragma clang fp contract(off)
+  if (nrep < 0) {  // a co-residency probe of this kernel (abort_flag: the probe's two words)
+    residency_probe(abort_flag, limit);
+    return;
+  }
   if (S && S->done) return;
   typedef typename Bits<T>::U U;
   __shared__ U suv[TRI_UCAP];    // values of the distinct external columns (sentinel: not yet)
@@ -2069,6 +2099,10 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ order
                                             int* where, const int* __restrict__ seg,
                                             const T* __restrict__ seg_shift) {
 #pragma clang fp contract(off)
+  if (n < 0) {  // a co-residency probe of this kernel (fail: the probe's two words)
+    residency_probe(fail, limit);
+    return;
+  }
   __shared__ int scol[MAXROW];
   __shared__ T sw[MAXROW];
   __shared__ T sa[MAXROW];

@@ -1,5 +1,7 @@
 #include "pattern.hpp"
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -723,6 +725,24 @@ std::string sweep_layouts(const LowerPattern& lp, SweepLayout* lower, SweepLayou
   for (SweepLayout* s : {lower, upper}) {
     s->dsrc.resize(n);
     for (i64 i = 0; i < n; ++i) s->dsrc[i] = (int32_t)(lp.offsets[i + 1] - 1);
+    // 16-bit distances from the row, when every one fits
+    bool fits = true;
+    for (i64 i = 0; i < n && fits; ++i) {
+      const i64 base = s->soff[i >> 6] + (i & 63);
+      for (i64 k = 0; k < s->slen[i]; ++k)
+        if (std::abs((i64)s->scol[base + (k << 6)] - i) > 65535) { fits = false; break; }
+    }
+    s->sdelta.clear();
+    if (fits) {
+      s->sdelta.assign(s->scol.size(), 0);
+      parallel_ranges(n, [&](i64 b, i64 e) {
+        for (i64 i = b; i < e; ++i) {
+          const i64 base = s->soff[i >> 6] + (i & 63);
+          for (i64 k = 0; k < s->slen[i]; ++k)
+            s->sdelta[base + (k << 6)] = (uint16_t)std::abs((i64)s->scol[base + (k << 6)] - i);
+        }
+      });
+    }
   }
   return std::string();
 }
@@ -758,8 +778,12 @@ std::string check_sweep_layouts(const LowerPattern& lp, const SweepLayout& lower
         const i64 c = u ? cols[i][k].first : lp.cols[lp.offsets[i] + k];
         const i64 src = u ? cols[i][k].second : lp.offsets[i] + k;
         if (s.scol[at] != c || s.ssrc[at] != src) return "sweep layout entry out of order";
+        if (!s.sdelta.empty() && (u ? i + s.sdelta[at] : i - s.sdelta[at]) != c)
+          return "sweep layout: wrong 16-bit distance";
       }
     }
+    if (!s.sdelta.empty() && s.sdelta.size() != s.scol.size())
+      return "sweep layout: 16-bit distances of the wrong size";
   }
   return std::string();
 }

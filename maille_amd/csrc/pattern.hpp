@@ -141,6 +141,9 @@ struct SweepLayout {
   int64_t slices = 0;
   std::vector<int32_t> soff;  // slices + 1, in entries
   std::vector<int32_t> scol, ssrc, slen, dsrc;  // dsrc[i]: the diagonal l_ii in lower-CSR order
+  // |i - j| of every entry as 16 bits (0 for padding) when all fit, else empty: the kernel then
+  // reads 2 bytes of column per entry instead of 4
+  std::vector<uint16_t> sdelta;
 };
 // Both triangles' layouts; an empty string or the reason they do not fit 32-bit offsets.
 std::string sweep_layouts(const LowerPattern& lp, SweepLayout* lower, SweepLayout* upper);

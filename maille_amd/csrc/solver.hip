@@ -13,7 +13,9 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -135,19 +137,10 @@ int device_cus() {
   return cus;
 }
 
-// Workgroups of a persistent dataflow kernel launched per CU: three quarters of the occupancy
-// the runtime reports. The reported occupancy of LDS-heavy kernels has not always been
-// co-resident on gfx950, and the dataflow kernels need every workgroup of their grid resident;
-// the margin costs little (their speed hardly changes between 7 and 13 workgroups per CU). A
-// dataflow launch that stalls anyway ends through its bounded waits, and the caller then
-// repeats the work on a smaller grid (Operator::shrink).
 int occupancy_per_cu(const void* kernel, int threads) {
   int per_cu = 0;
   HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
   return std::max(1, per_cu);
-}
-int dataflow_grid(const void* kernel, int threads) {
-  return std::max(1, (3 * occupancy_per_cu(kernel, threads)) / 4) * device_cus();
 }
 
 // The bound of every dataflow wait: 2 s of the device's steady wall clock (wall_clock64 ticks
@@ -158,6 +151,74 @@ long long wait_limit() {
   HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
   if (khz <= 0) khz = 100000;
   return 2000ll * khz;
+}
+
+// Workgroups of a persistent dataflow kernel that the device holds at once, per CU. The
+// dataflow kernels need every workgroup of their grid resident (the lowest unfinished block
+// must always have a running workgroup), and a cooperative launch admits whatever grid the
+// runtime's occupancy figure allows. That figure has over-reported LDS-heavy kernels on gfx950
+// by one workgroup per CU (26 reported against 25 held at 6 KB of LDS, 13 against 12 at
+// 12 KB), which would leave workgroups of such a grid waiting for a place. So the figure is
+// only where the search starts: the kernel itself is launched as a co-residency probe at that
+// many workgroups per CU, then one fewer, until every workgroup of the probe was resident at
+// once (the probe's waits are bounded, so a failed probe drains too). The result is cached
+// per device and kernel; it holds while no other kernel occupies the device during a dataflow
+// launch (a launch that stalls then ends through its bounded waits and is repeated on a
+// smaller grid, Operator::shrink).
+struct ProbeKey {
+  int device;
+  const void* kernel;
+  bool operator<(const ProbeKey& o) const {
+    return device != o.device ? device < o.device : kernel < o.kernel;
+  }
+};
+std::mutex g_probe_mu;
+std::map<ProbeKey, int> g_resident;
+
+template <class Launch>
+int resident_per_cu(const void* kernel, int threads, hipStream_t stream, Launch launch_probe) {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_probe_mu);
+  auto it = g_resident.find({dev, kernel});
+  if (it != g_resident.end()) return it->second;
+  const int cus = device_cus();
+  const long long limit = wait_limit() / 100;  // 20 ms: a resident grid gathers in microseconds
+  DBuf<int> probe;
+  probe.alloc(2);
+  int per_cu = occupancy_per_cu(kernel, threads);
+  for (; per_cu > 1; --per_cu) {
+    mlk::k_fill_i32<<<1, 64, 0, stream>>>(probe.p, 2, 0);
+    launch_probe(per_cu * cus, probe.p, limit);
+    HIP_CHECK(hipGetLastError());
+    int h[2] = {0, 0};
+    HIP_CHECK(hipMemcpyAsync(h, probe.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if (h[1] == 0 && h[0] == per_cu * cus) break;
+  }
+  g_resident[{dev, kernel}] = per_cu;
+  return per_cu;
+}
+
+// Probes of the dataflow kernels (normal launches: the probe measures what the hardware holds,
+// not what the runtime admits).
+template <typename... TArgs>
+int trisolve_resident(void (*kernel)(TArgs...), hipStream_t stream) {
+  return resident_per_cu((const void*)kernel, mlk::TRI_THREADS, stream,
+                         [&](int grid, int* probe, long long limit) {
+    kernel<<<grid, mlk::TRI_THREADS, 0, stream>>>(0, 0, nullptr, nullptr, nullptr, nullptr,
+                                                  nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                  nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                  nullptr, nullptr, probe, limit, nullptr, -1);
+  });
+}
+template <typename... TArgs>
+int ic0_resident(void (*kernel)(TArgs...), hipStream_t stream) {
+  return resident_per_cu((const void*)kernel, 64, stream,
+                         [&](int grid, int* probe, long long limit) {
+    kernel<<<grid, 64, 0, stream>>>(-1, nullptr, nullptr, nullptr, nullptr, nullptr, 0, probe,
+                                    nullptr, limit, nullptr, nullptr, nullptr);
+  });
 }
 
 // Launches a persistent dataflow kernel of `threads`-thread workgroups cooperatively, so that
@@ -260,10 +321,16 @@ struct Operator {
   // FP32 coefficients gathered after each factorisation, and RN64(1 / l_ii)
   struct SweepDev {
     DBuf<int> soff, scol, slen, ssrc;
+    DBuf<uint16_t> sdel;  // 16-bit column distances, when they fit (then scol is not kept)
     DBuf<float> sval;
+    bool d16 = false;
+    int64_t entries = 0;
     void set(const SweepLayout& l, hipStream_t st) {
+      entries = (int64_t)l.scol.size();
+      d16 = !l.sdelta.empty();
       soff.upload_vec(l.soff, st);
-      scol.upload_vec(l.scol, st);
+      if (d16) sdel.upload_vec(l.sdelta, st);
+      else scol.upload_vec(l.scol, st);
       slen.upload_vec(l.slen, st);
       ssrc.upload_vec(l.ssrc, st);
       sval.alloc(l.scol.size());
@@ -277,6 +344,7 @@ struct Operator {
   // grids of the dataflow kernels: the full ones, and the ones in use (full >> shrinks)
   int full_tri64 = 0, full_tri32 = 0, full_ic = 0, full_ic64 = 0;
   int cap_tri64 = 0, cap_tri32 = 0;  // dataflow grids before the cap by block count
+  int tri_per_cu = 1, ic_per_cu = 1, ic64_per_cu = 1;  // probed co-resident workgroups per CU
   int rhs_per_pass = 256;            // right-hand sides per batched pass (autotuned)
   int tri_grid64 = 0, tri_grid32 = 0, ic_grid = 0, ic_grid64 = 0;
   int shrinks = 0;   // grid halvings after a stalled dataflow launch
@@ -327,15 +395,26 @@ struct Operator {
     d_sval.alloc(d_scol.n);
     for (auto* b : {&w_x, &w_r, &w_p, &w_ap, &w_y, &w_z, &w_b, &w_d, &w_ax}) b->alloc(n);
     const int nblk = std::max(fw.nblk, bw.nblk);
-    cap_tri64 = dataflow_grid((const void*)mlk::k_trisolve<double, false, false>, mlk::TRI_THREADS);
-    cap_tri32 = dataflow_grid((const void*)mlk::k_trisolve<double, false, false, false, float>,
-                              mlk::TRI_THREADS);
+    // every dataflow grid is one whose co-residency was shown by a probe of the same kernel;
+    // the forward and backward solves share one grid, so it fits the tighter of the two
+    const int cus = device_cus();
+    if (precision == ML_FP64) {
+      tri_per_cu = std::min(trisolve_resident(mlk::k_trisolve<double, false, false>, stream),
+                            trisolve_resident(mlk::k_trisolve<double, true, true>, stream));
+      ic_per_cu = ic0_resident(mlk::k_ic0<double, 512>, stream);
+      ic64_per_cu = ic0_resident(mlk::k_ic0<double, 64>, stream);
+    } else {
+      tri_per_cu =
+          std::min(trisolve_resident(mlk::k_trisolve<double, false, false, false, float>, stream),
+                   trisolve_resident(mlk::k_trisolve<double, true, true, false, float>, stream));
+      ic_per_cu = ic0_resident(mlk::k_ic0<float, 512>, stream);
+      ic64_per_cu = ic0_resident(mlk::k_ic0<float, 64>, stream);
+    }
+    cap_tri64 = cap_tri32 = tri_per_cu * cus;
     full_tri64 = std::min<int>(nblk, cap_tri64);
     full_tri32 = std::min<int>(nblk, cap_tri32);
-    full_ic = std::min<int>(n, std::min(dataflow_grid((const void*)mlk::k_ic0<double, 512>, 64),
-                                        dataflow_grid((const void*)mlk::k_ic0<float, 512>, 64)));
-    full_ic64 = std::min<int>(n, std::min(dataflow_grid((const void*)mlk::k_ic0<double, 64>, 64),
-                                          dataflow_grid((const void*)mlk::k_ic0<float, 64>, 64)));
+    full_ic = std::min<int>(n, ic_per_cu * cus);
+    full_ic64 = std::min<int>(n, ic64_per_cu * cus);
     shrinks = 0;
     set_grids();
     w_partial.alloc(std::max<int64_t>(grid_for(n), 1));
@@ -385,8 +464,8 @@ struct Operator {
   // the FP32 factor's values into the sweep layouts, and the reciprocals of its diagonal
   void gather_sweeps() {
     for (SweepDev* t : {&sw_lo, &sw_up})
-      mlk::k_gather<float, float><<<grid_for(t->scol.n), 256, 0, stream>>>(
-          (int64_t)t->scol.n, t->ssrc.p, d_l32.p, t->sval.p);
+      mlk::k_gather<float, float><<<grid_for(t->entries), 256, 0, stream>>>(
+          t->entries, t->ssrc.p, d_l32.p, t->sval.p);
     mlk::k_gather<float, float><<<grid_for(n), 256, 0, stream>>>((int64_t)n, d_sw_dsrc.p, d_l32.p,
                                                                  d_sw_diag.p);
     mlk::k_recip64<float><<<grid_for(n), 256, 0, stream>>>((int64_t)n, d_sw_diag.p, d_sw_rd.p);
@@ -544,7 +623,10 @@ struct Operator {
     int& grid1 = c32 ? tri_grid32 : tri_grid64;
     int grid_rep = std::min<int64_t>((int64_t)std::max(fw.nblk, bw.nblk) * nrep,
                                      std::max(1, (c32 ? cap_tri32 : cap_tri64) >> shrinks));
-    int& grid = nrep > 1 ? grid_rep : grid1;
+    // the traced kernels have their own footprint: a launch of theirs that the runtime refuses
+    // lowers a copy of the grid, not the handle's
+    int grid_trace = grid1;
+    int& grid = nrep > 1 ? grid_rep : ((trace_fw || trace_bw) ? grid_trace : grid1);
     hipEvent_t ev = prof.begin(stream);
     int nn = n;
     long long limit = wait_ticks;
@@ -601,12 +683,13 @@ struct Operator {
                      double* out, bool last) {
       const double* dv = last ? dot : nullptr;
       double* pv = last ? partial : nullptr;
-      if (first)
-        mlk::k_sweep<true><<<g, 256, 0, stream>>>(n, t.soff.p, t.scol.p, t.slen.p, t.sval.p,
-                                                  d_sw_rd.p, rhs, nullptr, out, dv, pv, S);
-      else
-        mlk::k_sweep<false><<<g, 256, 0, stream>>>(n, t.soff.p, t.scol.p, t.slen.p, t.sval.p,
-                                                   d_sw_rd.p, rhs, in, out, dv, pv, S);
+      const int sign = &t == &sw_lo ? -1 : 1;
+      auto go = [&](auto kernel) {
+        kernel<<<g, 256, 0, stream>>>(n, t.soff.p, t.scol.p, t.sdel.p, sign, t.slen.p, t.sval.p,
+                                      d_sw_rd.p, rhs, first ? nullptr : in, out, dv, pv, S);
+      };
+      if (t.d16) first ? go(mlk::k_sweep<true, true>) : go(mlk::k_sweep<false, true>);
+      else first ? go(mlk::k_sweep<true, false>) : go(mlk::k_sweep<false, false>);
     };
     // forward, ping-pong between a and b; y ends in a
     sweep(sw_lo, true, r, nullptr, a, false);
@@ -818,11 +901,14 @@ struct Operator {
             : occupancy_per_cu((const void*)mlk::k_trisolve<double, false, false>, mlk::TRI_THREADS);
     const int cus = device_cus();
     const int nblk = std::max(fw.nblk, bw.nblk);
-    const int top = std::max(1, (17 * occ) / 20);  // stay clear of the reported occupancy
+    const int top = tri_per_cu;  // the probed co-resident bound
     mlk::k_fill_f64<<<grid_for(n), 256, 0, stream>>>(w_r.p, n, 1.0);
     double best = 1e300;
-    int best_pc = std::max(1, (3 * occ) / 4);
-    for (int pc = 2; pc <= top; pc += 2) {
+    int best_pc = tri_per_cu;
+    std::vector<int> cands;  // even counts below the bound, and the bound itself
+    for (int pc = 2; pc < top; pc += 2) cands.push_back(pc);
+    cands.push_back(top);
+    for (int pc : cands) {
       (c32 ? full_tri32 : full_tri64) = std::min(nblk, pc * cus);
       shrinks = 0;
       set_grids();
@@ -2588,6 +2674,11 @@ ml_status ml_model_set_sweeps(ml_model* h, int sweeps) {
 }
 
 int ml_model_sweeps(const ml_model* h) { return h ? h->m.op.sweeps : 0; }
+
+int ml_model_sweep_column_bytes(const ml_model* h) {
+  if (!h || !h->m.op.have_sweep) return 0;
+  return h->m.op.sw_lo.d16 && h->m.op.sw_up.d16 ? 2 : 4;
+}
 
 ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* trace_fw,
                        int64_t* trace_bw, int64_t capacity) {

@@ -112,6 +112,8 @@ def _declare(L):
         "or_thermal_steady": (C.c_int, [C.c_int64, _D, C.c_int64, C.c_int, _I, _D, C.c_int64,
                                         _I, _I, _D, _D, _U8, _D, _D, C.c_int, _D, _D]),
         "or_system_pcg_parallel": (C.c_double, [_P, _D, C.c_int64, C.c_int, _D]),
+        "or_system_solve_parallel": (C.c_int, [_P, _D, _D, C.c_double, C.c_int64, C.c_int, _I,
+                                               _D]),
         "or_omp_threads": (C.c_int, []),
         "or_large_response": (None, [C.c_int, _D, C.c_int, C.c_double, C.c_double, _D, _D, _D,
                                      _D]),
@@ -450,6 +452,17 @@ class System:
         rr = np.zeros(1)
         t = lib().or_system_pcg_parallel(self._h, _d(_f64(f)), int(iters), int(threads), _d(rr))
         return float(t), float(rr[0])
+
+    def solve_parallel(self, f, tol=TOLERANCE, maxit=MAX_ITERATIONS, threads=0):
+        """(status, u over all DOFs, iterations, |r|/|b|) of the OpenMP PCG run to the
+        reference's stop test: a reference solution at sizes the serial restatement takes
+        minutes on (its sums run in other orders, so it agrees with solve() to rounding)."""
+        u = np.zeros(3 * len(self.nodes))
+        it = np.zeros(1, dtype=np.int64)
+        res = np.zeros(1)
+        st = lib().or_system_solve_parallel(self._h, _d(_f64(f)), _d(u), tol, maxit, int(threads),
+                                            _i(it), _d(res))
+        return st, u, int(it[0]), float(res[0])
 
     def times(self):
         t = np.zeros(4)

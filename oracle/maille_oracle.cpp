@@ -1985,7 +1985,8 @@ int or_solid_response(int nen, const double* x, double young, double poisson, do
 // parallel, each row summed as the reference sums it), dot products and vector updates parallel.
 // Runs `iters` PCG iterations on the right-hand side of f (no stop test) and returns the wall
 // time per iteration (and, in *rrel, |r|/|b| at the end); threads <= 0 uses the OpenMP default.
-double or_system_pcg_parallel(void* h, const double* f, i64 iters, int threads, double* rrel) {
+static double pcg_parallel_run(void* h, const double* f, i64 iters, int threads, double tol,
+                               double* x_out, i64* iters_done, double* rrel) {
   System* s = (System*)h;
   if (!s->factored || !s->iterative) return -1.0;
   const Csr& a = s->csr;
@@ -2054,7 +2055,11 @@ double or_system_pcg_parallel(void* h, const double* f, i64 iters, int threads, 
   for (i64 k = 0; k < n; ++k) b[k] = -s->coupling[k];
   for (i64 g = 0; g < s->ndof; ++g)
     if (s->free_of[g] >= 0) b[s->rank[s->free_of[g]]] += f[g];
-  double rz = 0.0, pap = 0.0, rz_next = 0.0;
+  double rz = 0.0, pap = 0.0, rz_next = 0.0, rr = 0.0, bb = 0.0;
+  for (i64 k = 0; k < n; ++k) bb += b[k] * b[k];
+  const double stop = tol > 0.0 ? tol * std::sqrt(bb) : -1.0;
+  i64 done = 0;
+  bool stopped = bb == 0.0 && tol > 0.0;  // b = 0: x = 0 (iterative.rs:211-216)
   const double t0 = now();
 #pragma omp parallel
   {
@@ -2087,11 +2092,12 @@ double or_system_pcg_parallel(void* h, const double* f, i64 iters, int threads, 
       pv[k] = z[k];
       rz += r[k] * z[k];
     }
-    for (i64 it = 0; it < iters; ++it) {
+    for (i64 it = 0; it < iters && !stopped; ++it) {
 #pragma omp single
       {
         pap = 0.0;
         rz_next = 0.0;
+        rr = 0.0;
       }
 #pragma omp for schedule(static) reduction(+ : pap)
       for (i64 i = 0; i < n; ++i) {
@@ -2101,11 +2107,18 @@ double or_system_pcg_parallel(void* h, const double* f, i64 iters, int threads, 
         pap += pv[i] * sum;
       }
       const double alpha = rz / pap;
-#pragma omp for schedule(static)
+#pragma omp for schedule(static) reduction(+ : rr)
       for (i64 k = 0; k < n; ++k) {
         x[k] += alpha * pv[k];
         r[k] -= alpha * ap[k];
+        rr += r[k] * r[k];
       }
+#pragma omp single
+      {
+        done = it + 1;
+        if (std::sqrt(rr) <= stop) stopped = true;  // iterative.rs:233-235
+      }
+      if (stopped) break;
       precondition();
 #pragma omp for schedule(static) reduction(+ : rz_next)
       for (i64 k = 0; k < n; ++k) rz_next += r[k] * z[k];
@@ -2116,13 +2129,38 @@ double or_system_pcg_parallel(void* h, const double* f, i64 iters, int threads, 
       rz = rz_next;
     }
   }
-  const double per_iter = (now() - t0) / (double)std::max<i64>(iters, 1);
+  const double per_iter = (now() - t0) / (double)std::max<i64>(done, 1);
   if (rrel) {  // |r| / |b| after the iterations, to check the variant converges as the serial one
-    double rr = 0.0, bb = 0.0;
-    for (i64 k = 0; k < n; ++k) { rr += r[k] * r[k]; bb += b[k] * b[k]; }
-    *rrel = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+    double r2 = 0.0;
+    for (i64 k = 0; k < n; ++k) r2 += r[k] * r[k];
+    *rrel = bb > 0.0 ? std::sqrt(r2 / bb) : 0.0;
   }
+  if (iters_done) *iters_done = done;
+  if (x_out) std::copy(x.begin(), x.end(), x_out);
   return per_iter;
+}
+
+double or_system_pcg_parallel(void* h, const double* f, i64 iters, int threads, double* rrel) {
+  return pcg_parallel_run(h, f, iters, threads, 0.0, nullptr, nullptr, rrel);
+}
+
+// The same OpenMP PCG run to the reference's stop test |r| <= tol |b| (iterative.rs:210-248)
+// and scattered to u over all DOFs with the imposed values (model.rs:1502-1516): a full-size
+// reference solution for the device (its dot products and sums over rows run in another
+// order than the serial restatement's, so it agrees with it to rounding, not bit for bit).
+int or_system_solve_parallel(void* h, const double* f, double* u, double tol, i64 maxit,
+                             int threads, i64* iters, double* rrel) {
+  System* s = (System*)h;
+  if (!s->factored || !s->iterative) return INVALID;
+  std::vector<double> x(s->count);
+  i64 done = 0;
+  double res = 0.0;
+  pcg_parallel_run(h, f, maxit, threads, tol, x.data(), &done, &res);
+  for (i64 g = 0; g < s->ndof; ++g)
+    u[g] = s->free_of[g] >= 0 ? x[s->rank[s->free_of[g]]] : s->imposed[g];
+  if (iters) *iters = done;
+  if (rrel) *rrel = res;
+  return res <= tol ? OK : NOT_CONVERGED;
 }
 
 // OpenMP threads the parallel baseline would use (1 without OpenMP)
