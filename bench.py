@@ -157,30 +157,44 @@ def run_box_batch(args, ma, meshes):
 
 
 def run_tet10(args, ma, meshes):
-    """The Tet10 cantilever (60x12x12 cells, ~225k free DOF): FP64 against the mixed modes."""
+    """The Tet10 cantilever (60x12x12 cells, 225,000 free DOF): every mode's step (assemble +
+    IC(0) + PCG + reactions), iterations and error against FP64 and against the oracle's FP64
+    skyline solution (the committed fixture, when present)."""
     p = meshes.cantilever_tet10()
+    sky_path = os.path.join(ROOT, "tests", "golden", "cantilever_tet10_60x12x12_skyline.npz")
+    u_sky = np.load(sky_path)["displacements"] if os.path.exists(sky_path) else None
     res = {}
     u64 = None
-    for name, code in (("fp64", ma.FP64), ("mixed_ic0", ma.MIXED_IC0), ("mixed", ma.MIXED)):
-        m = ma.Model(p, precision=code)
-        m.factor()
-        m.solve(reactions=False)
-        t0 = time.perf_counter()
-        m.assemble()
-        m.factor()
-        u, r, info = m.solve()
-        step = time.perf_counter() - t0
+    codes = (("fp64", ma.FP64), ("mixed_ic0", ma.MIXED_IC0), ("mixed_jacobi", ma.MIXED_JACOBI),
+             ("mixed", ma.MIXED))
+    for name, code in codes:
+        with ma.Model(p, precision=code) as m:
+            m.autotune()
+            m.factor()
+            m.solve(reactions=False)
+            steps = []
+            for _ in range(max(1, args.steps)):
+                t0 = time.perf_counter()
+                m.assemble()
+                m.factor()
+                u, r, info = m.solve()
+                steps.append(time.perf_counter() - t0)
+            step = min(steps)
+            sweeps = m.sweeps()
+            f = m.loads()
         if u64 is None:
             u64 = u
-        f = m.loads()
         balance = float(np.abs(r.reshape(-1, 3).sum(axis=0) + f.reshape(-1, 3).sum(axis=0)).max()
                         / np.abs(f.reshape(-1, 3).sum(axis=0)).max())
-        res[name] = {"s_per_step": round(step, 3), "iterations": info["iterations"],
-                     "true_residual": info["true_residual"], "escalated": info["escalated"],
+        res[name] = {"s_per_step": round(step, 4), "iterations": info["iterations"],
+                     "sweeps": sweeps, "true_residual": info["true_residual"],
+                     "escalated": info["escalated"], "device_retries": info["device_retries"],
                      "rel_err_vs_fp64": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
+                     "relative_gap_vs_skyline": (float(np.abs(u - u_sky).max() /
+                                                       np.abs(u_sky).max())
+                                                 if u_sky is not None else None),
                      "reaction_balance": balance}
         log(f"tet10 {name}: {res[name]}")
-        del m
     best = min((k for k in res if res[k]["rel_err_vs_fp64"] <= 1e-8), key=lambda k: res[k]["s_per_step"])
     emit({"metric": "Tet10 cantilever solve s (assemble + IC(0) + PCG), FP64 vs mixed",
           "value": res[best]["s_per_step"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
@@ -190,8 +204,12 @@ def run_tet10(args, ma, meshes):
 
 
 def run_simp(args, ma, meshes):
-    """Adjoint compliance gradient on the SIMP box (80x40x40 cells, ~403k free DOF): solve,
-    gradient, central differences on random elements; the CPU adjoint timed beside it."""
+    """Adjoint compliance gradient on the SIMP box (80x40x40 cells, 403,440 free DOF): FP64
+    solve, device gradient, the oracle's FP64 CPU adjoint timed beside it, and central
+    differences on random elements computed without cancellation: with K+- = K +- h k_e,
+    (C(E_e + h) - C(E_e - h)) / 2h = -u+.k_e u- exactly, from two perturbed solves (run in
+    the Jacobi mode)."""
+    import oracle
     p = meshes.simp_box()
     m = ma.Model(p)
     m.factor()
@@ -202,35 +220,45 @@ def run_simp(args, ma, meshes):
     g = m.compliance_gradient()
     t_grad = time.perf_counter() - t0
     rng = np.random.default_rng(6)
+    elems = rng.choice(len(p.conn), args.fd, replace=False)
+    probe = ma.Model(p, precision=ma.MIXED_JACOBI)
+    probe.autotune()
     worst = 0.0
-    for e in rng.choice(len(p.conn), args.fd, replace=False):
-        h = 1e-3 * p.young[e]
-        cs = []
+    t0 = time.perf_counter()
+    for e in elems:
+        h = 1e-4 * p.young[e]
+        sols = []
         for sgn in (1.0, -1.0):
             young = p.young.copy()
             young[e] += sgn * h
-            m.set_young(young)
-            m.assemble()
-            m.factor()
-            ue, _, _ = m.solve(reactions=False)
-            cs.append(float(p.f @ ue))
-        fd = (cs[0] - cs[1]) / (2 * h)
+            probe.set_young(young)
+            probe.assemble()
+            probe.factor()
+            ue, _, _ = probe.solve(reactions=False)
+            sols.append(ue)
+        dofs = (3 * p.conn[e][:, None] + np.arange(3)).ravel()
+        ke = oracle.solid_stiffness(p.nodes[p.conn[e]], 1.0, p.nu[e])
+        fd = -float(sols[0][dofs] @ ke @ sols[1][dofs])
         worst = max(worst, abs(fd - g[e]) / abs(g[e]))
+    t_fd = time.perf_counter() - t0
+    probe.close()
     cpu = None
     if not args.no_cpu:
-        import oracle
         t0 = time.perf_counter()
-        oracle.compliance_gradient(p.nodes, p.conn, p.young, p.nu, u)
+        g_cpu = oracle.compliance_gradient(p.nodes, p.conn, p.young, p.nu, u)
         cpu = {"value": round(time.perf_counter() - t0, 3), "unit": "s", "cores": 1,
                "kind": "port", "sample": "the oracle's FP64 CPU adjoint over all elements at the "
-                                         "device solution"}
+                                         "device solution",
+               "bit_identical": bool(np.array_equal(g, g_cpu))}
+    m.close()
     emit({"metric": "adjoint compliance gradient s, SIMP Tet4 box 80x40x40 (403,440 free DOF)",
           "value": round(t_grad, 4), "unit": "s", "n_gpus": 1, "higher_is_better": False,
           "data": "synthetic", "dtype": "fp64",
           "config": {"workload": "simp_box_80x40x40", "free_dof": p.nfree,
                      "elements": len(p.conn)},
           "solve_s": round(t_solve, 3), "iterations": info["iterations"],
-          "fd_elements": args.fd, "fd_max_rel_diff": worst, "cpu_baseline": cpu})
+          "fd_elements": args.fd, "fd_relative_step": 1e-4, "fd_max_rel_diff": worst,
+          "fd_s": round(t_fd, 2), "cpu_baseline": cpu})
 
 
 def run_stresses(args, ma, meshes):
@@ -516,6 +544,8 @@ def main():
                          "against which HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled")
     ap.add_argument("--cpu-iters", type=int, default=0,
                     help="PCG iterations of the CPU sample (0: a tenth of the FP64 count)")
+    ap.add_argument("--sweeps", type=int, default=None,
+                    help="Jacobi sweeps of the mixed modes (default: the autotuner's choice)")
     ap.add_argument("--compare-steps", type=int, default=2,
                     help="steps timed per mode to pick the headline mode")
     ap.add_argument("--workload", choices=["cube", "beam_cases", "box_batch", "tet10", "simp",
@@ -558,12 +588,15 @@ def main():
         t1 = time.perf_counter()
         models[prec] = ma.Model(problem, precision=codes[prec])
         t_create = time.perf_counter() - t1
+        if args.sweeps is not None and prec != "fp64":
+            models[prec].set_sweeps(args.sweeps)
         chosen = models[prec].autotune()
         create[prec] = {"create_s": round(t_create, 3),
                         "host_pattern_s": round(models[prec].create_info["seconds"][5], 3),
                         "autotune_s": round(time.perf_counter() - t1 - t_create, 3),
                         "waves_per_cu": chosen[0], "reported_occupancy": chosen[2],
-                        "sweeps": models[prec].sweeps()}
+                        "sweeps": models[prec].sweeps(),
+                        "residency_reported_probed": models[prec].residency()}
         log(f"{prec}: model created in {t_create:.1f} s (host pattern "
             f"{create[prec]['host_pattern_s']:.1f} s), autotuned in "
             f"{create[prec]['autotune_s']:.1f} s: {chosen[0]} waves per CU of {chosen[2]}, "

@@ -189,8 +189,9 @@ ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* tr
 ml_status ml_pcg_set_wait_limit(ml_pcg* h, int64_t ticks);
 /* No counterpart in the reference: the Jacobi sweeps per triangular system with which a mixed
  * handle applies its FP32 factor (0..64; 0: the exact triangular solves). An FP64 handle
- * takes 0 only (ML_INVALID_ARGUMENT otherwise). Batched solves (nrhs > 1, ml_batch) always
- * use the exact solves. ml_pcg_sweeps returns the current count. */
+ * takes 0 only (ML_INVALID_ARGUMENT otherwise). A count set here is kept by the autotuner.
+ * Batched solves (nrhs > 1, ml_batch) always use the exact solves. ml_pcg_sweeps returns the
+ * current count. */
 ml_status ml_pcg_set_sweeps(ml_pcg* h, int sweeps);
 int ml_pcg_sweeps(const ml_pcg* h);
 
@@ -218,6 +219,11 @@ int ml_model_sweeps(const ml_model* m);
 /* Bytes per column index the sweeps read (2: 16-bit distances from the row, when every
  * |i - j| of the factor fits; 4: int32), 0 before any sweep layout is built. */
 int ml_model_sweep_column_bytes(const ml_model* m);
+/* Diagnostics (no counterpart in the reference): out[6] = workgroups per CU that the runtime's
+ * occupancy figure reports and that a co-residency probe of the kernel itself found the device
+ * to hold at once, for the forward triangular solve, the backward one and IC(0); the dataflow
+ * grids use the probed counts. Zeros for a model without free DOFs. */
+ml_status ml_model_residency(const ml_model* m, int32_t* out);
 /* Recompute the element stiffnesses and K_ff values on the device (after ml_model_set_young). */
 ml_status ml_model_assemble(ml_model* m);
 /* Replace the per-element Young's moduli (n_elements values, host) without touching the

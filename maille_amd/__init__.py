@@ -43,7 +43,7 @@ EXPORTS = (
     "ml_pcg_create", "ml_pcg_solve", "ml_pcg_solve_device", "ml_pcg_multiply",
     "ml_pcg_precondition", "ml_pcg_factor", "ml_pcg_destroy", "ml_pcg_trace",
     "ml_pcg_set_wait_limit", "ml_pcg_set_sweeps", "ml_pcg_sweeps", "ml_model_set_sweeps",
-    "ml_model_sweeps", "ml_model_sweep_column_bytes",
+    "ml_model_sweeps", "ml_model_sweep_column_bytes", "ml_model_residency",
     "ml_model_create", "ml_model_assemble", "ml_model_set_young", "ml_model_factor",
     "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_stresses_device",
     "ml_model_nodal_stresses",
@@ -191,6 +191,7 @@ def lib():
             "ml_model_set_sweeps": (_int, [_P, _int]),
             "ml_model_sweeps": (_int, [_P]),
             "ml_model_sweep_column_bytes": (_int, [_P]),
+            "ml_model_residency": (_int, [_P, _I32]),
             "ml_model_create": (_int, [_i64, _D, _i64, _int, _I, _D, _D, _U8, _D, _int, _int,
                                        _PP, _S]),
             "ml_model_assemble": (_int, [_P]),
@@ -617,6 +618,13 @@ class Model(_Handle):
 
     def sweeps(self) -> int:
         return lib().ml_model_sweeps(self._h)
+
+    def residency(self):
+        """{kernel: (reported, probed)} workgroups per CU of the dataflow kernels."""
+        out = (C.c_int32 * 6)()
+        _check(lib().ml_model_residency(self._h, out))
+        return {k: (out[2 * i], out[2 * i + 1])
+                for i, k in enumerate(("forward_solve", "backward_solve", "ic0"))}
 
     def sweep_column_bytes(self) -> int:
         """Bytes per column index of the sweep layouts (2 or 4; 0 before they exist)."""

@@ -102,6 +102,18 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Logical block of this workgroup for row-ordered kernels whose rows gather their neighbours
+// (SpMV, sweeps): workgroup b runs on XCD b mod 8, so XCD x is given the contiguous range of
+// logical blocks [x q + min(x, r), ...) (q, r: the quotient and remainder of the grid by 8),
+// in order. Each XCD then sweeps one eighth of the rows, and the vector entries its rows
+// gather stay within that eighth plus the matrix band, which its own L2 can hold, instead of
+// every XCD's L2 seeing the whole vector. A bijection of [0, gridDim.x).
+__device__ inline int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 // Deterministic block sum (fixed tree), result valid in thread 0.
 template <int BS>
 __device__ inline double block_sum(double v) {
@@ -1221,7 +1233,9 @@ __global__ __launch_bounds__(256) void k_nodal_stresses(int64_t n_nodes,
 // y = A x (Csr::multiply, iterative.rs:131-142). Lane j of a slice sums row j in the order
 // diagonal, lower entries, upper entries: the reference's accumulation order, so the FP64
 // result is bit-identical. Optionally accumulates x.y per block (the PCG curvature p.Ap).
-template <typename T>
+// Entries are taken BATCH at a time, so that their loads are in flight together; rows are
+// mapped to workgroups XCD by XCD (xcd_block) and partial[] is indexed by logical block.
+template <typename T, int BATCH = 8>
 __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ soff,
                                               const int* __restrict__ scol,
                                               const int* __restrict__ slen,
@@ -1232,27 +1246,26 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
                                               int only_on_update = 0) {
 #pragma clang fp contract(off)
   if (S && (S->done || (only_on_update && !S->update))) return;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = xcd_block();
+  const int j = lb * blockDim.x + threadIdx.x;
   double dotv = 0.0;
   if (j < n) {
     const int base = soff[j >> 6] + (j & 63);
     const int len = slen[j];
     T s = sval[base] * x[scol[base]];
-    // eight entries at a time: their columns and values are loaded together, then the eight
-    // gathers, so that a wave keeps many loads in flight; the sum stays in entry order
     int k = 1;
-    for (; k + 8 <= len; k += 8) {
-      int c[8];
-      T v[8], xv[8];
+    for (; k + BATCH <= len; k += BATCH) {
+      int c[BATCH];
+      T v[BATCH], xv[BATCH];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < BATCH; ++u) {
         c[u] = scol[base + ((k + u) << 6)];
         v[u] = sval[base + ((k + u) << 6)];
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) xv[u] = x[c[u]];
+      for (int u = 0; u < BATCH; ++u) xv[u] = x[c[u]];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) s += v[u] * xv[u];
+      for (int u = 0; u < BATCH; ++u) s += v[u] * xv[u];
     }
     for (; k < len; ++k) {
       const int at = base + (k << 6);
@@ -1263,7 +1276,7 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
   }
   if (partial) {
     double b = block_sum<256>(dotv);
-    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    if (threadIdx.x == 0) partial[lb] = b;
   }
 }
 
@@ -1315,7 +1328,7 @@ __global__ __launch_bounds__(256) void k_spmv_rep(int n, int nrep, const int* __
 // D16: columns are read as 16-bit distances from the row, j = i + sign * d (sign -1 for the
 // lower triangle, +1 for the upper one), else as int32 indices. With `partial`, the block's
 // share of dot.out (the PCG's r.z) is written, in a fixed tree.
-template <bool FIRST, bool D16>
+template <bool FIRST, bool D16, int BATCH = 8>
 __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ soff,
                                                const int* __restrict__ scol,
                                                const uint16_t* __restrict__ sdel, int sign,
@@ -1330,7 +1343,8 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
                                                const PcgScalars* __restrict__ S) {
 #pragma clang fp contract(off)
   if (S && S->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = xcd_block();
+  const int i = lb * blockDim.x + threadIdx.x;
   double dv = 0.0;
   if (i < n) {
     const int base = soff[i >> 6] + (i & 63);
@@ -1338,21 +1352,21 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
     auto column = [&](int at) { return D16 ? i + sign * (int)sdel[at] : scol[at]; };
     double s = 0.0;
     int k = 0;
-    // eight entries at a time: columns and coefficients, then the eight gathers, so that a
-    // wave keeps many loads in flight; the sum stays in entry order
-    for (; k + 8 <= len; k += 8) {
-      int c[8];
-      float v[8];
-      double x[8];
+    // BATCH entries at a time: columns and coefficients, then the gathers, so that a wave
+    // keeps many loads in flight; the sum stays in entry order
+    for (; k + BATCH <= len; k += BATCH) {
+      int c[BATCH];
+      float v[BATCH];
+      double x[BATCH];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < BATCH; ++u) {
         c[u] = column(base + ((k + u) << 6));
         v[u] = sval[base + ((k + u) << 6)];
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) x[u] = FIRST ? rhs[c[u]] * rd[c[u]] : in[c[u]];
+      for (int u = 0; u < BATCH; ++u) x[u] = FIRST ? rhs[c[u]] * rd[c[u]] : in[c[u]];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) s = s + (double)v[u] * x[u];
+      for (int u = 0; u < BATCH; ++u) s = s + (double)v[u] * x[u];
     }
     for (; k < len; ++k) {
       const int at = base + (k << 6);
@@ -1365,7 +1379,7 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
   }
   if (partial) {
     const double b = block_sum<256>(dv);
-    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    if (threadIdx.x == 0) partial[lb] = b;
   }
 }
 

@@ -221,22 +221,15 @@ int ic0_resident(void (*kernel)(TArgs...), hipStream_t stream) {
   });
 }
 
-// Launches a persistent dataflow kernel of `threads`-thread workgroups cooperatively, so that
-// the runtime admits the grid only if it can hold all of it at once; `grid` is lowered (and
-// remembered) while the device refuses it as too large.
+// Launches a persistent dataflow kernel of `threads`-thread workgroups on a grid whose
+// co-residency a probe of the same kernel has shown (resident_per_cu). An ordinary launch: the
+// runtime's admission check of a cooperative one rests on the occupancy figure the probe
+// replaces.
 template <typename... Args>
 void launch_dataflow(void (*kernel)(Args...), int& grid, int threads, hipStream_t stream,
                      Args... args) {
-  void* params[] = {(void*)&args...};
-  for (;;) {
-    hipError_t e = hipLaunchCooperativeKernel((const void*)kernel, dim3(grid), dim3(threads),
-                                              params, 0, stream);
-    if (e == hipSuccess) return;
-    (void)hipGetLastError();
-    if (e != hipErrorCooperativeLaunchTooLarge || grid <= 1)
-      throw HipError(std::string("cooperative launch: ") + hipGetErrorString(e));
-    grid = std::max(1, grid / 2);
-  }
+  kernel<<<grid, threads, 0, stream>>>(args...);
+  HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace
@@ -341,10 +334,14 @@ struct Operator {
   DBuf<double> d_sw_rd, w_s1, w_s2;
   bool have_sweep = false;
   int sweeps = 0;  // Jacobi sweeps per triangular system; 0: the exact dataflow solves
+  int sweep_batch = 8, spmv_batch = 8;  // entries per batch of loads (autotuned: 8 or 16)
+  bool sweeps_pinned = false;  // set by the caller: the autotuner keeps the count
   // grids of the dataflow kernels: the full ones, and the ones in use (full >> shrinks)
   int full_tri64 = 0, full_tri32 = 0, full_ic = 0, full_ic64 = 0;
   int cap_tri64 = 0, cap_tri32 = 0;  // dataflow grids before the cap by block count
   int tri_per_cu = 1, ic_per_cu = 1, ic64_per_cu = 1;  // probed co-resident workgroups per CU
+  // reported and probed workgroups per CU: forward solve, backward solve, IC(0) (512-entry rows)
+  int residency[6] = {0, 0, 0, 0, 0, 0};
   int rhs_per_pass = 256;            // right-hand sides per batched pass (autotuned)
   int tri_grid64 = 0, tri_grid32 = 0, ic_grid = 0, ic_grid64 = 0;
   int shrinks = 0;   // grid halvings after a stalled dataflow launch
@@ -398,16 +395,26 @@ struct Operator {
     // every dataflow grid is one whose co-residency was shown by a probe of the same kernel;
     // the forward and backward solves share one grid, so it fits the tighter of the two
     const int cus = device_cus();
+    auto note = [&](int slot, const void* kernel, int threads, int probed) {
+      residency[2 * slot] = occupancy_per_cu(kernel, threads);
+      residency[2 * slot + 1] = probed;
+      return probed;
+    };
     if (precision == ML_FP64) {
-      tri_per_cu = std::min(trisolve_resident(mlk::k_trisolve<double, false, false>, stream),
-                            trisolve_resident(mlk::k_trisolve<double, true, true>, stream));
-      ic_per_cu = ic0_resident(mlk::k_ic0<double, 512>, stream);
+      auto fw_k = mlk::k_trisolve<double, false, false>;
+      auto bw_k = mlk::k_trisolve<double, true, true>;
+      tri_per_cu = std::min(note(0, (const void*)fw_k, mlk::TRI_THREADS, trisolve_resident(fw_k, stream)),
+                            note(1, (const void*)bw_k, mlk::TRI_THREADS, trisolve_resident(bw_k, stream)));
+      ic_per_cu = note(2, (const void*)mlk::k_ic0<double, 512>, 64,
+                       ic0_resident(mlk::k_ic0<double, 512>, stream));
       ic64_per_cu = ic0_resident(mlk::k_ic0<double, 64>, stream);
     } else {
-      tri_per_cu =
-          std::min(trisolve_resident(mlk::k_trisolve<double, false, false, false, float>, stream),
-                   trisolve_resident(mlk::k_trisolve<double, true, true, false, float>, stream));
-      ic_per_cu = ic0_resident(mlk::k_ic0<float, 512>, stream);
+      auto fw_k = mlk::k_trisolve<double, false, false, false, float>;
+      auto bw_k = mlk::k_trisolve<double, true, true, false, float>;
+      tri_per_cu = std::min(note(0, (const void*)fw_k, mlk::TRI_THREADS, trisolve_resident(fw_k, stream)),
+                            note(1, (const void*)bw_k, mlk::TRI_THREADS, trisolve_resident(bw_k, stream)));
+      ic_per_cu = note(2, (const void*)mlk::k_ic0<float, 512>, 64,
+                       ic0_resident(mlk::k_ic0<float, 512>, stream));
       ic64_per_cu = ic0_resident(mlk::k_ic0<float, 64>, stream);
     }
     cap_tri64 = cap_tri32 = tri_per_cu * cus;
@@ -623,9 +630,16 @@ struct Operator {
     int& grid1 = c32 ? tri_grid32 : tri_grid64;
     int grid_rep = std::min<int64_t>((int64_t)std::max(fw.nblk, bw.nblk) * nrep,
                                      std::max(1, (c32 ? cap_tri32 : cap_tri64) >> shrinks));
-    // the traced kernels have their own footprint: a launch of theirs that the runtime refuses
-    // lowers a copy of the grid, not the handle's
+    // the traced kernels have their own footprint, so their grid is bounded by their own probe
     int grid_trace = grid1;
+    if (trace_fw || trace_bw) {
+      const int per_cu =
+          c32 ? std::min(trisolve_resident(mlk::k_trisolve<double, false, false, true, float>, stream),
+                         trisolve_resident(mlk::k_trisolve<double, true, true, true, float>, stream))
+              : std::min(trisolve_resident(mlk::k_trisolve<double, false, false, true>, stream),
+                         trisolve_resident(mlk::k_trisolve<double, true, true, true>, stream));
+      grid_trace = std::min(grid1, per_cu * device_cus());
+    }
     int& grid = nrep > 1 ? grid_rep : ((trace_fw || trace_bw) ? grid_trace : grid1);
     hipEvent_t ev = prof.begin(stream);
     int nn = n;
@@ -688,8 +702,13 @@ struct Operator {
         kernel<<<g, 256, 0, stream>>>(n, t.soff.p, t.scol.p, t.sdel.p, sign, t.slen.p, t.sval.p,
                                       d_sw_rd.p, rhs, first ? nullptr : in, out, dv, pv, S);
       };
-      if (t.d16) first ? go(mlk::k_sweep<true, true>) : go(mlk::k_sweep<false, true>);
-      else first ? go(mlk::k_sweep<true, false>) : go(mlk::k_sweep<false, false>);
+      if (sweep_batch == 16) {
+        if (t.d16) first ? go(mlk::k_sweep<true, true, 16>) : go(mlk::k_sweep<false, true, 16>);
+        else first ? go(mlk::k_sweep<true, false, 16>) : go(mlk::k_sweep<false, false, 16>);
+      } else {
+        if (t.d16) first ? go(mlk::k_sweep<true, true>) : go(mlk::k_sweep<false, true>);
+        else first ? go(mlk::k_sweep<true, false>) : go(mlk::k_sweep<false, false>);
+      }
     };
     // forward, ping-pong between a and b; y ends in a
     sweep(sw_lo, true, r, nullptr, a, false);
@@ -753,7 +772,8 @@ struct Operator {
   void spmv(const TA* x, TA* y, double* partial, const mlk::PcgScalars* S, int gated = 0) {
     const bool f32 = std::is_same<TA, float>::value;
     hipEvent_t ev = prof.begin(stream);
-    mlk::k_spmv<TA><<<grid_for(n), 256, 0, stream>>>(
+    auto kernel = spmv_batch == 16 ? mlk::k_spmv<TA, 16> : mlk::k_spmv<TA, 8>;
+    kernel<<<grid_for(n), 256, 0, stream>>>(
         n, d_soff.p, d_scol.p, d_slen.p, f32 ? (const TA*)d_sval32.p : (const TA*)d_sval.p, x, y,
         partial, S, gated);
     HIP_CHECK(hipGetLastError());
@@ -885,10 +905,11 @@ struct Operator {
   // Picks, by timing, the waves per CU of the triangular solves (single right-hand side) and,
   // for nrhs > 1, the right-hand sides per pass. Results do not change: the solves are
   // bit-exact under any schedule. chosen[3]: waves per CU, right-hand sides per pass, and the
-  // per-CU occupancy the runtime reports. An ML_MIXED_JACOBI handle then times whole solves of
-  // a fixed pseudo-random right-hand side with 0 (the exact solves), 2, 3, 4, 6 and 8 sweeps
-  // and keeps the fastest that converges (which changes its iterates, within the solve's
-  // tolerance).
+  // per-CU occupancy the runtime reports. The entries per batch of loads of the SpMV and the
+  // sweeps are timed too (bit-exact either way). An ML_MIXED_JACOBI handle whose sweep count
+  // the caller has not set then times whole solves of a fixed pseudo-random right-hand side
+  // with 0 (the exact solves), 2, 3, 4, 6 and 8 sweeps and keeps the fastest that converges
+  // (which changes its iterates, within the solve's tolerance).
   int autotune(int64_t nrhs, int32_t* chosen) {
     if (!factored) {
       int st = factor();
@@ -903,6 +924,47 @@ struct Operator {
     const int nblk = std::max(fw.nblk, bw.nblk);
     const int top = tri_per_cu;  // the probed co-resident bound
     mlk::k_fill_f64<<<grid_for(n), 256, 0, stream>>>(w_r.p, n, 1.0);
+    // entries per batch of loads of the SpMV and of the sweeps: the faster of 8 and 16
+    {
+      hipEvent_t a, b;
+      HIP_CHECK(hipEventCreate(&a));
+      HIP_CHECK(hipEventCreate(&b));
+      auto time_of = [&](auto&& work) {
+        float total = 0.0f;
+        for (int k = 0; k <= 3; ++k) {  // the first run warms up
+          HIP_CHECK(hipEventRecord(a, stream));
+          work();
+          HIP_CHECK(hipEventRecord(b, stream));
+          HIP_CHECK(hipEventSynchronize(b));
+          float t = 0.0f;
+          HIP_CHECK(hipEventElapsedTime(&t, a, b));
+          if (k > 0) total += t;
+        }
+        return total;
+      };
+      float best_t = 1e30f;
+      int best_b = 8;
+      for (int bsz : {8, 16}) {
+        spmv_batch = bsz;
+        const float t = time_of([&] { spmv<double>(w_r.p, w_ap.p, nullptr, nullptr); });
+        if (t < best_t) { best_t = t; best_b = bsz; }
+      }
+      spmv_batch = best_b;
+      if (have_sweep) {
+        const int k0 = sweeps;
+        sweeps = std::max(1, sweeps);
+        best_t = 1e30f;
+        for (int bsz : {8, 16}) {
+          sweep_batch = bsz;
+          const float t = time_of([&] { apply_sweeps(w_r.p, w_z.p, nullptr); });
+          if (t < best_t) { best_t = t; best_b = bsz; }
+        }
+        sweep_batch = best_b;
+        sweeps = k0;
+      }
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
     double best = 1e300;
     int best_pc = tri_per_cu;
     std::vector<int> cands;  // even counts below the bound, and the bound itself
@@ -941,7 +1003,7 @@ struct Operator {
       }
       rhs_per_pass = best_k;
     }
-    if (precision == ML_MIXED_JACOBI) {
+    if (precision == ML_MIXED_JACOBI && !sweeps_pinned) {
       DBuf<double> tb, tx;
       tb.alloc(n);
       tx.alloc(n);
@@ -2655,6 +2717,7 @@ ml_status ml_pcg_set_sweeps(ml_pcg* h, int sweeps) {
   return guarded([&]() -> int {
     if (!h) throw ArgError("null handle");
     h->op.set_sweeps(sweeps);
+    h->op.sweeps_pinned = true;
     return ML_OK;
   });
 }
@@ -2669,11 +2732,20 @@ ml_status ml_model_set_sweeps(ml_model* h, int sweeps) {
       return ML_OK;  // no free system
     }
     h->m.op.set_sweeps(sweeps);
+    h->m.op.sweeps_pinned = true;
     return ML_OK;
   });
 }
 
 int ml_model_sweeps(const ml_model* h) { return h ? h->m.op.sweeps : 0; }
+
+ml_status ml_model_residency(const ml_model* h, int32_t* out) {
+  return guarded([&]() -> int {
+    if (!h || !out) throw ArgError("null argument");
+    for (int k = 0; k < 6; ++k) out[k] = h->m.op.residency[k];
+    return ML_OK;
+  });
+}
 
 int ml_model_sweep_column_bytes(const ml_model* h) {
   if (!h || !h->m.op.have_sweep) return 0;

@@ -112,9 +112,19 @@ def test_256_load_cases_of_the_tet10_beam(gpu):
 
 
 # ------------------------------------------------------------------ config 4b
+@pytest.fixture(scope="module")
+def boxes():
+    bs = M.box_batch()
+    t = time.time()
+    refs = [oracle.model_solve(b, iterative=True) for b in bs]
+    assert all(r["status"] == oracle.OK for r in refs)
+    say(f"64 boxes: oracle PCG solves in {time.time() - t:.1f} s")
+    return bs, refs
+
+
 @pytest.mark.parametrize("precision,bound", [("FP64", 1e-10), ("MIXED_IC0", 1e-8)])
-def test_64_uneven_boxes_in_one_batch(gpu, precision, bound):
-    boxes = M.box_batch()
+def test_64_uneven_boxes_in_one_batch(gpu, boxes, precision, bound):
+    boxes, refs = boxes
     sizes = [b.ndof for b in boxes]
     assert len(boxes) == 64 and min(sizes) >= 400 and max(sizes) <= 24000
     models = [gpu.Model(b) for b in boxes]
@@ -125,9 +135,7 @@ def test_64_uneven_boxes_in_one_batch(gpu, precision, bound):
             say(f"{precision}: 64 boxes ({sum(b.nfree for b in boxes)} free DOF) in "
                 f"{time.time() - t:.2f} s")
         worst = 0.0
-        for b, u, info in zip(boxes, us, infos):
-            ref = oracle.model_solve(b, iterative=True)
-            assert ref["status"] == oracle.OK
+        for b, ref, u, info in zip(boxes, refs, us, infos):
             worst = max(worst, rel_l2(u, ref["displacements"]))
             assert info["device_retries"] == 0
         say(f"{precision}: worst rel L2 vs the oracle's PCG {worst:.2e}")

@@ -496,3 +496,18 @@ def test_a_model_with_every_dof_imposed(gpu):
     assert np.array_equal(u, u0) and info["iterations"] == 0
     ref = oracle.reactions(q.nodes, q.conn, q.young, q.nu, q.fixed, q.f, u0)
     assert np.array_equal(r, ref)
+
+
+def test_dataflow_grids_are_bounded_by_a_co_residency_probe(gpu):
+    """Each dataflow kernel's grid comes from a probe of the kernel itself (every workgroup of
+    the probe resident at once), never above the runtime's occupancy figure; solves on those
+    grids need no retry."""
+    p = PROBLEMS["jittered_cube_8"]
+    for precision in ("FP64", "MIXED_IC0"):
+        with gpu.Model(p, precision=getattr(gpu, precision)) as m:
+            res = m.residency()
+            print(f"{precision}: reported/probed workgroups per CU {res}", flush=True)
+            for kernel, (reported, probed) in res.items():
+                assert 1 <= probed <= reported, kernel
+            u, _, info = m.solve()
+            assert info["device_retries"] == 0
