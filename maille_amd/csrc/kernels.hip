@@ -102,18 +102,6 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Logical block of this workgroup for row-ordered kernels whose rows gather their neighbours
-// (SpMV, sweeps): workgroup b runs on XCD b mod 8, so XCD x is given the contiguous range of
-// logical blocks [x q + min(x, r), ...) (q, r: the quotient and remainder of the grid by 8),
-// in order. Each XCD then sweeps one eighth of the rows, and the vector entries its rows
-// gather stay within that eighth plus the matrix band, which its own L2 can hold, instead of
-// every XCD's L2 seeing the whole vector. A bijection of [0, gridDim.x).
-__device__ inline int xcd_block() {
-  const int nb = gridDim.x, b = blockIdx.x;
-  const int q = nb >> 3, r = nb & 7, x = b & 7;
-  return x * q + min(x, r) + (b >> 3);
-}
-
 // Deterministic block sum (fixed tree), result valid in thread 0.
 template <int BS>
 __device__ inline double block_sum(double v) {
@@ -1233,8 +1221,7 @@ __global__ __launch_bounds__(256) void k_nodal_stresses(int64_t n_nodes,
 // y = A x (Csr::multiply, iterative.rs:131-142). Lane j of a slice sums row j in the order
 // diagonal, lower entries, upper entries: the reference's accumulation order, so the FP64
 // result is bit-identical. Optionally accumulates x.y per block (the PCG curvature p.Ap).
-// Entries are taken BATCH at a time, so that their loads are in flight together; rows are
-// mapped to workgroups XCD by XCD (xcd_block) and partial[] is indexed by logical block.
+// Entries are taken BATCH at a time, so that their loads are in flight together.
 template <typename T, int BATCH = 8>
 __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ soff,
                                               const int* __restrict__ scol,
@@ -1246,8 +1233,7 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
                                               int only_on_update = 0) {
 #pragma clang fp contract(off)
   if (S && (S->done || (only_on_update && !S->update))) return;
-  const int lb = xcd_block();
-  const int j = lb * blockDim.x + threadIdx.x;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
   double dotv = 0.0;
   if (j < n) {
     const int base = soff[j >> 6] + (j & 63);
@@ -1276,7 +1262,7 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
   }
   if (partial) {
     double b = block_sum<256>(dotv);
-    if (threadIdx.x == 0) partial[lb] = b;
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
   }
 }
 
@@ -1326,8 +1312,9 @@ __global__ __launch_bounds__(256) void k_spmv_rep(int n, int nrep, const int* __
 // restatement reproduces it bit for bit). FIRST: the previous iterate is in = rhs * rd, formed
 // as it is read. Lane i of a 64-row slice owns row i; entries are column-major in the slice.
 // D16: columns are read as 16-bit distances from the row, j = i + sign * d (sign -1 for the
-// lower triangle, +1 for the upper one), else as int32 indices. With `partial`, the block's
-// share of dot.out (the PCG's r.z) is written, in a fixed tree.
+// lower triangle, +1 for the upper one), else as int32 indices. With out_rd, out * rd is also
+// written (the next system's starting iterate, the same product FIRST would form). With
+// `partial`, the block's share of dot.out (the PCG's r.z) is written, in a fixed tree.
 template <bool FIRST, bool D16, int BATCH = 8>
 __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ soff,
                                                const int* __restrict__ scol,
@@ -1338,13 +1325,13 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
                                                const double* __restrict__ rhs,
                                                const double* __restrict__ in,
                                                double* __restrict__ out,
+                                               double* __restrict__ out_rd,
                                                const double* __restrict__ dot,
                                                double* __restrict__ partial,
                                                const PcgScalars* __restrict__ S) {
 #pragma clang fp contract(off)
   if (S && S->done) return;
-  const int lb = xcd_block();
-  const int i = lb * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double dv = 0.0;
   if (i < n) {
     const int base = soff[i >> 6] + (i & 63);
@@ -1375,11 +1362,12 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
     }
     const double y = (rhs[i] - s) * rd[i];
     out[i] = y;
+    if (out_rd) out_rd[i] = y * rd[i];
     if (dot) dv = dot[i] * y;
   }
   if (partial) {
     const double b = block_sum<256>(dv);
-    if (threadIdx.x == 0) partial[lb] = b;
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
   }
 }
 
@@ -1393,15 +1381,25 @@ __global__ __launch_bounds__(256) void k_dot(int64_t n, const A* __restrict__ a,
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// Sum of `count` partials in a fixed order, by one block of 256 threads.
+// Sum of `count` partials in a fixed order, by one block of SUM_THREADS threads: each thread
+// adds its strided share four loads at a time (four running sums, then combined), then a fixed
+// tree over the block.
+constexpr int SUM_THREADS = 1024;
 __device__ inline double sum_partials(const double* __restrict__ partial, int count) {
-  double v = 0.0;
-  for (int k = threadIdx.x; k < count; k += 256) v += partial[k];
-  return block_sum<256>(v);
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int k = threadIdx.x;
+  for (; k + 3 * SUM_THREADS < count; k += 4 * SUM_THREADS) {
+    v0 += partial[k];
+    v1 += partial[k + SUM_THREADS];
+    v2 += partial[k + 2 * SUM_THREADS];
+    v3 += partial[k + 3 * SUM_THREADS];
+  }
+  for (; k < count; k += SUM_THREADS) v0 += partial[k];
+  return block_sum<SUM_THREADS>((v0 + v1) + (v2 + v3));
 }
 
 // r.r of b for the initial norm; b == 0 => done with x = 0 (iterative.rs:211-216)
-__global__ __launch_bounds__(256) void k_init_norm(const double* __restrict__ partial, int count,
+__global__ __launch_bounds__(SUM_THREADS) void k_init_norm(const double* __restrict__ partial, int count,
                                                    PcgScalars* S, double tol) {
   double s = sum_partials(partial, count);
   if (threadIdx.x == 0) {
@@ -1417,14 +1415,14 @@ __global__ __launch_bounds__(256) void k_init_norm(const double* __restrict__ pa
   }
 }
 
-__global__ __launch_bounds__(256) void k_init_rz(const double* __restrict__ partial, int count,
+__global__ __launch_bounds__(SUM_THREADS) void k_init_rz(const double* __restrict__ partial, int count,
                                                  PcgScalars* S) {
   double s = sum_partials(partial, count);
   if (threadIdx.x == 0 && !S->done) S->rz = s;
 }
 
 // curvature p.Ap -> alpha, or Singular (iterative.rs:224-228)
-__global__ __launch_bounds__(256) void k_alpha(const double* __restrict__ partial, int count,
+__global__ __launch_bounds__(SUM_THREADS) void k_alpha(const double* __restrict__ partial, int count,
                                                PcgScalars* S) {
   if (S->done) return;
   double curvature = sum_partials(partial, count);
@@ -1440,7 +1438,7 @@ __global__ __launch_bounds__(256) void k_alpha(const double* __restrict__ partia
 }
 
 // |r| after the update; converged (iterative.rs:233-235) or out of iterations (:244-247)
-__global__ __launch_bounds__(256) void k_converged(const double* __restrict__ partial, int count,
+__global__ __launch_bounds__(SUM_THREADS) void k_converged(const double* __restrict__ partial, int count,
                                                    PcgScalars* S) {
   if (S->done) return;
   double rr = sum_partials(partial, count);
@@ -1461,7 +1459,7 @@ __global__ __launch_bounds__(256) void k_converged(const double* __restrict__ pa
 
 // After a residual replacement: the true |r| becomes the recursive one; replacements stop
 // once one fails to halve the previous true residual (the FP64 floor is reached).
-__global__ __launch_bounds__(256) void k_replaced(const double* __restrict__ partial, int count,
+__global__ __launch_bounds__(SUM_THREADS) void k_replaced(const double* __restrict__ partial, int count,
                                                   PcgScalars* S) {
   if (S->done || !S->update) return;
   double rr = sum_partials(partial, count);
@@ -1518,7 +1516,7 @@ __global__ __launch_bounds__(256) void k_replace_residual(int n, const PcgScalar
 }
 
 // beta = rz' / rz (iterative.rs:237-239)
-__global__ __launch_bounds__(256) void k_beta(const double* __restrict__ partial, int count,
+__global__ __launch_bounds__(SUM_THREADS) void k_beta(const double* __restrict__ partial, int count,
                                               PcgScalars* S) {
   if (S->done) return;
   double rz_next = sum_partials(partial, count);
@@ -1529,7 +1527,8 @@ __global__ __launch_bounds__(256) void k_beta(const double* __restrict__ partial
 }
 
 // x += alpha p; r -= alpha Ap (iterative.rs:229-232); partial r.r; fills the two triangular
-// solve outputs (when given) with the sentinel for the coming preconditioner application.
+// solve outputs (when given) with the sentinel for the coming preconditioner application, or
+// writes r * rd, the starting iterate of the coming Jacobi sweeps.
 template <typename TV, typename TM>
 __global__ __launch_bounds__(256) void k_update_xr(int n, const PcgScalars* __restrict__ S,
                                                    double* __restrict__ x, double* __restrict__ r,
@@ -1537,7 +1536,9 @@ __global__ __launch_bounds__(256) void k_update_xr(int n, const PcgScalars* __re
                                                    const TV* __restrict__ ap,
                                                    TM* __restrict__ r_low, TM* __restrict__ fill_a,
                                                    TM* __restrict__ fill_b,
-                                                   double* __restrict__ partial) {
+                                                   double* __restrict__ partial,
+                                                   const double* __restrict__ rd = nullptr,
+                                                   double* __restrict__ r_rd = nullptr) {
 #pragma clang fp contract(off)
   if (S->done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1548,6 +1549,7 @@ __global__ __launch_bounds__(256) void k_update_xr(int n, const PcgScalars* __re
     double ri = r[i] - alpha * (double)ap[i];
     r[i] = ri;
     if (r_low) r_low[i] = (TM)ri;
+    if (r_rd) r_rd[i] = ri * rd[i];  // the first Jacobi sweep's starting iterate
     v = ri * ri;
     typedef typename Bits<TM>::U U;
     if (fill_a) reinterpret_cast<U*>(fill_a)[i] = Bits<TM>::S;

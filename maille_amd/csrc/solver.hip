@@ -331,7 +331,7 @@ struct Operator {
   } sw_lo, sw_up;
   DBuf<int> d_sw_dsrc;
   DBuf<float> d_sw_diag;
-  DBuf<double> d_sw_rd, w_s1, w_s2;
+  DBuf<double> d_sw_rd, w_s1, w_s2, w_s3, w_y0;  // w_s3: z0 = y rd, w_y0: r rd
   bool have_sweep = false;
   int sweeps = 0;  // Jacobi sweeps per triangular system; 0: the exact dataflow solves
   int sweep_batch = 8, spmv_batch = 8;  // entries per batch of loads (autotuned: 8 or 16)
@@ -459,8 +459,7 @@ struct Operator {
       d_sw_dsrc.upload_vec(lo.dsrc, stream);
       d_sw_diag.alloc(n);
       d_sw_rd.alloc(n);
-      w_s1.alloc(n);
-      w_s2.alloc(n);
+      for (auto* b : {&w_s1, &w_s2, &w_s3, &w_y0}) b->alloc(n);
       HIP_CHECK(hipStreamSynchronize(stream));  // the host layouts go out of scope
       have_sweep = true;
       if (factored) gather_sweeps();
@@ -685,22 +684,27 @@ struct Operator {
   // z ~ (L L^T)^-1 r by `sweeps` Jacobi sweeps of L y = r from y = D^-1 r, then as many of
   // L^T z = y from z = D^-1 y (the transposed iteration, so the preconditioner stays
   // symmetric), on the FP32 factor with FP64 vectors. With `dot`, the last sweep also leaves
-  // per-block partial sums of dot.z in `partial` (the PCG's r.z).
+  // per-block partial sums of dot.z in `partial` (the PCG's r.z). With y0 (= r * rd, written
+  // by the PCG's residual update) the first forward sweep reads it instead of forming it; the
+  // last forward sweep writes z0 = y * rd for the first backward one likewise.
   void apply_sweeps(const double* r, double* z, const mlk::PcgScalars* S,
-                    const double* dot = nullptr, double* partial = nullptr) {
+                    const double* dot = nullptr, double* partial = nullptr,
+                    const double* y0 = nullptr) {
     hipEvent_t ev = prof.begin(stream);
     const unsigned g = grid_for(n);
     const int k = sweeps;
     double* a = w_s1.p;
     double* b = w_s2.p;
-    auto sweep = [&](const SweepDev& t, bool first, const double* rhs, const double* in,
-                     double* out, bool last) {
+    double* z0 = w_s3.p;
+    auto sweep = [&](const SweepDev& t, const double* rhs, const double* in, double* out,
+                     double* out_rd, bool last) {
       const double* dv = last ? dot : nullptr;
       double* pv = last ? partial : nullptr;
       const int sign = &t == &sw_lo ? -1 : 1;
+      const bool first = in == nullptr;
       auto go = [&](auto kernel) {
         kernel<<<g, 256, 0, stream>>>(n, t.soff.p, t.scol.p, t.sdel.p, sign, t.slen.p, t.sval.p,
-                                      d_sw_rd.p, rhs, first ? nullptr : in, out, dv, pv, S);
+                                      d_sw_rd.p, rhs, in, out, out_rd, dv, pv, S);
       };
       if (sweep_batch == 16) {
         if (t.d16) first ? go(mlk::k_sweep<true, true, 16>) : go(mlk::k_sweep<false, true, 16>);
@@ -710,18 +714,19 @@ struct Operator {
         else first ? go(mlk::k_sweep<true, false>) : go(mlk::k_sweep<false, false>);
       }
     };
-    // forward, ping-pong between a and b; y ends in a
-    sweep(sw_lo, true, r, nullptr, a, false);
+    // forward, ping-pong between a and b; y ends in a, and z0 = y rd in z0
+    sweep(sw_lo, r, y0, a, k == 1 ? z0 : nullptr, false);
     for (int m = 2; m <= k; ++m) {
-      sweep(sw_lo, false, r, a, b, false);
+      sweep(sw_lo, r, a, b, m == k ? z0 : nullptr, false);
       std::swap(a, b);
     }
-    // backward from y = a; the sweeps alternate between b and z so that the last lands in z
+    // backward from y = a and z0; the sweeps alternate between b and z so that the last lands
+    // in z
     const double* y = a;
-    const double* prev = nullptr;
+    const double* prev = z0;
     for (int m = 1; m <= k; ++m) {
       double* out = ((k - m) & 1) ? b : z;
-      sweep(sw_up, m == 1, y, prev, out, m == k);
+      sweep(sw_up, y, prev, out, nullptr, m == k);
       prev = out;
     }
     HIP_CHECK(hipGetLastError());
@@ -815,7 +820,7 @@ struct Operator {
       HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
     }
     mlk::k_dot<double, double><<<vb, 256, 0, stream>>>(n, b, b, part);
-    mlk::k_init_norm<<<1, 256, 0, stream>>>(part, vb, S, tol);
+    mlk::k_init_norm<<<1, mlk::SUM_THREADS, 0, stream>>>(part, vb, S, tol);
     HIP_CHECK(hipGetLastError());
     if (jac) {
       apply_sweeps(rin, z, S, r, part);
@@ -825,7 +830,7 @@ struct Operator {
       precondition(rin, y, z, S);
       mlk::k_dot<double, TM><<<vb, 256, 0, stream>>>(n, r, z, part);
     }
-    mlk::k_init_rz<<<1, 256, 0, stream>>>(part, vb, S);
+    mlk::k_init_rz<<<1, mlk::SUM_THREADS, 0, stream>>>(part, vb, S);
     mlk::k_convert<TM, TA><<<vb, 256, 0, stream>>>(n, z, p);
     HIP_CHECK(hipGetLastError());
     *hS = read_scalars(S);
@@ -835,27 +840,30 @@ struct Operator {
     for (;;) {
       for (int64_t k = 0; k < batch; ++k) {
         spmv<TA>(p, ap, part, S);
-        mlk::k_alpha<<<1, 256, 0, stream>>>(part, vb, S);
+        mlk::k_alpha<<<1, mlk::SUM_THREADS, 0, stream>>>(part, vb, S);
         hipEvent_t ev = prof.begin(stream);
         mlk::k_update_xr<TA, TM><<<vb, 256, 0, stream>>>(n, S, x, r, p, ap, rlow,
                                                          jac ? nullptr : y, jac ? nullptr : z,
-                                                         part);
+                                                         part, jac ? d_sw_rd.p : nullptr,
+                                                         jac ? w_y0.p : nullptr);
         prof.end(P_VEC, ev, stream);
-        mlk::k_converged<<<1, 256, 0, stream>>>(part, vb, S);
+        mlk::k_converged<<<1, mlk::SUM_THREADS, 0, stream>>>(part, vb, S);
         if (a32) {
           spmv<double>(x, w_ax.p, nullptr, S, 1);
           mlk::k_replace_residual<TM><<<vb, 256, 0, stream>>>(n, S, b, w_ax.p, r, rlow, part);
-          mlk::k_replaced<<<1, 256, 0, stream>>>(part, vb, S);
+          mlk::k_replaced<<<1, mlk::SUM_THREADS, 0, stream>>>(part, vb, S);
         }
         if (jac) {
-          apply_sweeps(rin, z, S, r, part);
+          // the residual replacement (ML_MIXED) rewrites r after k_update_xr: then y0 is
+          // formed by the first sweep itself
+          apply_sweeps(rin, z, S, r, part, a32 ? nullptr : w_y0.p);
         } else {
           precondition(rin, y, z, S);
           ev = prof.begin(stream);
           mlk::k_dot<double, TM><<<vb, 256, 0, stream>>>(n, r, z, part);
           prof.end(P_VEC, ev, stream);
         }
-        mlk::k_beta<<<1, 256, 0, stream>>>(part, vb, S);
+        mlk::k_beta<<<1, mlk::SUM_THREADS, 0, stream>>>(part, vb, S);
         ev = prof.begin(stream);
         mlk::k_update_p<TA, TM><<<vb, 256, 0, stream>>>(n, S, p, z);
         prof.end(P_VEC, ev, stream);
@@ -1205,7 +1213,7 @@ struct Operator {
     const int vb = grid_for(n);
     spmv<double>(x, w_ax.p, nullptr, nullptr);
     mlk::k_residual<<<vb, 256, 0, stream>>>(n, b, w_ax.p, w_d.p, w_partial.p);
-    mlk::k_init_norm<<<1, 256, 0, stream>>>(w_partial.p, vb, d_Sin.p, 1.0);
+    mlk::k_init_norm<<<1, mlk::SUM_THREADS, 0, stream>>>(w_partial.p, vb, d_Sin.p, 1.0);
     HIP_CHECK(hipGetLastError());
     mlk::PcgScalars h = read_scalars(d_Sin.p);
     return bnorm > 0 ? h.bnorm / bnorm : 0.0;
