@@ -220,9 +220,10 @@ int ml_model_sweeps(const ml_model* m);
  * |i - j| of the factor fits; 4: int32), 0 before any sweep layout is built. */
 int ml_model_sweep_column_bytes(const ml_model* m);
 /* Diagnostics (no counterpart in the reference): out[6] = workgroups per CU that the runtime's
- * occupancy figure reports and that a co-residency probe of the kernel itself found the device
- * to hold at once, for the forward triangular solve, the backward one and IC(0); the dataflow
- * grids use the probed counts. Zeros for a model without free DOFs. */
+ * occupancy figure reports and that the dataflow grids use, for the forward triangular solve,
+ * the backward one and IC(0): one fewer than the largest count every workgroup of which was
+ * resident at once in three co-residency probes of the kernel itself. Zeros for a model
+ * without free DOFs. */
 ml_status ml_model_residency(const ml_model* m, int32_t* out);
 /* Recompute the element stiffnesses and K_ff values on the device (after ml_model_set_young). */
 ml_status ml_model_assemble(ml_model* m);

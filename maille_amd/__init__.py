@@ -620,7 +620,8 @@ class Model(_Handle):
         return lib().ml_model_sweeps(self._h)
 
     def residency(self):
-        """{kernel: (reported, probed)} workgroups per CU of the dataflow kernels."""
+        """{kernel: (reported, used)} workgroups per CU of the dataflow kernels: the runtime's
+        occupancy figure and the probed count the grids use."""
         out = (C.c_int32 * 6)()
         _check(lib().ml_model_residency(self._h, out))
         return {k: (out[2 * i], out[2 * i + 1])

@@ -153,18 +153,20 @@ long long wait_limit() {
   return 2000ll * khz;
 }
 
-// Workgroups of a persistent dataflow kernel that the device holds at once, per CU. The
-// dataflow kernels need every workgroup of their grid resident (the lowest unfinished block
-// must always have a running workgroup), and a cooperative launch admits whatever grid the
-// runtime's occupancy figure allows. That figure has over-reported LDS-heavy kernels on gfx950
-// by one workgroup per CU (26 reported against 25 held at 6 KB of LDS, 13 against 12 at
-// 12 KB), which would leave workgroups of such a grid waiting for a place. So the figure is
-// only where the search starts: the kernel itself is launched as a co-residency probe at that
-// many workgroups per CU, then one fewer, until every workgroup of the probe was resident at
-// once (the probe's waits are bounded, so a failed probe drains too). The result is cached
-// per device and kernel; it holds while no other kernel occupies the device during a dataflow
-// launch (a launch that stalls then ends through its bounded waits and is repeated on a
-// smaller grid, Operator::shrink).
+// Workgroups of a persistent dataflow kernel to launch per CU. The dataflow kernels need every
+// workgroup of their grid resident (the lowest unfinished block must always have a running
+// workgroup), and the runtime's occupancy figure over-reports what gfx950 holds: the probes
+// below found 28 of a reported 32 one-wave IC(0) workgroups per CU, 25 of 26 at 6 KB of LDS,
+// 7 of 8 three-wave triangular-solve workgroups, and one count that fitted in one run and not
+// in another. A cooperative launch admits the reported grid, which explains a dataflow launch
+// that stalled until its waits expired. So the figure is only where the search starts: the
+// kernel itself is launched as a co-residency probe at that many workgroups per CU, then one
+// fewer, until every workgroup of three probes in a row was resident at once (the probe's
+// waits are bounded, so a failed probe drains too); the grid then keeps one workgroup per CU
+// below that count. The result is cached per device and kernel. It assumes no other kernel
+// occupies the device during a dataflow launch; a launch that stalls anyway ends through its
+// bounded waits and is repeated on a smaller grid (Operator::shrink), counted in
+// ml_solve_info.device_retries.
 struct ProbeKey {
   int device;
   const void* kernel;
@@ -187,15 +189,21 @@ int resident_per_cu(const void* kernel, int threads, hipStream_t stream, Launch 
   DBuf<int> probe;
   probe.alloc(2);
   int per_cu = occupancy_per_cu(kernel, threads);
-  for (; per_cu > 1; --per_cu) {
+  auto resident = [&](int count) {
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(probe.p, 2, 0);
-    launch_probe(per_cu * cus, probe.p, limit);
+    launch_probe(count * cus, probe.p, limit);
     HIP_CHECK(hipGetLastError());
     int h[2] = {0, 0};
     HIP_CHECK(hipMemcpyAsync(h, probe.p, sizeof(h), hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
-    if (h[1] == 0 && h[0] == per_cu * cus) break;
-  }
+    return h[1] == 0 && h[0] == count * cus;
+  };
+  // at the very limit, whether every workgroup finds a place varies from one launch to the
+  // next: a count must pass three probes in a row, and the grid keeps one workgroup per CU
+  // below it
+  for (; per_cu > 1; --per_cu)
+    if (resident(per_cu) && resident(per_cu) && resident(per_cu)) break;
+  per_cu = std::max(1, per_cu - 1);
   g_resident[{dev, kernel}] = per_cu;
   return per_cu;
 }

@@ -499,15 +499,15 @@ def test_a_model_with_every_dof_imposed(gpu):
 
 
 def test_dataflow_grids_are_bounded_by_a_co_residency_probe(gpu):
-    """Each dataflow kernel's grid comes from a probe of the kernel itself (every workgroup of
-    the probe resident at once), never above the runtime's occupancy figure; solves on those
-    grids need no retry."""
+    """Each dataflow kernel's grid comes from probes of the kernel itself (every workgroup of
+    three probes in a row resident at once), one workgroup per CU below that, hence below the
+    runtime's occupancy figure; solves on those grids need no retry."""
     p = PROBLEMS["jittered_cube_8"]
     for precision in ("FP64", "MIXED_IC0"):
         with gpu.Model(p, precision=getattr(gpu, precision)) as m:
             res = m.residency()
             print(f"{precision}: reported/probed workgroups per CU {res}", flush=True)
-            for kernel, (reported, probed) in res.items():
-                assert 1 <= probed <= reported, kernel
+            for kernel, (reported, used) in res.items():
+                assert 1 <= used and (used < reported or used == 1), kernel
             u, _, info = m.solve()
             assert info["device_retries"] == 0
