@@ -20,6 +20,7 @@
 #ifndef MAILLE_HIP_H
 #define MAILLE_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -110,6 +111,18 @@ ml_status ml_solid_d(double young, double nu, double* d);
  * (ke[e][j][i] as the reference's Vec<Vec<f64>>). Host buffers; computed on the device. */
 ml_status ml_stiffness_batch(int nen, int64_t n_elements, const double* coords,
                              const double* young, const double* nu, double* ke);
+/* No counterpart in the reference: ml_stiffness_batch for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc). Same
+ * arguments and results, bit for bit; it calls no hipMalloc, hipFree or stream creation and
+ * stages the batch in a file-scope device array of ML_STIFFNESS_SCRATCH_BYTES, one per GPU,
+ * held until the process ends, on the null stream. A batch needs (24 nen + 16 + 4 nen +
+ * 2 (3 nen)^2 x 8) bytes per element (plus 256-byte alignment of six regions): at most 27,593
+ * Tet4 or 4,566 Tet10 elements per call, larger batches are refused with ML_INVALID_ARGUMENT.
+ * Calls on one GPU share the array: calls that overlap on one GPU (from several threads)
+ * race. */
+#define ML_STIFFNESS_SCRATCH_BYTES ((size_t)64 << 20)
+ml_status ml_stiffness_batch_noalloc(int nen, int64_t n_elements, const double* coords,
+                                     const double* young, const double* nu, double* ke);
 
 /* ------------------------------------------------------------------ pattern */
 

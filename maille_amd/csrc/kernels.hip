@@ -135,6 +135,23 @@ __global__ void k_fill_i32(int* p, int64_t n, int v) {
   if (i < n) p[i] = v;
 }
 
+// p[i] = i (the identity connectivity of a batch of separate elements)
+__global__ void k_iota_i32(int* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (int)i;
+}
+
+// Element matrices from the entry-major layout of k_stiffness (slot j * ne + e) to one
+// row-major m x m block per element (out[e * mm + j]), mm = m * m
+__global__ void k_entry_to_element_major(int64_t ne, int mm, const double* __restrict__ in,
+                                         double* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ne * mm) {
+    const int64_t e = t / mm, j = t - e * mm;
+    out[t] = in[j * ne + e];
+  }
+}
+
 // Values uniform in [-1, 1) from SplitMix64 of seed + index: a fixed pseudo-random vector.
 __global__ void k_fill_hash(double* p, int64_t n, unsigned long long seed) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

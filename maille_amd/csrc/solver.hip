@@ -2499,6 +2499,98 @@ ml_status ml_stiffness_batch(int nen, int64_t ne, const double* coords, const do
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Staging of ml_stiffness_batch_noalloc: a file-scope device array, one copy per GPU, held
+// until the process ends. Host code never uses its name as an address: each device's address
+// comes from hipGetSymbolAddress, taken once per device on its first call with that device
+// current, and kept in g_scratch.
+static __device__ __align__(256) unsigned char scratch[ML_STIFFNESS_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+void* g_scratch[64] = {};
+
+void* scratch_address() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw ArgError("device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw HipError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = p;
+  }
+  return g_scratch[dev];
+}
+
+// Bytes of scratch a batch of ne elements of nen nodes needs: coordinates, E, nu, the identity
+// connectivity, the entry-major stiffnesses and their element-major copy, each region 256-byte
+// aligned.
+size_t noalloc_bytes(int nen, int64_t ne, size_t* off) {
+  const size_t m = 3 * (size_t)nen;
+  const size_t sizes[6] = {sizeof(double) * 3 * nen * (size_t)ne, sizeof(double) * (size_t)ne,
+                           sizeof(double) * (size_t)ne, sizeof(int) * nen * (size_t)ne,
+                           sizeof(double) * m * m * (size_t)ne, sizeof(double) * m * m * (size_t)ne};
+  size_t at = 0;
+  for (int k = 0; k < 6; ++k) {
+    off[k] = at;
+    at += (sizes[k] + 255) / 256 * 256;
+  }
+  return at;
+}
+
+}  // namespace
+
+extern "C" {
+
+ml_status ml_stiffness_batch_noalloc(int nen, int64_t ne, const double* coords,
+                                     const double* young, const double* nu, double* ke) {
+  return guarded([&]() -> int {
+    if ((nen != 4 && nen != 10) || ne <= 0 || !coords || !young || !nu || !ke)
+      throw ArgError("bad stiffness batch arguments");
+    size_t off[6];
+    if (ne > (int64_t)ML_STIFFNESS_SCRATCH_BYTES || noalloc_bytes(nen, ne, off) > ML_STIFFNESS_SCRATCH_BYTES) {
+      size_t probe[6];
+      int64_t most = 0;
+      for (int64_t lo = 1, hi = ML_STIFFNESS_SCRATCH_BYTES; lo <= hi;) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (noalloc_bytes(nen, mid, probe) <= ML_STIFFNESS_SCRATCH_BYTES) { most = mid; lo = mid + 1; }
+        else hi = mid - 1;
+      }
+      throw ArgError("batch of " + std::to_string(ne) + " elements exceeds the " +
+                     std::to_string(ML_STIFFNESS_SCRATCH_BYTES >> 20) + " MiB scratch: at most " +
+                     std::to_string(most) + " " + (nen == 4 ? "Tet4" : "Tet10") +
+                     " elements per call");
+    }
+    unsigned char* base = (unsigned char*)scratch_address();
+    double* dx = (double*)(base + off[0]);
+    double* dE = (double*)(base + off[1]);
+    double* dv = (double*)(base + off[2]);
+    int* dc = (int*)(base + off[3]);
+    double* dk = (double*)(base + off[4]);
+    double* dke = (double*)(base + off[5]);
+    const int m = 3 * nen;
+    hipStream_t st = nullptr;  // the null stream: nothing is created
+    HIP_CHECK(hipMemcpyAsync(dx, coords, sizeof(double) * 3 * nen * ne, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(dE, young, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(dv, nu, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+    mlk::k_iota_i32<<<grid_for(ne * nen), 256, 0, st>>>(dc, ne * nen);
+    if (nen == 4)
+      mlk::k_stiffness<4><<<grid_for(12 * ne), 256, 0, st>>>(ne, dx, dc, dE, dv, dk);
+    else
+      mlk::k_stiffness<10><<<grid_for(30 * ne), 256, 0, st>>>(ne, dx, dc, dE, dv, dk);
+    mlk::k_entry_to_element_major<<<grid_for(ne * m * m), 256, 0, st>>>(ne, m * m, dk, dke);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(ke, dke, sizeof(double) * m * m * ne, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    return ML_OK;
+  });
+}
+
 ml_status ml_pattern_build(int64_t n, int64_t ne, const int64_t* eoff, const int64_t* eit,
                            ml_pattern** out) {
   return guarded([&]() -> int {
