@@ -124,6 +124,12 @@ ml_status ml_stiffness_batch(int nen, int64_t n_elements, const double* coords,
 ml_status ml_stiffness_batch_noalloc(int nen, int64_t n_elements, const double* coords,
                                      const double* young, const double* nu, double* ke);
 
+/* Diagnostics (no counterpart in the reference; host only): out[i] = num[i] / den[i] by the
+ * sequence the FP64 triangular solves divide with (the reciprocal of the diagonal and two
+ * exact-remainder corrections, the division itself outside their exact range), so that it can
+ * be compared with the division on the host. */
+ml_status ml_quotient64(int64_t n, const double* num, const double* den, double* out);
+
 /* ------------------------------------------------------------------ pattern */
 
 typedef struct ml_pattern ml_pattern;

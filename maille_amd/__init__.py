@@ -38,7 +38,7 @@ ITERATIVE_FROM = 100_000   # solve.rs:185
 
 EXPORTS = (
     "ml_last_error", "ml_device_count", "ml_solid_d", "ml_stiffness_batch",
-    "ml_stiffness_batch_noalloc",
+    "ml_stiffness_batch_noalloc", "ml_quotient64",
     "ml_pattern_build", "ml_pattern_n", "ml_pattern_nnz", "ml_pattern_get",
     "ml_pattern_ntargets", "ml_pattern_graph", "ml_pattern_free", "ml_layout_verify",
     "ml_pcg_create", "ml_pcg_solve", "ml_pcg_solve_device", "ml_pcg_multiply",
@@ -171,6 +171,7 @@ def lib():
             "ml_solid_d": (_int, [C.c_double, C.c_double, _D]),
             "ml_stiffness_batch": (_int, [_int, _i64, _D, _D, _D, _D]),
             "ml_stiffness_batch_noalloc": (_int, [_int, _i64, _D, _D, _D, _D]),
+            "ml_quotient64": (_int, [_i64, _D, _D, _D]),
             "ml_pattern_build": (_int, [_i64, _i64, _I, _I, _PP]),
             "ml_pattern_n": (_i64, [_P]),
             "ml_pattern_nnz": (_i64, [_P]),

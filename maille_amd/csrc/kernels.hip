@@ -1790,6 +1790,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
     const T* __restrict__ rhs, T* out, const PcgScalars* __restrict__ S, int* abort_flag,
     long long limit, long long* __restrict__ trace, int nrep) {
 #pragma clang fp contract(off)
+  static_assert(std::is_same<T, double>::value, "the solves run on FP64 vectors and sums");
   if (nrep < 0) {  // a co-residency probe of this kernel (abort_flag: the probe's two words)
     residency_probe(abort_flag, limit);
     return;
@@ -1832,7 +1833,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
     // per-lane state of the solver
     int p = 0, x1 = 0, ip = 0, i1 = 0, pl = 0, l = 0;
     T acc = T(0), rl = T(0), dl = T(1);
-    double rd = 1.0;  // FP32: 1 / (double)diag, see the row loop
+    double rd = 1.0;  // RN64(1 / diag), see the row loop
     // stage the block: the poller the distinct external columns with the values published so
     // far, read at device scope (a plain load could hit a stale line of this XCD's L2 that
     // still holds the sentinel); the solver the external entries (coefficient, column index)
@@ -1872,7 +1873,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
         i1 = ioff[l + 1] - ib;
         rl = rhs[pl];
         rd = rdiag[l];
-        if constexpr (!std::is_same<T, float>::value) dl = (T)diag[l];  // FP64: the corrections
+        dl = (T)diag[l];
         acc = SUBSEQ ? rl : T(0);
       }
     }
@@ -2053,27 +2054,10 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
         for (; k < kend; ++k) {
           T y = acc;
           if (lane == k) {
-            // FP32: the quotient num / d is RN32(RN64(num * RN64(1 / d))). That product is
-            // within 2^-52 (relative) of num / d, and a quotient of two binary32 numbers is
-            // never a binary32 rounding midpoint (that needs 25 significant bits) and never
-            // within 2^-49 of one, so both round to the same binary32 value (results in the
-            // normal range). Three dependent operations replace the division sequence on the
-            // row chain.
-            // FP64: q = num * RN(1/d), then two exact-remainder corrections
-            // q += fma(-q, d, num) * RN(1/d). After the first, q is within one ulp of num / d;
-            // with a faithful q and the correctly rounded reciprocal, the second gives the
-            // correctly rounded quotient (Markstein's theorem; normal range). Five dependent
-            // operations replace the division sequence.
-            if constexpr (std::is_same<T, float>::value) {
-              y = (float)((double)(SUBSEQ ? acc : rl - acc) * rd);
-            } else {
-              const double num = SUBSEQ ? acc : rl - acc;
-              double q = num * rd;
-              double r = fma(-q, dl, num);
-              q = fma(r, rd, q);
-              r = fma(-q, dl, num);
-              y = fma(r, rd, q);
-            }
+            // the row's quotient, the division's bit for bit, from the reciprocal of the
+            // diagonal and two exact-remainder corrections (quotient64): five dependent
+            // operations on the row chain instead of the division sequence
+            y = quotient64(SUBSEQ ? acc : rl - acc, dl, rd);
             yown = y;
             publish(out + pl, y);
             if (TRACE && rep == 0) trace[10 * (long long)nblocks + l] = wall_clock64();

@@ -2591,6 +2591,14 @@ ml_status ml_stiffness_batch_noalloc(int nen, int64_t ne, const double* coords,
   });
 }
 
+ml_status ml_quotient64(int64_t n, const double* num, const double* den, double* out) {
+  return guarded([&]() -> int {
+    if (n < 0 || (n > 0 && (!num || !den || !out))) throw ArgError("bad quotient arguments");
+    for (int64_t i = 0; i < n; ++i) out[i] = mlk::quotient64(num[i], den[i], 1.0 / den[i]);
+    return ML_OK;
+  });
+}
+
 ml_status ml_pattern_build(int64_t n, int64_t ne, const int64_t* eoff, const int64_t* eit,
                            ml_pattern** out) {
   return guarded([&]() -> int {
