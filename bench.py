@@ -35,9 +35,10 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def spmv_bytes(n, nnz_full, val):
-    """Algorithmic SpMV bytes: values + column indices + row pointers + x + y."""
-    return nnz_full * (val + 4) + (n + 1) * 4 + 2 * n * val
+def spmv_bytes(n, nnz_full, val, idx=4):
+    """Algorithmic SpMV bytes: values + column indices (int32, or 16-bit offsets from the row)
+    + row lengths + x + y."""
+    return nnz_full * (val + idx) + (n + 1) * 4 + 2 * n * val
 
 
 def sweep_bytes(n, nnz_strict, col_bytes=4):
@@ -167,6 +168,8 @@ def run_tet10(args, ma, meshes):
     u64 = None
     codes = (("fp64", ma.FP64), ("mixed_ic0", ma.MIXED_IC0), ("mixed_jacobi", ma.MIXED_JACOBI),
              ("mixed", ma.MIXED))
+    if args.precision != "best":
+        codes = tuple(c for c in codes if c[0] in ("fp64", args.precision))
     for name, code in codes:
         with ma.Model(p, precision=code) as m:
             m.autotune()
@@ -660,10 +663,11 @@ def main():
     mval = 8 if head == "fp64" else 4
     kinds = {"spmv": "spmv32" if head == "mixed" else "spmv64",
              "precondition": "precond64" if head == "fp64" else "precond32"}
-    col_bytes = models[head].sweep_column_bytes() or 4
+    spmv_idx, col_bytes = models[head].index_bytes()
+    col_bytes = col_bytes or 4
     pre_bytes = (2 * sweeps * sweep_bytes(n, nnz_l - n, col_bytes) if sweeps
                  else 2 * trisolve_bytes(n, nnz_l, mval))
-    bytes_per = {"spmv": spmv_bytes(n, nnz_full, aval), "precondition": pre_bytes}
+    bytes_per = {"spmv": spmv_bytes(n, nnz_full, aval, spmv_idx), "precondition": pre_bytes}
     kernels = {}
     for name, kind in kinds.items():
         ms, cnt = prof[kind]
@@ -702,7 +706,8 @@ def main():
     p64 = compare["fp64"]["prof"]
     spmv64_ms, spmv64_cnt = p64["spmv64"]
     spmv64_avg = spmv64_ms / max(spmv64_cnt, 1)
-    spmv64_gbs = spmv_bytes(n, nnz_full, 8) / (spmv64_avg * 1e-3) / 1e9 if spmv64_cnt else 0.0
+    spmv64_gbs = (spmv_bytes(n, nnz_full, 8, m0.index_bytes()[0]) / (spmv64_avg * 1e-3) / 1e9
+                  if spmv64_cnt else 0.0)
 
     # the whole Model::solve (model.rs:548-610) of the headline mode: model creation (mesh
     # checks, free numbering, Graph, RCM, pattern on the host; upload, element stiffness and

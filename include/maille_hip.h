@@ -126,8 +126,9 @@ ml_status ml_stiffness_batch_noalloc(int nen, int64_t n_elements, const double* 
 
 /* Diagnostics (no counterpart in the reference; host only): out[i] = num[i] / den[i] by the
  * sequence the FP64 triangular solves divide with (the reciprocal of the diagonal and two
- * exact-remainder corrections, the division itself outside their exact range), so that it can
- * be compared with the division on the host. */
+ * exact-remainder corrections: the division's result bit for bit for den > 0 and
+ * 2^-969 <= |num| (or num = +-0) with |num / den| < 2^1020, within one ulp outside that),
+ * so that it can be compared with the division on the host. */
 ml_status ml_quotient64(int64_t n, const double* num, const double* den, double* out);
 
 /* ------------------------------------------------------------------ pattern */
@@ -235,9 +236,10 @@ ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elemen
 /* ml_pcg_set_sweeps / ml_pcg_sweeps for the model's operator. */
 ml_status ml_model_set_sweeps(ml_model* m, int sweeps);
 int ml_model_sweeps(const ml_model* m);
-/* Bytes per column index the sweeps read (2: 16-bit distances from the row, when every
- * |i - j| of the factor fits; 4: int32), 0 before any sweep layout is built. */
-int ml_model_sweep_column_bytes(const ml_model* m);
+/* Bytes per column index the kernels read: out[0] the SpMV's (2: 16-bit offsets from the row,
+ * when the matrix band fits; 4: int32), out[1] the sweeps' (2 or 4; 0 before any sweep layout
+ * is built). */
+ml_status ml_model_index_bytes(const ml_model* m, int32_t* out);
 /* Diagnostics (no counterpart in the reference): out[6] = workgroups per CU that the runtime's
  * occupancy figure reports and that the dataflow grids use, for the forward triangular solve,
  * the backward one and IC(0): one fewer than the largest count every workgroup of which was

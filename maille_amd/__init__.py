@@ -44,7 +44,7 @@ EXPORTS = (
     "ml_pcg_create", "ml_pcg_solve", "ml_pcg_solve_device", "ml_pcg_multiply",
     "ml_pcg_precondition", "ml_pcg_factor", "ml_pcg_destroy", "ml_pcg_trace",
     "ml_pcg_set_wait_limit", "ml_pcg_set_sweeps", "ml_pcg_sweeps", "ml_model_set_sweeps",
-    "ml_model_sweeps", "ml_model_sweep_column_bytes", "ml_model_residency",
+    "ml_model_sweeps", "ml_model_index_bytes", "ml_model_residency",
     "ml_model_create", "ml_model_assemble", "ml_model_set_young", "ml_model_factor",
     "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_stresses_device",
     "ml_model_nodal_stresses",
@@ -193,7 +193,7 @@ def lib():
             "ml_pcg_sweeps": (_int, [_P]),
             "ml_model_set_sweeps": (_int, [_P, _int]),
             "ml_model_sweeps": (_int, [_P]),
-            "ml_model_sweep_column_bytes": (_int, [_P]),
+            "ml_model_index_bytes": (_int, [_P, _I32]),
             "ml_model_residency": (_int, [_P, _I32]),
             "ml_model_create": (_int, [_i64, _D, _i64, _int, _I, _D, _D, _U8, _D, _int, _int,
                                        _PP, _S]),
@@ -630,9 +630,11 @@ class Model(_Handle):
         return {k: (out[2 * i], out[2 * i + 1])
                 for i, k in enumerate(("forward_solve", "backward_solve", "ic0"))}
 
-    def sweep_column_bytes(self) -> int:
-        """Bytes per column index of the sweep layouts (2 or 4; 0 before they exist)."""
-        return lib().ml_model_sweep_column_bytes(self._h)
+    def index_bytes(self):
+        """(SpMV, sweeps) bytes per column index (2 or 4; sweeps 0 before their layout)."""
+        out = (C.c_int32 * 2)()
+        _check(lib().ml_model_index_bytes(self._h, out))
+        return out[0], out[1]
 
     def solve_many(self, F):
         """Load cases F (rows: nodal loads over all DOFs, without dead pressures) solved

@@ -351,6 +351,136 @@ __global__ __launch_bounds__(256) void k_stiffness(int64_t ne, const double* __r
   for (int j = 0; j < M; ++j) ke[((int64_t)j * M + i) * ne + e] = acc[j];
 }
 
+// tet_mapping for element e at one Gauss point, with the same operations in the same order,
+// but coordinates read as they are used and the gradients stored to g_out as they are formed
+// (fewer live registers); returns det J.
+template <int NEN, typename Out>
+__device__ inline double tet_mapping_to(const double* __restrict__ xyz, const int* __restrict__ en,
+                                        double xi, double eta, double zeta, Out&& g_out) {
+#pragma clang fp contract(off)
+  double dn[NEN][3];
+  const double dl[4][3] = {{-1.0, -1.0, -1.0}, {1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+  if constexpr (NEN == 4) {
+    for (int a = 0; a < 4; ++a)
+      for (int c = 0; c < 3; ++c) dn[a][c] = dl[a][c];
+  } else {
+    const double l[4] = {1.0 - xi - eta - zeta, xi, eta, zeta};
+    const int ed[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+    for (int a = 0; a < 4; ++a)
+      for (int c = 0; c < 3; ++c) dn[a][c] = (4.0 * l[a] - 1.0) * dl[a][c];
+    for (int k = 0; k < 6; ++k)
+      for (int c = 0; c < 3; ++c)
+        dn[4 + k][c] = 4.0 * (l[ed[k][0]] * dl[ed[k][1]][c] + l[ed[k][1]] * dl[ed[k][0]][c]);
+  }
+  double j[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int k = 0; k < NEN; ++k) {
+    const int64_t node = en[k];
+    const double xk[3] = {xyz[3 * node], xyz[3 * node + 1], xyz[3 * node + 2]};
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) j[r][c] += dn[k][r] * xk[c];
+  }
+  double cof[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      cof[r][c] = j[r1][c1] * j[r2][c2] - j[r1][c2] * j[r2][c1];
+    }
+  double det = 0.0;
+  for (int c = 0; c < 3; ++c) det += j[0][c] * cof[0][c];
+  double inv[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) inv[r][c] = cof[c][r] / det;
+  for (int k = 0; k < NEN; ++k)
+    for (int r = 0; r < 3; ++r) {
+      double s = 0.0;
+      for (int c = 0; c < 3; ++c) s += inv[r][c] * dn[k][c];
+      g_out(k, r, s);
+    }
+  return det;
+}
+
+// solid_stiffness (element.rs:223-243) with each Gauss point's mapping computed once per
+// element: a workgroup takes EB consecutive elements. First one thread per (element, Gauss
+// point) computes the shape gradients dN/dx and w |det J| (tet_mapping) and, for the first
+// point, D (solid_d), into LDS; then one thread per (column i, half of the rows j, element)
+// accumulates ke[j][i] += w |det J| (b_j . D b_i) over the Gauss points from LDS, with the same
+// functions, the same zero entries of B and the same order as k_stiffness, so the results are
+// identical bit for bit (each half recomputes D b_i, which keeps 15 sums live instead of 30).
+// Consecutive lanes take consecutive elements, so that every store of slot
+// (j*M + i) * ne + e covers EB contiguous words.
+template <int NEN, int EB = 16>
+__global__ __launch_bounds__(256) void k_stiffness_lds(int64_t ne, const double* __restrict__ xyz,
+                                                       const int* __restrict__ conn,
+                                                       const double* __restrict__ young,
+                                                       const double* __restrict__ nu,
+                                                       double* __restrict__ ke) {
+#pragma clang fp contract(off)
+  constexpr int M = 3 * NEN, NG = Rule<NEN>::N;
+  static_assert(EB * NG <= 256, "one thread per (element, Gauss point)");
+  __shared__ double sg[EB][NG][NEN][3];
+  __shared__ double sf[EB][NG];
+  __shared__ double sd[EB][6][6];
+  const int64_t e0 = (int64_t)blockIdx.x * EB;
+  const int t = threadIdx.x;
+  if (t < EB * NG) {
+    const int el = t / NG, q = t - el * NG;
+    const int64_t e = e0 + el;
+    if (e < ne) {
+      double xi, eta, zeta, w;
+      gauss_point(NEN, q, xi, eta, zeta, w);
+      const double det = tet_mapping_to<NEN>(xyz, conn + e * NEN, xi, eta, zeta,
+                                             [&](int a, int c, double v) { sg[el][q][a][c] = v; });
+      sf[el][q] = w * fabs(det);
+      if (q == 0) {
+        double d[6][6];
+        solid_d(young[e], nu[e], d);
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c < 6; ++c) sd[el][r][c] = d[r][c];
+      }
+    }
+  }
+  __syncthreads();
+  // column i of B from the staged gradients, as b_column builds it
+  auto column = [&](int el, int q, int i, double (&col)[6]) {
+    const int node = i / 3, c = i % 3;
+    const double gx = sg[el][q][node][0], gy = sg[el][q][node][1], gz = sg[el][q][node][2];
+    col[0] = c == 0 ? gx : 0.0;
+    col[1] = c == 1 ? gy : 0.0;
+    col[2] = c == 2 ? gz : 0.0;
+    col[3] = c == 0 ? 0.0 : (c == 1 ? gz : gy);
+    col[4] = c == 0 ? gz : (c == 1 ? 0.0 : gx);
+    col[5] = c == 0 ? gy : (c == 1 ? gx : 0.0);
+  };
+  constexpr int H = M / 2;  // rows j per task
+  static_assert(M % 2 == 0, "two halves of the rows");
+  for (int task = t; task < EB * M * 2; task += blockDim.x) {
+    const int el = task % EB, rest = task / EB;
+    const int i = rest >> 1, j0 = (rest & 1) * H;
+    const int64_t e = e0 + el;
+    if (e >= ne) continue;
+    double acc[H];
+    for (int jj = 0; jj < H; ++jj) acc[jj] = 0.0;
+    for (int q = 0; q < NG; ++q) {
+      const double factor = sf[el][q];
+      double bi[6], db[6];
+      column(el, q, i, bi);
+      for (int r = 0; r < 6; ++r) {
+        double s = 0.0;
+        for (int c = 0; c < 6; ++c) s += sd[el][r][c] * bi[c];
+        db[r] = s;
+      }
+      for (int jj = 0; jj < H; ++jj) {
+        double bj[6];
+        column(el, q, j0 + jj, bj);
+        double s = 0.0;
+        for (int r = 0; r < 6; ++r) s += bj[r] * db[r];
+        acc[jj] += factor * s;
+      }
+    }
+    for (int jj = 0; jj < H; ++jj) ke[((int64_t)(j0 + jj) * M + i) * ne + e] = acc[jj];
+  }
+}
+
 // solid_stresses (element.rs:270-302) at Gauss point q of element e, thread per (e, q):
 // x, y, z, sxx, syy, szz, tyz, txz, txy.
 template <int NEN>
@@ -1238,10 +1368,12 @@ __global__ __launch_bounds__(256) void k_nodal_stresses(int64_t n_nodes,
 // y = A x (Csr::multiply, iterative.rs:131-142). Lane j of a slice sums row j in the order
 // diagonal, lower entries, upper entries: the reference's accumulation order, so the FP64
 // result is bit-identical. Optionally accumulates x.y per block (the PCG curvature p.Ap).
-// Entries are taken BATCH at a time, so that their loads are in flight together.
-template <typename T, int BATCH = 8>
+// Entries are taken BATCH at a time, so that their loads are in flight together. D16: the
+// columns are read as 16-bit offsets from the row (col = j + sdel), else as int32 indices.
+template <typename T, int BATCH = 8, bool D16 = false>
 __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ soff,
                                               const int* __restrict__ scol,
+                                              const int16_t* __restrict__ sdel,
                                               const int* __restrict__ slen,
                                               const T* __restrict__ sval,
                                               const T* __restrict__ x, T* __restrict__ y,
@@ -1255,14 +1387,15 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
   if (j < n) {
     const int base = soff[j >> 6] + (j & 63);
     const int len = slen[j];
-    T s = sval[base] * x[scol[base]];
+    auto column = [&](int at) { return D16 ? j + (int)sdel[at] : scol[at]; };
+    T s = sval[base] * x[column(base)];
     int k = 1;
     for (; k + BATCH <= len; k += BATCH) {
       int c[BATCH];
       T v[BATCH], xv[BATCH];
 #pragma unroll
       for (int u = 0; u < BATCH; ++u) {
-        c[u] = scol[base + ((k + u) << 6)];
+        c[u] = column(base + ((k + u) << 6));
         v[u] = sval[base + ((k + u) << 6)];
       }
 #pragma unroll
@@ -1272,7 +1405,7 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
     }
     for (; k < len; ++k) {
       const int at = base + (k << 6);
-      s += sval[at] * x[scol[at]];
+      s += sval[at] * x[column(at)];
     }
     y[j] = s;
     dotv = (double)x[j] * (double)s;

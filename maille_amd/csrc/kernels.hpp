@@ -8,23 +8,22 @@
 
 namespace mlk {
 
-// num / d, correctly rounded, from rd = RN(1 / d), as the FP64 triangular solves divide each
-// row by its diagonal: q = num * rd, then two exact-remainder corrections
-// q = fma(fma(-q, d, num), rd, q). After the first, q is within one ulp of num / d; with a
-// faithful q and the correctly rounded reciprocal, the second gives the correctly rounded
-// quotient (Markstein's theorem). That needs the remainders num - q d to be exact, which holds
-// while they stay in the normal range, and it loses the sign of a zero numerator (-0 gives
-// +0). So zero and numerators below 2^-969 in magnitude, reciprocals outside the normal range
-// and quotients near overflow (NaN and infinity among them) take the division itself: the
-// result is the division's, bit for bit, for every operand pair with d > 0.
+// num / d from rd = RN(1 / d), as the FP64 triangular solves divide each row by its diagonal:
+// q = num * rd, then two exact-remainder corrections q = fma(fma(-q, d, num), rd, q). After
+// the first, q is within one ulp of num / d; with a faithful q and the correctly rounded
+// reciprocal, the second gives the correctly rounded quotient (Markstein's theorem). The
+// remainders num - q d are exact while they stay in the normal range, so the result is the
+// division's bit for bit for d > 0 with a normal rd, |num| >= 2^-969 and |num / d| < 2^1020,
+// and (through the copysign, which keeps the numerator's sign, the quotient's for d > 0) for
+// num = +-0. Smaller nonzero numerators and quotients near overflow may differ from the
+// division by one ulp: the solves of a positive-definite factor do not produce them, and a
+// range test on the row chain, whose latency bounds these solves, costs more than it guards.
 __host__ __device__ inline double quotient64(double num, double d, double rd) {
-  const double a = fabs(num);
-  if (!(a >= 0x1p-969) || !(rd >= 0x1p-1021) || !(a * rd < 0x1p1020)) return num / d;
   double q = num * rd;
   double r = fma(-q, d, num);
   q = fma(r, rd, q);
   r = fma(-q, d, num);
-  return fma(r, rd, q);
+  return copysign(fma(r, rd, q), num);
 }
 
 // Values never produced by arithmetic (signalling-NaN payloads): a slot holding one has not been

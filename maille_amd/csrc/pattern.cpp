@@ -482,6 +482,23 @@ std::string device_layouts(const LowerPattern& lp, const TriCaps& caps, DeviceLa
   d.slen.resize(n);
   for (i64 j = 0; j < n; ++j)
     d.slen[j] = (int32_t)((lp.offsets[j + 1] - lp.offsets[j]) + (t.offsets[j + 1] - t.offsets[j]));
+  {
+    // 16-bit column offsets from the row, when the matrix band allows
+    bool fits = true;
+    std::vector<int16_t> del(d.scol.size(), 0);
+    for (i64 sl = 0; sl < d.slices && fits; ++sl) {
+      const i64 w = ((i64)d.soff[sl + 1] - d.soff[sl]) / SellLayout::C;
+      for (i64 k = 0; k < w && fits; ++k)
+        for (i64 lane = 0; lane < SellLayout::C; ++lane) {
+          const i64 j = sl * SellLayout::C + lane;
+          const i64 at = d.soff[sl] + k * SellLayout::C + lane;
+          const i64 dlt = j < n ? (i64)d.scol[at] - j : 0;
+          if (dlt < -32768 || dlt > 32767) { fits = false; break; }
+          del[at] = (int16_t)dlt;
+        }
+    }
+    if (fits) d.sdel.swap(del);
+  }
   // forward solve: the strictly lower rows of L
   const i64 nf = lp.nnz() - n;
   std::vector<int32_t> foff(n + 1), fcol(nf), fsrc(nf), fdsrc(n);
@@ -633,6 +650,15 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, const 
   for (size_t q = 0; q < d.scol.size(); ++q)
     if (d.scol[q] < 0 || d.scol[q] >= n || d.ssrc[q] < -1 || d.ssrc[q] >= nnz)
       return "sliced-ELL entry out of range";
+  if (!d.sdel.empty()) {
+    if (d.sdel.size() != d.scol.size()) return "sliced-ELL 16-bit offsets of the wrong size";
+    for (i64 j = 0; j < n; ++j) {
+      const i64 sl = j >> 6, base = d.soff[sl] + (j & 63);
+      for (i64 k = 0; k < d.slen[j]; ++k)
+        if (j + d.sdel[base + (k << 6)] != d.scol[base + (k << 6)])
+          return "sliced-ELL 16-bit offset disagrees with its column";
+    }
+  }
   // triangular solves, against the logical rows they were built from
   const i64 nf = nnz - n;
   std::vector<int32_t> foff(n + 1), fcol(nf), fsrc(nf);

@@ -123,6 +123,8 @@ struct DeviceLayouts {
   // topological, and neighbouring positions are mostly independent rows
   std::vector<int32_t> ic_order;
   std::vector<int32_t> soff, scol, slen, ssrc;
+  // col - row of every sliced-ELL entry as 16 bits when all fit (padding: 0), else empty
+  std::vector<int16_t> sdel;
   TriLayout fw, bw;
 };
 // Returns an empty string or the reason the pattern cannot be laid out (row too long).

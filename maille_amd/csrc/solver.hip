@@ -264,6 +264,8 @@ struct Operator {
   // sliced ELL of the full symmetric matrix
   int slices = 0;
   DBuf<int> d_soff, d_scol, d_slen, d_ssrc;
+  DBuf<int16_t> d_sdel;  // 16-bit column offsets of the sliced ELL, when the band allows
+  bool spmv_d16 = false;
   DBuf<double> d_sval;
   DBuf<float> d_sval32;
   // the IC(0) factor and the layouts of its two triangular solves
@@ -385,6 +387,9 @@ struct Operator {
     d_icorder.upload_vec(d.ic_order, stream);
     d_soff.upload_vec(d.soff, stream);
     d_scol.upload_vec(d.scol, stream);
+    spmv_d16 = !d.sdel.empty();
+    if (spmv_d16) d_sdel.upload_vec(d.sdel, stream);
+    else d_sdel.release();
     d_slen.upload_vec(d.slen, stream);
     d_ssrc.upload_vec(d.ssrc, stream);
     nf = nnz - n;
@@ -785,10 +790,11 @@ struct Operator {
   void spmv(const TA* x, TA* y, double* partial, const mlk::PcgScalars* S, int gated = 0) {
     const bool f32 = std::is_same<TA, float>::value;
     hipEvent_t ev = prof.begin(stream);
-    auto kernel = spmv_batch == 16 ? mlk::k_spmv<TA, 16> : mlk::k_spmv<TA, 8>;
+    auto kernel = spmv_d16 ? (spmv_batch == 16 ? mlk::k_spmv<TA, 16, true> : mlk::k_spmv<TA, 8, true>)
+                           : (spmv_batch == 16 ? mlk::k_spmv<TA, 16> : mlk::k_spmv<TA, 8>);
     kernel<<<grid_for(n), 256, 0, stream>>>(
-        n, d_soff.p, d_scol.p, d_slen.p, f32 ? (const TA*)d_sval32.p : (const TA*)d_sval.p, x, y,
-        partial, S, gated);
+        n, d_soff.p, d_scol.p, d_sdel.p, d_slen.p, f32 ? (const TA*)d_sval32.p : (const TA*)d_sval.p,
+        x, y, partial, S, gated);
     HIP_CHECK(hipGetLastError());
     prof.end(f32 ? P_SPMV32 : P_SPMV64, ev, stream);
   }
@@ -1144,7 +1150,8 @@ struct Operator {
         mlk::k_spmv_rep<double><<<vb, 256, 0, stream>>>(n, nrep, d_soff.p, d_scol.p, d_slen.p,
                                                         d_sval.p, in, out);
       else
-        mlk::k_spmv<double><<<vb, 256, 0, stream>>>(n, d_soff.p, d_scol.p, d_slen.p, d_sval.p,
+        mlk::k_spmv<double><<<vb, 256, 0, stream>>>(n, d_soff.p, d_scol.p, nullptr, d_slen.p,
+                                                     d_sval.p,
                                                      in, out, nullptr, nullptr, 0);
       prof.end(P_SPMV64, ev, stream);
     };
@@ -1369,17 +1376,50 @@ struct Model {
     std::vector<uint32_t>().swap(plan.asm_slot);
   }
 
+  // Tet10: the kernel that stages each Gauss point's mapping in LDS (k_stiffness_lds), or the
+  // one that recomputes it per column (k_stiffness); bit-identical, the autotuner keeps the
+  // faster
+  bool stiffness_lds = true;
+
   void stiffness() {
     const int64_t ne = plan.n_elements;
     hipEvent_t ev = op.prof.begin(s());
     if (plan.nen == 4)
       mlk::k_stiffness<4><<<grid_for(12 * ne), 256, 0, s()>>>(ne, d_xyz.p, d_conn.p, d_young.p,
                                                              d_nu.p, d_ke.p);
+    else if (stiffness_lds)
+      mlk::k_stiffness_lds<10><<<grid_for(ne, 16), 256, 0, s()>>>(ne, d_xyz.p, d_conn.p, d_young.p,
+                                                                 d_nu.p, d_ke.p);
     else
       mlk::k_stiffness<10><<<grid_for(30 * ne), 256, 0, s()>>>(ne, d_xyz.p, d_conn.p, d_young.p,
                                                               d_nu.p, d_ke.p);
     HIP_CHECK(hipGetLastError());
     op.prof.end(P_STIFF, ev, s());
+  }
+
+  // Tet10: time both stiffness kernels (three runs each after a warm-up), keep the faster
+  void tune_stiffness() {
+    if (plan.nen != 10) return;
+    hipEvent_t a, b;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    float best = 1e30f;
+    bool pick = true;
+    for (bool lds : {true, false}) {
+      stiffness_lds = lds;
+      stiffness();
+      HIP_CHECK(hipEventRecord(a, s()));
+      for (int k = 0; k < 3; ++k) stiffness();
+      HIP_CHECK(hipEventRecord(b, s()));
+      HIP_CHECK(hipEventSynchronize(b));
+      float t = 0.0f;
+      HIP_CHECK(hipEventElapsedTime(&t, a, b));
+      if (t < best) { best = t; pick = lds; }
+    }
+    stiffness_lds = pick;
+    stiffness();  // d_ke holds the chosen kernel's (identical) element matrices
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
   }
 
   // model.rs:1480-1492 on the device: Ke of every element, gathered into K_ff and K_fc u_c.
@@ -2488,7 +2528,7 @@ ml_status ml_stiffness_batch(int nen, int64_t ne, const double* coords, const do
     if (nen == 4)
       mlk::k_stiffness<4><<<grid_for(12 * ne), 256, 0, st>>>(ne, dx.p, dc.p, dE.p, dv.p, dk.p);
     else
-      mlk::k_stiffness<10><<<grid_for(30 * ne), 256, 0, st>>>(ne, dx.p, dc.p, dE.p, dv.p, dk.p);
+      mlk::k_stiffness_lds<10><<<grid_for(ne, 16), 256, 0, st>>>(ne, dx.p, dc.p, dE.p, dv.p, dk.p);
     HIP_CHECK(hipGetLastError());
     std::vector<double> h((size_t)m * m * ne);
     HIP_CHECK(hipMemcpyAsync(h.data(), dk.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -2582,7 +2622,7 @@ ml_status ml_stiffness_batch_noalloc(int nen, int64_t ne, const double* coords,
     if (nen == 4)
       mlk::k_stiffness<4><<<grid_for(12 * ne), 256, 0, st>>>(ne, dx, dc, dE, dv, dk);
     else
-      mlk::k_stiffness<10><<<grid_for(30 * ne), 256, 0, st>>>(ne, dx, dc, dE, dv, dk);
+      mlk::k_stiffness_lds<10><<<grid_for(ne, 16), 256, 0, st>>>(ne, dx, dc, dE, dv, dk);
     mlk::k_entry_to_element_major<<<grid_for(ne * m * m), 256, 0, st>>>(ne, m * m, dk, dke);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(ke, dke, sizeof(double) * m * m * ne, hipMemcpyDeviceToHost, st));
@@ -2863,9 +2903,14 @@ ml_status ml_model_residency(const ml_model* h, int32_t* out) {
   });
 }
 
-int ml_model_sweep_column_bytes(const ml_model* h) {
-  if (!h || !h->m.op.have_sweep) return 0;
-  return h->m.op.sw_lo.d16 && h->m.op.sw_up.d16 ? 2 : 4;
+ml_status ml_model_index_bytes(const ml_model* h, int32_t* out) {
+  return guarded([&]() -> int {
+    if (!h || !out) throw ArgError("null argument");
+    const Operator& op = h->m.op;
+    out[0] = op.spmv_d16 ? 2 : 4;
+    out[1] = !op.have_sweep ? 0 : (op.sw_lo.d16 && op.sw_up.d16 ? 2 : 4);
+    return ML_OK;
+  });
 }
 
 ml_status ml_pcg_trace(ml_pcg* h, const double* r, int64_t* nblocks, int64_t* trace_fw,
@@ -3628,6 +3673,7 @@ ml_status ml_model_autotune(ml_model* h, int64_t nrhs, int32_t* chosen) {
   return guarded([&]() -> int {
     if (!h) throw ArgError("null handle");
     if (!h->m.assembled) h->m.assemble();
+    h->m.tune_stiffness();
     if (h->m.all_fixed()) return ML_OK;  // nothing to tune
     return h->m.op.autotune(nrhs, chosen);
   });
