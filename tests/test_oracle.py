@@ -150,6 +150,40 @@ def test_dot_matches_the_plain_sum_for_every_length_around_the_chunk_size():
         assert abs(oracle.dot(a, b) - plain) <= 1e-12 * (1 + abs(plain))
 
 
+_B = float(2 ** 53)  # B + 1 rounds to B (ties to even); B + 2 is exact
+
+
+@pytest.mark.parametrize("a,expected", [
+    # lanes: a[0], a[8] accumulate in lane 0 and a[4], a[12] in lane 4, so lane 0 is
+    # B - B = 0, lane 4 is 1 + 1 = 2 and the dot is 2. A sequential sum (B, B + 1 = B,
+    # 0, 1) and four lanes (lane 0: B + 1 - B + 1 = 1) both give 1.
+    ([_B, 0, 0, 0, 1, 0, 0, 0, -_B, 0, 0, 0, 1, 0, 0, 0], 2.0),
+    # the combination tree ((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7)): the left half is
+    # B, the right half 1 + 1 = 2, the dot B + 2. Lanes summed in a line (B + 1 = B twice),
+    # paired as (s0 + s4) + (s1 + s5) + ... (B + 1 = B), a sequential sum and four lanes
+    # (B + 1 = B) all give B.
+    ([_B, 0, 0, 0, 1, 1, 0, 0], _B + 2),
+    # the tail after the last full chunk is summed on its own, in order from the first
+    # element (B, B + 1 = B, B + 1 = B), and added to the chunks' tree last: B. Summed from its
+    # end (1 + 1 = 2, 2 + B) it would give B + 2.
+    ([0, 0, 0, 0, 0, 0, 0, 0, _B, 1, 1], _B),
+    # the tail's own sum (1 + 1 = 2) meets the chunk's B only at the end: B + 2. Folding the
+    # tail into lanes 0 and 1 gives (B + 1) + 1 = B.
+    ([_B, 0, 0, 0, 0, 0, 0, 0, 1, 1], _B + 2),
+])
+def test_dot_known_answers_fix_the_eight_lane_order(a, expected):
+    """solve.rs:100-110: eight lane sums over the full chunks, combined as
+    ((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7)), plus the tail summed in order. Every
+    expected value is derived by hand in the case's comment, and each case is built so that
+    the plausible other orders, named there, round to a different answer."""
+    a = np.asarray(a, dtype=float)
+    assert oracle.dot(a, np.ones_like(a)) == expected
+    assert oracle.dot(np.ones_like(a), a) == expected
+    # products, not the operands, are summed: scaling one operand by 2 and the other by 1/2
+    # is exact and leaves every product, hence the dot, unchanged
+    assert oracle.dot(2.0 * a, np.full_like(a, 0.5)) == expected
+
+
 def test_graph_lists_each_neighbour_once():
     """iterative.rs:285-293."""
     g = oracle.Graph.from_elements(4, [[0, 1, 2], [1, 2, 3]])
