@@ -48,6 +48,16 @@ def sweep_bytes(n, nnz_strict, col_bytes=4):
     return nnz_strict * (4 + col_bytes) + n * (4 + 8 + 8 + 8 + 8)
 
 
+def fused_sweep_bytes(n, nnz_strict, sweeps):
+    """Algorithmic bytes of one launch of the fused sweeps (k_sweeps_fused: every sweep of one
+    triangular system): the FP32 coefficients and 16-bit LDS slots of the strict triangle once,
+    the row lengths (1 byte), the FP64 right-hand side, reciprocal diagonal and result once,
+    and each of the sweeps - 1 intermediate iterates written once and read once. The staging
+    of each block's external columns re-reads values other blocks wrote, and the sentinel
+    reset of the other system's levels is bookkeeping: both count as traffic, not here."""
+    return nnz_strict * (4 + 2) + n * (1 + 8 + 8 + 8) + (sweeps - 1) * n * (8 + 8)
+
+
 def trisolve_bytes(n, nnz_l, val):
     """Algorithmic bytes of one triangular solve with L (or L^T): entries + row pointers +
     right-hand side + solution."""
@@ -665,7 +675,9 @@ def main():
              "precondition": "precond64" if head == "fp64" else "precond32"}
     spmv_idx, col_bytes = models[head].index_bytes()
     col_bytes = col_bytes or 4
-    pre_bytes = (2 * sweeps * sweep_bytes(n, nnz_l - n, col_bytes) if sweeps
+    fused = models[head].sweeps_fused() if sweeps else 0  # register capacity, 0: per launch
+    pre_bytes = (2 * fused_sweep_bytes(n, nnz_l - n, sweeps) if fused else
+                 2 * sweeps * sweep_bytes(n, nnz_l - n, col_bytes) if sweeps
                  else 2 * trisolve_bytes(n, nnz_l, mval))
     bytes_per = {"spmv": spmv_bytes(n, nnz_full, aval, spmv_idx), "precondition": pre_bytes}
     kernels = {}
@@ -677,16 +689,23 @@ def main():
                          "bytes_per_launch": bytes_per[name],
                          "achieved_GBs": round(achieved, 1),
                          "share_of_step": round(ms / 1e3 / elapsed, 3)}
-    kernels["precondition"]["form"] = (f"{sweeps} Jacobi sweeps per triangular system "
-                                       f"(k_sweep, {2 * sweeps} launches, {col_bytes}-byte "
-                                       f"columns)" if sweeps else
-                                       "exact dataflow triangular solves (k_trisolve, 2 launches)")
+    kernels["precondition"]["form"] = (
+        f"{sweeps} Jacobi sweeps per triangular system, all in one launch per system "
+        f"(k_sweeps_fused, 2 launches, {fused} coefficients per row in registers)" if fused else
+        f"{sweeps} Jacobi sweeps per triangular system (k_sweep, {2 * sweeps} launches, "
+        f"{col_bytes}-byte columns)" if sweeps else
+        "exact dataflow triangular solves (k_trisolve, 2 launches)")
     ic_ms, ic_cnt = prof["ic0"]
     kernels["ic0_factor"] = {"ms_per_step": round(ic_ms / args.steps, 2),
                              "share_of_step": round(ic_ms / 1e3 / elapsed, 3)}
     dominant = max(("spmv", "precondition"), key=lambda k: kernels[k]["share_of_step"])
     d = kernels[dominant]
-    if dominant == "precondition" and sweeps:
+    if dominant == "precondition" and fused:
+        # the application is two launches of k_sweeps_fused, one per triangular system
+        per_ms = d["avg_ms"] / 2
+        one = fused_sweep_bytes(n, nnz_l - n, sweeps)
+        name, achieved, per_launch = "k_sweeps_fused", one / (per_ms * 1e-3) / 1e9, one
+    elif dominant == "precondition" and sweeps:
         # the application is 2 x sweeps launches of one kernel, k_sweep: its roofline is that
         # of one launch (one sweep), timed as the application's events over its launch count
         per_ms = d["avg_ms"] / (2 * sweeps)

@@ -155,7 +155,7 @@ void ml_pattern_free(ml_pattern* p);
  * (offsets[n+1], cols[]; ascending columns, diagonal last) and checks each bound they rely
  * on: index ranges, block capacities, each row's entries covered once and in order, and a
  * topological processing order of the triangular-solve blocks; likewise the two Jacobi-sweep
- * layouts (ML_MIXED_JACOBI). ML_INVALID_ARGUMENT with the
+ * layouts (ML_MIXED_JACOBI) and, where the pattern fits them, the fused ones. ML_INVALID_ARGUMENT with the
  * failed check in ml_last_error otherwise. stats[8] (optional): forward and backward block
  * counts, their largest external and internal entry counts (forward, then backward), the
  * DAG depth and the longest row. With lvalues (the factor, lower-CSR order), r and z
@@ -163,6 +163,15 @@ void ml_pattern_free(ml_pattern* p);
  * in the device's block order and summation order. */
 ml_status ml_layout_verify(int64_t n, const int64_t* offsets, const int64_t* cols,
                            const double* lvalues, const double* r, double* z, int64_t* stats);
+/* Diagnostics (no counterpart in the reference; host only): builds and checks the fused
+ * Jacobi-sweep layouts of the pattern (ML_INVALID_ARGUMENT, with the reason, when the pattern
+ * does not fit them) and, with lvalues, r and z non-null, applies `sweeps` sweeps of each
+ * triangular system on the host through those layouts, staging every value the way the fused
+ * kernel does, with the factor rounded to FP32 (z = P^T P r). stats[4] (optional): blocks, the
+ * longest row, the most external columns of a lower and of an upper block. */
+ml_status ml_layout_sweeps(int64_t n, const int64_t* offsets, const int64_t* cols,
+                           const double* lvalues, const double* r, int sweeps, double* z,
+                           int64_t* stats);
 
 /* ------------------------------------------------------------------ Pcg on a given Csr */
 
@@ -214,6 +223,19 @@ ml_status ml_pcg_set_wait_limit(ml_pcg* h, int64_t ticks);
  * current count. */
 ml_status ml_pcg_set_sweeps(ml_pcg* h, int sweeps);
 int ml_pcg_sweeps(const ml_pcg* h);
+/* No counterpart in the reference: how the sweeps are launched. fused 1 (the default): all
+ * sweeps of a triangular system in one launch of a persistent kernel that keeps each row's
+ * coefficients in registers, so the factor is read once per system instead of once per sweep,
+ * wherever the pattern fits it (rows of at most 252 strict entries; blocks of 256 rows that
+ * read at most 1792 distinct columns outside themselves), in the variant ml_pcg_autotune
+ * timed fastest (before that, the one that holds every row in registers); 16, 24, 32, 48 or
+ * 80: the fused kernel keeping that many coefficients per row in registers (a longer row
+ * re-reads the rest each sweep; 16 keeps registers for 8 waves per SIMD); 0: one launch per
+ * sweep. Every form gives bit-identical results. ml_pcg_sweeps_fused returns the register
+ * capacity of the fused kernel applying the handle's sweeps, 0 when they are launched one by
+ * one (fused 0, no sweeps, or a pattern the fused kernel does not fit). */
+ml_status ml_pcg_set_sweeps_fused(ml_pcg* h, int fused);
+int ml_pcg_sweeps_fused(const ml_pcg* h);
 
 /* ------------------------------------------------------------------ the static model */
 
@@ -236,15 +258,18 @@ ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elemen
 /* ml_pcg_set_sweeps / ml_pcg_sweeps for the model's operator. */
 ml_status ml_model_set_sweeps(ml_model* m, int sweeps);
 int ml_model_sweeps(const ml_model* m);
+/* ml_pcg_set_sweeps_fused / ml_pcg_sweeps_fused for the model's operator. */
+ml_status ml_model_set_sweeps_fused(ml_model* m, int fused);
+int ml_model_sweeps_fused(const ml_model* m);
 /* Bytes per column index the kernels read: out[0] the SpMV's (2: 16-bit offsets from the row,
- * when the matrix band fits; 4: int32), out[1] the sweeps' (2 or 4; 0 before any sweep layout
- * is built). */
+ * when the matrix band fits; 4: int32), out[1] the sweeps' (2 or 4, always 2 in the fused
+ * kernel, whose 16-bit indices name LDS slots; 0 before any sweep layout is built). */
 ml_status ml_model_index_bytes(const ml_model* m, int32_t* out);
-/* Diagnostics (no counterpart in the reference): out[6] = workgroups per CU that the runtime's
+/* Diagnostics (no counterpart in the reference): out[8] = workgroups per CU that the runtime's
  * occupancy figure reports and that the dataflow grids use, for the forward triangular solve,
- * the backward one and IC(0): one fewer than the largest count every workgroup of which was
- * resident at once in three co-residency probes of the kernel itself. Zeros for a model
- * without free DOFs. */
+ * the backward one, IC(0) and the fused sweeps (zeros until fused sweep layouts are built):
+ * one fewer than the largest count every workgroup of which was resident at once in three
+ * co-residency probes of the kernel itself. Zeros for a model without free DOFs. */
 ml_status ml_model_residency(const ml_model* m, int32_t* out);
 /* Recompute the element stiffnesses and K_ff values on the device (after ml_model_set_young). */
 ml_status ml_model_assemble(ml_model* m);

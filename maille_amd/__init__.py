@@ -41,10 +41,13 @@ EXPORTS = (
     "ml_stiffness_batch_noalloc", "ml_quotient64",
     "ml_pattern_build", "ml_pattern_n", "ml_pattern_nnz", "ml_pattern_get",
     "ml_pattern_ntargets", "ml_pattern_graph", "ml_pattern_free", "ml_layout_verify",
+    "ml_layout_sweeps",
     "ml_pcg_create", "ml_pcg_solve", "ml_pcg_solve_device", "ml_pcg_multiply",
     "ml_pcg_precondition", "ml_pcg_factor", "ml_pcg_destroy", "ml_pcg_trace",
     "ml_pcg_set_wait_limit", "ml_pcg_set_sweeps", "ml_pcg_sweeps", "ml_model_set_sweeps",
     "ml_model_sweeps", "ml_model_index_bytes", "ml_model_residency",
+    "ml_pcg_set_sweeps_fused", "ml_pcg_sweeps_fused", "ml_model_set_sweeps_fused",
+    "ml_model_sweeps_fused",
     "ml_model_create", "ml_model_assemble", "ml_model_set_young", "ml_model_factor",
     "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_stresses_device",
     "ml_model_nodal_stresses",
@@ -180,6 +183,7 @@ def lib():
             "ml_pattern_graph": (_int, [_P, _I, _I]),
             "ml_pattern_free": (None, [_P]),
             "ml_layout_verify": (_int, [_i64, _I, _I, _D, _D, _D, _I]),
+            "ml_layout_sweeps": (_int, [_i64, _I, _I, _D, _D, _int, _D, _I]),
             "ml_pcg_create": (_int, [_i64, _I, _I, _D, _int, _PP]),
             "ml_pcg_solve": (_int, [_P, _D, _D, _i64, C.c_double, _i64, _S]),
             "ml_pcg_solve_device": (_int, [_P, _P, _P, C.c_double, _i64, _S]),
@@ -195,6 +199,10 @@ def lib():
             "ml_model_sweeps": (_int, [_P]),
             "ml_model_index_bytes": (_int, [_P, _I32]),
             "ml_model_residency": (_int, [_P, _I32]),
+            "ml_pcg_set_sweeps_fused": (_int, [_P, _int]),
+            "ml_pcg_sweeps_fused": (_int, [_P]),
+            "ml_model_set_sweeps_fused": (_int, [_P, _int]),
+            "ml_model_sweeps_fused": (_int, [_P]),
             "ml_model_create": (_int, [_i64, _D, _i64, _int, _I, _D, _D, _U8, _D, _int, _int,
                                        _PP, _S]),
             "ml_model_assemble": (_int, [_P]),
@@ -468,6 +476,30 @@ def layout_verify(offsets, cols, l=None, r=None):
     return (out, z) if z is not None else out
 
 
+def _fused_code(fused):
+    return int(fused) if not isinstance(fused, bool) else (1 if fused else 0)
+
+
+def layout_sweeps(offsets, cols, l=None, r=None, sweeps=1):
+    """Builds and checks, on the host, the fused Jacobi-sweep layouts of this lower CSR
+    pattern (ValueError when it does not fit them); returns their statistics (blocks, longest
+    row, most external columns of a lower and of an upper block) and, given the factor values
+    ``l`` and ``r``, the sweeps applied on the host through those layouts, with the factor
+    rounded to FP32."""
+    off, cols = _ai64(offsets), _ai64(cols)
+    stats = np.zeros(4, dtype=np.int64)
+    z = None
+    if l is not None and r is not None:
+        l, r = _f64(l), _f64(r)
+        z = np.zeros(len(off) - 1)
+    _check(lib().ml_layout_sweeps(len(off) - 1, _i(off), _i(cols),
+                                  _d(l) if z is not None else None,
+                                  _d(r) if z is not None else None, int(sweeps),
+                                  _d(z) if z is not None else None, _i(stats)))
+    out = dict(zip(("blocks", "width", "max_ext_lower", "max_ext_upper"), (int(v) for v in stats)))
+    return (out, z) if z is not None else out
+
+
 class Pcg(_Handle):
     """IC(0)-preconditioned CG on the device (iterative.rs:194-248). ``precision`` FP64 is the
     drop-in; MIXED runs an FP32 operator inside FP64 refinement."""
@@ -529,6 +561,16 @@ class Pcg(_Handle):
 
     def sweeps(self) -> int:
         return lib().ml_pcg_sweeps(self._h)
+
+    def set_sweeps_fused(self, fused):
+        """All sweeps of a triangular system in one launch (True, the default, where the
+        pattern fits the fused kernel; or an int register capacity 16, 24, 32, 48, 80 for that
+        variant) or one launch per sweep (False); the results are identical."""
+        _check(lib().ml_pcg_set_sweeps_fused(self._h, _fused_code(fused)))
+
+    def sweeps_fused(self) -> int:
+        """Register capacity of the fused kernel applying the sweeps (0: one launch each)."""
+        return lib().ml_pcg_sweeps_fused(self._h)
 
 
 
@@ -622,13 +664,21 @@ class Model(_Handle):
     def sweeps(self) -> int:
         return lib().ml_model_sweeps(self._h)
 
+    def set_sweeps_fused(self, fused):
+        """Pcg.set_sweeps_fused for the model's operator."""
+        _check(lib().ml_model_set_sweeps_fused(self._h, _fused_code(fused)))
+
+    def sweeps_fused(self) -> int:
+        return lib().ml_model_sweeps_fused(self._h)
+
     def residency(self):
         """{kernel: (reported, used)} workgroups per CU of the dataflow kernels: the runtime's
-        occupancy figure and the probed count the grids use."""
-        out = (C.c_int32 * 6)()
+        occupancy figure and the probed count the grids use (fused sweeps: zeros until their
+        layouts are built)."""
+        out = (C.c_int32 * 8)()
         _check(lib().ml_model_residency(self._h, out))
         return {k: (out[2 * i], out[2 * i + 1])
-                for i, k in enumerate(("forward_solve", "backward_solve", "ic0"))}
+                for i, k in enumerate(("forward_solve", "backward_solve", "ic0", "fused_sweeps"))}
 
     def index_bytes(self):
         """(SpMV, sweeps) bytes per column index (2 or 4; sweeps 0 before their layout)."""

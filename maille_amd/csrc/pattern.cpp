@@ -814,6 +814,178 @@ std::string check_sweep_layouts(const LowerPattern& lp, const SweepLayout& lower
   return std::string();
 }
 
+namespace {
+
+// Position of entry q of row i (lane t = i - b*256 of block b) in a fused layout.
+inline i64 fused_at(const FusedSweepLayout& f, i64 b, i64 t, i64 q) {
+  return f.eoff[b] + (q >> 2) * 4 * FUSED_ROWS + 4 * t + (q & 3);
+}
+
+// Rows listed by `row_entries(i, put)` (put(col, src), ascending column) with lengths `len`.
+template <class F>
+std::string fused_layout(i64 n, const std::vector<i64>& len, F row_entries,
+                         FusedSweepLayout* out) {
+  const i64 R = FUSED_ROWS;
+  FusedSweepLayout& f = *out;
+  f.blocks = (n + R - 1) / R;
+  f.width = 0;
+  for (i64 i = 0; i < n; ++i) f.width = std::max<int>(f.width, (int)len[i]);
+  if (f.width > FUSED_WIDTH) return "row too long for the fused sweep kernel";
+  f.elen.resize(n);
+  for (i64 i = 0; i < n; ++i) f.elen[i] = (uint8_t)len[i];
+  std::vector<i64> eoff(f.blocks + 1, 0);
+  for (i64 b = 0; b < f.blocks; ++b) {
+    i64 w = 0;
+    for (i64 i = b * R; i < std::min(n, (b + 1) * R); ++i) w = std::max(w, len[i]);
+    eoff[b + 1] = eoff[b] + ((w + 3) / 4) * 4 * R;
+  }
+  if (eoff[f.blocks] >= (i64)INT32_MAX) return "strict triangle too large for 32-bit offsets";
+  // each block's distinct columns outside it, ascending
+  std::vector<std::vector<int32_t>> ext(f.blocks);
+  parallel_ranges(f.blocks, [&](i64 b0, i64 b1) {
+    for (i64 b = b0; b < b1; ++b) {
+      std::vector<int32_t>& x = ext[b];
+      for (i64 i = b * R; i < std::min(n, (b + 1) * R); ++i)
+        row_entries(i, [&](i64 c, i64) {
+          if (c < b * R || c >= (b + 1) * R) x.push_back((int32_t)c);
+        });
+      std::sort(x.begin(), x.end());
+      x.erase(std::unique(x.begin(), x.end()), x.end());
+    }
+  });
+  f.xoff.assign(f.blocks + 1, 0);
+  f.max_ext = 0;
+  for (i64 b = 0; b < f.blocks; ++b) {
+    f.max_ext = std::max<int>(f.max_ext, (int)ext[b].size());
+    f.xoff[b + 1] = f.xoff[b] + (int32_t)ext[b].size();
+  }
+  if (f.max_ext > FUSED_EXT) return "a block reads too many external columns for the fused sweep kernel";
+  f.xcol.resize(f.xoff[f.blocks]);
+  f.eoff.assign(eoff.begin(), eoff.end());
+  f.esrc.assign(eoff[f.blocks], -1);
+  f.eidx.assign(eoff[f.blocks], 0);
+  parallel_ranges(f.blocks, [&](i64 b0, i64 b1) {
+    for (i64 b = b0; b < b1; ++b) {
+      const std::vector<int32_t>& x = ext[b];
+      std::copy(x.begin(), x.end(), f.xcol.begin() + f.xoff[b]);
+      const i64 w = (eoff[b + 1] - eoff[b]) / R;
+      for (i64 i = b * R; i < std::min(n, (b + 1) * R); ++i) {
+        const i64 t = i - b * R;
+        i64 q = 0;
+        row_entries(i, [&](i64 c, i64 src) {
+          const i64 at = fused_at(f, b, t, q++);
+          f.esrc[at] = (int32_t)src;
+          if (c >= b * R && c < (b + 1) * R)
+            f.eidx[at] = (uint16_t)(c - b * R);
+          else
+            f.eidx[at] = (uint16_t)(R + (std::lower_bound(x.begin(), x.end(), (int32_t)c) - x.begin()));
+        });
+        for (; q < w; ++q) f.eidx[fused_at(f, b, t, q)] = (uint16_t)t;
+      }
+    }
+  });
+  return std::string();
+}
+
+}  // namespace
+
+std::string fused_sweep_layouts(const LowerPattern& lp, FusedSweepLayout* lower,
+                                FusedSweepLayout* upper) {
+  const i64 n = lp.n();
+  if (lp.nnz() >= (i64)INT32_MAX) return "factor too large for 32-bit sources";
+  std::vector<i64> len(n);
+  for (i64 i = 0; i < n; ++i) len[i] = lp.offsets[i + 1] - lp.offsets[i] - 1;
+  std::string err = fused_layout(n, len, [&](i64 i, auto put) {
+    for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) put(lp.cols[p], p);
+  }, lower);
+  if (!err.empty()) return err;
+  TransposeLayout t = transpose_layout(lp);
+  for (i64 j = 0; j < n; ++j) len[j] = t.offsets[j + 1] - t.offsets[j];
+  return fused_layout(n, len, [&](i64 j, auto put) {
+    for (i64 q = t.offsets[j]; q < t.offsets[j + 1]; ++q) put(t.rows[q], t.src[q]);
+  }, upper);
+}
+
+std::string check_fused_sweep_layouts(const LowerPattern& lp, const FusedSweepLayout& lower,
+                                      const FusedSweepLayout& upper) {
+  const i64 n = lp.n(), nnz = lp.nnz(), R = FUSED_ROWS;
+  TransposeLayout tr = transpose_layout(lp);
+  for (int u = 0; u < 2; ++u) {
+    const FusedSweepLayout& f = u ? upper : lower;
+    if (f.blocks != (n + R - 1) / R || (i64)f.eoff.size() != f.blocks + 1 || f.eoff[0] != 0 ||
+        (size_t)f.eoff[f.blocks] != f.esrc.size() || f.esrc.size() != f.eidx.size() ||
+        (i64)f.elen.size() != n || (i64)f.xoff.size() != f.blocks + 1 || f.xoff[0] != 0 ||
+        (size_t)f.xoff[f.blocks] != f.xcol.size())
+      return "fused sweep layout arrays of the wrong size";
+    for (i64 b = 0; b < f.blocks; ++b) {
+      const i64 span = (i64)f.eoff[b + 1] - f.eoff[b];
+      if (span < 0 || span % (4 * R)) return "fused sweep block of a bad width";
+      const i64 x0 = f.xoff[b], nx = (i64)f.xoff[b + 1] - x0;
+      if (nx < 0 || nx > FUSED_EXT) return "fused sweep block with too many external columns";
+      for (i64 e = 0; e < nx; ++e) {
+        const i64 c = f.xcol[x0 + e];
+        if (c < 0 || c >= n || (e > 0 && c <= f.xcol[x0 + e - 1]))
+          return "fused sweep external columns not ascending and distinct";
+        // a lower block reads earlier blocks only, an upper one later blocks only: both are
+        // processed before it
+        if (u ? c < (b + 1) * R : c >= b * R) return "fused sweep external column in a block processed later";
+      }
+      for (i64 i = b * R; i < std::min(n, (b + 1) * R); ++i) {
+        const i64 t = i - b * R, len = f.elen[i];
+        const i64 want = u ? tr.offsets[i + 1] - tr.offsets[i] : lp.offsets[i + 1] - lp.offsets[i] - 1;
+        if (len != want || 4 * R * ((len + 3) / 4) > span) return "fused sweep row of the wrong length";
+        for (i64 q = 0; q < span / R; ++q) {
+          const i64 at = fused_at(f, b, t, q);
+          if (q >= len) {
+            if (f.esrc[at] != -1 || f.eidx[at] != t) return "fused sweep padding not neutral";
+            continue;
+          }
+          const i64 c = u ? tr.rows[tr.offsets[i] + q] : lp.cols[lp.offsets[i] + q];
+          const i64 src = u ? tr.src[tr.offsets[i] + q] : lp.offsets[i] + q;
+          if (f.esrc[at] != src || src < 0 || src >= nnz) return "fused sweep entry out of order";
+          const i64 s = f.eidx[at];
+          const bool own = c >= b * R && c < (b + 1) * R;
+          if (own ? s != c - b * R : (s < R || s - R >= nx || f.xcol[x0 + s - R] != c))
+            return "fused sweep entry in the wrong LDS slot";
+          if (u ? c <= i : c >= i) return "fused sweep entry outside its triangle";
+        }
+      }
+    }
+  }
+  return std::string();
+}
+
+void fused_sweeps_host(const LowerPattern& lp, const FusedSweepLayout& lower,
+                       const FusedSweepLayout& upper, const double* l, const double* r, int k,
+                       double* z) {
+  const i64 n = lp.n(), R = FUSED_ROWS;
+  std::vector<double> rd(n), y(n), prev(n), next(n);
+  for (i64 i = 0; i < n; ++i) rd[i] = 1.0 / (double)(float)l[lp.offsets[i + 1] - 1];
+  auto system = [&](const FusedSweepLayout& f, const double* rhs, double* out) {
+    for (i64 i = 0; i < n; ++i) prev[i] = rhs[i] * rd[i];
+    std::vector<double> buf(R + FUSED_EXT);
+    for (int j = 1; j <= k; ++j) {
+      for (i64 b = 0; b < f.blocks; ++b) {
+        // the kernel's LDS block: own rows, then the external columns, of the previous level
+        for (i64 t = 0; t < R; ++t) buf[t] = b * R + t < n ? prev[b * R + t] : 0.0;
+        for (i64 e = f.xoff[b]; e < f.xoff[b + 1]; ++e) buf[R + e - f.xoff[b]] = prev[f.xcol[e]];
+        for (i64 i = b * R; i < std::min(n, (b + 1) * R); ++i) {
+          double s = 0.0;
+          for (i64 q = 0; q < f.elen[i]; ++q) {
+            const i64 at = fused_at(f, b, i - b * R, q);
+            s = s + (double)(float)l[f.esrc[at]] * buf[f.eidx[at]];
+          }
+          next[i] = (rhs[i] - s) * rd[i];
+        }
+      }
+      prev.swap(next);
+    }
+    std::copy(prev.begin(), prev.end(), out);
+  };
+  system(lower, r, y.data());
+  system(upper, y.data(), z);
+}
+
 void layout_precondition(const DeviceLayouts& d, const double* l, const double* r, double* z) {
   const i64 n = d.n;
   std::vector<double> y(n);

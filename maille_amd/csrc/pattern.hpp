@@ -154,6 +154,49 @@ std::string sweep_layouts(const LowerPattern& lp, SweepLayout* lower, SweepLayou
 std::string check_sweep_layouts(const LowerPattern& lp, const SweepLayout& lower,
                                 const SweepLayout& upper);
 
+// One strict triangle of the factor laid out for the fused sweep kernel, which runs every
+// sweep of a block of FUSED_ROWS consecutive rows in one workgroup over a block of LDS that
+// holds the block's own rows (slots 0..255, row b*256 + s in slot s) and the distinct columns
+// outside the block that its rows read (slots 256.., ascending column). Block b's entries start
+// at eoff[b] and are stored in quads: entry q of row b*256 + t at
+// eoff[b] + (q >> 2) * 4 * FUSED_ROWS + 4 t + (q & 3), so that a lane reads four entries with
+// one load. Each entry has its coefficient's index in the lower-CSR value array (esrc; -1 for
+// padding) and its LDS slot (eidx; padding names the row's own slot and is never summed). Rows
+// list the same entries as SweepLayout, in ascending column: l_ij, j < i (lower) or l_ji, j > i
+// (upper).
+constexpr int FUSED_ROWS = 256;
+constexpr int FUSED_EXT = 1792;  // external columns a block may read
+constexpr int FUSED_WIDTH = 252; // longest row (entries per row are counted in 8 bits)
+struct FusedSweepLayout {
+  int64_t blocks = 0;
+  int width = 0;  // the longest row
+  std::vector<int32_t> eoff;  // blocks + 1
+  std::vector<int32_t> esrc;
+  std::vector<uint16_t> eidx;
+  std::vector<uint8_t> elen;  // per row
+  std::vector<int32_t> xoff;  // blocks + 1
+  std::vector<int32_t> xcol;
+  int max_ext = 0;  // the most external columns of any block
+};
+// Both triangles' fused layouts; an empty string, or the reason they do not fit (a row longer
+// than FUSED_WIDTH, a block reading more than FUSED_EXT external columns, 32-bit offsets).
+std::string fused_sweep_layouts(const LowerPattern& lp, FusedSweepLayout* lower,
+                                FusedSweepLayout* upper);
+// Verifies every index the fused kernel reads: each row's entries are its strict triangle in
+// ascending column with the right sources, every slot names that column's LDS place, the
+// external lists are ascending, distinct and outside the block, and every external column of
+// a block lies in a block the kernel processes earlier (lower blocks for the lower triangle,
+// higher ones for the upper); empty string when all hold.
+std::string check_fused_sweep_layouts(const LowerPattern& lp, const FusedSweepLayout& lower,
+                                      const FusedSweepLayout& upper);
+// The fused kernel's sweeps restated on the host, block by block in its processing order and
+// staging order: k sweeps of L y = r from y = r * rd, then k of L^T z = y from z = y * rd, with
+// FP32 coefficients (l rounded to float), rd = 1 / (double)(float)l_ii and FP64 sums in each
+// row's order. l holds the factor's values in lower-CSR order.
+void fused_sweeps_host(const LowerPattern& lp, const FusedSweepLayout& lower,
+                       const FusedSweepLayout& upper, const double* l, const double* r, int k,
+                       double* z);
+
 // The preconditioner z = (L L^T)^-1 r computed on the host through the triangular-solve
 // layouts, block by block in processing order, each row summed in the order the device sums
 // it. l holds the factor's values in lower-CSR order.

@@ -86,6 +86,9 @@ inline unsigned grid_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 
 
 // per-block capacities of the triangular-solve layouts, as the kernel's LDS holds them
 constexpr TriCaps kTriCaps = {mlk::TRI_XCAP, mlk::TRI_ICAP, mlk::TRI_UCAP};
+// the fused sweep layouts are built for the kernel's block and LDS capacities
+static_assert(FUSED_ROWS == mlk::SWEEP_ROWS && FUSED_EXT == mlk::SWEEP_EXT,
+              "fused sweep layout and kernel disagree");
 
 // Kernel-duration accounting with HIP events on the handle's stream (enabled on demand).
 enum ProfKind {
@@ -229,6 +232,17 @@ int ic0_resident(void (*kernel)(TArgs...), hipStream_t stream) {
   });
 }
 
+template <typename... TArgs>
+int fused_resident(void (*kernel)(TArgs...), hipStream_t stream) {
+  return resident_per_cu((const void*)kernel, mlk::SWEEP_ROWS, stream,
+                         [&](int grid, int* probe, long long limit) {
+    kernel<<<grid, mlk::SWEEP_ROWS, 0, stream>>>(0, -1, 0, nullptr, nullptr, nullptr, nullptr,
+                                                 nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                 nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                 probe, limit);
+  });
+}
+
 // Launches a persistent dataflow kernel of `threads`-thread workgroups on a grid whose
 // co-residency a probe of the same kernel has shown (resident_per_cu). An ordinary launch: the
 // runtime's admission check of a cooperative one rests on the occupancy figure the probe
@@ -342,6 +356,41 @@ struct Operator {
   DBuf<int> d_sw_dsrc;
   DBuf<float> d_sw_diag;
   DBuf<double> d_sw_rd, w_s1, w_s2, w_s3, w_y0;  // w_s3: z0 = y rd, w_y0: r rd
+  // the same sweeps laid out for k_sweeps_fused (all sweeps of a system in one launch), when
+  // the pattern fits its capacities, and the levels each system publishes between sweeps
+  struct FusedDev {
+    int nblk = 0;
+    int64_t entries = 0;
+    DBuf<int> eoff, esrc, xoff, xcol;
+    DBuf<uint8_t> elen;
+    DBuf<uint16_t> eidx;
+    DBuf<float> eval;
+    void set(const FusedSweepLayout& f, hipStream_t st) {
+      nblk = (int)f.blocks;
+      entries = (int64_t)f.esrc.size();
+      eoff.upload_vec(f.eoff, st);
+      esrc.upload_vec(f.esrc, st);
+      xoff.upload_vec(f.xoff, st);
+      xcol.upload_vec(f.xcol, st);
+      elen.upload_vec(f.elen, st);
+      eidx.upload_vec(f.eidx, st);
+      eval.alloc(f.esrc.size());
+    }
+  } fz_lo, fz_up;
+  bool have_fused = false;  // fused layouts built (the pattern fits them)
+  bool fused_on = true;     // the caller's choice (ml_pcg_set_sweeps_fused)
+  // variants of k_sweeps_fused: coefficients per row kept in registers (a longer row re-reads
+  // the rest each sweep) and the waves per SIMD its registers must allow
+  struct FusedVariant {
+    int len, waves;
+  };
+  static constexpr int FUSED_VARIANTS = 5;
+  static constexpr FusedVariant kFused[FUSED_VARIANTS] = {{16, 8}, {24, 1}, {32, 1}, {48, 1}, {80, 1}};
+  int fused_var = 1;        // the variant in use
+  int fused_width = 0;      // the longest row of the fused layouts
+  int fused_per_cu = 0, full_fused = 0, fused_grid = 0;
+  DBuf<double> w_lev_lo, w_lev_up;  // sweeps - 1 levels of n values per system
+  int lev_sweeps = 0;               // the sweep count the level buffers are sized for
   bool have_sweep = false;
   int sweeps = 0;  // Jacobi sweeps per triangular system; 0: the exact dataflow solves
   int sweep_batch = 8, spmv_batch = 8;  // entries per batch of loads (autotuned: 8 or 16)
@@ -350,8 +399,9 @@ struct Operator {
   int full_tri64 = 0, full_tri32 = 0, full_ic = 0, full_ic64 = 0;
   int cap_tri64 = 0, cap_tri32 = 0;  // dataflow grids before the cap by block count
   int tri_per_cu = 1, ic_per_cu = 1, ic64_per_cu = 1;  // probed co-resident workgroups per CU
-  // reported and probed workgroups per CU: forward solve, backward solve, IC(0) (512-entry rows)
-  int residency[6] = {0, 0, 0, 0, 0, 0};
+  // reported and probed workgroups per CU: forward solve, backward solve, IC(0) (512-entry
+  // rows), fused sweeps (the tighter of the two systems; zeros without fused layouts)
+  int residency[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int rhs_per_pass = 256;            // right-hand sides per batched pass (autotuned)
   int tri_grid64 = 0, tri_grid32 = 0, ic_grid = 0, ic_grid64 = 0;
   int shrinks = 0;   // grid halvings after a stalled dataflow launch
@@ -473,11 +523,89 @@ struct Operator {
       d_sw_diag.alloc(n);
       d_sw_rd.alloc(n);
       for (auto* b : {&w_s1, &w_s2, &w_s3, &w_y0}) b->alloc(n);
+      // the fused layouts, where the pattern fits them (else every sweep is its own launch)
+      FusedSweepLayout flo, fup;
+      if (fused_sweep_layouts(lp, &flo, &fup).empty()) {
+        err = check_fused_sweep_layouts(lp, flo, fup);
+        if (!err.empty()) throw ArgError(err);
+        fz_lo.set(flo, stream);
+        fz_up.set(fup, stream);
+        fused_width = std::max(flo.width, fup.width);
+        have_fused = true;
+        select_fused(default_fused());
+      }
       HIP_CHECK(hipStreamSynchronize(stream));  // the host layouts go out of scope
       have_sweep = true;
       if (factored) gather_sweeps();
     }
     sweeps = k;
+    if (have_fused && k > 1 && lev_sweeps != k) {
+      for (auto* b : {&w_lev_lo, &w_lev_up}) b->alloc((size_t)(k - 1) * n);
+      lev_sweeps = k;
+      fill_levels();
+    }
+  }
+
+  typedef void (*FusedKernel)(int, int, int, const int*, const uint8_t*, const float*,
+                              const uint16_t*, const int*, const int*, const double*,
+                              const double*, double*, double*, double*, const double*, double*,
+                              const mlk::PcgScalars*, int*, long long);
+  // k_sweeps_fused in one of its variants (default: the one in use)
+  FusedKernel fused_kernel(bool upper, int var = -1) const {
+    switch (var < 0 ? fused_var : var) {
+      case 0: return upper ? mlk::k_sweeps_fused<16, true, 8> : mlk::k_sweeps_fused<16, false, 8>;
+      case 1: return upper ? mlk::k_sweeps_fused<24, true> : mlk::k_sweeps_fused<24, false>;
+      case 2: return upper ? mlk::k_sweeps_fused<32, true> : mlk::k_sweeps_fused<32, false>;
+      case 3: return upper ? mlk::k_sweeps_fused<48, true> : mlk::k_sweeps_fused<48, false>;
+      default: return upper ? mlk::k_sweeps_fused<80, true> : mlk::k_sweeps_fused<80, false>;
+    }
+  }
+  // the variant that keeps every row in registers (the widest one for longer rows)
+  int default_fused() const {
+    for (int v = 1; v < FUSED_VARIANTS; ++v)
+      if (kFused[v].len >= fused_width) return v;
+    return FUSED_VARIANTS - 1;
+  }
+  // puts a variant in use: its grid is bounded by probes of both of its kernels
+  void select_fused(int var) {
+    fused_var = var;
+    fused_per_cu = std::min(fused_resident(fused_kernel(false), stream),
+                            fused_resident(fused_kernel(true), stream));
+    residency[6] = occupancy_per_cu((const void*)fused_kernel(false), mlk::SWEEP_ROWS);
+    residency[7] = fused_per_cu;
+    full_fused = std::min<int>(fz_lo.nblk, fused_per_cu * device_cus());
+    set_grids();
+  }
+  bool fused_active() const { return have_fused && fused_on && sweeps > 0; }
+  // coefficients per row the fused kernel in use keeps in registers; 0 when it is not in use
+  int fused_capacity() const { return fused_active() ? kFused[fused_var].len : 0; }
+  // 0: one launch per sweep; 1: the fused kernel in its chosen variant; 16, 24, 32, 48, 80:
+  // the fused kernel with that register capacity
+  void set_fused(int fused) {
+    if (fused == 0 || fused == 1) {
+      fused_on = fused == 1;
+      return;
+    }
+    for (int v = 0; v < FUSED_VARIANTS; ++v)
+      if (kFused[v].len == fused) {
+        fused_on = true;
+        if (have_fused) {
+          select_fused(v);
+          fill_levels();
+        }
+        return;
+      }
+    throw ArgError("fused must be 0, 1 or a register capacity of 16, 24, 32, 48 or 80");
+  }
+
+  // every level of both systems back to the sentinel (before a solve: a launch that gave up
+  // may have left published values behind)
+  void fill_levels() {
+    if (lev_sweeps < 2) return;
+    const int64_t m = (int64_t)(lev_sweeps - 1) * n;
+    for (auto* b : {&w_lev_lo, &w_lev_up})
+      mlk::k_fill_bits<double><<<grid_for(m), 256, 0, stream>>>(b->p, m, mlk::Bits<double>::S);
+    HIP_CHECK(hipGetLastError());
   }
 
   // the FP32 factor's values into the sweep layouts, and the reciprocals of its diagonal
@@ -485,6 +613,11 @@ struct Operator {
     for (SweepDev* t : {&sw_lo, &sw_up})
       mlk::k_gather<float, float><<<grid_for(t->entries), 256, 0, stream>>>(
           t->entries, t->ssrc.p, d_l32.p, t->sval.p);
+    if (have_fused)
+      for (FusedDev* t : {&fz_lo, &fz_up})
+        if (t->entries)
+          mlk::k_gather<float, float><<<grid_for(t->entries), 256, 0, stream>>>(
+              t->entries, t->esrc.p, d_l32.p, t->eval.p);
     mlk::k_gather<float, float><<<grid_for(n), 256, 0, stream>>>((int64_t)n, d_sw_dsrc.p, d_l32.p,
                                                                  d_sw_diag.p);
     mlk::k_recip64<float><<<grid_for(n), 256, 0, stream>>>((int64_t)n, d_sw_diag.p, d_sw_rd.p);
@@ -566,6 +699,7 @@ struct Operator {
     tri_grid32 = std::max(1, full_tri32 >> shrinks);
     ic_grid = std::max(1, full_ic >> shrinks);
     ic_grid64 = std::max(1, full_ic64 >> shrinks);
+    fused_grid = std::max(1, full_fused >> shrinks);
   }
 
   // After a dataflow launch stalled (its bounded waits expired): halve the dataflow grids so
@@ -709,6 +843,20 @@ struct Operator {
     double* a = w_s1.p;
     double* b = w_s2.p;
     double* z0 = w_s3.p;
+    if (fused_active()) {
+      // one launch per system; each resets the other's levels to the sentinel as it goes
+      double* lo = k > 1 ? w_lev_lo.p : nullptr;
+      double* up = k > 1 ? w_lev_up.p : nullptr;
+      fused_kernel(false)<<<fused_grid, mlk::SWEEP_ROWS, 0, stream>>>(
+          n, fz_lo.nblk, k, fz_lo.eoff.p, fz_lo.elen.p, fz_lo.eval.p, fz_lo.eidx.p, fz_lo.xoff.p,
+          fz_lo.xcol.p, d_sw_rd.p, r, lo, up, a, nullptr, nullptr, S, d_flags.p, wait_ticks);
+      fused_kernel(true)<<<fused_grid, mlk::SWEEP_ROWS, 0, stream>>>(
+          n, fz_up.nblk, k, fz_up.eoff.p, fz_up.elen.p, fz_up.eval.p, fz_up.eidx.p, fz_up.xoff.p,
+          fz_up.xcol.p, d_sw_rd.p, a, up, lo, z, dot, partial, S, d_flags.p, wait_ticks);
+      HIP_CHECK(hipGetLastError());
+      prof.end(P_TRI32, ev, stream);
+      return;
+    }
     auto sweep = [&](const SweepDev& t, const double* rhs, const double* in, double* out,
                      double* out_rd, bool last) {
       const double* dv = last ? dot : nullptr;
@@ -758,6 +906,7 @@ struct Operator {
   void apply_precondition(long long* trace_fw = nullptr, long long* trace_bw = nullptr) {
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 2, 0);
     if (sweeps > 0 && !trace_fw && !trace_bw) {
+      if (fused_active()) fill_levels();
       apply_sweeps(w_r.p, w_z.p, nullptr);
     } else {
       fill_sentinel<double>(w_y.p);
@@ -824,7 +973,10 @@ struct Operator {
     const int vb = grid_for(n);
     double* part = w_partial.p;
     const bool jac = sweeps > 0;
+    // the fused sweeps form r * rd themselves; the per-launch ones read it from the update
+    const bool y0 = jac && !fused_active();
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);  // abort flag
+    if (fused_active()) fill_levels();
     init_scalars(S, maxit, a32, a32);
     if (warm) {
       spmv<double>(x, w_ax.p, nullptr, nullptr);
@@ -858,8 +1010,8 @@ struct Operator {
         hipEvent_t ev = prof.begin(stream);
         mlk::k_update_xr<TA, TM><<<vb, 256, 0, stream>>>(n, S, x, r, p, ap, rlow,
                                                          jac ? nullptr : y, jac ? nullptr : z,
-                                                         part, jac ? d_sw_rd.p : nullptr,
-                                                         jac ? w_y0.p : nullptr);
+                                                         part, y0 ? d_sw_rd.p : nullptr,
+                                                         y0 ? w_y0.p : nullptr);
         prof.end(P_VEC, ev, stream);
         mlk::k_converged<<<1, mlk::SUM_THREADS, 0, stream>>>(part, vb, S);
         if (a32) {
@@ -870,7 +1022,7 @@ struct Operator {
         if (jac) {
           // the residual replacement (ML_MIXED) rewrites r after k_update_xr: then y0 is
           // formed by the first sweep itself
-          apply_sweeps(rin, z, S, r, part, a32 ? nullptr : w_y0.p);
+          apply_sweeps(rin, z, S, r, part, a32 || !y0 ? nullptr : w_y0.p);
         } else {
           precondition(rin, y, z, S);
           ev = prof.begin(stream);
@@ -972,7 +1124,7 @@ struct Operator {
         if (t < best_t) { best_t = t; best_b = bsz; }
       }
       spmv_batch = best_b;
-      if (have_sweep) {
+      if (have_sweep && !(have_fused && fused_on)) {  // the fused kernel has no batches
         const int k0 = sweeps;
         sweeps = std::max(1, sweeps);
         best_t = 1e30f;
@@ -1025,6 +1177,42 @@ struct Operator {
       }
       rhs_per_pass = best_k;
     }
+    if (have_fused && fused_on) {
+      // the fused kernel's variant: the fastest application, among the variants no wider
+      // than the one that holds every row in registers
+      const int k0 = sweeps;
+      if (sweeps < 1) set_sweeps(4);
+      hipEvent_t a, b;
+      HIP_CHECK(hipEventCreate(&a));
+      HIP_CHECK(hipEventCreate(&b));
+      float best_t = 1e30f;
+      int best_v = fused_var;
+      for (int v = 0; v <= default_fused(); ++v) {
+        select_fused(v);
+        fill_levels();
+        float total = 0.0f;
+        for (int r = 0; r <= 3; ++r) {  // the first run warms up
+          HIP_CHECK(hipEventRecord(a, stream));
+          apply_sweeps(w_r.p, w_z.p, nullptr);
+          HIP_CHECK(hipEventRecord(b, stream));
+          HIP_CHECK(hipEventSynchronize(b));
+          float t = 0.0f;
+          HIP_CHECK(hipEventElapsedTime(&t, a, b));
+          if (r > 0) total += t;
+        }
+        if (read_flag(0)) total = 1e30f;  // a wait gave up: not a candidate
+        mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);
+        if (total < best_t) {
+          best_t = total;
+          best_v = v;
+        }
+      }
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+      select_fused(best_v);
+      fill_levels();
+      if (k0 != sweeps) set_sweeps(k0);
+    }
     if (precision == ML_MIXED_JACOBI && !sweeps_pinned) {
       DBuf<double> tb, tx;
       tb.alloc(n);
@@ -1033,7 +1221,7 @@ struct Operator {
       HIP_CHECK(hipGetLastError());
       double best_t = 1e300;
       int best_k = sweeps;
-      for (int k : {0, 2, 3, 4, 6, 8}) {
+      for (int k : {0, 2, 3, 4, 5, 6, 8}) {
         set_sweeps(k);
         mlk::PcgScalars hS;
         HIP_CHECK(hipStreamSynchronize(stream));
@@ -2689,6 +2877,9 @@ ml_status ml_layout_verify(int64_t n, const int64_t* offsets, const int64_t* col
     SweepLayout lo, up;
     if (err.empty()) err = sweep_layouts(lp, &lo, &up);
     if (err.empty()) err = check_sweep_layouts(lp, lo, up);
+    FusedSweepLayout flo, fup;
+    if (err.empty() && fused_sweep_layouts(lp, &flo, &fup).empty())
+      err = check_fused_sweep_layouts(lp, flo, fup);
     if (!err.empty()) throw ArgError(err);
     if (stats) {
       stats[0] = (int64_t)d.fw.bstart.size() - 1;
@@ -2701,6 +2892,27 @@ ml_status ml_layout_verify(int64_t n, const int64_t* offsets, const int64_t* col
       stats[7] = d.max_row;
     }
     if (lvalues && r && z) layout_precondition(d, lvalues, r, z);
+    return ML_OK;
+  });
+}
+
+ml_status ml_layout_sweeps(int64_t n, const int64_t* offsets, const int64_t* cols,
+                           const double* lvalues, const double* r, int sweeps, double* z,
+                           int64_t* stats) {
+  return guarded([&]() -> int {
+    if (sweeps < 1 || sweeps > 64) throw ArgError("sweeps must be in 1..64");
+    LowerPattern lp = pattern_from_csr(n, offsets, cols);
+    FusedSweepLayout flo, fup;
+    std::string err = fused_sweep_layouts(lp, &flo, &fup);
+    if (err.empty()) err = check_fused_sweep_layouts(lp, flo, fup);
+    if (!err.empty()) throw ArgError(err);
+    if (stats) {
+      stats[0] = flo.blocks;
+      stats[1] = std::max(flo.width, fup.width);
+      stats[2] = flo.max_ext;
+      stats[3] = fup.max_ext;
+    }
+    if (lvalues && r && z) fused_sweeps_host(lp, flo, fup, lvalues, r, sweeps, z);
     return ML_OK;
   });
 }
@@ -2895,10 +3107,31 @@ ml_status ml_model_set_sweeps(ml_model* h, int sweeps) {
 
 int ml_model_sweeps(const ml_model* h) { return h ? h->m.op.sweeps : 0; }
 
+ml_status ml_pcg_set_sweeps_fused(ml_pcg* h, int fused) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    h->op.set_fused(fused);
+    return ML_OK;
+  });
+}
+
+int ml_pcg_sweeps_fused(const ml_pcg* h) { return h ? h->op.fused_capacity() : 0; }
+
+ml_status ml_model_set_sweeps_fused(ml_model* h, int fused) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    if (h->m.all_fixed()) return ML_OK;  // no free system
+    h->m.op.set_fused(fused);
+    return ML_OK;
+  });
+}
+
+int ml_model_sweeps_fused(const ml_model* h) { return h ? h->m.op.fused_capacity() : 0; }
+
 ml_status ml_model_residency(const ml_model* h, int32_t* out) {
   return guarded([&]() -> int {
     if (!h || !out) throw ArgError("null argument");
-    for (int k = 0; k < 6; ++k) out[k] = h->m.op.residency[k];
+    for (int k = 0; k < 8; ++k) out[k] = h->m.op.residency[k];
     return ML_OK;
   });
 }
@@ -2908,7 +3141,7 @@ ml_status ml_model_index_bytes(const ml_model* h, int32_t* out) {
     if (!h || !out) throw ArgError("null argument");
     const Operator& op = h->m.op;
     out[0] = op.spmv_d16 ? 2 : 4;
-    out[1] = !op.have_sweep ? 0 : (op.sw_lo.d16 && op.sw_up.d16 ? 2 : 4);
+    out[1] = !op.have_sweep ? 0 : (op.fused_active() || (op.sw_lo.d16 && op.sw_up.d16)) ? 2 : 4;
     return ML_OK;
   });
 }

@@ -101,20 +101,30 @@ def test_mixed_preconditioner_applies_the_fp32_factor(gpu, precision):
     assert np.abs(l - l64).max() <= 1e-5 * np.abs(l64).max()
 
 
-@pytest.mark.parametrize("sweeps", [1, 2, 3, 4, 7])
-def test_jacobi_sweeps_match_their_restatement(gpu, sweeps):
+SWEEP_CASES = [("jittered_cube_8", k, f) for k in (1, 2, 3, 4, 7) for f in (1, 16, 0)] + \
+    [("cantilever_tet10_small", k, f) for k in (1, 3) for f in (1, 16, 48, 0)]
+
+
+@pytest.mark.parametrize("name,sweeps,fused", SWEEP_CASES)
+def test_jacobi_sweeps_match_their_restatement(gpu, name, sweeps, fused):
     """ML_MIXED_JACOBI's preconditioner: `sweeps` Jacobi sweeps of L y = r and of L^T z = y on
-    the FP32 factor, bit for bit the float64 restatement, on every call; more sweeps approach
-    the exact triangular solves."""
+    the FP32 factor, bit for bit the float64 restatement, on every call, whether all sweeps of
+    a system run in one launch (fused 1: the variant that keeps every row in registers, 24
+    coefficients on the cube and 80 on the Tet10 mesh, whose rows reach 146 and re-read their
+    tails; 16 and 48: variants that keep fewer and re-read more) or one launch per sweep (0);
+    more sweeps approach the exact triangular solves."""
     from sweeps import jacobi_precondition
-    p = PROBLEMS["jittered_cube_8"]
+    p = PROBLEMS[name]
     s = _oracle_system(p)
     off, cols, vals = s.csr()
-    print(f"jacobi sweeps={sweeps} n={s.n}", flush=True)
+    print(f"jacobi {name} sweeps={sweeps} fused={fused} n={s.n}", flush=True)
     pcg = gpu.Pcg(gpu.Csr(off, cols, vals), precision=gpu.MIXED_JACOBI)
     assert pcg.sweeps() == 4
     pcg.set_sweeps(sweeps)
+    pcg.set_sweeps_fused(fused)
     assert pcg.sweeps() == sweeps
+    capacity = {1: 24 if name == "jittered_cube_8" else 80}.get(fused, fused)
+    assert pcg.sweeps_fused() == capacity
     r = np.cos(0.11 * np.arange(s.n))
     z = pcg.precondition(r)
     l, _ = pcg.factor()
@@ -123,6 +133,30 @@ def test_jacobi_sweeps_match_their_restatement(gpu, sweeps):
     exact = oracle.Csr.from_arrays(off, cols, vals).precondition(l, r)
     far = rel_l2(jacobi_precondition(off, cols, l, r, 1), exact)
     assert rel_l2(jacobi_precondition(off, cols, l, r, 12), exact) < far
+
+
+def test_fused_and_per_launch_sweeps_solve_identically(gpu):
+    """The two forms of the sweeps give the same preconditioner bit for bit and the same
+    per-block partial sums of r.z, so whole MIXED_JACOBI solves agree exactly: the same
+    iterates, iteration counts and residuals."""
+    p = PROBLEMS["jittered_cube_8"]
+    print("fused vs per-launch solve", flush=True)
+    out = {}
+    with gpu.Model(p, precision=gpu.MIXED_JACOBI) as m:
+        m.set_sweeps(3)
+        for fused in (1, 16, 0):
+            m.set_sweeps_fused(fused)
+            assert m.sweeps_fused() == {1: 24}.get(fused, fused)
+            u, r, info = m.solve()
+            assert info["device_retries"] == 0
+            out[fused] = (u, r, info)
+    u1, r1, i1 = out[0]
+    for fused in (1, 16):
+        u2, r2, i2 = out[fused]
+        assert np.array_equal(u1, u2) and np.array_equal(r1, r2)
+        assert i1["iterations"] == i2["iterations"] and i1["residual"] == i2["residual"]
+    ref = oracle.model_solve(p, iterative=True)
+    assert rel_l2(u1, ref["displacements"]) <= 1e-8
 
 
 def test_fp64_handle_refuses_sweeps(gpu):
@@ -503,11 +537,14 @@ def test_dataflow_grids_are_bounded_by_a_co_residency_probe(gpu):
     three probes in a row resident at once), one workgroup per CU below that, hence below the
     runtime's occupancy figure; solves on those grids need no retry."""
     p = PROBLEMS["jittered_cube_8"]
-    for precision in ("FP64", "MIXED_IC0"):
+    for precision in ("FP64", "MIXED_IC0", "MIXED_JACOBI"):
         with gpu.Model(p, precision=getattr(gpu, precision)) as m:
             res = m.residency()
             print(f"{precision}: reported/probed workgroups per CU {res}", flush=True)
             for kernel, (reported, used) in res.items():
+                if kernel == "fused_sweeps" and precision != "MIXED_JACOBI":
+                    assert (reported, used) == (0, 0)  # no sweep layouts: nothing probed
+                    continue
                 assert 1 <= used and (used < reported or used == 1), kernel
             u, _, info = m.solve()
             assert info["device_retries"] == 0

@@ -1,6 +1,7 @@
 """The Jacobi-sweep restatement (tests/sweeps.py) pinned by what the mathematics fixes, and
 the library's sweep layouts checked on the host."""
 import numpy as np
+import pytest
 import scipy.sparse as sp
 import scipy.sparse.linalg as spl
 
@@ -100,3 +101,49 @@ def test_library_sweep_layouts_pass_their_host_checks():
     off, cols, l, n = _factored(p)
     stats = ma.layout_verify(off, cols)
     assert stats["depth"] > 1
+
+
+def _fp32_factor(p):
+    off, cols, l, n = _factored(p)
+    return off, cols, l.astype(np.float32).astype(np.float64), n
+
+
+def test_fused_layouts_reproduce_the_restatement_on_the_host():
+    """The fused kernel's layouts (blocks of 256 rows, each row's entries as LDS slots of the
+    block's own rows and of its distinct external columns), applied on the host the way the
+    kernel stages them, give the restatement bit for bit: every slot names the right value.
+    The Tet10 mesh has rows of up to 146 strict entries and blocks reading many external
+    columns."""
+    import maille_amd as ma
+    for p in (meshes.jittered_cube(8),
+              meshes.cantilever_tet10((12, 3, 3), (60.0, 15.0, 15.0))):
+        off, cols, l, n = _fp32_factor(p)
+        r = np.cos(0.11 * np.arange(n))
+        for k in (1, 2, 3, 5):
+            stats, z = ma.layout_sweeps(off, cols, l, r, k)
+            assert np.array_equal(z, jacobi_precondition(off, cols, l, r, k)), (p.name, k)
+        assert stats["blocks"] == (n + 255) // 256
+        assert 0 < stats["max_ext_lower"] <= 1792 and 0 < stats["max_ext_upper"] <= 1792
+        widths = np.diff(off) - 1
+        assert stats["width"] == max(widths.max(), np.bincount(
+            cols[np.setdiff1d(np.arange(len(cols)), off[1:] - 1)], minlength=n).max())
+
+
+def test_fused_layout_refuses_a_row_too_long_for_it():
+    """A row of more than 252 strict entries does not fit the fused kernel's 8-bit row
+    lengths: the layout is refused with the reason, and the library then keeps one launch per
+    sweep (ml_layout_verify still accepts the pattern)."""
+    import maille_amd as ma
+    n = 254
+    # an arrow: the last row couples to every other row (253 strict entries, within the
+    # triangular solves' capacity of 256 distinct columns per block)
+    off = np.zeros(n + 1, dtype=np.int64)
+    cols = []
+    for i in range(n):
+        row = [i] if i < n - 1 else list(range(n))
+        cols += row
+        off[i + 1] = len(cols)
+    cols = np.array(cols, dtype=np.int64)
+    with pytest.raises(ValueError, match="too long"):
+        ma.layout_sweeps(off, cols)
+    assert ma.layout_verify(off, cols)["max_row"] == n
