@@ -164,12 +164,13 @@ void ml_pattern_free(ml_pattern* p);
 ml_status ml_layout_verify(int64_t n, const int64_t* offsets, const int64_t* cols,
                            const double* lvalues, const double* r, double* z, int64_t* stats);
 /* Diagnostics (no counterpart in the reference; host only): builds and checks the fused
- * Jacobi-sweep layouts of the pattern (ML_INVALID_ARGUMENT, with the reason, when the pattern
- * does not fit them) and, with lvalues, r and z non-null, applies `sweeps` sweeps of each
- * triangular system on the host through those layouts, staging every value the way the fused
- * kernel does, with the factor rounded to FP32 (z = P^T P r). stats[4] (optional): blocks, the
- * longest row, the most external columns of a lower and of an upper block. */
-ml_status ml_layout_sweeps(int64_t n, const int64_t* offsets, const int64_t* cols,
+ * Jacobi-sweep layouts of the pattern in blocks of `rows` rows (256 or 512;
+ * ML_INVALID_ARGUMENT, with the reason, when the pattern does not fit them) and, with
+ * lvalues, r and z non-null, applies `sweeps` sweeps of each triangular system on the host
+ * through those layouts, staging every value the way the fused kernel does, with the factor
+ * rounded to FP32 (z = P^T P r). stats[4] (optional): blocks, the longest row, the most
+ * external columns of a lower and of an upper block. */
+ml_status ml_layout_sweeps(int64_t n, const int64_t* offsets, const int64_t* cols, int rows,
                            const double* lvalues, const double* r, int sweeps, double* z,
                            int64_t* stats);
 
@@ -228,12 +229,14 @@ int ml_pcg_sweeps(const ml_pcg* h);
  * coefficients in registers, so the factor is read once per system instead of once per sweep,
  * wherever the pattern fits it (rows of at most 252 strict entries; blocks of 256 rows that
  * read at most 1792 distinct columns outside themselves), in the variant ml_pcg_autotune
- * timed fastest (before that, the one that holds every row in registers); 16, 24, 32, 48 or
- * 80: the fused kernel keeping that many coefficients per row in registers (a longer row
- * re-reads the rest each sweep; 16 keeps registers for 8 waves per SIMD); 0: one launch per
- * sweep. Every form gives bit-identical results. ml_pcg_sweeps_fused returns the register
- * capacity of the fused kernel applying the handle's sweeps, 0 when they are launched one by
- * one (fused 0, no sweeps, or a pattern the fused kernel does not fit). */
+ * timed fastest (before that, one row per lane with every row in registers); a variant code
+ * 100 r + c: the fused kernel with r rows per lane (1, or 2 in blocks of 512 rows for rows of
+ * at most 32 strict entries) and c coefficients per row held in registers (a longer row
+ * re-reads the rest each sweep): 116 (registers bounded for 8 waves per SIMD), 124, 132, 148,
+ * 180, 216 (bounded for 5 waves), 224, 232; 0: one launch per sweep. Every form gives bit-identical results.
+ * ml_pcg_sweeps_fused returns the code of the fused variant applying the handle's sweeps, 0
+ * when they are launched one by one (fused 0, no sweeps, or a pattern the fused kernel does
+ * not fit). */
 ml_status ml_pcg_set_sweeps_fused(ml_pcg* h, int fused);
 int ml_pcg_sweeps_fused(const ml_pcg* h);
 

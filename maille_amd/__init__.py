@@ -183,7 +183,7 @@ def lib():
             "ml_pattern_graph": (_int, [_P, _I, _I]),
             "ml_pattern_free": (None, [_P]),
             "ml_layout_verify": (_int, [_i64, _I, _I, _D, _D, _D, _I]),
-            "ml_layout_sweeps": (_int, [_i64, _I, _I, _D, _D, _int, _D, _I]),
+            "ml_layout_sweeps": (_int, [_i64, _I, _I, _int, _D, _D, _int, _D, _I]),
             "ml_pcg_create": (_int, [_i64, _I, _I, _D, _int, _PP]),
             "ml_pcg_solve": (_int, [_P, _D, _D, _i64, C.c_double, _i64, _S]),
             "ml_pcg_solve_device": (_int, [_P, _P, _P, C.c_double, _i64, _S]),
@@ -480,19 +480,19 @@ def _fused_code(fused):
     return int(fused) if not isinstance(fused, bool) else (1 if fused else 0)
 
 
-def layout_sweeps(offsets, cols, l=None, r=None, sweeps=1):
+def layout_sweeps(offsets, cols, l=None, r=None, sweeps=1, rows=256):
     """Builds and checks, on the host, the fused Jacobi-sweep layouts of this lower CSR
-    pattern (ValueError when it does not fit them); returns their statistics (blocks, longest
-    row, most external columns of a lower and of an upper block) and, given the factor values
-    ``l`` and ``r``, the sweeps applied on the host through those layouts, with the factor
-    rounded to FP32."""
+    pattern in blocks of ``rows`` (256 or 512) rows (ValueError when it does not fit them);
+    returns their statistics (blocks, longest row, most external columns of a lower and of an
+    upper block) and, given the factor values ``l`` and ``r``, the sweeps applied on the host
+    through those layouts, with the factor rounded to FP32."""
     off, cols = _ai64(offsets), _ai64(cols)
     stats = np.zeros(4, dtype=np.int64)
     z = None
     if l is not None and r is not None:
         l, r = _f64(l), _f64(r)
         z = np.zeros(len(off) - 1)
-    _check(lib().ml_layout_sweeps(len(off) - 1, _i(off), _i(cols),
+    _check(lib().ml_layout_sweeps(len(off) - 1, _i(off), _i(cols), int(rows),
                                   _d(l) if z is not None else None,
                                   _d(r) if z is not None else None, int(sweeps),
                                   _d(z) if z is not None else None, _i(stats)))
@@ -564,12 +564,13 @@ class Pcg(_Handle):
 
     def set_sweeps_fused(self, fused):
         """All sweeps of a triangular system in one launch (True, the default, where the
-        pattern fits the fused kernel; or an int register capacity 16, 24, 32, 48, 80 for that
-        variant) or one launch per sweep (False); the results are identical."""
+        pattern fits the fused kernel; or an int variant code 100 * rows per lane + register
+        capacity: 116, 124, 132, 148, 180, 216, 224, 232) or one launch per sweep (False); the
+        results are identical."""
         _check(lib().ml_pcg_set_sweeps_fused(self._h, _fused_code(fused)))
 
     def sweeps_fused(self) -> int:
-        """Register capacity of the fused kernel applying the sweeps (0: one launch each)."""
+        """Variant code of the fused kernel applying the sweeps (0: one launch each)."""
         return lib().ml_pcg_sweeps_fused(self._h)
 
 

@@ -1522,9 +1522,9 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
 }
 
 // ------------------------------------------------------------------ fused Jacobi sweeps
-constexpr int SWEEP_ROWS = 256;  // rows of a block: one per lane of the workgroup
-constexpr int SWEEP_EXT = 1792;  // LDS slots for a block's external columns
-constexpr int SWEEP_STAGE = 4;   // external values a lane has in flight while staging them
+constexpr int SWEEP_LANES = 256;  // threads of a workgroup; rows of an entry sub-block
+constexpr int SWEEP_EXT = 1792;   // LDS slots for a block's external columns
+constexpr int SWEEP_STAGE = 4;    // external values a lane has in flight while staging them
 
 // All k Jacobi sweeps of one triangular system in one launch, with the arithmetic of k
 // launches of k_sweep (each row summed in ascending column from 0, FP32 coefficients widened
@@ -1532,122 +1532,151 @@ constexpr int SWEEP_STAGE = 4;   // external values a lane has in flight while s
 // factor's coefficients are read from memory once instead of k times.
 //
 // A persistent grid (every workgroup co-resident, shown by a probe of this kernel) takes the
-// blocks of 256 rows at positions w, w + G, ...: the lower system (L y = r) in ascending block
-// order and the upper one (REV, L^T z = y) in descending order, so that every column a block
-// reads from outside lies in a block at a lower position. Lane t owns row b*256 + t and keeps
-// its first MAXLEN coefficients and LDS slots in registers (a longer row re-reads the rest each
-// sweep). The sweeps run over an LDS block holding the previous iterate at the block's own rows
-// (slots 0..255) and at its distinct external columns (slots 256..): sweep 1 starts from
-// rhs * rd; the result of sweep j < k is published (device-scope) into level j of lev, and the
-// external columns of that level are then awaited (bounded waits on the sentinel, as in the
-// triangular solves) and staged for sweep j + 1. The last sweep writes out and, with partial,
-// the block's share of dot.out. Each row of refill (the other system's levels, which nothing
-// reads in this launch) is reset to the sentinel for that system's next launch.
+// blocks of R = 256 RPL rows at positions w, w + G, ...: the lower system (L y = r) in
+// ascending block order and the upper one (REV, L^T z = y) in descending order, so that every
+// column a block reads from outside lies in a block at a lower position. Lane t owns rows
+// b R + 256 h + t (h < RPL) and keeps the first MAXLEN coefficients and LDS slots of each in
+// registers (a longer row re-reads the rest each sweep). The sweeps run over an LDS block
+// holding the previous iterate at the block's own rows (slots 0..R-1) and at its distinct
+// external columns (slots R..; the columns themselves are kept in LDS too): sweep 1 starts
+// from rhs * rd; the result of sweep j < k is
+// published (device-scope) into level j of lev, and the external columns of that level are
+// then awaited (bounded waits on the sentinel, as in the triangular solves) and staged for
+// sweep j + 1. The last sweep writes out and, with partial, each 256-row sub-block's share of
+// dot.out (the partials of k_sweep's grid). Each row of refill (the other system's levels,
+// which nothing reads in this launch) is reset to the sentinel for that system's next launch.
 // MINW: waves per SIMD the register allocation must allow (1: no bound).
 // nblk < 0: co-residency probe of the grid.
-template <int MAXLEN, bool REV, int MINW = 1>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MINW))) void k_sweeps_fused(
-    int n, int nblk, int k, const int* __restrict__ eoff, const uint8_t* __restrict__ elen,
-    const float* __restrict__ eval, const uint16_t* __restrict__ eidx,
-    const int* __restrict__ xoff, const int* __restrict__ xcol, const double* __restrict__ rd,
-    const double* __restrict__ rhs, double* __restrict__ lev, double* __restrict__ refill,
-    double* __restrict__ out, const double* __restrict__ dot, double* __restrict__ partial,
-    const PcgScalars* __restrict__ S, int* abort_flag, long long limit) {
+template <int MAXLEN, bool REV, int MINW = 1, int RPL = 1>
+__global__ __launch_bounds__(SWEEP_LANES) __attribute__((amdgpu_waves_per_eu(MINW))) void
+k_sweeps_fused(int n, int nblk, int k, const int* __restrict__ eoff,
+               const uint8_t* __restrict__ elen, const float* __restrict__ eval,
+               const uint16_t* __restrict__ eidx, const int* __restrict__ xoff,
+               const int* __restrict__ xcol, const double* __restrict__ rd,
+               const double* __restrict__ rhs, double* __restrict__ lev,
+               double* __restrict__ refill, double* __restrict__ out,
+               const double* __restrict__ dot, double* __restrict__ partial,
+               const PcgScalars* __restrict__ S, int* abort_flag, long long limit) {
 #pragma clang fp contract(off)
   static_assert(MAXLEN % 4 == 0, "entries are held in quads");
+  constexpr int R = SWEEP_LANES * RPL;
   if (nblk < 0) {
     residency_probe(abort_flag, limit);
     return;
   }
   if (S && S->done) return;
-  __shared__ double buf[SWEEP_ROWS + SWEEP_EXT];
+  __shared__ double buf[R + SWEEP_EXT];
+  __shared__ int cols[SWEEP_EXT];
   const int t = threadIdx.x;
+  const int nsub = (n + SWEEP_LANES - 1) / SWEEP_LANES;
   for (int p = blockIdx.x; p < nblk; p += gridDim.x) {
     const int b = REV ? nblk - 1 - p : p;
-    const int i = b * SWEEP_ROWS + t;
-    const bool live = i < n;
-    const int len = live ? (int)elen[i] : 0;
-    // entry q of this row: base + (q >> 2) * 4 * SWEEP_ROWS + (q & 3)
-    const int base = eoff[b] + 4 * t;
-    float v[MAXLEN];
-    uint32_t ix[MAXLEN / 2];  // two 16-bit slots each
+    float v[RPL][MAXLEN];
+    uint32_t ix[RPL][MAXLEN / 2];  // two 16-bit slots each
+    int len[RPL], base[RPL];
+    double ri[RPL], rdi[RPL], y[RPL];
 #pragma unroll
-    for (int g = 0; g < MAXLEN / 4; ++g) {
-      float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      uint2 d = make_uint2(0u, 0u);
-      if (4 * g < len) {
-        f = *reinterpret_cast<const float4*>(eval + base + g * 4 * SWEEP_ROWS);
-        d = *reinterpret_cast<const uint2*>(eidx + base + g * 4 * SWEEP_ROWS);
+    for (int h = 0; h < RPL; ++h) {
+      const int i = b * R + h * SWEEP_LANES + t;
+      len[h] = i < n ? (int)elen[i] : 0;
+      // entry q of row i: base + (q >> 2) * 4 * SWEEP_LANES + (q & 3); a sub-block past the
+      // last one has no live rows and is never read
+      base[h] = eoff[min(b * RPL + h, nsub)] + 4 * t;
+#pragma unroll
+      for (int g = 0; g < MAXLEN / 4; ++g) {
+        float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint2 d = make_uint2(0u, 0u);
+        if (4 * g < len[h]) {
+          f = *reinterpret_cast<const float4*>(eval + base[h] + g * 4 * SWEEP_LANES);
+          d = *reinterpret_cast<const uint2*>(eidx + base[h] + g * 4 * SWEEP_LANES);
+        }
+        v[h][4 * g] = f.x;
+        v[h][4 * g + 1] = f.y;
+        v[h][4 * g + 2] = f.z;
+        v[h][4 * g + 3] = f.w;
+        ix[h][2 * g] = d.x;
+        ix[h][2 * g + 1] = d.y;
       }
-      v[4 * g] = f.x;
-      v[4 * g + 1] = f.y;
-      v[4 * g + 2] = f.z;
-      v[4 * g + 3] = f.w;
-      ix[2 * g] = d.x;
-      ix[2 * g + 1] = d.y;
+      rdi[h] = i < n ? rd[i] : 0.0;
+      ri[h] = i < n ? rhs[i] : 0.0;
+      buf[h * SWEEP_LANES + t] = ri[h] * rdi[h];
     }
-    const double rdi = live ? rd[i] : 0.0;
-    const double ri = live ? rhs[i] : 0.0;
     const int x0 = xoff[b], nx = xoff[b + 1] - x0;
-    buf[t] = ri * rdi;
-    for (int e = t; e < nx; e += SWEEP_ROWS) {
+    // the block's external columns, kept in LDS for every sweep's staging
+    for (int e = t; e < nx; e += SWEEP_LANES) {
       const int c = xcol[x0 + e];
-      buf[SWEEP_ROWS + e] = rhs[c] * rd[c];
+      cols[e] = c;
+      buf[R + e] = rhs[c] * rd[c];
     }
     __syncthreads();
-    double y = 0.0;
     for (int j = 1;; ++j) {
-      // an empty statement that "changes" the row's registers each sweep: otherwise the
+      // an empty statement that "changes" the rows' registers each sweep: otherwise the
       // widened coefficients and the LDS addresses, the same in every sweep, are computed
       // once before the loop and held, which triples the registers per entry
 #pragma unroll
-      for (int q = 0; q < MAXLEN; ++q) asm volatile("" : "+v"(v[q]));
+      for (int h = 0; h < RPL; ++h) {
 #pragma unroll
-      for (int h = 0; h < MAXLEN / 2; ++h) asm volatile("" : "+v"(ix[h]));
-      double s = 0.0;
+        for (int q = 0; q < MAXLEN; ++q) asm volatile("" : "+v"(v[h][q]));
 #pragma unroll
-      for (int q = 0; q < MAXLEN; ++q)
-        if (q < len) s = s + (double)v[q] * buf[(ix[q >> 1] >> (16 * (q & 1))) & 0xffffu];
-      for (int q = MAXLEN; q < len; ++q) {
-        const int at = base + (q >> 2) * 4 * SWEEP_ROWS + (q & 3);
-        s = s + (double)eval[at] * buf[eidx[at]];
+        for (int q = 0; q < MAXLEN / 2; ++q) asm volatile("" : "+v"(ix[h][q]));
       }
-      y = (ri - s) * rdi;
+#pragma unroll
+      for (int h = 0; h < RPL; ++h) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < MAXLEN; ++q)
+          if (q < len[h])
+            s = s + (double)v[h][q] * buf[(ix[h][q >> 1] >> (16 * (q & 1))) & 0xffffu];
+        for (int q = MAXLEN; q < len[h]; ++q) {
+          const int at = base[h] + (q >> 2) * 4 * SWEEP_LANES + (q & 3);
+          s = s + (double)eval[at] * buf[eidx[at]];
+        }
+        y[h] = (ri[h] - s) * rdi[h];
+      }
       if (j >= k) break;
       double* level = lev + (int64_t)(j - 1) * n;
-      if (live) publish(level + i, y);
+#pragma unroll
+      for (int h = 0; h < RPL; ++h) {
+        const int i = b * R + h * SWEEP_LANES + t;
+        if (i < n) publish(level + i, y[h]);
+      }
       __syncthreads();  // every lane has read the previous iterate from LDS
-      buf[t] = y;
-      for (int e0 = 0; e0 < nx; e0 += SWEEP_STAGE * SWEEP_ROWS) {
-        int col[SWEEP_STAGE];
+#pragma unroll
+      for (int h = 0; h < RPL; ++h) buf[h * SWEEP_LANES + t] = y[h];
+      for (int e0 = 0; e0 < nx; e0 += SWEEP_STAGE * SWEEP_LANES) {
+        int c[SWEEP_STAGE];
         unsigned long long got[SWEEP_STAGE];
 #pragma unroll
         for (int u = 0; u < SWEEP_STAGE; ++u) {
-          const int e = e0 + u * SWEEP_ROWS + t;
-          col[u] = e < nx ? xcol[x0 + e] : -1;
+          const int e = e0 + u * SWEEP_LANES + t;
+          c[u] = e < nx ? cols[e] : -1;
         }
 #pragma unroll
         for (int u = 0; u < SWEEP_STAGE; ++u)
-          got[u] = col[u] >= 0 ? __hip_atomic_load(reinterpret_cast<unsigned long long*>(level + col[u]),
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                               : 0ull;
+          got[u] = c[u] >= 0 ? __hip_atomic_load(reinterpret_cast<unsigned long long*>(level + c[u]),
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                             : 0ull;
 #pragma unroll
         for (int u = 0; u < SWEEP_STAGE; ++u)
-          if (col[u] >= 0)
-            buf[SWEEP_ROWS + e0 + u * SWEEP_ROWS + t] =
+          if (c[u] >= 0)
+            buf[R + e0 + u * SWEEP_LANES + t] =
                 got[u] != SENTINEL64 ? __longlong_as_double((long long)got[u])
-                                     : await_value(level + col[u], abort_flag, limit);
+                                     : await_value(level + c[u], abort_flag, limit);
       }
       __syncthreads();
     }
-    if (live) {
-      out[i] = y;
-      for (int m = 0; m + 1 < k; ++m)
-        reinterpret_cast<unsigned long long*>(refill)[(int64_t)m * n + i] = SENTINEL64;
-    }
-    if (partial) {
-      const double bs = block_sum<SWEEP_ROWS>(live && dot ? dot[i] * y : 0.0);
-      if (t == 0) partial[b] = bs;
+#pragma unroll
+    for (int h = 0; h < RPL; ++h) {
+      const int i = b * R + h * SWEEP_LANES + t;
+      if (i < n) {
+        out[i] = y[h];
+        for (int m = 0; m + 1 < k; ++m)
+          reinterpret_cast<unsigned long long*>(refill)[(int64_t)m * n + i] = SENTINEL64;
+      }
+      if (partial) {
+        const double bs = block_sum<SWEEP_LANES>(i < n && dot ? dot[i] * y[h] : 0.0);
+        if (t == 0 && b * RPL + h < nsub) partial[b * RPL + h] = bs;
+      }
     }
     __syncthreads();  // the next block overwrites buf
   }

@@ -816,30 +816,33 @@ std::string check_sweep_layouts(const LowerPattern& lp, const SweepLayout& lower
 
 namespace {
 
-// Position of entry q of row i (lane t = i - b*256 of block b) in a fused layout.
-inline i64 fused_at(const FusedSweepLayout& f, i64 b, i64 t, i64 q) {
-  return f.eoff[b] + (q >> 2) * 4 * FUSED_ROWS + 4 * t + (q & 3);
+// Position of entry q of row i in a fused layout (sub-block i / 256, lane i % 256).
+inline i64 fused_at(const FusedSweepLayout& f, i64 i, i64 q) {
+  return f.eoff[i / FUSED_LANES] + (q >> 2) * 4 * FUSED_LANES + 4 * (i % FUSED_LANES) + (q & 3);
 }
 
-// Rows listed by `row_entries(i, put)` (put(col, src), ascending column) with lengths `len`.
+// Rows listed by `row_entries(i, put)` (put(col, src), ascending column) with lengths `len`,
+// in blocks of R rows.
 template <class F>
-std::string fused_layout(i64 n, const std::vector<i64>& len, F row_entries,
+std::string fused_layout(i64 n, i64 R, const std::vector<i64>& len, F row_entries,
                          FusedSweepLayout* out) {
-  const i64 R = FUSED_ROWS;
+  const i64 L = FUSED_LANES;
   FusedSweepLayout& f = *out;
+  f.rows = (int)R;
   f.blocks = (n + R - 1) / R;
+  const i64 subs = (n + L - 1) / L;
   f.width = 0;
   for (i64 i = 0; i < n; ++i) f.width = std::max<int>(f.width, (int)len[i]);
   if (f.width > FUSED_WIDTH) return "row too long for the fused sweep kernel";
   f.elen.resize(n);
   for (i64 i = 0; i < n; ++i) f.elen[i] = (uint8_t)len[i];
-  std::vector<i64> eoff(f.blocks + 1, 0);
-  for (i64 b = 0; b < f.blocks; ++b) {
+  std::vector<i64> eoff(subs + 1, 0);
+  for (i64 s = 0; s < subs; ++s) {
     i64 w = 0;
-    for (i64 i = b * R; i < std::min(n, (b + 1) * R); ++i) w = std::max(w, len[i]);
-    eoff[b + 1] = eoff[b] + ((w + 3) / 4) * 4 * R;
+    for (i64 i = s * L; i < std::min(n, (s + 1) * L); ++i) w = std::max(w, len[i]);
+    eoff[s + 1] = eoff[s] + ((w + 3) / 4) * 4 * L;
   }
-  if (eoff[f.blocks] >= (i64)INT32_MAX) return "strict triangle too large for 32-bit offsets";
+  if (eoff[subs] >= (i64)INT32_MAX) return "strict triangle too large for 32-bit offsets";
   // each block's distinct columns outside it, ascending
   std::vector<std::vector<int32_t>> ext(f.blocks);
   parallel_ranges(f.blocks, [&](i64 b0, i64 b1) {
@@ -862,25 +865,24 @@ std::string fused_layout(i64 n, const std::vector<i64>& len, F row_entries,
   if (f.max_ext > FUSED_EXT) return "a block reads too many external columns for the fused sweep kernel";
   f.xcol.resize(f.xoff[f.blocks]);
   f.eoff.assign(eoff.begin(), eoff.end());
-  f.esrc.assign(eoff[f.blocks], -1);
-  f.eidx.assign(eoff[f.blocks], 0);
+  f.esrc.assign(eoff[subs], -1);
+  f.eidx.assign(eoff[subs], 0);
   parallel_ranges(f.blocks, [&](i64 b0, i64 b1) {
     for (i64 b = b0; b < b1; ++b) {
       const std::vector<int32_t>& x = ext[b];
       std::copy(x.begin(), x.end(), f.xcol.begin() + f.xoff[b]);
-      const i64 w = (eoff[b + 1] - eoff[b]) / R;
       for (i64 i = b * R; i < std::min(n, (b + 1) * R); ++i) {
-        const i64 t = i - b * R;
+        const i64 s = i / L, w = (eoff[s + 1] - eoff[s]) / L;
         i64 q = 0;
         row_entries(i, [&](i64 c, i64 src) {
-          const i64 at = fused_at(f, b, t, q++);
+          const i64 at = fused_at(f, i, q++);
           f.esrc[at] = (int32_t)src;
           if (c >= b * R && c < (b + 1) * R)
             f.eidx[at] = (uint16_t)(c - b * R);
           else
             f.eidx[at] = (uint16_t)(R + (std::lower_bound(x.begin(), x.end(), (int32_t)c) - x.begin()));
         });
-        for (; q < w; ++q) f.eidx[fused_at(f, b, t, q)] = (uint16_t)t;
+        for (; q < w; ++q) f.eidx[fused_at(f, i, q)] = (uint16_t)(i - b * R);
       }
     }
   });
@@ -889,37 +891,43 @@ std::string fused_layout(i64 n, const std::vector<i64>& len, F row_entries,
 
 }  // namespace
 
-std::string fused_sweep_layouts(const LowerPattern& lp, FusedSweepLayout* lower,
+std::string fused_sweep_layouts(const LowerPattern& lp, int rows, FusedSweepLayout* lower,
                                 FusedSweepLayout* upper) {
   const i64 n = lp.n();
+  if (rows != FUSED_LANES && rows != 2 * FUSED_LANES) return "fused sweep blocks of 256 or 512 rows";
   if (lp.nnz() >= (i64)INT32_MAX) return "factor too large for 32-bit sources";
   std::vector<i64> len(n);
   for (i64 i = 0; i < n; ++i) len[i] = lp.offsets[i + 1] - lp.offsets[i] - 1;
-  std::string err = fused_layout(n, len, [&](i64 i, auto put) {
+  std::string err = fused_layout(n, rows, len, [&](i64 i, auto put) {
     for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) put(lp.cols[p], p);
   }, lower);
   if (!err.empty()) return err;
   TransposeLayout t = transpose_layout(lp);
   for (i64 j = 0; j < n; ++j) len[j] = t.offsets[j + 1] - t.offsets[j];
-  return fused_layout(n, len, [&](i64 j, auto put) {
+  return fused_layout(n, rows, len, [&](i64 j, auto put) {
     for (i64 q = t.offsets[j]; q < t.offsets[j + 1]; ++q) put(t.rows[q], t.src[q]);
   }, upper);
 }
 
 std::string check_fused_sweep_layouts(const LowerPattern& lp, const FusedSweepLayout& lower,
                                       const FusedSweepLayout& upper) {
-  const i64 n = lp.n(), nnz = lp.nnz(), R = FUSED_ROWS;
+  const i64 n = lp.n(), nnz = lp.nnz(), L = FUSED_LANES;
+  const i64 subs = (n + L - 1) / L;
   TransposeLayout tr = transpose_layout(lp);
   for (int u = 0; u < 2; ++u) {
     const FusedSweepLayout& f = u ? upper : lower;
-    if (f.blocks != (n + R - 1) / R || (i64)f.eoff.size() != f.blocks + 1 || f.eoff[0] != 0 ||
-        (size_t)f.eoff[f.blocks] != f.esrc.size() || f.esrc.size() != f.eidx.size() ||
+    const i64 R = f.rows;
+    if ((R != L && R != 2 * L) || f.blocks != (n + R - 1) / R ||
+        (i64)f.eoff.size() != subs + 1 || f.eoff[0] != 0 ||
+        (size_t)f.eoff[subs] != f.esrc.size() || f.esrc.size() != f.eidx.size() ||
         (i64)f.elen.size() != n || (i64)f.xoff.size() != f.blocks + 1 || f.xoff[0] != 0 ||
         (size_t)f.xoff[f.blocks] != f.xcol.size())
       return "fused sweep layout arrays of the wrong size";
+    for (i64 s = 0; s < subs; ++s) {
+      const i64 span = (i64)f.eoff[s + 1] - f.eoff[s];
+      if (span < 0 || span % (4 * L)) return "fused sweep sub-block of a bad width";
+    }
     for (i64 b = 0; b < f.blocks; ++b) {
-      const i64 span = (i64)f.eoff[b + 1] - f.eoff[b];
-      if (span < 0 || span % (4 * R)) return "fused sweep block of a bad width";
       const i64 x0 = f.xoff[b], nx = (i64)f.xoff[b + 1] - x0;
       if (nx < 0 || nx > FUSED_EXT) return "fused sweep block with too many external columns";
       for (i64 e = 0; e < nx; ++e) {
@@ -931,21 +939,21 @@ std::string check_fused_sweep_layouts(const LowerPattern& lp, const FusedSweepLa
         if (u ? c < (b + 1) * R : c >= b * R) return "fused sweep external column in a block processed later";
       }
       for (i64 i = b * R; i < std::min(n, (b + 1) * R); ++i) {
-        const i64 t = i - b * R, len = f.elen[i];
+        const i64 len = f.elen[i], span = (i64)f.eoff[i / L + 1] - f.eoff[i / L];
         const i64 want = u ? tr.offsets[i + 1] - tr.offsets[i] : lp.offsets[i + 1] - lp.offsets[i] - 1;
-        if (len != want || 4 * R * ((len + 3) / 4) > span) return "fused sweep row of the wrong length";
-        for (i64 q = 0; q < span / R; ++q) {
-          const i64 at = fused_at(f, b, t, q);
+        if (len != want || 4 * L * ((len + 3) / 4) > span) return "fused sweep row of the wrong length";
+        for (i64 q = 0; q < span / L; ++q) {
+          const i64 at = fused_at(f, i, q);
           if (q >= len) {
-            if (f.esrc[at] != -1 || f.eidx[at] != t) return "fused sweep padding not neutral";
+            if (f.esrc[at] != -1 || f.eidx[at] != i - b * R) return "fused sweep padding not neutral";
             continue;
           }
           const i64 c = u ? tr.rows[tr.offsets[i] + q] : lp.cols[lp.offsets[i] + q];
           const i64 src = u ? tr.src[tr.offsets[i] + q] : lp.offsets[i] + q;
           if (f.esrc[at] != src || src < 0 || src >= nnz) return "fused sweep entry out of order";
-          const i64 s = f.eidx[at];
+          const i64 sl = f.eidx[at];
           const bool own = c >= b * R && c < (b + 1) * R;
-          if (own ? s != c - b * R : (s < R || s - R >= nx || f.xcol[x0 + s - R] != c))
+          if (own ? sl != c - b * R : (sl < R || sl - R >= nx || f.xcol[x0 + sl - R] != c))
             return "fused sweep entry in the wrong LDS slot";
           if (u ? c <= i : c >= i) return "fused sweep entry outside its triangle";
         }
@@ -958,10 +966,11 @@ std::string check_fused_sweep_layouts(const LowerPattern& lp, const FusedSweepLa
 void fused_sweeps_host(const LowerPattern& lp, const FusedSweepLayout& lower,
                        const FusedSweepLayout& upper, const double* l, const double* r, int k,
                        double* z) {
-  const i64 n = lp.n(), R = FUSED_ROWS;
+  const i64 n = lp.n();
   std::vector<double> rd(n), y(n), prev(n), next(n);
   for (i64 i = 0; i < n; ++i) rd[i] = 1.0 / (double)(float)l[lp.offsets[i + 1] - 1];
   auto system = [&](const FusedSweepLayout& f, const double* rhs, double* out) {
+    const i64 R = f.rows;
     for (i64 i = 0; i < n; ++i) prev[i] = rhs[i] * rd[i];
     std::vector<double> buf(R + FUSED_EXT);
     for (int j = 1; j <= k; ++j) {
@@ -972,7 +981,7 @@ void fused_sweeps_host(const LowerPattern& lp, const FusedSweepLayout& lower,
         for (i64 i = b * R; i < std::min(n, (b + 1) * R); ++i) {
           double s = 0.0;
           for (i64 q = 0; q < f.elen[i]; ++q) {
-            const i64 at = fused_at(f, b, i - b * R, q);
+            const i64 at = fused_at(f, i, q);
             s = s + (double)(float)l[f.esrc[at]] * buf[f.eidx[at]];
           }
           next[i] = (rhs[i] - s) * rd[i];

@@ -155,22 +155,23 @@ std::string check_sweep_layouts(const LowerPattern& lp, const SweepLayout& lower
                                 const SweepLayout& upper);
 
 // One strict triangle of the factor laid out for the fused sweep kernel, which runs every
-// sweep of a block of FUSED_ROWS consecutive rows in one workgroup over a block of LDS that
-// holds the block's own rows (slots 0..255, row b*256 + s in slot s) and the distinct columns
-// outside the block that its rows read (slots 256.., ascending column). Block b's entries start
-// at eoff[b] and are stored in quads: entry q of row b*256 + t at
-// eoff[b] + (q >> 2) * 4 * FUSED_ROWS + 4 t + (q & 3), so that a lane reads four entries with
-// one load. Each entry has its coefficient's index in the lower-CSR value array (esrc; -1 for
-// padding) and its LDS slot (eidx; padding names the row's own slot and is never summed). Rows
-// list the same entries as SweepLayout, in ascending column: l_ij, j < i (lower) or l_ji, j > i
-// (upper).
-constexpr int FUSED_ROWS = 256;
+// sweep of a block of `rows` (256 or 512) consecutive rows in one workgroup over a block of LDS
+// that holds the block's own rows (slots 0..rows-1, row b*rows + s in slot s) and the distinct
+// columns outside the block that its rows read (slots rows.., ascending column). Entries are
+// stored by sub-block of FUSED_LANES rows (one row per lane of the workgroup) in quads: entry q
+// of row 256 s + t at eoff[s] + (q >> 2) * 4 * FUSED_LANES + 4 t + (q & 3), so that a lane
+// reads four entries with one load. Each entry has its coefficient's index in the lower-CSR
+// value array (esrc; -1 for padding) and its LDS slot (eidx; padding names the row's own slot
+// and is never summed). Rows list the same entries as SweepLayout, in ascending column: l_ij,
+// j < i (lower) or l_ji, j > i (upper).
+constexpr int FUSED_LANES = 256;
 constexpr int FUSED_EXT = 1792;  // external columns a block may read
 constexpr int FUSED_WIDTH = 252; // longest row (entries per row are counted in 8 bits)
 struct FusedSweepLayout {
-  int64_t blocks = 0;
-  int width = 0;  // the longest row
-  std::vector<int32_t> eoff;  // blocks + 1
+  int rows = 0;        // rows per block
+  int64_t blocks = 0;  // blocks of `rows` rows
+  int width = 0;       // the longest row
+  std::vector<int32_t> eoff;  // sub-blocks + 1
   std::vector<int32_t> esrc;
   std::vector<uint16_t> eidx;
   std::vector<uint8_t> elen;  // per row
@@ -178,9 +179,10 @@ struct FusedSweepLayout {
   std::vector<int32_t> xcol;
   int max_ext = 0;  // the most external columns of any block
 };
-// Both triangles' fused layouts; an empty string, or the reason they do not fit (a row longer
-// than FUSED_WIDTH, a block reading more than FUSED_EXT external columns, 32-bit offsets).
-std::string fused_sweep_layouts(const LowerPattern& lp, FusedSweepLayout* lower,
+// Both triangles' fused layouts in blocks of `rows` rows; an empty string, or the reason they
+// do not fit (a row longer than FUSED_WIDTH, a block reading more than FUSED_EXT external
+// columns, 32-bit offsets).
+std::string fused_sweep_layouts(const LowerPattern& lp, int rows, FusedSweepLayout* lower,
                                 FusedSweepLayout* upper);
 // Verifies every index the fused kernel reads: each row's entries are its strict triangle in
 // ascending column with the right sources, every slot names that column's LDS place, the

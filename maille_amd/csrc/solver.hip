@@ -87,7 +87,7 @@ inline unsigned grid_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 
 // per-block capacities of the triangular-solve layouts, as the kernel's LDS holds them
 constexpr TriCaps kTriCaps = {mlk::TRI_XCAP, mlk::TRI_ICAP, mlk::TRI_UCAP};
 // the fused sweep layouts are built for the kernel's block and LDS capacities
-static_assert(FUSED_ROWS == mlk::SWEEP_ROWS && FUSED_EXT == mlk::SWEEP_EXT,
+static_assert(FUSED_LANES == mlk::SWEEP_LANES && FUSED_EXT == mlk::SWEEP_EXT,
               "fused sweep layout and kernel disagree");
 
 // Kernel-duration accounting with HIP events on the handle's stream (enabled on demand).
@@ -234,9 +234,9 @@ int ic0_resident(void (*kernel)(TArgs...), hipStream_t stream) {
 
 template <typename... TArgs>
 int fused_resident(void (*kernel)(TArgs...), hipStream_t stream) {
-  return resident_per_cu((const void*)kernel, mlk::SWEEP_ROWS, stream,
+  return resident_per_cu((const void*)kernel, mlk::SWEEP_LANES, stream,
                          [&](int grid, int* probe, long long limit) {
-    kernel<<<grid, mlk::SWEEP_ROWS, 0, stream>>>(0, -1, 0, nullptr, nullptr, nullptr, nullptr,
+    kernel<<<grid, mlk::SWEEP_LANES, 0, stream>>>(0, -1, 0, nullptr, nullptr, nullptr, nullptr,
                                                  nullptr, nullptr, nullptr, nullptr, nullptr,
                                                  nullptr, nullptr, nullptr, nullptr, nullptr,
                                                  probe, limit);
@@ -376,16 +376,18 @@ struct Operator {
       eidx.upload_vec(f.eidx, st);
       eval.alloc(f.esrc.size());
     }
-  } fz_lo, fz_up;
-  bool have_fused = false;  // fused layouts built (the pattern fits them)
-  bool fused_on = true;     // the caller's choice (ml_pcg_set_sweeps_fused)
+  } fz[2][2];  // [rows per lane - 1][0 lower, 1 upper]: blocks of 256 and of 512 rows
+  bool have_fz[2] = {false, false};  // the pattern fits blocks of 256 / 512 rows
+  bool have_fused = false;           // some fused layout is built
+  bool fused_on = true;              // the caller's choice (ml_pcg_set_sweeps_fused)
   // variants of k_sweeps_fused: coefficients per row kept in registers (a longer row re-reads
-  // the rest each sweep) and the waves per SIMD its registers must allow
+  // the rest each sweep), the waves per SIMD its registers must allow, and the rows per lane
   struct FusedVariant {
-    int len, waves;
+    int len, waves, rpl;
   };
-  static constexpr int FUSED_VARIANTS = 5;
-  static constexpr FusedVariant kFused[FUSED_VARIANTS] = {{16, 8}, {24, 1}, {32, 1}, {48, 1}, {80, 1}};
+  static constexpr int FUSED_VARIANTS = 8;
+  static constexpr FusedVariant kFused[FUSED_VARIANTS] = {
+      {16, 8, 1}, {24, 1, 1}, {32, 1, 1}, {48, 1, 1}, {80, 1, 1}, {16, 5, 2}, {24, 1, 2}, {32, 1, 2}};
   int fused_var = 1;        // the variant in use
   int fused_width = 0;      // the longest row of the fused layouts
   int fused_per_cu = 0, full_fused = 0, fused_grid = 0;
@@ -523,17 +525,22 @@ struct Operator {
       d_sw_diag.alloc(n);
       d_sw_rd.alloc(n);
       for (auto* b : {&w_s1, &w_s2, &w_s3, &w_y0}) b->alloc(n);
-      // the fused layouts, where the pattern fits them (else every sweep is its own launch)
-      FusedSweepLayout flo, fup;
-      if (fused_sweep_layouts(lp, &flo, &fup).empty()) {
+      // the fused layouts, where the pattern fits them (else every sweep is its own launch):
+      // blocks of 256 rows, and of 512 (two rows per lane) for rows short enough to hold two
+      for (int h = 0; h < 2; ++h) {
+        FusedSweepLayout flo, fup;
+        if (!fused_sweep_layouts(lp, FUSED_LANES * (h + 1), &flo, &fup).empty()) continue;
+        if (h == 1 && std::max(flo.width, fup.width) > 32) continue;
         err = check_fused_sweep_layouts(lp, flo, fup);
         if (!err.empty()) throw ArgError(err);
-        fz_lo.set(flo, stream);
-        fz_up.set(fup, stream);
+        fz[h][0].set(flo, stream);
+        fz[h][1].set(fup, stream);
+        HIP_CHECK(hipStreamSynchronize(stream));  // the host layouts go out of scope
         fused_width = std::max(flo.width, fup.width);
-        have_fused = true;
-        select_fused(default_fused());
+        have_fz[h] = true;
       }
+      have_fused = have_fz[0] || have_fz[1];
+      if (have_fused) select_fused(default_fused());
       HIP_CHECK(hipStreamSynchronize(stream));  // the host layouts go out of scope
       have_sweep = true;
       if (factored) gather_sweeps();
@@ -557,45 +564,66 @@ struct Operator {
       case 1: return upper ? mlk::k_sweeps_fused<24, true> : mlk::k_sweeps_fused<24, false>;
       case 2: return upper ? mlk::k_sweeps_fused<32, true> : mlk::k_sweeps_fused<32, false>;
       case 3: return upper ? mlk::k_sweeps_fused<48, true> : mlk::k_sweeps_fused<48, false>;
-      default: return upper ? mlk::k_sweeps_fused<80, true> : mlk::k_sweeps_fused<80, false>;
+      case 4: return upper ? mlk::k_sweeps_fused<80, true> : mlk::k_sweeps_fused<80, false>;
+      case 5: return upper ? mlk::k_sweeps_fused<16, true, 5, 2> : mlk::k_sweeps_fused<16, false, 5, 2>;
+      case 6: return upper ? mlk::k_sweeps_fused<24, true, 1, 2> : mlk::k_sweeps_fused<24, false, 1, 2>;
+      default: return upper ? mlk::k_sweeps_fused<32, true, 1, 2> : mlk::k_sweeps_fused<32, false, 1, 2>;
     }
   }
-  // the variant that keeps every row in registers (the widest one for longer rows)
-  int default_fused() const {
-    for (int v = 1; v < FUSED_VARIANTS; ++v)
-      if (kFused[v].len >= fused_width) return v;
-    return FUSED_VARIANTS - 1;
+  static int fused_code(int v) { return 100 * kFused[v].rpl + kFused[v].len; }
+  // the variants whose layout is built and that hold no more coefficients in registers than
+  // the longest row needs (the narrowest one holding every row, else all)
+  bool fused_candidate(int v) const {
+    const int h = kFused[v].rpl - 1;
+    if (!have_fz[h]) return false;
+    int fit = 1000;
+    for (int u = 0; u < FUSED_VARIANTS; ++u)
+      if (kFused[u].rpl == kFused[v].rpl && kFused[u].len >= fused_width)
+        fit = std::min(fit, kFused[u].len);
+    return kFused[v].len <= fit;
   }
+  // the variant used before the autotuner has timed them: one row per lane, every row in
+  // registers (the widest variant for longer rows)
+  int default_fused() const {
+    int best = -1;
+    for (int v = 0; v < FUSED_VARIANTS; ++v)
+      if (fused_candidate(v) && kFused[v].waves == 1 &&
+          (best < 0 || kFused[v].rpl < kFused[best].rpl ||
+           (kFused[v].rpl == kFused[best].rpl && kFused[v].len > kFused[best].len)))
+        best = v;
+    return best;
+  }
+  const FusedDev& fused_layout(bool upper) const { return fz[kFused[fused_var].rpl - 1][upper]; }
   // puts a variant in use: its grid is bounded by probes of both of its kernels
   void select_fused(int var) {
     fused_var = var;
     fused_per_cu = std::min(fused_resident(fused_kernel(false), stream),
                             fused_resident(fused_kernel(true), stream));
-    residency[6] = occupancy_per_cu((const void*)fused_kernel(false), mlk::SWEEP_ROWS);
+    residency[6] = occupancy_per_cu((const void*)fused_kernel(false), mlk::SWEEP_LANES);
     residency[7] = fused_per_cu;
-    full_fused = std::min<int>(fz_lo.nblk, fused_per_cu * device_cus());
+    full_fused = std::min<int>(fused_layout(false).nblk, fused_per_cu * device_cus());
     set_grids();
   }
   bool fused_active() const { return have_fused && fused_on && sweeps > 0; }
-  // coefficients per row the fused kernel in use keeps in registers; 0 when it is not in use
-  int fused_capacity() const { return fused_active() ? kFused[fused_var].len : 0; }
-  // 0: one launch per sweep; 1: the fused kernel in its chosen variant; 16, 24, 32, 48, 80:
-  // the fused kernel with that register capacity
+  // the fused kernel's variant in use as 100 * rows per lane + coefficients per row held in
+  // registers; 0 when the sweeps are launched one by one
+  int fused_capacity() const { return fused_active() ? fused_code(fused_var) : 0; }
+  // 0: one launch per sweep; 1: the fused kernel in its chosen variant; a variant code
   void set_fused(int fused) {
     if (fused == 0 || fused == 1) {
       fused_on = fused == 1;
       return;
     }
     for (int v = 0; v < FUSED_VARIANTS; ++v)
-      if (kFused[v].len == fused) {
+      if (fused_code(v) == fused) {
+        if (!have_fz[kFused[v].rpl - 1])
+          throw ArgError("the pattern does not fit that fused sweep variant");
         fused_on = true;
-        if (have_fused) {
-          select_fused(v);
-          fill_levels();
-        }
+        select_fused(v);
+        fill_levels();
         return;
       }
-    throw ArgError("fused must be 0, 1 or a register capacity of 16, 24, 32, 48 or 80");
+    throw ArgError("fused must be 0, 1 or a variant code 116, 124, 132, 148, 180, 216, 224, 232");
   }
 
   // every level of both systems back to the sentinel (before a solve: a launch that gave up
@@ -613,9 +641,9 @@ struct Operator {
     for (SweepDev* t : {&sw_lo, &sw_up})
       mlk::k_gather<float, float><<<grid_for(t->entries), 256, 0, stream>>>(
           t->entries, t->ssrc.p, d_l32.p, t->sval.p);
-    if (have_fused)
-      for (FusedDev* t : {&fz_lo, &fz_up})
-        if (t->entries)
+    for (int h = 0; h < 2; ++h)
+      for (FusedDev* t : {&fz[h][0], &fz[h][1]})
+        if (have_fz[h] && t->entries)
           mlk::k_gather<float, float><<<grid_for(t->entries), 256, 0, stream>>>(
               t->entries, t->esrc.p, d_l32.p, t->eval.p);
     mlk::k_gather<float, float><<<grid_for(n), 256, 0, stream>>>((int64_t)n, d_sw_dsrc.p, d_l32.p,
@@ -847,12 +875,14 @@ struct Operator {
       // one launch per system; each resets the other's levels to the sentinel as it goes
       double* lo = k > 1 ? w_lev_lo.p : nullptr;
       double* up = k > 1 ? w_lev_up.p : nullptr;
-      fused_kernel(false)<<<fused_grid, mlk::SWEEP_ROWS, 0, stream>>>(
-          n, fz_lo.nblk, k, fz_lo.eoff.p, fz_lo.elen.p, fz_lo.eval.p, fz_lo.eidx.p, fz_lo.xoff.p,
-          fz_lo.xcol.p, d_sw_rd.p, r, lo, up, a, nullptr, nullptr, S, d_flags.p, wait_ticks);
-      fused_kernel(true)<<<fused_grid, mlk::SWEEP_ROWS, 0, stream>>>(
-          n, fz_up.nblk, k, fz_up.eoff.p, fz_up.elen.p, fz_up.eval.p, fz_up.eidx.p, fz_up.xoff.p,
-          fz_up.xcol.p, d_sw_rd.p, a, up, lo, z, dot, partial, S, d_flags.p, wait_ticks);
+      const FusedDev& fl = fused_layout(false);
+      const FusedDev& fu = fused_layout(true);
+      fused_kernel(false)<<<fused_grid, mlk::SWEEP_LANES, 0, stream>>>(
+          n, fl.nblk, k, fl.eoff.p, fl.elen.p, fl.eval.p, fl.eidx.p, fl.xoff.p, fl.xcol.p,
+          d_sw_rd.p, r, lo, up, a, nullptr, nullptr, S, d_flags.p, wait_ticks);
+      fused_kernel(true)<<<fused_grid, mlk::SWEEP_LANES, 0, stream>>>(
+          n, fu.nblk, k, fu.eoff.p, fu.elen.p, fu.eval.p, fu.eidx.p, fu.xoff.p, fu.xcol.p,
+          d_sw_rd.p, a, up, lo, z, dot, partial, S, d_flags.p, wait_ticks);
       HIP_CHECK(hipGetLastError());
       prof.end(P_TRI32, ev, stream);
       return;
@@ -1187,7 +1217,8 @@ struct Operator {
       HIP_CHECK(hipEventCreate(&b));
       float best_t = 1e30f;
       int best_v = fused_var;
-      for (int v = 0; v <= default_fused(); ++v) {
+      for (int v = 0; v < FUSED_VARIANTS; ++v) {
+        if (!fused_candidate(v)) continue;
         select_fused(v);
         fill_levels();
         float total = 0.0f;
@@ -2877,9 +2908,11 @@ ml_status ml_layout_verify(int64_t n, const int64_t* offsets, const int64_t* col
     SweepLayout lo, up;
     if (err.empty()) err = sweep_layouts(lp, &lo, &up);
     if (err.empty()) err = check_sweep_layouts(lp, lo, up);
-    FusedSweepLayout flo, fup;
-    if (err.empty() && fused_sweep_layouts(lp, &flo, &fup).empty())
-      err = check_fused_sweep_layouts(lp, flo, fup);
+    for (int rows : {FUSED_LANES, 2 * FUSED_LANES}) {
+      FusedSweepLayout flo, fup;
+      if (err.empty() && fused_sweep_layouts(lp, rows, &flo, &fup).empty())
+        err = check_fused_sweep_layouts(lp, flo, fup);
+    }
     if (!err.empty()) throw ArgError(err);
     if (stats) {
       stats[0] = (int64_t)d.fw.bstart.size() - 1;
@@ -2896,14 +2929,14 @@ ml_status ml_layout_verify(int64_t n, const int64_t* offsets, const int64_t* col
   });
 }
 
-ml_status ml_layout_sweeps(int64_t n, const int64_t* offsets, const int64_t* cols,
+ml_status ml_layout_sweeps(int64_t n, const int64_t* offsets, const int64_t* cols, int rows,
                            const double* lvalues, const double* r, int sweeps, double* z,
                            int64_t* stats) {
   return guarded([&]() -> int {
     if (sweeps < 1 || sweeps > 64) throw ArgError("sweeps must be in 1..64");
     LowerPattern lp = pattern_from_csr(n, offsets, cols);
     FusedSweepLayout flo, fup;
-    std::string err = fused_sweep_layouts(lp, &flo, &fup);
+    std::string err = fused_sweep_layouts(lp, rows, &flo, &fup);
     if (err.empty()) err = check_fused_sweep_layouts(lp, flo, fup);
     if (!err.empty()) throw ArgError(err);
     if (stats) {

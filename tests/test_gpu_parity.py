@@ -101,18 +101,19 @@ def test_mixed_preconditioner_applies_the_fp32_factor(gpu, precision):
     assert np.abs(l - l64).max() <= 1e-5 * np.abs(l64).max()
 
 
-SWEEP_CASES = [("jittered_cube_8", k, f) for k in (1, 2, 3, 4, 7) for f in (1, 16, 0)] + \
-    [("cantilever_tet10_small", k, f) for k in (1, 3) for f in (1, 16, 48, 0)]
+SWEEP_CASES = [("jittered_cube_8", k, f) for k in (1, 2, 3, 4, 7) for f in (1, 116, 216, 224, 0)] + \
+    [("cantilever_tet10_small", k, f) for k in (1, 3) for f in (1, 116, 148, 0)]
 
 
 @pytest.mark.parametrize("name,sweeps,fused", SWEEP_CASES)
 def test_jacobi_sweeps_match_their_restatement(gpu, name, sweeps, fused):
     """ML_MIXED_JACOBI's preconditioner: `sweeps` Jacobi sweeps of L y = r and of L^T z = y on
     the FP32 factor, bit for bit the float64 restatement, on every call, whether all sweeps of
-    a system run in one launch (fused 1: the variant that keeps every row in registers, 24
+    a system run in one launch (fused 1: one row per lane with every row in registers, 24
     coefficients on the cube and 80 on the Tet10 mesh, whose rows reach 146 and re-read their
-    tails; 16 and 48: variants that keep fewer and re-read more) or one launch per sweep (0);
-    more sweeps approach the exact triangular solves."""
+    tails; 116 and 148: variants that keep fewer and re-read more; 216, 224: two rows per lane
+    in blocks of 512 rows) or one launch per sweep (0); more sweeps approach the exact
+    triangular solves."""
     from sweeps import jacobi_precondition
     p = PROBLEMS[name]
     s = _oracle_system(p)
@@ -123,8 +124,8 @@ def test_jacobi_sweeps_match_their_restatement(gpu, name, sweeps, fused):
     pcg.set_sweeps(sweeps)
     pcg.set_sweeps_fused(fused)
     assert pcg.sweeps() == sweeps
-    capacity = {1: 24 if name == "jittered_cube_8" else 80}.get(fused, fused)
-    assert pcg.sweeps_fused() == capacity
+    code = {1: 124 if name == "jittered_cube_8" else 180}.get(fused, fused)
+    assert pcg.sweeps_fused() == code
     r = np.cos(0.11 * np.arange(s.n))
     z = pcg.precondition(r)
     l, _ = pcg.factor()
@@ -144,14 +145,14 @@ def test_fused_and_per_launch_sweeps_solve_identically(gpu):
     out = {}
     with gpu.Model(p, precision=gpu.MIXED_JACOBI) as m:
         m.set_sweeps(3)
-        for fused in (1, 16, 0):
+        for fused in (1, 116, 224, 0):
             m.set_sweeps_fused(fused)
-            assert m.sweeps_fused() == {1: 24}.get(fused, fused)
+            assert m.sweeps_fused() == {1: 124}.get(fused, fused)
             u, r, info = m.solve()
             assert info["device_retries"] == 0
             out[fused] = (u, r, info)
     u1, r1, i1 = out[0]
-    for fused in (1, 16):
+    for fused in (1, 116, 224):
         u2, r2, i2 = out[fused]
         assert np.array_equal(u1, u2) and np.array_equal(r1, r2)
         assert i1["iterations"] == i2["iterations"] and i1["residual"] == i2["residual"]

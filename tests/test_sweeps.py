@@ -109,9 +109,9 @@ def _fp32_factor(p):
 
 
 def test_fused_layouts_reproduce_the_restatement_on_the_host():
-    """The fused kernel's layouts (blocks of 256 rows, each row's entries as LDS slots of the
-    block's own rows and of its distinct external columns), applied on the host the way the
-    kernel stages them, give the restatement bit for bit: every slot names the right value.
+    """The fused kernel's layouts (blocks of 256 or 512 rows, each row's entries as LDS slots of
+    the block's own rows and of its distinct external columns), applied on the host the way
+    the kernel stages them, give the restatement bit for bit: every slot names the right value.
     The Tet10 mesh has rows of up to 146 strict entries and blocks reading many external
     columns."""
     import maille_amd as ma
@@ -119,10 +119,11 @@ def test_fused_layouts_reproduce_the_restatement_on_the_host():
               meshes.cantilever_tet10((12, 3, 3), (60.0, 15.0, 15.0))):
         off, cols, l, n = _fp32_factor(p)
         r = np.cos(0.11 * np.arange(n))
-        for k in (1, 2, 3, 5):
-            stats, z = ma.layout_sweeps(off, cols, l, r, k)
-            assert np.array_equal(z, jacobi_precondition(off, cols, l, r, k)), (p.name, k)
-        assert stats["blocks"] == (n + 255) // 256
+        for rows in (256, 512):
+            for k in (1, 2, 3, 5):
+                stats, z = ma.layout_sweeps(off, cols, l, r, k, rows=rows)
+                assert np.array_equal(z, jacobi_precondition(off, cols, l, r, k)), (p.name, rows, k)
+            assert stats["blocks"] == (n + rows - 1) // rows
         assert 0 < stats["max_ext_lower"] <= 1792 and 0 < stats["max_ext_upper"] <= 1792
         widths = np.diff(off) - 1
         assert stats["width"] == max(widths.max(), np.bincount(
