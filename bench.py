@@ -716,7 +716,8 @@ def main():
         per_ms, per_launch = d["avg_ms"], d["bytes_per_launch"]
     roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(name, f"jittered_cube_{args.cells}", head),
+                "traffic": measured_traffic(name, f"jittered_cube_{args.cells}", head, sweeps,
+                                            fused),
                 "avg_launch_ms": round(per_ms, 5), "algorithmic_bytes_per_launch": per_launch}
     if roofline["traffic"] is not None:
         roofline["traffic_source"] = ("replayed from the committed FETCH_SIZE/WRITE_SIZE passes "
@@ -818,14 +819,17 @@ def copy_probe(torch):
     log("copy probe: 268435456 bytes read, 268435456 bytes written")
 
 
-def measured_traffic(kernel, workload, precision):
+def measured_traffic(kernel, workload, precision, sweeps, variant):
     """HBM bytes per launch of `kernel` from the committed counter passes (profiles/), when
-    they were taken on this workload and precision; None otherwise."""
+    they were taken on this workload and precision with the same sweep count and fused-kernel
+    variant as this run; None otherwise."""
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}_{precision}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         rec = json.load(f)
+    if rec.get("sweeps", sweeps) != sweeps or rec.get("variant", variant) != variant:
+        return None
     return rec.get("bytes_per_launch")
 
 

@@ -16,8 +16,13 @@
 //   solid_d ....................... Lame constants closed form
 //   solid_stiffness ............... symmetry, 6 rigid modes in the null space, u'Ku = V e'De
 //   face_pressure ................. A/3 per Tet4 vertex, A/3 per Tet10 mid-node, resultant
-//   Graph / RCM / Csr pattern ..... brute-force pure-Python restatement on small graphs,
-//                                   permutation + profile shrink, reference unit tests
+//   Graph / RCM / Csr pattern ..... hand-derived known-answer orders, one graph per tie-break
+//                                   rule (first minimum seed, George-Liu restart, stable degree
+//                                   sort, one global reverse), each failing under its planted
+//                                   mistake (tests/test_rcm_known_answers.py); brute-force
+//                                   restatement on random graphs, permutation + profile shrink
+//   dot (8 lanes) ................. hand-derived known answers fixing the lane split, the
+//                                   combination tree and the tail order (tests/test_oracle.py)
 //   IC(0) ......................... tridiagonal => exact Cholesky; dense-pattern => Cholesky
 //   PCG ........................... grid Laplacian recovers the chosen x; skyline agreement
 //   skyline Cholesky .............. numpy dense solve; reference 4x4 known answer
@@ -35,7 +40,7 @@
 //   thermal, theta method ......... element closed forms; tests/transient.rs restated (Fourier
 //                                   series, amplification factor, steady state, long time)
 // Parity unpinned by any reference fixture (the reference ships none and cannot run here):
-// the exact RCM permutation, exact PCG and Newton iteration counts and bitwise CSR values.
+// exact PCG and Newton iteration counts and bitwise CSR values (element-order sums).
 // The multi-core PCG variant (or_system_pcg_parallel) is a timing baseline only.
 
 #include <algorithm>
