@@ -279,7 +279,8 @@ class _Handle:
         if h and _lib is not None:
             getattr(_lib, self._destroy)(h)
         self._h = None
-        _live.pop(getattr(self, "_key", None), None)
+        if _live is not None:  # None once the interpreter has torn the module down
+            _live.pop(getattr(self, "_key", None), None)
 
     def __enter__(self):
         return self

@@ -1364,6 +1364,15 @@ __global__ __launch_bounds__(256) void k_nodal_stresses(int64_t n_nodes,
   for (int c = 0; c < 6; ++c) out[9 * n + 3 + c] = s[c] / cnt;
 }
 
+// A load of data read once per launch (matrix entries, factor coefficients): non-temporal, so
+// that it does not evict from the L2 the vector entries that neighbouring rows gather again.
+template <typename T>
+__device__ inline T stream_load(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
 // ------------------------------------------------------------------ SpMV (sliced ELL)
 // y = A x (Csr::multiply, iterative.rs:131-142). Lane j of a slice sums row j in the order
 // diagonal, lower entries, upper entries: the reference's accumulation order, so the FP64
@@ -1387,8 +1396,10 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
   if (j < n) {
     const int base = soff[j >> 6] + (j & 63);
     const int len = slen[j];
-    auto column = [&](int at) { return D16 ? j + (int)sdel[at] : scol[at]; };
-    T s = sval[base] * x[column(base)];
+    auto column = [&](int at) {
+      return D16 ? j + (int)stream_load(sdel + at) : stream_load(scol + at);
+    };
+    T s = stream_load(sval + base) * x[column(base)];
     int k = 1;
     for (; k + BATCH <= len; k += BATCH) {
       int c[BATCH];
@@ -1396,7 +1407,7 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
 #pragma unroll
       for (int u = 0; u < BATCH; ++u) {
         c[u] = column(base + ((k + u) << 6));
-        v[u] = sval[base + ((k + u) << 6)];
+        v[u] = stream_load(sval + base + ((k + u) << 6));
       }
 #pragma unroll
       for (int u = 0; u < BATCH; ++u) xv[u] = x[c[u]];
@@ -1405,7 +1416,7 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
     }
     for (; k < len; ++k) {
       const int at = base + (k << 6);
-      s += sval[at] * x[column(at)];
+      s += stream_load(sval + at) * x[column(at)];
     }
     y[j] = s;
     dotv = (double)x[j] * (double)s;
@@ -1587,8 +1598,12 @@ k_sweeps_fused(int n, int nblk, int k, const int* __restrict__ eoff,
         float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         uint2 d = make_uint2(0u, 0u);
         if (4 * g < len[h]) {
-          f = *reinterpret_cast<const float4*>(eval + base[h] + g * 4 * SWEEP_LANES);
-          d = *reinterpret_cast<const uint2*>(eidx + base[h] + g * 4 * SWEEP_LANES);
+          const f32x4 fv =
+              stream_load(reinterpret_cast<const f32x4*>(eval + base[h] + g * 4 * SWEEP_LANES));
+          const u32x2 dv =
+              stream_load(reinterpret_cast<const u32x2*>(eidx + base[h] + g * 4 * SWEEP_LANES));
+          f = make_float4(fv.x, fv.y, fv.z, fv.w);
+          d = make_uint2(dv.x, dv.y);
         }
         v[h][4 * g] = f.x;
         v[h][4 * g + 1] = f.y;
@@ -1601,10 +1616,10 @@ k_sweeps_fused(int n, int nblk, int k, const int* __restrict__ eoff,
       ri[h] = i < n ? rhs[i] : 0.0;
       buf[h * SWEEP_LANES + t] = ri[h] * rdi[h];
     }
-    const int x0 = xoff[b], nx = xoff[b + 1] - x0;
+    const int xb = xoff[b], nx = xoff[b + 1] - xb;
     // the block's external columns, kept in LDS for every sweep's staging
     for (int e = t; e < nx; e += SWEEP_LANES) {
-      const int c = xcol[x0 + e];
+      const int c = xcol[xb + e];
       cols[e] = c;
       buf[R + e] = rhs[c] * rd[c];
     }

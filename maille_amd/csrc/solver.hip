@@ -1003,7 +1003,9 @@ struct Operator {
     const int vb = grid_for(n);
     double* part = w_partial.p;
     const bool jac = sweeps > 0;
-    // the fused sweeps form r * rd themselves; the per-launch ones read it from the update
+    // the fused sweeps form r * rd themselves (reading it from a vector written by the residual
+    // update measured slower: the extra writes cost more than the gathers they save, which
+    // mostly hit the L2); the per-launch ones read it from the update
     const bool y0 = jac && !fused_active();
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);  // abort flag
     if (fused_active()) fill_levels();
