@@ -35,10 +35,11 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def spmv_bytes(n, nnz_full, val, idx=4):
-    """Algorithmic SpMV bytes: values + column indices (int32, or 16-bit offsets from the row)
-    + row lengths + x + y."""
-    return nnz_full * (val + idx) + (n + 1) * 4 + 2 * n * val
+def spmv_bytes(n, nnz_full, val, index_bytes):
+    """Algorithmic SpMV bytes: values + the column indices and per-row fields of the form in use
+    (int32 or 16-bit offsets per entry with row lengths, or one 16-bit stream per group of rows
+    with equal columns with its per-row fields: Model.spmv_index) + x + y."""
+    return nnz_full * val + index_bytes + 2 * n * val
 
 
 def sweep_bytes(n, nnz_strict, col_bytes=4):
@@ -332,9 +333,12 @@ def run_plastic(args, ma, meshes):
     are scaled to the device run's counts."""
     from maille_amd import plasticity as P
     p = meshes.plastic_bar()
-    codes = {"fp64": ma.FP64, "mixed": ma.MIXED, "mixed_ic0": ma.MIXED_IC0}
+    codes = {"fp64": ma.FP64, "mixed": ma.MIXED, "mixed_ic0": ma.MIXED_IC0,
+             "mixed_jacobi": ma.MIXED_JACOBI}
     prec = "fp64" if args.precision == "best" else args.precision
     m = ma.Model(p, precision=codes[prec])
+    if args.sweeps is not None and prec != "fp64":
+        m.set_sweeps(args.sweeps)
     sy, hd = 250.0, 2000.0
     step, amp = P.Step(1.0, args.increments), P.Amplitude.ramp(1.0)
     incs, _ = P.solve_quasi_static(m, P.Step(1.0, 1), P.Amplitude([(0.0, 0.0), (1.0, 1e-3)]),
@@ -420,7 +424,8 @@ def run_transient(args, ma, meshes):
     conv = [(e, fc, 10.0, 0.0) for e in np.nonzero(on[conn].sum(axis=1) >= 3)[0]
             for fc, vs in enumerate(faces) if on[conn[e, list(vs)]].all()]
     steps, period = args.increments if args.increments != 5 else 50, 0.1
-    codes = {"fp64": ma.FP64, "mixed": ma.MIXED, "mixed_ic0": ma.MIXED_IC0}
+    codes = {"fp64": ma.FP64, "mixed": ma.MIXED, "mixed_ic0": ma.MIXED_IC0,
+             "mixed_jacobi": ma.MIXED_JACOBI}
     prec = "fp64" if args.precision == "best" else args.precision
     t0 = time.perf_counter()
     m = T.ThermalModel(nodes, conn, 1.0, 1.0, fixed, imposed, convection=conv,
@@ -487,7 +492,12 @@ def run_large(args, ma, meshes):
                     pressures.append((int(e), fc, p))
     prob = meshes.Problem("block", nodes, conn, np.full(ne, young), np.full(ne, nu), fixed,
                           np.zeros(3 * len(nodes)), np.zeros(3 * len(nodes)))
-    m = ma.Model(prob)
+    codes = {"fp64": ma.FP64, "mixed": ma.MIXED, "mixed_ic0": ma.MIXED_IC0,
+             "mixed_jacobi": ma.MIXED_JACOBI}
+    prec = "fp64" if args.precision == "best" else args.precision
+    m = ma.Model(prob, precision=codes[prec])
+    if args.sweeps is not None and prec != "fp64":
+        m.set_sweeps(args.sweeps)
     step, amp = P.Step(1.0, 4), P.Amplitude.ramp(1.0)
     P.solve_large_strain(m, P.Step(1.0, 1), P.Amplitude([(0.0, 0.0), (1.0, 1e-3)]),
                          neo_hookean=True, pressures=pressures)  # warm-up
@@ -521,7 +531,7 @@ def run_large(args, ma, meshes):
     emit({"metric": "large-strain Neo-Hookean Newton-Raphson with follower pressures s, Tet4 "
                     "block 40^3, 4 increments",
           "value": round(wall, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
-          "data": "synthetic", "dtype": "fp64",
+          "data": "synthetic", "dtype": "fp64" if prec == "fp64" else f"fp64+{prec}",
           "config": {"workload": "neo_hookean_block_40", "elements": int(ne),
                      "free_dof": prob.nfree, "follower_faces": len(pressures),
                      "pressure": round(p, 3), "stretch": stretch},
@@ -673,7 +683,8 @@ def main():
     mval = 8 if head == "fp64" else 4
     kinds = {"spmv": "spmv32" if head == "mixed" else "spmv64",
              "precondition": "precond64" if head == "fp64" else "precond32"}
-    spmv_idx, col_bytes = models[head].index_bytes()
+    spmv_form, spmv_idx = models[head].spmv_index()
+    col_bytes = models[head].index_bytes()[1]
     col_bytes = col_bytes or 4
     fused = models[head].sweeps_fused() if sweeps else 0  # register capacity, 0: per launch
     pre_bytes = (2 * fused_sweep_bytes(n, nnz_l - n, sweeps) if fused else
@@ -689,6 +700,10 @@ def main():
                          "bytes_per_launch": bytes_per[name],
                          "achieved_GBs": round(achieved, 1),
                          "share_of_step": round(ms / 1e3 / elapsed, 3)}
+    kernels["spmv"]["index_form"] = {1: "one 16-bit column stream per group of rows with "
+                                         "equal columns (a node's DOFs)",
+                                      2: "16-bit column offsets per entry",
+                                      4: "int32 columns"}[spmv_form]
     kernels["precondition"]["form"] = (
         f"{sweeps} Jacobi sweeps per triangular system, all in one launch per system "
         f"(k_sweeps_fused, 2 launches, {fused} coefficients per row in registers)" if fused else
@@ -726,7 +741,7 @@ def main():
     p64 = compare["fp64"]["prof"]
     spmv64_ms, spmv64_cnt = p64["spmv64"]
     spmv64_avg = spmv64_ms / max(spmv64_cnt, 1)
-    spmv64_gbs = (spmv_bytes(n, nnz_full, 8, m0.index_bytes()[0]) / (spmv64_avg * 1e-3) / 1e9
+    spmv64_gbs = (spmv_bytes(n, nnz_full, 8, m0.spmv_index()[1]) / (spmv64_avg * 1e-3) / 1e9
                   if spmv64_cnt else 0.0)
 
     # the whole Model::solve (model.rs:548-610) of the headline mode: model creation (mesh

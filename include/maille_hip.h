@@ -168,8 +168,9 @@ ml_status ml_layout_verify(int64_t n, const int64_t* offsets, const int64_t* col
  * ML_INVALID_ARGUMENT, with the reason, when the pattern does not fit them) and, with
  * lvalues, r and z non-null, applies `sweeps` sweeps of each triangular system on the host
  * through those layouts, staging every value the way the fused kernel does, with the factor
- * rounded to FP32 (z = P^T P r). stats[4] (optional): blocks, the longest row, the most
- * external columns of a lower and of an upper block. */
+ * rounded to FP32 (z = P^T P r). stats[5] (optional): blocks, the longest row, the most
+ * external columns of a lower and of an upper block, and the code of the fused variant a
+ * handle on this pattern starts with before autotuning (0: one launch per sweep). */
 ml_status ml_layout_sweeps(int64_t n, const int64_t* offsets, const int64_t* cols, int rows,
                            const double* lvalues, const double* r, int sweeps, double* z,
                            int64_t* stats);
@@ -265,9 +266,15 @@ int ml_model_sweeps(const ml_model* m);
 ml_status ml_model_set_sweeps_fused(ml_model* m, int fused);
 int ml_model_sweeps_fused(const ml_model* m);
 /* Bytes per column index the kernels read: out[0] the SpMV's (2: 16-bit offsets from the row,
- * when the matrix band fits; 4: int32), out[1] the sweeps' (2 or 4, always 2 in the fused
- * kernel, whose 16-bit indices name LDS slots; 0 before any sweep layout is built). */
+ * when the matrix band fits; 4: int32; 0: the grouped streams, see ml_model_spmv_index),
+ * out[1] the sweeps' (2 or 4, always 2 in the fused kernel, whose 16-bit indices name LDS
+ * slots; 0 before any sweep layout is built). */
 ml_status ml_model_index_bytes(const ml_model* m, int32_t* out);
+/* No counterpart in the reference: the SpMV's index form in use, out[0] = 1 when the rows of a
+ * node share one 16-bit column stream (grouped, chosen by ml_model_autotune when faster), 2 for
+ * 16-bit offsets per entry, 4 for int32 columns; out[1] = the bytes of column indices and
+ * per-row fields one SpMV reads in that form. */
+ml_status ml_model_spmv_index(const ml_model* m, int64_t* out);
 /* Diagnostics (no counterpart in the reference): out[8] = workgroups per CU that the runtime's
  * occupancy figure reports and that the dataflow grids use, for the forward triangular solve,
  * the backward one, IC(0) and the fused sweeps (zeros until fused sweep layouts are built):

@@ -45,7 +45,7 @@ EXPORTS = (
     "ml_pcg_create", "ml_pcg_solve", "ml_pcg_solve_device", "ml_pcg_multiply",
     "ml_pcg_precondition", "ml_pcg_factor", "ml_pcg_destroy", "ml_pcg_trace",
     "ml_pcg_set_wait_limit", "ml_pcg_set_sweeps", "ml_pcg_sweeps", "ml_model_set_sweeps",
-    "ml_model_sweeps", "ml_model_index_bytes", "ml_model_residency",
+    "ml_model_sweeps", "ml_model_index_bytes", "ml_model_spmv_index", "ml_model_residency",
     "ml_pcg_set_sweeps_fused", "ml_pcg_sweeps_fused", "ml_model_set_sweeps_fused",
     "ml_model_sweeps_fused",
     "ml_model_create", "ml_model_assemble", "ml_model_set_young", "ml_model_factor",
@@ -198,6 +198,7 @@ def lib():
             "ml_model_set_sweeps": (_int, [_P, _int]),
             "ml_model_sweeps": (_int, [_P]),
             "ml_model_index_bytes": (_int, [_P, _I32]),
+            "ml_model_spmv_index": (_int, [_P, _I]),
             "ml_model_residency": (_int, [_P, _I32]),
             "ml_pcg_set_sweeps_fused": (_int, [_P, _int]),
             "ml_pcg_sweeps_fused": (_int, [_P]),
@@ -485,10 +486,11 @@ def layout_sweeps(offsets, cols, l=None, r=None, sweeps=1, rows=256):
     """Builds and checks, on the host, the fused Jacobi-sweep layouts of this lower CSR
     pattern in blocks of ``rows`` (256 or 512) rows (ValueError when it does not fit them);
     returns their statistics (blocks, longest row, most external columns of a lower and of an
-    upper block) and, given the factor values ``l`` and ``r``, the sweeps applied on the host
+    upper block, the fused variant code a handle on this pattern starts with, 0 for one launch
+    per sweep) and, given the factor values ``l`` and ``r``, the sweeps applied on the host
     through those layouts, with the factor rounded to FP32."""
     off, cols = _ai64(offsets), _ai64(cols)
-    stats = np.zeros(4, dtype=np.int64)
+    stats = np.zeros(5, dtype=np.int64)
     z = None
     if l is not None and r is not None:
         l, r = _f64(l), _f64(r)
@@ -497,7 +499,8 @@ def layout_sweeps(offsets, cols, l=None, r=None, sweeps=1, rows=256):
                                   _d(l) if z is not None else None,
                                   _d(r) if z is not None else None, int(sweeps),
                                   _d(z) if z is not None else None, _i(stats)))
-    out = dict(zip(("blocks", "width", "max_ext_lower", "max_ext_upper"), (int(v) for v in stats)))
+    out = dict(zip(("blocks", "width", "max_ext_lower", "max_ext_upper", "default_variant"),
+                   (int(v) for v in stats)))
     return (out, z) if z is not None else out
 
 
@@ -687,6 +690,14 @@ class Model(_Handle):
         out = (C.c_int32 * 2)()
         _check(lib().ml_model_index_bytes(self._h, out))
         return out[0], out[1]
+
+    def spmv_index(self):
+        """(form, bytes): the SpMV's column-index form (1: one 16-bit stream per group of rows
+        with equal columns, 2: 16-bit offsets per entry, 4: int32) and the bytes of indices and
+        per-row fields one SpMV reads in it."""
+        out = np.zeros(2, dtype=np.int64)
+        _check(lib().ml_model_spmv_index(self._h, _i(out)))
+        return int(out[0]), int(out[1])
 
     def solve_many(self, F):
         """Load cases F (rows: nodal loads over all DOFs, without dead pressures) solved

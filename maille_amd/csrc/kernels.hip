@@ -1427,6 +1427,66 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ sof
   }
 }
 
+// k_spmv with the grouped column streams: the rows of a group (consecutive rows of the slice
+// with equal column sets, the DOFs of one node) read one shared list of columns, ascending, as
+// 16-bit offsets from the group's first row, so a wave loads each group's index once instead of
+// once per row. Row j walks the list and skips its own column, which it took first as the
+// diagonal: list entry q < pos is its sliced-ELL entry q + 1, and q > pos is entry q, so the
+// values, the products and the order of the sum are those of k_spmv (bit-identical).
+template <typename T, int BATCH = 8>
+__global__ __launch_bounds__(256) void k_spmv_grouped(int n, const int* __restrict__ soff,
+                                                      const int* __restrict__ goff,
+                                                      const int* __restrict__ gnum,
+                                                      const int16_t* __restrict__ gdel,
+                                                      const uint32_t* __restrict__ ginfo,
+                                                      const T* __restrict__ sval,
+                                                      const T* __restrict__ x, T* __restrict__ y,
+                                                      double* __restrict__ partial,
+                                                      const PcgScalars* __restrict__ S,
+                                                      int only_on_update = 0) {
+#pragma clang fp contract(off)
+  if (S && (S->done || (only_on_update && !S->update))) return;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  double dotv = 0.0;
+  if (j < n) {
+    const int sl = j >> 6;
+    const int base = soff[sl] + (j & 63);
+    const uint32_t gi = ginfo[j];
+    const int len = gi & 255, pos = (gi >> 8) & 255, g = (gi >> 16) & 63;
+    const int first = j - (int)((gi >> 24) & 63);
+    const int G = gnum[sl];
+    const int16_t* list = gdel + goff[sl] + g;
+    T s = stream_load(sval + base) * x[j];
+    int q = 0;
+    for (; q + BATCH <= len; q += BATCH) {
+      int c[BATCH];
+      T v[BATCH], xv[BATCH];
+#pragma unroll
+      for (int u = 0; u < BATCH; ++u) {
+        const int qq = q + u;
+        c[u] = first + (int)stream_load(list + qq * G);
+        // the own position loads an entry that is not summed
+        v[u] = stream_load(sval + base + ((qq < pos ? qq + 1 : qq) << 6));
+      }
+#pragma unroll
+      for (int u = 0; u < BATCH; ++u) xv[u] = x[c[u]];
+#pragma unroll
+      for (int u = 0; u < BATCH; ++u) s = q + u == pos ? s : s + v[u] * xv[u];
+    }
+    for (; q < len; ++q) {
+      if (q == pos) continue;
+      const int c = first + (int)stream_load(list + q * G);
+      s += stream_load(sval + base + ((q < pos ? q + 1 : q) << 6)) * x[c];
+    }
+    y[j] = s;
+    dotv = (double)x[j] * (double)s;
+  }
+  if (partial) {
+    double b = block_sum<256>(dotv);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+  }
+}
+
 // y = A x for nrep right-hand sides stored one after the other (x[b n + j]): thread per
 // (b, j), the same row sums as k_spmv.
 template <typename T>

@@ -445,6 +445,62 @@ TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const in
   return t;
 }
 
+// The grouped column streams of the sliced ELL (DeviceLayouts::goff ...), built from its
+// per-row columns; left empty when some row or offset does not fit their fields.
+static void grouped_columns(DeviceLayouts& d) {
+  const i64 n = d.n;
+  d.goff.clear(); d.gnum.clear(); d.gdel.clear(); d.ginfo.clear();
+  std::vector<int32_t> goff(d.slices + 1, 0), gnum(d.slices, 0);
+  std::vector<uint32_t> ginfo(n, 0);
+  std::vector<int16_t> gdel;
+  auto row_cols = [&](i64 j, std::vector<int32_t>& out) {  // the row's columns, ascending
+    const i64 base = d.soff[j >> 6] + (j & 63);
+    out.resize(d.slen[j]);
+    for (i64 k = 0; k < d.slen[j]; ++k) out[k] = d.scol[base + (k << 6)];
+    std::sort(out.begin(), out.end());
+  };
+  std::vector<int32_t> cur, next;
+  for (i64 sl = 0; sl < d.slices; ++sl) {
+    const i64 j0 = sl * 64, j1 = std::min<i64>(n, j0 + 64);
+    std::vector<i64> first;  // first row of each group
+    std::vector<std::vector<int32_t>> lists;
+    for (i64 j = j0; j < j1; ++j) {
+      if (d.slen[j] > 255) return;
+      row_cols(j, next);
+      if (!first.empty() && next == lists.back() && j - first.back() < 64) {
+        // same column set as the group in progress
+      } else {
+        first.push_back(j);
+        lists.push_back(next);
+      }
+      const i64 g = (i64)first.size() - 1;
+      const auto& L = lists.back();
+      const i64 pos = std::lower_bound(L.begin(), L.end(), (int32_t)j) - L.begin();
+      if (pos >= (i64)L.size() || L[pos] != j || pos > 255) return;
+      ginfo[j] = (uint32_t)d.slen[j] | ((uint32_t)pos << 8) | ((uint32_t)g << 16) |
+                 ((uint32_t)(j - first.back()) << 24);
+    }
+    const i64 G = (i64)first.size();
+    i64 W = 0;
+    for (const auto& L : lists) W = std::max<i64>(W, (i64)L.size());
+    gnum[sl] = (int32_t)G;
+    const i64 at = (i64)gdel.size();
+    if (at + W * G > INT32_MAX) return;
+    gdel.resize(at + W * G, 0);
+    for (i64 g = 0; g < G; ++g)
+      for (i64 q = 0; q < (i64)lists[g].size(); ++q) {
+        const i64 off = (i64)lists[g][q] - first[g];
+        if (off < -32768 || off > 32767) return;
+        gdel[at + q * G + g] = (int16_t)off;
+      }
+    goff[sl + 1] = (int32_t)gdel.size();
+  }
+  d.goff.swap(goff);
+  d.gnum.swap(gnum);
+  d.gdel.swap(gdel);
+  d.ginfo.swap(ginfo);
+}
+
 std::string device_layouts(const LowerPattern& lp, const TriCaps& caps, DeviceLayouts* out) {
   DeviceLayouts& d = *out;
   const i64 n = lp.n();
@@ -499,6 +555,7 @@ std::string device_layouts(const LowerPattern& lp, const TriCaps& caps, DeviceLa
     }
     if (fits) d.sdel.swap(del);
   }
+  grouped_columns(d);
   // forward solve: the strictly lower rows of L
   const i64 nf = lp.nnz() - n;
   std::vector<int32_t> foff(n + 1), fcol(nf), fsrc(nf), fdsrc(n);
@@ -657,6 +714,31 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, const 
       for (i64 k = 0; k < d.slen[j]; ++k)
         if (j + d.sdel[base + (k << 6)] != d.scol[base + (k << 6)])
           return "sliced-ELL 16-bit offset disagrees with its column";
+    }
+  }
+  if (!d.ginfo.empty()) {
+    // every row's grouped stream decodes to its sliced-ELL columns, in order
+    if ((i64)d.goff.size() != d.slices + 1 || (i64)d.gnum.size() != d.slices ||
+        (i64)d.ginfo.size() != n || d.goff[0] != 0 || (size_t)d.goff[d.slices] != d.gdel.size())
+      return "grouped columns of the wrong size";
+    for (i64 sl = 0; sl < d.slices; ++sl) {
+      const i64 G = d.gnum[sl], span = (i64)d.goff[sl + 1] - d.goff[sl];
+      if (G < 1 || G > 64 || span < 0 || span % G) return "grouped columns: bad slice";
+    }
+    for (i64 j = 0; j < n; ++j) {
+      const uint32_t gi = d.ginfo[j];
+      const i64 len = gi & 255, pos = (gi >> 8) & 255, g = (gi >> 16) & 63, sub = (gi >> 24) & 63;
+      const i64 sl = j >> 6, G = d.gnum[sl], W = ((i64)d.goff[sl + 1] - d.goff[sl]) / G;
+      if (len != d.slen[j] || pos >= len || g >= G || len > W || sub > (j & 63))
+        return "grouped columns: bad row fields";
+      const i64 base = d.soff[sl] + (j & 63);
+      if (j - sub + d.gdel[d.goff[sl] + pos * G + g] != j)
+        return "grouped columns: the row's own position does not hold its column";
+      for (i64 k = 1; k < len; ++k) {
+        const i64 q = k - 1 < pos ? k - 1 : k;
+        if (j - sub + d.gdel[d.goff[sl] + q * G + g] != d.scol[base + (k << 6)])
+          return "grouped columns disagree with the sliced ELL";
+      }
     }
   }
   // triangular solves, against the logical rows they were built from

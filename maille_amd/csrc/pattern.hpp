@@ -125,6 +125,19 @@ struct DeviceLayouts {
   std::vector<int32_t> soff, scol, slen, ssrc;
   // col - row of every sliced-ELL entry as 16 bits when all fit (padding: 0), else empty
   std::vector<int16_t> sdel;
+  // The same columns as one stream per group of rows: consecutive rows of a slice whose
+  // column sets are equal (the DOFs of one node) share the ascending list of those columns,
+  // as 16-bit offsets from the group's first row. Group g of slice sl with G groups and lists
+  // of W entries holds entry q at goff[sl] + q G + g (W G entries per slice, padding 0). Per
+  // row, ginfo packs the row length (bits 0-7), the position of the row's own column in its
+  // group's list (8-15), the group's index in the slice (16-21) and the row's distance from
+  // the group's first row (24-29). Row j's sliced-ELL entry k is list entry k - 1 below its
+  // own position and k above it (k = 0 is the diagonal). Empty when some row is longer than
+  // 255 entries or an offset does not fit 16 bits.
+  std::vector<int32_t> goff;   // slices + 1
+  std::vector<int32_t> gnum;   // groups per slice
+  std::vector<int16_t> gdel;
+  std::vector<uint32_t> ginfo;
   TriLayout fw, bw;
 };
 // Returns an empty string or the reason the pattern cannot be laid out (row too long).

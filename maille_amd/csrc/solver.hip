@@ -256,6 +256,43 @@ void launch_dataflow(void (*kernel)(Args...), int& grid, int threads, hipStream_
 
 }  // namespace
 
+// Variants of k_sweeps_fused: coefficients per row kept in registers (a longer row re-reads
+// the rest each sweep), the waves per SIMD its registers must allow, and the rows per lane
+// (blocks of 256 rows, or 512 with two rows per lane, for rows of at most 32 strict entries).
+struct FusedVariant {
+  int len, waves, rpl;
+};
+constexpr int FUSED_VARIANTS = 8;
+constexpr FusedVariant kFused[FUSED_VARIANTS] = {
+    {16, 8, 1}, {24, 1, 1}, {32, 1, 1}, {48, 1, 1}, {80, 1, 1}, {16, 5, 2}, {24, 1, 2}, {32, 1, 2}};
+// The code a caller names a variant by: 100 * rows per lane + coefficients held per row.
+int fused_code(int v) { return 100 * kFused[v].rpl + kFused[v].len; }
+// The variants worth timing for a pattern whose fused layouts exist as have[rows per lane - 1]
+// and whose longest row holds `width` strict entries: their layout is built, and they hold no
+// more coefficients in registers than the longest row needs (the narrowest variant of their
+// shape holding every row; all of that shape when none does).
+bool fused_variant_candidate(int v, const bool have[2], int width) {
+  if (!have[kFused[v].rpl - 1]) return false;
+  int fit = 1000;
+  for (int u = 0; u < FUSED_VARIANTS; ++u)
+    if (kFused[u].rpl == kFused[v].rpl && kFused[u].len >= width) fit = std::min(fit, kFused[u].len);
+  return kFused[v].len <= fit;
+}
+// The variant a handle uses before the autotuner has timed them: one row per lane before two,
+// then one compiled without a register bound (waves 1) before a bounded one, then the widest;
+// -1 when no variant is a candidate (the sweeps then run one launch each).
+int default_fused_variant(const bool have[2], int width) {
+  int best = -1;
+  auto better = [&](int v, int b) {
+    if (kFused[v].rpl != kFused[b].rpl) return kFused[v].rpl < kFused[b].rpl;
+    if ((kFused[v].waves == 1) != (kFused[b].waves == 1)) return kFused[v].waves == 1;
+    return kFused[v].len > kFused[b].len;
+  };
+  for (int v = 0; v < FUSED_VARIANTS; ++v)
+    if (fused_variant_candidate(v, have, width) && (best < 0 || better(v, best))) best = v;
+  return best;
+}
+
 // ====================================================================== Operator
 struct Operator {
   int n = 0;
@@ -280,6 +317,12 @@ struct Operator {
   DBuf<int> d_soff, d_scol, d_slen, d_ssrc;
   DBuf<int16_t> d_sdel;  // 16-bit column offsets of the sliced ELL, when the band allows
   bool spmv_d16 = false;
+  // the grouped column streams (one per group of rows with equal column sets), when they fit
+  DBuf<int> d_goff, d_gnum;
+  DBuf<int16_t> d_gdel;
+  DBuf<uint32_t> d_ginfo;
+  bool have_grouped = false, spmv_grouped = false;  // built; in use (autotuned)
+  int64_t grouped_bytes = 0;  // the grouped streams' bytes (columns + per-row fields)
   DBuf<double> d_sval;
   DBuf<float> d_sval32;
   // the IC(0) factor and the layouts of its two triangular solves
@@ -380,14 +423,6 @@ struct Operator {
   bool have_fz[2] = {false, false};  // the pattern fits blocks of 256 / 512 rows
   bool have_fused = false;           // some fused layout is built
   bool fused_on = true;              // the caller's choice (ml_pcg_set_sweeps_fused)
-  // variants of k_sweeps_fused: coefficients per row kept in registers (a longer row re-reads
-  // the rest each sweep), the waves per SIMD its registers must allow, and the rows per lane
-  struct FusedVariant {
-    int len, waves, rpl;
-  };
-  static constexpr int FUSED_VARIANTS = 8;
-  static constexpr FusedVariant kFused[FUSED_VARIANTS] = {
-      {16, 8, 1}, {24, 1, 1}, {32, 1, 1}, {48, 1, 1}, {80, 1, 1}, {16, 5, 2}, {24, 1, 2}, {32, 1, 2}};
   int fused_var = 1;        // the variant in use
   int fused_width = 0;      // the longest row of the fused layouts
   int fused_per_cu = 0, full_fused = 0, fused_grid = 0;
@@ -442,6 +477,22 @@ struct Operator {
     spmv_d16 = !d.sdel.empty();
     if (spmv_d16) d_sdel.upload_vec(d.sdel, stream);
     else d_sdel.release();
+    have_grouped = !d.ginfo.empty();
+    spmv_grouped = have_grouped;
+    if (have_grouped) {
+      d_goff.upload_vec(d.goff, stream);
+      d_gnum.upload_vec(d.gnum, stream);
+      d_gdel.upload_vec(d.gdel, stream);
+      d_ginfo.upload_vec(d.ginfo, stream);
+      grouped_bytes = (int64_t)d.gdel.size() * 2 + (int64_t)d.ginfo.size() * 4 +
+                      (int64_t)(d.goff.size() + d.gnum.size()) * 4;
+    } else {
+      d_goff.release();
+      d_gnum.release();
+      d_gdel.release();
+      d_ginfo.release();
+      grouped_bytes = 0;
+    }
     d_slen.upload_vec(d.slen, stream);
     d_ssrc.upload_vec(d.ssrc, stream);
     nf = nnz - n;
@@ -539,7 +590,8 @@ struct Operator {
         fused_width = std::max(flo.width, fup.width);
         have_fz[h] = true;
       }
-      have_fused = have_fz[0] || have_fz[1];
+      // a pattern without a candidate variant keeps one launch per sweep
+      have_fused = default_fused() >= 0;
       if (have_fused) select_fused(default_fused());
       HIP_CHECK(hipStreamSynchronize(stream));  // the host layouts go out of scope
       have_sweep = true;
@@ -570,29 +622,8 @@ struct Operator {
       default: return upper ? mlk::k_sweeps_fused<32, true, 1, 2> : mlk::k_sweeps_fused<32, false, 1, 2>;
     }
   }
-  static int fused_code(int v) { return 100 * kFused[v].rpl + kFused[v].len; }
-  // the variants whose layout is built and that hold no more coefficients in registers than
-  // the longest row needs (the narrowest one holding every row, else all)
-  bool fused_candidate(int v) const {
-    const int h = kFused[v].rpl - 1;
-    if (!have_fz[h]) return false;
-    int fit = 1000;
-    for (int u = 0; u < FUSED_VARIANTS; ++u)
-      if (kFused[u].rpl == kFused[v].rpl && kFused[u].len >= fused_width)
-        fit = std::min(fit, kFused[u].len);
-    return kFused[v].len <= fit;
-  }
-  // the variant used before the autotuner has timed them: one row per lane, every row in
-  // registers (the widest variant for longer rows)
-  int default_fused() const {
-    int best = -1;
-    for (int v = 0; v < FUSED_VARIANTS; ++v)
-      if (fused_candidate(v) && kFused[v].waves == 1 &&
-          (best < 0 || kFused[v].rpl < kFused[best].rpl ||
-           (kFused[v].rpl == kFused[best].rpl && kFused[v].len > kFused[best].len)))
-        best = v;
-    return best;
-  }
+  bool fused_candidate(int v) const { return fused_variant_candidate(v, have_fz, fused_width); }
+  int default_fused() const { return default_fused_variant(have_fz, fused_width); }
   const FusedDev& fused_layout(bool upper) const { return fz[kFused[fused_var].rpl - 1][upper]; }
   // puts a variant in use: its grid is bounded by probes of both of its kernels
   void select_fused(int var) {
@@ -969,11 +1000,17 @@ struct Operator {
   void spmv(const TA* x, TA* y, double* partial, const mlk::PcgScalars* S, int gated = 0) {
     const bool f32 = std::is_same<TA, float>::value;
     hipEvent_t ev = prof.begin(stream);
-    auto kernel = spmv_d16 ? (spmv_batch == 16 ? mlk::k_spmv<TA, 16, true> : mlk::k_spmv<TA, 8, true>)
-                           : (spmv_batch == 16 ? mlk::k_spmv<TA, 16> : mlk::k_spmv<TA, 8>);
-    kernel<<<grid_for(n), 256, 0, stream>>>(
-        n, d_soff.p, d_scol.p, d_sdel.p, d_slen.p, f32 ? (const TA*)d_sval32.p : (const TA*)d_sval.p,
-        x, y, partial, S, gated);
+    const TA* val = f32 ? (const TA*)d_sval32.p : (const TA*)d_sval.p;
+    if (spmv_grouped) {
+      auto kernel = spmv_batch == 16 ? mlk::k_spmv_grouped<TA, 16> : mlk::k_spmv_grouped<TA, 8>;
+      kernel<<<grid_for(n), 256, 0, stream>>>(n, d_soff.p, d_goff.p, d_gnum.p, d_gdel.p,
+                                              d_ginfo.p, val, x, y, partial, S, gated);
+    } else {
+      auto kernel = spmv_d16 ? (spmv_batch == 16 ? mlk::k_spmv<TA, 16, true> : mlk::k_spmv<TA, 8, true>)
+                             : (spmv_batch == 16 ? mlk::k_spmv<TA, 16> : mlk::k_spmv<TA, 8>);
+      kernel<<<grid_for(n), 256, 0, stream>>>(n, d_soff.p, d_scol.p, d_sdel.p, d_slen.p, val, x,
+                                              y, partial, S, gated);
+    }
     HIP_CHECK(hipGetLastError());
     prof.end(f32 ? P_SPMV32 : P_SPMV64, ev, stream);
   }
@@ -1148,14 +1185,19 @@ struct Operator {
         }
         return total;
       };
+      // the SpMV's column streams (grouped, when built, or per row) and loads in flight
       float best_t = 1e30f;
       int best_b = 8;
-      for (int bsz : {8, 16}) {
-        spmv_batch = bsz;
-        const float t = time_of([&] { spmv<double>(w_r.p, w_ap.p, nullptr, nullptr); });
-        if (t < best_t) { best_t = t; best_b = bsz; }
-      }
+      bool best_g = false;
+      for (int grouped = have_grouped ? 1 : 0; grouped >= 0; --grouped)
+        for (int bsz : {8, 16}) {
+          spmv_grouped = grouped == 1;
+          spmv_batch = bsz;
+          const float t = time_of([&] { spmv<double>(w_r.p, w_ap.p, nullptr, nullptr); });
+          if (t < best_t) { best_t = t; best_b = bsz; best_g = spmv_grouped; }
+        }
       spmv_batch = best_b;
+      spmv_grouped = best_g;
       if (have_sweep && !(have_fused && fused_on)) {  // the fused kernel has no batches
         const int k0 = sweeps;
         sweeps = std::max(1, sweeps);
@@ -2946,6 +2988,15 @@ ml_status ml_layout_sweeps(int64_t n, const int64_t* offsets, const int64_t* col
       stats[1] = std::max(flo.width, fup.width);
       stats[2] = flo.max_ext;
       stats[3] = fup.max_ext;
+      // the fused variant a handle on this pattern starts with (0: one launch per sweep)
+      bool have[2] = {false, false};
+      for (int h = 0; h < 2; ++h) {
+        FusedSweepLayout a, b;
+        have[h] = fused_sweep_layouts(lp, FUSED_LANES * (h + 1), &a, &b).empty() &&
+                  (h == 0 || std::max(a.width, b.width) <= 32);
+      }
+      const int v = default_fused_variant(have, (int)stats[1]);
+      stats[4] = v < 0 ? 0 : fused_code(v);
     }
     if (lvalues && r && z) fused_sweeps_host(lp, flo, fup, lvalues, r, sweeps, z);
     return ML_OK;
@@ -3171,11 +3222,23 @@ ml_status ml_model_residency(const ml_model* h, int32_t* out) {
   });
 }
 
+ml_status ml_model_spmv_index(const ml_model* h, int64_t* out) {
+  return guarded([&]() -> int {
+    if (!h || !out) throw ArgError("null argument");
+    const Operator& op = h->m.op;
+    const int64_t full = 2 * op.nnz - op.n;
+    out[0] = op.spmv_grouped ? 1 : op.spmv_d16 ? 2 : 4;
+    out[1] = op.spmv_grouped ? op.grouped_bytes
+                             : full * (op.spmv_d16 ? 2 : 4) + (int64_t)op.n * 4 + (op.slices + 1) * 4;
+    return ML_OK;
+  });
+}
+
 ml_status ml_model_index_bytes(const ml_model* h, int32_t* out) {
   return guarded([&]() -> int {
     if (!h || !out) throw ArgError("null argument");
     const Operator& op = h->m.op;
-    out[0] = op.spmv_d16 ? 2 : 4;
+    out[0] = op.spmv_grouped ? 0 : op.spmv_d16 ? 2 : 4;
     out[1] = !op.have_sweep ? 0 : (op.fused_active() || (op.sw_lo.d16 && op.sw_up.d16)) ? 2 : 4;
     return ML_OK;
   });

@@ -176,3 +176,29 @@ def test_a_force_beyond_the_limit_load_fails_on_the_same_increment(gpu):
     ref = oracle.quasi_static(nodes, conn, E, NU, SY, 0.0, fixed, imposed, f, 1.0, 6,
                               [(0.0, 0.0), (0.6, 0.6), (1.0, 0.6)], iterative=True)
     compare(ref, ok)
+
+
+@pytest.mark.parametrize("nen", [4, 10])
+def test_bending_with_jacobi_sweeps_follows_the_oracle(gpu, nen):
+    """The Newton caller with the FP32 factor applied by Jacobi sweeps (ML_MIXED_JACOBI): each
+    PCG solve still meets the reference's 1e-12 residual, so Newton takes the oracle's iterations
+    increment by increment, with displacements within 1e-8 (relative L2), plastic strains within
+    1e-10 and stresses within 1e-6 (relative) of the oracle's FP64 Newton."""
+    nodes, conn, fixed, imposed = bending_bar(nen)
+    amp = [(0.0, 0.0), (1.0, 1.0)]
+    prob = M.Problem(f"bar{nen}", nodes, conn, np.full(len(conn), E), np.full(len(conn), NU),
+                     np.asarray(fixed, dtype=bool), imposed, np.zeros(3 * len(nodes)))
+    model = gpu.Model(prob, precision=gpu.MIXED_JACOBI)
+    incs, info = P.solve_quasi_static(model, P.Step(1.0, 5), P.Amplitude(amp), SY, H)
+    run = oracle.quasi_static(nodes, conn, E, NU, SY, H, fixed, imposed, np.zeros(3 * len(nodes)),
+                              1.0, 5, amp, iterative=True)
+    assert run["status"] == oracle.OK
+    assert info["newton_iterations"] == run["newton"] and info["cuts"] == 0
+    for k, inc in enumerate(incs):
+        if k:
+            assert len(inc.residuals) == len(run["residuals"][k - 1]), k
+        assert rel_l2(inc.displacements, run["u"][k]) <= 1e-8, k
+        assert np.abs(inc.equivalent_plastic_strains - run["p"][k]).max() <= 1e-10, k
+        scale = max(np.abs(run["stresses"][k][..., 3:]).max(), 1.0)
+        assert np.abs(inc.stresses[..., 3:] - run["stresses"][k][..., 3:]).max() <= 1e-6 * scale
+    assert incs[-1].equivalent_plastic_strains.max() > 0

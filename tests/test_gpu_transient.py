@@ -130,3 +130,25 @@ def test_steady_thermal_solve_matches_the_oracle(gpu, name):
     incs, _ = m.solve_step(T.Step(1.0, 4).with_theta(0.5), inc.temperatures)
     for i in incs[1:]:
         assert np.abs(i.temperatures - inc.temperatures).max() < 1e-9 * np.abs(t_ref).max()
+
+
+@pytest.mark.parametrize("name", ["Tet4", "Tet10"])
+def test_jacobi_sweeps_apply_to_the_transient_caller(gpu, name):
+    """The thermal handle with the FP32 factor applied by Jacobi sweeps (ML_MIXED_JACOBI): its
+    one-DOF-per-node pattern, whose rows hold at most 16 strict entries (Tet4), starts on the
+    narrowest fused variant; every step still meets PCG's 1e-12 residual, so the temperatures
+    stay within 1e-8 of the oracle's FP64 steps (relative to the range) and the reactions within
+    1e-6."""
+    import maille_amd as ma
+    nodes, conn, fixed, imposed, f = walled(name)
+    sources = [(e, 300.0) for e in range(len(conn))]
+    m = T.ThermalModel(nodes, conn, K, RHO_C, fixed, imposed, heat_sources=sources,
+                       precision=ma.MIXED_JACOBI)
+    incs, info = m.solve_step(T.Step(4.0, 8), np.zeros(len(nodes)), fluxes=False)
+    run = oracle.transient(nodes, conn, K, RHO_C, fixed, imposed, f, np.zeros(len(nodes)), 4.0, 8)
+    assert run["status"] == oracle.OK
+    for k, inc in enumerate(incs):
+        assert np.abs(inc.temperatures - run["temperatures"][k]).max() <= 1e-8 * 100.0, k
+        rs = max(np.abs(run["reactions"][k]).max(), 1e-300)
+        assert np.abs(inc.reactions - run["reactions"][k]).max() <= 1e-6 * max(rs, 100.0), k
+    assert info["iterations"] > 0

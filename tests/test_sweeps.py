@@ -148,3 +148,23 @@ def test_fused_layout_refuses_a_row_too_long_for_it():
     with pytest.raises(ValueError, match="too long"):
         ma.layout_sweeps(off, cols)
     assert ma.layout_verify(off, cols)["max_row"] == n
+
+
+def test_every_sweep_pattern_starts_on_a_variant_that_exists():
+    """The fused variant a handle starts with before autotuning, chosen on the host from the
+    pattern: one row per lane with every row in registers on the elasticity cube and, for the
+    one-DOF-per-node (thermal) pattern of the same mesh, whose rows hold at most 16 strict
+    entries, the only variant of that width (116, compiled for a bounded register count):
+    never a variant outside the table."""
+    import maille_amd as ma
+    p = meshes.jittered_cube(6)
+    off, cols, l, n = _factored(p)
+    stats = ma.layout_sweeps(off, cols)
+    assert stats["default_variant"] in (124, 132, 148)
+    assert stats["width"] <= stats["default_variant"] % 100
+    # one DOF per node: the node graph's lower pattern in its RCM order (Csr::zeros)
+    g = ma.Graph.from_elements(len(p.nodes), [list(e) for e in p.conn])
+    soff, scols = g.lower_pattern()
+    st = ma.layout_sweeps(soff, scols)
+    assert st["width"] <= 16
+    assert st["default_variant"] == 116
