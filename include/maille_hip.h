@@ -453,8 +453,9 @@ typedef struct ml_batch ml_batch;
 /* Several models of any sizes (already created; they must outlive the batch and keep their
  * stiffness while it is used) solved in one set of launches: their K_ff are concatenated
  * block-diagonally into one operator, each model climbs its own shift ladder in shared IC(0)
- * launches, and one PCG per model runs with its own scalars. precision: ML_FP64 or
- * ML_MIXED_IC0. */
+ * launches, and one PCG per model runs with its own scalars. precision: ML_FP64,
+ * ML_MIXED_IC0 or ML_MIXED_JACOBI (the FP32 factor applied by Jacobi sweeps over the whole
+ * block-diagonal operator, which couple no two models). */
 ml_status ml_batch_create(int64_t count, ml_model* const* models, ml_precision precision,
                           ml_batch** out);
 /* f[k], u[k]: host arrays of 3*n_nodes values of model k; infos[count] optional. The

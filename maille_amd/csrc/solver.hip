@@ -1363,6 +1363,7 @@ struct Operator {
       if (!any) break;
     }
     gather_factor<T>(l);
+    if (f32 && have_sweep) gather_sweeps();
     factored = true;
     return ML_OK;
   }
@@ -1372,7 +1373,9 @@ struct Operator {
   // systems of this block-diagonal operator (nrep = 1). System s owns the vector entries
   // [seg_off[s], seg_off[s+1]) and seg_of[i] is the system of entry i. FP64 operator,
   // iterate and preconditioner vectors (the factor's coefficients in the handle's precision).
-  // The scalars of every system end in *hS.
+  // A block-diagonal operator with Jacobi sweeps (nrep = 1) applies its factor by the sweeps,
+  // which act on each model's rows alone since no entry couples two models; replicated right-
+  // hand sides use the exact solves. The scalars of every system end in *hS.
   template <typename TM>
   int pcg_batch(int nrep, int nseg, const int64_t* seg_off, const int* seg_of, const double* b,
                 double* x, double tol, int64_t maxit, std::vector<mlk::PcgScalars>* hS) {
@@ -1395,14 +1398,20 @@ struct Operator {
     mlk::PcgScalars* S = wb_S.p;
     const TM* rin = r;
     const unsigned vb = grid_for(N), sb = grid_for(nseg);
+    const bool jac = sweeps > 0 && nrep == 1;
+    auto apply_m = [&]() {
+      if (jac) apply_sweeps(rin, z, nullptr);
+      else precondition(rin, y, z, nullptr, nullptr, nullptr, nrep);
+    };
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);  // abort flag
+    if (jac && fused_active()) fill_levels();
     mlk::k_fill_f64<<<vb, 256, 0, stream>>>(x, N, 0.0);
     HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
     mlk::k_seg_dot<double, double><<<nseg, 256, 0, stream>>>(seg_off, b, b, bb);
     mlk::k_seg_init<<<sb, 256, 0, stream>>>(nseg, bb, S, tol, maxit);
     mlk::k_fill_bits<TM><<<vb, 256, 0, stream>>>(y, N, mlk::Bits<TM>::S);
     mlk::k_fill_bits<TM><<<vb, 256, 0, stream>>>(z, N, mlk::Bits<TM>::S);
-    precondition(rin, y, z, nullptr, nullptr, nullptr, nrep);
+    apply_m();
     mlk::k_seg_dot<double, TM><<<nseg, 256, 0, stream>>>(seg_off, r, z, rz);
     mlk::k_seg_rz<<<sb, 256, 0, stream>>>(nseg, rz, S);
     mlk::k_convert<TM, double><<<vb, 256, 0, stream>>>(N, z, p);
@@ -1438,7 +1447,7 @@ struct Operator {
         prof.end(P_VEC, ev, stream);
         mlk::k_seg_dot<double, double><<<nseg, 256, 0, stream>>>(seg_off, r, r, rr);
         mlk::k_seg_converged<<<sb, 256, 0, stream>>>(nseg, rr, S);
-        precondition(rin, y, z, nullptr, nullptr, nullptr, nrep);
+        apply_m();
         mlk::k_seg_dot<double, TM><<<nseg, 256, 0, stream>>>(seg_off, r, z, rz);
         mlk::k_seg_beta<<<sb, 256, 0, stream>>>(nseg, rz, S);
         ev = prof.begin(stream);
@@ -3900,8 +3909,8 @@ ml_status ml_batch_create(int64_t count, ml_model* const* models, ml_precision p
                           ml_batch** out) {
   return guarded([&]() -> int {
     if (!out || count <= 0 || !models) throw ArgError("bad arguments");
-    if (precision != ML_FP64 && precision != ML_MIXED_IC0)
-      throw ArgError("batched solves take ML_FP64 or ML_MIXED_IC0");
+    if (precision != ML_FP64 && precision != ML_MIXED_IC0 && precision != ML_MIXED_JACOBI)
+      throw ArgError("batched solves take ML_FP64, ML_MIXED_IC0 or ML_MIXED_JACOBI");
     for (int64_t k = 0; k < count; ++k)
       if (!models[k] || models[k]->m.all_fixed())
         throw ArgError("every model of a batch needs a free DOF");

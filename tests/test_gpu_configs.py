@@ -122,7 +122,8 @@ def boxes():
     return bs, refs
 
 
-@pytest.mark.parametrize("precision,bound", [("FP64", 1e-10), ("MIXED_IC0", 1e-8)])
+@pytest.mark.parametrize("precision,bound", [("FP64", 1e-10), ("MIXED_IC0", 1e-8),
+                                             ("MIXED_JACOBI", 1e-8)])
 def test_64_uneven_boxes_in_one_batch(gpu, boxes, precision, bound):
     boxes, refs = boxes
     sizes = [b.ndof for b in boxes]

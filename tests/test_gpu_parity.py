@@ -490,7 +490,7 @@ def test_load_cases_of_one_model(gpu):
         assert abs(infos[k]["iterations"] - info1["iterations"]) <= 3
 
 
-@pytest.mark.parametrize("precision", ["FP64", "MIXED_IC0"])
+@pytest.mark.parametrize("precision", ["FP64", "MIXED_IC0", "MIXED_JACOBI"])
 def test_batch_of_uneven_models(gpu, precision):
     """Boxes of uneven sizes and materials solved in one batch (block-diagonal operator, one
     PCG per model) against each model solved alone and against the oracle's PCG."""
