@@ -240,6 +240,15 @@ int ml_pcg_sweeps(const ml_pcg* h);
  * not fit). */
 ml_status ml_pcg_set_sweeps_fused(ml_pcg* h, int fused);
 int ml_pcg_sweeps_fused(const ml_pcg* h);
+/* dot (solve.rs:100-110) as PCG's scalars are computed: on 1, every PCG dot product (|b|, r.z,
+ * p.Ap, |r|) runs in the reference's order (eight sequential lane sums over the chunks of
+ * eight, their fixed combination, the tail in order; one workgroup, about 1 ms per dot on the
+ * benchmark cube) instead of fixed-tree sums of block partials. With ML_FP64, whose SpMV, IC(0),
+ * triangular solves and vector updates already follow the reference's operations, the whole
+ * Pcg::solve (iterative.rs:210-248) is then the reference's bit for bit: the same iterates,
+ * iteration count and solution. 0 (the default) restores the tree sums. Refused for ML_MIXED
+ * (its FP32 operator and residual replacements have no reference counterpart). */
+ml_status ml_pcg_set_reference_dots(ml_pcg* h, int on);
 
 /* ------------------------------------------------------------------ the static model */
 
@@ -262,6 +271,9 @@ ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elemen
 /* ml_pcg_set_sweeps / ml_pcg_sweeps for the model's operator. */
 ml_status ml_model_set_sweeps(ml_model* m, int sweeps);
 int ml_model_sweeps(const ml_model* m);
+/* ml_pcg_set_reference_dots for the model's operator: with ML_FP64 and zero imposed
+ * displacements, Model::solve's displacements are then the reference's bit for bit. */
+ml_status ml_model_set_reference_dots(ml_model* m, int on);
 /* ml_pcg_set_sweeps_fused / ml_pcg_sweeps_fused for the model's operator. */
 ml_status ml_model_set_sweeps_fused(ml_model* m, int fused);
 int ml_model_sweeps_fused(const ml_model* m);

@@ -47,6 +47,7 @@ EXPORTS = (
     "ml_pcg_set_wait_limit", "ml_pcg_set_sweeps", "ml_pcg_sweeps", "ml_model_set_sweeps",
     "ml_model_sweeps", "ml_model_index_bytes", "ml_model_spmv_index", "ml_model_residency",
     "ml_pcg_set_sweeps_fused", "ml_pcg_sweeps_fused", "ml_model_set_sweeps_fused",
+    "ml_pcg_set_reference_dots", "ml_model_set_reference_dots",
     "ml_model_sweeps_fused",
     "ml_model_create", "ml_model_assemble", "ml_model_set_young", "ml_model_factor",
     "ml_model_solve", "ml_model_solve_device", "ml_model_stresses", "ml_model_stresses_device",
@@ -198,6 +199,8 @@ def lib():
             "ml_model_set_sweeps": (_int, [_P, _int]),
             "ml_model_sweeps": (_int, [_P]),
             "ml_model_index_bytes": (_int, [_P, _I32]),
+            "ml_pcg_set_reference_dots": (_int, [_P, _int]),
+            "ml_model_set_reference_dots": (_int, [_P, _int]),
             "ml_model_spmv_index": (_int, [_P, _I]),
             "ml_model_residency": (_int, [_P, _I32]),
             "ml_pcg_set_sweeps_fused": (_int, [_P, _int]),
@@ -577,6 +580,11 @@ class Pcg(_Handle):
         """Variant code of the fused kernel applying the sweeps (0: one launch each)."""
         return lib().ml_pcg_sweeps_fused(self._h)
 
+    def set_reference_dots(self, on: bool = True):
+        """PCG dot products in the reference's eight-lane order (solve.rs:100-110): with FP64,
+        Pcg::solve is then the reference's bit for bit."""
+        _check(lib().ml_pcg_set_reference_dots(self._h, int(bool(on))))
+
 
 
 # ------------------------------------------------------------------ the static model
@@ -675,6 +683,10 @@ class Model(_Handle):
 
     def sweeps_fused(self) -> int:
         return lib().ml_model_sweeps_fused(self._h)
+
+    def set_reference_dots(self, on: bool = True):
+        """Pcg.set_reference_dots for the model's operator."""
+        _check(lib().ml_model_set_reference_dots(self._h, int(bool(on))))
 
     def residency(self):
         """{kernel: (reported, used)} workgroups per CU of the dataflow kernels: the runtime's

@@ -1757,6 +1757,70 @@ k_sweeps_fused(int n, int nblk, int k, const int* __restrict__ eoff,
   }
 }
 
+// ------------------------------------------------------------------ the reference's dot
+constexpr int DOT8_THREADS = 256;
+constexpr int DOT8_PER_THREAD = 16;                        // elements a thread stages per tile
+constexpr int DOT8_TILE = DOT8_THREADS * DOT8_PER_THREAD;  // a multiple of 8
+
+// dot(a, b) in the reference's order (solve.rs:100-110): eight lane sums over the chunks of
+// eight, s_k += a[8c + k] * b[8c + k] for c in order, combined as ((s0 + s1) + (s2 + s3)) +
+// ((s4 + s5) + (s6 + s7)), plus the products of the tail (n mod 8 elements) summed in order,
+// added last. The eight sums are sequential chains, so one workgroup computes them: lanes 0-7
+// of the first wave run the chains over the products of a tile held in LDS, while every
+// thread loads the next tile and, once the chains are through, stores its products in the
+// other LDS buffer. Each product is rounded once and each sum in the reference's order, so
+// the result equals the reference's bit for bit. Written to out[0]; gated off by S->done.
+__global__ __launch_bounds__(DOT8_THREADS) void k_dot8(int64_t n, const double* __restrict__ a,
+                                                      const double* __restrict__ b,
+                                                      double* __restrict__ out,
+                                                      const PcgScalars* __restrict__ S) {
+#pragma clang fp contract(off)
+  if (S && S->done) return;
+  __shared__ double prod[2][DOT8_TILE];
+  const int t = threadIdx.x;
+  const int64_t body = n / 8 * 8;  // elements in full chunks
+  const int64_t tiles = (body + DOT8_TILE - 1) / DOT8_TILE;
+  double s = 0.0;  // lane t < 8: the sum of lane t
+  double va[DOT8_PER_THREAD], vb[DOT8_PER_THREAD];
+  auto load = [&](int64_t tile) {
+#pragma unroll
+    for (int u = 0; u < DOT8_PER_THREAD; ++u) {
+      const int64_t i = tile * DOT8_TILE + (int64_t)u * DOT8_THREADS + t;
+      va[u] = i < body ? a[i] : 0.0;
+      vb[u] = i < body ? b[i] : 0.0;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < DOT8_PER_THREAD; ++u) prod[buf][u * DOT8_THREADS + t] = va[u] * vb[u];
+  };
+  if (tiles > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int64_t tile = 0; tile < tiles; ++tile) {
+    const int buf = (int)(tile & 1);
+    if (tile + 1 < tiles) load(tile + 1);
+    if (t < 8) {
+      const int64_t len = min<int64_t>(DOT8_TILE, body - tile * DOT8_TILE);
+      const double* pr = prod[buf];
+      for (int64_t c = 0; c < len / 8; ++c) s = s + pr[8 * c + t];
+    }
+    if (tile + 1 < tiles) store(buf ^ 1);  // the other buffer, read by no lane this tile
+    __syncthreads();
+  }
+  __shared__ double lane[8];
+  if (t < 8) lane[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    double tail = 0.0;
+    for (int64_t i = body; i < n; ++i) tail = tail + a[i] * b[i];
+    out[0] = ((lane[0] + lane[1]) + (lane[2] + lane[3])) + ((lane[4] + lane[5]) + (lane[6] + lane[7])) +
+             tail;
+  }
+}
+
 // ------------------------------------------------------------------ reductions
 template <typename A, typename B>
 __global__ __launch_bounds__(256) void k_dot(int64_t n, const A* __restrict__ a,

@@ -432,6 +432,7 @@ struct Operator {
   int sweeps = 0;  // Jacobi sweeps per triangular system; 0: the exact dataflow solves
   int sweep_batch = 8, spmv_batch = 8;  // entries per batch of loads (autotuned: 8 or 16)
   bool sweeps_pinned = false;  // set by the caller: the autotuner keeps the count
+  bool ref_dots = false;       // PCG dot products in the reference's order (k_dot8)
   // grids of the dataflow kernels: the full ones, and the ones in use (full >> shrinks)
   int full_tri64 = 0, full_tri32 = 0, full_ic = 0, full_ic64 = 0;
   int cap_tri64 = 0, cap_tri32 = 0;  // dataflow grids before the cap by block count
@@ -1044,6 +1045,13 @@ struct Operator {
     // update measured slower: the extra writes cost more than the gathers they save, which
     // mostly hit the L2); the per-launch ones read it from the update
     const bool y0 = jac && !fused_active();
+    // the reference's dot products (eight sequential lane sums, k_dot8) instead of the tree
+    // sums of block partials: each scalar is then the reference's, bit for bit
+    const bool d8 = ref_dots && !a32;
+    const int parts = d8 ? 1 : vb;  // the partials the scalar kernels reduce
+    auto dot8 = [&](const double* u, const double* v, const mlk::PcgScalars* gate) {
+      mlk::k_dot8<<<1, mlk::DOT8_THREADS, 0, stream>>>(n, u, v, part, gate);
+    };
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);  // abort flag
     if (fused_active()) fill_levels();
     init_scalars(S, maxit, a32, a32);
@@ -1054,18 +1062,20 @@ struct Operator {
       mlk::k_fill_f64<<<vb, 256, 0, stream>>>(x, n, 0.0);
       HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
     }
-    mlk::k_dot<double, double><<<vb, 256, 0, stream>>>(n, b, b, part);
-    mlk::k_init_norm<<<1, mlk::SUM_THREADS, 0, stream>>>(part, vb, S, tol);
+    if (d8) dot8(b, b, nullptr);
+    else mlk::k_dot<double, double><<<vb, 256, 0, stream>>>(n, b, b, part);
+    mlk::k_init_norm<<<1, mlk::SUM_THREADS, 0, stream>>>(part, parts, S, tol);
     HIP_CHECK(hipGetLastError());
     if (jac) {
-      apply_sweeps(rin, z, S, r, part);
+      apply_sweeps(rin, z, S, d8 ? nullptr : r, d8 ? nullptr : part);
     } else {
       fill_sentinel<TM>(y);
       fill_sentinel<TM>(z);
       precondition(rin, y, z, S);
-      mlk::k_dot<double, TM><<<vb, 256, 0, stream>>>(n, r, z, part);
+      if (!d8) mlk::k_dot<double, TM><<<vb, 256, 0, stream>>>(n, r, z, part);
     }
-    mlk::k_init_rz<<<1, mlk::SUM_THREADS, 0, stream>>>(part, vb, S);
+    if (d8) dot8(r, z, S);
+    mlk::k_init_rz<<<1, mlk::SUM_THREADS, 0, stream>>>(part, parts, S);
     mlk::k_convert<TM, TA><<<vb, 256, 0, stream>>>(n, z, p);
     HIP_CHECK(hipGetLastError());
     *hS = read_scalars(S);
@@ -1074,15 +1084,17 @@ struct Operator {
     int64_t launched = 0, batch = 4;
     for (;;) {
       for (int64_t k = 0; k < batch; ++k) {
-        spmv<TA>(p, ap, part, S);
-        mlk::k_alpha<<<1, mlk::SUM_THREADS, 0, stream>>>(part, vb, S);
+        spmv<TA>(p, ap, d8 ? nullptr : part, S);
+        if (d8) dot8((const double*)p, (const double*)ap, S);
+        mlk::k_alpha<<<1, mlk::SUM_THREADS, 0, stream>>>(part, parts, S);
         hipEvent_t ev = prof.begin(stream);
         mlk::k_update_xr<TA, TM><<<vb, 256, 0, stream>>>(n, S, x, r, p, ap, rlow,
                                                          jac ? nullptr : y, jac ? nullptr : z,
                                                          part, y0 ? d_sw_rd.p : nullptr,
                                                          y0 ? w_y0.p : nullptr);
         prof.end(P_VEC, ev, stream);
-        mlk::k_converged<<<1, mlk::SUM_THREADS, 0, stream>>>(part, vb, S);
+        if (d8) dot8(r, r, S);
+        mlk::k_converged<<<1, mlk::SUM_THREADS, 0, stream>>>(part, parts, S);
         if (a32) {
           spmv<double>(x, w_ax.p, nullptr, S, 1);
           mlk::k_replace_residual<TM><<<vb, 256, 0, stream>>>(n, S, b, w_ax.p, r, rlow, part);
@@ -1091,14 +1103,18 @@ struct Operator {
         if (jac) {
           // the residual replacement (ML_MIXED) rewrites r after k_update_xr: then y0 is
           // formed by the first sweep itself
-          apply_sweeps(rin, z, S, r, part, a32 || !y0 ? nullptr : w_y0.p);
+          apply_sweeps(rin, z, S, d8 ? nullptr : r, d8 ? nullptr : part,
+                       a32 || !y0 ? nullptr : w_y0.p);
         } else {
           precondition(rin, y, z, S);
-          ev = prof.begin(stream);
-          mlk::k_dot<double, TM><<<vb, 256, 0, stream>>>(n, r, z, part);
-          prof.end(P_VEC, ev, stream);
+          if (!d8) {
+            ev = prof.begin(stream);
+            mlk::k_dot<double, TM><<<vb, 256, 0, stream>>>(n, r, z, part);
+            prof.end(P_VEC, ev, stream);
+          }
         }
-        mlk::k_beta<<<1, mlk::SUM_THREADS, 0, stream>>>(part, vb, S);
+        if (d8) dot8(r, z, S);
+        mlk::k_beta<<<1, mlk::SUM_THREADS, 0, stream>>>(part, parts, S);
         ev = prof.begin(stream);
         mlk::k_update_p<TA, TM><<<vb, 256, 0, stream>>>(n, S, p, z);
         prof.end(P_VEC, ev, stream);
@@ -3222,6 +3238,24 @@ ml_status ml_model_set_sweeps_fused(ml_model* h, int fused) {
 }
 
 int ml_model_sweeps_fused(const ml_model* h) { return h ? h->m.op.fused_capacity() : 0; }
+
+ml_status ml_pcg_set_reference_dots(ml_pcg* h, int on) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    if (on && h->op.precision == ML_MIXED) throw ArgError("ML_MIXED keeps its tree sums");
+    h->op.ref_dots = on != 0;
+    return ML_OK;
+  });
+}
+
+ml_status ml_model_set_reference_dots(ml_model* h, int on) {
+  return guarded([&]() -> int {
+    if (!h) throw ArgError("null handle");
+    if (on && h->m.op.precision == ML_MIXED) throw ArgError("ML_MIXED keeps its tree sums");
+    h->m.op.ref_dots = on != 0;
+    return ML_OK;
+  });
+}
 
 ml_status ml_model_residency(const ml_model* h, int32_t* out) {
   return guarded([&]() -> int {

@@ -2,9 +2,9 @@
 elements, 3,060,300 free DOF): the assembled K_ff, the SpMV, the IC(0) factor and the
 preconditioner application equal the oracle's bit for bit, and the solves meet size-independent
 properties (true residual, balance of reactions against the applied load, mixed precision
-against FP64). The oracle's full PCG would take minutes on one core, so the solution itself is
-checked through these properties at this size and against the oracle's PCG at the small sizes
-of the other tests."""
+against FP64). The displacements are compared with the oracle's OpenMP PCG (to rounding, its sums
+run in another order), and, with the PCG's dot products in the reference's order, with the
+oracle's serial PCG bit for bit."""
 import time
 
 import numpy as np
@@ -115,3 +115,28 @@ def test_full_size_displacements_match_the_oracle_pcg(gpu, cube, cube_oracle):
         assert info["device_retries"] == 0
         if name == "FP64":
             assert abs(info["iterations"] - it_ref) <= max(3, 0.12 * it_ref)
+
+
+def test_full_size_fp64_solve_with_reference_dots_is_the_oracle_bit_for_bit(gpu, cube,
+                                                                            cube_oracle):
+    """The whole FP64 Pcg::solve of the benchmark cube with the PCG's dot products in the
+    reference's eight-lane order: the device takes the oracle's serial PCG's iteration count and
+    returns its displacements bit for bit at 3,060,300 free DOF (the oracle's serial solve takes
+    about a minute on one host core)."""
+    p = cube
+    s = cube_oracle
+    f = p.f.copy()
+    oracle.add_pressures(p.nodes, p.conn, p.pressures, f)
+    t = time.time()
+    st, u_ref, it_ref, res_ref = s.solve(f)
+    assert st == oracle.OK
+    say(f"oracle serial PCG: {it_ref} iterations in {time.time() - t:.1f} s")
+    t = time.time()
+    with gpu.Model(p) as m:
+        m.set_reference_dots(True)
+        u, _, info = m.solve(f, reactions=False)
+    say(f"device FP64 with reference dots: {info['iterations']} iterations in "
+        f"{time.time() - t:.1f} s, identical: {np.array_equal(u, u_ref)}")
+    assert info["iterations"] == it_ref
+    assert info["residual"] == res_ref
+    assert np.array_equal(u, u_ref)

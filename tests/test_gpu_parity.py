@@ -191,6 +191,46 @@ def test_fp64_solve_matches_oracle_pcg(gpu, name):
     assert np.array_equal(st[..., :3], ref["stresses"][..., :3])
 
 
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_fp64_solve_with_reference_dots_is_the_oracle_bit_for_bit(gpu, name):
+    """With the PCG's dot products in the reference's eight-lane order (k_dot8, solve.rs:
+    100-110), every operation of the FP64 Pcg::solve follows the reference's: the device takes
+    the oracle's iteration count and returns its displacements and reactions bit for bit."""
+    p = PROBLEMS[name]
+    print(f"reference-dot solve {name}", flush=True)
+    ref = oracle.model_solve(p, iterative=True)
+    assert ref["status"] == oracle.OK
+    with gpu.Model(p) as m:
+        m.set_reference_dots(True)
+        u, r, info = m.solve()
+        assert info["iterations"] == ref["iterations"]
+        assert np.array_equal(u, ref["displacements"])
+        assert info["residual"] == ref["residual"]
+        # and back to the tree sums: rounding-level agreement only
+        m.set_reference_dots(False)
+        u2, _, info2 = m.solve()
+        assert rel_l2(u2, ref["displacements"]) <= 1e-10
+
+
+def test_pcg_with_reference_dots_is_the_oracle_pcg_bit_for_bit(gpu):
+    """Pcg::new + Pcg::solve on an assembled CSR, FP64 with the reference's dot order: the
+    oracle's PCG iterate for iterate (solution, iteration count, final residual)."""
+    p = PROBLEMS["jittered_cube_8"]
+    s = _oracle_system(p)
+    off, cols, vals = s.csr()
+    b = np.sin(0.3 * np.arange(s.n)) + 0.5
+    st, x_ref, it_ref, res_ref, shift = oracle.Csr.from_arrays(off, cols, vals).pcg(b)
+    assert st == oracle.OK
+    pcg = gpu.Pcg(gpu.Csr(off, cols, vals))
+    pcg.set_reference_dots(True)
+    x, info = pcg.solve(b, info=True)
+    assert info["iterations"] == it_ref and info["shift_index"] == shift
+    assert np.array_equal(x, x_ref)
+    assert info["residual"] == res_ref
+    with pytest.raises(ValueError):
+        gpu.Pcg(gpu.Csr(off, cols, vals), precision=gpu.MIXED).set_reference_dots(True)
+
+
 @pytest.mark.parametrize("precision", ["MIXED", "MIXED_IC0", "MIXED_JACOBI"])
 @pytest.mark.parametrize("name", list(PROBLEMS))
 def test_mixed_solve_matches_oracle(gpu, name, precision):
