@@ -63,6 +63,37 @@ __device__ inline T await_value(T* p, int* abort_flag, long long limit) {
   }
 }
 
+// Tagged FP64 levels: a value published for other workgroups carries an 8-bit launch tag in
+// the low bits of its significand, so that a slot needs no reset between launches (a reader
+// waits for its own launch's tag instead of for the end of a sentinel). The value itself is
+// level_value(v), v with those bits cleared: v rounded toward zero to 44 fraction bits, a
+// deterministic rounding every reader and writer applies alike.
+constexpr unsigned long long LEVEL_TAG_MASK = 0xffull;
+__device__ inline double level_value(double v) {
+  return __longlong_as_double((long long)(bits_of(v) & ~LEVEL_TAG_MASK));
+}
+
+// Value of a tagged level slot once it holds `tag`, polled like await_value (bounded; 0 on
+// giving up), with the tag cleared.
+__device__ inline double await_tagged(double* p, unsigned tag, int* abort_flag, long long limit) {
+  typedef unsigned long long U;
+  long long t0 = -1;
+  for (int it = 0;; ++it) {
+    U u = __hip_atomic_load(reinterpret_cast<U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((u & LEVEL_TAG_MASK) == tag) return __longlong_as_double((long long)(u & ~LEVEL_TAG_MASK));
+    if ((it & 15) == 0) {
+      if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return 0.0;
+      const long long now = wall_clock64();
+      if (t0 < 0) t0 = now;
+      if (now - t0 > limit) {
+        __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return 0.0;
+      }
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
 // Co-residency probe of a persistent kernel's grid (a launch that does nothing else): thread
 // 0 of every workgroup counts it in probe[0], then waits, at most `limit` ticks of the steady
 // wall clock, until all gridDim.x workgroups have arrived; one that gives up sets probe[1]. The
@@ -1535,7 +1566,9 @@ __global__ __launch_bounds__(256) void k_spmv_rep(int n, int nrep, const int* __
 // D16: columns are read as 16-bit distances from the row, j = i + sign * d (sign -1 for the
 // lower triangle, +1 for the upper one), else as int32 indices. With out_rd, out * rd is also
 // written (the next system's starting iterate, the same product FIRST would form). With
-// `partial`, the block's share of dot.out (the PCG's r.z) is written, in a fixed tree.
+// `partial`, the block's share of dot.out (the PCG's r.z) is written, in a fixed tree. With
+// `level`, out is an intermediate iterate and is written as level_value(out), the rounding the
+// fused sweeps give the iterates they exchange.
 template <bool FIRST, bool D16, int BATCH = 8>
 __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ soff,
                                                const int* __restrict__ scol,
@@ -1545,7 +1578,7 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
                                                const double* __restrict__ rd,
                                                const double* __restrict__ rhs,
                                                const double* __restrict__ in,
-                                               double* __restrict__ out,
+                                               double* __restrict__ out, int level,
                                                double* __restrict__ out_rd,
                                                const double* __restrict__ dot,
                                                double* __restrict__ partial,
@@ -1582,7 +1615,7 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
       s = s + (double)sval[at] * (FIRST ? rhs[c] * rd[c] : in[c]);
     }
     const double y = (rhs[i] - s) * rd[i];
-    out[i] = y;
+    out[i] = level ? level_value(y) : y;
     if (out_rd) out_rd[i] = y * rd[i];
     if (dot) dv = dot[i] * y;
   }
@@ -1611,11 +1644,13 @@ constexpr int SWEEP_STAGE = 4;    // external values a lane has in flight while 
 // holding the previous iterate at the block's own rows (slots 0..R-1) and at its distinct
 // external columns (slots R..; the columns themselves are kept in LDS too): sweep 1 starts
 // from rhs * rd; the result of sweep j < k is
-// published (device-scope) into level j of lev, and the external columns of that level are
-// then awaited (bounded waits on the sentinel, as in the triangular solves) and staged for
-// sweep j + 1. The last sweep writes out and, with partial, each 256-row sub-block's share of
-// dot.out (the partials of k_sweep's grid). Each row of refill (the other system's levels,
-// which nothing reads in this launch) is reset to the sentinel for that system's next launch.
+// rounded to level_value and published (device-scope) into level j of lev with this launch's
+// `tag` (1..255) in its low bits, and the external columns of that level are then awaited
+// (bounded waits for the tag, as the triangular solves wait on the sentinel) and staged for
+// sweep j + 1. A slot holds another launch's tag until this launch writes it, so the levels
+// need no reset between launches as long as the host gives every launch since the levels
+// were last cleared a tag of its own. The last sweep writes out (not rounded) and, with
+// partial, each 256-row sub-block's share of dot.out (the partials of k_sweep's grid).
 // MINW: waves per SIMD the register allocation must allow (1: no bound).
 // nblk < 0: co-residency probe of the grid.
 template <int MAXLEN, bool REV, int MINW = 1, int RPL = 1>
@@ -1624,10 +1659,10 @@ k_sweeps_fused(int n, int nblk, int k, const int* __restrict__ eoff,
                const uint8_t* __restrict__ elen, const float* __restrict__ eval,
                const uint16_t* __restrict__ eidx, const int* __restrict__ xoff,
                const int* __restrict__ xcol, const double* __restrict__ rd,
-               const double* __restrict__ rhs, double* __restrict__ lev,
-               double* __restrict__ refill, double* __restrict__ out,
-               const double* __restrict__ dot, double* __restrict__ partial,
-               const PcgScalars* __restrict__ S, int* abort_flag, long long limit) {
+               const double* __restrict__ rhs, double* __restrict__ lev, unsigned tag,
+               double* __restrict__ out, const double* __restrict__ dot,
+               double* __restrict__ partial, const PcgScalars* __restrict__ S,
+               int* abort_flag, long long limit) {
 #pragma clang fp contract(off)
   static_assert(MAXLEN % 4 == 0, "entries are held in quads");
   constexpr int R = SWEEP_LANES * RPL;
@@ -1710,14 +1745,19 @@ k_sweeps_fused(int n, int nblk, int k, const int* __restrict__ eoff,
       }
       if (j >= k) break;
       double* level = lev + (int64_t)(j - 1) * n;
+      double yl[RPL];
 #pragma unroll
       for (int h = 0; h < RPL; ++h) {
         const int i = b * R + h * SWEEP_LANES + t;
-        if (i < n) publish(level + i, y[h]);
+        yl[h] = level_value(y[h]);
+        if (i < n)
+          __hip_atomic_store(reinterpret_cast<unsigned long long*>(level + i),
+                             (unsigned long long)bits_of(yl[h]) | tag, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
       }
       __syncthreads();  // every lane has read the previous iterate from LDS
 #pragma unroll
-      for (int h = 0; h < RPL; ++h) buf[h * SWEEP_LANES + t] = y[h];
+      for (int h = 0; h < RPL; ++h) buf[h * SWEEP_LANES + t] = yl[h];
       for (int e0 = 0; e0 < nx; e0 += SWEEP_STAGE * SWEEP_LANES) {
         int c[SWEEP_STAGE];
         unsigned long long got[SWEEP_STAGE];
@@ -1735,19 +1775,16 @@ k_sweeps_fused(int n, int nblk, int k, const int* __restrict__ eoff,
         for (int u = 0; u < SWEEP_STAGE; ++u)
           if (c[u] >= 0)
             buf[R + e0 + u * SWEEP_LANES + t] =
-                got[u] != SENTINEL64 ? __longlong_as_double((long long)got[u])
-                                     : await_value(level + c[u], abort_flag, limit);
+                (got[u] & LEVEL_TAG_MASK) == tag
+                    ? __longlong_as_double((long long)(got[u] & ~LEVEL_TAG_MASK))
+                    : await_tagged(level + c[u], tag, abort_flag, limit);
       }
       __syncthreads();
     }
 #pragma unroll
     for (int h = 0; h < RPL; ++h) {
       const int i = b * R + h * SWEEP_LANES + t;
-      if (i < n) {
-        out[i] = y[h];
-        for (int m = 0; m + 1 < k; ++m)
-          reinterpret_cast<unsigned long long*>(refill)[(int64_t)m * n + i] = SENTINEL64;
-      }
+      if (i < n) out[i] = y[h];
       if (partial) {
         const double bs = block_sum<SWEEP_LANES>(i < n && dot ? dot[i] * y[h] : 0.0);
         if (t == 0 && b * RPL + h < nsub) partial[b * RPL + h] = bs;

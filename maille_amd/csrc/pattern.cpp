@@ -1069,6 +1069,14 @@ void fused_sweeps_host(const LowerPattern& lp, const FusedSweepLayout& lower,
           next[i] = (rhs[i] - s) * rd[i];
         }
       }
+      // the iterates the kernel exchanges between sweeps: low 8 significand bits cleared
+      if (j < k)
+        for (i64 i = 0; i < n; ++i) {
+          uint64_t u;
+          std::memcpy(&u, &next[i], 8);
+          u &= ~uint64_t(0xff);
+          std::memcpy(&next[i], &u, 8);
+        }
       prev.swap(next);
     }
     std::copy(prev.begin(), prev.end(), out);

@@ -237,9 +237,9 @@ int fused_resident(void (*kernel)(TArgs...), hipStream_t stream) {
   return resident_per_cu((const void*)kernel, mlk::SWEEP_LANES, stream,
                          [&](int grid, int* probe, long long limit) {
     kernel<<<grid, mlk::SWEEP_LANES, 0, stream>>>(0, -1, 0, nullptr, nullptr, nullptr, nullptr,
-                                                 nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                 nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                 probe, limit);
+                                                 nullptr, nullptr, nullptr, nullptr, nullptr, 0u,
+                                                 nullptr, nullptr, nullptr, nullptr, probe,
+                                                 limit);
   });
 }
 
@@ -428,6 +428,7 @@ struct Operator {
   int fused_per_cu = 0, full_fused = 0, fused_grid = 0;
   DBuf<double> w_lev_lo, w_lev_up;  // sweeps - 1 levels of n values per system
   int lev_sweeps = 0;               // the sweep count the level buffers are sized for
+  int lev_tag = 1;                  // the next fused launch's level tag
   bool have_sweep = false;
   int sweeps = 0;  // Jacobi sweeps per triangular system; 0: the exact dataflow solves
   int sweep_batch = 8, spmv_batch = 8;  // entries per batch of loads (autotuned: 8 or 16)
@@ -608,7 +609,7 @@ struct Operator {
 
   typedef void (*FusedKernel)(int, int, int, const int*, const uint8_t*, const float*,
                               const uint16_t*, const int*, const int*, const double*,
-                              const double*, double*, double*, double*, const double*, double*,
+                              const double*, double*, unsigned, double*, const double*, double*,
                               const mlk::PcgScalars*, int*, long long);
   // k_sweeps_fused in one of its variants (default: the one in use)
   FusedKernel fused_kernel(bool upper, int var = -1) const {
@@ -658,14 +659,22 @@ struct Operator {
     throw ArgError("fused must be 0, 1 or a variant code 116, 124, 132, 148, 180, 216, 224, 232");
   }
 
-  // every level of both systems back to the sentinel (before a solve: a launch that gave up
-  // may have left published values behind)
+  // every level slot of both systems cleared to tag 0, which no launch uses; the tags start
+  // again from 1
   void fill_levels() {
+    lev_tag = 1;
     if (lev_sweeps < 2) return;
     const int64_t m = (int64_t)(lev_sweeps - 1) * n;
     for (auto* b : {&w_lev_lo, &w_lev_up})
-      mlk::k_fill_bits<double><<<grid_for(m), 256, 0, stream>>>(b->p, m, mlk::Bits<double>::S);
+      mlk::k_fill_bits<double><<<grid_for(m), 256, 0, stream>>>(b->p, m, 0ull);
     HIP_CHECK(hipGetLastError());
+  }
+  // the tag of the next fused launch: each launch since the levels were last cleared has a tag
+  // of its own (1..255), so a slot holds the current launch's tag only once that launch has
+  // written it; after 255 launches the levels are cleared (stream-ordered, a fill kernel)
+  unsigned next_level_tag() {
+    if (lev_tag > (int)mlk::LEVEL_TAG_MASK) fill_levels();
+    return (unsigned)lev_tag++;
   }
 
   // the FP32 factor's values into the sweep layouts, and the reciprocals of its diagonal
@@ -904,30 +913,34 @@ struct Operator {
     double* b = w_s2.p;
     double* z0 = w_s3.p;
     if (fused_active()) {
-      // one launch per system; each resets the other's levels to the sentinel as it goes
+      // one launch per system, each with a level tag of its own
       double* lo = k > 1 ? w_lev_lo.p : nullptr;
       double* up = k > 1 ? w_lev_up.p : nullptr;
       const FusedDev& fl = fused_layout(false);
       const FusedDev& fu = fused_layout(true);
+      const unsigned tag_lo = k > 1 ? next_level_tag() : 0u;
       fused_kernel(false)<<<fused_grid, mlk::SWEEP_LANES, 0, stream>>>(
           n, fl.nblk, k, fl.eoff.p, fl.elen.p, fl.eval.p, fl.eidx.p, fl.xoff.p, fl.xcol.p,
-          d_sw_rd.p, r, lo, up, a, nullptr, nullptr, S, d_flags.p, wait_ticks);
+          d_sw_rd.p, r, lo, tag_lo, a, nullptr, nullptr, S, d_flags.p, wait_ticks);
+      const unsigned tag_up = k > 1 ? next_level_tag() : 0u;
       fused_kernel(true)<<<fused_grid, mlk::SWEEP_LANES, 0, stream>>>(
           n, fu.nblk, k, fu.eoff.p, fu.elen.p, fu.eval.p, fu.eidx.p, fu.xoff.p, fu.xcol.p,
-          d_sw_rd.p, a, up, lo, z, dot, partial, S, d_flags.p, wait_ticks);
+          d_sw_rd.p, a, up, tag_up, z, dot, partial, S, d_flags.p, wait_ticks);
       HIP_CHECK(hipGetLastError());
       prof.end(P_TRI32, ev, stream);
       return;
     }
+    // `level`: an intermediate iterate, rounded as the fused sweeps round the ones they exchange
     auto sweep = [&](const SweepDev& t, const double* rhs, const double* in, double* out,
-                     double* out_rd, bool last) {
+                     double* out_rd, bool last, bool level) {
       const double* dv = last ? dot : nullptr;
       double* pv = last ? partial : nullptr;
       const int sign = &t == &sw_lo ? -1 : 1;
       const bool first = in == nullptr;
       auto go = [&](auto kernel) {
         kernel<<<g, 256, 0, stream>>>(n, t.soff.p, t.scol.p, t.sdel.p, sign, t.slen.p, t.sval.p,
-                                      d_sw_rd.p, rhs, in, out, out_rd, dv, pv, S);
+                                      d_sw_rd.p, rhs, in, out, level ? 1 : 0, out_rd, dv, pv,
+                                      S);
       };
       if (sweep_batch == 16) {
         if (t.d16) first ? go(mlk::k_sweep<true, true, 16>) : go(mlk::k_sweep<false, true, 16>);
@@ -938,9 +951,9 @@ struct Operator {
       }
     };
     // forward, ping-pong between a and b; y ends in a, and z0 = y rd in z0
-    sweep(sw_lo, r, y0, a, k == 1 ? z0 : nullptr, false);
+    sweep(sw_lo, r, y0, a, k == 1 ? z0 : nullptr, false, k > 1);
     for (int m = 2; m <= k; ++m) {
-      sweep(sw_lo, r, a, b, m == k ? z0 : nullptr, false);
+      sweep(sw_lo, r, a, b, m == k ? z0 : nullptr, false, m < k);
       std::swap(a, b);
     }
     // backward from y = a and z0; the sweeps alternate between b and z so that the last lands
@@ -949,7 +962,7 @@ struct Operator {
     const double* prev = z0;
     for (int m = 1; m <= k; ++m) {
       double* out = ((k - m) & 1) ? b : z;
-      sweep(sw_up, y, prev, out, nullptr, m == k);
+      sweep(sw_up, y, prev, out, nullptr, m == k, m < k);
       prev = out;
     }
     HIP_CHECK(hipGetLastError());

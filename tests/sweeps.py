@@ -7,6 +7,9 @@ ascending columns, diagonal last):
     y = r * rd, then k times  y_i <- (r_i - sum_{j<i} l_ij y_j) * rd_i
     z = y * rd, then k times  z_i <- (y_i - sum_{j>i} l_ji z_j) * rd_i
 each row sum taken sequentially in ascending j from 0, every operation rounded to float64.
+The k - 1 intermediate iterates of each system are the values the fused kernel exchanges
+between workgroups with a launch tag in their low 8 significand bits: they are rounded toward
+zero to 44 fraction bits (those bits cleared); the last iterate of each system is not.
 """
 import numpy as np
 
@@ -36,6 +39,11 @@ def _sweep(c, v, m, rhs, x, rd):
     return (rhs - s) * rd
 
 
+def level_value(v):
+    """v with the low 8 bits of its float64 significand cleared."""
+    return (np.asarray(v, dtype=np.float64).view(np.uint64) & np.uint64(0xFFFFFFFFFFFFFF00)).view(np.float64)
+
+
 def jacobi_precondition(off, cols, l, r, sweeps):
     off = np.asarray(off, dtype=np.int64)
     cols = np.asarray(cols, dtype=np.int64)
@@ -51,9 +59,13 @@ def jacobi_precondition(off, cols, l, r, sweeps):
     lo = _padded_rows(n, lr, lc, lv)
     up = _padded_rows(n, lc, lr, lv)  # rows of L^T: column j of L, ascending row
     y = r * rd
-    for _ in range(sweeps):
+    for m in range(1, sweeps + 1):
         y = _sweep(*lo, r, y, rd)
+        if m < sweeps:
+            y = level_value(y)
     z = y * rd
-    for _ in range(sweeps):
+    for m in range(1, sweeps + 1):
         z = _sweep(*up, y, z, rd)
+        if m < sweeps:
+            z = level_value(z)
     return z
