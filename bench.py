@@ -7,9 +7,10 @@ free DOF), interior nodes jittered by up to 0.2h (seed 1), per-element E lognorm
 A step is one device solve with the mesh, pattern and loads resident in HBM: element
 stiffness + gather assembly of K_ff, IC(0) factorisation with the shift ladder, PCG to the
 reference's 1e-12 relative residual, reactions. Three modes are timed: the FP64 drop-in,
-``mixed`` (FP32 SpMV and FP32 IC(0)/triangular solves, FP64 iterate/residual/dots with
-residual replacement) and ``mixed_ic0`` (FP64 SpMV, FP32 IC(0)/triangular solves); the value is
-the fastest mode whose displacements stay within 1e-8 (relative L2) of the FP64 drop-in.
+``mixed_ic0`` (FP64 SpMV, FP32 IC(0) applied by exact triangular solves with FP64 vectors) and
+``mixed_jacobi`` (the FP32 factor applied by Jacobi sweeps; ``--precision mixed`` adds the FP32
+SpMV with residual replacement); the value is the fastest mode whose displacements stay within
+1e-8 (relative L2) of the FP64 drop-in.
 
 Prints one JSON line (see the keys below). Kernel durations come from HIP events recorded on
 the library's stream around every launch of the kernel class during the timed steps.
@@ -107,22 +108,43 @@ def emit(out):
 
 def run_beam_cases(args, ma, meshes):
     """256 load cases of the Tet10 beam solved together (ml_model_solve_many) against the same
-    model's one-at-a-time solves (the reference's sequential Model::solve calls)."""
+    model's one-at-a-time solves (the reference's sequential Model::solve calls), in FP64
+    (replicated exact triangular solves) and mixed_jacobi (the FP32 factor's Jacobi sweeps over
+    every case at once); the line's value is the faster mode within 1e-8 of FP64."""
     p, F = meshes.beam_load_cases(cases=args.cases)
-    m = ma.Model(p)
-    m.factor()
-    chosen = m.autotune(nrhs=len(F))
-    log(f"beam: {p.nfree} free DOF, {len(F)} cases; autotuned {chosen}")
-    m.solve_many(F[:8])
-    t0 = time.perf_counter()
-    U, infos = m.solve_many(F)
-    batched = time.perf_counter() - t0
-    seq = min(args.seq, len(F))
-    t0 = time.perf_counter()
-    for k in range(seq):
-        u1, _, _ = m.solve(F[k], reactions=False)
-        err = float(np.linalg.norm(U[k] - u1) / np.linalg.norm(u1))
-    sequential = (time.perf_counter() - t0) / seq
+    modes, U64 = {}, None
+    for name, code in (("fp64", ma.FP64), ("mixed_jacobi", ma.MIXED_JACOBI)):
+        with ma.Model(p, precision=code) as m:
+            m.factor()
+            chosen = m.autotune(nrhs=len(F))
+            log(f"beam {name}: {p.nfree} free DOF, {len(F)} cases; autotuned {chosen}, "
+                f"{m.sweeps()} sweeps")
+            m.solve_many(F[:8])
+            t0 = time.perf_counter()
+            U, infos = m.solve_many(F)
+            batched = time.perf_counter() - t0
+            seq = min(args.seq, len(F))
+            t0 = time.perf_counter()
+            for k in range(seq):
+                u1, _, _ = m.solve(F[k], reactions=False)
+                err_seq = float(np.linalg.norm(U[k] - u1) / np.linalg.norm(u1))
+            sequential = (time.perf_counter() - t0) / seq
+            if U64 is None:
+                U64 = U
+            err64 = float(max(np.linalg.norm(u - r) / np.linalg.norm(r) for u, r in zip(U, U64)))
+            modes[name] = {"batched_s": round(batched, 3), "cases_per_s": round(len(F) / batched, 2),
+                           "sequential_s_per_case": round(sequential, 4),
+                           "speedup_vs_sequential": round(sequential * len(F) / batched, 2),
+                           "iterations_max": max(i["iterations"] for i in infos),
+                           "sweeps": m.sweeps(), "waves_per_cu": chosen[0],
+                           "cases_per_pass": chosen[1],
+                           "rel_err_last_checked_vs_sequential": err_seq,
+                           "max_rel_err_vs_fp64": err64,
+                           "device_retries": sum(i["device_retries"] for i in infos)}
+            log(f"beam {name}: {batched:.3f} s for {len(F)} cases ({len(F) / batched:.1f} cases/s), "
+                f"sequential {sequential:.4f} s per case, max rel err vs FP64 {err64:.1e}")
+    head = min((k for k in modes if modes[k]["max_rel_err_vs_fp64"] <= 1e-8),
+               key=lambda k: modes[k]["batched_s"])
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -140,16 +162,19 @@ def run_beam_cases(args, ma, meshes):
                "sample": f"the oracle's FreeSystem build with IC(0) ({t_fac:.2f} s) and "
                          f"{nsamp} of the {len(F)} FP64 PCG solves ({per:.2f} s each) measured, "
                          f"scaled to all of them"}
-    emit({"metric": "load cases per second, Tet10 beam 40x8x8 (69,360 free DOF), FP64",
-          "value": round(len(F) / batched, 2), "unit": "cases/s", "n_gpus": 1,
-          "higher_is_better": True, "data": "synthetic", "dtype": "fp64",
+    h = modes[head]
+    emit({"metric": "load cases per second, Tet10 beam 40x8x8 (69,360 free DOF), the faster of "
+                    "FP64 and mixed_jacobi within 1e-8 of FP64",
+          "value": h["cases_per_s"], "unit": "cases/s", "n_gpus": 1,
+          "higher_is_better": True, "data": "synthetic",
+          "dtype": "fp64" if head == "fp64" else "fp64+fp32-ic0-jacobi",
           "config": {"workload": "beam_load_cases", "free_dof": p.nfree, "cases": len(F),
-                     "waves_per_cu": chosen[0], "cases_per_pass": chosen[1]},
-          "batched_s": round(batched, 3), "sequential_s_per_case": round(sequential, 4),
-          "speedup_vs_sequential": round(sequential * len(F) / batched, 2),
-          "iterations_max": max(i["iterations"] for i in infos),
-          "rel_err_last_checked_vs_sequential": err, "cpu_baseline": cpu,
-          "speedup_vs_cpu": round(len(F) / batched / cpu["value"], 1) if cpu else None})
+                     "precision": head, "sweeps": h["sweeps"], "waves_per_cu": h["waves_per_cu"],
+                     "cases_per_pass": h["cases_per_pass"]},
+          "batched_s": h["batched_s"], "sequential_s_per_case": h["sequential_s_per_case"],
+          "speedup_vs_sequential": h["speedup_vs_sequential"], "modes": modes,
+          "cpu_baseline": cpu,
+          "speedup_vs_cpu": round(h["cases_per_s"] / cpu["value"], 1) if cpu else None})
 
 
 def run_box_batch(args, ma, meshes):

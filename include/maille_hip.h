@@ -197,8 +197,17 @@ ml_status ml_pcg_solve_device(ml_pcg* h, const double* b_device, double* x_devic
                               double tol, int64_t maxit, ml_solve_info* info);
 /* y = A x (Csr::multiply, iterative.rs:131-142), host buffers. */
 ml_status ml_pcg_multiply(ml_pcg* h, const double* x, double* y);
+/* The same for nrhs vectors stored one after the other (x[k n + i], y likewise), as the
+ * handle's batched solves multiply (every vector in one launch); each y equals ml_pcg_multiply
+ * of its x bit for bit. n * nrhs below 2^31. */
+ml_status ml_pcg_multiply_many(ml_pcg* h, int64_t nrhs, const double* x, double* y);
 /* z = (L L^T)^-1 r (Csr::precondition, iterative.rs:173-188), host buffers. */
 ml_status ml_pcg_precondition(ml_pcg* h, const double* r, double* z);
+/* The same for nrhs vectors stored one after the other (r[k n + i], z likewise), as the handle's
+ * batched solves (ml_pcg_solve with nrhs > 1) apply it: the exact solves over every vector, or
+ * the handle's Jacobi sweeps over every vector in one launch per sweep; each z equals
+ * ml_pcg_precondition of its r bit for bit. n * nrhs below 2^31. */
+ml_status ml_pcg_precondition_many(ml_pcg* h, int64_t nrhs, const double* r, double* z);
 /* The IC(0) factor values aligned with the CSR (Pcg::l, iterative.rs:196) and the shift. */
 ml_status ml_pcg_factor(ml_pcg* h, double* l, int32_t* shift_index);
 void ml_pcg_destroy(ml_pcg* h);

@@ -43,7 +43,9 @@ EXPORTS = (
     "ml_pattern_ntargets", "ml_pattern_graph", "ml_pattern_free", "ml_layout_verify",
     "ml_layout_sweeps",
     "ml_pcg_create", "ml_pcg_solve", "ml_pcg_solve_device", "ml_pcg_multiply",
-    "ml_pcg_precondition", "ml_pcg_factor", "ml_pcg_destroy", "ml_pcg_trace",
+    "ml_pcg_multiply_many",
+    "ml_pcg_precondition", "ml_pcg_precondition_many", "ml_pcg_factor", "ml_pcg_destroy",
+    "ml_pcg_trace",
     "ml_pcg_set_wait_limit", "ml_pcg_set_sweeps", "ml_pcg_sweeps", "ml_model_set_sweeps",
     "ml_model_sweeps", "ml_model_index_bytes", "ml_model_spmv_index", "ml_model_residency",
     "ml_pcg_set_sweeps_fused", "ml_pcg_sweeps_fused", "ml_model_set_sweeps_fused",
@@ -189,7 +191,9 @@ def lib():
             "ml_pcg_solve": (_int, [_P, _D, _D, _i64, C.c_double, _i64, _S]),
             "ml_pcg_solve_device": (_int, [_P, _P, _P, C.c_double, _i64, _S]),
             "ml_pcg_multiply": (_int, [_P, _D, _D]),
+            "ml_pcg_multiply_many": (_int, [_P, _i64, _D, _D]),
             "ml_pcg_precondition": (_int, [_P, _D, _D]),
+            "ml_pcg_precondition_many": (_int, [_P, _i64, _D, _D]),
             "ml_pcg_factor": (_int, [_P, _D, _I32]),
             "ml_pcg_destroy": (None, [_P]),
             "ml_pcg_trace": (_int, [_P, _D, _I, _I, _I, _i64]),
@@ -542,6 +546,20 @@ class Pcg(_Handle):
         z = np.zeros(self.n)
         _check(lib().ml_pcg_precondition(self._h, _d(r), _d(z)))
         return z
+
+    def multiply_many(self, X):
+        """A x for the rows of X (k x n) as batched solves multiply them."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        Y = np.zeros_like(X)
+        _check(lib().ml_pcg_multiply_many(self._h, X.shape[0], _d(X), _d(Y)))
+        return Y
+
+    def precondition_many(self, R):
+        """The preconditioner applied to the rows of R (k x n) as batched solves apply it."""
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        Z = np.zeros_like(R)
+        _check(lib().ml_pcg_precondition_many(self._h, R.shape[0], _d(R), _d(Z)))
+        return Z
 
     def factor(self):
         """(L values aligned with the CSR, shift index)."""

@@ -1518,43 +1518,6 @@ __global__ __launch_bounds__(256) void k_spmv_grouped(int n, const int* __restri
   }
 }
 
-// y = A x for nrep right-hand sides stored one after the other (x[b n + j]): thread per
-// (b, j), the same row sums as k_spmv.
-template <typename T>
-__global__ __launch_bounds__(256) void k_spmv_rep(int n, int nrep, const int* __restrict__ soff,
-                                                  const int* __restrict__ scol,
-                                                  const int* __restrict__ slen,
-                                                  const T* __restrict__ sval,
-                                                  const T* __restrict__ x, T* __restrict__ y) {
-#pragma clang fp contract(off)
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= (int64_t)n * nrep) return;
-  const int j = (int)(g % n);
-  const T* xb = x + (g - j);
-  const int base = soff[j >> 6] + (j & 63);
-  const int len = slen[j];
-  T s = sval[base] * xb[scol[base]];
-  int k = 1;
-  for (; k + 8 <= len; k += 8) {
-    int c[8];
-    T v[8], xv[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      c[u] = scol[base + ((k + u) << 6)];
-      v[u] = sval[base + ((k + u) << 6)];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) xv[u] = xb[c[u]];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) s += v[u] * xv[u];
-  }
-  for (; k < len; ++k) {
-    const int at = base + (k << 6);
-    s += sval[at] * xb[scol[at]];
-  }
-  y[g] = s;
-}
-
 // ------------------------------------------------------------------ Jacobi sweeps
 // One Jacobi sweep of a triangular system of the IC(0) factor (no counterpart in the
 // reference, which solves it exactly): out_i = (rhs_i - sum_j m_ij in_j) * rd_i over the
@@ -1623,6 +1586,145 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
     const double b = block_sum<256>(dv);
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
   }
+}
+
+// ------------------------------------------------------------------ interleaved vectors
+// nrep vectors of n entries are stacked (entry i of vector b at b n + i) in the batched PCG
+// and interleaved (at i nrep + b) in the batched SpMV and Jacobi sweeps below, where the W
+// lanes of a group take W vectors of one row: the row's entries are one load for the group
+// and each gathered column is W consecutive values.
+
+// out = in moved between the two layouts (TO_IL: stacked -> interleaved), in tiles of 32
+// vectors x 32 entries through LDS so that both sides are read and written in runs. With
+// out_rd (TO_IL only), out_rd[i nrep + b] = in * rd[i] is written as well.
+template <bool TO_IL>
+__global__ __launch_bounds__(256) void k_interleave(int n, int nrep, const double* __restrict__ in,
+                                                    double* __restrict__ out,
+                                                    const double* __restrict__ rd = nullptr,
+                                                    double* __restrict__ out_rd = nullptr) {
+  __shared__ double tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int i0 = blockIdx.x * 32, b0 = blockIdx.y * 32;
+#pragma unroll
+  for (int r = 0; r < 32; r += 8) {
+    if (TO_IL) {  // rows of the tile: vectors, read along the entries
+      const int b = b0 + ty + r, i = i0 + tx;
+      if (b < nrep && i < n) tile[ty + r][tx] = in[(int64_t)b * n + i];
+    } else {      // rows of the tile: entries, read along the vectors
+      const int i = i0 + ty + r, b = b0 + tx;
+      if (i < n && b < nrep) tile[ty + r][tx] = in[(int64_t)i * nrep + b];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 32; r += 8) {
+    if (TO_IL) {
+      const int i = i0 + ty + r, b = b0 + tx;
+      if (i < n && b < nrep) {
+        const double v = tile[tx][ty + r];
+        out[(int64_t)i * nrep + b] = v;
+        if (out_rd) out_rd[(int64_t)i * nrep + b] = v * rd[i];
+      }
+    } else {
+      const int b = b0 + ty + r, i = i0 + tx;
+      if (b < nrep && i < n) out[(int64_t)b * n + i] = tile[tx][ty + r];
+    }
+  }
+}
+
+// y = A x for nrep interleaved vectors: group of W lanes per (row j, W consecutive vectors),
+// lane b of the group sums row j in k_spmv's order (sliced-ELL entry order: the diagonal, then
+// the lower and the upper entries; contraction off), so each vector's result is bit-identical
+// to its own SpMV.
+template <typename T, int W>
+__global__ __launch_bounds__(256) void k_spmm_il(int n, int nrep, const int* __restrict__ soff,
+                                                 const int* __restrict__ scol,
+                                                 const int* __restrict__ slen,
+                                                 const T* __restrict__ sval,
+                                                 const T* __restrict__ x, T* __restrict__ y) {
+#pragma clang fp contract(off)
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int chunks = (nrep + W - 1) / W;
+  const int64_t item = g / W;
+  const int j = (int)(item / chunks);
+  const int b = (int)(item % chunks) * W + (int)(g % W);
+  if (j >= n || b >= nrep) return;
+  const int base = soff[j >> 6] + (j & 63);
+  const int len = slen[j];
+  T s = sval[base] * x[(int64_t)scol[base] * nrep + b];
+  int k = 1;
+  for (; k + 8 <= len; k += 8) {
+    int c[8];
+    T v[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      c[u] = scol[base + ((k + u) << 6)];
+      v[u] = sval[base + ((k + u) << 6)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = x[(int64_t)c[u] * nrep + b];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u] * xv[u];
+  }
+  for (; k < len; ++k) {
+    const int at = base + (k << 6);
+    s += sval[at] * x[(int64_t)scol[at] * nrep + b];
+  }
+  y[(int64_t)j * nrep + b] = s;
+}
+
+// One Jacobi sweep of nrep interleaved vectors (rhs, in and out alike): group of W lanes per
+// (row i, W consecutive vectors). For each vector the arithmetic is k_sweep's (the row summed
+// in ascending column from 0, FP32 coefficients widened exactly, FP64 vectors, contraction
+// off; `level`: the result rounded to level_value), so each is bit-identical to its own
+// k_sweep; a first sweep reads in = rhs * rd written beforehand (the product k_sweep forms).
+// With out_rd, out * rd is written as well (the next system's first iterate).
+template <bool D16, int W>
+__global__ __launch_bounds__(256) void k_sweep_il(int n, int nrep, const int* __restrict__ soff,
+                                                  const int* __restrict__ scol,
+                                                  const uint16_t* __restrict__ sdel, int sign,
+                                                  const int* __restrict__ slen,
+                                                  const float* __restrict__ sval,
+                                                  const double* __restrict__ rd,
+                                                  const double* __restrict__ rhs,
+                                                  const double* __restrict__ in,
+                                                  double* __restrict__ out, int level,
+                                                  double* __restrict__ out_rd) {
+#pragma clang fp contract(off)
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int chunks = (nrep + W - 1) / W;
+  const int64_t item = g / W;
+  const int i = (int)(item / chunks);
+  const int b = (int)(item % chunks) * W + (int)(g % W);
+  if (i >= n || b >= nrep) return;
+  const int base = soff[i >> 6] + (i & 63);
+  const int len = slen[i];
+  auto column = [&](int at) { return D16 ? i + sign * (int)sdel[at] : scol[at]; };
+  auto iterate = [&](int c) { return in[(int64_t)c * nrep + b]; };
+  double s = 0.0;
+  int k = 0;
+  for (; k + 8 <= len; k += 8) {
+    int c[8];
+    float v[8];
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      c[u] = column(base + ((k + u) << 6));
+      v[u] = sval[base + ((k + u) << 6)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = iterate(c[u]);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s = s + (double)v[u] * x[u];
+  }
+  for (; k < len; ++k) {
+    const int at = base + (k << 6);
+    s = s + (double)sval[at] * iterate(column(at));
+  }
+  const int64_t o = (int64_t)i * nrep + b;
+  const double y = (rhs[o] - s) * rd[i];
+  out[o] = level ? level_value(y) : y;
+  if (out_rd) out_rd[o] = y * rd[i];
 }
 
 // ------------------------------------------------------------------ fused Jacobi sweeps

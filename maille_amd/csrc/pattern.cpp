@@ -3,6 +3,7 @@
 #include <cstdlib>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -228,9 +229,10 @@ TransposeLayout transpose_layout(const LowerPattern& lp) {
   return t;
 }
 
-SellLayout sell_layout(const LowerPattern& lp) {
+SellLayout sell_layout(const LowerPattern& lp) { return sell_layout(lp, transpose_layout(lp)); }
+
+SellLayout sell_layout(const LowerPattern& lp, const TransposeLayout& t) {
   const i64 n = lp.n();
-  TransposeLayout t = transpose_layout(lp);
   SellLayout s;
   s.n = n;
   s.slices = (n + SellLayout::C - 1) / SellLayout::C;
@@ -446,54 +448,91 @@ TriLayout tri_layout(int64_t n, const int32_t* off, const int32_t* col, const in
 }
 
 // The grouped column streams of the sliced ELL (DeviceLayouts::goff ...), built from its
-// per-row columns; left empty when some row or offset does not fit their fields.
+// per-row columns; left empty when some row or offset does not fit their fields. Slices are
+// built in contiguous ranges, one per worker, each into a buffer of its own; the buffers are
+// then joined in slice order.
 static void grouped_columns(DeviceLayouts& d) {
   const i64 n = d.n;
   d.goff.clear(); d.gnum.clear(); d.gdel.clear(); d.ginfo.clear();
-  std::vector<int32_t> goff(d.slices + 1, 0), gnum(d.slices, 0);
+  std::vector<int32_t> gnum(d.slices, 0);
   std::vector<uint32_t> ginfo(n, 0);
-  std::vector<int16_t> gdel;
-  auto row_cols = [&](i64 j, std::vector<int32_t>& out) {  // the row's columns, ascending
-    const i64 base = d.soff[j >> 6] + (j & 63);
-    out.resize(d.slen[j]);
-    for (i64 k = 0; k < d.slen[j]; ++k) out[k] = d.scol[base + (k << 6)];
-    std::sort(out.begin(), out.end());
+  struct Part {
+    std::vector<int16_t> gdel;
+    std::vector<i64> ends;  // the buffer's length after each of the range's slices
+    bool ok = true;
   };
-  std::vector<int32_t> cur, next;
-  for (i64 sl = 0; sl < d.slices; ++sl) {
-    const i64 j0 = sl * 64, j1 = std::min<i64>(n, j0 + 64);
+  auto build = [&](i64 s0, i64 s1, Part& part) {
+    std::vector<int32_t> next;
+    auto row_cols = [&](i64 j, std::vector<int32_t>& out) {  // the row's columns, ascending
+      const i64 base = d.soff[j >> 6] + (j & 63);
+      out.resize(d.slen[j]);
+      for (i64 k = 0; k < d.slen[j]; ++k) out[k] = d.scol[base + (k << 6)];
+      std::sort(out.begin(), out.end());
+    };
     std::vector<i64> first;  // first row of each group
     std::vector<std::vector<int32_t>> lists;
-    for (i64 j = j0; j < j1; ++j) {
-      if (d.slen[j] > 255) return;
-      row_cols(j, next);
-      if (!first.empty() && next == lists.back() && j - first.back() < 64) {
-        // same column set as the group in progress
-      } else {
-        first.push_back(j);
-        lists.push_back(next);
+    for (i64 sl = s0; sl < s1; ++sl) {
+      const i64 j0 = sl * 64, j1 = std::min<i64>(n, j0 + 64);
+      first.clear();
+      lists.clear();
+      for (i64 j = j0; j < j1; ++j) {
+        if (d.slen[j] > 255) { part.ok = false; return; }
+        row_cols(j, next);
+        if (!first.empty() && next == lists.back() && j - first.back() < 64) {
+          // same column set as the group in progress
+        } else {
+          first.push_back(j);
+          lists.push_back(next);
+        }
+        const i64 g = (i64)first.size() - 1;
+        const auto& L = lists.back();
+        const i64 pos = std::lower_bound(L.begin(), L.end(), (int32_t)j) - L.begin();
+        if (pos >= (i64)L.size() || L[pos] != j || pos > 255) { part.ok = false; return; }
+        ginfo[j] = (uint32_t)d.slen[j] | ((uint32_t)pos << 8) | ((uint32_t)g << 16) |
+                   ((uint32_t)(j - first.back()) << 24);
       }
-      const i64 g = (i64)first.size() - 1;
-      const auto& L = lists.back();
-      const i64 pos = std::lower_bound(L.begin(), L.end(), (int32_t)j) - L.begin();
-      if (pos >= (i64)L.size() || L[pos] != j || pos > 255) return;
-      ginfo[j] = (uint32_t)d.slen[j] | ((uint32_t)pos << 8) | ((uint32_t)g << 16) |
-                 ((uint32_t)(j - first.back()) << 24);
+      const i64 G = (i64)first.size();
+      i64 W = 0;
+      for (const auto& L : lists) W = std::max<i64>(W, (i64)L.size());
+      gnum[sl] = (int32_t)G;
+      const i64 at = (i64)part.gdel.size();
+      if (at + W * G > INT32_MAX) { part.ok = false; return; }
+      part.gdel.resize(at + W * G, 0);
+      for (i64 g = 0; g < G; ++g)
+        for (i64 q = 0; q < (i64)lists[g].size(); ++q) {
+          const i64 off = (i64)lists[g][q] - first[g];
+          if (off < -32768 || off > 32767) { part.ok = false; return; }
+          part.gdel[at + q * G + g] = (int16_t)off;
+        }
+      part.ends.push_back((i64)part.gdel.size());
     }
-    const i64 G = (i64)first.size();
-    i64 W = 0;
-    for (const auto& L : lists) W = std::max<i64>(W, (i64)L.size());
-    gnum[sl] = (int32_t)G;
-    const i64 at = (i64)gdel.size();
-    if (at + W * G > INT32_MAX) return;
-    gdel.resize(at + W * G, 0);
-    for (i64 g = 0; g < G; ++g)
-      for (i64 q = 0; q < (i64)lists[g].size(); ++q) {
-        const i64 off = (i64)lists[g][q] - first[g];
-        if (off < -32768 || off > 32767) return;
-        gdel[at + q * G + g] = (int16_t)off;
-      }
-    goff[sl + 1] = (int32_t)gdel.size();
+  };
+  const i64 T = d.slices < 256 ? 1 : worker_count();
+  const i64 step = (d.slices + T - 1) / std::max<i64>(T, 1);
+  std::vector<Part> parts(T);
+  {
+    std::vector<std::thread> pool;
+    for (i64 k = 0; k < T; ++k) {
+      const i64 s0 = std::min(d.slices, k * step), s1 = std::min(d.slices, s0 + step);
+      pool.emplace_back([&, s0, s1, k] { build(s0, s1, parts[k]); });
+    }
+    for (auto& th : pool) th.join();
+  }
+  std::vector<int32_t> goff(d.slices + 1, 0);
+  i64 total = 0;
+  for (i64 k = 0; k < T; ++k) {
+    if (!parts[k].ok) return;
+    const i64 s0 = std::min(d.slices, k * step);
+    for (size_t u = 0; u < parts[k].ends.size(); ++u) {
+      if (total + parts[k].ends[u] > INT32_MAX) return;
+      goff[s0 + (i64)u + 1] = (int32_t)(total + parts[k].ends[u]);
+    }
+    total += (i64)parts[k].gdel.size();
+  }
+  std::vector<int16_t> gdel(total);
+  for (i64 k = 0, at = 0; k < T; ++k) {
+    std::copy(parts[k].gdel.begin(), parts[k].gdel.end(), gdel.begin() + at);
+    at += (i64)parts[k].gdel.size();
   }
   d.goff.swap(goff);
   d.gnum.swap(gnum);
@@ -529,30 +568,37 @@ std::string device_layouts(const LowerPattern& lp, const TriCaps& caps, DeviceLa
     for (i64 i = 0; i < n; ++i) d.ic_order[at[level[i]]++] = (int32_t)i;
   }
   // sliced ELL of the full symmetric matrix
-  SellLayout s = sell_layout(lp);
   TransposeLayout t = transpose_layout(lp);
-  d.slices = s.slices;
-  d.soff.assign(s.slice_offsets.begin(), s.slice_offsets.end());
-  d.scol = s.cols;
-  d.ssrc.assign(s.src.begin(), s.src.end());
+  {
+    SellLayout s = sell_layout(lp, t);
+    d.slices = s.slices;
+    d.soff.assign(s.slice_offsets.begin(), s.slice_offsets.end());
+    d.scol.swap(s.cols);
+    d.ssrc.resize(s.src.size());
+    parallel_ranges((i64)s.src.size(), [&](i64 b, i64 e) {
+      for (i64 q = b; q < e; ++q) d.ssrc[q] = (int32_t)s.src[q];
+    });
+  }
   d.slen.resize(n);
   for (i64 j = 0; j < n; ++j)
     d.slen[j] = (int32_t)((lp.offsets[j + 1] - lp.offsets[j]) + (t.offsets[j + 1] - t.offsets[j]));
   {
-    // 16-bit column offsets from the row, when the matrix band allows
-    bool fits = true;
+    // 16-bit column offsets from the row, when the matrix band allows (slices in parallel)
+    std::atomic<bool> fits{true};
     std::vector<int16_t> del(d.scol.size(), 0);
-    for (i64 sl = 0; sl < d.slices && fits; ++sl) {
-      const i64 w = ((i64)d.soff[sl + 1] - d.soff[sl]) / SellLayout::C;
-      for (i64 k = 0; k < w && fits; ++k)
-        for (i64 lane = 0; lane < SellLayout::C; ++lane) {
-          const i64 j = sl * SellLayout::C + lane;
-          const i64 at = d.soff[sl] + k * SellLayout::C + lane;
-          const i64 dlt = j < n ? (i64)d.scol[at] - j : 0;
-          if (dlt < -32768 || dlt > 32767) { fits = false; break; }
-          del[at] = (int16_t)dlt;
-        }
-    }
+    parallel_ranges(d.slices, [&](i64 b, i64 e) {
+      for (i64 sl = b; sl < e && fits; ++sl) {
+        const i64 w = ((i64)d.soff[sl + 1] - d.soff[sl]) / SellLayout::C;
+        for (i64 k = 0; k < w; ++k)
+          for (i64 lane = 0; lane < SellLayout::C; ++lane) {
+            const i64 j = sl * SellLayout::C + lane;
+            const i64 at = d.soff[sl] + k * SellLayout::C + lane;
+            const i64 dlt = j < n ? (i64)d.scol[at] - j : 0;
+            if (dlt < -32768 || dlt > 32767) fits = false;
+            del[at] = (int16_t)dlt;
+          }
+      }
+    });
     if (fits) d.sdel.swap(del);
   }
   grouped_columns(d);
@@ -560,34 +606,44 @@ std::string device_layouts(const LowerPattern& lp, const TriCaps& caps, DeviceLa
   const i64 nf = lp.nnz() - n;
   std::vector<int32_t> foff(n + 1), fcol(nf), fsrc(nf), fdsrc(n);
   for (i64 i = 0; i <= n; ++i) foff[i] = (int32_t)(lp.offsets[i] - i);
-  for (i64 i = 0; i < n; ++i) {
-    i64 o = foff[i];
-    for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p, ++o) {
-      fcol[o] = lp.cols[p];
-      fsrc[o] = (int32_t)p;
+  parallel_ranges(n, [&](i64 b, i64 e) {
+    for (i64 i = b; i < e; ++i) {
+      i64 o = foff[i];
+      for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p, ++o) {
+        fcol[o] = lp.cols[p];
+        fsrc[o] = (int32_t)p;
+      }
+      fdsrc[i] = (int32_t)(lp.offsets[i + 1] - 1);
     }
-    fdsrc[i] = (int32_t)(lp.offsets[i + 1] - 1);
-  }
+  });
   // backward solve in logical rows l = n-1-j
   std::vector<int32_t> boff(n + 1, 0), bcol(nf), bsrc(nf), bdsrc(n);
   for (i64 l = 0; l < n; ++l) {
     const i64 j = n - 1 - l;
     boff[l + 1] = boff[l] + (int32_t)(t.offsets[j + 1] - t.offsets[j]);
   }
-  for (i64 l = 0; l < n; ++l) {
-    const i64 j = n - 1 - l;
-    i64 o = boff[l];
-    for (i64 q = t.offsets[j + 1] - 1; q >= t.offsets[j]; --q, ++o) {
-      bcol[o] = (int32_t)(n - 1 - t.rows[q]);
-      bsrc[o] = (int32_t)t.src[q];
+  parallel_ranges(n, [&](i64 b, i64 e) {
+    for (i64 l = b; l < e; ++l) {
+      const i64 j = n - 1 - l;
+      i64 o = boff[l];
+      for (i64 q = t.offsets[j + 1] - 1; q >= t.offsets[j]; --q, ++o) {
+        bcol[o] = (int32_t)(n - 1 - t.rows[q]);
+        bsrc[o] = (int32_t)t.src[q];
+      }
+      bdsrc[l] = (int32_t)(lp.offsets[j + 1] - 1);
     }
-    bdsrc[l] = (int32_t)(lp.offsets[j + 1] - 1);
-  }
-  std::string err;
+  });
+  TransposeLayout().offsets.swap(t.offsets);
+  std::vector<int32_t>().swap(t.rows);
+  std::vector<int64_t>().swap(t.src);
+  // the two solves' layouts are independent: the backward one on a thread of its own
+  std::string err, err_bw;
+  std::thread bw_thread([&] {
+    d.bw = tri_layout(n, boff.data(), bcol.data(), bsrc.data(), bdsrc.data(), 64, caps, &err_bw);
+  });
   d.fw = tri_layout(n, foff.data(), fcol.data(), fsrc.data(), fdsrc.data(), 64, caps, &err);
-  if (!err.empty()) return err;
-  d.bw = tri_layout(n, boff.data(), bcol.data(), bsrc.data(), bdsrc.data(), 64, caps, &err);
-  return err;
+  bw_thread.join();
+  return err.empty() ? err_bw : err;
 }
 
 namespace {
@@ -674,10 +730,68 @@ std::string check_tri(const char* name, const TriLayout& t, i64 n, i64 nnz, cons
 
 }  // namespace
 
+static std::string check_ic_and_sell(const LowerPattern& lp, const DeviceLayouts& d);
+
+// The checks of the IC(0) order, the sliced ELL and the two triangular-solve layouts are
+// independent: the two solves' run on threads of their own, and the first failure in that
+// order is reported.
 std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, const TriCaps& caps) {
   const i64 n = lp.n(), nnz = lp.nnz();
   if (d.n != n || d.nnz != nnz || (i64)d.off32.size() != n + 1 || (i64)d.col32.size() != nnz)
     return "CSR copy of the wrong size";
+  const i64 nf = nnz - n;
+  std::string err_fw, err_bw;
+  std::thread fw_check([&] {
+    std::vector<int32_t> foff(n + 1), fcol(nf), fsrc(nf);
+    for (i64 i = 0; i <= n; ++i) foff[i] = (int32_t)(lp.offsets[i] - i);
+    for (i64 i = 0; i < n; ++i)
+      for (i64 p = lp.offsets[i], o = foff[i]; p < lp.offsets[i + 1] - 1; ++p, ++o) {
+        fcol[o] = lp.cols[p];
+        fsrc[o] = (int32_t)p;
+      }
+    err_fw = check_tri("forward", d.fw, n, nnz, caps, foff, fcol, fsrc);
+    if (!err_fw.empty()) return;
+    for (i64 q = 0; q < n; ++q)
+      if (d.fw.dsrc[q] != lp.offsets[d.fw.brow[q] + 1] - 1) {
+        err_fw = "forward layout: wrong diagonal";
+        return;
+      }
+  });
+  std::thread bw_check([&] {
+    // backward rows, restated independently of device_layouts: entry (i, j) of L, i > j, goes
+    // to logical row n-1-j; visiting i downwards appends each row's entries in ascending
+    // logical column n-1-i
+    std::vector<int32_t> boff(n + 1, 0), bcol(nf), bsrc(nf);
+    for (i64 i = 0; i < n; ++i)
+      for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) boff[n - lp.cols[p]] += 1;
+    for (i64 r = 0; r < n; ++r) boff[r + 1] += boff[r];
+    {
+      std::vector<int32_t> fill(boff.begin(), boff.end() - 1);
+      for (i64 i = n - 1; i >= 0; --i)
+        for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) {
+          const i64 q = fill[n - 1 - lp.cols[p]]++;
+          bcol[q] = (int32_t)(n - 1 - i);
+          bsrc[q] = (int32_t)p;
+        }
+    }
+    err_bw = check_tri("backward", d.bw, n, nnz, caps, boff, bcol, bsrc);
+    if (!err_bw.empty()) return;
+    for (i64 q = 0; q < n; ++q)
+      if (d.bw.dsrc[q] != lp.offsets[n - d.bw.brow[q]] - 1) {
+        err_bw = "backward layout: wrong diagonal";
+        return;
+      }
+  });
+  std::string err = check_ic_and_sell(lp, d);
+  fw_check.join();
+  bw_check.join();
+  if (!err.empty()) return err;
+  return err_fw.empty() ? err_bw : err_fw;
+}
+
+// The IC(0) order, the sliced ELL and its 16-bit and grouped column forms.
+static std::string check_ic_and_sell(const LowerPattern& lp, const DeviceLayouts& d) {
+  const i64 n = lp.n(), nnz = lp.nnz();
   // IC(0) order: a permutation in which every row comes after the rows it reads
   {
     if ((i64)d.ic_order.size() != n) return "IC(0) order of the wrong length";
@@ -741,39 +855,6 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, const 
       }
     }
   }
-  // triangular solves, against the logical rows they were built from
-  const i64 nf = nnz - n;
-  std::vector<int32_t> foff(n + 1), fcol(nf), fsrc(nf);
-  for (i64 i = 0; i <= n; ++i) foff[i] = (int32_t)(lp.offsets[i] - i);
-  for (i64 i = 0; i < n; ++i)
-    for (i64 p = lp.offsets[i], o = foff[i]; p < lp.offsets[i + 1] - 1; ++p, ++o) {
-      fcol[o] = lp.cols[p];
-      fsrc[o] = (int32_t)p;
-    }
-  std::string err = check_tri("forward", d.fw, n, nnz, caps, foff, fcol, fsrc);
-  if (!err.empty()) return err;
-  for (i64 q = 0; q < n; ++q)
-    if (d.fw.dsrc[q] != lp.offsets[d.fw.brow[q] + 1] - 1) return "forward layout: wrong diagonal";
-  // backward rows, restated independently of device_layouts: entry (i, j) of L, i > j, goes
-  // to logical row n-1-j; visiting i downwards appends each row's entries in ascending logical
-  // column n-1-i
-  std::vector<int32_t> boff(n + 1, 0), bcol(nf), bsrc(nf);
-  for (i64 i = 0; i < n; ++i)
-    for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) boff[n - lp.cols[p]] += 1;
-  for (i64 r = 0; r < n; ++r) boff[r + 1] += boff[r];
-  {
-    std::vector<int32_t> fill(boff.begin(), boff.end() - 1);
-    for (i64 i = n - 1; i >= 0; --i)
-      for (i64 p = lp.offsets[i]; p < lp.offsets[i + 1] - 1; ++p) {
-        const i64 q = fill[n - 1 - lp.cols[p]]++;
-        bcol[q] = (int32_t)(n - 1 - i);
-        bsrc[q] = (int32_t)p;
-      }
-  }
-  err = check_tri("backward", d.bw, n, nnz, caps, boff, bcol, bsrc);
-  if (!err.empty()) return err;
-  for (i64 q = 0; q < n; ++q)
-    if (d.bw.dsrc[q] != lp.offsets[n - d.bw.brow[q]] - 1) return "backward layout: wrong diagonal";
   return std::string();
 }
 

@@ -51,6 +51,8 @@ struct SellLayout {
   i64 entries() const { return (i64)cols.size(); }
 };
 SellLayout sell_layout(const LowerPattern& lp);
+struct TransposeLayout;
+SellLayout sell_layout(const LowerPattern& lp, const TransposeLayout& t);  // t: lp's transpose
 
 // Rows of L^T (the columns of L): for row j, the rows i > j with l_ij != 0, ascending i, and
 // the lower-CSR index of l_ij.

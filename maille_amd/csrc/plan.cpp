@@ -1,6 +1,7 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <thread>
@@ -140,53 +141,68 @@ int build_plan(const MeshView& m, const uint8_t* fixed, const double* imposed, P
     if (free_of[g] >= 0) p.rank_of_dof[g] = (int32_t)rank[free_of[g]];
 
   // model.rs:1480-1492 scatter order: element e, row li, column lj; the entry (rank[a],
-  // rank[b]) with rank[b] <= rank[a] receives ke[li][lj]. Each element reaches an entry at
-  // most once, so listing contributions element by element keeps the reference's order.
-  const int pairs = mm * (mm + 1) / 2;
-  std::vector<uint32_t> hit((size_t)ne * pairs, UINT32_MAX);  // 2*q + (li,lj) orientation
-  std::vector<int64_t> counts(p.lower.nnz() + 1, 0);
-  bool bad = false;
-  parallel_for(ne, [&](i64 b, i64 en) {
-    for (i64 e = b; e < en; ++e) {
-      int k = 0;
-      for (int li = 0; li < mm; ++li)
-        for (int lj = li; lj < mm; ++lj, ++k) {
-          int32_t ra = p.rank_of_dof[dof_of(m, e, li)];
-          int32_t rb = p.rank_of_dof[dof_of(m, e, lj)];
-          if (ra < 0 || rb < 0) continue;
-          // the reference adds ke[row][col] where row is the DOF of larger rank
-          i64 q = p.lower.find(std::max(ra, rb), std::min(ra, rb));
-          if (q < 0) { bad = true; continue; }
-          uint32_t orient = (ra >= rb) ? 0u : 1u;  // 0: ke[li][lj], 1: ke[lj][li]
-          hit[(size_t)e * pairs + k] = (uint32_t)(2 * q) + orient;
-        }
-    }
-  });
-  if (bad) {
-    *err = "element entry outside the pattern";
-    return 4;
-  }
-  for (size_t t = 0; t < hit.size(); ++t)
-    if (hit[t] != UINT32_MAX) counts[(hit[t] >> 1) + 1] += 1;
-  for (i64 q = 0; q < p.lower.nnz(); ++q) counts[q + 1] += counts[q];
-  p.asm_off = counts;
-  p.asm_slot.resize(counts[p.lower.nnz()]);
+  // rank[b]) with rank[b] <= rank[a] receives ke[li][lj] (the row is the DOF of larger rank).
+  // Each element reaches an entry at most once, so an entry's contributions listed in
+  // ascending element order keep the reference's order. They are built row by row, rows in
+  // parallel: row ra walks the elements of its DOF's node in ascending order, and for each the
+  // element's DOFs of rank <= ra (two passes: counts per entry, then slots), so that each entry
+  // is written only by the thread that owns its row.
   {
-    std::vector<int64_t> cursor(counts.begin(), counts.end() - 1);
-    std::vector<int> pair_li(pairs), pair_lj(pairs);
-    int k = 0;
-    for (int li = 0; li < mm; ++li)
-      for (int lj = li; lj < mm; ++lj, ++k) { pair_li[k] = li; pair_lj[k] = lj; }
+    const int nen = m.nen, dpn = m.dpn;
+    std::vector<int64_t> nel_off(m.n_nodes + 1, 0);
     for (i64 e = 0; e < ne; ++e)
-      for (int k2 = 0; k2 < pairs; ++k2) {
-        uint32_t h = hit[(size_t)e * pairs + k2];
-        if (h == UINT32_MAX) continue;
-        int row = (h & 1u) ? pair_lj[k2] : pair_li[k2];
-        int col = (h & 1u) ? pair_li[k2] : pair_lj[k2];
-        p.asm_slot[cursor[h >> 1]++] = (uint32_t)((i64)(row * mm + col) * ne + e);
+      for (int a = 0; a < nen; ++a) nel_off[m.conn[e * nen + a] + 1] += 1;
+    for (i64 v = 0; v < m.n_nodes; ++v) nel_off[v + 1] += nel_off[v];
+    std::vector<int64_t> nel_item(nel_off[m.n_nodes]);  // e * nen + position of the node
+    {
+      std::vector<int64_t> cursor(nel_off.begin(), nel_off.end() - 1);
+      for (i64 e = 0; e < ne; ++e)
+        for (int a = 0; a < nen; ++a) nel_item[cursor[m.conn[e * nen + a]]++] = e * nen + a;
+    }
+    std::vector<i64> dof_of_rank(count);
+    for (i64 g = 0; g < p.ndof; ++g)
+      if (p.rank_of_dof[g] >= 0) dof_of_rank[p.rank_of_dof[g]] = g;
+    // visits (entry q, Ke slot) of row ra in ascending element order; false on an entry
+    // outside the pattern
+    auto row_entries = [&](i64 ra, auto&& visit) {
+      const i64 g = dof_of_rank[ra];
+      const i64 v = g / dpn;
+      const int c = (int)(g % dpn);
+      for (int64_t t = nel_off[v]; t < nel_off[v + 1]; ++t) {
+        const i64 e = nel_item[t] / nen;
+        const int la = dpn * (int)(nel_item[t] % nen) + c;
+        for (int lj = 0; lj < mm; ++lj) {
+          const int32_t rb = p.rank_of_dof[dof_of(m, e, lj)];
+          if (rb < 0 || rb > ra) continue;
+          const i64 q = p.lower.find(ra, rb);
+          if (q < 0) return false;
+          visit(q, (uint32_t)((i64)(la * mm + lj) * ne + e));
+        }
       }
+      return true;
+    };
+    std::vector<int64_t> counts(p.lower.nnz() + 1, 0);
+    std::atomic<bool> bad{false};
+    parallel_for(count, [&](i64 b, i64 en) {
+      for (i64 ra = b; ra < en; ++ra)
+        if (!row_entries(ra, [&](i64 q, uint32_t) { counts[q + 1] += 1; })) bad = true;
+    });
+    if (bad) {
+      *err = "element entry outside the pattern";
+      return 4;
+    }
+    for (i64 q = 0; q < p.lower.nnz(); ++q) counts[q + 1] += counts[q];
+    p.asm_slot.resize(counts[p.lower.nnz()]);
+    parallel_for(count, [&](i64 b, i64 en) {
+      std::vector<int64_t> cursor;
+      for (i64 ra = b; ra < en; ++ra) {
+        const i64 q0 = p.lower.offsets[ra];
+        cursor.assign(counts.begin() + q0, counts.begin() + p.lower.offsets[ra + 1]);
+        row_entries(ra, [&](i64 q, uint32_t slot) { p.asm_slot[cursor[q - q0]++] = slot; });
+      }
+    });
+    p.asm_off.swap(counts);
   }
-  std::vector<uint32_t>().swap(hit);
 
   // coupling K_fc u_c (model.rs:1489): row a free, column gj fixed, element then lj order
   bool any_imposed = false;

@@ -4,7 +4,10 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
+#include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "pattern.hpp"
@@ -25,6 +28,27 @@ struct MeshView {
 // with the offending element (or node) in *where.
 int check_mesh(const MeshView& m, i64* where);
 
+// std::allocator whose value-less construct() leaves the element uninitialised, so that
+// resizing a large index array that is written in full next costs no serial zero fill.
+template <class T>
+struct DefaultInitAllocator : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = DefaultInitAllocator<U>;
+  };
+  DefaultInitAllocator() = default;
+  template <class U>
+  DefaultInitAllocator(const DefaultInitAllocator<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+  }
+};
+
 struct Plan {
   i64 n_nodes = 0, n_elements = 0, ndof = 0, nfree = 0;
   int nen = 0, m = 0;  // m = dpn nen
@@ -36,7 +60,7 @@ struct Plan {
   // K_ff entry q = sum over asm_slot[asm_off[q] .. asm_off[q+1]) of Ke slots, element order.
   // A Ke slot is (j*m + i) * n_elements + e for ke[e][j][i] (entry-major, element fastest).
   std::vector<int64_t> asm_off;
-  std::vector<uint32_t> asm_slot;
+  std::vector<uint32_t, DefaultInitAllocator<uint32_t>> asm_slot;  // every element written
   // coupling (K_fc u_c) of solver row a: sum of ke[slot] * imposed[dof] over
   // cpl_off[a]..cpl_off[a+1], element order; empty when every imposed value is zero.
   std::vector<int64_t> cpl_off;

@@ -12,12 +12,14 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/maille_hip.h"
@@ -293,6 +295,41 @@ int default_fused_variant(const bool have[2], int width) {
   return best;
 }
 
+// The host layouts of the Jacobi sweeps of one lower pattern, built and checked together: the
+// per-sweep layouts and the fused ones in blocks of 256 and of 512 rows (two rows per lane,
+// kept only for rows of at most 32 strict entries), the three on threads of their own.
+struct SweepHostLayouts {
+  std::string err;  // the first failed check, or empty
+  SweepLayout lo, up;
+  FusedSweepLayout fused[2][2];  // [rows per lane - 1][0 lower, 1 upper]
+  bool have[2] = {false, false};
+};
+
+static SweepHostLayouts build_sweep_layouts(const std::vector<int64_t>& offsets,
+                                            const std::vector<int32_t>& cols) {
+  LowerPattern lp;
+  lp.offsets = offsets;
+  lp.cols = cols;
+  SweepHostLayouts s;
+  std::string ferr[2];
+  auto fused = [&](int h) {
+    FusedSweepLayout& flo = s.fused[h][0];
+    FusedSweepLayout& fup = s.fused[h][1];
+    // a pattern that does not fit the layout keeps the other one, or one launch per sweep
+    if (!fused_sweep_layouts(lp, FUSED_LANES * (h + 1), &flo, &fup).empty()) return;
+    if (h == 1 && std::max(flo.width, fup.width) > 32) return;
+    ferr[h] = check_fused_sweep_layouts(lp, flo, fup);
+    s.have[h] = ferr[h].empty();
+  };
+  std::thread f256(fused, 0), f512(fused, 1);
+  s.err = sweep_layouts(lp, &s.lo, &s.up);
+  if (s.err.empty()) s.err = check_sweep_layouts(lp, s.lo, s.up);
+  f256.join();
+  f512.join();
+  for (int h = 0; h < 2 && s.err.empty(); ++h) s.err = ferr[h];
+  return s;
+}
+
 // ====================================================================== Operator
 struct Operator {
   int n = 0;
@@ -308,6 +345,10 @@ struct Operator {
   // reference lower CSR
   std::vector<int64_t> h_off;
   std::vector<int32_t> h_col;
+  // the sweep layouts under construction from h_off and h_col (a handle whose precision runs
+  // sweeps from the start begins them while its other layouts are built); declared after
+  // them, so that a handle destroyed early waits for the build before they go
+  std::future<SweepHostLayouts> sweep_build;
   DBuf<int> d_off, d_col;
   DBuf<int> d_icorder;   // IC(0) processing order (rows by dependency level)
   DBuf<double> d_a;      // values
@@ -460,14 +501,17 @@ struct Operator {
   // against every bound the kernels rely on before any of them reaches the device.
   void set_pattern(const LowerPattern& lp) {
     init_stream();
+    h_off = lp.offsets;
+    h_col = lp.cols;
+    if (precision == ML_MIXED_JACOBI)
+      sweep_build = std::async(std::launch::async, build_sweep_layouts, std::cref(h_off),
+                               std::cref(h_col));
     DeviceLayouts d;
     std::string err = device_layouts(lp, kTriCaps, &d);
     if (err.empty()) err = check_layouts(lp, d, kTriCaps);
     if (!err.empty()) throw ArgError(err);
     n = (int)lp.n();
     nnz = lp.nnz();
-    h_off = lp.offsets;
-    h_col = lp.cols;
     depth = d.depth;
     max_row = d.max_row;
     slices = (int)d.slices;
@@ -565,31 +609,22 @@ struct Operator {
     if (k > 0 && precision == ML_FP64)
       throw ArgError("the FP64 drop-in applies its factor exactly (sweeps need a mixed handle)");
     if (k > 0 && !have_sweep) {
-      LowerPattern lp;
-      lp.offsets = h_off;
-      lp.cols = h_col;
-      SweepLayout lo, up;
-      std::string err = sweep_layouts(lp, &lo, &up);
-      if (err.empty()) err = check_sweep_layouts(lp, lo, up);
-      if (!err.empty()) throw ArgError(err);
-      sw_lo.set(lo, stream);
-      sw_up.set(up, stream);
-      d_sw_dsrc.upload_vec(lo.dsrc, stream);
+      SweepHostLayouts L = sweep_build.valid() ? sweep_build.get()
+                                               : build_sweep_layouts(h_off, h_col);
+      if (!L.err.empty()) throw ArgError(L.err);
+      sw_lo.set(L.lo, stream);
+      sw_up.set(L.up, stream);
+      d_sw_dsrc.upload_vec(L.lo.dsrc, stream);
       d_sw_diag.alloc(n);
       d_sw_rd.alloc(n);
       for (auto* b : {&w_s1, &w_s2, &w_s3, &w_y0}) b->alloc(n);
       // the fused layouts, where the pattern fits them (else every sweep is its own launch):
       // blocks of 256 rows, and of 512 (two rows per lane) for rows short enough to hold two
       for (int h = 0; h < 2; ++h) {
-        FusedSweepLayout flo, fup;
-        if (!fused_sweep_layouts(lp, FUSED_LANES * (h + 1), &flo, &fup).empty()) continue;
-        if (h == 1 && std::max(flo.width, fup.width) > 32) continue;
-        err = check_fused_sweep_layouts(lp, flo, fup);
-        if (!err.empty()) throw ArgError(err);
-        fz[h][0].set(flo, stream);
-        fz[h][1].set(fup, stream);
-        HIP_CHECK(hipStreamSynchronize(stream));  // the host layouts go out of scope
-        fused_width = std::max(flo.width, fup.width);
+        if (!L.have[h]) continue;
+        fz[h][0].set(L.fused[h][0], stream);
+        fz[h][1].set(L.fused[h][1], stream);
+        fused_width = std::max(L.fused[h][0].width, L.fused[h][1].width);
         have_fz[h] = true;
       }
       // a pattern without a candidate variant keeps one launch per sweep
@@ -969,6 +1004,108 @@ struct Operator {
     prof.end(P_TRI32, ev, stream);
   }
 
+  // ---------------------------------------------------------------- interleaved vectors
+  // Scratch for nrep vectors interleaved (entry i of vector b at i nrep + b): the batched
+  // SpMV and Jacobi sweeps run in that layout, between transposes of the stacked vectors.
+  DBuf<double> il[5];
+  void ensure_interleaved(int64_t N) {
+    for (auto& b : il)
+      if ((int64_t)b.n < N) b.alloc(N);
+  }
+  // with inter_rd, inter * rd (the reciprocal diagonal of the sweeps) is written as well
+  void interleave(const double* stacked, double* inter, int nrep, double* inter_rd = nullptr) {
+    const dim3 grid((unsigned)((n + 31) / 32), (unsigned)((nrep + 31) / 32));
+    mlk::k_interleave<true><<<grid, 256, 0, stream>>>(n, nrep, stacked, inter,
+                                                      inter_rd ? d_sw_rd.p : nullptr, inter_rd);
+  }
+  void deinterleave(const double* inter, double* stacked, int nrep) {
+    const dim3 grid((unsigned)((n + 31) / 32), (unsigned)((nrep + 31) / 32));
+    mlk::k_interleave<false><<<grid, 256, 0, stream>>>(n, nrep, inter, stacked);
+  }
+  // lanes per row of the interleaved kernels: the vectors of a pass, up to a wave
+  static int lanes_for(int nrep) {
+    return nrep > 32 ? 64 : nrep > 16 ? 32 : nrep > 8 ? 16 : 8;
+  }
+  unsigned groups_grid(int nrep, int w) const {
+    const int64_t threads = (int64_t)n * ((nrep + w - 1) / w) * w;
+    return (unsigned)((threads + 255) / 256);
+  }
+
+  // y = A x (FP64) for nrep stacked vectors: the SpMV of every vector, bit-identical to its own
+  // (k_spmm_il on the interleaved copies).
+  void spmv_many(const double* x, double* y, int nrep) {
+    ensure_interleaved((int64_t)n * nrep);
+    interleave(x, il[0].p, nrep);
+    const int w = lanes_for(nrep);
+    const unsigned g = groups_grid(nrep, w);
+    auto go = [&](auto kernel) {
+      kernel<<<g, 256, 0, stream>>>(n, nrep, d_soff.p, d_scol.p, d_slen.p, d_sval.p, il[0].p,
+                                    il[1].p);
+    };
+    switch (w) {
+      case 8: go(mlk::k_spmm_il<double, 8>); break;
+      case 16: go(mlk::k_spmm_il<double, 16>); break;
+      case 32: go(mlk::k_spmm_il<double, 32>); break;
+      default: go(mlk::k_spmm_il<double, 64>); break;
+    }
+    deinterleave(il[1].p, y, nrep);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // z ~ (L L^T)^-1 r for nrep stacked right-hand sides: the sweeps of apply_sweeps, one
+  // launch per sweep over every right-hand side (k_sweep_il on interleaved copies), each
+  // system's result bit-identical to apply_sweeps on it alone. The first iterate of each
+  // system (rhs * rd, the product the single system's forms take) is written beforehand: the
+  // forward one by the transpose of r, the backward one by the last forward sweep.
+  void apply_sweeps_rep(const double* r, double* z, int nrep) {
+    hipEvent_t ev = prof.begin(stream);
+    ensure_interleaved((int64_t)n * nrep);
+    double* ri = il[0].p;   // r
+    double* a = il[1].p;    // the forward first iterate, then backward scratch
+    double* b = il[2].p;    // forward scratch, then z
+    double* yi = il[3].p;   // y
+    double* z0 = il[4].p;   // the backward first iterate
+    interleave(r, ri, nrep, a);
+    const int w = lanes_for(nrep);
+    const unsigned g = groups_grid(nrep, w);
+    const int k = sweeps;
+    auto sweep = [&](const SweepDev& t, const double* rhs, const double* in, double* out,
+                     bool level, double* out_rd) {
+      const int sign = &t == &sw_lo ? -1 : 1;
+      auto go = [&](auto kernel) {
+        kernel<<<g, 256, 0, stream>>>(n, nrep, t.soff.p, t.scol.p, t.sdel.p, sign, t.slen.p,
+                                      t.sval.p, d_sw_rd.p, rhs, in, out, level ? 1 : 0, out_rd);
+      };
+      auto width = [&](auto d16) {
+        constexpr bool D = decltype(d16)::value;
+        switch (w) {
+          case 8: go(mlk::k_sweep_il<D, 8>); break;
+          case 16: go(mlk::k_sweep_il<D, 16>); break;
+          case 32: go(mlk::k_sweep_il<D, 32>); break;
+          default: go(mlk::k_sweep_il<D, 64>); break;
+        }
+      };
+      if (t.d16) width(std::true_type());
+      else width(std::false_type());
+    };
+    // forward from a into yi (alternating with b), z0 = y rd; backward from z0 into b (with a)
+    const double* prev = a;
+    for (int m = 1; m <= k; ++m) {
+      double* out = ((k - m) & 1) ? b : yi;
+      sweep(sw_lo, ri, prev, out, m < k, m == k ? z0 : nullptr);
+      prev = out;
+    }
+    prev = z0;
+    for (int m = 1; m <= k; ++m) {
+      double* out = ((k - m) & 1) ? a : b;
+      sweep(sw_up, yi, prev, out, m < k, nullptr);
+      prev = out;
+    }
+    deinterleave(b, z, nrep);
+    HIP_CHECK(hipGetLastError());
+    prof.end(P_TRI32, ev, stream);
+  }
+
   template <typename T>
   void fill_sentinel(T* p) {
     mlk::k_fill_bits<T><<<grid_for(n), 256, 0, stream>>>(p, n, mlk::Bits<T>::S);
@@ -1174,6 +1311,41 @@ struct Operator {
     return total / reps;
   }
 
+  // Seconds of one run of `work` (stream-ordered device work), after one that warms up.
+  template <class F>
+  double seconds_of(F work) {
+    hipEvent_t a, b;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    work();
+    HIP_CHECK(hipEventRecord(a, stream));
+    work();
+    HIP_CHECK(hipEventRecord(b, stream));
+    HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return 1e-3 * ms;
+  }
+
+  // Seconds of one application of the handle's preconditioner as its PCG applies it (the
+  // exact solves of the factor, or the handle's Jacobi sweeps), after one that warms up;
+  // -1 when a wait gave up.
+  double application_seconds() {
+    if (sweeps == 0) {
+      const double ms = time_precondition(1, 1, w_r.p, w_y.p, w_z.p);
+      return ms < 0.0 ? -1.0 : 1e-3 * ms;
+    }
+    mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);
+    const double t = seconds_of([&] { apply_sweeps(w_r.p, w_z.p, nullptr); });
+    if (read_flag(0)) {
+      mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);
+      return -1.0;
+    }
+    return t;
+  }
+
   // Picks, by timing, the waves per CU of the triangular solves (single right-hand side) and,
   // for nrhs > 1, the right-hand sides per pass. Results do not change: the solves are
   // bit-exact under any schedule. chosen[3]: waves per CU, right-hand sides per pass, and the
@@ -1272,7 +1444,12 @@ struct Operator {
         DBuf<double> r64, y64, z64;
         r64.alloc(N); y64.alloc(N); z64.alloc(N);
         mlk::k_fill_f64<<<grid_for(N), 256, 0, stream>>>(r64.p, N, 1.0);
-        const double t = time_precondition(2, k, r64.p, y64.p, z64.p);
+        double t;
+        if (sweeps > 0) {  // the handle's Jacobi sweeps over every right-hand side
+          t = 1e3 * seconds_of([&] { apply_sweeps_rep(r64.p, z64.p, k); });
+        } else {
+          t = time_precondition(2, k, r64.p, y64.p, z64.p);
+        }
         if (t > 0 && t / k < best_per) {
           best_per = t / k;
           best_k = k;
@@ -1323,20 +1500,30 @@ struct Operator {
       tx.alloc(n);
       mlk::k_fill_hash<<<grid_for(n), 256, 0, stream>>>(tb.p, n, 0x6d61696c6c65ull);
       HIP_CHECK(hipGetLastError());
-      double best_t = 1e300;
+      // whole solves at each sweep count, a likely winner first: a later count runs only as
+      // many iterations as could still beat the fastest solve so far, estimated from one timed
+      // application of its preconditioner and the rest of an iteration of that fastest solve
+      // (with a 10% margin); a count that reaches that cap is slower and is dropped
+      double best_t = 1e300, other = -1.0;
       int best_k = sweeps;
-      for (int k : {0, 2, 3, 4, 5, 6, 8}) {
+      for (int k : {6, 4, 8, 5, 3, 2, 0}) {
         set_sweeps(k);
+        const double app = application_seconds();
+        if (app < 0.0) continue;  // a wait gave up: not a candidate
+        int64_t cap = ML_MAX_ITERATIONS;
+        if (other >= 0.0)
+          cap = std::min<int64_t>(cap, (int64_t)(1.1 * best_t / (other + app)) + 10);
         mlk::PcgScalars hS;
         HIP_CHECK(hipStreamSynchronize(stream));
         const double t0 = now_s();
-        const int st = pcg<double, double>(tb.p, tx.p, w_r.p, ML_TOLERANCE, ML_MAX_ITERATIONS,
-                                           false, d_S.p, &hS);
+        const int st = pcg<double, double>(tb.p, tx.p, w_r.p, ML_TOLERANCE, cap, false, d_S.p,
+                                           &hS);
         const double t = now_s() - t0;
         if (st == ML_DEVICE_TIMEOUT) return st;
         if (st == ML_OK && t < best_t) {
           best_t = t;
           best_k = k;
+          other = std::max(0.0, t / (double)std::max<int64_t>(hS.it, 1) - app);
         }
       }
       set_sweeps(best_k);
@@ -1402,9 +1589,10 @@ struct Operator {
   // systems of this block-diagonal operator (nrep = 1). System s owns the vector entries
   // [seg_off[s], seg_off[s+1]) and seg_of[i] is the system of entry i. FP64 operator,
   // iterate and preconditioner vectors (the factor's coefficients in the handle's precision).
-  // A block-diagonal operator with Jacobi sweeps (nrep = 1) applies its factor by the sweeps,
-  // which act on each model's rows alone since no entry couples two models; replicated right-
-  // hand sides use the exact solves. The scalars of every system end in *hS.
+  // A handle with Jacobi sweeps applies its factor by the sweeps: on a block-diagonal operator
+  // (nrep = 1) they act on each model's rows alone, since no entry couples two models, and
+  // replicated right-hand sides take them one launch per sweep over all of them
+  // (apply_sweeps_rep). The scalars of every system end in *hS.
   template <typename TM>
   int pcg_batch(int nrep, int nseg, const int64_t* seg_off, const int* seg_of, const double* b,
                 double* x, double tol, int64_t maxit, std::vector<mlk::PcgScalars>* hS) {
@@ -1414,6 +1602,7 @@ struct Operator {
     if ((int64_t)wb_dot.n < 4 * nseg) wb_dot.alloc(4 * nseg);
     if ((int64_t)wb_S.n < nseg) wb_S.alloc(nseg);
     for (auto* v : {&wb_y64, &wb_z64}) if ((int64_t)v->n < N) v->alloc(N);
+    const bool jac = sweeps > 0;
     TM* y = wb_y64.p;
     TM* z = wb_z64.p;
     TM* rlow = nullptr;
@@ -1427,13 +1616,13 @@ struct Operator {
     mlk::PcgScalars* S = wb_S.p;
     const TM* rin = r;
     const unsigned vb = grid_for(N), sb = grid_for(nseg);
-    const bool jac = sweeps > 0 && nrep == 1;
     auto apply_m = [&]() {
-      if (jac) apply_sweeps(rin, z, nullptr);
+      if (jac && nrep == 1) apply_sweeps(rin, z, nullptr);
+      else if (jac) apply_sweeps_rep(rin, z, nrep);
       else precondition(rin, y, z, nullptr, nullptr, nullptr, nrep);
     };
     mlk::k_fill_i32<<<1, 64, 0, stream>>>(d_flags.p, 1, 0);  // abort flag
-    if (jac && fused_active()) fill_levels();
+    if (jac && nrep == 1 && fused_active()) fill_levels();
     mlk::k_fill_f64<<<vb, 256, 0, stream>>>(x, N, 0.0);
     HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
     mlk::k_seg_dot<double, double><<<nseg, 256, 0, stream>>>(seg_off, b, b, bb);
@@ -1448,8 +1637,7 @@ struct Operator {
     auto spmv_all = [&](const double* in, double* out) {
       hipEvent_t ev = prof.begin(stream);
       if (nrep > 1)
-        mlk::k_spmv_rep<double><<<vb, 256, 0, stream>>>(n, nrep, d_soff.p, d_scol.p, d_slen.p,
-                                                        d_sval.p, in, out);
+        spmv_many(in, out, nrep);
       else
         mlk::k_spmv<double><<<vb, 256, 0, stream>>>(n, d_soff.p, d_scol.p, nullptr, d_slen.p,
                                                      d_sval.p,
@@ -1515,7 +1703,7 @@ struct Operator {
     int st;
     for (;;) {
       // batched passes apply the factor by the exact solves (FP32 coefficients for the mixed
-      // handles), whose replicated blocks fill the device
+      // handles), whose replicated blocks fill the device, or by the handle's Jacobi sweeps
       st = pcg_batch<double>(nrep, nrep, d_off_seg.p, d_seg.p, b, x, tol, maxit, hS);
       if (st == ML_DEVICE_TIMEOUT && shrink()) continue;
       break;
@@ -1674,7 +1862,7 @@ struct Model {
     d_R.alloc(plan.ndof);
     HIP_CHECK(hipStreamSynchronize(st));
     // the host copies of the big lists are no longer needed
-    std::vector<uint32_t>().swap(plan.asm_slot);
+    decltype(plan.asm_slot)().swap(plan.asm_slot);
   }
 
   // Tet10: the kernel that stages each Gauss point's mapping in LDS (k_stiffness_lds), or the
@@ -2635,7 +2823,7 @@ struct Thermal {
       HIP_CHECK(hipGetLastError());
     }
     HIP_CHECK(hipStreamSynchronize(st));
-    std::vector<uint32_t>().swap(plan.asm_slot);
+    decltype(plan.asm_slot)().swap(plan.asm_slot);
     return ML_OK;
   }
 
@@ -3159,6 +3347,23 @@ ml_status ml_pcg_multiply(ml_pcg* h, const double* x, double* y) {
   });
 }
 
+ml_status ml_pcg_multiply_many(ml_pcg* h, int64_t nrhs, const double* x, double* y) {
+  return guarded([&]() -> int {
+    if (!h || !x || !y || nrhs < 1) throw ArgError("bad arguments");
+    Operator& op = h->op;
+    if ((int64_t)op.n * nrhs > INT32_MAX) throw ArgError("too many vectors at once");
+    const int64_t N = (int64_t)op.n * nrhs;
+    DBuf<double> xd, yd;
+    xd.alloc(N);
+    yd.alloc(N);
+    HIP_CHECK(hipMemcpyAsync(xd.p, x, sizeof(double) * N, hipMemcpyHostToDevice, op.stream));
+    op.spmv_many(xd.p, yd.p, (int)nrhs);
+    HIP_CHECK(hipMemcpyAsync(y, yd.p, sizeof(double) * N, hipMemcpyDeviceToHost, op.stream));
+    HIP_CHECK(hipStreamSynchronize(op.stream));
+    return ML_OK;
+  });
+}
+
 ml_status ml_pcg_precondition(ml_pcg* h, const double* r, double* z) {
   return guarded([&]() -> int {
     if (!h || !r || !z) throw ArgError("null argument");
@@ -3170,6 +3375,37 @@ ml_status ml_pcg_precondition(ml_pcg* h, const double* r, double* z) {
     HIP_CHECK(hipMemcpyAsync(op.w_r.p, r, sizeof(double) * op.n, hipMemcpyHostToDevice, op.stream));
     op.apply_precondition();
     HIP_CHECK(hipMemcpyAsync(z, op.w_z.p, sizeof(double) * op.n, hipMemcpyDeviceToHost, op.stream));
+    HIP_CHECK(hipStreamSynchronize(op.stream));
+    if (op.read_flag(0)) return ML_DEVICE_TIMEOUT;
+    return ML_OK;
+  });
+}
+
+ml_status ml_pcg_precondition_many(ml_pcg* h, int64_t nrhs, const double* r, double* z) {
+  return guarded([&]() -> int {
+    if (!h || !r || !z || nrhs < 1) throw ArgError("bad arguments");
+    Operator& op = h->op;
+    if ((int64_t)op.n * nrhs > INT32_MAX) throw ArgError("too many right-hand sides at once");
+    if (!op.factored) {
+      int st = op.factor();
+      if (st != ML_OK) return st;
+    }
+    const int k = (int)nrhs;
+    const int64_t N = (int64_t)op.n * k;
+    DBuf<double> rd, yd, zd;
+    rd.alloc(N);
+    yd.alloc(N);
+    zd.alloc(N);
+    HIP_CHECK(hipMemcpyAsync(rd.p, r, sizeof(double) * N, hipMemcpyHostToDevice, op.stream));
+    mlk::k_fill_i32<<<1, 64, 0, op.stream>>>(op.d_flags.p, 2, 0);
+    if (op.sweeps > 0) {
+      op.apply_sweeps_rep(rd.p, zd.p, k);
+    } else {
+      mlk::k_fill_bits<double><<<grid_for(N), 256, 0, op.stream>>>(yd.p, N, mlk::Bits<double>::S);
+      mlk::k_fill_bits<double><<<grid_for(N), 256, 0, op.stream>>>(zd.p, N, mlk::Bits<double>::S);
+      op.precondition(rd.p, yd.p, zd.p, nullptr, nullptr, nullptr, k);
+    }
+    HIP_CHECK(hipMemcpyAsync(z, zd.p, sizeof(double) * N, hipMemcpyDeviceToHost, op.stream));
     HIP_CHECK(hipStreamSynchronize(op.stream));
     if (op.read_flag(0)) return ML_DEVICE_TIMEOUT;
     return ML_OK;

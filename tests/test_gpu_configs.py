@@ -80,33 +80,51 @@ def test_tet10_cantilever_on_the_device(gpu, tet10, precision, bound):
 
 
 # ------------------------------------------------------------------ config 4a
-def test_256_load_cases_of_the_tet10_beam(gpu):
+BEAM_SAMPLE = (0, 85, 170, 255)
+
+
+@pytest.fixture(scope="module")
+def beam():
     p, F = M.beam_load_cases()
     assert F.shape[0] == 256 and p.nfree == 69360
-    with gpu.Model(p) as m:
-        m.factor()
-        m.autotune(nrhs=len(F))
-        t = time.time()
-        U, infos = m.solve_many(F)
-        say(f"beam: 256 cases solved together in {time.time() - t:.1f} s, iterations "
-            f"{min(i['iterations'] for i in infos)}-{max(i['iterations'] for i in infos)}")
-        sample = (0, 85, 170, 255)
-        seq = {k: m.solve(F[k], reactions=False)[0] for k in sample}
     s = oracle.System(p.nodes, p.conn, p.young, p.nu, p.fixed)
     assert s.factorize() == oracle.OK
     d = oracle.System(p.nodes, p.conn, p.young, p.nu, p.fixed, iterative=False)
     assert d.factorize() == oracle.OK
-    for k in sample:
+    refs = {}
+    for k in BEAM_SAMPLE:
         st, u_ref, it, _ = s.solve(F[k])
         assert st == oracle.OK
         st, u_sky, _, _ = d.solve(F[k])
         assert st == oracle.OK
+        refs[k] = (u_ref, it, u_sky)
+    return p, F, refs
+
+
+@pytest.mark.parametrize("precision,bound", [("FP64", 1e-10), ("MIXED_JACOBI", 1e-8)])
+def test_256_load_cases_of_the_tet10_beam(gpu, beam, precision, bound):
+    """The 256 cases as one multi-right-hand-side block (one PCG per case in shared launches):
+    FP64 with replicated exact triangular solves, MIXED_JACOBI with the FP32 factor's Jacobi
+    sweeps over every case at once; against the oracle's PCG and skyline on four cases and the
+    same model's one-at-a-time solves."""
+    p, F, refs = beam
+    with gpu.Model(p, precision=getattr(gpu, precision)) as m:
+        m.factor()
+        m.autotune(nrhs=len(F))
+        t = time.time()
+        U, infos = m.solve_many(F)
+        say(f"beam {precision} (sweeps {m.sweeps()}): 256 cases solved together in "
+            f"{time.time() - t:.1f} s, iterations {min(i['iterations'] for i in infos)}-"
+            f"{max(i['iterations'] for i in infos)}")
+        seq = {k: m.solve(F[k], reactions=False)[0] for k in BEAM_SAMPLE}
+    for k in BEAM_SAMPLE:
+        u_ref, it, u_sky = refs[k]
         err, gap = rel_l2(U[k], u_ref), relative_gap(U[k], u_sky)
         say(f"case {k}: {infos[k]['iterations']} iterations (oracle {it}), rel L2 vs oracle "
             f"PCG {err:.2e}, vs the sequential device solve {rel_l2(U[k], seq[k]):.2e}, "
             f"relative gap vs skyline {gap:.2e}")
-        assert err <= 1e-10
-        assert rel_l2(U[k], seq[k]) <= 1e-10
+        assert err <= bound
+        assert rel_l2(U[k], seq[k]) <= (1e-10 if precision == "FP64" else 1e-9)
         assert gap < 1e-8
     assert all(i["device_retries"] == 0 for i in infos)
 

@@ -3,6 +3,8 @@ C ABI. Bit-exact where the device restates the reference's arithmetic operation 
 (element stiffness, assembled K_ff, SpMV, IC(0) factor, triangular solves, reactions,
 stresses); displacements of the PCG solve within 1e-10 relative L2 of the oracle's PCG (FP64)
 and 1e-8 (mixed), the parity tolerances of the solve path."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -312,7 +314,6 @@ def test_edge_loads_match_oracle(gpu, name):
     """edge_load (assembly.rs:680-710) on the edges of the x = max end, added after the
     pressures as Model::assemble does (model.rs:1113-1127): the library's F equals the oracle's
     bit for bit, and the FP64 solve under it stays within 1e-10 of the oracle's PCG."""
-    import dataclasses
     base = PROBLEMS[name]
     xmax = base.nodes[:, 0].max()
     loads = []
@@ -530,6 +531,74 @@ def test_load_cases_of_one_model(gpu):
         assert abs(infos[k]["iterations"] - info1["iterations"]) <= 3
 
 
+@pytest.mark.parametrize("precision,sweeps", [("FP64", 0), ("MIXED_IC0", 0), ("MIXED_JACOBI", 1),
+                                              ("MIXED_JACOBI", 3), ("MIXED_JACOBI", 4)])
+def test_batched_preconditioner_is_the_single_one_bit_for_bit(gpu, precision, sweeps):
+    """The preconditioner as batched solves apply it to many right-hand sides (replicated exact
+    solves, or the Jacobi sweeps of every vector in one launch per sweep, eight vectors per
+    thread) gives each vector exactly what the single application gives it; the sweeps equal
+    the float64 restatement bit for bit. 11 and 40 vectors (lane groups of 16 and 64, the last
+    group partial), with a zero vector among them."""
+    from sweeps import jacobi_precondition
+    p = PROBLEMS["jittered_cube_8"]
+    s = _oracle_system(p)
+    off, cols, vals = s.csr()
+    print(f"precondition_many {precision} sweeps={sweeps} n={s.n}", flush=True)
+    pcg = gpu.Pcg(gpu.Csr(off, cols, vals), precision=getattr(gpu, precision))
+    if precision == "MIXED_JACOBI":
+        pcg.set_sweeps(sweeps)
+    rng = np.random.default_rng(19)
+    l, _ = pcg.factor()
+    for count in (11, 40):
+        R = rng.normal(size=(count, s.n))
+        R[6] = 0.0
+        Z = pcg.precondition_many(R)
+        for k in range(len(R)):
+            assert np.array_equal(Z[k], pcg.precondition(R[k]))
+            if precision == "MIXED_JACOBI" and k < 12:
+                assert np.array_equal(Z[k], jacobi_precondition(off, cols, l, R[k], sweeps))
+        assert np.all(Z[6] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["jittered_cube_8", "cantilever_tet10_small"])
+def test_batched_spmv_is_the_single_one_bit_for_bit(gpu, name):
+    """The SpMV of many vectors at once (interleaved copies, a group of lanes per row and up to
+    64 vectors) equals the single SpMV of each vector, and the oracle's Csr::multiply, bit for
+    bit; 1, 5, 11, 40 and 70 vectors cover every lane-group width."""
+    p = PROBLEMS[name]
+    s = _oracle_system(p)
+    off, cols, vals = s.csr()
+    print(f"multiply_many {name} n={s.n}", flush=True)
+    pcg = gpu.Pcg(gpu.Csr(off, cols, vals))
+    csr = oracle.Csr.from_arrays(off, cols, vals)
+    rng = np.random.default_rng(23)
+    for k in (1, 5, 11, 40, 70):
+        X = rng.normal(size=(k, s.n))
+        Y = pcg.multiply_many(X)
+        for v in range(k):
+            y1 = pcg.multiply(X[v])
+            assert np.array_equal(Y[v], y1)
+            if v < 2:
+                assert np.array_equal(Y[v], csr.multiply(X[v]))
+
+
+def test_load_cases_with_jacobi_sweeps(gpu):
+    """Load cases of one MIXED_JACOBI model solved together (the preconditioner's sweeps over
+    every case at once) against the model's one-at-a-time solves and the oracle's FP64 PCG."""
+    p, F = M.beam_load_cases(n=(12, 3, 3), size=(60.0, 15.0, 15.0), cases=10)
+    print("load cases with Jacobi sweeps", flush=True)
+    with gpu.Model(p, precision=gpu.MIXED_JACOBI) as m:
+        m.set_sweeps(2)
+        U, infos = m.solve_many(F)
+        assert len(infos) == 10 and all(i["device_retries"] == 0 for i in infos)
+        for k in (0, 3, 9):
+            u1, _, info1 = m.solve(F[k], reactions=False)
+            assert rel_l2(U[k], u1) <= 1e-9
+            assert abs(infos[k]["iterations"] - info1["iterations"]) <= 3
+            ref = oracle.model_solve(dataclasses.replace(p, f=F[k].copy()), iterative=True)
+            assert rel_l2(U[k], ref["displacements"]) <= 1e-8
+
+
 @pytest.mark.parametrize("precision", ["FP64", "MIXED_IC0", "MIXED_JACOBI"])
 def test_batch_of_uneven_models(gpu, precision):
     """Boxes of uneven sizes and materials solved in one batch (block-diagonal operator, one
@@ -577,7 +646,6 @@ def test_a_model_with_every_dof_imposed(gpu):
                    strain[5] / 2 * x[:, 0] + strain[1] * x[:, 1] + strain[3] / 2 * x[:, 2],
                    strain[4] / 2 * x[:, 0] + strain[3] / 2 * x[:, 1] + strain[2] * x[:, 2]],
                   axis=1).ravel()
-    import dataclasses
     q = dataclasses.replace(p, fixed=np.ones(p.ndof, dtype=bool), imposed=u0, pressures=[])
     m = gpu.Model(q)
     u, r, info = m.solve()
