@@ -55,8 +55,8 @@ def fused_sweep_bytes(n, nnz_strict, sweeps):
     triangular system): the FP32 coefficients and 16-bit LDS slots of the strict triangle once,
     the row lengths (1 byte), the FP64 right-hand side, reciprocal diagonal and result once,
     and each of the sweeps - 1 intermediate iterates written once and read once. The staging
-    of each block's external columns re-reads values other blocks wrote, and the sentinel
-    reset of the other system's levels is bookkeeping: both count as traffic, not here."""
+    of each block's external columns re-reads values other blocks wrote: it counts as traffic,
+    not here."""
     return nnz_strict * (4 + 2) + n * (1 + 8 + 8 + 8) + (sweeps - 1) * n * (8 + 8)
 
 

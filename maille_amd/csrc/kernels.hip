@@ -2164,17 +2164,26 @@ __global__ __launch_bounds__(256) void k_update_p(int n, const PcgScalars* __res
 // Every system follows iterative.rs:210-248 on its own (its own alpha, beta and stop), so its
 // iterates are those of a solve of that system alone.
 
-// a.b over each system's entries: one workgroup per system, fixed order.
+// a.b over each system's entries: one workgroup per system, fixed order. Each thread keeps
+// four sums (entries i, i + 256, i + 512, i + 768 of every 1,024), so that eight loads are in
+// flight, and adds them as (s0 + s1) + (s2 + s3).
 template <typename A, typename B>
 __global__ __launch_bounds__(256) void k_seg_dot(const int64_t* __restrict__ seg_off,
                                                  const A* __restrict__ a,
                                                  const B* __restrict__ b,
                                                  double* __restrict__ out) {
   const int s = blockIdx.x;
-  double v = 0.0;
-  for (int64_t i = seg_off[s] + threadIdx.x; i < seg_off[s + 1]; i += 256)
-    v += (double)a[i] * (double)b[i];
-  const double t = block_sum<256>(v);
+  const int64_t hi = seg_off[s + 1];
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int64_t i = seg_off[s] + threadIdx.x;
+  for (; i + 768 < hi; i += 1024) {
+    v0 += (double)a[i] * (double)b[i];
+    v1 += (double)a[i + 256] * (double)b[i + 256];
+    v2 += (double)a[i + 512] * (double)b[i + 512];
+    v3 += (double)a[i + 768] * (double)b[i + 768];
+  }
+  for (; i < hi; i += 256) v0 += (double)a[i] * (double)b[i];
+  const double t = block_sum<256>((v0 + v1) + (v2 + v3));
   if (threadIdx.x == 0) out[s] = t;
 }
 
