@@ -118,7 +118,7 @@ def run_beam_cases(args, ma, meshes):
             m.factor()
             chosen = m.autotune(nrhs=len(F))
             log(f"beam {name}: {p.nfree} free DOF, {len(F)} cases; autotuned {chosen}, "
-                f"{m.sweeps()} sweeps")
+                f"{m.sweeps()} sweeps, {m.batch_lanes()} lanes per row")
             m.solve_many(F[:8])
             t0 = time.perf_counter()
             U, infos = m.solve_many(F)
@@ -138,6 +138,7 @@ def run_beam_cases(args, ma, meshes):
                            "iterations_max": max(i["iterations"] for i in infos),
                            "sweeps": m.sweeps(), "waves_per_cu": chosen[0],
                            "cases_per_pass": chosen[1],
+                           "rows_per_wave": 64 // max(m.batch_lanes(), 1) if m.batch_lanes() else None,
                            "rel_err_last_checked_vs_sequential": err_seq,
                            "max_rel_err_vs_fp64": err64,
                            "device_retries": sum(i["device_retries"] for i in infos)}
@@ -170,7 +171,7 @@ def run_beam_cases(args, ma, meshes):
           "dtype": "fp64" if head == "fp64" else "fp64+fp32-ic0-jacobi",
           "config": {"workload": "beam_load_cases", "free_dof": p.nfree, "cases": len(F),
                      "precision": head, "sweeps": h["sweeps"], "waves_per_cu": h["waves_per_cu"],
-                     "cases_per_pass": h["cases_per_pass"]},
+                     "cases_per_pass": h["cases_per_pass"], "rows_per_wave": h["rows_per_wave"]},
           "batched_s": h["batched_s"], "sequential_s_per_case": h["sequential_s_per_case"],
           "speedup_vs_sequential": h["speedup_vs_sequential"], "modes": modes,
           "cpu_baseline": cpu,

@@ -230,8 +230,9 @@ ml_status ml_pcg_set_wait_limit(ml_pcg* h, int64_t ticks);
 /* No counterpart in the reference: the Jacobi sweeps per triangular system with which a mixed
  * handle applies its FP32 factor (0..64; 0: the exact triangular solves). An FP64 handle
  * takes 0 only (ML_INVALID_ARGUMENT otherwise). A count set here is kept by the autotuner.
- * Batched solves (nrhs > 1, ml_batch) always use the exact solves. ml_pcg_sweeps returns the
- * current count. */
+ * Batched solves apply the same sweeps: a batch of models (ml_batch) by the fused kernel over
+ * the block-diagonal operator, several right-hand sides (nrhs > 1) by one launch per sweep
+ * over all of them (ml_pcg_precondition_many). ml_pcg_sweeps returns the current count. */
 ml_status ml_pcg_set_sweeps(ml_pcg* h, int sweeps);
 int ml_pcg_sweeps(const ml_pcg* h);
 /* No counterpart in the reference: how the sweeps are launched. fused 1 (the default): all
@@ -492,6 +493,12 @@ void ml_batch_destroy(ml_batch* b);
  * sides per pass, reported occupancy per CU. Factorises first if needed. */
 ml_status ml_pcg_autotune(ml_pcg* h, int64_t nrhs, int32_t* chosen);
 ml_status ml_model_autotune(ml_model* m, int64_t nrhs, int32_t* chosen);
+/* No counterpart in the reference: lanes per row of the batched SpMV and Jacobi sweeps over
+ * several right-hand sides (8, 16, 32 or 64: a wave takes 64 / lanes rows), as the autotuner
+ * timed fastest for nrhs > 1 at its right-hand sides per pass; 0 before, when the width
+ * follows the vectors of a pass (up to a wave). */
+int ml_pcg_batch_lanes(const ml_pcg* h);
+int ml_model_batch_lanes(const ml_model* m);
 
 /* Kernel-duration accounting with HIP events on the model's stream (no counterpart in the
  * reference; for benchmarks). Returns the accumulated milliseconds and launch counts per kind

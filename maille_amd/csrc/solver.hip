@@ -1022,9 +1022,12 @@ struct Operator {
     const dim3 grid((unsigned)((n + 31) / 32), (unsigned)((nrep + 31) / 32));
     mlk::k_interleave<false><<<grid, 256, 0, stream>>>(n, nrep, inter, stacked);
   }
-  // lanes per row of the interleaved kernels: the vectors of a pass, up to a wave
-  static int lanes_for(int nrep) {
-    return nrep > 32 ? 64 : nrep > 16 ? 32 : nrep > 8 ? 16 : 8;
+  // lanes per row of the interleaved kernels (a wave takes 64 / lanes rows): the autotuned
+  // width when set, else the vectors of a pass up to a wave; never wider than the pass needs
+  int il_lanes = 0;
+  int lanes_for(int nrep) const {
+    const int fit = nrep > 32 ? 64 : nrep > 16 ? 32 : nrep > 8 ? 16 : 8;
+    return il_lanes > 0 ? std::min(il_lanes, fit) : fit;
   }
   unsigned groups_grid(int nrep, int w) const {
     const int64_t threads = (int64_t)n * ((nrep + w - 1) / w) * w;
@@ -1456,6 +1459,28 @@ struct Operator {
         }
       }
       rhs_per_pass = best_k;
+      // rows per wave of the interleaved SpMV and sweeps (64 / lanes per row), at that pass
+      const int k = (int)std::min<int64_t>(rhs_per_pass, std::max<int64_t>(nrhs, 8));
+      const int64_t N = (int64_t)n * k;
+      DBuf<double> r64, z64;
+      r64.alloc(N);
+      z64.alloc(N);
+      mlk::k_fill_f64<<<grid_for(N), 256, 0, stream>>>(r64.p, N, 1.0);
+      double best_w = 1e300;
+      int lanes = 0;
+      for (int w : {64, 32, 16, 8}) {
+        il_lanes = w;
+        if (lanes_for(k) != w) continue;
+        const double t = seconds_of([&] {
+          spmv_many(r64.p, z64.p, k);
+          if (sweeps > 0) apply_sweeps_rep(r64.p, z64.p, k);
+        });
+        if (t < best_w) {
+          best_w = t;
+          lanes = w;
+        }
+      }
+      il_lanes = lanes;
     }
     if (have_fused && fused_on) {
       // the fused kernel's variant: the fastest application, among the variants no wider
@@ -3466,6 +3491,10 @@ ml_status ml_model_set_sweeps(ml_model* h, int sweeps) {
 }
 
 int ml_model_sweeps(const ml_model* h) { return h ? h->m.op.sweeps : 0; }
+
+int ml_pcg_batch_lanes(const ml_pcg* h) { return h ? h->op.il_lanes : 0; }
+
+int ml_model_batch_lanes(const ml_model* h) { return h ? h->m.op.il_lanes : 0; }
 
 ml_status ml_pcg_set_sweeps_fused(ml_pcg* h, int fused) {
   return guarded([&]() -> int {

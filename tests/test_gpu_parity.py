@@ -582,6 +582,23 @@ def test_batched_spmv_is_the_single_one_bit_for_bit(gpu, name):
                 assert np.array_equal(Y[v], csr.multiply(X[v]))
 
 
+def test_batched_width_from_the_autotuner_keeps_results(gpu):
+    """The autotuner's lanes per row of the batched SpMV and sweeps (and its right-hand sides
+    per pass) change no result: each case's arithmetic is the same at every width, so the
+    batched solve after autotuning equals the one before bit for bit (sweep count pinned)."""
+    p, F = M.beam_load_cases(n=(12, 3, 3), size=(60.0, 15.0, 15.0), cases=20)
+    print("batched width from the autotuner", flush=True)
+    with gpu.Model(p, precision=gpu.MIXED_JACOBI) as m:
+        m.set_sweeps(2)
+        assert m.batch_lanes() == 0
+        U1, i1 = m.solve_many(F)
+        m.autotune(nrhs=len(F))
+        assert m.batch_lanes() in (8, 16, 32) and m.sweeps() == 2
+        U2, i2 = m.solve_many(F)
+    assert np.array_equal(U1, U2)
+    assert [a["iterations"] for a in i1] == [b["iterations"] for b in i2]
+
+
 def test_load_cases_with_jacobi_sweeps(gpu):
     """Load cases of one MIXED_JACOBI model solved together (the preconditioner's sweeps over
     every case at once) against the model's one-at-a-time solves and the oracle's FP64 PCG."""
