@@ -495,9 +495,12 @@ ml_status ml_pcg_autotune(ml_pcg* h, int64_t nrhs, int32_t* chosen);
 ml_status ml_model_autotune(ml_model* m, int64_t nrhs, int32_t* chosen);
 /* No counterpart in the reference: lanes per row of the batched SpMV and Jacobi sweeps over
  * several right-hand sides (8, 16, 32 or 64: a wave takes 64 / lanes rows), as the autotuner
- * timed fastest for nrhs > 1 at its right-hand sides per pass; 0 before, when the width
- * follows the vectors of a pass (up to a wave). */
+ * timed fastest for nrhs > 1 at its right-hand sides per pass, or as set here; 0 (before
+ * either) when the width follows the vectors of a pass (up to a wave). Every width gives
+ * bit-identical results. */
+ml_status ml_pcg_set_batch_lanes(ml_pcg* h, int lanes);
 int ml_pcg_batch_lanes(const ml_pcg* h);
+ml_status ml_model_set_batch_lanes(ml_model* m, int lanes);
 int ml_model_batch_lanes(const ml_model* m);
 
 /* Kernel-duration accounting with HIP events on the model's stream (no counterpart in the

@@ -59,6 +59,7 @@ EXPORTS = (
     "ml_model_compliance_gradient", "ml_model_destroy", "ml_model_profile",
     "ml_model_solve_many", "ml_batch_create", "ml_batch_solve", "ml_batch_destroy",
     "ml_pcg_autotune", "ml_model_autotune", "ml_pcg_batch_lanes", "ml_model_batch_lanes",
+    "ml_pcg_set_batch_lanes", "ml_model_set_batch_lanes",
     "ml_msh_read", "ml_msh_sizes", "ml_msh_geometry", "ml_msh_set_info", "ml_msh_set_items",
     "ml_msh_destroy", "ml_model_solve_quasi_static",
     "ml_thermal_create", "ml_thermal_add_loads", "ml_thermal_solve_step", "ml_thermal_destroy",
@@ -237,6 +238,8 @@ def lib():
             "ml_pcg_autotune": (_int, [_P, _i64, _I32]),
             "ml_model_autotune": (_int, [_P, _i64, _I32]),
             "ml_pcg_batch_lanes": (_int, [_P]),
+            "ml_pcg_set_batch_lanes": (_int, [_P, _int]),
+            "ml_model_set_batch_lanes": (_int, [_P, _int]),
             "ml_model_batch_lanes": (_int, [_P]),
             "ml_msh_read": (_int, [_P, _i64, _PP, _I32, _I]),
             "ml_msh_sizes": (_int, [_P, _I]),
@@ -579,8 +582,12 @@ class Pcg(_Handle):
 
     def batch_lanes(self) -> int:
         """Lanes per row of the batched SpMV and sweeps (64 / rows per wave) the autotuner
-        chose for several right-hand sides; 0 before."""
+        chose for several right-hand sides, or as set; 0: from the vectors of a pass."""
         return lib().ml_pcg_batch_lanes(self._h)
+
+    def set_batch_lanes(self, lanes: int):
+        """Lanes per row of the batched SpMV and sweeps: 8, 16, 32, 64, or 0 (from the pass)."""
+        _check(lib().ml_pcg_set_batch_lanes(self._h, int(lanes)))
 
     def set_wait_limit(self, ticks: int = 0):
         """Bound of the device's dataflow waits in wall-clock ticks (0: the default, 2 s)."""
@@ -698,6 +705,10 @@ class Model(_Handle):
     def batch_lanes(self) -> int:
         """Pcg.batch_lanes for the model's operator."""
         return lib().ml_model_batch_lanes(self._h)
+
+    def set_batch_lanes(self, lanes: int):
+        """Pcg.set_batch_lanes for the model's operator."""
+        _check(lib().ml_model_set_batch_lanes(self._h, int(lanes)))
 
     def set_sweeps(self, sweeps: int):
         """Jacobi sweeps per triangular system for a mixed model's FP32 factor (0: exact)."""

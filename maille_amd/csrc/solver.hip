@@ -3492,6 +3492,24 @@ ml_status ml_model_set_sweeps(ml_model* h, int sweeps) {
 
 int ml_model_sweeps(const ml_model* h) { return h ? h->m.op.sweeps : 0; }
 
+static ml_status set_batch_lanes(Operator* op, int lanes) {
+  return guarded([&]() -> int {
+    if (!op) throw ArgError("null handle");
+    if (lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+      throw ArgError("lanes per row must be 0, 8, 16, 32 or 64");
+    op->il_lanes = lanes;
+    return ML_OK;
+  });
+}
+
+ml_status ml_pcg_set_batch_lanes(ml_pcg* h, int lanes) {
+  return set_batch_lanes(h ? &h->op : nullptr, lanes);
+}
+
+ml_status ml_model_set_batch_lanes(ml_model* h, int lanes) {
+  return set_batch_lanes(h ? &h->m.op : nullptr, lanes);
+}
+
 int ml_pcg_batch_lanes(const ml_pcg* h) { return h ? h->op.il_lanes : 0; }
 
 int ml_model_batch_lanes(const ml_model* h) { return h ? h->m.op.il_lanes : 0; }

@@ -531,14 +531,15 @@ def test_load_cases_of_one_model(gpu):
         assert abs(infos[k]["iterations"] - info1["iterations"]) <= 3
 
 
-@pytest.mark.parametrize("precision,sweeps", [("FP64", 0), ("MIXED_IC0", 0), ("MIXED_JACOBI", 1),
-                                              ("MIXED_JACOBI", 3), ("MIXED_JACOBI", 4)])
-def test_batched_preconditioner_is_the_single_one_bit_for_bit(gpu, precision, sweeps):
+@pytest.mark.parametrize("precision,sweeps,lanes", [
+    ("FP64", 0, 0), ("MIXED_IC0", 0, 0), ("MIXED_JACOBI", 1, 0), ("MIXED_JACOBI", 3, 0),
+    ("MIXED_JACOBI", 4, 0), ("MIXED_JACOBI", 3, 8), ("MIXED_JACOBI", 2, 32)])
+def test_batched_preconditioner_is_the_single_one_bit_for_bit(gpu, precision, sweeps, lanes):
     """The preconditioner as batched solves apply it to many right-hand sides (replicated exact
-    solves, or the Jacobi sweeps of every vector in one launch per sweep, eight vectors per
-    thread) gives each vector exactly what the single application gives it; the sweeps equal
-    the float64 restatement bit for bit. 11 and 40 vectors (lane groups of 16 and 64, the last
-    group partial), with a zero vector among them."""
+    solves, or the Jacobi sweeps of every vector in one launch per sweep on interleaved copies,
+    with the lanes per row following the vectors of a pass or set) gives each vector exactly
+    what the single application gives it; the sweeps equal the float64 restatement bit for
+    bit. 11 and 40 vectors (the last lane group partial), with a zero vector among them."""
     from sweeps import jacobi_precondition
     p = PROBLEMS["jittered_cube_8"]
     s = _oracle_system(p)
@@ -547,6 +548,8 @@ def test_batched_preconditioner_is_the_single_one_bit_for_bit(gpu, precision, sw
     pcg = gpu.Pcg(gpu.Csr(off, cols, vals), precision=getattr(gpu, precision))
     if precision == "MIXED_JACOBI":
         pcg.set_sweeps(sweeps)
+    pcg.set_batch_lanes(lanes)
+    assert pcg.batch_lanes() == lanes
     rng = np.random.default_rng(19)
     l, _ = pcg.factor()
     for count in (11, 40):
