@@ -412,6 +412,8 @@ def run_plastic(args, ma, meshes):
     m = ma.Model(p, precision=codes[prec])
     if args.sweeps is not None and prec != "fp64":
         m.set_sweeps(args.sweeps)
+    elif prec == "mixed_jacobi":
+        m.autotune()  # the sweep count whose whole solve of the elastic operator is fastest
     sy, hd = 250.0, 2000.0
     step, amp = P.Step(1.0, args.increments), P.Amplitude.ramp(1.0)
     incs, _ = P.solve_quasi_static(m, P.Step(1.0, 1), P.Amplitude([(0.0, 0.0), (1.0, 1e-3)]),
@@ -477,7 +479,8 @@ def run_plastic(args, ma, meshes):
           "data": "synthetic", "dtype": "fp64" if prec == "fp64" else f"fp64+{prec}",
           "config": {"workload": "plastic_bar_120x24x24", "free_dof": p.nfree,
                      "elements": int(ne), "increments": args.increments,
-                     "yield_stress": sy, "hardening": hd},
+                     "yield_stress": sy, "hardening": hd, "precision": prec,
+                     "sweeps": m.sweeps()},
           "newton_iterations": info["newton_iterations"], "cuts": info["cuts"],
           "pcg_iterations": info["pcg_iterations"],
           "residuals": [[float(f"{r:.3e}") for r in inc.residuals] for inc in incs[1:]],
@@ -580,6 +583,8 @@ def run_large(args, ma, meshes):
     m = ma.Model(prob, precision=codes[prec])
     if args.sweeps is not None and prec != "fp64":
         m.set_sweeps(args.sweeps)
+    elif prec == "mixed_jacobi":
+        m.autotune()  # the sweep count whose whole solve of the initial operator is fastest
     step, amp = P.Step(1.0, 4), P.Amplitude.ramp(1.0)
     P.solve_large_strain(m, P.Step(1.0, 1), P.Amplitude([(0.0, 0.0), (1.0, 1e-3)]),
                          neo_hookean=True, pressures=pressures)  # warm-up
@@ -616,7 +621,8 @@ def run_large(args, ma, meshes):
           "data": "synthetic", "dtype": "fp64" if prec == "fp64" else f"fp64+{prec}",
           "config": {"workload": "neo_hookean_block_40", "elements": int(ne),
                      "free_dof": prob.nfree, "follower_faces": len(pressures),
-                     "pressure": round(p, 3), "stretch": stretch},
+                     "pressure": round(p, 3), "stretch": stretch, "precision": prec,
+                     "sweeps": m.sweeps()},
           "newton_iterations": info["newton_iterations"], "cuts": info["cuts"],
           "pcg_iterations": info["pcg_iterations"],
           "residuals": [[float(f"{r:.3e}") for r in inc.residuals] for inc in incs[1:]],
