@@ -26,14 +26,14 @@ void parallel_ranges(i64 n, F body) {
     body((i64)0, n);
     return;
   }
-  std::vector<std::thread> pool;
+  ThreadGroup pool;
   i64 step = (n + t - 1) / t;
   for (int k = 0; k < t; ++k) {
     i64 b = k * step, e = std::min(n, b + step);
     if (b >= e) break;
-    pool.emplace_back([=] { body(b, e); });
+    pool.run([=] { body(b, e); });
   }
-  for (auto& th : pool) th.join();
+  pool.join();
 }
 
 }  // namespace
@@ -511,12 +511,12 @@ static void grouped_columns(DeviceLayouts& d) {
   const i64 step = (d.slices + T - 1) / std::max<i64>(T, 1);
   std::vector<Part> parts(T);
   {
-    std::vector<std::thread> pool;
+    ThreadGroup pool;
     for (i64 k = 0; k < T; ++k) {
       const i64 s0 = std::min(d.slices, k * step), s1 = std::min(d.slices, s0 + step);
-      pool.emplace_back([&, s0, s1, k] { build(s0, s1, parts[k]); });
+      pool.run([&, s0, s1, k] { build(s0, s1, parts[k]); });
     }
-    for (auto& th : pool) th.join();
+    pool.join();
   }
   std::vector<int32_t> goff(d.slices + 1, 0);
   i64 total = 0;
@@ -638,7 +638,8 @@ std::string device_layouts(const LowerPattern& lp, const TriCaps& caps, DeviceLa
   std::vector<int64_t>().swap(t.src);
   // the two solves' layouts are independent: the backward one on a thread of its own
   std::string err, err_bw;
-  std::thread bw_thread([&] {
+  ThreadGroup bw_thread;
+  bw_thread.run([&] {
     d.bw = tri_layout(n, boff.data(), bcol.data(), bsrc.data(), bdsrc.data(), 64, caps, &err_bw);
   });
   d.fw = tri_layout(n, foff.data(), fcol.data(), fsrc.data(), fdsrc.data(), 64, caps, &err);
@@ -741,7 +742,8 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, const 
     return "CSR copy of the wrong size";
   const i64 nf = nnz - n;
   std::string err_fw, err_bw;
-  std::thread fw_check([&] {
+  ThreadGroup checks;
+  checks.run([&] {
     std::vector<int32_t> foff(n + 1), fcol(nf), fsrc(nf);
     for (i64 i = 0; i <= n; ++i) foff[i] = (int32_t)(lp.offsets[i] - i);
     for (i64 i = 0; i < n; ++i)
@@ -757,7 +759,7 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, const 
         return;
       }
   });
-  std::thread bw_check([&] {
+  checks.run([&] {
     // backward rows, restated independently of device_layouts: entry (i, j) of L, i > j, goes
     // to logical row n-1-j; visiting i downwards appends each row's entries in ascending
     // logical column n-1-i
@@ -783,8 +785,7 @@ std::string check_layouts(const LowerPattern& lp, const DeviceLayouts& d, const 
       }
   });
   std::string err = check_ic_and_sell(lp, d);
-  fw_check.join();
-  bw_check.join();
+  checks.join();
   if (!err.empty()) return err;
   return err_fw.empty() ? err_bw : err_fw;
 }

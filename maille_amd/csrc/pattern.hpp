@@ -5,12 +5,51 @@
 #pragma once
 
 #include <cstdint>
+#include <exception>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace ml {
 
 typedef int64_t i64;
+
+// Threads of the host-side builders whose exceptions reach the caller: run(f) starts f on a
+// thread of its own; join() waits for all of them and rethrows the first exception one raised.
+// The destructor joins any thread still running (when the caller unwinds before join()).
+class ThreadGroup {
+ public:
+  template <class F>
+  void run(F f) {
+    threads_.emplace_back([this, f]() mutable {
+      try {
+        f();
+      } catch (...) {
+        std::lock_guard<std::mutex> lock(mutex_);
+        if (!error_) error_ = std::current_exception();
+      }
+    });
+  }
+  void join() {
+    for (auto& t : threads_) t.join();
+    threads_.clear();
+    if (error_) {
+      std::exception_ptr e = error_;
+      error_ = nullptr;
+      std::rethrow_exception(e);
+    }
+  }
+  ~ThreadGroup() {
+    for (auto& t : threads_)
+      if (t.joinable()) t.join();
+  }
+
+ private:
+  std::vector<std::thread> threads_;
+  std::mutex mutex_;
+  std::exception_ptr error_;
+};
 
 // iterative.rs:16-22: neighbours of v are targets[offsets[v]..offsets[v+1]], in first-seen order
 struct Graph {

@@ -25,14 +25,14 @@ void parallel_for(i64 n, F body) {
     body((i64)0, n);
     return;
   }
-  std::vector<std::thread> pool;
+  ThreadGroup pool;
   i64 step = (n + t - 1) / t;
   for (unsigned k = 0; k < t; ++k) {
     i64 b = k * step, e = std::min(n, b + step);
     if (b >= e) break;
-    pool.emplace_back([=] { body(b, e); });
+    pool.run([=] { body(b, e); });
   }
-  for (auto& th : pool) th.join();
+  pool.join();
 }
 
 // element_dofs (assembly.rs:20-32, Tet branch): [3n, 3n+1, 3n+2] per node, or the element's

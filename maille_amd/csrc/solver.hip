@@ -321,11 +321,12 @@ static SweepHostLayouts build_sweep_layouts(const std::vector<int64_t>& offsets,
     ferr[h] = check_fused_sweep_layouts(lp, flo, fup);
     s.have[h] = ferr[h].empty();
   };
-  std::thread f256(fused, 0), f512(fused, 1);
+  ThreadGroup fused_builds;
+  fused_builds.run([&] { fused(0); });
+  fused_builds.run([&] { fused(1); });
   s.err = sweep_layouts(lp, &s.lo, &s.up);
   if (s.err.empty()) s.err = check_sweep_layouts(lp, s.lo, s.up);
-  f256.join();
-  f512.join();
+  fused_builds.join();
   for (int h = 0; h < 2 && s.err.empty(); ++h) s.err = ferr[h];
   return s;
 }
