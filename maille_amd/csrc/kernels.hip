@@ -2688,6 +2688,11 @@ __global__ __launch_bounds__(TRI_THREADS) void k_trisolve(
 // over its dense work vector; l_ij = (a_ij - s) / l_jj; then the pivot a_ii(1+shift) - sum of
 // squares must exceed 1e-12 a_ii. Rows of L are published entry by entry; a failed pivot
 // raises *fail and every later row only writes zeros so the launch drains quickly.
+// Loads run ahead of the sums, so that their round trips overlap instead of adding up along
+// the dependency chain: at the row's start every entry's row bounds and diagonal l_jj are
+// loaded at once, and while entry p is summed the columns and values of the next entry's row
+// are loaded. A slot still holding the sentinel when it was loaded is awaited as before; slots
+// are written once, so the values, and the sums, are those of the loads done in order.
 template <typename T, int MAXROW>
 __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ order,
                                             const int* __restrict__ off,
@@ -2701,10 +2706,28 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ order
     residency_probe(fail, limit);
     return;
   }
+  typedef typename Bits<T>::U U;
+  constexpr int PRE = MAXROW < 64 ? MAXROW : 64;  // entries whose rows are looked up at the start
   __shared__ int scol[MAXROW];
   __shared__ T sw[MAXROW];
   __shared__ T sa[MAXROW];
+  __shared__ int sjs[PRE];  // first strict entry of each column's row
+  __shared__ int sjd[PRE];  // diagonal entry of each column's row
+  __shared__ U sdg[PRE];    // that diagonal as loaded at the row's start (maybe the sentinel)
   const int lane = threadIdx.x;
+  auto row_js = [&](int q) { return q < PRE ? sjs[q] : off[scol[q]]; };
+  auto row_jd = [&](int q) { return q < PRE ? sjd[q] : off[scol[q] + 1] - 1; };
+  // the first 64 strict entries of row scol[q]: lane's column (-1 past the row) and value bits
+  auto fetch = [&](int q, int& k, U& v) {
+    const int idx = row_js(q) + lane;
+    k = -1;
+    v = Bits<T>::S;
+    if (idx < row_jd(q)) {
+      k = col[idx];
+      v = __hip_atomic_load(reinterpret_cast<U*>(l + idx), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+    }
+  };
   for (int pos = blockIdx.x; pos < n; pos += gridDim.x) {
     const int i = order[pos];
     const int start = off[i], dg = off[i + 1] - 1, len = dg - start;
@@ -2719,20 +2742,33 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ order
       for (int t = lane; t <= len; t += 64) publish(l + start + t, T(0));
     } else {
       for (int t = lane; t < len; t += 64) {
-        scol[t] = col[start + t];
+        const int j = col[start + t];
+        scol[t] = j;
         sa[t] = a[start + t];
+        if (t < PRE) {
+          const int jd = off[j + 1] - 1;
+          sjs[t] = off[j];
+          sjd[t] = jd;
+          sdg[t] = __hip_atomic_load(reinterpret_cast<U*>(l + jd), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
       wave_sync();
+      int ck = -1, nk = -1;
+      U cv = Bits<T>::S, nv = Bits<T>::S;
+      if (len > 0) fetch(0, ck, cv);
       for (int q = 0; q < len; ++q) {
+        if (q + 1 < len) fetch(q + 1, nk, nv);  // used by the next entry
         const int j = scol[q];
-        const int js = off[j], jd = off[j + 1] - 1;
+        const int js = row_js(q), jd = row_jd(q);
         T sacc = T(0);
         for (int base = js; base < jd; base += 64) {
           const int idx = base + lane;
+          const bool first = base == js;  // the prefetched chunk
           T term = T(0);
           bool hit = false;
           if (idx < jd) {
-            const int k = col[idx];
+            const int k = first ? ck : col[idx];
             int lo = 0, hi = q;  // search k among the computed columns scol[0..q)
             while (lo < hi) {
               int mid = (lo + hi) >> 1;
@@ -2740,7 +2776,8 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ order
               else hi = mid;
             }
             if (lo < q && scol[lo] == k) {
-              const T ljk = await_value(l + idx, abort_flag, limit);
+              const T ljk = first && cv != Bits<T>::S ? from_bits<T>(cv)
+                                                      : await_value(l + idx, abort_flag, limit);
               term = ljk * sw[lo];
               hit = true;
             }
@@ -2752,7 +2789,8 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ order
             mask &= mask - 1;
           }
         }
-        const T ljj = await_value(l + jd, abort_flag, limit);
+        const U dg = q < PRE ? sdg[q] : Bits<T>::S;
+        const T ljj = dg != Bits<T>::S ? from_bits<T>(dg) : await_value(l + jd, abort_flag, limit);
         if (lane == 0 && __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
             __hip_atomic_load(where, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 0) {
           __hip_atomic_store(where, i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2763,6 +2801,8 @@ __global__ __launch_bounds__(64) void k_ic0(int n, const int* __restrict__ order
         const T lij = (sa[q] - sacc) / ljj;
         if (lane == 0) sw[q] = lij;
         wave_sync();
+        ck = nk;
+        cv = nv;
       }
       T squares = T(0);
       for (int q = 0; q < len; ++q) squares = squares + sw[q] * sw[q];
