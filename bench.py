@@ -792,6 +792,15 @@ def main():
                                          "equal columns (a node's DOFs)",
                                       2: "16-bit column offsets per entry",
                                       4: "int32 columns"}[spmv_form]
+    spmv_traffic = measured_spmv_traffic(f"jittered_cube_{args.cells}", head, spmv_form)
+    if spmv_traffic and kernels["spmv"]["launches"]:
+        # the same launches' rate on the HBM bytes the committed counter passes measured
+        rate = spmv_traffic / (kernels["spmv"]["avg_ms"] * 1e-3) / 1e9
+        kernels["spmv"]["measured_traffic"] = {
+            "bytes_per_launch": spmv_traffic, "GBs": round(rate, 1),
+            "frac_of_peak": round(rate / HBM_PEAK_GBS, 4),
+            "source": "replayed from the committed FETCH_SIZE/WRITE_SIZE passes, not measured "
+                      "in this run"}
     kernels["precondition"]["form"] = (
         f"{sweeps} Jacobi sweeps per triangular system, all in one launch per system "
         f"(k_sweeps_fused, 2 launches, {fused} coefficients per row in registers)" if fused else
@@ -920,6 +929,22 @@ def copy_probe(torch):
     torch.mul(a, 1.0, out=b)
     torch.cuda.synchronize()
     log("copy probe: 268435456 bytes read, 268435456 bytes written")
+
+
+def measured_spmv_traffic(workload, precision, index_form):
+    """HBM bytes per launch of the FP64 SpMV in the index form in use (1: grouped column
+    streams, 2: 16-bit offsets per entry), from the committed counter passes of the fused sweeps
+    on this workload and precision (the SpMV's launches of the same runs); None otherwise."""
+    path = os.path.join(ROOT, "profiles", f"traffic_k_sweeps_fused_{workload}_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    name = {1: "k_spmv_grouped<double, 8>", 2: "k_spmv<double, 8, true>"}.get(index_form)
+    other = rec.get("other_kernels_same_runs", {}).get(name)
+    if not other or "fetch_bytes" not in other or "write_bytes" not in other:
+        return None
+    return int(other["fetch_bytes"] + other["write_bytes"])
 
 
 def measured_traffic(kernel, workload, precision, sweeps, variant):
