@@ -118,7 +118,7 @@ def run_beam_cases(args, ma, meshes):
             m.factor()
             chosen = m.autotune(nrhs=len(F))
             log(f"beam {name}: {p.nfree} free DOF, {len(F)} cases; autotuned {chosen}, "
-                f"{m.sweeps()} sweeps, {m.batch_lanes()} lanes per row")
+                f"{m.sweeps()} sweeps (weight {m.sweep_weight()}), {m.batch_lanes()} lanes per row")
             m.solve_many(F[:8])
             t0 = time.perf_counter()
             U, infos = m.solve_many(F)
@@ -136,7 +136,8 @@ def run_beam_cases(args, ma, meshes):
                            "sequential_s_per_case": round(sequential, 4),
                            "speedup_vs_sequential": round(sequential * len(F) / batched, 2),
                            "iterations_max": max(i["iterations"] for i in infos),
-                           "sweeps": m.sweeps(), "waves_per_cu": chosen[0],
+                           "sweeps": m.sweeps(), "sweep_weight": m.sweep_weight(),
+                           "waves_per_cu": chosen[0],
                            "cases_per_pass": chosen[1],
                            "rows_per_wave": 64 // max(m.batch_lanes(), 1) if m.batch_lanes() else None,
                            "rel_err_last_checked_vs_sequential": err_seq,
@@ -255,13 +256,15 @@ def run_tet10(args, ma, meshes):
                 steps.append(time.perf_counter() - t0)
             step = min(steps)
             sweeps = m.sweeps()
+            weight = m.sweep_weight()
             f = m.loads()
         if u64 is None:
             u64 = u
         balance = float(np.abs(r.reshape(-1, 3).sum(axis=0) + f.reshape(-1, 3).sum(axis=0)).max()
                         / np.abs(f.reshape(-1, 3).sum(axis=0)).max())
         res[name] = {"s_per_step": round(step, 4), "iterations": info["iterations"],
-                     "sweeps": sweeps, "true_residual": info["true_residual"],
+                     "sweeps": sweeps, "sweep_weight": weight,
+                     "true_residual": info["true_residual"],
                      "escalated": info["escalated"], "device_retries": info["device_retries"],
                      "rel_err_vs_fp64": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
                      "relative_gap_vs_skyline": (float(np.abs(u - u_sky).max() /
@@ -480,7 +483,7 @@ def run_plastic(args, ma, meshes):
           "config": {"workload": "plastic_bar_120x24x24", "free_dof": p.nfree,
                      "elements": int(ne), "increments": args.increments,
                      "yield_stress": sy, "hardening": hd, "precision": prec,
-                     "sweeps": m.sweeps()},
+                     "sweeps": m.sweeps(), "sweep_weight": m.sweep_weight()},
           "newton_iterations": info["newton_iterations"], "cuts": info["cuts"],
           "pcg_iterations": info["pcg_iterations"],
           "residuals": [[float(f"{r:.3e}") for r in inc.residuals] for inc in incs[1:]],
@@ -622,7 +625,7 @@ def run_large(args, ma, meshes):
           "config": {"workload": "neo_hookean_block_40", "elements": int(ne),
                      "free_dof": prob.nfree, "follower_faces": len(pressures),
                      "pressure": round(p, 3), "stretch": stretch, "precision": prec,
-                     "sweeps": m.sweeps()},
+                     "sweeps": m.sweeps(), "sweep_weight": m.sweep_weight()},
           "newton_iterations": info["newton_iterations"], "cuts": info["cuts"],
           "pcg_iterations": info["pcg_iterations"],
           "residuals": [[float(f"{r:.3e}") for r in inc.residuals] for inc in incs[1:]],
@@ -707,6 +710,7 @@ def main():
                         "autotune_s": round(time.perf_counter() - t1 - t_create, 3),
                         "waves_per_cu": chosen[0], "reported_occupancy": chosen[2],
                         "sweeps": models[prec].sweeps(),
+                        "sweep_weight": models[prec].sweep_weight(),
                         "residency_reported_used": models[prec].residency()}
         log(f"{prec}: model created in {t_create:.1f} s (host pattern "
             f"{create[prec]['host_pattern_s']:.1f} s), autotuned in "
@@ -887,6 +891,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"jittered_cube_{args.cells}", "free_dof": n, "nnz_L": nnz_l,
                    "elements": len(problem.conn), "precision": head, "sweeps": sweeps,
+                   "sweep_weight": models[head].sweep_weight(),
                    "step": "assemble + IC(0) + PCG to 1e-12 + reactions, inputs in HBM",
                    "dag_depth": m0.levels()},
         "iterations": int(info["iterations"]), "true_residual": info["true_residual"],

@@ -235,6 +235,14 @@ ml_status ml_pcg_set_wait_limit(ml_pcg* h, int64_t ticks);
  * over all of them (ml_pcg_precondition_many). ml_pcg_sweeps returns the current count. */
 ml_status ml_pcg_set_sweeps(ml_pcg* h, int sweeps);
 int ml_pcg_sweeps(const ml_pcg* h);
+/* No counterpart in the reference: the weight w of the sweeps (0 < w <= 1; 1, the default, is
+ * undamped). A damped sweep is y <- (1 - w) y + w D^-1 (r - N y), in both systems, so the
+ * preconditioner stays symmetric positive definite; it converges in fewer PCG iterations where
+ * the factor's off-diagonal part is large against its diagonal (Tet10). The autotuner picks the
+ * weight with the count (1 or 0.8) unless the caller set either; ML_INVALID_ARGUMENT on an FP64
+ * handle or a weight outside (0, 1]. */
+ml_status ml_pcg_set_sweep_weight(ml_pcg* h, double w);
+double ml_pcg_sweep_weight(const ml_pcg* h);
 /* No counterpart in the reference: how the sweeps are launched. fused 1 (the default): all
  * sweeps of a triangular system in one launch of a persistent kernel that keeps each row's
  * coefficients in registers, so the factor is read once per system instead of once per sweep,
@@ -281,6 +289,9 @@ ml_status ml_model_create(int64_t n_nodes, const double* nodes, int64_t n_elemen
 /* ml_pcg_set_sweeps / ml_pcg_sweeps for the model's operator. */
 ml_status ml_model_set_sweeps(ml_model* m, int sweeps);
 int ml_model_sweeps(const ml_model* m);
+/* ml_pcg_set_sweep_weight / ml_pcg_sweep_weight for the model's operator. */
+ml_status ml_model_set_sweep_weight(ml_model* m, double w);
+double ml_model_sweep_weight(const ml_model* m);
 /* ml_pcg_set_reference_dots for the model's operator: with ML_FP64 and zero imposed
  * displacements, Model::solve's displacements are then the reference's bit for bit. */
 ml_status ml_model_set_reference_dots(ml_model* m, int on);

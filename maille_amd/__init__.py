@@ -47,6 +47,8 @@ EXPORTS = (
     "ml_pcg_precondition", "ml_pcg_precondition_many", "ml_pcg_factor", "ml_pcg_destroy",
     "ml_pcg_trace",
     "ml_pcg_set_wait_limit", "ml_pcg_set_sweeps", "ml_pcg_sweeps", "ml_model_set_sweeps",
+    "ml_pcg_set_sweep_weight", "ml_pcg_sweep_weight", "ml_model_set_sweep_weight",
+    "ml_model_sweep_weight",
     "ml_model_sweeps", "ml_model_index_bytes", "ml_model_spmv_index", "ml_model_residency",
     "ml_pcg_set_sweeps_fused", "ml_pcg_sweeps_fused", "ml_model_set_sweeps_fused",
     "ml_pcg_set_reference_dots", "ml_model_set_reference_dots",
@@ -203,6 +205,10 @@ def lib():
             "ml_pcg_sweeps": (_int, [_P]),
             "ml_model_set_sweeps": (_int, [_P, _int]),
             "ml_model_sweeps": (_int, [_P]),
+            "ml_pcg_set_sweep_weight": (_int, [_P, C.c_double]),
+            "ml_pcg_sweep_weight": (C.c_double, [_P]),
+            "ml_model_set_sweep_weight": (_int, [_P, C.c_double]),
+            "ml_model_sweep_weight": (C.c_double, [_P]),
             "ml_model_index_bytes": (_int, [_P, _I32]),
             "ml_pcg_set_reference_dots": (_int, [_P, _int]),
             "ml_model_set_reference_dots": (_int, [_P, _int]),
@@ -601,6 +607,13 @@ class Pcg(_Handle):
     def sweeps(self) -> int:
         return lib().ml_pcg_sweeps(self._h)
 
+    def set_sweep_weight(self, w: float):
+        """Weight of the sweeps (0 < w <= 1; 1: undamped)."""
+        _check(lib().ml_pcg_set_sweep_weight(self._h, float(w)))
+
+    def sweep_weight(self) -> float:
+        return lib().ml_pcg_sweep_weight(self._h)
+
     def set_sweeps_fused(self, fused):
         """All sweeps of a triangular system in one launch (True, the default, where the
         pattern fits the fused kernel; or an int variant code 100 * rows per lane + register
@@ -716,6 +729,13 @@ class Model(_Handle):
 
     def sweeps(self) -> int:
         return lib().ml_model_sweeps(self._h)
+
+    def set_sweep_weight(self, w: float):
+        """Pcg.set_sweep_weight for the model's operator."""
+        _check(lib().ml_model_set_sweep_weight(self._h, float(w)))
+
+    def sweep_weight(self) -> float:
+        return lib().ml_model_sweep_weight(self._h)
 
     def set_sweeps_fused(self, fused):
         """Pcg.set_sweeps_fused for the model's operator."""

@@ -1531,7 +1531,9 @@ __global__ __launch_bounds__(256) void k_spmv_grouped(int n, const int* __restri
 // written (the next system's starting iterate, the same product FIRST would form). With
 // `partial`, the block's share of dot.out (the PCG's r.z) is written, in a fixed tree. With
 // `level`, out is an intermediate iterate and is written as level_value(out), the rounding the
-// fused sweeps give the iterates they exchange.
+// fused sweeps give the iterates they exchange. With a weight w != 1 the sweep is damped:
+// out_i = c1 * in_i + w * t_i, with t_i the undamped result above and c1 = 1 - w (each product
+// and the sum rounded, in that order), the same in every form of the sweeps.
 template <bool FIRST, bool D16, int BATCH = 8>
 __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ soff,
                                                const int* __restrict__ scol,
@@ -1545,7 +1547,8 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
                                                double* __restrict__ out_rd,
                                                const double* __restrict__ dot,
                                                double* __restrict__ partial,
-                                               const PcgScalars* __restrict__ S) {
+                                               const PcgScalars* __restrict__ S, double w,
+                                               double c1) {
 #pragma clang fp contract(off)
   if (S && S->done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1577,7 +1580,8 @@ __global__ __launch_bounds__(256) void k_sweep(int n, const int* __restrict__ so
       const int c = column(at);
       s = s + (double)sval[at] * (FIRST ? rhs[c] * rd[c] : in[c]);
     }
-    const double y = (rhs[i] - s) * rd[i];
+    const double tv = (rhs[i] - s) * rd[i];
+    const double y = w != 1.0 ? c1 * (FIRST ? rhs[i] * rd[i] : in[i]) + w * tv : tv;
     out[i] = level ? level_value(y) : y;
     if (out_rd) out_rd[i] = y * rd[i];
     if (dot) dv = dot[i] * y;
@@ -1677,8 +1681,9 @@ __global__ __launch_bounds__(256) void k_spmm_il(int n, int nrep, const int* __r
 // (row i, W consecutive vectors). For each vector the arithmetic is k_sweep's (the row summed
 // in ascending column from 0, FP32 coefficients widened exactly, FP64 vectors, contraction
 // off; `level`: the result rounded to level_value), so each is bit-identical to its own
-// k_sweep; a first sweep reads in = rhs * rd written beforehand (the product k_sweep forms).
-// With out_rd, out * rd is written as well (the next system's first iterate).
+// k_sweep; a first sweep reads in = rhs * rd written beforehand (the product k_sweep forms),
+// and a weight w != 1 damps the sweep as k_sweep does. With out_rd, out * rd is written as well
+// (the next system's first iterate).
 template <bool D16, int W>
 __global__ __launch_bounds__(256) void k_sweep_il(int n, int nrep, const int* __restrict__ soff,
                                                   const int* __restrict__ scol,
@@ -1689,7 +1694,8 @@ __global__ __launch_bounds__(256) void k_sweep_il(int n, int nrep, const int* __
                                                   const double* __restrict__ rhs,
                                                   const double* __restrict__ in,
                                                   double* __restrict__ out, int level,
-                                                  double* __restrict__ out_rd) {
+                                                  double* __restrict__ out_rd, double w,
+                                                  double c1) {
 #pragma clang fp contract(off)
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int chunks = (nrep + W - 1) / W;
@@ -1722,7 +1728,8 @@ __global__ __launch_bounds__(256) void k_sweep_il(int n, int nrep, const int* __
     s = s + (double)sval[at] * iterate(column(at));
   }
   const int64_t o = (int64_t)i * nrep + b;
-  const double y = (rhs[o] - s) * rd[i];
+  const double tv = (rhs[o] - s) * rd[i];
+  const double y = w != 1.0 ? c1 * in[o] + w * tv : tv;
   out[o] = level ? level_value(y) : y;
   if (out_rd) out_rd[o] = y * rd[i];
 }
@@ -1752,7 +1759,8 @@ constexpr int SWEEP_STAGE = 4;    // external values a lane has in flight while 
 // sweep j + 1. A slot holds another launch's tag until this launch writes it, so the levels
 // need no reset between launches as long as the host gives every launch since the levels
 // were last cleared a tag of its own. The last sweep writes out (not rounded) and, with
-// partial, each 256-row sub-block's share of dot.out (the partials of k_sweep's grid).
+// partial, each 256-row sub-block's share of dot.out (the partials of k_sweep's grid). A weight
+// w != 1 damps every sweep as k_sweep does, from the row's previous iterate held in LDS.
 // MINW: waves per SIMD the register allocation must allow (1: no bound).
 // nblk < 0: co-residency probe of the grid.
 template <int MAXLEN, bool REV, int MINW = 1, int RPL = 1>
@@ -1764,7 +1772,7 @@ k_sweeps_fused(int n, int nblk, int k, const int* __restrict__ eoff,
                const double* __restrict__ rhs, double* __restrict__ lev, unsigned tag,
                double* __restrict__ out, const double* __restrict__ dot,
                double* __restrict__ partial, const PcgScalars* __restrict__ S,
-               int* abort_flag, long long limit) {
+               int* abort_flag, long long limit, double w, double c1) {
 #pragma clang fp contract(off)
   static_assert(MAXLEN % 4 == 0, "entries are held in quads");
   constexpr int R = SWEEP_LANES * RPL;
@@ -1843,7 +1851,8 @@ k_sweeps_fused(int n, int nblk, int k, const int* __restrict__ eoff,
           const int at = base[h] + (q >> 2) * 4 * SWEEP_LANES + (q & 3);
           s = s + (double)eval[at] * buf[eidx[at]];
         }
-        y[h] = (ri[h] - s) * rdi[h];
+        const double tv = (ri[h] - s) * rdi[h];
+        y[h] = w != 1.0 ? c1 * buf[h * SWEEP_LANES + t] + w * tv : tv;
       }
       if (j >= k) break;
       double* level = lev + (int64_t)(j - 1) * n;

@@ -240,8 +240,8 @@ int fused_resident(void (*kernel)(TArgs...), hipStream_t stream) {
                          [&](int grid, int* probe, long long limit) {
     kernel<<<grid, mlk::SWEEP_LANES, 0, stream>>>(0, -1, 0, nullptr, nullptr, nullptr, nullptr,
                                                  nullptr, nullptr, nullptr, nullptr, nullptr, 0u,
-                                                 nullptr, nullptr, nullptr, nullptr, probe,
-                                                 limit);
+                                                 nullptr, nullptr, nullptr, nullptr, probe, limit,
+                                                 1.0, 0.0);
   });
 }
 
@@ -473,6 +473,7 @@ struct Operator {
   int lev_tag = 1;                  // the next fused launch's level tag
   bool have_sweep = false;
   int sweeps = 0;  // Jacobi sweeps per triangular system; 0: the exact dataflow solves
+  double sweep_w = 1.0;  // the sweeps' weight (1: undamped)
   int sweep_batch = 8, spmv_batch = 8;  // entries per batch of loads (autotuned: 8 or 16)
   bool sweeps_pinned = false;  // set by the caller: the autotuner keeps the count
   bool ref_dots = false;       // PCG dot products in the reference's order (k_dot8)
@@ -646,7 +647,7 @@ struct Operator {
   typedef void (*FusedKernel)(int, int, int, const int*, const uint8_t*, const float*,
                               const uint16_t*, const int*, const int*, const double*,
                               const double*, double*, unsigned, double*, const double*, double*,
-                              const mlk::PcgScalars*, int*, long long);
+                              const mlk::PcgScalars*, int*, long long, double, double);
   // k_sweeps_fused in one of its variants (default: the one in use)
   FusedKernel fused_kernel(bool upper, int var = -1) const {
     switch (var < 0 ? fused_var : var) {
@@ -957,11 +958,13 @@ struct Operator {
       const unsigned tag_lo = k > 1 ? next_level_tag() : 0u;
       fused_kernel(false)<<<fused_grid, mlk::SWEEP_LANES, 0, stream>>>(
           n, fl.nblk, k, fl.eoff.p, fl.elen.p, fl.eval.p, fl.eidx.p, fl.xoff.p, fl.xcol.p,
-          d_sw_rd.p, r, lo, tag_lo, a, nullptr, nullptr, S, d_flags.p, wait_ticks);
+          d_sw_rd.p, r, lo, tag_lo, a, nullptr, nullptr, S, d_flags.p, wait_ticks, sweep_w,
+          1.0 - sweep_w);
       const unsigned tag_up = k > 1 ? next_level_tag() : 0u;
       fused_kernel(true)<<<fused_grid, mlk::SWEEP_LANES, 0, stream>>>(
           n, fu.nblk, k, fu.eoff.p, fu.elen.p, fu.eval.p, fu.eidx.p, fu.xoff.p, fu.xcol.p,
-          d_sw_rd.p, a, up, tag_up, z, dot, partial, S, d_flags.p, wait_ticks);
+          d_sw_rd.p, a, up, tag_up, z, dot, partial, S, d_flags.p, wait_ticks, sweep_w,
+          1.0 - sweep_w);
       HIP_CHECK(hipGetLastError());
       prof.end(P_TRI32, ev, stream);
       return;
@@ -976,7 +979,7 @@ struct Operator {
       auto go = [&](auto kernel) {
         kernel<<<g, 256, 0, stream>>>(n, t.soff.p, t.scol.p, t.sdel.p, sign, t.slen.p, t.sval.p,
                                       d_sw_rd.p, rhs, in, out, level ? 1 : 0, out_rd, dv, pv,
-                                      S);
+                                      S, sweep_w, 1.0 - sweep_w);
       };
       if (sweep_batch == 16) {
         if (t.d16) first ? go(mlk::k_sweep<true, true, 16>) : go(mlk::k_sweep<false, true, 16>);
@@ -1078,7 +1081,8 @@ struct Operator {
       const int sign = &t == &sw_lo ? -1 : 1;
       auto go = [&](auto kernel) {
         kernel<<<g, 256, 0, stream>>>(n, nrep, t.soff.p, t.scol.p, t.sdel.p, sign, t.slen.p,
-                                      t.sval.p, d_sw_rd.p, rhs, in, out, level ? 1 : 0, out_rd);
+                                      t.sval.p, d_sw_rd.p, rhs, in, out, level ? 1 : 0, out_rd,
+                                      sweep_w, 1.0 - sweep_w);
       };
       auto width = [&](auto d16) {
         constexpr bool D = decltype(d16)::value;
@@ -1532,8 +1536,16 @@ struct Operator {
       // (with a 10% margin); a count that reaches that cap is slower and is dropped
       double best_t = 1e300, other = -1.0;
       int best_k = sweeps;
-      for (int k : {6, 4, 8, 5, 3, 2, 0}) {
+      double best_w = sweep_w;
+      // (sweeps, weight): undamped first, then damped by 0.8, which shortens the solves whose
+      // factor's off-diagonal part is large against its diagonal (Tet10)
+      static const std::pair<int, double> kCandidates[] = {
+          {6, 1.0}, {4, 1.0}, {8, 1.0}, {5, 1.0}, {3, 1.0}, {2, 1.0},
+          {4, 0.8}, {6, 0.8}, {3, 0.8}, {8, 0.8}, {2, 0.8}, {0, 1.0}};
+      for (const auto& cand : kCandidates) {
+        const int k = cand.first;
         set_sweeps(k);
+        sweep_w = cand.second;
         const double app = application_seconds();
         if (app < 0.0) continue;  // a wait gave up: not a candidate
         int64_t cap = ML_MAX_ITERATIONS;
@@ -1549,10 +1561,12 @@ struct Operator {
         if (st == ML_OK && t < best_t) {
           best_t = t;
           best_k = k;
+          best_w = cand.second;
           other = std::max(0.0, t / (double)std::max<int64_t>(hS.it, 1) - app);
         }
       }
       set_sweeps(best_k);
+      sweep_w = best_w;
     }
     if (chosen) {
       chosen[0] = best_pc;
@@ -3492,6 +3506,29 @@ ml_status ml_model_set_sweeps(ml_model* h, int sweeps) {
 }
 
 int ml_model_sweeps(const ml_model* h) { return h ? h->m.op.sweeps : 0; }
+
+static ml_status set_sweep_weight(Operator* op, double w) {
+  return guarded([&]() -> int {
+    if (!op) throw ArgError("null handle");
+    if (!(w > 0.0 && w <= 1.0)) throw ArgError("the sweep weight must be in (0, 1]");
+    if (op->precision == ML_FP64) throw ArgError("the FP64 drop-in has no sweeps");
+    op->sweep_w = w;
+    op->sweeps_pinned = true;
+    return ML_OK;
+  });
+}
+
+ml_status ml_pcg_set_sweep_weight(ml_pcg* h, double w) {
+  return set_sweep_weight(h ? &h->op : nullptr, w);
+}
+
+double ml_pcg_sweep_weight(const ml_pcg* h) { return h ? h->op.sweep_w : 0.0; }
+
+ml_status ml_model_set_sweep_weight(ml_model* h, double w) {
+  return set_sweep_weight(h ? &h->m.op : nullptr, w);
+}
+
+double ml_model_sweep_weight(const ml_model* h) { return h ? h->m.op.sweep_w : 0.0; }
 
 static ml_status set_batch_lanes(Operator* op, int lanes) {
   return guarded([&]() -> int {

@@ -10,6 +10,8 @@ each row sum taken sequentially in ascending j from 0, every operation rounded t
 The k - 1 intermediate iterates of each system are the values the fused kernel exchanges
 between workgroups with a launch tag in their low 8 significand bits: they are rounded toward
 zero to 44 fraction bits (those bits cleared); the last iterate of each system is not.
+With a weight w != 1 every sweep is damped: y_i <- c1 * y_i + w * t_i, t_i the undamped
+result and c1 = 1 - w, each product and the sum rounded to float64 in that order.
 """
 import numpy as np
 
@@ -32,11 +34,12 @@ def _padded_rows(n, rows, cols, vals):
     return c, v, m
 
 
-def _sweep(c, v, m, rhs, x, rd):
+def _sweep(c, v, m, rhs, x, rd, w=1.0):
     s = np.zeros(len(rhs))
     for q in range(c.shape[1]):
         s = np.where(m[:, q], s + v[:, q] * x[c[:, q]], s)
-    return (rhs - s) * rd
+    t = (rhs - s) * rd
+    return t if w == 1.0 else (1.0 - w) * x + w * t
 
 
 def level_value(v):
@@ -44,7 +47,7 @@ def level_value(v):
     return (np.asarray(v, dtype=np.float64).view(np.uint64) & np.uint64(0xFFFFFFFFFFFFFF00)).view(np.float64)
 
 
-def jacobi_precondition(off, cols, l, r, sweeps):
+def jacobi_precondition(off, cols, l, r, sweeps, w=1.0):
     off = np.asarray(off, dtype=np.int64)
     cols = np.asarray(cols, dtype=np.int64)
     l = np.asarray(l, dtype=np.float64)
@@ -60,12 +63,12 @@ def jacobi_precondition(off, cols, l, r, sweeps):
     up = _padded_rows(n, lc, lr, lv)  # rows of L^T: column j of L, ascending row
     y = r * rd
     for m in range(1, sweeps + 1):
-        y = _sweep(*lo, r, y, rd)
+        y = _sweep(*lo, r, y, rd, w)
         if m < sweeps:
             y = level_value(y)
     z = y * rd
     for m in range(1, sweeps + 1):
-        z = _sweep(*up, y, z, rd)
+        z = _sweep(*up, y, z, rd, w)
         if m < sweeps:
             z = level_value(z)
     return z

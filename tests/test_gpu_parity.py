@@ -103,27 +103,30 @@ def test_mixed_preconditioner_applies_the_fp32_factor(gpu, precision):
     assert np.abs(l - l64).max() <= 1e-5 * np.abs(l64).max()
 
 
-SWEEP_CASES = [("jittered_cube_8", k, f) for k in (1, 2, 3, 4, 7) for f in (1, 116, 216, 224, 0)] + \
-    [("cantilever_tet10_small", k, f) for k in (1, 3) for f in (1, 116, 148, 0)]
+SWEEP_CASES = [("jittered_cube_8", k, f, 1.0) for k in (1, 2, 3, 4, 7) for f in (1, 116, 216, 224, 0)] + \
+    [("cantilever_tet10_small", k, f, 1.0) for k in (1, 3) for f in (1, 116, 148, 0)] + \
+    [("jittered_cube_8", 3, f, 0.8) for f in (1, 116, 216, 0)] + \
+    [("cantilever_tet10_small", k, f, w) for k in (2, 4) for f in (1, 148, 0) for w in (0.8, 0.6)]
 
 
-@pytest.mark.parametrize("name,sweeps,fused", SWEEP_CASES)
-def test_jacobi_sweeps_match_their_restatement(gpu, name, sweeps, fused):
+@pytest.mark.parametrize("name,sweeps,fused,weight", SWEEP_CASES)
+def test_jacobi_sweeps_match_their_restatement(gpu, name, sweeps, fused, weight):
     """ML_MIXED_JACOBI's preconditioner: `sweeps` Jacobi sweeps of L y = r and of L^T z = y on
     the FP32 factor, bit for bit the float64 restatement, on every call, whether all sweeps of
     a system run in one launch (fused 1: one row per lane with every row in registers, 24
     coefficients on the cube and 80 on the Tet10 mesh, whose rows reach 146 and re-read their
     tails; 116 and 148: variants that keep fewer and re-read more; 216, 224: two rows per lane
-    in blocks of 512 rows) or one launch per sweep (0); more sweeps approach the exact
-    triangular solves."""
+    in blocks of 512 rows) or one launch per sweep (0), undamped or damped by a weight; more
+    sweeps approach the exact triangular solves."""
     from sweeps import jacobi_precondition
     p = PROBLEMS[name]
     s = _oracle_system(p)
     off, cols, vals = s.csr()
-    print(f"jacobi {name} sweeps={sweeps} fused={fused} n={s.n}", flush=True)
+    print(f"jacobi {name} sweeps={sweeps} fused={fused} weight={weight} n={s.n}", flush=True)
     pcg = gpu.Pcg(gpu.Csr(off, cols, vals), precision=gpu.MIXED_JACOBI)
-    assert pcg.sweeps() == 4
+    assert pcg.sweeps() == 4 and pcg.sweep_weight() == 1.0
     pcg.set_sweeps(sweeps)
+    pcg.set_sweep_weight(weight)
     pcg.set_sweeps_fused(fused)
     assert pcg.sweeps() == sweeps
     code = {1: 124 if name == "jittered_cube_8" else 180}.get(fused, fused)
@@ -131,7 +134,7 @@ def test_jacobi_sweeps_match_their_restatement(gpu, name, sweeps, fused):
     r = np.cos(0.11 * np.arange(s.n))
     z = pcg.precondition(r)
     l, _ = pcg.factor()
-    assert np.array_equal(z, jacobi_precondition(off, cols, l, r, sweeps))
+    assert np.array_equal(z, jacobi_precondition(off, cols, l, r, sweeps, weight))
     assert np.array_equal(pcg.precondition(r), z)
     exact = oracle.Csr.from_arrays(off, cols, vals).precondition(l, r)
     far = rel_l2(jacobi_precondition(off, cols, l, r, 1), exact)
@@ -144,22 +147,24 @@ def test_fused_and_per_launch_sweeps_solve_identically(gpu):
     iterates, iteration counts and residuals."""
     p = PROBLEMS["jittered_cube_8"]
     print("fused vs per-launch solve", flush=True)
-    out = {}
-    with gpu.Model(p, precision=gpu.MIXED_JACOBI) as m:
-        m.set_sweeps(3)
-        for fused in (1, 116, 224, 0):
-            m.set_sweeps_fused(fused)
-            assert m.sweeps_fused() == {1: 124}.get(fused, fused)
-            u, r, info = m.solve()
-            assert info["device_retries"] == 0
-            out[fused] = (u, r, info)
-    u1, r1, i1 = out[0]
-    for fused in (1, 116, 224):
-        u2, r2, i2 = out[fused]
-        assert np.array_equal(u1, u2) and np.array_equal(r1, r2)
-        assert i1["iterations"] == i2["iterations"] and i1["residual"] == i2["residual"]
     ref = oracle.model_solve(p, iterative=True)
-    assert rel_l2(u1, ref["displacements"]) <= 1e-8
+    for weight in (1.0, 0.8):
+        out = {}
+        with gpu.Model(p, precision=gpu.MIXED_JACOBI) as m:
+            m.set_sweeps(3)
+            m.set_sweep_weight(weight)
+            for fused in (1, 116, 224, 0):
+                m.set_sweeps_fused(fused)
+                assert m.sweeps_fused() == {1: 124}.get(fused, fused)
+                u, r, info = m.solve()
+                assert info["device_retries"] == 0
+                out[fused] = (u, r, info)
+        u1, r1, i1 = out[0]
+        for fused in (1, 116, 224):
+            u2, r2, i2 = out[fused]
+            assert np.array_equal(u1, u2) and np.array_equal(r1, r2)
+            assert i1["iterations"] == i2["iterations"] and i1["residual"] == i2["residual"]
+        assert rel_l2(u1, ref["displacements"]) <= 1e-8
 
 
 def test_fp64_handle_refuses_sweeps(gpu):
@@ -170,6 +175,12 @@ def test_fp64_handle_refuses_sweeps(gpu):
     pcg.set_sweeps(0)
     with pytest.raises(ValueError, match="exactly"):
         pcg.set_sweeps(2)
+    with pytest.raises(ValueError, match="no sweeps"):
+        pcg.set_sweep_weight(0.8)
+    mixed = gpu.Pcg(gpu.Csr(off, cols, vals), precision=gpu.MIXED_JACOBI)
+    for bad in (0.0, -0.5, 1.5):
+        with pytest.raises(ValueError, match="weight"):
+            mixed.set_sweep_weight(bad)
 
 
 @pytest.mark.parametrize("name", list(PROBLEMS))
@@ -531,10 +542,12 @@ def test_load_cases_of_one_model(gpu):
         assert abs(infos[k]["iterations"] - info1["iterations"]) <= 3
 
 
-@pytest.mark.parametrize("precision,sweeps,lanes", [
-    ("FP64", 0, 0), ("MIXED_IC0", 0, 0), ("MIXED_JACOBI", 1, 0), ("MIXED_JACOBI", 3, 0),
-    ("MIXED_JACOBI", 4, 0), ("MIXED_JACOBI", 3, 8), ("MIXED_JACOBI", 2, 32)])
-def test_batched_preconditioner_is_the_single_one_bit_for_bit(gpu, precision, sweeps, lanes):
+@pytest.mark.parametrize("precision,sweeps,lanes,weight", [
+    ("FP64", 0, 0, 1.0), ("MIXED_IC0", 0, 0, 1.0), ("MIXED_JACOBI", 1, 0, 1.0),
+    ("MIXED_JACOBI", 3, 0, 1.0), ("MIXED_JACOBI", 4, 0, 1.0), ("MIXED_JACOBI", 3, 8, 1.0),
+    ("MIXED_JACOBI", 2, 32, 1.0), ("MIXED_JACOBI", 3, 0, 0.8), ("MIXED_JACOBI", 2, 16, 0.6)])
+def test_batched_preconditioner_is_the_single_one_bit_for_bit(gpu, precision, sweeps, lanes,
+                                                              weight):
     """The preconditioner as batched solves apply it to many right-hand sides (replicated exact
     solves, or the Jacobi sweeps of every vector in one launch per sweep on interleaved copies,
     with the lanes per row following the vectors of a pass or set) gives each vector exactly
@@ -548,6 +561,7 @@ def test_batched_preconditioner_is_the_single_one_bit_for_bit(gpu, precision, sw
     pcg = gpu.Pcg(gpu.Csr(off, cols, vals), precision=getattr(gpu, precision))
     if precision == "MIXED_JACOBI":
         pcg.set_sweeps(sweeps)
+        pcg.set_sweep_weight(weight)
     pcg.set_batch_lanes(lanes)
     assert pcg.batch_lanes() == lanes
     rng = np.random.default_rng(19)
@@ -559,7 +573,8 @@ def test_batched_preconditioner_is_the_single_one_bit_for_bit(gpu, precision, sw
         for k in range(len(R)):
             assert np.array_equal(Z[k], pcg.precondition(R[k]))
             if precision == "MIXED_JACOBI" and k < 12:
-                assert np.array_equal(Z[k], jacobi_precondition(off, cols, l, R[k], sweeps))
+                assert np.array_equal(Z[k], jacobi_precondition(off, cols, l, R[k], sweeps,
+                                                                weight))
         assert np.all(Z[6] == 0.0)
 
 

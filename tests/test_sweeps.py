@@ -83,6 +83,45 @@ def test_sweeps_are_the_polynomial_preconditioner():
         assert a @ ma > 0.0
 
 
+def test_damped_sweeps_are_a_symmetric_positive_definite_preconditioner():
+    """With a weight w < 1 each sweep is y <- (1 - w) y + w D^-1 (r - N y) (and the transposed
+    iteration on L^T): checked against scipy's sparse products; M^-1 stays symmetric and
+    positive definite; and since the iteration matrix (1 - w) I - w D^-1 N has spectral radius
+    1 - w, many sweeps converge to the exact triangular solves, as with w = 1."""
+    p = meshes.jittered_cube(4)
+    off, cols, l, n = _factored(p)
+    rows = np.repeat(np.arange(n), np.diff(off))
+    L = sp.csr_matrix((l, (rows, cols)), shape=(n, n))
+    d = L.diagonal()
+    N = sp.tril(L, -1).tocsr()
+    rng = np.random.default_rng(7)
+    exact = oracle.Csr.from_arrays(off, cols, np.zeros(len(cols)))
+    for w in (0.8, 0.6):
+        for k in (1, 3):
+            r = rng.standard_normal(n)
+            y = r / d
+            for _ in range(k):
+                y = (1 - w) * y + w * (r - N @ y) / d
+            z = y / d
+            for _ in range(k):
+                z = (1 - w) * z + w * (y - N.T @ z) / d
+            got = jacobi_precondition(off, cols, l, r, k, w)
+            assert np.linalg.norm(got - z) <= 1e-12 * np.linalg.norm(z)
+            a, b = rng.standard_normal(n), rng.standard_normal(n)
+            ma = jacobi_precondition(off, cols, l, a, k, w)
+            mb = jacobi_precondition(off, cols, l, b, k, w)
+            assert abs(b @ ma - a @ mb) <= 1e-10 * abs(a @ mb)
+            assert a @ ma > 0.0
+        r = np.sin(0.3 * np.arange(n))
+        ref = exact.precondition(l, r)
+        far = jacobi_precondition(off, cols, l, r, 200, w)
+        assert np.linalg.norm(far - ref) <= 1e-10 * np.linalg.norm(ref)
+    # w = 1 is the undamped sweep exactly
+    r = rng.standard_normal(n)
+    assert np.array_equal(jacobi_precondition(off, cols, l, r, 3, 1.0),
+                          jacobi_precondition(off, cols, l, r, 3))
+
+
 def test_exchanged_iterates_are_rounded_toward_zero_to_44_fraction_bits():
     """The rounding of the iterates the fused sweeps exchange clears the low 8 bits of the
     significand: toward zero in magnitude for either sign, values on the 2^-44 grid unchanged,
