@@ -1538,12 +1538,18 @@ struct Operator {
       int best_k = sweeps;
       double best_w = sweep_w;
       // (sweeps, weight): undamped first, then damped by 0.8, which shortens the solves whose
-      // factor's off-diagonal part is large against its diagonal (Tet10)
+      // factor's off-diagonal part is large against its diagonal (Tet10); when the first damped
+      // count does not beat the fastest undamped one, the other damped counts are skipped
       static const std::pair<int, double> kCandidates[] = {
           {6, 1.0}, {4, 1.0}, {8, 1.0}, {5, 1.0}, {3, 1.0}, {2, 1.0},
           {4, 0.8}, {6, 0.8}, {3, 0.8}, {8, 0.8}, {2, 0.8}, {0, 1.0}};
+      bool damped_seen = false, skip_damped = false;
       for (const auto& cand : kCandidates) {
         const int k = cand.first;
+        const bool damped = cand.second != 1.0;
+        if (damped && skip_damped) continue;
+        const bool first_damped = damped && !damped_seen;
+        damped_seen = damped_seen || damped;
         set_sweeps(k);
         sweep_w = cand.second;
         const double app = application_seconds();
@@ -1563,6 +1569,8 @@ struct Operator {
           best_k = k;
           best_w = cand.second;
           other = std::max(0.0, t / (double)std::max<int64_t>(hS.it, 1) - app);
+        } else if (first_damped) {
+          skip_damped = true;
         }
       }
       set_sweeps(best_k);
